@@ -1,0 +1,229 @@
+// libgsolve_hip: the C ABI of include/gsolve.h over the MI355X kernels of this folder.
+//
+// gs_solve_batch_f64 replaces `_launch!` + `_solve_kernel!` of the reference's R2 batched solve
+// (src/nonlinear/batched.jl:171-195): one launch over the batch, work-item i solving system
+// idx = order[i] (or i), writing X[idx] and S[idx] (or codes[idx]), then a stream synchronisation.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/gsolve.h"
+#include "small_solver.hpp"
+
+namespace gs {
+
+static thread_local std::string g_last_error;
+
+// Record the message of a failed call for gs_last_error() and return its code.
+int set_error(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+static int fail(int code, const std::string& msg) { return set_error(code, msg); }
+
+struct KernelArgs {
+    FamArgs fam;
+    DevParams q;
+    int64_t nsys;
+    double* X;
+    gs_status_f64* S;
+    uint8_t* codes;
+    const int64_t* order;
+};
+
+// The status of one system in the byte layout of SolverStatus{Float64}, as nine 8-byte words.
+GS_DEV void store_status(gs_status_f64* dst, uint8_t code, int32_t iterations, double fnorm, double stepnorm,
+                         int32_t failures, bool promoted, double gradratio, double rate, bool contracted) {
+    uint64_t* w = reinterpret_cast<uint64_t*>(dst);
+    uint64_t w0 = uint64_t(code) | (uint64_t(uint32_t(iterations)) << 32);
+    uint64_t w3 = uint64_t(uint32_t(failures)) | (uint64_t(promoted ? 1 : 0) << 32);
+    uint64_t w4 = uint64_t(uint32_t(-1));  // rank = -1: an LU factor reports no rank
+    w[0] = w0;
+    w[1] = __builtin_bit_cast(uint64_t, fnorm);
+    w[2] = __builtin_bit_cast(uint64_t, stepnorm);
+    w[3] = w3;
+    w[4] = w4;
+    w[5] = __builtin_bit_cast(uint64_t, dnan());  // nullnorm
+    w[6] = __builtin_bit_cast(uint64_t, gradratio);
+    w[7] = __builtin_bit_cast(uint64_t, rate);
+    w[8] = uint64_t(contracted ? 1 : 0);
+}
+
+// One work-item per system (the launch shape of `_solve_kernel!`, batched.jl:171-179).
+template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG>
+__global__ __launch_bounds__(256) void solve_small_kernel(const KernelArgs a) {
+    constexpr int N = Fam::N;
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= a.nsys) return;
+    const int64_t idx = a.order ? a.order[i] - 1 : i;
+    Fam fam;
+    fam.load(a.fam, idx);
+    double x[N];
+    double* xp = a.X + idx * N;
+#pragma unroll
+    for (int k = 0; k < N; ++k) x[k] = xp[k];
+    SmallSolve<Fam, GLOB, MIXED, REFINE, DIAG> solver(fam, a.q);
+    auto R = solver.solve(x);
+#pragma unroll
+    for (int k = 0; k < N; ++k) xp[k] = x[k];
+    if (DIAG) {
+        store_status(a.S + idx, R.code, R.iterations, R.fnorm, R.stepnorm, R.failures, R.promoted, R.gradratio,
+                     R.rate, R.contracted);
+    } else {
+        a.codes[idx] = R.code;
+    }
+}
+
+template <class Fam, int GLOB, bool MIXED, bool REFINE>
+static int launch_diag(const KernelArgs& a, bool diag, int wg, hipStream_t st) {
+    const int64_t blocks = (a.nsys + wg - 1) / wg;
+    if (blocks > 0x7fffffffLL) return fail(GS_EINVAL, "batch too large for one launch");
+    if (diag)
+        hipLaunchKernelGGL((solve_small_kernel<Fam, GLOB, MIXED, REFINE, true>), dim3(unsigned(blocks)), dim3(wg), 0, st, a);
+    else
+        hipLaunchKernelGGL((solve_small_kernel<Fam, GLOB, MIXED, REFINE, false>), dim3(unsigned(blocks)), dim3(wg), 0, st, a);
+    return GS_OK;
+}
+
+template <class Fam>
+static int launch_family(const KernelArgs& a, int glob, bool mixed, bool refine, bool diag, int wg, hipStream_t st) {
+#define GS_L(G, M, R) return launch_diag<Fam, G, M, R>(a, diag, wg, st)
+    if (glob == GS_STATIC) {
+        if (!mixed && !refine) GS_L(kStatic, false, false);
+        if (!mixed && refine) GS_L(kStatic, false, true);
+        if (mixed && !refine) GS_L(kStatic, true, false);
+        GS_L(kStatic, true, true);
+    }
+    if (!mixed && !refine) GS_L(kBacktracking, false, false);
+    if (!mixed && refine) GS_L(kBacktracking, false, true);
+    if (mixed && !refine) GS_L(kBacktracking, true, false);
+    GS_L(kBacktracking, true, true);
+#undef GS_L
+}
+
+struct FamilySpec {
+    int id, n, p_len, p_bytes;  // p_bytes: bytes of one record
+    size_t align;               // alignment the device loads assume
+};
+
+static const FamilySpec* family_spec(int id) {
+    static const FamilySpec specs[] = {
+        {GS_FAM_COUPLED2, 2, 2, 16, 8},
+        {GS_FAM_HETERO2, 2, 1, 8, 8},
+        {GS_FAM_SCALAR_SQUARE, 1, 1, 8, 8},
+        {GS_FAM_TANH_LINEAR3, 3, 12, 96, 8},
+        {GS_FAM_HENON_HEILES_MIDPOINT, 4, 4, 32, 16},
+    };
+    for (const auto& s : specs)
+        if (s.id == id) return &s;
+    return nullptr;
+}
+
+}  // namespace gs
+
+using namespace gs;
+
+extern "C" {
+
+const char* gs_last_error(void) { return g_last_error.c_str(); }
+
+int32_t gs_version(void) { return GS_ABI_VERSION; }
+
+void gs_abi_sizes(int64_t out[4]) {
+    out[0] = sizeof(gs_status_f64);
+    out[1] = sizeof(gs_options_f64);
+    out[2] = sizeof(gs_method);
+    out[3] = sizeof(gs_family);
+}
+
+int gs_solve_batch_f64(const gs_method* method, const gs_family* family, int64_t nsys, double* X, const void* P,
+                       int64_t p_stride, gs_status_f64* S, uint8_t* codes, const int64_t* order,
+                       const gs_options_f64* options, int32_t workgroupsize, void* stream) {
+    g_last_error.clear();
+    if (!method || !family || !options) return fail(GS_EINVAL, "method, family and options must not be NULL");
+    if (nsys < 0) return fail(GS_EINVAL, "nsys must be non-negative");
+    if (workgroupsize < 0 || workgroupsize > 1024)
+        return fail(GS_EINVAL, "workgroupsize must be 0 (default) or in 1:1024, got " + std::to_string(workgroupsize));
+    if (!S && !codes) return fail(GS_EINVAL, "one of S and codes must be given");
+    const FamilySpec* fs = family_spec(family->id);
+    if (!fs) return fail(GS_EFAMILY, "problem family " + std::to_string(family->id) + " is not registered");
+    if (family->n != fs->n || family->m != fs->n)
+        return fail(GS_EINVAL, "family " + std::to_string(family->id) + " has n = m = " + std::to_string(fs->n));
+    if (method->linsolve != GS_LU) return fail(GS_EFAMILY, "only LUFactorization has a device form");
+    if (method->globalization != GS_STATIC && method->globalization != GS_BACKTRACKING)
+        return fail(GS_EFAMILY, "globalization " + std::to_string(method->globalization) +
+                                    " has no lane-per-system form for this family");
+    if (method->factor_precision != GS_FACTOR_WORKING && method->factor_precision != GS_FACTOR_F32)
+        return fail(GS_EINVAL, "factor_precision must be GS_FACTOR_WORKING or GS_FACTOR_F32");
+    if (method->refinement != GS_NO_REFINEMENT && method->refinement != GS_REFINEMENT)
+        return fail(GS_EINVAL, "refinement must be GS_NO_REFINEMENT or GS_REFINEMENT");
+    if (nsys == 0) return GS_OK;
+    if (!X) return fail(GS_EINVAL, "X must not be NULL for a non-empty batch");
+    if (!P) return fail(GS_EINVAL, "this family needs its parameter records P");
+    if (p_stride != 0 && p_stride < fs->p_bytes)
+        return fail(GS_EINVAL, "p_stride " + std::to_string(p_stride) + " is below the record size " + std::to_string(fs->p_bytes));
+    if ((reinterpret_cast<uintptr_t>(P) % fs->align) != 0 || (p_stride % int64_t(fs->align)) != 0)
+        return fail(GS_EINVAL, "P and p_stride must be " + std::to_string(fs->align) + "-byte aligned for this family");
+    if ((reinterpret_cast<uintptr_t>(X) % 8) != 0) return fail(GS_EINVAL, "X must be 8-byte aligned");
+
+    KernelArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.fam.P = static_cast<const char*>(P);
+    a.fam.p_stride = p_stride;
+    for (int k = 0; k < 8; ++k) a.fam.shared[k] = family->shared[k];
+    a.nsys = nsys;
+    a.X = X;
+    a.S = S;
+    a.codes = codes;
+    a.order = order;
+    DevParams& q = a.q;
+    q.f_abstol = options->f_abstol;
+    q.f_reltol = options->f_reltol;
+    q.x_abstol = options->x_abstol;
+    q.x_reltol = options->x_reltol;
+    q.kappa = options->kappa;
+    q.min_iterations = options->min_iterations;
+    q.max_iterations = options->max_iterations;
+    q.max_stalls = options->max_stalls;
+    q.norm_max = options->norm == GS_NORM_MAX;
+    q.roundoff = options->roundoff != 0;
+    q.rate_test = options->rate_test != 0;
+    q.alpha = method->static_alpha;
+    q.c1 = method->bt_c1;
+    q.p = method->bt_p;
+    q.bt_maxiter = method->bt_maxiter;
+    q.ref_maxiter = method->ref_maxiter;
+    q.ref_eta_eff = method->ref_eta > 1.4901161193847656e-8 ? method->ref_eta : 1.4901161193847656e-8;
+    q.mu = method->regularization;
+    q.best = method->best != 0;
+
+    const bool diag = S != nullptr;
+    if (!diag && !codes) return fail(GS_EINVAL, "codes must not be NULL");
+    const int wg = workgroupsize > 0 ? workgroupsize : 256;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const bool mixed = method->factor_precision == GS_FACTOR_F32;
+    const bool refine = method->refinement == GS_REFINEMENT;
+    const int glob = method->globalization;
+    int rc;
+    switch (family->id) {
+        case GS_FAM_COUPLED2: rc = launch_family<FamCoupled2>(a, glob, mixed, refine, diag, wg, st); break;
+        case GS_FAM_HETERO2: rc = launch_family<FamHetero2>(a, glob, mixed, refine, diag, wg, st); break;
+        case GS_FAM_SCALAR_SQUARE: rc = launch_family<FamScalarSquare>(a, glob, mixed, refine, diag, wg, st); break;
+        case GS_FAM_TANH_LINEAR3: rc = launch_family<FamTanhLinear3>(a, glob, mixed, refine, diag, wg, st); break;
+        case GS_FAM_HENON_HEILES_MIDPOINT:
+            rc = launch_family<FamHenonHeilesMidpoint>(a, glob, mixed, refine, diag, wg, st);
+            break;
+        default: return fail(GS_EFAMILY, "problem family is not registered");
+    }
+    if (rc != GS_OK) return rc;
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(GS_EHIP, std::string("kernel launch failed: ") + hipGetErrorString(e));
+    e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return fail(GS_EHIP, std::string("hipStreamSynchronize failed: ") + hipGetErrorString(e));
+    return GS_OK;
+}
+
+}  // extern "C"

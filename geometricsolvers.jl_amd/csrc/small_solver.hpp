@@ -1,0 +1,671 @@
+// The lane-per-system solve: one work-item owns one system of N ≤ 8 unknowns, with the iterate, the
+// residual, the Jacobian and its LU factors in VGPRs. It fuses the whole R1 loop of the reference —
+// `solve!!` (src/nonlinear/solver.jl:277-378) with Newton's plain and mixed-precision steps
+// (src/nonlinear/newton.jl:1302-1485, 1604-1720), the static LU with power-of-two scaling
+// (src/linear/static/lu.jl:24-92, src/linear/factors.jl:802-949), iterative refinement
+// (src/nonlinear/refinement.jl:211-244) and the Static / Backtracking globalizations
+// (src/globalization/linefunction.jl:114-199, linesearch.jl:154-258, backtracking.jl:62-132) — into
+// straight-line device code, so that one launch solves the whole batch with no host round trip.
+//
+// Compile-time choices (template parameters): the family (N, F, J), the globalization, the factor
+// precision, refinement, and whether the status diagnostics are produced. Everything else in the
+// method and the options is a runtime value.
+#pragma once
+
+#include "device_common.hpp"
+#include "families.hpp"
+
+namespace gs {
+
+enum : uint8_t { kSuccess = 0, kStalled = 1, kMaxIters = 2, kSingular = 3, kNonfinite = 4, kLsFailed = 5 };
+
+// The runtime part of the method and of Options, in the form the kernel reads.
+struct DevParams {
+    // Options (src/base/options.jl:58-71)
+    double f_abstol, f_reltol, x_abstol, x_reltol, kappa;
+    int32_t min_iterations, max_iterations, max_stalls;
+    int32_t norm_max;      // 1 = MaxNorm
+    int32_t roundoff, rate_test;
+    // method
+    double alpha;          // Static(α)
+    double c1, p;          // Backtracking
+    int32_t bt_maxiter;
+    int32_t ref_maxiter;
+    double ref_eta_eff;    // max(η, √eps)
+    double mu;             // regularization
+    int32_t best;
+};
+
+enum { kStatic = 0, kBacktracking = 1 };
+
+// The outcome of one step (StepInfo, src/base/status.jl:220-234; `limited` is always false here).
+struct Step {
+    uint8_t code;
+    bool promoted, shortened;
+    int32_t failures;
+    double fnorm, stepnorm;
+};
+
+// ------------------------------------------------------------------------------------------------
+// Static LU in registers
+// ------------------------------------------------------------------------------------------------
+
+// LU with partial pivoting of A (already divided by its power of two), first-largest |a| pivot, row
+// exchanges of whole rows, exact multipliers a/p, rank-1 updates without FMA; info = first zero pivot.
+// The row exchange is a select per entry; a wave in which no lane exchanges skips it.
+template <class TF, int N> GS_DEV int lu_inplace(TF (&A)[N][N], int (&piv)[N]) {
+    int info = 0;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        int p = k;
+        TF amax = __builtin_fabs(A[k][k]);
+#pragma unroll
+        for (int i = k + 1; i < N; ++i) {
+            TF a = __builtin_fabs(A[i][k]);
+            if (a > amax) {
+                amax = a;
+                p = i;
+            }
+        }
+        piv[k] = p;
+        if (k + 1 < N && __any(p != k)) {
+#pragma unroll
+            for (int i = k + 1; i < N; ++i) {
+                bool s = (p == i);
+#pragma unroll
+                for (int j = 0; j < N; ++j) {
+                    TF t = A[k][j];
+                    A[k][j] = s ? A[i][j] : t;
+                    A[i][j] = s ? t : A[i][j];
+                }
+            }
+        }
+        TF akk = A[k][k];
+        if (akk == TF(0)) {
+            if (info == 0) info = k + 1;
+        } else {
+#pragma unroll
+            for (int i = k + 1; i < N; ++i) A[i][k] = A[i][k] / akk;
+#pragma unroll
+            for (int j = k + 1; j < N; ++j)
+#pragma unroll
+                for (int i = k + 1; i < N; ++i) A[i][j] = A[i][j] - A[i][k] * A[k][j];
+        }
+    }
+    return info;
+}
+
+// backsolve (lu.jl:67-92): permutation, unit-lower forward, upper backward substitution.
+template <class TF, int N> GS_DEV void lu_backsolve(const TF (&A)[N][N], const int (&piv)[N], TF (&x)[N]) {
+#pragma unroll
+    for (int k = 0; k < N - 1; ++k) {
+        int p = piv[k];
+        if (__any(p != k)) {
+#pragma unroll
+            for (int i = k + 1; i < N; ++i) {
+                bool s = (p == i);
+                TF t = x[k];
+                x[k] = s ? x[i] : t;
+                x[i] = s ? t : x[i];
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        TF xk = x[k];
+#pragma unroll
+        for (int i = k + 1; i < N; ++i) x[i] = x[i] - A[i][k] * xk;
+    }
+#pragma unroll
+    for (int k = N - 1; k >= 0; --k) {
+        TF xk = x[k] / A[k][k];
+        x[k] = xk;
+#pragma unroll
+        for (int i = 0; i < k; ++i) x[i] = x[i] - A[i][k] * xk;
+    }
+}
+
+// The factor of a square matrix in TF: J/2^e, pivots, info, e (StaticLU, factors.jl:134-148).
+template <class TF, int N> struct Factor {
+    TF A[N][N];
+    int piv[N];
+    int info;
+    int e;
+};
+
+// factorize!! (lu.jl:24-64) after factor_matrix (factors.jl:802-811): in the working precision only
+// e < 0 divides; a narrower TF divides for every e ≠ 0.
+template <class TF, int N> GS_DEV void factorize(const double (&J)[N][N], Factor<TF, N>& f) {
+    double flat[N * N];
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = 0; j < N; ++j) flat[i * N + j] = J[i][j];
+    int e = max_exponent<N * N>(flat, kEmax64);
+    constexpr bool same = sizeof(TF) == sizeof(double);
+    bool scale = e != 0 && (!same || e < 0);
+    f.e = scale ? e : 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = 0; j < N; ++j) f.A[i][j] = TF(scale ? ldexp2(J[i][j], -e) : J[i][j]);
+    f.info = lu_inplace<TF, N>(f.A, f.piv);
+}
+
+// ldiv!! in the working precision (factors.jl:931-936).
+template <int N> GS_DEV void ldiv_same(const Factor<double, N>& f, const double (&r)[N], double (&d)[N]) {
+    int k = rhs_exponent<N>(r, f.e);
+#pragma unroll
+    for (int i = 0; i < N; ++i) d[i] = k != 0 ? ldexp2(r[i], -k) : r[i];
+    lu_backsolve<double, N>(f.A, f.piv, d);
+    int u = k - f.e;
+    if (u != 0) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) d[i] = ldexp2(d[i], u);
+    }
+}
+
+// ldiv!! with FP32 factors and an FP64 right-hand side (factors.jl:943-949).
+template <int N> GS_DEV void ldiv_mixed(const Factor<float, N>& f, const double (&r)[N], double (&d)[N]) {
+    double s = __builtin_fabs(r[0]);
+#pragma unroll
+    for (int i = 1; i < N; ++i) s = jmax(s, __builtin_fabs(r[i]));
+    if (s == 0.0) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) d[i] = 0.0;
+        return;
+    }
+    float t[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) t[i] = float(r[i] / s);
+    lu_backsolve<float, N>(f.A, f.piv, t);
+    int ex = isfin(s) ? __builtin_amdgcn_frexp_exp(s) : 0;
+    double frac = isfin(s) ? __builtin_amdgcn_frexp_mant(s) : s;
+#pragma unroll
+    for (int i = 0; i < N; ++i) d[i] = ldexp2(double(t[i]) * frac, ex - f.e);
+}
+
+template <int N> GS_DEV void ldiv(const Factor<double, N>& f, const double (&r)[N], double (&d)[N]) { ldiv_same<N>(f, r, d); }
+template <int N> GS_DEV void ldiv(const Factor<float, N>& f, const double (&r)[N], double (&d)[N]) { ldiv_mixed<N>(f, r, d); }
+
+// StaticArrays' SMatrix × SVector: entry i = fma-fold of row i (combine_products' muladd chain).
+template <int N> GS_DEV void matvec(const double (&A)[N][N], const double (&v)[N], double (&y)[N]) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        double acc = A[i][0] * v[0];
+#pragma unroll
+        for (int j = 1; j < N; ++j) acc = __builtin_fma(A[i][j], v[j], acc);
+        y[i] = acc;
+    }
+}
+
+// ‖Jᴴ F‖/(‖J‖_F ‖F‖) (grad_ratio_and_norm, newton.jl:1287-1298).
+template <int N> __device__ __noinline__ double grad_ratio_scaled(const double (&J)[N][N], const double (&Fx)[N], double jnorm, double fnorm) {
+    int e = exponent_of(fnorm);
+    double sc = ldexp2(1.0, -e);
+    double g[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        double acc = J[0][k] * (Fx[0] * sc);
+#pragma unroll
+        for (int j = 1; j < N; ++j) acc = __builtin_fma(J[j][k], Fx[j] * sc, acc);
+        g[k] = acc;
+    }
+    return rnorm<N>(g) / jnorm / ldexp2(fnorm, -e);
+}
+
+template <int N> GS_DEV double grad_ratio(const double (&J)[N][N], const double (&Fx)[N], double fnorm) {
+    double flat[N * N];
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+#pragma unroll
+        for (int i = 0; i < N; ++i) flat[i + j * N] = J[i][j];
+    double jnorm = rnorm<N * N>(flat);
+    if (jnorm == 0.0 || fnorm == 0.0) return 0.0;
+    double g[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        double acc = J[0][k] * Fx[0];
+#pragma unroll
+        for (int j = 1; j < N; ++j) acc = __builtin_fma(J[j][k], Fx[j], acc);
+        g[k] = acc;
+    }
+    double gnorm = rnorm<N>(g);
+    if (isfin(gnorm) || !(isfin(jnorm) && isfin(fnorm))) return gnorm / jnorm / fnorm;
+    return grad_ratio_scaled<N>(J, Fx, jnorm, fnorm);
+}
+
+// ------------------------------------------------------------------------------------------------
+// One system
+// ------------------------------------------------------------------------------------------------
+
+template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG> struct SmallSolve {
+    static constexpr int N = Fam::N;
+    const Fam& fam;
+    const DevParams& q;
+
+    // the per-solve state (NewtonState, newton.jl:715-744, as the R1 value state holds it)
+    double merit;       // norm2(Fx) for Backtracking
+    double sfnorm;      // ‖Fx‖ that the state carries
+    double gradratio;   // diagnostics of the last factorisation
+    bool promoted;      // mixed precision: factorising in FP64
+    int slow_steps;
+
+    GS_DEV SmallSolve(const Fam& f, const DevParams& p) : fam(f), q(p) {}
+
+    // residual_norm (newton.jl:1274-1278)
+    GS_DEV double residual_norm(const double (&Fx)[N]) const {
+        if (GLOB == kBacktracking) {
+            if (isfin(merit) && merit >= double(N) * kDblMin) return __builtin_sqrt(merit);
+            return rnorm<N>(Fx);
+        }
+        return isnan_(sfnorm) ? rnorm<N>(Fx) : sfnorm;
+    }
+
+    // --- globalization --------------------------------------------------------------------------
+
+    // Static (linefunction.jl:114-131).
+    GS_DEV Step step_static(double (&x)[N], double (&Fx)[N], const double (&d)[N]) const {
+        double s = q.alpha;
+        double dn = rnorm<N>(d);
+        double stepnorm = __builtin_fabs(s) * dn;
+        if (!isfin(stepnorm)) return Step{kNonfinite, false, false, 0, rnorm<N>(Fx), stepnorm};
+        double xp[N], Fp[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) xp[i] = x[i] + s * d[i];
+        fam.F(xp, Fp);
+        double fn = rnorm<N>(Fp);
+        if (!isfin(fn)) return Step{kNonfinite, false, false, 0, rnorm<N>(Fx), stepnorm};
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            x[i] = xp[i];
+            Fx[i] = Fp[i];
+        }
+        return Step{kSuccess, false, false, 0, fn, stepnorm};
+    }
+
+    // backtrack_step (backtracking.jl:320-340)
+    GS_DEV static double backtrack_step(double phi0, double d0, double a, double pa, double ap, double pp, double p) {
+        if (!isfin(pa)) return 0.1 * a;
+        double an = dnan();
+        double s = __builtin_fabs(d0);
+        double r1 = (pa - phi0) / s + a;
+        if (!(ap > 0.0)) {
+            if (r1 > 0.0) an = (a * a) / (2.0 * r1);
+        } else if (isfin(pp) && a != ap) {
+            double q1 = r1 / (a * a);
+            double q2 = ((pp - phi0) / s + ap) / (ap * ap);
+            double A = (q1 - q2) / (a - ap);
+            double B = (a * q2 - ap * q1) / (a - ap);
+            double disc = B * B + 3.0 * A;
+            if (disc >= 0.0) an = B > 0.0 ? 1.0 / (B + __builtin_sqrt(disc)) : (-B + __builtin_sqrt(disc)) / (3.0 * A);
+        }
+        if (!(isfin(an) && an > 0.0)) an = p * a;
+        return jmin(jmax(an, 0.1 * a), p * a);
+    }
+
+    // globalize_merit!! for Backtracking (linefunction.jl:159-199 → linesearch.jl:154-171 →
+    // backtracking.jl:354-390). `exact`: ExactStep (slope −2φ₀) or MeasuredSlope (φ′(0) = 2⟨F, J d⟩).
+    GS_DEV Step step_backtracking(double (&x)[N], double (&Fx)[N], const double (&d)[N], bool exact) {
+        const double phi0 = merit;
+        double dn = rnorm<N>(d);
+        if (!isfin(dn)) return Step{kNonfinite, false, false, 0, rnorm<N>(Fx), dn};
+        double d0;
+        if (exact) {
+            d0 = -2.0 * phi0;
+        } else {
+            double xa[N], Fa[N], Ja[N][N], Jv[N];
+#pragma unroll
+            for (int i = 0; i < N; ++i) xa[i] = x[i] + 0.0 * d[i];
+            fam.F(xa, Fa);
+            fam.J(xa, Ja);
+            matvec<N>(Ja, d, Jv);
+            d0 = 2.0 * rdot<N>(Fa, Jv);
+        }
+        if (!(isfin(phi0) && isfin(d0) && d0 < 0.0)) {
+            uint8_t c = !(isfin(phi0) && isfin(d0)) ? kNonfinite : (d0 > 0.0 ? kLsFailed : kStalled);
+            // a LINESEARCH_FAILED anchor keeps x with φ = NaN: the failure takes no step
+            if (c == kLsFailed) return Step{kLsFailed, false, false, 1, rnorm<N>(Fx), 0.0};
+            return Step{c, false, false, 0, rnorm<N>(Fx), 0.0};
+        }
+        const double tau = __builtin_fmax(4.0 * kDblEps * __builtin_fabs(phi0), 4.0 * kDblTiny);
+        const double amin = jmax(tau / __builtin_fabs(d0), kDblMin);
+        double alpha = __builtin_fmin(__builtin_fmax(1.0, amin), __builtin_inf());
+        double aa = alpha, pa = dnan();
+        double Fa[N];
+        double ap = dnan(), pp = dnan();
+        int frozen = 0;
+        uint8_t code = kLsFailed;
+        const double sqrteps = 1.4901161193847656e-8;
+        for (int trips = 0; trips < q.bt_maxiter; ++trips) {
+            aa = alpha;
+            double xa[N];
+#pragma unroll
+            for (int i = 0; i < N; ++i) xa[i] = x[i] + alpha * d[i];
+            fam.F(xa, Fa);
+            pa = norm2<N>(Fa);
+            frozen = (pa == phi0 && alpha <= sqrteps) ? frozen + 1 : 0;
+            if (pa - phi0 <= jmin(0.0, q.c1 * alpha * d0 + tau)) {
+                code = (isfin(pa) && pa - phi0 < -tau) ? kSuccess : (__builtin_fabs(pa - phi0) <= tau ? kStalled : kLsFailed);
+                break;
+            } else if (frozen >= 2) {
+                code = kStalled;
+                break;
+            } else if (alpha <= amin) {
+                code = __builtin_fabs(pa - phi0) <= tau ? kStalled : kLsFailed;
+                break;
+            }
+            double an = isfin(pa) ? backtrack_step(phi0, d0, alpha, pa, ap, pp, q.p)
+                                  : jmin(0.1 * alpha, __builtin_sqrt(alpha) * __builtin_sqrt(amin));
+            ap = alpha;
+            pp = pa;
+            alpha = jmax(an, amin);
+        }
+        if (code == kSuccess) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                x[i] = x[i] + aa * d[i];
+                Fx[i] = Fa[i];
+            }
+            merit = pa;
+            return Step{kSuccess, false, aa < 1.0, 0, __builtin_sqrt(pa), aa * dn};
+        }
+        if (code == kLsFailed) {
+            if (isfin(pa) && pa < phi0) {
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    x[i] = x[i] + aa * d[i];
+                    Fx[i] = Fa[i];
+                }
+                merit = pa;
+                return Step{kLsFailed, false, aa < 1.0, 1, __builtin_sqrt(pa), aa * dn};
+            }
+            return Step{kLsFailed, false, false, 1, rnorm<N>(Fx), 0.0};
+        }
+        return Step{code == kStalled ? kStalled : kNonfinite, false, false, 0, rnorm<N>(Fx), 0.0};
+    }
+
+    GS_DEV Step globalize(double (&x)[N], double (&Fx)[N], const double (&d)[N], bool exact) {
+        if (GLOB == kStatic) return step_static(x, Fx, d);
+        return step_backtracking(x, Fx, d, exact);
+    }
+
+    // --- refinement -----------------------------------------------------------------------------
+
+    // refine_passes!! (refinement.jl:211-244) with the held Jacobian J.
+    template <class TF>
+    GS_DEV void refine(double (&d)[N], const Factor<TF, N>& f, const double (&J)[N][N], const double (&Fx)[N], bool& failed) const {
+        failed = false;
+        double fnorm = rnorm<N>(Fx);
+        if (fnorm == 0.0) return;
+        if (!isfin(fnorm)) {
+            failed = true;
+            return;
+        }
+        double Jv[N], r[N];
+        matvec<N>(J, d, Jv);
+#pragma unroll
+        for (int i = 0; i < N; ++i) r[i] = -Fx[i] + -Jv[i];
+        double rprev = rnorm<N>(r);
+        if (!isfin(rprev)) {
+            failed = true;
+            return;
+        }
+        if (rprev <= q.ref_eta_eff * fnorm) return;
+        for (int it = 1; it <= q.ref_maxiter; ++it) {
+            double dd[N], dc[N];
+            ldiv<N>(f, r, dd);
+#pragma unroll
+            for (int i = 0; i < N; ++i) dc[i] = d[i] + dd[i];
+            matvec<N>(J, dc, Jv);
+#pragma unroll
+            for (int i = 0; i < N; ++i) r[i] = -Fx[i] + -Jv[i];
+            double rcur = rnorm<N>(r);
+            if (!(rcur < rprev)) {
+                failed = true;
+                return;
+            }
+#pragma unroll
+            for (int i = 0; i < N; ++i) d[i] = dc[i];
+            rprev = rcur;
+            if (rcur <= q.ref_eta_eff * fnorm) return;
+        }
+        failed = true;
+    }
+
+    // --- steps ----------------------------------------------------------------------------------
+
+    GS_DEV void jacobian(const double (&x)[N], const double (&Fx)[N], double (&J)[N][N]) {
+        if (DIAG) {
+            double fnorm = residual_norm(Fx);
+            fam.J(x, J);
+            gradratio = grad_ratio<N>(J, Fx, fnorm);
+        } else {
+            fam.J(x, J);
+        }
+        if (q.mu != 0.0) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) J[i][i] = J[i][i] + q.mu;
+        }
+    }
+
+    // newton_step!! in the working precision (newton.jl:1679-1688, 1302-1311, 1396-1416).
+    GS_DEV Step plain_step(double (&x)[N], double (&Fx)[N]) {
+        double J[N][N];
+        jacobian(x, Fx, J);
+        Factor<double, N> f;
+        factorize<double, N>(J, f);
+        Step s;
+        if (f.info > 0) {
+            s = Step{kSingular, false, false, 0, rnorm<N>(Fx), 0.0};
+        } else {
+            double d[N];
+            ldiv<N>(f, Fx, d);
+#pragma unroll
+            for (int i = 0; i < N; ++i) d[i] = -d[i];
+            if (REFINE) {
+                bool failed;
+                refine<double>(d, f, J, Fx, failed);
+            }
+            bool exact = q.mu == 0.0 && !REFINE;
+            s = globalize(x, Fx, d, exact);
+        }
+        sfnorm = s.fnorm;
+        return s;
+    }
+
+    // mixed_step!! / mixed_newton!! with fresh factors (newton.jl:1604-1666, 1692-1720).
+    GS_DEV Step mixed_step(double (&x)[N], double (&Fx)[N]) {
+        double fnorm_old = residual_norm(Fx);
+        const bool was_promoted = promoted;
+        double J[N][N];
+        jacobian(x, Fx, J);
+        double d[N];
+        bool ok = true;
+        int failures = 0;
+        double lin_rel = 0.0;
+        if (promoted) {
+            Factor<double, N> f;
+            factorize<double, N>(J, f);
+            if (f.info > 0) {
+                ok = false;
+            } else {
+                ldiv<N>(f, Fx, d);
+#pragma unroll
+                for (int i = 0; i < N; ++i) d[i] = -d[i];
+                if (REFINE) {
+                    bool failed;
+                    refine<double>(d, f, J, Fx, failed);
+                }
+            }
+        } else {
+            Factor<float, N> f;
+            factorize<float, N>(J, f);
+            bool ref_failed = false;
+            if (f.info > 0) {
+                ref_failed = true;
+            } else {
+                ldiv<N>(f, Fx, d);
+#pragma unroll
+                for (int i = 0; i < N; ++i) d[i] = -d[i];
+                if (REFINE) refine<float>(d, f, J, Fx, ref_failed);
+            }
+            if (ref_failed) {
+                // promote_factor!! (newton.jl:1332-1339): refactorise the held Jacobian in FP64
+                failures += 1;
+                promoted = true;
+                Factor<double, N> g;
+                factorize<double, N>(J, g);
+                if (g.info > 0) {
+                    ok = false;
+                } else {
+                    ldiv<N>(g, Fx, d);
+#pragma unroll
+                    for (int i = 0; i < N; ++i) d[i] = -d[i];
+                }
+            } else if (fnorm_old != 0.0 && isfin(fnorm_old)) {
+                // linear_residual_ratio (newton.jl:1422-1430)
+                double Jd[N], lin[N];
+                matvec<N>(J, d, Jd);
+#pragma unroll
+                for (int i = 0; i < N; ++i) lin[i] = 1.0 * Fx[i] + 1.0 * Jd[i];
+                lin_rel = rnorm<N>(lin) / fnorm_old;
+            }
+        }
+        if (!ok) {
+            Step s{kSingular, promoted, false, failures, rnorm<N>(Fx), 0.0};
+            return s;
+        }
+        const bool promoted_step = promoted && !was_promoted;
+        const bool exact = q.mu == 0.0 && promoted;
+        Step s = globalize(x, Fx, d, exact);
+        s.failures += failures;
+        if (promoted_step) {
+            s.promoted = true;
+            slow_steps = 0;
+        } else if (!was_promoted) {
+            // read_mixed_stall (newton.jl:1437-1451), η_lin = max(√eps, 1e-4)
+            if (s.fnorm > 0.5 * fnorm_old && lin_rel > 1e-4) {
+                int slow = slow_steps + 1;
+                if (slow >= 2) {
+                    promoted = true;
+                    slow_steps = 0;
+                    s.failures += 1;
+                    s.promoted = true;
+                } else {
+                    slow_steps = slow;
+                }
+            } else {
+                slow_steps = 0;
+            }
+        }
+        sfnorm = s.fnorm;
+        return s;
+    }
+
+    // --- the loop -------------------------------------------------------------------------------
+
+    struct Result {
+        uint8_t code;
+        bool contracted, promoted;
+        int32_t iterations, failures;
+        double fnorm, stepnorm, rate, gradratio;
+    };
+
+    // solve!! (solver.jl:277-378) for a square NonlinearProblem with a Newton method.
+    GS_DEV Result solve(double (&x)[N]) {
+        double Fx[N];
+        fam.F(x, Fx);
+        const double fnorm0 = rnorm<N>(Fx);
+        // newton_start!! (newton.jl:1135-1144)
+        merit = GLOB == kBacktracking ? norm2<N>(Fx) : dnan();
+        sfnorm = fnorm0;
+        gradratio = dnan();
+        promoted = false;
+        slow_steps = 0;
+        double best_x[N];
+        double best_fn = fnorm0;
+        if (q.best) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) best_x[i] = x[i];
+        }
+        Result R{kSuccess, false, false, 0, 0, fnorm0, 0.0, dnan(), dnan()};
+        if (!isfin(fnorm0)) {
+            R.code = kNonfinite;
+        } else {
+            const double ftest0 = q.norm_max ? rmaxnorm<N>(Fx) : fnorm0;
+            const double ftol = jmax(q.f_abstol, q.f_reltol * ftest0);
+            if (0 >= q.min_iterations && ftest0 <= ftol) {
+                R.code = kSuccess;
+            } else if (q.max_iterations == 0) {
+                R.code = kMaxIters;
+            } else {
+                int stalls = 0;
+                while (true) {
+                    const double rate_k = R.rate;
+                    Step s = MIXED ? mixed_step(x, Fx) : plain_step(x, Fx);
+                    if (q.best && s.fnorm < best_fn) {
+                        best_fn = s.fnorm;
+#pragma unroll
+                        for (int i = 0; i < N; ++i) best_x[i] = x[i];
+                    }
+                    // record (status.jl:245-262)
+                    R.rate = R.stepnorm > 0.0 ? s.stepnorm / R.stepnorm : dnan();
+                    R.code = s.code;
+                    R.iterations += 1;
+                    R.fnorm = s.fnorm;
+                    R.stepnorm = s.stepnorm;
+                    R.failures += s.failures;
+                    R.promoted |= s.promoted;
+                    if (s.code == kSingular || s.code == kNonfinite || s.code == kMaxIters) break;
+                    if (s.code == kLsFailed && s.stepnorm == 0.0) break;
+                    const double ftest = q.norm_max ? rmaxnorm<N>(Fx) : R.fnorm;
+                    if (R.iterations >= q.min_iterations && ftest <= ftol) {
+                        R.code = kSuccess;
+                        break;
+                    }
+                    const bool steptested = s.code != kStalled;
+                    const double xnorm = steptested ? rnorm<N>(x) : dnan();
+                    const double xtol = jmax(q.x_abstol, q.x_reltol * xnorm);
+                    const bool past_min = R.iterations >= q.min_iterations;
+                    bool contracted = false;
+                    if (!s.shortened && q.rate_test && past_min) {
+                        double th = jmax(R.rate, rate_k);
+                        contracted = R.stepnorm > 0.0 && th < 1.0 && th / (1.0 - th) * R.stepnorm <= q.kappa * xtol;
+                    }
+                    bool step_conv = past_min && R.stepnorm > 0.0 && R.stepnorm <= xtol;
+                    bool zero_step = s.code == kSuccess && s.stepnorm == 0.0 && past_min;
+                    bool roundoff = q.roundoff && past_min && R.rate >= 1.0 && rate_k < 1.0;
+                    if (contracted || (steptested && (step_conv || zero_step || roundoff))) {
+                        R.code = contracted ? kSuccess : kStalled;
+                        R.contracted = contracted;
+                        break;
+                    }
+                    if (s.code == kStalled) {
+                        stalls += 1;
+                        if (stalls >= q.max_stalls) {
+                            R.code = kStalled;
+                            break;
+                        }
+                    } else {
+                        stalls = 0;
+                    }
+                    if (R.iterations >= q.max_iterations) {
+                        R.code = kMaxIters;
+                        break;
+                    }
+                }
+            }
+        }
+        R.gradratio = gradratio;
+        // restore_best (newton.jl:792-799)
+        if (q.best && R.code != kSuccess && isfin(best_fn) && best_fn < R.fnorm) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) x[i] = best_x[i];
+            R.fnorm = best_fn;
+        }
+        return R;
+    }
+};
+
+}  // namespace gs

@@ -1,0 +1,187 @@
+/*
+ * gsolve.h — C ABI of libgsolve_hip, the MI355X (gfx950) batched nonlinear solver.
+ *
+ * The library replaces the body of the R2 batched solve of GeometricSolvers.jl:
+ *   solve_batch!(X, S, prob, method, p; order, options, workgroupsize)   src/nonlinear/batched.jl:257-287
+ *   solve_batch!(X, prob, method, p; ...)  (counts form)                src/nonlinear/batched.jl:289-307
+ * whose device part is the KernelAbstractions kernel `_solve_kernel!` (batched.jl:171-179) launched by
+ * `_launch!` (batched.jl:183-195). A Julia package extension (ext/GeometricSolversHIPExt.jl) keeps the
+ * Julia signature and `ccall`s the entry points below; the Python package in this repository binds
+ * the same symbols through ctypes.
+ *
+ * Conventions
+ *  - Every pointer to batch data (X, P, S, codes, order) is a device pointer of the caller.
+ *  - Every call is synchronous: it enqueues on `stream` and waits for it before returning, as
+ *    `_launch!` synchronises (batched.jl:193).
+ *  - The return value is 0 or a negative GS_E* code; gs_last_error() then holds the message.
+ *    Numerical failures never surface here: they are ReturnCodes inside S / codes.
+ *  - No C++ exception crosses this boundary.
+ */
+#ifndef GSOLVE_H
+#define GSOLVE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GS_ABI_VERSION 1
+
+/* ReturnCode, one byte (src/base/returncodes.jl:14). */
+enum gs_return_code {
+    GS_SUCCESS = 0,
+    GS_STALLED = 1,
+    GS_MAXITERS = 2,
+    GS_SINGULAR = 3,
+    GS_NONFINITE = 4,
+    GS_LINESEARCH_FAILED = 5
+};
+
+/* Error codes of the entry points (negative). */
+enum gs_error {
+    GS_OK = 0,
+    GS_EINVAL = -1,     /* an argument is invalid: Julia ArgumentError                    */
+    GS_EDIM = -2,       /* a length does not match: Julia DimensionMismatch               */
+    GS_EFAMILY = -3,    /* the problem family or method combination is not registered     */
+    GS_EHIP = -4        /* a HIP runtime call failed                                       */
+};
+
+/* SolverStatus{Float64} (src/base/status.jl:101-117), byte-identical to Julia's C layout:
+ * code u8 @0, f_calls i16 @2, iterations i32 @4, fnorm @8, stepnorm @16, step_failures i32 @24,
+ * promoted @28, stationary @29, jvp_calls i16 @30, rank i32 @32, nullnorm @40, gradratio @48,
+ * rate @56, contracted @64, reused @65; size 72. */
+typedef struct gs_status_f64 {
+    uint8_t code;
+    uint8_t pad0_;
+    int16_t f_calls;
+    int32_t iterations;
+    double fnorm;
+    double stepnorm;
+    int32_t step_failures;
+    uint8_t promoted;
+    uint8_t stationary;
+    int16_t jvp_calls;
+    int32_t rank;
+    int32_t pad1_;
+    double nullnorm;
+    double gradratio;
+    double rate;
+    uint8_t contracted;
+    uint8_t reused;
+    uint8_t pad2_[6];
+} gs_status_f64;
+
+/* The stopping norm of Options (src/base/options.jl:8-25). */
+enum gs_norm { GS_NORM_L2 = 0, GS_NORM_MAX = 1 };
+
+/* Options{Float64} (src/base/options.jl:58-71); `norm` is the StoppingNorm singleton as an enum. */
+typedef struct gs_options_f64 {
+    double f_abstol;
+    double f_reltol;
+    double x_abstol;
+    double x_reltol;
+    int32_t min_iterations;
+    int32_t max_iterations;
+    int32_t max_stalls;
+    int32_t norm; /* gs_norm */
+    double g_tol;
+    double kappa;
+    uint8_t roundoff;
+    uint8_t rate_test;
+    uint8_t pad_[6];
+} gs_options_f64;
+
+/* The method: Newton(; linsolve, globalization, refinement, regularization, best)
+ * (src/nonlinear/newton.jl:138-250) with LUFactorization{Nothing|Float32} (src/linear/methods.jl:37-47),
+ * Static (src/globalization/static.jl:8-30), Backtracking (src/globalization/backtracking.jl:28-50),
+ * LevenbergMarquardt (src/globalization/levenbergmarquardt.jl:48-98) and Refinement
+ * (src/nonlinear/refinement.jl:27-49). `reuse` and `keep_factor` have no device form here. */
+enum gs_globalization { GS_STATIC = 0, GS_BACKTRACKING = 1, GS_LEVENBERG_MARQUARDT = 2 };
+enum gs_factor_precision { GS_FACTOR_WORKING = 0 /* LUFactorization() */, GS_FACTOR_F32 = 1 };
+enum gs_refinement { GS_NO_REFINEMENT = 0, GS_REFINEMENT = 1 };
+enum gs_linsolve { GS_LU = 0 };
+
+typedef struct gs_method {
+    int32_t linsolve;          /* gs_linsolve                                            */
+    int32_t factor_precision;  /* gs_factor_precision                                    */
+    int32_t globalization;     /* gs_globalization                                       */
+    int32_t refinement;        /* gs_refinement                                          */
+    double static_alpha;       /* Static(α)                                              */
+    double bt_c1;              /* Backtracking(c₁, p, maxiter)                           */
+    double bt_p;
+    int32_t bt_maxiter;
+    int32_t ref_maxiter;       /* Refinement(η, maxiter)                                 */
+    double ref_eta;
+    double regularization;     /* μ ≥ 0                                                  */
+    double lm_mu0;             /* LevenbergMarquardt(μ₀, μ_min, μ_max, μ_dec, μ_inc, ρ)  */
+    double lm_mu_min;
+    double lm_mu_max;
+    double lm_mu_decrease;
+    double lm_mu_increase;
+    double lm_rho_accept;
+    int32_t best;              /* best = true                                            */
+    int32_t pad_;
+} gs_method;
+
+/* The registered problem families. Each has its residual F(x, p) and its analytic Jacobian J(x, p)
+ * as HIP device functions; the Julia extension carries the same F and J as Julia functions, which
+ * are the CPU oracle. `P` holds one record of `p_len` scalars per system at a byte stride of
+ * `p_stride` (0 = one record shared by every system, as a `p` that is not a PerSystem). */
+enum gs_family_id {
+    GS_FAM_COUPLED2 = 1,         /* x + y²/8 − p : n = 2, P = 2 doubles (test/nonlinear/batched.jl:92-105)        */
+    GS_FAM_HETERO2 = 2,          /* six-outcome fixture : n = 2, P = 1 int64 code (batched.jl:54-90)             */
+    GS_FAM_SCALAR_SQUARE = 3,    /* x² − p : n = 1, P = 1 double (batched.jl:107-110)                            */
+    GS_FAM_TANH_LINEAR3 = 4,     /* A_p x + 0.1 tanh(x) − b_p : n = 3, P = A (9, column-major) + b (3)           */
+    GS_FAM_HENON_HEILES_MIDPOINT = 5, /* implicit-midpoint step of Hénon–Heiles : n = 4, P = x_k (4), shared[0] = h */
+    GS_FAM_FPU_GL3 = 6,          /* Gauss–Legendre s = 3 stage system of a 16-site FPU-β chain : n = 48           */
+    GS_FAM_EXPSUM_LSQ = 7        /* exponential-sum least-squares fit (LevenbergMarquardt)                       */
+};
+
+typedef struct gs_family {
+    int32_t id;       /* gs_family_id                                   */
+    int32_t n;        /* unknowns per system                            */
+    int32_t m;        /* residual entries per system (= n when square)  */
+    int32_t p_len;    /* scalars per parameter record                   */
+    double shared[8]; /* scalar parameters that every system shares     */
+} gs_family;
+
+/* Status form of solve_batch! (batched.jl:257-287, 311-321) on FP64 systems.
+ *   X        nsys × n doubles, AoS (Julia Vector{SVector{n,Float64}}); overwritten with the iterates.
+ *   P        parameter records (see gs_family); may be NULL for a family with p_len == 0.
+ *   p_stride byte stride between records; 0 = shared record.
+ *   S        nsys statuses (may be NULL when codes != NULL).
+ *   codes    nsys ReturnCode bytes (may be NULL when S != NULL) — the counts form's buffer.
+ *   order    NULL, or a permutation of 1..nsys (1-based, as Julia): work-item i solves order[i].
+ *            The caller validates it (isperm, batched.jl:134); the library reads it as given.
+ *   workgroupsize 0 = library default, else the workgroup size of the launch (1..1024).
+ */
+int gs_solve_batch_f64(const gs_method* method, const gs_family* family, int64_t nsys,
+                       double* X, const void* P, int64_t p_stride, gs_status_f64* S,
+                       uint8_t* codes, const int64_t* order, const gs_options_f64* options,
+                       int32_t workgroupsize, void* stream);
+
+/* The six-count tally of a codes buffer (batched.jl:197-199), on the device; counts[6] is host memory. */
+int gs_tally_codes(const uint8_t* codes, int64_t nsys, int64_t counts[6], void* stream);
+
+/* batch_summary(S) (batched.jl:80-89) on the device: out[0] = fnorm (the 2-norm of the per-system
+ * fnorm), out_i[0] = failure_count, out_i[1] = max_iterations, out_i[2] = total_iterations. */
+int gs_batch_summary_f64(const gs_status_f64* S, int64_t nsys, double* fnorm, int64_t out_i[3],
+                         void* stream);
+
+/* The message of the last failed call on this thread ("" if none). */
+const char* gs_last_error(void);
+
+/* GS_ABI_VERSION, for the load-time check of a binding. */
+int32_t gs_version(void);
+
+/* sizeof(gs_status_f64), sizeof(gs_options_f64), sizeof(gs_method), sizeof(gs_family), in order:
+ * a binding asserts these against its own layout at load time. */
+void gs_abi_sizes(int64_t out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GSOLVE_H */

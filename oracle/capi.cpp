@@ -1,0 +1,252 @@
+// CPU ORACLE — TEST INFRASTRUCTURE ONLY.
+//
+// The C entry points of the oracle library (oracle/_build/liboracle.so): the batched solve of the
+// registered families on the host, with the ABI structs of include/gsolve.h, and the building blocks
+// that the known-answer tests reach one by one. The batch loop is OpenMP `schedule(static)` over
+// systems, the host analogue of `Threads.@threads :static` over R1 solves
+// (scripts/r2_throughput.jl:119, benchmark/devices.jl:454-458).
+#include <omp.h>
+
+#include <cstdio>
+#include <cstring>
+
+#include "../include/gsolve.h"
+#include "families.hpp"
+
+namespace orc {
+
+template <class R> static Options<R> options_from(const gs_options_f64& g) {
+    Options<R> o;
+    o.f_abstol = R(g.f_abstol);
+    o.f_reltol = R(g.f_reltol);
+    o.x_abstol = R(g.x_abstol);
+    o.x_reltol = R(g.x_reltol);
+    o.min_iterations = g.min_iterations;
+    o.max_iterations = g.max_iterations;
+    o.max_stalls = g.max_stalls;
+    o.g_tol = R(g.g_tol);
+    o.norm = g.norm;
+    o.kappa = R(g.kappa);
+    o.roundoff = g.roundoff != 0;
+    o.rate_test = g.rate_test != 0;
+    return o;
+}
+
+template <class R> static Method<R> method_from(const gs_method& g) {
+    Method<R> m;
+    m.factor_f32 = g.factor_precision == GS_FACTOR_F32;
+    m.glob = g.globalization;
+    m.static_alpha = R(g.static_alpha);
+    m.bt_c1 = R(g.bt_c1);
+    m.bt_p = R(g.bt_p);
+    m.bt_maxiter = g.bt_maxiter;
+    m.refinement = g.refinement == GS_REFINEMENT;
+    m.ref_eta = R(g.ref_eta);
+    m.ref_maxiter = g.ref_maxiter;
+    m.regularization = R(g.regularization);
+    m.best = g.best != 0;
+    m.lm_mu0 = R(g.lm_mu0);
+    m.lm_mu_min = R(g.lm_mu_min);
+    m.lm_mu_max = R(g.lm_mu_max);
+    m.lm_mu_decrease = R(g.lm_mu_decrease);
+    m.lm_mu_increase = R(g.lm_mu_increase);
+    m.lm_rho_accept = R(g.lm_rho_accept);
+    return m;
+}
+
+static void write_status(gs_status_f64* out, const Status<double>& s) {
+    gs_status_f64 t;
+    std::memset(&t, 0, sizeof(t));
+    t.code = s.code;
+    t.f_calls = s.f_calls;
+    t.iterations = s.iterations;
+    t.fnorm = s.fnorm;
+    t.stepnorm = s.stepnorm;
+    t.step_failures = s.step_failures;
+    t.promoted = s.promoted;
+    t.stationary = s.stationary;
+    t.jvp_calls = s.jvp_calls;
+    t.rank = s.rank;
+    t.nullnorm = s.nullnorm;
+    t.gradratio = s.gradratio;
+    t.rate = s.rate;
+    t.contracted = s.contracted;
+    t.reused = s.reused;
+    *out = t;
+}
+
+template <int N, class MakeProb>
+static void run_batch(const Method<double>& m, const Options<double>& o, int64_t nsys, double* X,
+                      gs_status_f64* S, uint8_t* codes, const int64_t* order, int nthreads,
+                      MakeProb make) {
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+    for (int64_t i = 0; i < nsys; ++i) {
+        int64_t idx = order ? order[i] - 1 : i;
+        auto prob = make(idx);
+        Solver<double, N, decltype(prob)> solver(prob, m, o);
+        Vec<double, N> x;
+        for (int k = 0; k < N; ++k) x[k] = X[idx * N + k];
+        Status<double> st = solver.solve(x);
+        for (int k = 0; k < N; ++k) X[idx * N + k] = x[k];
+        if (S) write_status(S + idx, st);
+        if (codes) codes[idx] = st.code;
+    }
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+extern "C" {
+
+// The host restatement of gs_solve_batch_f64 (include/gsolve.h) on host pointers.
+int orc_solve_batch_f64(const gs_method* method, const gs_family* fam, int64_t nsys, double* X,
+                        const void* P, int64_t p_stride, gs_status_f64* S, uint8_t* codes,
+                        const int64_t* order, const gs_options_f64* options, int nthreads) {
+    if (!method || !fam || !options || nsys < 0) return GS_EINVAL;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    Method<double> m = method_from<double>(*method);
+    Options<double> o = options_from<double>(*options);
+    const char* Pb = static_cast<const char*>(P);
+    auto rec = [&](int64_t idx) -> const char* { return Pb + (p_stride ? idx * p_stride : 0); };
+    switch (fam->id) {
+        case GS_FAM_COUPLED2:
+            run_batch<2>(m, o, nsys, X, S, codes, order, nthreads, [&](int64_t idx) {
+                const double* p = reinterpret_cast<const double*>(rec(idx));
+                return Coupled2<double>{p[0], p[1]};
+            });
+            return 0;
+        case GS_FAM_HETERO2:
+            run_batch<2>(m, o, nsys, X, S, codes, order, nthreads, [&](int64_t idx) {
+                const int64_t* p = reinterpret_cast<const int64_t*>(rec(idx));
+                return Hetero2<double>{p[0]};
+            });
+            return 0;
+        case GS_FAM_SCALAR_SQUARE:
+            run_batch<1>(m, o, nsys, X, S, codes, order, nthreads, [&](int64_t idx) {
+                const double* p = reinterpret_cast<const double*>(rec(idx));
+                return ScalarSquare<double>{p[0]};
+            });
+            return 0;
+        case GS_FAM_TANH_LINEAR3:
+            run_batch<3>(m, o, nsys, X, S, codes, order, nthreads, [&](int64_t idx) {
+                const double* p = reinterpret_cast<const double*>(rec(idx));
+                TanhLinear3<double> pr;
+                for (int k = 0; k < 9; ++k) pr.A[k] = p[k];
+                for (int k = 0; k < 3; ++k) pr.b[k] = p[9 + k];
+                return pr;
+            });
+            return 0;
+        case GS_FAM_HENON_HEILES_MIDPOINT: {
+            double h = fam->shared[0];
+            run_batch<4>(m, o, nsys, X, S, codes, order, nthreads, [&](int64_t idx) {
+                const double* p = reinterpret_cast<const double*>(rec(idx));
+                HenonHeilesMidpoint<double> pr;
+                pr.h = h;
+                for (int k = 0; k < 4; ++k) pr.xk[k] = p[k];
+                return pr;
+            });
+            return 0;
+        }
+        default:
+            return GS_EFAMILY;
+    }
+}
+
+// --- building blocks for the known-answer tests -------------------------------------------------
+
+// factorize!! StaticLU on an n×n column-major matrix (n ≤ 8), in Float64 (tf32 = 0) or Float32 factors.
+int orc_lu_factor(int n, const double* J, int tf32, double* factors, int32_t* ipiv, int32_t* info,
+                  int32_t* scale_exponent) {
+#define ORC_LU_CASE(NN)                                                                  \
+    case NN: {                                                                           \
+        Mat<double, NN> A;                                                               \
+        for (int k = 0; k < NN * NN; ++k) A[k] = J[k];                                   \
+        if (tf32) {                                                                      \
+            auto F = lu_factorize<float, double, NN>(A);                                 \
+            for (int k = 0; k < NN * NN; ++k) factors[k] = F.f[k];                       \
+            for (int k = 0; k < NN; ++k) ipiv[k] = F.ipiv[k];                            \
+            *info = F.info;                                                              \
+            *scale_exponent = F.scale_exponent;                                          \
+        } else {                                                                         \
+            auto F = lu_factorize<double, double, NN>(A);                                \
+            for (int k = 0; k < NN * NN; ++k) factors[k] = F.f[k];                       \
+            for (int k = 0; k < NN; ++k) ipiv[k] = F.ipiv[k];                            \
+            *info = F.info;                                                              \
+            *scale_exponent = F.scale_exponent;                                          \
+        }                                                                                \
+        return 0;                                                                        \
+    }
+    switch (n) {
+        ORC_LU_CASE(1)
+        ORC_LU_CASE(2)
+        ORC_LU_CASE(3)
+        ORC_LU_CASE(4)
+        ORC_LU_CASE(5)
+        ORC_LU_CASE(6)
+        ORC_LU_CASE(7)
+        ORC_LU_CASE(8)
+        default:
+            return GS_EINVAL;
+    }
+#undef ORC_LU_CASE
+}
+
+// The LU solve J \ r through factorize!! and ldiv!! (same-type or mixed), n ≤ 8.
+int orc_lu_solve(int n, const double* J, const double* r, int tf32, double* x) {
+#define ORC_SOLVE_CASE(NN)                                                               \
+    case NN: {                                                                           \
+        Mat<double, NN> A;                                                               \
+        Vec<double, NN> b;                                                               \
+        for (int k = 0; k < NN * NN; ++k) A[k] = J[k];                                   \
+        for (int k = 0; k < NN; ++k) b[k] = r[k];                                        \
+        Vec<double, NN> y;                                                               \
+        if (tf32)                                                                        \
+            y = ldiv<float, double, NN>(lu_factorize<float, double, NN>(A), b);          \
+        else                                                                             \
+            y = ldiv<double, double, NN>(lu_factorize<double, double, NN>(A), b);        \
+        for (int k = 0; k < NN; ++k) x[k] = y[k];                                        \
+        return 0;                                                                        \
+    }
+    switch (n) {
+        ORC_SOLVE_CASE(1)
+        ORC_SOLVE_CASE(2)
+        ORC_SOLVE_CASE(3)
+        ORC_SOLVE_CASE(4)
+        ORC_SOLVE_CASE(5)
+        ORC_SOLVE_CASE(6)
+        ORC_SOLVE_CASE(7)
+        ORC_SOLVE_CASE(8)
+        default:
+            return GS_EINVAL;
+    }
+#undef ORC_SOLVE_CASE
+}
+
+// rnorm over a vector of length L (≤ 64), for the reduction KATs.
+double orc_rnorm(const double* a, int L) {
+#define ORC_RN(LL) \
+    case LL: {     \
+        std::array<double, LL> v; for (int k = 0; k < LL; ++k) v[k] = a[k]; return rnorm(v); }
+    switch (L) {
+        ORC_RN(1) ORC_RN(2) ORC_RN(3) ORC_RN(4) ORC_RN(8) ORC_RN(16)
+        default: return nan_<double>();
+    }
+#undef ORC_RN
+}
+
+// rhs_exponent and matrix_exponent of a length-4 vector, for the scaling KATs.
+int orc_rhs_exponent4(const double* r, int e) {
+    std::array<double, 4> v{r[0], r[1], r[2], r[3]};
+    return rhs_exponent(v, e);
+}
+int orc_headroom(int n) { return headroom(n); }
+
+// backtrack_step (backtracking.jl:320-340), for the interpolation KATs.
+double orc_backtrack_step(double phi0, double d0, double a, double pa, double ap, double pp, double p) {
+    return backtrack_step<double>(phi0, d0, a, pa, ap, pp, p);
+}
+
+int orc_max_threads(void) { return omp_get_max_threads(); }
+
+}  // extern "C"

@@ -1,0 +1,123 @@
+// CPU ORACLE — TEST INFRASTRUCTURE ONLY.
+//
+// The registered problem families, restated from their Julia definitions (the `F` and `J` of the
+// `NonlinearProblem`s that ext/GeometricSolversHIPExt.jl registers, and the fixtures of the
+// reference's own batched tests). Each evaluation follows the Julia expression's operation order
+// with no contraction; `nf` counts residual evaluations.
+#pragma once
+
+#include "oracle.hpp"
+
+namespace orc {
+
+// _r2_coupled_res / _r2_coupled_jac (test/nonlinear/batched.jl:92-103):
+//   F = (x₁ + x₂²/8 − p₁, x₂ + x₁²/8 − p₂),  J = [1 x₂/4; x₁/4 1].
+template <class T> struct Coupled2 {
+    static constexpr int N = 2;
+    T p0, p1;
+    long nf = 0;
+    Vec<T, 2> F(const Vec<T, 2>& x) {
+        ++nf;
+        return {(x[0] + (x[1] * x[1]) / T(8)) - p0, (x[1] + (x[0] * x[0]) / T(8)) - p1};
+    }
+    Mat<T, 2> J(const Vec<T, 2>& x) const { return {T(1), x[0] / T(4), x[1] / T(4), T(1)}; }
+};
+
+// _r2_hetero_res / _r2_hetero_jac (test/nonlinear/batched.jl:54-90): six fixtures selected by an
+// integer code, one per ReturnCode.
+template <class T> struct Hetero2 {
+    static constexpr int N = 2;
+    int64_t code;
+    long nf = 0;
+    Vec<T, 2> F(const Vec<T, 2>& x) {
+        ++nf;
+        switch (code) {
+            case 1: return {x[0] - T(2), x[1] - T(3)};
+            case 2: return {T(1), T(1)};
+            case 3: return {x[0] - T(100), x[1] - T(100)};
+            case 4: return {T(1), T(1)};
+            case 5: return {nan_<T>(), nan_<T>()};
+            default: return {x[0] * x[0] + T(1), x[1] * x[1] + T(1)};
+        }
+    }
+    Mat<T, 2> J(const Vec<T, 2>&) const {
+        switch (code) {
+            case 1: return {T(1), T(0), T(0), T(1)};
+            case 2: return {T(1e16), T(0), T(0), T(1e16)};
+            case 3: return {T(2), T(0), T(0), T(2)};
+            case 4: return {T(0), T(0), T(0), T(0)};
+            case 5: return {T(1), T(0), T(0), T(1)};
+            default: return {T(-1), T(0), T(0), T(-1)};
+        }
+    }
+};
+
+// _r2_scalar_res / _r2_scalar_jac (test/nonlinear/batched.jl:107-108): F = x² − p, J = 2x.
+template <class T> struct ScalarSquare {
+    static constexpr int N = 1;
+    T p;
+    long nf = 0;
+    Vec<T, 1> F(const Vec<T, 1>& x) {
+        ++nf;
+        return {x[0] * x[0] - p};
+    }
+    Mat<T, 1> J(const Vec<T, 1>& x) const { return {T(2) * x[0]}; }
+};
+
+// The n = 3 linear-plus-tanh family: F(x, (A, b)) = A x + 0.1 tanh.(x) − b and
+// J = A + Diagonal(0.1 (1 − tanh.(x)²)); A x is StaticArrays' muladd fold.
+template <class T> struct TanhLinear3 {
+    static constexpr int N = 3;
+    Mat<T, 3> A;
+    Vec<T, 3> b;
+    long nf = 0;
+    Vec<T, 3> F(const Vec<T, 3>& x) {
+        ++nf;
+        Vec<T, 3> Ax = matvec<T, 3>(A, x);
+        Vec<T, 3> r;
+        for (int i = 0; i < 3; ++i) r[i] = (Ax[i] + T(0.1) * std::tanh(x[i])) - b[i];
+        return r;
+    }
+    Mat<T, 3> J(const Vec<T, 3>& x) const {
+        Mat<T, 3> J = A;
+        for (int i = 0; i < 3; ++i) {
+            T t = std::tanh(x[i]);
+            J[i + 3 * i] = A[i + 3 * i] + T(0.1) * (T(1) - t * t);
+        }
+        return J;
+    }
+};
+
+// The implicit-midpoint step of the Hénon–Heiles system, x = (q₁, q₂, p₁, p₂):
+//   m = (x_k + x)/2,  f(m) = (p₁, p₂, −q₁ − 2q₁q₂, −q₂ − q₁² + q₂²),  F = x − x_k − h f(m),
+//   J = I − (h/2) Df(m).
+template <class T> struct HenonHeilesMidpoint {
+    static constexpr int N = 4;
+    T h;
+    Vec<T, 4> xk;
+    long nf = 0;
+    Vec<T, 4> F(const Vec<T, 4>& x) {
+        ++nf;
+        Vec<T, 4> m;
+        for (int i = 0; i < 4; ++i) m[i] = (xk[i] + x[i]) / T(2);
+        T f[4] = {m[2], m[3], -m[0] - (T(2) * m[0]) * m[1], (-m[1] - m[0] * m[0]) + m[1] * m[1]};
+        Vec<T, 4> r;
+        for (int i = 0; i < 4; ++i) r[i] = (x[i] - xk[i]) - h * f[i];
+        return r;
+    }
+    Mat<T, 4> J(const Vec<T, 4>& x) const {
+        T q1 = (xk[0] + x[0]) / T(2);
+        T q2 = (xk[1] + x[1]) / T(2);
+        T c = h / T(2);
+        // rows: (1, 0, −c, 0), (0, 1, 0, −c), (c(1+2q₂), c(2q₁), 1, 0), (c(2q₁), c(1−2q₂), 0, 1)
+        Mat<T, 4> J;
+        auto S = [&](int i, int j, T v) { J[i + 4 * j] = v; };
+        S(0, 0, T(1)); S(0, 1, T(0)); S(0, 2, -c); S(0, 3, T(0));
+        S(1, 0, T(0)); S(1, 1, T(1)); S(1, 2, T(0)); S(1, 3, -c);
+        S(2, 0, c * (T(1) + T(2) * q2)); S(2, 1, c * (T(2) * q1)); S(2, 2, T(1)); S(2, 3, T(0));
+        S(3, 0, c * (T(2) * q1)); S(3, 1, c * (T(1) - T(2) * q2)); S(3, 2, T(0)); S(3, 3, T(1));
+        return J;
+    }
+};
+
+}  // namespace orc
