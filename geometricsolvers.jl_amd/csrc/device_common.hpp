@@ -43,9 +43,9 @@ template <int L> GS_DEV double norm2(const double (&a)[L]) {
     return s;
 }
 
-// The scaled path of rnorm (reductions.jl:355-363), kept out of line: it runs only for a sum that
-// overflowed or fell below L·floatmin.
-template <int L> __device__ __noinline__ double rnorm_scaled(const double (&a)[L]) {
+// The scaled path of rnorm (reductions.jl:355-363): it runs only for a sum that overflowed or fell
+// below L·floatmin. Inline, so that no array becomes addressable (and no lane touches scratch).
+template <int L> GS_DEV double rnorm_scaled(const double (&a)[L]) {
     double m = __builtin_fabs(a[0]);
 #pragma unroll
     for (int i = 1; i < L; ++i) m = jmax(m, __builtin_fabs(a[i]));
@@ -64,7 +64,7 @@ template <int L> __device__ __noinline__ double rnorm_scaled(const double (&a)[L
 // rnorm (reductions.jl:41-57).
 template <int L> GS_DEV double rnorm(const double (&a)[L]) {
     double s = norm2(a);
-    if (isfin(s) && s >= double(L) * kDblMin) return __builtin_sqrt(s);
+    if (__builtin_expect(isfin(s) && s >= double(L) * kDblMin, 1)) return __builtin_sqrt(s);
     return rnorm_scaled(a);
 }
 

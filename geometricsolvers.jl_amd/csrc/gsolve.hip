@@ -24,6 +24,11 @@ int set_error(int code, const std::string& msg) {
 
 static int fail(int code, const std::string& msg) { return set_error(code, msg); }
 
+// Kernel timing (gs_set_kernel_timing): events are created once per thread and reused.
+static thread_local bool g_timing = false;
+static thread_local double g_last_ms = -1.0;
+static thread_local hipEvent_t g_ev[2] = {nullptr, nullptr};
+
 struct KernelArgs {
     FamArgs fam;
     DevParams q;
@@ -52,9 +57,12 @@ GS_DEV void store_status(gs_status_f64* dst, uint8_t code, int32_t iterations, d
     w[8] = uint64_t(contracted ? 1 : 0);
 }
 
+// The largest workgroup of the lane-per-system kernels (their launch bound) and their default.
+constexpr int kSmallMaxWorkgroup = 256;
+
 // One work-item per system (the launch shape of `_solve_kernel!`, batched.jl:171-179).
 template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG>
-__global__ __launch_bounds__(256) void solve_small_kernel(const KernelArgs a) {
+__global__ __launch_bounds__(kSmallMaxWorkgroup) void solve_small_kernel(const KernelArgs a) {
     constexpr int N = Fam::N;
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= a.nsys) return;
@@ -202,11 +210,21 @@ int gs_solve_batch_f64(const gs_method* method, const gs_family* family, int64_t
 
     const bool diag = S != nullptr;
     if (!diag && !codes) return fail(GS_EINVAL, "codes must not be NULL");
-    const int wg = workgroupsize > 0 ? workgroupsize : 256;
+    if (workgroupsize > kSmallMaxWorkgroup)
+        return fail(GS_EINVAL, "workgroupsize " + std::to_string(workgroupsize) +
+                                   " exceeds the solve kernel's maximum of " + std::to_string(kSmallMaxWorkgroup) +
+                                   " work-items");
+    const int wg = workgroupsize > 0 ? workgroupsize : kSmallMaxWorkgroup;
     hipStream_t st = static_cast<hipStream_t>(stream);
     const bool mixed = method->factor_precision == GS_FACTOR_F32;
     const bool refine = method->refinement == GS_REFINEMENT;
     const int glob = method->globalization;
+    bool timed = false;
+    if (g_timing) {
+        if (!g_ev[0] && (hipEventCreate(&g_ev[0]) != hipSuccess || hipEventCreate(&g_ev[1]) != hipSuccess))
+            return fail(GS_EHIP, "hipEventCreate failed");
+        timed = hipEventRecord(g_ev[0], st) == hipSuccess;
+    }
     int rc;
     switch (family->id) {
         case GS_FAM_COUPLED2: rc = launch_family<FamCoupled2>(a, glob, mixed, refine, diag, wg, st); break;
@@ -219,11 +237,19 @@ int gs_solve_batch_f64(const gs_method* method, const gs_family* family, int64_t
         default: return fail(GS_EFAMILY, "problem family is not registered");
     }
     if (rc != GS_OK) return rc;
+    if (timed) timed = hipEventRecord(g_ev[1], st) == hipSuccess;
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(GS_EHIP, std::string("kernel launch failed: ") + hipGetErrorString(e));
     e = hipStreamSynchronize(st);
     if (e != hipSuccess) return fail(GS_EHIP, std::string("hipStreamSynchronize failed: ") + hipGetErrorString(e));
+    g_last_ms = -1.0;
+    float ms = 0.0f;
+    if (timed && hipEventElapsedTime(&ms, g_ev[0], g_ev[1]) == hipSuccess) g_last_ms = ms;
     return GS_OK;
 }
+
+void gs_set_kernel_timing(int enable) { g_timing = enable != 0; }
+
+double gs_last_kernel_ms(void) { return g_last_ms; }
 
 }  // extern "C"

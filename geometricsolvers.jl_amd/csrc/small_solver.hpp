@@ -145,21 +145,35 @@ template <class TF, int N> GS_DEV void factorize(const double (&J)[N][N], Factor
     constexpr bool same = sizeof(TF) == sizeof(double);
     bool scale = e != 0 && (!same || e < 0);
     f.e = scale ? e : 0;
+    // the prescale is exact, and a wave in which no lane scales skips it
+    const int s = scale ? -e : 0;
+    if (__any(scale)) {
 #pragma unroll
-    for (int i = 0; i < N; ++i)
+        for (int i = 0; i < N; ++i)
 #pragma unroll
-        for (int j = 0; j < N; ++j) f.A[i][j] = TF(scale ? ldexp2(J[i][j], -e) : J[i][j]);
+            for (int j = 0; j < N; ++j) f.A[i][j] = TF(ldexp2(J[i][j], s));
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+#pragma unroll
+            for (int j = 0; j < N; ++j) f.A[i][j] = TF(J[i][j]);
+    }
     f.info = lu_inplace<TF, N>(f.A, f.piv);
 }
 
 // ldiv!! in the working precision (factors.jl:931-936).
 template <int N> GS_DEV void ldiv_same(const Factor<double, N>& f, const double (&r)[N], double (&d)[N]) {
-    int k = rhs_exponent<N>(r, f.e);
+    const int k = rhs_exponent<N>(r, f.e);
+    if (__any(k != 0)) {
 #pragma unroll
-    for (int i = 0; i < N; ++i) d[i] = k != 0 ? ldexp2(r[i], -k) : r[i];
+        for (int i = 0; i < N; ++i) d[i] = ldexp2(r[i], -k);  // ldexp by 0 is exact
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i) d[i] = r[i];
+    }
     lu_backsolve<double, N>(f.A, f.piv, d);
-    int u = k - f.e;
-    if (u != 0) {
+    const int u = k - f.e;
+    if (__any(u != 0)) {
 #pragma unroll
         for (int i = 0; i < N; ++i) d[i] = ldexp2(d[i], u);
     }
@@ -200,7 +214,7 @@ template <int N> GS_DEV void matvec(const double (&A)[N][N], const double (&v)[N
 }
 
 // ‖Jᴴ F‖/(‖J‖_F ‖F‖) (grad_ratio_and_norm, newton.jl:1287-1298).
-template <int N> __device__ __noinline__ double grad_ratio_scaled(const double (&J)[N][N], const double (&Fx)[N], double jnorm, double fnorm) {
+template <int N> GS_DEV double grad_ratio_scaled(const double (&J)[N][N], const double (&Fx)[N], double jnorm, double fnorm) {
     int e = exponent_of(fnorm);
     double sc = ldexp2(1.0, -e);
     double g[N];
@@ -231,7 +245,7 @@ template <int N> GS_DEV double grad_ratio(const double (&J)[N][N], const double 
         g[k] = acc;
     }
     double gnorm = rnorm<N>(g);
-    if (isfin(gnorm) || !(isfin(jnorm) && isfin(fnorm))) return gnorm / jnorm / fnorm;
+    if (__builtin_expect(isfin(gnorm) || !(isfin(jnorm) && isfin(fnorm)), 1)) return gnorm / jnorm / fnorm;
     return grad_ratio_scaled<N>(J, Fx, jnorm, fnorm);
 }
 

@@ -1,0 +1,76 @@
+"""Synthetic batches of the benchmark configurations, from a documented generator.
+
+Julia's Xoshiro/randn streams cannot be reproduced without Julia, so every batch here comes from
+numpy's PCG64 (`numpy.random.default_rng(seed)`), drawn in the order written below. The oracle and
+the device solve always receive the same arrays.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+
+def henon_heiles_potential(q1, q2):
+    """V(q) = ½(q₁² + q₂²) + q₁²q₂ − q₂³/3."""
+    return 0.5 * (q1 * q1 + q2 * q2) + q1 * q1 * q2 - q2 ** 3 / 3.0
+
+
+def henon_heiles_states(nsys: int, seed: int = 0, e_lo: float = 0.01, e_hi: float = 0.15) -> np.ndarray:
+    """States x_k = (q₁, q₂, p₁, p₂) on energy shells E ~ U[e_lo, e_hi] (below the escape energy 1/6).
+
+    Per system: E, then q uniform in the bounded component of {V(q) ≤ E} — the inside of the
+    triangle q₂ > −½, q₂ < 1 ± √3 q₁ whose sides are the level set V = 1/6 — by rejection from the
+    box [−1, 1] × [−½, 1], then |p| = √(2(E − V(q))) at a uniform angle θ ∈ [0, 2π).
+    """
+    rng = np.random.default_rng(seed)
+    E = rng.uniform(e_lo, e_hi, size=nsys)
+    q = np.empty((nsys, 2))
+    todo = np.arange(nsys)
+    s3 = math.sqrt(3.0)
+    while todo.size:
+        c1 = rng.uniform(-1.0, 1.0, size=todo.size)
+        c2 = rng.uniform(-0.5, 1.0, size=todo.size)
+        inside = (c2 > -0.5) & (c2 < 1.0 - s3 * c1) & (c2 < 1.0 + s3 * c1)
+        ok = inside & (henon_heiles_potential(c1, c2) <= E[todo])
+        q[todo[ok], 0] = c1[ok]
+        q[todo[ok], 1] = c2[ok]
+        todo = todo[~ok]
+    V = henon_heiles_potential(q[:, 0], q[:, 1])
+    pm = np.sqrt(np.maximum(2.0 * (E - V), 0.0))
+    theta = rng.uniform(0.0, 2.0 * math.pi, size=nsys)
+    x = np.empty((nsys, 4))
+    x[:, 0:2] = q
+    x[:, 2] = pm * np.cos(theta)
+    x[:, 3] = pm * np.sin(theta)
+    return x
+
+
+def henon_heiles_energy(x: np.ndarray) -> np.ndarray:
+    return 0.5 * (x[:, 2] ** 2 + x[:, 3] ** 2) + henon_heiles_potential(x[:, 0], x[:, 1])
+
+
+def henon_heiles_midpoint_residual(x: np.ndarray, xk: np.ndarray, h: float) -> np.ndarray:
+    """F(x) = x − x_k − h f((x_k + x)/2), vectorised over systems (the convergence recheck)."""
+    m = (xk + x) / 2.0
+    f = np.stack([m[:, 2], m[:, 3], -m[:, 0] - 2.0 * m[:, 0] * m[:, 1],
+                  -m[:, 1] - m[:, 0] ** 2 + m[:, 1] ** 2], axis=1)
+    return (x - xk) - h * f
+
+
+def tanh_linear3_batch(nsys: int, seed: int = 0):
+    """A_p = 3I + 0.5·N(0,1) per entry (9 draws, column-major), then b_p ~ N(0,1) (3 draws), per
+    system in that order; x₀ = 0. Returns (records (nsys, 12): vec(A_p) then b_p, x₀ (nsys, 3))."""
+    rng = np.random.default_rng(seed)
+    z = rng.standard_normal(size=(nsys, 12))
+    rec = np.empty((nsys, 12))
+    rec[:, :9] = 0.5 * z[:, :9]
+    for i in range(3):
+        rec[:, i + 3 * i] += 3.0
+    rec[:, 9:] = z[:, 9:]
+    return rec, np.zeros((nsys, 3))
+
+
+def tanh_linear3_residual(x: np.ndarray, rec: np.ndarray) -> np.ndarray:
+    A = rec[:, :9].reshape(-1, 3, 3).transpose(0, 2, 1)  # column-major records
+    return np.einsum("sij,sj->si", A, x) + 0.1 * np.tanh(x) - rec[:, 9:]

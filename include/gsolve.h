@@ -170,6 +170,12 @@ int gs_tally_codes(const uint8_t* codes, int64_t nsys, int64_t counts[6], void* 
 int gs_batch_summary_f64(const gs_status_f64* S, int64_t nsys, double* fnorm, int64_t out_i[3],
                          void* stream);
 
+/* Instrumentation, with no counterpart in the reference: when enabled, gs_solve_batch_f64 brackets
+ * its kernel with two HIP events on `stream`, and gs_last_kernel_ms returns the elapsed time of the
+ * last solve kernel of this thread (ms), or a negative value if none was timed. */
+void gs_set_kernel_timing(int enable);
+double gs_last_kernel_ms(void);
+
 /* The message of the last failed call on this thread ("" if none). */
 const char* gs_last_error(void);
 

@@ -249,4 +249,27 @@ double orc_backtrack_step(double phi0, double d0, double a, double pa, double ap
 
 int orc_max_threads(void) { return omp_get_max_threads(); }
 
+// One solve of a loop-test fixture (families.hpp, 101-103) with parameters p; x is overwritten.
+int orc_solve_fixture(int fixture, const gs_method* method, const gs_options_f64* options, double* x,
+                      const double* p, gs_status_f64* S) {
+    Method<double> m = method_from<double>(*method);
+    Options<double> o = options_from<double>(*options);
+    auto run = [&](auto prob) {
+        constexpr int N = decltype(prob)::N;
+        Solver<double, N, decltype(prob)> solver(prob, m, o);
+        Vec<double, N> v;
+        for (int k = 0; k < N; ++k) v[k] = x[k];
+        Status<double> st = solver.solve(v);
+        for (int k = 0; k < N; ++k) x[k] = v[k];
+        write_status(S, st);
+        return 0;
+    };
+    switch (fixture) {
+        case 101: return run(DiagSquare2<double>{p[0], p[1]});
+        case 102: return run(Affine2<double>{p[0], p[1]});
+        case 103: return run(InfJacobian1<double>{});
+    }
+    return GS_EINVAL;
+}
+
 }  // extern "C"

@@ -120,4 +120,38 @@ template <class T> struct HenonHeilesMidpoint {
     }
 };
 
+// Fixtures of the reference's loop tests (test/nonlinear/solver.jl:183-290), for the oracle's KATs:
+//   101: y.^2 − q, J = Diagonal(2y), n = 2         (solver.jl:242-246)
+//   102: 2y − q,  J = 2I, n = 2                    (solver.jl:279-281)
+//   103: x·x − 4, J = Inf, n = 1 (a zero step)     (solver.jl:185)
+template <class T> struct DiagSquare2 {
+    static constexpr int N = 2;
+    T q0, q1;
+    long nf = 0;
+    Vec<T, 2> F(const Vec<T, 2>& y) {
+        ++nf;
+        return {y[0] * y[0] - q0, y[1] * y[1] - q1};
+    }
+    Mat<T, 2> J(const Vec<T, 2>& y) const { return {T(2) * y[0], T(0), T(0), T(2) * y[1]}; }
+};
+template <class T> struct Affine2 {
+    static constexpr int N = 2;
+    T q0, q1;
+    long nf = 0;
+    Vec<T, 2> F(const Vec<T, 2>& y) {
+        ++nf;
+        return {T(2) * y[0] - q0, T(2) * y[1] - q1};
+    }
+    Mat<T, 2> J(const Vec<T, 2>&) const { return {T(2), T(0), T(0), T(2)}; }
+};
+template <class T> struct InfJacobian1 {
+    static constexpr int N = 1;
+    long nf = 0;
+    Vec<T, 1> F(const Vec<T, 1>& x) {
+        ++nf;
+        return {x[0] * x[0] - T(4)};
+    }
+    Mat<T, 1> J(const Vec<T, 1>&) const { return {std::numeric_limits<T>::infinity()}; }
+};
+
 }  // namespace orc

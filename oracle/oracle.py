@@ -1,0 +1,176 @@
+"""CPU ORACLE — TEST INFRASTRUCTURE ONLY.
+
+ctypes binding of oracle/_build/liboracle.so, the C++ restatement of the reference's R1 solve
+(oracle.hpp). Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg may use it, as
+the checker or as the timed CPU baseline; the product path never does.
+"""
+from __future__ import annotations
+
+import ctypes
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "_build" / "liboracle.so"
+
+_LIB = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            build()
+        L = ctypes.CDLL(str(LIB_PATH))
+        vp = ctypes.c_void_p
+        L.orc_solve_batch_f64.restype = ctypes.c_int
+        L.orc_solve_batch_f64.argtypes = [vp, vp, ctypes.c_int64, vp, vp, ctypes.c_int64, vp, vp, vp, vp, ctypes.c_int]
+        L.orc_standard_cell.restype = ctypes.c_int
+        L.orc_standard_cell.argtypes = [ctypes.c_int] * 4 + [vp] * 5
+        L.orc_standard_u0.restype = ctypes.c_int
+        L.orc_standard_u0.argtypes = [ctypes.c_int, ctypes.c_int, vp]
+        L.orc_lu_factor.restype = ctypes.c_int
+        L.orc_lu_factor.argtypes = [ctypes.c_int, vp, ctypes.c_int, vp, vp, vp, vp]
+        L.orc_lu_solve.restype = ctypes.c_int
+        L.orc_lu_solve.argtypes = [ctypes.c_int, vp, vp, ctypes.c_int, vp]
+        L.orc_rnorm.restype = ctypes.c_double
+        L.orc_rnorm.argtypes = [vp, ctypes.c_int]
+        L.orc_rhs_exponent4.restype = ctypes.c_int
+        L.orc_rhs_exponent4.argtypes = [vp, ctypes.c_int]
+        L.orc_headroom.restype = ctypes.c_int
+        L.orc_headroom.argtypes = [ctypes.c_int]
+        L.orc_backtrack_step.restype = ctypes.c_double
+        L.orc_backtrack_step.argtypes = [ctypes.c_double] * 7
+        L.orc_max_threads.restype = ctypes.c_int
+        L.orc_solve_fixture.restype = ctypes.c_int
+        L.orc_solve_fixture.argtypes = [ctypes.c_int, vp, vp, vp, vp, vp]
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a: np.ndarray):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def host_records(family, p, nsys: int):
+    """The parameter records of a batch on the host: (array, byte stride, shared scalars)."""
+    name = type(family).__name__
+    if name == "HenonHeilesMidpoint":
+        rec = np.ascontiguousarray(np.asarray(p.per_system, dtype=np.float64).reshape(nsys, 4))
+        return rec, rec.strides[0], (float(p.shared),)
+    dtype = np.int64 if name == "Hetero2" else np.float64
+    if hasattr(p, "data"):
+        rec = np.ascontiguousarray(np.asarray(p.data, dtype=dtype).reshape(nsys, family.p_len))
+        return rec, rec.strides[0], ()
+    rec = np.ascontiguousarray(np.asarray(p, dtype=dtype).reshape(1, family.p_len))
+    return rec, 0, ()
+
+
+def solve_batch(method, family, X: np.ndarray, p, options, order=None, nthreads: int = 0):
+    """The oracle's batched solve on host arrays: returns (X, statuses) with X overwritten.
+
+    `method`, `options`: objects with `to_c()` giving the ABI structs of include/gsolve.h;
+    `family`: a registered family (id, n, p_len); `order`: 1-based permutation or None.
+    """
+    from_pkg_status = _status_dtype()
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    nsys = X.shape[0]
+    rec, stride, shared = host_records(family, p, nsys)
+    cm = method.to_c()
+    co = options.to_c()
+    fam = _family_struct(family, shared)
+    S = np.zeros(nsys, dtype=from_pkg_status)
+    o = None if order is None else np.ascontiguousarray(np.asarray(order, dtype=np.int64))
+    rc = lib().orc_solve_batch_f64(
+        ctypes.byref(cm), ctypes.byref(fam), nsys, _ptr(X), _ptr(rec), stride, _ptr(S), None,
+        None if o is None else _ptr(o), ctypes.byref(co), nthreads)
+    if rc != 0:
+        raise RuntimeError(f"oracle failed with {rc}")
+    return X, S
+
+
+def _status_dtype():
+    return np.dtype({
+        "names": ["code", "f_calls", "iterations", "fnorm", "stepnorm", "step_failures", "promoted",
+                  "stationary", "jvp_calls", "rank", "nullnorm", "gradratio", "rate", "contracted", "reused"],
+        "formats": ["u1", "<i2", "<i4", "<f8", "<f8", "<i4", "u1", "u1", "<i2", "<i4", "<f8", "<f8", "<f8", "u1", "u1"],
+        "offsets": [0, 2, 4, 8, 16, 24, 28, 29, 30, 32, 40, 48, 56, 64, 65],
+        "itemsize": 72,
+    })
+
+
+class _Fam(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_int32), ("n", ctypes.c_int32), ("m", ctypes.c_int32),
+                ("p_len", ctypes.c_int32), ("shared", ctypes.c_double * 8)]
+
+
+def _family_struct(family, shared) -> _Fam:
+    f = _Fam()
+    f.id, f.n, f.m, f.p_len = family.id, family.n, family.n, family.p_len
+    for k, v in enumerate(shared):
+        f.shared[k] = v
+    return f
+
+
+CODE_NAMES = ("SUCCESS", "STALLED", "MAXITERS", "SINGULAR", "NONFINITE", "LINESEARCH_FAILED")
+
+
+def standard_cell(problem: int, scale: int, column: int, eltype: int):
+    """One cell of the standard set: (code name, iterations, f_evals, fnorm, x)."""
+    n = lib().orc_standard_u0(problem, eltype, None)
+    code, it, nf, fn = ctypes.c_int(), ctypes.c_int(), ctypes.c_long(), ctypes.c_double()
+    x = np.zeros(n)
+    lib().orc_standard_cell(problem, scale, column, eltype, ctypes.byref(code), ctypes.byref(it),
+                            ctypes.byref(nf), ctypes.byref(fn), _ptr(x))
+    return CODE_NAMES[code.value], it.value, nf.value, fn.value, x
+
+
+def lu_factor(J: np.ndarray, f32: bool = False):
+    """factorize!! StaticLU of a square matrix (n ≤ 8): (factors, ipiv (1-based), info, e)."""
+    J = np.asfortranarray(J, dtype=np.float64)
+    n = J.shape[0]
+    fac = np.zeros(n * n)
+    ipiv = np.zeros(n, dtype=np.int32)
+    info, e = ctypes.c_int32(), ctypes.c_int32()
+    rc = lib().orc_lu_factor(n, _ptr(J.reshape(-1, order="F").copy()), int(f32), _ptr(fac), _ptr(ipiv),
+                             ctypes.byref(info), ctypes.byref(e))
+    assert rc == 0
+    return fac.reshape(n, n, order="F"), ipiv, info.value, e.value
+
+
+def lu_solve(J: np.ndarray, r: np.ndarray, f32: bool = False) -> np.ndarray:
+    J = np.asarray(J, dtype=np.float64)
+    n = J.shape[0]
+    x = np.zeros(n)
+    flat = np.ascontiguousarray(J.reshape(-1, order="F"))
+    rr = np.ascontiguousarray(r, dtype=np.float64)
+    assert lib().orc_lu_solve(n, _ptr(flat), _ptr(rr), int(f32), _ptr(x)) == 0
+    return x
+
+
+def rnorm(a) -> float:
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return lib().orc_rnorm(_ptr(a), len(a))
+
+
+def solve_fixture(fixture: int, method, options, x, p=(0.0, 0.0)):
+    """One solve of a loop-test fixture (101 y.^2 − q, 102 2y − q, 103 x² − 4 with J = Inf)."""
+    x = np.ascontiguousarray(np.atleast_1d(np.asarray(x, dtype=np.float64))).copy()
+    pp = np.ascontiguousarray(np.asarray(p, dtype=np.float64))
+    S = np.zeros(1, dtype=_status_dtype())
+    cm, co = method.to_c(), options.to_c()
+    rc = lib().orc_solve_fixture(fixture, ctypes.byref(cm), ctypes.byref(co), _ptr(x), _ptr(pp), _ptr(S))
+    if rc != 0:
+        raise RuntimeError(f"oracle fixture failed with {rc}")
+    return x, S[0]
+
+
+def max_threads() -> int:
+    return lib().orc_max_threads()

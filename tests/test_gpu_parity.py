@@ -1,0 +1,278 @@
+"""Parity of the HIP batched solve with the CPU oracle, through the C ABI (libgsolve_hip).
+
+The bar: return codes and iteration counts identical (except, for a family with transcendental
+functions, systems whose final residual is within 1% of the tolerance), iterates within 1e-10
+relative, and the floating-point status fields within 8 eps relative — the tolerance of the
+reference's own R2-vs-R1 comparison `status_agrees` (test/nonlinear/batched.jl:42-52). For the
+polynomial families, whose device arithmetic follows the oracle's operation order exactly, the
+results are compared bit for bit as well.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+EPS = np.finfo(float).eps
+FLOAT_FIELDS = ("fnorm", "stepnorm", "gradratio", "rate", "nullnorm")
+INT_FIELDS = ("code", "f_calls", "iterations", "step_failures", "promoted", "stationary", "jvp_calls",
+              "rank", "contracted", "reused")
+
+
+def _close(a, b, rtol):
+    a = np.asarray(a, dtype=float)
+    b = np.asarray(b, dtype=float)
+    both_nan = np.isnan(a) & np.isnan(b)
+    same = a == b
+    with np.errstate(invalid="ignore"):
+        rel = np.abs(a - b) <= rtol * np.maximum(np.abs(a), np.abs(b))
+    return both_nan | same | rel
+
+
+def assert_parity(st, x, so, xo, *, bitwise=False, near_tol=None, tol=None):
+    """Compare device statuses/iterates (st, x) with the oracle's (so, xo)."""
+    mask = np.ones(len(st), dtype=bool)
+    if near_tol is not None:
+        # systems whose final residual lies within 1% of the tolerance may differ in code
+        mask = ~((np.abs(so["fnorm"] - tol) <= 0.01 * tol) | (np.abs(st["fnorm"] - tol) <= 0.01 * tol))
+    for f in INT_FIELDS:
+        bad = np.nonzero((st[f] != so[f]) & mask)[0]
+        assert bad.size == 0, f"{f} differs at {bad[:10]}: device {st[f][bad[:5]]}, oracle {so[f][bad[:5]]}"
+    for f in FLOAT_FIELDS:
+        if near_tol is None:
+            ok = _close(st[f], so[f], 8 * EPS) | ~mask
+        else:
+            # a transcendental residual differs in its last bit between ROCm's and glibc's libm, so
+            # the norms at the round-off floor (below 1e-10) agree in magnitude only
+            a, b = st[f], so[f]
+            floor = (np.abs(a) < 1e-10) & (np.abs(b) < 1e-10)
+            if f in ("rate", "gradratio"):
+                # the rate, and the gradient ratio at the last factorised iterate, of a last step
+                # that started at the floor are ratios of floor-level norms
+                floor |= (st["stepnorm"] < 1e-10) | (so["stepnorm"] < 1e-10)
+            ok = _close(a, b, 1e-6) | floor | ~mask
+        assert ok.all(), f"{f} differs at {np.nonzero(~ok)[0][:10]}"
+    fin = np.isfinite(xo)
+    scale = np.maximum(np.where(fin, np.abs(xo), 0.0).max(axis=1, keepdims=True), 1e-300)
+    with np.errstate(invalid="ignore"):
+        rel = np.where(fin, np.abs(np.where(fin, x, 0.0) - np.where(fin, xo, 0.0)) / scale, 0.0)
+    # a non-finite oracle entry must be the same non-finite value on the device
+    rel = np.where(~fin & ~((np.isnan(xo) & np.isnan(x)) | (xo == x)), np.inf, rel)
+    rel = np.where(fin & ~np.isfinite(x), np.inf, rel)
+    assert (rel[mask] <= 1e-10).all(), f"iterates differ by {rel[mask].max():.3e}"
+    if bitwise:
+        assert np.array_equal(x.view(np.uint64), xo.view(np.uint64)), "iterates differ in bits"
+        assert np.array_equal(st.view(np.uint8).reshape(len(st), 72)[:, :66],
+                              so.view(np.uint8).reshape(len(so), 72)[:, :66]), "statuses differ in bits"
+
+
+def same_bytes(a, b):
+    """Equality of two status arrays byte for byte (NaN fields compare equal to themselves)."""
+    return a.shape == b.shape and np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
+def run_gpu(gs, device, method, family, X0, p_host, opts, order=None, workgroupsize=None, counts=False):
+    X = torch.as_tensor(X0, device=device).clone()
+    if isinstance(p_host, gs.PerSystem):
+        p = gs.PerSystem(torch.as_tensor(np.asarray(p_host.data), device=device))
+    elif isinstance(p_host, gs.BatchParams):
+        p = gs.BatchParams(p_host.shared, torch.as_tensor(np.asarray(p_host.per_system), device=device))
+    else:
+        p = p_host
+    prob = gs.NonlinearProblem(family)
+    if counts:
+        c = gs.solve_batch(X, prob, method, p, opts, order=order, workgroupsize=workgroupsize)
+        return X.cpu().numpy(), c
+    S = gs.status_buffer(X.shape[0], device)
+    gs.solve_batch(X, S, prob, method, p, opts, order=order, workgroupsize=workgroupsize)
+    return X.cpu().numpy(), gs.statuses(S)
+
+
+def run_both(gs, orc, device, method, family, X0, p, opts, **kw):
+    x, st = run_gpu(gs, device, method, family, X0, p, opts, **kw)
+    xo, so = orc.solve_batch(method, family, X0.copy(), p, opts, order=kw.get("order"))
+    return x, st, xo, so
+
+
+HH_METHODS = {
+    "newton-static": lambda gs: gs.Newton(),
+    "newton-backtracking": lambda gs: gs.Newton(globalization=gs.Backtracking()),
+    "mixed-static-refine": lambda gs: gs.Newton(linsolve=gs.LUFactorization("float32"), refinement=gs.Refinement()),
+    "mixed-backtracking-refine": lambda gs: gs.Newton(linsolve=gs.LUFactorization("float32"),
+                                                      globalization=gs.Backtracking(), refinement=gs.Refinement()),
+    "mixed-static-norefine": lambda gs: gs.Newton(linsolve=gs.LUFactorization("float32")),
+    "plain-refine": lambda gs: gs.Newton(refinement=gs.Refinement()),
+    "best-regularized": lambda gs: gs.Newton(best=True, regularization=1e-3),
+}
+
+
+@pytest.mark.parametrize("name", list(HH_METHODS))
+def test_henon_heiles_midpoint_parity(gs, orc, workloads, device, name):
+    """The n = 4 implicit-midpoint family at h = 0.1 on 65536 seeded states, every method form."""
+    n = 65536
+    xk = workloads.henon_heiles_states(n, seed=1)
+    m = HH_METHODS[name](gs)
+    opts = gs.Options(f_abstol=1e-12, f_reltol=0.0)
+    x, st, xo, so = run_both(gs, orc, device, m, gs.HenonHeilesMidpoint(), xk.copy(), gs.BatchParams(0.1, xk), opts)
+    assert_parity(st, x, so, xo, bitwise=True)
+    assert (st["code"] == gs.SUCCESS).mean() > 0.99
+
+
+def test_henon_heiles_hard_steps(gs, orc, workloads, device):
+    """Large steps (h = 2) from far-out starts: a mix of codes, with Backtracking and MaxNorm."""
+    n = 16384
+    xk = workloads.henon_heiles_states(n, seed=2, e_lo=0.1, e_hi=0.166)
+    for m in (gs.Newton(), gs.Newton(globalization=gs.Backtracking())):
+        for opts in (gs.Options(f_abstol=1e-12, f_reltol=0.0, max_iterations=8),
+                     gs.Options(norm=gs.MaxNorm(), min_iterations=1, x_reltol=1e-6, roundoff=True)):
+            x, st, xo, so = run_both(gs, orc, device, m, gs.HenonHeilesMidpoint(), xk * 3.0, gs.BatchParams(2.0, xk * 3.0), opts)
+            assert_parity(st, x, so, xo, bitwise=True)
+            assert len(np.unique(st["code"])) >= 2
+
+
+def test_full_size_henon_heiles(gs, orc, workloads, device):
+    """The benchmark size, 2^20 systems: every code and iterate against the oracle, and the FP64
+    residual recheck ‖F(x)‖₂ ≤ 1e-12 of every SUCCESS."""
+    n = 1 << 20
+    xk = workloads.henon_heiles_states(n, seed=0)
+    opts = gs.Options(f_abstol=1e-12, f_reltol=0.0)
+    x, st, xo, so = run_both(gs, orc, device, gs.Newton(), gs.HenonHeilesMidpoint(), xk.copy(), gs.BatchParams(0.1, xk), opts)
+    assert_parity(st, x, so, xo, bitwise=True)
+    ok = st["code"] == gs.SUCCESS
+    r = np.sqrt((workloads.henon_heiles_midpoint_residual(x, xk, 0.1) ** 2).sum(axis=1))
+    assert ok.all() and (r[ok] <= 1e-12).all()
+    # the implicit midpoint rule conserves the quadratic part only; the energy drift stays O(h²)
+    assert np.abs(workloads.henon_heiles_energy(x) - workloads.henon_heiles_energy(xk)).max() < 1e-3
+
+
+def test_tanh_linear3_parity(gs, orc, workloads, device):
+    """The n = 3 family of the CPU configuration (batch 4096, max_iterations 50, seed 0) — tanh is
+    ROCm's on the device and glibc's on the host, so codes may differ within 1% of the tolerance."""
+    rec, x0 = workloads.tanh_linear3_batch(4096, 0)
+    for m in (gs.Newton(), gs.Newton(globalization=gs.Backtracking())):
+        opts = gs.Options(f_abstol=1e-12, f_reltol=0.0, max_iterations=50)
+        x, st, xo, so = run_both(gs, orc, device, m, gs.TanhLinear3(), x0.copy(), gs.PerSystem(rec), opts)
+        assert_parity(st, x, so, xo, near_tol=True, tol=1e-12)
+        r = np.sqrt((workloads.tanh_linear3_residual(x, rec) ** 2).sum(axis=1))
+        assert (r[st["code"] == gs.SUCCESS] <= 1e-12 * (1 + 1e-9)).all()
+
+
+def test_coupled_exact_root_singular_and_counts(gs, orc, device):
+    """test/nonlinear/batched.jl:263-375: bit-identical exact root with 0 iterations, a singular
+    start, and the counts form."""
+    P = np.array([[2.125, 1.5], [1.25, 2.0], [1.5, 2.5], [1.75, 3.0]])
+    X0 = np.array([[2.0, 1.0], [2.5, 1.5], [3.5, 2.5], [4.0, 3.0]])
+    opts = gs.Options(max_iterations=150)
+    for m in (gs.Newton(), gs.Newton(globalization=gs.Backtracking())):
+        x, st, xo, so = run_both(gs, orc, device, m, gs.Coupled2(), X0.copy(), gs.PerSystem(P), opts)
+        assert_parity(st, x, so, xo, bitwise=True)
+        assert st["iterations"][0] == 0 and np.array_equal(x[0], X0[0])
+        assert (st["code"] == gs.SUCCESS).all()
+        _, counts = run_gpu(gs, device, m, gs.Coupled2(), X0.copy(), gs.PerSystem(P), opts, counts=True)
+        assert counts == dict(SUCCESS=4, STALLED=0, MAXITERS=0, SINGULAR=0, NONFINITE=0, LINESEARCH_FAILED=0)
+    Xs = X0.copy()
+    Xs[1] = [4.0, 4.0]
+    x, st, xo, so = run_both(gs, orc, device, gs.Newton(), gs.Coupled2(), Xs, gs.PerSystem(P), gs.Options(max_iterations=60))
+    assert list(st["code"]) == [gs.SUCCESS, gs.SINGULAR, gs.SUCCESS, gs.SUCCESS]
+    assert_parity(st, x, so, xo, bitwise=True)
+    # a shared parameter (not PerSystem) reaches every system
+    x, st, xo, so = run_both(gs, orc, device, gs.Newton(), gs.Coupled2(), X0[1:].copy(), (2.125, 1.5), opts)
+    assert_parity(st, x, so, xo, bitwise=True)
+
+
+def test_six_codes_and_tally(gs, orc, device):
+    """batched.jl:461-498: the six codes from one Backtracking batch with max_iterations = 3."""
+    codes = np.array([1, 2, 3, 4, 5, 6, 1, 3], dtype=np.int64)
+    X0 = np.tile([2.5, 3.5], (8, 1))
+    m = gs.Newton(globalization=gs.Backtracking())
+    opts = gs.Options(max_iterations=3)
+    x, st, xo, so = run_both(gs, orc, device, m, gs.Hetero2(), X0.copy(), gs.PerSystem(codes), opts)
+    assert list(st["code"]) == [0, 1, 2, 3, 4, 5, 0, 2]
+    assert_parity(st, x, so, xo, bitwise=True)
+    x2, counts = run_gpu(gs, device, m, gs.Hetero2(), X0.copy(), gs.PerSystem(codes), opts, counts=True)
+    assert counts == dict(SUCCESS=2, STALLED=1, MAXITERS=2, SINGULAR=1, NONFINITE=1, LINESEARCH_FAILED=1)
+    assert not gs.isconverged(counts)
+    assert np.array_equal(x2[0], x[0]) and np.array_equal(x2[6], x[6])
+
+
+def test_order_permutation(gs, orc, device):
+    """batched.jl:501-583: order = perm solves the same systems into the same slots."""
+    N = 7
+    P = np.array([[1.0 + 0.2 * i, 1.5 + 0.2 * i] for i in range(1, N + 1)])
+    X0 = np.array([[1.5 + 0.2 * i, 0.8 + 0.1 * i] for i in range(1, N + 1)])
+    perm = [5, 2, 7, 1, 4, 6, 3]
+    opts = gs.Options()
+    x_u, st_u = run_gpu(gs, device, gs.Newton(), gs.Coupled2(), X0.copy(), gs.PerSystem(P), opts)
+    x_o, st_o = run_gpu(gs, device, gs.Newton(), gs.Coupled2(), X0.copy(), gs.PerSystem(P), opts, order=perm)
+    assert np.array_equal(x_u, x_o) and same_bytes(st_u, st_o)
+    x_o2, st_o2 = run_gpu(gs, device, gs.Newton(), gs.Coupled2(), X0.copy(), gs.PerSystem(P), opts,
+                          order=torch.tensor(perm))
+    assert np.array_equal(x_u, x_o2)
+    inv = np.argsort(np.array(perm) - 1)
+    x_p, st_p = run_gpu(gs, device, gs.Newton(), gs.Coupled2(), X0[np.array(perm) - 1].copy(),
+                        gs.PerSystem(P[np.array(perm) - 1]), opts)
+    assert np.array_equal(x_p[inv], x_u)
+
+
+def test_hostile_starts(gs, orc, device):
+    """batched.jl:647-667: NaN/Inf starts → NONFINITE; a subnormal start converges."""
+    X0 = np.array([[np.nan, 1.0], [np.inf, 1.0], [np.finfo(float).tiny / 16, 1.0], [2.0, 1.0]])
+    P = np.tile([2.125, 1.5], (4, 1))
+    x, st, xo, so = run_both(gs, orc, device, gs.Newton(), gs.Coupled2(), X0.copy(), gs.PerSystem(P), gs.Options())
+    assert list(st["code"]) == [gs.NONFINITE, gs.NONFINITE, gs.SUCCESS, gs.SUCCESS]
+    assert_parity(st, x, so, xo, bitwise=True)
+
+
+def test_scalar_trajectory_and_best(gs, orc, device):
+    """newton.jl:727-733: ‖F(x_k)‖ of Newton on x² + 1 from 0.5 for k = 0..8, and best = true."""
+    best_nr_f = (1.25, 1.5625, 1.0850694444444444, 3.4600428713151907, 1.2166349630462578,
+                 1.7081737551644687, 1.0300641461766078, 8.823069005566088, 2.487724019230605)
+    for k, want in enumerate(best_nr_f):
+        x, st = run_gpu(gs, device, gs.Newton(), gs.ScalarSquare(), np.array([[0.5]]), -1.0, gs.Options(max_iterations=k))
+        assert st[0]["fnorm"] == pytest.approx(want, rel=8 * EPS)
+    x, st = run_gpu(gs, device, gs.Newton(best=True), gs.ScalarSquare(), np.array([[0.5]]), -1.0, gs.Options(max_iterations=8))
+    assert st[0]["code"] == gs.MAXITERS and st[0]["fnorm"] == pytest.approx(best_nr_f[6], rel=8 * EPS)
+    # the step test of solver.jl:199-224
+    x, st = run_gpu(gs, device, gs.Newton(), gs.ScalarSquare(), np.array([[10.0]]), 2.0,
+                    gs.Options(x_reltol=0.5, f_reltol=1e-12, kappa=1e-6))
+    assert st[0]["code"] == gs.STALLED and st[0]["iterations"] == 4
+
+
+def test_workgroup_sizes_agree(gs, workloads, device):
+    """The result does not depend on the launch's workgroup size (batched.jl:183-195)."""
+    n = 5000
+    xk = workloads.henon_heiles_states(n, seed=3)
+    opts = gs.Options(f_abstol=1e-12, f_reltol=0.0)
+    ref = run_gpu(gs, device, gs.Newton(), gs.HenonHeilesMidpoint(), xk.copy(), gs.BatchParams(0.1, xk), opts)
+    for ws in (1, 7, 64, 256):
+        got = run_gpu(gs, device, gs.Newton(), gs.HenonHeilesMidpoint(), xk.copy(), gs.BatchParams(0.1, xk), opts,
+                      workgroupsize=ws)
+        assert np.array_equal(got[0], ref[0]) and same_bytes(got[1], ref[1])
+    # a size above the kernel's maximum is reported before any launch, as an ArgumentError
+    with pytest.raises(gs.ArgumentError, match="exceeds the solve kernel's maximum of 256"):
+        run_gpu(gs, device, gs.Newton(), gs.HenonHeilesMidpoint(), xk.copy(), gs.BatchParams(0.1, xk), opts,
+                workgroupsize=1024)
+
+
+def test_batch_summary(gs, workloads, device):
+    """batch_summary (batched.jl:80-89) against the host reduction of the same statuses."""
+    n = 20000
+    xk = workloads.henon_heiles_states(n, seed=4, e_lo=0.1, e_hi=0.166)
+    X = torch.as_tensor(xk * 3.0, device=device)
+    S = gs.status_buffer(n, device)
+    gs.solve_batch(X, S, gs.NonlinearProblem(gs.HenonHeilesMidpoint()), gs.Newton(),
+                   gs.BatchParams(2.0, torch.as_tensor(xk * 3.0, device=device)), max_iterations=6)
+    st = gs.statuses(S)
+    bs = gs.batch_summary(S)
+    fin = st["fnorm"]
+    want = math.sqrt(float(np.sum(fin.astype(np.longdouble) ** 2))) if np.isfinite(fin).all() else float("nan")
+    assert bs.failure_count == int((st["code"] != 0).sum())
+    assert bs.max_iterations == int(st["iterations"].max())
+    assert bs.total_iterations == int(st["iterations"].sum())
+    assert bs.all_converged == (bs.failure_count == 0)
+    if math.isfinite(want):
+        assert bs.fnorm == pytest.approx(want, rel=1e-12)
+    empty = gs.status_buffer(0, device)
+    assert gs.batch_summary(empty) == gs.BatchSummary(0.0, True, 0, 0, 0)

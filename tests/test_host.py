@@ -1,0 +1,135 @@
+"""Host logic of the binding: the C-ABI library loads and exports what include/gsolve.h declares,
+the struct layouts match Julia's SolverStatus{Float64}, and solve_batch validates its arguments with
+the reference's errors (src/nonlinear/batched.jl:113-150, 205-210, 257-307; options.jl:207-245)
+before any device work."""
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def test_library_exports_every_declared_symbol(gs):
+    header = (ROOT / "include" / "gsolve.h").read_text()
+    declared = set(re.findall(r"^\S.*?\b(gs_\w+)\(", header, flags=re.M))
+    from importlib import import_module
+
+    abi = import_module(gs.__name__ + "._abi")
+    lib = ctypes.CDLL(str(abi.LIB_PATH))
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert declared == set(abi.EXPORTED_SYMBOLS)
+    loaded = gs.load_library()
+    assert loaded.gs_version() == 1
+
+
+def test_status_layout_matches_julia(gs):
+    dt = gs.STATUS_DTYPE
+    assert dt.itemsize == 72
+    offs = {n: dt.fields[n][1] for n in dt.names}
+    assert offs == {"code": 0, "f_calls": 2, "iterations": 4, "fnorm": 8, "stepnorm": 16,
+                    "step_failures": 24, "promoted": 28, "stationary": 29, "jvp_calls": 30, "rank": 32,
+                    "nullnorm": 40, "gradratio": 48, "rate": 56, "contracted": 64, "reused": 65}
+
+
+def test_return_codes(gs):
+    assert [int(c) for c in gs.ReturnCode] == list(range(6))
+
+
+def test_options_validation(gs):
+    with pytest.raises(gs.ArgumentError, match="f_abstol must be non-negative"):
+        gs.Options(f_abstol=-1.0)
+    with pytest.raises(gs.ArgumentError, match="x_reltol must be non-negative"):
+        gs.Options(x_reltol=-1)
+    with pytest.raises(gs.ArgumentError):
+        gs.Options(f_abstol=float("nan"))
+    with pytest.raises(gs.ArgumentError, match="max_iterations = 1 is below min_iterations = 2"):
+        gs.Options(min_iterations=2, max_iterations=1)
+    with pytest.raises(gs.ArgumentError, match="max_stalls must be at least 1"):
+        gs.Options(max_stalls=0)
+    with pytest.raises(gs.ArgumentError, match="kappa"):
+        gs.Options(kappa=1.0)
+    o = gs.Options()
+    assert o.f_reltol == np.sqrt(np.finfo(float).eps) and o.x_reltol == 2 * np.finfo(float).eps
+    assert o.max_iterations == 100 and o.max_stalls == 2 and o.kappa == 0.1 and o.rate_test
+
+
+def test_method_validation(gs):
+    with pytest.raises(gs.ArgumentError, match="Backtracking needs 0 < c₁ < 1"):
+        gs.Backtracking(c1=1.5)
+    with pytest.raises(gs.ArgumentError, match="the step of Static must be positive and finite"):
+        gs.Static(0.0)
+    with pytest.raises(gs.ArgumentError, match="Refinement maxiter must be ≥ 1"):
+        gs.Refinement(maxiter=0)
+    with pytest.raises(gs.ArgumentError, match="regularization"):
+        gs.Newton(regularization=-1.0)
+    with pytest.raises(gs.ArgumentError, match="keep_factor"):
+        gs.Newton(keep_factor=True)
+    assert isinstance(gs.Newton(refinement=True).refinement, gs.Refinement)
+
+
+def _cpu_batch(gs, n=3):
+    X = torch.zeros((n, 2), dtype=torch.float64)
+    return X, gs.NonlinearProblem(gs.Coupled2())
+
+
+def test_validation_order_and_messages(gs):
+    """The host checks raise before any launch, so they run without a GPU on CPU tensors."""
+    X, prob = _cpu_batch(gs)
+    P = torch.zeros((2, 2), dtype=torch.float64)
+    with pytest.raises(gs.DimensionMismatch, match="PerSystem parameter length 2 does not match batch length 3"):
+        gs.solve_batch(X, prob, gs.Newton(), gs.PerSystem(P))
+    with pytest.raises(gs.DimensionMismatch, match="order length 1 does not match batch length 3"):
+        gs.solve_batch(X, prob, gs.Newton(), (2.125, 1.5), order=[1])
+    with pytest.raises(gs.ArgumentError, match=r"order is not a permutation of 1:3"):
+        gs.solve_batch(X, prob, gs.Newton(), (2.125, 1.5), order=[1, 1, 2])
+    with pytest.raises(gs.ArgumentError, match=r"order is not a permutation of 1:3"):
+        gs.solve_batch(X, prob, gs.Newton(), (2.125, 1.5), order=[1, 4, 2])
+    S = torch.empty((2, 72), dtype=torch.uint8)
+    with pytest.raises(gs.DimensionMismatch, match="status vector length 2 does not match batch length 3"):
+        gs.solve_batch(X, S, prob, gs.Newton(), (2.125, 1.5))
+    with pytest.raises(gs.ArgumentError, match="workgroupsize must be a positive integer"):
+        gs.solve_batch(X, prob, gs.Newton(), (2.125, 1.5), workgroupsize=0)
+    with pytest.raises(gs.ArgumentError, match="pass either `options` or keyword arguments, not both"):
+        gs.solve_batch(X, prob, gs.Newton(), (2.125, 1.5), options=gs.Options(), max_iterations=3)
+    with pytest.raises(gs.ArgumentError, match="SolverStatus real type Float64"):
+        gs.solve_batch(X.float(), prob, gs.Newton(), (2.125, 1.5))
+
+
+def test_host_tensor_is_refused_not_solved(gs):
+    """There is no CPU fallback: a host X raises."""
+    X, prob = _cpu_batch(gs)
+    with pytest.raises(gs.ArgumentError, match="device array"):
+        gs.solve_batch(X, prob, gs.Newton(), (2.125, 1.5))
+
+
+def test_empty_batch(gs):
+    X = torch.zeros((0, 2), dtype=torch.float64, device="cuda" if torch.cuda.is_available() else "cpu")
+    prob = gs.NonlinearProblem(gs.Coupled2())
+    if X.device.type == "cpu":
+        pytest.skip("empty-batch call needs a device tensor")
+    counts = gs.solve_batch(X, prob, gs.Newton(), (2.125, 1.5))
+    assert counts == {k: 0 for k in ("SUCCESS", "STALLED", "MAXITERS", "SINGULAR", "NONFINITE",
+                                     "LINESEARCH_FAILED")}
+    assert gs.isconverged(counts)
+
+
+def test_isconverged_counts(gs):
+    ok = dict(SUCCESS=10, STALLED=0, MAXITERS=0, SINGULAR=0, NONFINITE=0, LINESEARCH_FAILED=0)
+    assert gs.isconverged(ok)
+    for k in ("STALLED", "MAXITERS", "SINGULAR", "NONFINITE", "LINESEARCH_FAILED"):
+        assert not gs.isconverged({**ok, k: 1})
+
+
+def test_workloads_are_reproducible(workloads):
+    a = workloads.henon_heiles_states(1000, 0)
+    b = workloads.henon_heiles_states(1000, 0)
+    assert np.array_equal(a, b)
+    E = workloads.henon_heiles_energy(a)
+    assert E.min() >= 0.01 - 1e-12 and E.max() <= 0.15 + 1e-12
+    rec, x0 = workloads.tanh_linear3_batch(100, 0)
+    assert rec.shape == (100, 12) and not x0.any()

@@ -1,0 +1,253 @@
+"""The oracle pinned against the reference's golden vectors, known-answer tests and closed forms.
+
+Each check below would fail for a plausible mistake in the restatement: a wrong branch of the loop
+(golden codes/iterations/f_evals), a dropped term in the line search (f_evals), a wrong pivot rule
+(LU KATs), a wrong exit order (the solver.jl KATs).
+"""
+import csv
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+GOLDEN = Path(__file__).parent / "golden" / "standard_reference_newton_lu.csv"
+SCALE = {"u0": 1, "10u0": 10, "100u0": 100}
+COLUMN = {"Newton Static LU": 0, "Newton Backtracking LU": 1}
+
+
+def _rows():
+    with open(GOLDEN, newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def test_golden_table_has_both_columns():
+    rows = _rows()
+    assert len(rows) == 268
+    assert {r["column"] for r in rows} == set(COLUMN)
+
+
+@pytest.mark.parametrize("eltype", ["Float32", "Float64"])
+def test_golden_table_codes_iterations_fevals(orc, eltype):
+    """test/verification/standard_reference.csv: every stable cell of the Newton/LU columns
+    (test/helpers/standard_cells.jl:96-102 compares unstable/platform cells only loosely)."""
+    rows = [r for r in _rows() if r["eltype"] == eltype]
+    checked = 0
+    bad = []
+    for r in rows:
+        if r["unstable"] == "true" or r["platform"] == "true":
+            continue
+        code, it, nf, fn, _ = orc.standard_cell(int(r["problem"]), SCALE[r["start"]],
+                                                COLUMN[r["column"]], 32 if eltype == "Float32" else 64)
+        checked += 1
+        if (code, it, nf) != (r["code"], int(r["iterations"]), int(r["f_evals"])):
+            bad.append((r["problem"], r["start"], r["column"], r["code"], r["iterations"], r["f_evals"],
+                        code, it, nf))
+        # fnorm well above the round-off floor of the element type agrees to a relative
+        # tolerance (the golden run factorised with LAPACK, not the static LU)
+        ref = float(r["fnorm"])
+        floor, rtol = (1e-2, 1e-2) if eltype == "Float32" else (1e-6, 1e-6)
+        if math.isfinite(ref) and ref > floor and code == r["code"]:
+            assert abs(fn - ref) <= rtol * ref, (r, fn)
+    assert checked > 100
+    assert not bad, bad
+
+
+def test_lu_pivot_ties_keep_first_index(orc):
+    """test/linear/static/lu.jl: equal |entries| keep the first index (ipiv == [1, 2] and [1, 2, 3])."""
+    _, ipiv, info, _ = orc.lu_factor(np.array([[1.0, 2.0], [-1.0, 3.0]]))
+    assert list(ipiv) == [1, 2] and info == 0
+    # step 2 ties the subcolumn [1, -1] that the elimination of column 1 leaves
+    _, ipiv, info, _ = orc.lu_factor(np.array([[2.0, 0.0, 0.0], [1.0, 1.0, 0.0], [1.0, -1.0, 1.0]]))
+    assert list(ipiv) == [1, 2, 3] and info == 0
+
+
+def test_lu_zero_pivot_info(orc):
+    """lu.jl KATs: a zero column or matrix gives info == 1; equal rows give info == 2."""
+    assert orc.lu_factor(np.zeros((3, 3)))[2] == 1
+    assert orc.lu_factor(np.array([[0.0, 1.0], [0.0, 2.0]]))[2] == 1
+    # the equal-rows LU at n = 2: an exact multiplier of one cancels the second pivot (lu.jl:83-96)
+    rng = np.random.default_rng(1)
+    for _ in range(100):
+        r = rng.standard_normal(2)
+        assert orc.lu_factor(np.array([[r[0], r[1]], [r[0], r[1]]]))[2] == 2
+    # two equal rows near floatmax: SINGULAR with finite factors (lu.jl:98-107)
+    c = 0.6 * np.finfo(float).max
+    fac, _, info, _ = orc.lu_factor(np.full((2, 2), c))
+    assert info == 2 and np.all(np.isfinite(fac))
+
+
+def test_lu_solve_backward_error(orc):
+    """lu.jl check 1: ‖x̂ − x‖∞/‖x‖∞ ≤ 4n·eps·κ∞ against a long-double reference solve."""
+    rng = np.random.default_rng(2)
+    for n in (2, 3, 4, 6, 8):
+        for _ in range(50):
+            A = rng.standard_normal((n, n))
+            x = rng.standard_normal(n)
+            b = A @ x
+            xs = orc.lu_solve(A, b)
+            Al = A.astype(np.longdouble)
+            xl = np.linalg.solve(A, b).astype(np.longdouble)
+            for _ in range(3):  # refine in long double
+                xl = xl + np.linalg.solve(A, (b.astype(np.longdouble) - Al @ xl).astype(np.float64))
+            kappa = np.linalg.cond(A, np.inf)
+            err = float(np.max(np.abs(xs - xl.astype(np.float64))) / np.max(np.abs(xl.astype(np.float64))))
+            assert err <= 4 * n * np.finfo(float).eps * kappa
+
+
+def test_lu_power_of_two_scaling_is_exact(orc):
+    """factors.jl:802-811: J/2^e and the unscaled solve give the same bits for small J."""
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((4, 4))
+    b = rng.standard_normal(4)
+    for s in (2.0 ** -40, 2.0 ** -600, 2.0 ** -1000):  # every entry stays normal
+        x1 = orc.lu_solve(A, b)
+        xs = orc.lu_solve(A * s, b * s)
+        assert np.array_equal(x1, xs)
+    _, _, _, e = orc.lu_factor(A * 2.0 ** -40)
+    assert e == int(np.floor(np.log2(np.abs(A * 2.0 ** -40).max())))
+    assert orc.lu_factor(A * 2.0 ** 40)[3] == 0  # the same type divides only for e < 0
+
+
+def test_mixed_lu_solve_accuracy(orc):
+    """The FP32 factor solve (factors.jl:943-949) is accurate to FP32 and overflows nowhere."""
+    rng = np.random.default_rng(4)
+    A = rng.standard_normal((4, 4)) + 4 * np.eye(4)
+    b = rng.standard_normal(4)
+    x = np.linalg.solve(A, b)
+    for s in (1.0, 1e200, 1e-200):
+        xs = orc.lu_solve(A, b * s, f32=True)
+        assert np.allclose(xs, x * s, rtol=1e-5, atol=0)
+
+
+def test_rnorm_scaled_path(orc):
+    """reductions.jl:41-57: no overflow or underflow where the norm is representable."""
+    assert orc.rnorm([3e200, 4e200]) == pytest.approx(5e200, rel=1e-15)
+    assert orc.rnorm([3e-200, 4e-200]) == pytest.approx(5e-200, rel=1e-15)
+    assert orc.rnorm([3.0, 4.0]) == 5.0
+    assert math.isnan(orc.rnorm([np.nan, 1.0]))
+    assert orc.rnorm([np.inf, 1.0]) == np.inf
+    assert orc.rnorm([0.0, 0.0]) == 0.0
+
+
+def test_headroom_and_rhs_exponent(orc):
+    """headroom(n) = 2 + ⌈log2 n⌉ (factors.jl:889-890); rhs_exponent clamps to [min(e,0), 0]."""
+    assert [orc.lib().orc_headroom(n) for n in (1, 2, 3, 4, 5, 8, 9, 48)] == [2, 3, 4, 4, 5, 5, 6, 8]
+    r = np.ascontiguousarray([2.0 ** -10, 0.0, 0.0, 0.0])
+    f = orc.lib().orc_rhs_exponent4
+    assert f(orc._ptr(r), 0) == 0
+    assert f(orc._ptr(r), -20) == -10
+    assert f(orc._ptr(r), -5) == -5
+    big = np.ascontiguousarray([2.0 ** 1022, 0.0, 0.0, 0.0])
+    assert f(orc._ptr(big), 0) == 1022 - (1023 - 4)
+
+
+def test_backtrack_step_closed_forms(orc):
+    """backtracking.jl:320-340: the quadratic minimiser of φ(α) = (1−α)² scaled, and its clamps."""
+    bt = orc.lib().orc_backtrack_step
+    # φ(α) = φ0 (1 − α)^2 with φ0 = 1: d0 = −2, φ(1) = 0 → quadratic minimiser α = 1, clamped to p·α
+    assert bt(1.0, -2.0, 1.0, 0.0, float("nan"), float("nan"), 0.5) == 0.5
+    # φ(1) = 10 (huge increase): minimiser 1/(2·(10/2 + 1 − ... )) → clamped below to 0.1
+    assert bt(1.0, -2.0, 1.0, 10.0, float("nan"), float("nan"), 0.5) == pytest.approx(0.1)
+    # a non-finite trial gives 0.1 α
+    assert bt(1.0, -2.0, 0.5, float("inf"), float("nan"), float("nan"), 0.5) == pytest.approx(0.05)
+    # the quadratic model through φ0, d0, φ(α): α² / (2 r1) with r1 = (φα − φ0)/|d0| + α
+    a, pa = 1.0, 0.9
+    r1 = (pa - 1.0) / 2.0 + a
+    assert bt(1.0, -2.0, a, pa, float("nan"), float("nan"), 0.5) == pytest.approx(min(max(a * a / (2 * r1), 0.1), 0.5))
+
+
+def _newton(gs, **kw):
+    return gs.Newton(**kw)
+
+
+def test_trajectory_kat_x2_plus_1(gs, orc):
+    """test/nonlinear/newton.jl:727-733 BEST_NR_F: ‖F(x_k)‖ of Newton on x² + 1 from 0.5."""
+    best_nr_f = (1.25, 1.5625, 1.0850694444444444, 3.4600428713151907, 1.2166349630462578,
+                 1.7081737551644687, 1.0300641461766078, 8.823069005566088, 2.487724019230605)
+    fam = gs.ScalarSquare()
+    for k, want in enumerate(best_nr_f):
+        x, S = orc.solve_batch(gs.Newton(), fam, np.array([[0.5]]), -1.0, gs.Options(max_iterations=k))
+        assert S[0]["fnorm"] == pytest.approx(want, rel=8 * np.finfo(float).eps), k
+        assert S[0]["code"] == (gs.SUCCESS if k == 0 and False else gs.MAXITERS if k > 0 else S[0]["code"])
+    # best = true returns the smallest, at k = 6, with the status of the last step otherwise
+    x, S = orc.solve_batch(gs.Newton(best=True), fam, np.array([[0.5]]), -1.0, gs.Options(max_iterations=8))
+    assert S[0]["code"] == gs.MAXITERS
+    assert S[0]["fnorm"] == pytest.approx(best_nr_f[6], rel=8 * np.finfo(float).eps)
+
+
+def test_step_test_stalls_sqrt2(gs, orc):
+    """solver.jl:199-224: √2 from 10 with x_reltol = 0.5, kappa = 1e-6 → STALLED at k = 4."""
+    fam = gs.ScalarSquare()
+    x, S = orc.solve_batch(gs.Newton(), fam, np.array([[10.0]]), 2.0,
+                           gs.Options(x_reltol=0.5, f_reltol=1e-12, kappa=1e-6))
+    assert S[0]["code"] == gs.STALLED and S[0]["iterations"] == 4 and S[0]["fnorm"] > 0.08
+    x, S = orc.solve_batch(gs.Newton(), fam, np.array([[10.0]]), 2.0,
+                           gs.Options(x_reltol=0.5, f_reltol=1e-12, kappa=1e-6, min_iterations=6))
+    assert S[0]["code"] == gs.STALLED and S[0]["iterations"] == 6
+    x, S = orc.solve_batch(gs.Newton(), fam, np.array([[10.0]]), 2.0, gs.Options())
+    assert S[0]["code"] == gs.SUCCESS and S[0]["fnorm"] <= math.sqrt(np.finfo(float).eps) * 98
+
+
+def test_zero_step_stalls_at_once(gs, orc):
+    """solver.jl:183-196: J = Inf gives a zero step → STALLED at iteration 1, fnorm 5, x = 3."""
+    x, s = orc.solve_fixture(103, gs.Newton(), gs.Options(), [3.0])
+    assert s["code"] == gs.STALLED and s["iterations"] == 1 and x[0] == 3.0 and s["fnorm"] == 5.0
+    assert s["stepnorm"] == 0.0
+    x, s = orc.solve_fixture(103, gs.Newton(), gs.Options(min_iterations=3), [3.0])
+    assert s["code"] == gs.STALLED and s["iterations"] == 3 and x[0] == 3.0
+
+
+def test_exact_root_and_zero_tolerances(gs, orc):
+    """solver.jl:238-281: exact root → SUCCESS with 0 iterations and the status constructor's
+    defaults; all tolerances 0 on an affine F → SUCCESS at x = (1, 1)."""
+    x, s = orc.solve_fixture(101, gs.Newton(), gs.Options(), [2.0, 3.0], (4.0, 9.0))
+    assert s["code"] == gs.SUCCESS and s["iterations"] == 0 and list(x) == [2.0, 3.0]
+    assert s["fnorm"] == 0.0 and s["stepnorm"] == 0.0 and s["rank"] == -1
+    assert math.isnan(s["gradratio"]) and math.isnan(s["rate"]) and math.isnan(s["nullnorm"])
+    x, s = orc.solve_fixture(101, gs.Newton(), gs.Options(), [3.0, 5.0], (4.0, 9.0))
+    assert s["code"] == gs.SUCCESS and s["iterations"] >= 1 and np.allclose(x, [2, 3], rtol=10 * 1.5e-8)
+    x, s = orc.solve_fixture(102, gs.Newton(), gs.Options(f_abstol=0, f_reltol=0, x_abstol=0, x_reltol=0),
+                             [3.0, 5.0], (2.0, 2.0))
+    assert s["code"] == gs.SUCCESS and list(x) == [1.0, 1.0]
+    # one ulp from a root: f_abstol at the residual floor takes 0 steps (solver.jl:255-261)
+    xu = [np.nextafter(2.0, 3.0), 3.0]
+    floor = abs(xu[0] * xu[0] - 4.0)
+    x, s = orc.solve_fixture(101, gs.Newton(), gs.Options(f_abstol=floor), xu, (4.0, 9.0))
+    assert s["code"] == gs.SUCCESS and s["iterations"] == 0
+    # min_iterations = 2 from the root runs two steps
+    x, s = orc.solve_fixture(101, gs.Newton(), gs.Options(min_iterations=2), [2.0, 3.0], (4.0, 9.0))
+    assert s["code"] == gs.SUCCESS and s["iterations"] == 2
+
+
+def test_six_codes_fixture(gs, orc):
+    """test/nonlinear/batched.jl:461-498: all six codes from one Backtracking batch, max_iterations = 3."""
+    codes = np.array([1, 2, 3, 4, 5, 6, 1, 3])
+    X = np.tile([2.5, 3.5], (8, 1))
+    m = gs.Newton(globalization=gs.Backtracking())
+    x, S = orc.solve_batch(m, gs.Hetero2(), X, gs.PerSystem(codes), gs.Options(max_iterations=3))
+    want = [gs.SUCCESS, gs.STALLED, gs.MAXITERS, gs.SINGULAR, gs.NONFINITE, gs.LINESEARCH_FAILED,
+            gs.SUCCESS, gs.MAXITERS]
+    assert list(S["code"]) == want
+
+
+def test_coupled_exact_dyadic_root_and_singular(gs, orc):
+    """batched.jl:263-375: the exact dyadic root takes 0 iterations bit-identically; a start with a
+    singular Jacobian gives SINGULAR while the others converge."""
+    P = np.array([[2.125, 1.5], [1.25, 2.0], [1.5, 2.5], [1.75, 3.0]])
+    X0 = np.array([[2.0, 1.0], [2.5, 1.5], [3.5, 2.5], [4.0, 3.0]])
+    x, S = orc.solve_batch(gs.Newton(), gs.Coupled2(), X0.copy(), gs.PerSystem(P), gs.Options(max_iterations=150))
+    assert S[0]["iterations"] == 0 and np.array_equal(x[0], X0[0])
+    assert all(S["code"] == gs.SUCCESS) and all(S["iterations"][1:] > 0)
+    Xs = X0.copy()
+    Xs[1] = [4.0, 4.0]
+    x, S = orc.solve_batch(gs.Newton(), gs.Coupled2(), Xs, gs.PerSystem(P), gs.Options(max_iterations=60))
+    assert list(S["code"]) == [gs.SUCCESS, gs.SINGULAR, gs.SUCCESS, gs.SUCCESS]
+
+
+def test_hostile_starts(gs, orc):
+    """batched.jl:647-667: NaN and Inf starts → NONFINITE, a subnormal start converges."""
+    X = np.array([[np.nan, 1.0], [np.inf, 1.0], [np.finfo(float).tiny / 16, 1.0], [2.0, 1.0]])
+    P = np.tile([2.125, 1.5], (4, 1))
+    x, S = orc.solve_batch(gs.Newton(), gs.Coupled2(), X, gs.PerSystem(P), gs.Options())
+    assert list(S["code"]) == [gs.NONFINITE, gs.NONFINITE, gs.SUCCESS, gs.SUCCESS]
