@@ -121,6 +121,9 @@ class Options:
             raise ArgumentError("norm must be L2Norm() or MaxNorm()")
 
     def to_c(self) -> _abi.GsOptionsF64:
+        c = self.__dict__.get("_c")
+        if c is not None:
+            return c
         o = _abi.GsOptionsF64()
         o.f_abstol, o.f_reltol = self.f_abstol, self.f_reltol
         o.x_abstol, o.x_reltol = self.x_abstol, self.x_reltol
@@ -129,6 +132,7 @@ class Options:
         o.norm = _abi.GS_NORM_MAX if isinstance(self.norm, MaxNorm) else _abi.GS_NORM_L2
         o.g_tol, o.kappa = self.g_tol, self.kappa
         o.roundoff, o.rate_test = int(bool(self.roundoff)), int(bool(self.rate_test))
+        object.__setattr__(self, "_c", o)  # the options are frozen: build the struct once
         return o
 
 
@@ -242,6 +246,9 @@ class Newton:
     def to_c(self) -> _abi.GsMethod:
         if self.reuse is not None:
             raise ArgumentError("Newton(; reuse = k) has no device form in libgsolve_hip")
+        c = self.__dict__.get("_c")
+        if c is not None:
+            return c
         m = _abi.GsMethod()
         m.linsolve = _abi.GS_LU
         m.factor_precision = _abi.GS_FACTOR_F32 if self.linsolve.precision is not None else _abi.GS_FACTOR_WORKING
@@ -256,6 +263,7 @@ class Newton:
         m.ref_eta, m.ref_maxiter = rr.eta, rr.maxiter
         m.regularization = self.regularization
         m.best = int(bool(self.best))
+        object.__setattr__(self, "_c", m)  # the method is frozen: build the struct once
         return m
 
 
@@ -309,6 +317,9 @@ def _torch():
 
 def _records(data, nsys: int, p_len: int, dtype, device, what: str):
     torch = _torch()
+    if (isinstance(data, torch.Tensor) and data.device == device and data.dtype == dtype
+            and data.dim() == 2 and data.shape[1] == p_len and data.is_contiguous()):
+        return data
     t = data if isinstance(data, torch.Tensor) else torch.as_tensor(np.asarray(data))
     t = t.to(device=device, dtype=dtype).reshape(len(t), -1) if t.dim() > 0 else t
     if t.dim() != 2 or t.shape[1] != p_len:
@@ -427,7 +438,11 @@ _CODE_NAMES = ("SUCCESS", "STALLED", "MAXITERS", "SINGULAR", "NONFINITE", "LINES
 
 
 def _stream_ptr(X):
+    """The raw hipStream_t of PyTorch's current stream on the device of X."""
     torch = _torch()
+    raw = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+    if raw is not None:
+        return ctypes.c_void_p(raw(X.device.index if X.device.index is not None else torch.cuda.current_device()))
     return ctypes.c_void_p(torch.cuda.current_stream(X.device).cuda_stream)
 
 

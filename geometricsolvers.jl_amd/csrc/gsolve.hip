@@ -61,8 +61,16 @@ GS_DEV void store_status(gs_status_f64* dst, uint8_t code, int32_t iterations, d
 constexpr int kSmallMaxWorkgroup = 256;
 
 // One work-item per system (the launch shape of `_solve_kernel!`, batched.jl:171-179).
+// Waves per SIMD the register allocator targets: 4 (≤ 128 VGPRs) for the plain FP64 Static form,
+// which fit without scratch; the heavier forms keep the allocator's own choice.
+template <int GLOB, bool MIXED, bool REFINE> constexpr int small_waves_per_eu() {
+    return (GLOB == kStatic && !MIXED && !REFINE) ? 4 : 1;
+}
+
 template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG>
-__global__ __launch_bounds__(kSmallMaxWorkgroup) void solve_small_kernel(const KernelArgs a) {
+__global__ __launch_bounds__(kSmallMaxWorkgroup)
+__attribute__((amdgpu_waves_per_eu(small_waves_per_eu<GLOB, MIXED, REFINE>())))
+void solve_small_kernel(const KernelArgs a) {
     constexpr int N = Fam::N;
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= a.nsys) return;
