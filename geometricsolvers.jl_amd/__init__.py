@@ -30,9 +30,10 @@ from ._abi import STATUS_DTYPE, load_library
 __all__ = [
     "ArgumentError", "DimensionMismatch", "ReturnCode", "SUCCESS", "STALLED", "MAXITERS",
     "SINGULAR", "NONFINITE", "LINESEARCH_FAILED", "STATUS_DTYPE", "L2Norm", "MaxNorm", "Options",
-    "LUFactorization", "Static", "Backtracking", "NoRefinement", "Refinement", "Newton",
+    "LUFactorization", "Static", "Backtracking", "LevenbergMarquardt", "NoRefinement", "Refinement", "Newton",
     "NonlinearProblem", "Coupled2", "Hetero2", "ScalarSquare", "TanhLinear3",
     "HenonHeilesMidpoint", "PerSystem", "BatchParams", "parameters", "solve_batch",
+    "LeastSquaresProblem", "ExpSumFits", "ExpSumData",
     "status_buffer", "statuses", "isconverged", "batch_summary", "BatchSummary", "load_library",
 ]
 
@@ -192,6 +193,32 @@ class Backtracking:
 
 
 @dataclass(frozen=True)
+class LevenbergMarquardt:
+    """LevenbergMarquardt(; μ₀, μ_min, μ_max, μ_decrease, μ_increase, ρ_accept)
+    (src/globalization/levenbergmarquardt.jl:48-98), `scaling = :none`, the stacked Givens step."""
+
+    mu0: float = 1e-6
+    mu_min: float = max(1e-15, 4 * _EPS)
+    mu_max: float = 1e12
+    mu_decrease: float = 1.0 / 3.0
+    mu_increase: float = 4.0
+    rho_accept: float = 0.0
+
+    def __post_init__(self):
+        if not (all(math.isfinite(v) for v in (self.mu0, self.mu_min, self.mu_max))
+                and 0 < self.mu_min <= self.mu0 <= self.mu_max):
+            raise ArgumentError(
+                "LevenbergMarquardt requires finite damping bounds 0 < μ_min ≤ μ₀ ≤ μ_max < Inf; "
+                f"got μ_min = {self.mu_min}, μ₀ = {self.mu0}, μ_max = {self.mu_max}")
+        if not (math.isfinite(self.mu_decrease) and 0 < self.mu_decrease < 1):
+            raise ArgumentError(f"LevenbergMarquardt requires 0 < μ_decrease < 1; got μ_decrease = {self.mu_decrease}")
+        if not (math.isfinite(self.mu_increase) and self.mu_increase > 1):
+            raise ArgumentError(f"LevenbergMarquardt requires μ_increase > 1; got μ_increase = {self.mu_increase}")
+        if not (math.isfinite(self.rho_accept) and 0 <= self.rho_accept < 1):
+            raise ArgumentError(f"LevenbergMarquardt requires 0 ≤ ρ_accept < 1; got ρ_accept = {self.rho_accept}")
+
+
+@dataclass(frozen=True)
 class NoRefinement:
     """NoRefinement() (src/nonlinear/solver.jl:7)."""
 
@@ -238,10 +265,17 @@ class Newton:
             raise ArgumentError("Newton reuse must be ≥ 1 or nothing")
         if not isinstance(self.linsolve, LUFactorization):
             raise ArgumentError("the device solve takes LUFactorization() or LUFactorization(Float32)")
-        if not isinstance(self.globalization, (Static, Backtracking)):
-            raise ArgumentError("the device solve takes Static() or Backtracking() with LU")
+        if not isinstance(self.globalization, (Static, Backtracking, LevenbergMarquardt)):
+            raise ArgumentError("the device solve takes Static(), Backtracking() or LevenbergMarquardt() with LU")
         if not isinstance(self.refinement, (NoRefinement, Refinement)):
             raise ArgumentError("refinement must be NoRefinement() or Refinement()")
+        if isinstance(self.globalization, LevenbergMarquardt):
+            if self.reuse is not None:
+                raise ArgumentError("Newton with a LevenbergMarquardt globalization takes no reuse")
+            if isinstance(self.refinement, Refinement):
+                raise ArgumentError(
+                    "Newton with a LevenbergMarquardt globalization takes no refinement: a refinement corrects "
+                    "the solve of J d = -F, which the damped step does not take; use refinement = NoRefinement()")
 
     def to_c(self) -> _abi.GsMethod:
         if self.reuse is not None:
@@ -253,7 +287,11 @@ class Newton:
         m.linsolve = _abi.GS_LU
         m.factor_precision = _abi.GS_FACTOR_F32 if self.linsolve.precision is not None else _abi.GS_FACTOR_WORKING
         g = self.globalization
-        m.globalization = _abi.GS_BACKTRACKING if isinstance(g, Backtracking) else _abi.GS_STATIC
+        m.globalization = (_abi.GS_BACKTRACKING if isinstance(g, Backtracking) else
+                           _abi.GS_LEVENBERG_MARQUARDT if isinstance(g, LevenbergMarquardt) else _abi.GS_STATIC)
+        lm = g if isinstance(g, LevenbergMarquardt) else LevenbergMarquardt()
+        m.lm_mu0, m.lm_mu_min, m.lm_mu_max = lm.mu0, lm.mu_min, lm.mu_max
+        m.lm_mu_decrease, m.lm_mu_increase, m.lm_rho_accept = lm.mu_decrease, lm.mu_increase, lm.rho_accept
         m.static_alpha = g.alpha if isinstance(g, Static) else 1.0
         bt = g if isinstance(g, Backtracking) else Backtracking()
         m.bt_c1, m.bt_p, m.bt_maxiter = bt.c1, bt.p, bt.maxiter
@@ -411,6 +449,55 @@ class NonlinearProblem:
         self.J = J if J is not None else F  # each family carries its analytic Jacobian
 
 
+class LeastSquaresProblem:
+    """LeastSquaresProblem(F, J) (src/nonlinear/leastsquares.jl:1-40) over a registered least-squares
+    family: minimise ½‖F(x, p)‖² over x, with the family's analytic Jacobian."""
+
+    def __init__(self, F: Family, J=None):
+        if not isinstance(F, ExpSumFits):
+            raise ArgumentError("the device least-squares solve takes a registered least-squares family (ExpSumFits)")
+        self.F = F
+        self.J = J if J is not None else F
+
+
+class ExpSumFits(Family):
+    """Exponential-sum fits y(t) ≈ Σ_{k≤K} a_k exp(−λ_k t) at t_j = 5j/(m−1), j = 0..m−1.
+
+    Unknowns x = (a_1..a_K, λ_1..λ_K), K ∈ 1..4, in the first 2K of 8 slots of a row of X.
+    F_j = (Σ_k a_k e^(−λ_k t_j)) − y_j (a left fold over k), J_jk = e^(−λ_k t_j),
+    J_j,K+k = ((−a_k) t_j) e^(−λ_k t_j). The exponential is a fixed algorithm (Cody–Waite reduction
+    and a degree-13 Taylor polynomial with fused multiply-adds, within 1 ulp), so the device and the
+    CPU oracle evaluate F and J to the same bits. Parameter: an ExpSumData.
+    """
+
+    id, n, p_len = _abi.GS_FAM_EXPSUM_LSQ, 8, 0
+    max_points = 4096
+
+
+class ExpSumData:
+    """The per-fit data of an ExpSumFits batch: K (int32), m (int32) and y_offset (int64) per fit, and
+    the concatenated samples Y (float64); fit i's samples are Y[y_offset[i] : y_offset[i] + m[i]].
+    All four are HIP tensors on the device of X. It plays the part of a PerSystem (batched.jl:8-10)."""
+
+    def __init__(self, K, m, y_offset, Y):
+        torch = _torch()
+        for name, t, dt in (("K", K, torch.int32), ("m", m, torch.int32), ("y_offset", y_offset, torch.int64),
+                            ("Y", Y, torch.float64)):
+            if not isinstance(t, torch.Tensor) or t.dtype != dt or t.dim() != 1:
+                raise ArgumentError(f"ExpSumData {name} must be a 1-d torch tensor of {dt}")
+            if not t.is_contiguous():
+                raise ArgumentError(f"ExpSumData {name} must be contiguous")
+        if not (len(K) == len(m) == len(y_offset)):
+            raise DimensionMismatch(
+                f"ExpSumData lengths differ: K {len(K)}, m {len(m)}, y_offset {len(y_offset)}")
+        if not (K.device == m.device == y_offset.device == Y.device):
+            raise ArgumentError("ExpSumData tensors must share one device")
+        self.K, self.m, self.y_offset, self.Y = K, m, y_offset, Y
+
+    def __len__(self):
+        return len(self.K)
+
+
 # --------------------------------------------------------------------------------------------
 # Status buffers and the batched solve (batched.jl:49-332)
 # --------------------------------------------------------------------------------------------
@@ -504,7 +591,7 @@ def solve_batch(X, *args, order=None, options: Optional[Options] = None, workgro
     `order` is a permutation of 1:nsys (1-based, as Julia's): work-item i solves system order[i].
     The call is synchronous, as `_launch!` is (batched.jl:193).
     """
-    if args and isinstance(args[0], NonlinearProblem):
+    if args and isinstance(args[0], (NonlinearProblem, LeastSquaresProblem)):
         S = None
         rest = args
     else:
@@ -519,10 +606,16 @@ def solve_batch(X, *args, order=None, options: Optional[Options] = None, workgro
         options = rest[3]
     if len(rest) > 4:
         raise ArgumentError("too many positional arguments")
+    if isinstance(prob, LeastSquaresProblem):
+        return _solve_lsq(X, S, prob, method, p, order, options, workgroupsize, kwargs)
     if not isinstance(prob, NonlinearProblem):
         raise ArgumentError("prob must be a NonlinearProblem over a registered family")
     if not isinstance(method, Newton):
         raise ArgumentError("method must be a Newton")
+    if isinstance(method.globalization, LevenbergMarquardt):
+        raise ArgumentError(
+            "the device solve of a NonlinearProblem takes Static() or Backtracking(); "
+            "LevenbergMarquardt() is the least-squares batch's (a LeastSquaresProblem)")
     fam = prob.F
     _check_x(X, fam.n)
     order_h = _validate_batch(X, p, order)
@@ -572,6 +665,68 @@ def solve_batch(X, *args, order=None, options: Optional[Options] = None, workgro
         return S
     counts = (ctypes.c_int64 * 6)()
     _raise_on(lib.gs_tally_codes(ctypes.c_void_p(codes.data_ptr()), nsys, counts, _stream_ptr(X)), lib)
+    return {k: int(counts[i]) for i, k in enumerate(_CODE_NAMES)}
+
+
+def _check_status_buffer(S, nsys: int, X) -> None:
+    torch = _torch()
+    if S.dtype != torch.uint8 or S.dim() != 2 or S.shape[1] != STATUS_DTYPE.itemsize:
+        raise ArgumentError("S must be a status buffer (status_buffer(nsys, device))")
+    if S.shape[0] != nsys:
+        raise DimensionMismatch(f"status vector length {S.shape[0]} does not match batch length {nsys}")
+    if S.device != X.device:
+        raise ArgumentError(f"status vector backend {S.device} does not match batch backend {X.device}")
+    if not S.is_contiguous():
+        raise ArgumentError("S must be contiguous")
+
+
+def _solve_lsq(X, S, prob, method, p, order, options, workgroupsize, kwargs):
+    """solve_batch on a LeastSquaresProblem: the per-fit R1 solve (solver.jl:148-189, 277-378) with
+    Newton(; globalization = LevenbergMarquardt()) for every fit of the batch, in one call."""
+    if not isinstance(method, Newton):
+        raise ArgumentError("method must be a Newton")
+    if not isinstance(method.globalization, LevenbergMarquardt):
+        raise ArgumentError(
+            "the device least-squares batch takes Newton(; globalization = LevenbergMarquardt())")
+    _check_x(X, prob.F.n)
+    nfit = X.shape[0]
+    if not isinstance(p, ExpSumData):
+        raise ArgumentError("an ExpSumFits batch takes its data as p = ExpSumData(K, m, y_offset, Y)")
+    if len(p) != nfit:
+        raise DimensionMismatch(f"ExpSumData length {len(p)} does not match batch length {nfit}")
+    if order is not None:
+        raise ArgumentError("the least-squares batch orders its fits itself (bucketed by size); order must be None")
+    if workgroupsize is not None:
+        raise ArgumentError(
+            "the least-squares batch takes its launch shapes from tune_lsq (or the library defaults); "
+            "workgroupsize must be None")
+    if S is not None:
+        _check_status_buffer(S, nfit, X)
+    opt = resolve_options(options, kwargs)
+    cm = method.to_c()
+    if X.device.type != "cuda":
+        raise ArgumentError(
+            "X must be a device array (a HIP tensor): libgsolve_hip solves on the GPU and has no host path")
+    if p.K.device != X.device:
+        raise ArgumentError(f"ExpSumData backend {p.K.device} does not match batch backend {X.device}")
+    if nfit == 0:
+        return S if S is not None else {k: 0 for k in _CODE_NAMES}
+    lib = load_library()
+    own = S is None
+    if own:
+        S = status_buffer(nfit, X.device)
+    copt = opt.to_c()
+    stream = _stream_ptr(X)
+    rc = lib.gs_solve_expsum_lsq_f64(
+        ctypes.byref(cm), nfit, ctypes.c_void_p(X.data_ptr()), ctypes.c_void_p(p.K.data_ptr()),
+        ctypes.c_void_p(p.m.data_ptr()), ctypes.c_void_p(p.y_offset.data_ptr()), ctypes.c_void_p(p.Y.data_ptr()),
+        len(p.Y), ctypes.c_void_p(S.data_ptr()), ctypes.byref(copt), stream)
+    _raise_on(rc, lib)
+    if not own:
+        return S
+    codes = S[:, 0].contiguous()
+    counts = (ctypes.c_int64 * 6)()
+    _raise_on(lib.gs_tally_codes(ctypes.c_void_p(codes.data_ptr()), nfit, counts, stream), lib)
     return {k: int(counts[i]) for i, k in enumerate(_CODE_NAMES)}
 
 

@@ -138,6 +138,8 @@ EXPORTED_SYMBOLS = (
     "gs_abi_sizes",
     "gs_set_kernel_timing",
     "gs_last_kernel_ms",
+    "gs_solve_expsum_lsq_f64",
+    "gs_lsq_set_shape",
 )
 
 _LIB = None
@@ -181,6 +183,13 @@ def load_library(path: os.PathLike | None = None):
     lib.gs_set_kernel_timing.argtypes = [ctypes.c_int]
     lib.gs_last_kernel_ms.restype = ctypes.c_double
     lib.gs_last_kernel_ms.argtypes = []
+    lib.gs_solve_expsum_lsq_f64.restype = ctypes.c_int
+    lib.gs_solve_expsum_lsq_f64.argtypes = [
+        ctypes.POINTER(GsMethod), ctypes.c_int64, vp, vp, vp, vp, vp, ctypes.c_int64, vp,
+        ctypes.POINTER(GsOptionsF64), vp,
+    ]
+    lib.gs_lsq_set_shape.restype = ctypes.c_int
+    lib.gs_lsq_set_shape.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
     if lib.gs_version() != ABI_VERSION:
         raise RuntimeError(f"libgsolve_hip ABI version {lib.gs_version()} != {ABI_VERSION}")
     sizes = (ctypes.c_int64 * 4)()

@@ -16,32 +16,46 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIB = HERE / "lib" / "libgsolve_hip.so"
-SOURCES = ["gsolve.hip", "batch_reduce.hip"]
+SOURCES = ["gsolve.hip", "batch_reduce.hip", "lsq.hip"]
 ARCH = os.environ.get("GSOLVE_OFFLOAD_ARCH", "gfx950")
+FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+    "-ffp-contract=off", "-fno-gpu-flush-denormals-to-zero",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function",
+]
+OBJ = HERE / "lib" / "obj"
 
 
-def hipcc_command(out: Path) -> list[str]:
-    return [
-        "hipcc", f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
-        "-ffp-contract=off", "-fno-gpu-flush-denormals-to-zero",
-        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-        "-Wall", "-Wno-unused-function",
-        "-o", str(out), *[str(CSRC / s) for s in SOURCES],
-    ]
+def _filter(text: str) -> str:
+    noise = "is not a recognized feature for this target"
+    return "\n".join(l for l in text.splitlines() if noise not in l and "(ignoring feature)" not in l)
 
 
 def build(verbose: bool = False) -> Path:
-    """Compile libgsolve_hip.so (always; the caller decides when a rebuild is needed)."""
-    LIB.parent.mkdir(parents=True, exist_ok=True)
+    """Compile libgsolve_hip.so (always; the caller decides when a rebuild is needed): one hipcc per
+    source, in parallel, then one link."""
+    OBJ.mkdir(parents=True, exist_ok=True)
+    procs = []
+    for src in SOURCES:
+        obj = OBJ / (Path(src).stem + ".o")
+        cmd = ["hipcc", *FLAGS, "-c", "-o", str(obj), str(CSRC / src)]
+        procs.append((src, obj, subprocess.Popen(cmd, cwd=CSRC, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+    errors, objs = [], []
+    for src, obj, p in procs:
+        out, _ = p.communicate()
+        out = _filter(out)
+        if p.returncode != 0:
+            errors.append(f"hipcc {src} failed ({p.returncode}):\n{out}")
+        elif verbose and out.strip():
+            print(out, file=sys.stderr)
+        objs.append(str(obj))
+    if errors:
+        raise RuntimeError("\n".join(errors))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = hipcc_command(tmp)
-    res = subprocess.run(cmd, cwd=CSRC, capture_output=True, text=True)
-    noise = "is not a recognized feature for this target"
-    err = "\n".join(l for l in (res.stdout + res.stderr).splitlines() if noise not in l and "(ignoring feature)" not in l)
+    res = subprocess.run(["hipcc", *FLAGS, "-shared", "-o", str(tmp), *objs], cwd=CSRC, capture_output=True, text=True)
     if res.returncode != 0:
-        raise RuntimeError(f"hipcc failed ({res.returncode}):\n{err}")
-    if verbose and err.strip():
-        print(err, file=sys.stderr)
+        raise RuntimeError(f"hipcc link failed ({res.returncode}):\n{_filter(res.stdout + res.stderr)}")
     os.replace(tmp, LIB)
     return LIB
 

@@ -1,0 +1,188 @@
+// The least-squares batch: Newton + LevenbergMarquardt on exponential-sum fits, one lane per fit, in
+// a persistent kernel whose lanes refill from a work queue.
+//
+// Per fit this is the reference's R1 `solve!!` on a LeastSquaresProblem (src/nonlinear/solver.jl:
+// 148-189, 277-378) with the LevenbergMarquardt step (src/nonlinear/newton.jl:1876-1971) on the
+// stacked Givens reduction of [J | −F] (src/globalization/levenbergmarquardt.jl:359-482, 575-584).
+// The fit's m rows are never stored: every pass over them recomputes the row (t_i, its exponentials,
+// F_i and J_i) and streams it — into the Givens reduction in the reference's row order, into the
+// left-fold sums of the gain ratio, or into the diagnostics ‖Jᴴ F‖ and ‖J‖_F. The triangular
+// factor, its right-hand side, the iterate and the solve's status live in VGPRs.
+//
+// A lane whose fit has ended takes the next fit of its bucket from the queue at the top of the
+// next outer step, so a wave is never held by the slowest of its 64 fits for longer than one step.
+#pragma once
+
+#include "device_common.hpp"
+
+namespace gs {
+
+// ReturnCode values (src/base/returncodes.jl:14) that the least-squares solve can end with.
+enum : uint8_t { kLsqSuccess = 0, kLsqStalled = 1, kLsqMaxIters = 2, kLsqNonfinite = 4 };
+
+struct LsqParams {
+    // Options
+    double f_abstol, f_reltol, x_abstol, x_reltol, kappa, g_tol;
+    int32_t min_iterations, max_iterations, max_stalls, norm_max, roundoff, rate_test;
+    // LevenbergMarquardt
+    double mu0, mu_min_eff, mu_max, mu_dec, mu_inc, rho_accept;
+};
+
+// LinearAlgebra's givensAlgorithm (the LAPACK DLARTG scheme): r = √(f² + g²), c = f/r, s = g/r,
+// power-of-two rescaling outside [2^-537, 2^537], c ≥ 0 where |f| > |g|.
+GS_DEV void givens(double f, double g, double& c, double& s, double& r) {
+    constexpr double safmn2 = 0x1p-537, safmx2 = 0x1p537;
+    if (g == 0.0) {
+        c = 1.0;
+        s = 0.0;
+        r = f;
+        return;
+    }
+    if (f == 0.0) {
+        c = 0.0;
+        s = 1.0;
+        r = g;
+        return;
+    }
+    double f1 = f, g1 = g;
+    double scale = jmax(__builtin_fabs(f1), __builtin_fabs(g1));
+    // the reference's branch order: a NaN scale fails both comparisons and takes the direct path;
+    // each rescaling loop is bounded (count ≤ 20 above; below, at most four doublings by 2^537)
+    if (__builtin_expect(!(scale >= safmx2) && !(scale <= safmn2), 1)) {
+        r = __builtin_sqrt(f1 * f1 + g1 * g1);
+        c = f1 / r;
+        s = g1 / r;
+    } else if (scale >= safmx2) {
+        int count = 0;
+        do {
+            ++count;
+            f1 *= safmn2;
+            g1 *= safmn2;
+            scale = jmax(__builtin_fabs(f1), __builtin_fabs(g1));
+        } while (!(scale < safmx2 || count >= 20));
+        r = __builtin_sqrt(f1 * f1 + g1 * g1);
+        c = f1 / r;
+        s = g1 / r;
+        for (int i = 0; i < count; ++i) r *= safmx2;
+    } else {
+        int count = 0;
+        do {
+            ++count;
+            f1 *= safmx2;
+            g1 *= safmx2;
+            scale = jmax(__builtin_fabs(f1), __builtin_fabs(g1));
+        } while (!(scale > safmn2) && count < 8);
+        r = __builtin_sqrt(f1 * f1 + g1 * g1);
+        c = f1 / r;
+        s = g1 / r;
+        for (int i = 0; i < count; ++i) r *= safmn2;
+    }
+    if (__builtin_fabs(f) > __builtin_fabs(g) && c < 0.0) {
+        c = -c;
+        s = -s;
+        r = -r;
+    }
+}
+
+// The exponential of the exponential-sum family, a fixed algorithm so that the device and the CPU
+// oracle evaluate F and J to the same bits: e^x = 2^k e^r with k = rint(x / ln 2) and
+// r = (x − k L1) − k L2 (Cody–Waite: L1 = ln 2 rounded to a multiple of 2^-41, so k L1 is exact for
+// |k| < 2^12, and L2 = ln 2 − L1), e^r from its Taylor polynomial of degree 13 in Horner form with
+// fused multiply-adds (|r| ≤ 0.3466: truncation below 2^-57 relative), then scaled by 2^k. Overflow
+// above ln(2^1024), 0 below ln(2^-1075), NaN unchanged. Within 1 ulp of e^x.
+GS_DEV double expsum_exp(double x) {
+    if (!(x == x)) return x;
+    if (x > 0x1.62e42fefa39efp+9) return __builtin_inf();
+    if (x < -0x1.74910d52d3052p+9) return 0.0;
+    const double kd = __builtin_rint(x * 0x1.71547652b82fep+0);
+    double r = __builtin_fma(-kd, 0x1.62e42fefa4000p-1, x);
+    r = __builtin_fma(-kd, -0x1.8432a1b0e2634p-43, r);
+    double p = 0x1.6124613a86d09p-33;       // 1/13!
+    p = __builtin_fma(p, r, 0x1.1eed8eff8d898p-29);  // 1/12!
+    p = __builtin_fma(p, r, 0x1.ae64567f544e4p-26);  // 1/11!
+    p = __builtin_fma(p, r, 0x1.27e4fb7789f5cp-22);  // 1/10!
+    p = __builtin_fma(p, r, 0x1.71de3a556c734p-19);  // 1/9!
+    p = __builtin_fma(p, r, 0x1.a01a01a01a01ap-16);  // 1/8!
+    p = __builtin_fma(p, r, 0x1.a01a01a01a01ap-13);  // 1/7!
+    p = __builtin_fma(p, r, 0x1.6c16c16c16c17p-10);  // 1/6!
+    p = __builtin_fma(p, r, 0x1.1111111111111p-7);   // 1/5!
+    p = __builtin_fma(p, r, 0x1.5555555555555p-5);   // 1/4!
+    p = __builtin_fma(p, r, 0x1.5555555555555p-3);   // 1/3!
+    p = __builtin_fma(p, r, 0.5);
+    p = __builtin_fma(p, r, 1.0);
+    p = __builtin_fma(p, r, 1.0);
+    return __builtin_ldexp(p, int(kd));
+}
+
+// One fit of the exponential-sum family, x = (a_1..a_K, λ_1..λ_K), t_i = 5i/(m−1) (i from 0):
+// F_i = (Σ_k a_k exp(−λ_k t_i)) − y_i, J_ik = exp(−λ_k t_i), J_i,K+k = ((−a_k) t_i) exp(−λ_k t_i).
+template <int N> struct ExpSumRow {
+    static constexpr int K = N / 2;
+    const double* y;
+    int m;
+    GS_DEV double t(int i) const { return (5.0 * double(i)) / double(m - 1); }
+    GS_DEV double F(const double (&x)[N], int i) const {
+        const double ti = t(i);
+        double s = x[0] * expsum_exp(-x[K] * ti);
+#pragma unroll
+        for (int k = 1; k < K; ++k) s = s + x[k] * expsum_exp(-x[K + k] * ti);
+        return s - y[i];
+    }
+    GS_DEV double FJ(const double (&x)[N], int i, double (&Ji)[N]) const {
+        const double ti = t(i);
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const double e = expsum_exp(-x[K + k] * ti);
+            s = k == 0 ? x[0] * e : s + x[k] * e;
+            Ji[k] = e;
+            Ji[K + k] = ((-x[k]) * ti) * e;
+        }
+        return s - y[i];
+    }
+    // J_ik alone (column k of row i)
+    GS_DEV double J(const double (&x)[N], int i, int k) const {
+        const double ti = t(i);
+        if (k < K) return expsum_exp(-x[K + k] * ti);
+        return ((-x[k - K]) * ti) * expsum_exp(-x[k] * ti);
+    }
+};
+
+// Compile-time loop: fn(IC<i>{}) for i = B, …, E−1, as straight-line code (no loop for the unroller
+// to give up on, so every array index stays a constant and the arrays stay in VGPRs).
+template <int V> struct IC {
+    static constexpr int value = V;
+    constexpr operator int() const { return V; }
+};
+template <int B, int E, class Fn> GS_DEV void static_for(Fn&& fn) {
+    if constexpr (B < E) {
+        fn(IC<B>{});
+        static_for<B + 1, E>(fn);
+    }
+}
+
+// Rotate row `w` (with right-hand side wr) into the upper-triangular factor (R, c), starting at
+// column K0 — the inner step of lm_qr_factor! (:402-419) and of lm_qr_solve! (:449-466).
+template <int K0, int N> GS_DEV void rotate_in(double (&R)[N][N], double (&c)[N], double (&w)[N], double& wr) {
+    static_for<K0, N>([&](auto kk) {
+        constexpr int k = kk;
+        const double b = w[k];
+        if (b != 0.0) {
+            double cs, sn, r;
+            givens(R[k][k], b, cs, sn, r);
+            R[k][k] = r;
+            w[k] = 0.0;
+            static_for<k + 1, N>([&](auto ll) {
+                constexpr int l = ll;
+                const double u = R[k][l], v = w[l];
+                R[k][l] = cs * u + sn * v;
+                w[l] = -sn * u + cs * v;
+            });
+            const double ur = c[k];
+            c[k] = cs * ur + sn * wr;
+            wr = -sn * ur + cs * wr;
+        }
+    });
+}
+
+}  // namespace gs

@@ -74,3 +74,53 @@ def tanh_linear3_batch(nsys: int, seed: int = 0):
 def tanh_linear3_residual(x: np.ndarray, rec: np.ndarray) -> np.ndarray:
     A = rec[:, :9].reshape(-1, 3, 3).transpose(0, 2, 1)  # column-major records
     return np.einsum("sij,sj->si", A, x) + 0.1 * np.tanh(x) - rec[:, 9:]
+
+
+def expsum_fits(nfit: int, seed: int = 0, m_lo: int = 16, m_hi: int = 128, sigma: float = 1e-3):
+    """Exponential-sum least-squares fits y(t) = Σ_{k≤K} a_k exp(−λ_k t) + noise.
+
+    Per fit, drawn in this order: K ~ U{1..4}, m ~ U{m_lo..m_hi}, a_k ~ U[0.5, 2] (K draws),
+    log10 λ_k ~ U[−1, 1] (K draws), then m noise samples N(0, σ²). The points t_i = 5(i−1)/(m−1)
+    are equispaced on [0, 5], so only y travels. The start is a_k = 1, λ_k = 10^(−1 + (2k−1)/K).
+    Returns (K (int32), m (int32), y offsets (int64), Y (concatenated), X0 (nfit, 8), truth (nfit, 8)),
+    with x = (a_1..a_K, λ_1..λ_K) in the first 2K slots of a row.
+    """
+    rng = np.random.default_rng(seed)
+    K = rng.integers(1, 5, size=nfit).astype(np.int32)
+    m = rng.integers(m_lo, m_hi + 1, size=nfit).astype(np.int32)
+    a = rng.uniform(0.5, 2.0, size=(nfit, 4))
+    lam = 10.0 ** rng.uniform(-1.0, 1.0, size=(nfit, 4))
+    off = np.zeros(nfit, dtype=np.int64)
+    off[1:] = np.cumsum(m[:-1], dtype=np.int64)
+    Y = np.empty(int(m.sum()))
+    noise = rng.standard_normal(size=len(Y)) * sigma
+    X0 = np.zeros((nfit, 8))
+    truth = np.zeros((nfit, 8))
+    for k in range(1, 5):
+        sel = K == k
+        X0[sel, :k] = 1.0
+        X0[sel, k:2 * k] = 10.0 ** (-1.0 + (2.0 * np.arange(1, k + 1) - 1.0) / k)
+        truth[sel, :k] = a[sel, :k]
+        truth[sel, k:2 * k] = lam[sel, :k]
+    for mm in np.unique(m):
+        sel = np.nonzero(m == mm)[0]
+        t = 5.0 * np.arange(mm) / (mm - 1)
+        yy = np.zeros((len(sel), mm))
+        for k in range(4):
+            use = (K[sel] > k)[:, None]
+            yy += np.where(use, a[sel, k:k + 1] * np.exp(-lam[sel, k:k + 1] * t[None, :]), 0.0)
+        idx = off[sel][:, None] + np.arange(mm)[None, :]
+        Y[idx] = yy
+    Y += noise
+    return K, m, off, Y, X0, truth
+
+
+def expsum_residual_norms(X8, K, m, off, Y):
+    """‖F(x)‖₂ of each fit (a host recheck)."""
+    out = np.empty(len(K))
+    for i in range(len(K)):
+        k, mm = int(K[i]), int(m[i])
+        t = 5.0 * np.arange(mm) / (mm - 1)
+        f = (X8[i, :k, None] * np.exp(-X8[i, k:2 * k, None] * t[None, :])).sum(axis=0) - Y[off[i]:off[i] + mm]
+        out[i] = np.sqrt((f * f).sum())
+    return out

@@ -170,6 +170,30 @@ int gs_tally_codes(const uint8_t* codes, int64_t nsys, int64_t counts[6], void* 
 int gs_batch_summary_f64(const gs_status_f64* S, int64_t nsys, double* fnorm, int64_t out_i[3],
                          void* stream);
 
+/* The least-squares batch: Newton(; globalization = LevenbergMarquardt()) on nfit exponential-sum
+ * fits y(t) ≈ Σ_{k≤K} a_k exp(−λ_k t) of uneven sizes, all in one call.
+ *
+ * The reference solves a LeastSquaresProblem (src/nonlinear/leastsquares.jl:1-40) only one at a time,
+ * `solve(prob, x₀, method, p)` (src/nonlinear/solver.jl:148-189, 277-378); its solve_batch! takes a
+ * NonlinearProblem only (batched.jl:257-332). This entry point is that per-fit R1 solve for a batch,
+ * with the status form's semantics (one SolverStatus per fit), and the LevenbergMarquardt step on the
+ * stacked Givens reduction of [J | −F] (src/nonlinear/newton.jl:1876-1971;
+ * src/globalization/levenbergmarquardt.jl:48-98, 162-164, 359-482, 575-584).
+ *   X         nfit × 8 doubles; row i holds x = (a_1..a_K, λ_1..λ_K) in its first 2K slots; overwritten.
+ *   K, m      int32 per fit: K ∈ 1..4 terms (n = 2K unknowns), m ∈ 2K..4096 points t_j = 5j/(m−1), j = 0..m−1.
+ *   y_offset  int64 per fit: fit i's data is Y[y_offset[i] .. y_offset[i] + m[i] − 1], inside Y[0 .. ny − 1].
+ *   S         nfit statuses.
+ * Fits with K, m or y_offset out of range make the call fail with GS_EINVAL before any fit is solved. The fits are
+ * bucketed by K on the device and run concurrently on library streams joined to `stream`. */
+int gs_solve_expsum_lsq_f64(const gs_method* method, int64_t nfit, double* X, const int32_t* K,
+                            const int32_t* m, const int64_t* y_offset, const double* Y, int64_t ny,
+                            gs_status_f64* S, const gs_options_f64* options, void* stream);
+
+/* The launch shape of the least-squares bucket K = k (1..4): `threads` per workgroup (64, 128, 192 or
+ * 256) and `fits_per_lane` (1..64; a lane whose fit ends takes the next fit of its bucket). Set by the
+ * package's autotuner; no counterpart in the reference. */
+int gs_lsq_set_shape(int k, int threads, int fits_per_lane);
+
 /* Instrumentation, with no counterpart in the reference: when enabled, gs_solve_batch_f64 brackets
  * its kernel with two HIP events on `stream`, and gs_last_kernel_ms returns the elapsed time of the
  * last solve kernel of this thread (ms), or a negative value if none was timed. */

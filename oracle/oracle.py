@@ -49,6 +49,14 @@ def lib():
         L.orc_backtrack_step.restype = ctypes.c_double
         L.orc_backtrack_step.argtypes = [ctypes.c_double] * 7
         L.orc_max_threads.restype = ctypes.c_int
+        L.orc_solve_expsum_batch.restype = ctypes.c_int
+        L.orc_solve_expsum_batch.argtypes = [vp, vp, ctypes.c_int64, vp, vp, vp, vp, vp, vp, ctypes.c_int]
+        L.orc_solve_mgh.restype = ctypes.c_int
+        L.orc_solve_mgh.argtypes = [ctypes.c_int, vp, vp, vp, vp, vp]
+        L.orc_exp_family.restype = None
+        L.orc_exp_family.argtypes = [vp, vp, ctypes.c_int64]
+        L.orc_givens.restype = None
+        L.orc_givens.argtypes = [ctypes.c_double, ctypes.c_double, vp]
         L.orc_solve_fixture.restype = ctypes.c_int
         L.orc_solve_fixture.argtypes = [ctypes.c_int, vp, vp, vp, vp, vp]
         _LIB = L
@@ -170,6 +178,46 @@ def solve_fixture(fixture: int, method, options, x, p=(0.0, 0.0)):
     if rc != 0:
         raise RuntimeError(f"oracle fixture failed with {rc}")
     return x, S[0]
+
+
+def solve_expsum_batch(method, options, X8, K, m, y_offset, Y, nthreads: int = 0):
+    """The exponential-sum fits (LevenbergMarquardt), X8 an (nfit, 8) array (first 2K slots used)."""
+    X8 = np.ascontiguousarray(X8, dtype=np.float64).copy()
+    K = np.ascontiguousarray(K, dtype=np.int32)
+    m = np.ascontiguousarray(m, dtype=np.int32)
+    off = np.ascontiguousarray(y_offset, dtype=np.int64)
+    Y = np.ascontiguousarray(Y, dtype=np.float64)
+    S = np.zeros(len(X8), dtype=_status_dtype())
+    cm, co = method.to_c(), options.to_c()
+    rc = lib().orc_solve_expsum_batch(ctypes.byref(cm), ctypes.byref(co), len(X8), _ptr(X8), _ptr(K), _ptr(m),
+                                      _ptr(off), _ptr(Y), _ptr(S), nthreads)
+    if rc != 0:
+        raise RuntimeError(f"oracle failed with {rc}")
+    return X8, S
+
+
+def solve_mgh(problem: int, method, options, x0):
+    """One Moré–Garbow–Hillstrom least-squares problem (11, 12, 8, 6): (x, status, f_evals)."""
+    x = np.ascontiguousarray(x0, dtype=np.float64).copy()
+    S = np.zeros(1, dtype=_status_dtype())
+    nf = ctypes.c_long()
+    cm, co = method.to_c(), options.to_c()
+    assert lib().orc_solve_mgh(problem, ctypes.byref(cm), ctypes.byref(co), _ptr(x), _ptr(S), ctypes.byref(nf)) == 0
+    return x, S[0], nf.value
+
+
+def exp_family(x):
+    """The exponential of the exponential-sum family (a fixed algorithm, see lsq_capi.cpp)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.empty_like(x)
+    lib().orc_exp_family(_ptr(x), _ptr(out), x.size)
+    return out
+
+
+def givens(f: float, g: float):
+    out = np.zeros(3)
+    lib().orc_givens(f, g, _ptr(out))
+    return tuple(out)
 
 
 def max_threads() -> int:

@@ -133,3 +133,59 @@ def test_workloads_are_reproducible(workloads):
     assert E.min() >= 0.01 - 1e-12 and E.max() <= 0.15 + 1e-12
     rec, x0 = workloads.tanh_linear3_batch(100, 0)
     assert rec.shape == (100, 12) and not x0.any()
+
+
+def test_least_squares_batch_validation(gs, workloads):
+    """The least-squares batch (ExpSumFits + LevenbergMarquardt) checks its arguments on the host
+    before any device work."""
+    K, m, off, Y, X0, _ = workloads.expsum_fits(8, seed=1)
+    data = gs.ExpSumData(torch.as_tensor(K), torch.as_tensor(m), torch.as_tensor(off), torch.as_tensor(Y))
+    prob = gs.LeastSquaresProblem(gs.ExpSumFits())
+    lm = gs.Newton(globalization=gs.LevenbergMarquardt())
+    X = torch.as_tensor(X0).clone()
+    with pytest.raises(gs.ArgumentError, match="LevenbergMarquardt"):
+        gs.solve_batch(X, prob, gs.Newton(), data)
+    with pytest.raises(gs.ArgumentError, match="order"):
+        gs.solve_batch(X, prob, lm, data, order=np.arange(1, 9))
+    with pytest.raises(gs.ArgumentError, match="workgroupsize"):
+        gs.solve_batch(X, prob, lm, data, workgroupsize=64)
+    with pytest.raises(gs.ArgumentError, match="ExpSumData"):
+        gs.solve_batch(X, prob, lm, gs.PerSystem(np.zeros(8)))
+    with pytest.raises(gs.DimensionMismatch, match="ExpSumData length 8 does not match batch length 7"):
+        gs.solve_batch(X[:7].clone(), prob, lm, data)
+    with pytest.raises(gs.DimensionMismatch):
+        gs.solve_batch(torch.zeros(8, 4, dtype=torch.float64), prob, lm, data)
+    with pytest.raises(gs.ArgumentError, match="device array"):
+        gs.solve_batch(X, prob, lm, data)
+    with pytest.raises(gs.DimensionMismatch, match="lengths differ"):
+        gs.ExpSumData(torch.as_tensor(K), torch.as_tensor(m[:4]), torch.as_tensor(off), torch.as_tensor(Y))
+    with pytest.raises(gs.ArgumentError, match="int32"):
+        gs.ExpSumData(torch.as_tensor(K).long(), torch.as_tensor(m), torch.as_tensor(off), torch.as_tensor(Y))
+    with pytest.raises(gs.ArgumentError, match="least-squares"):
+        gs.solve_batch(torch.zeros(4, 4, dtype=torch.float64), gs.NonlinearProblem(gs.HenonHeilesMidpoint()), lm,
+                       gs.BatchParams(0.1, torch.zeros(4, 4, dtype=torch.float64)))
+    with pytest.raises(gs.ArgumentError, match="least-squares family"):
+        gs.LeastSquaresProblem(gs.HenonHeilesMidpoint())
+    with pytest.raises(gs.ArgumentError, match="μ_increase > 1"):
+        gs.LevenbergMarquardt(mu_increase=1.0)
+
+
+def test_least_squares_entry_point_checks_method_without_gpu(gs):
+    """gs_solve_expsum_lsq_f64 rejects a method outside the LevenbergMarquardt constructor's bounds
+    (which also bound its damping loop) before touching any pointer."""
+    lib = gs.load_library()
+    from importlib import import_module
+
+    abi = import_module(gs.__name__ + "._abi")
+    m = gs.Newton(globalization=gs.LevenbergMarquardt()).to_c()
+    bad = abi.GsMethod.from_buffer_copy(m)
+    bad.lm_mu_increase = 1.0
+    o = gs.Options().to_c()
+    rc = lib.gs_solve_expsum_lsq_f64(ctypes.byref(bad), 1, None, None, None, None, None, 0, None, ctypes.byref(o), None)
+    assert rc == abi.GS_EINVAL and "μ_increase" in abi.last_error(lib)
+    bad = abi.GsMethod.from_buffer_copy(m)
+    bad.globalization = abi.GS_STATIC
+    rc = lib.gs_solve_expsum_lsq_f64(ctypes.byref(bad), 1, None, None, None, None, None, 0, None, ctypes.byref(o), None)
+    assert rc == abi.GS_EFAMILY
+    rc = lib.gs_lsq_set_shape(5, 256, 1)
+    assert rc == abi.GS_EINVAL

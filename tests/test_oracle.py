@@ -251,3 +251,89 @@ def test_hostile_starts(gs, orc):
     P = np.tile([2.125, 1.5], (4, 1))
     x, S = orc.solve_batch(gs.Newton(), gs.Coupled2(), X, gs.PerSystem(P), gs.Options())
     assert list(S["code"]) == [gs.NONFINITE, gs.NONFINITE, gs.SUCCESS, gs.SUCCESS]
+
+
+# ---- Levenberg–Marquardt on least-squares problems ------------------------------------------
+
+
+def test_givens_closed_forms(orc):
+    """LinearAlgebra's givensAlgorithm as levenbergmarquardt.jl:374 calls it: r = √(f²+g²), c = f/r,
+    s = g/r, c ≥ 0 when |f| > |g|, and the rotation annihilates g."""
+    assert orc.givens(3.0, 4.0) == pytest.approx((0.6, 0.8, 5.0))
+    assert orc.givens(2.0, 0.0) == (1.0, 0.0, 2.0)
+    assert orc.givens(0.0, 2.0) == (0.0, 1.0, 2.0)
+    c, s, r = orc.givens(-4.0, 3.0)
+    assert (c, s, r) == pytest.approx((0.8, -0.6, -5.0))
+    rng = np.random.default_rng(5)
+    for f, g in rng.standard_normal((200, 2)) * 10.0 ** rng.integers(-300, 300, size=(200, 1)):
+        c, s, r = orc.givens(f, g)
+        assert c * f + s * g == pytest.approx(r, rel=1e-14)
+        assert abs(-s * f + c * g) <= 1e-14 * abs(r)
+        assert c * c + s * s == pytest.approx(1.0, rel=1e-14)
+
+
+MGH_CASES = [
+    (11, np.ones(5), 5.0),
+    (12, np.zeros(5), 15.0 / 7.0),
+    (8, np.ones(3), 8.214877306578994e-3),
+    (6, np.array([0.3, 0.4]), 124.3621823556148),
+    (6, np.array([0.25, 0.26]), 124.3621823556148),
+]
+
+
+@pytest.mark.parametrize("problem,x0,fstar", MGH_CASES)
+def test_mgh_minima_levenberg_marquardt(gs, orc, problem, x0, fstar):
+    """test/nonlinear/leastsquares.jl:601-740: Newton + LevenbergMarquardt reaches the published
+    nonzero-residual minima, ending on the stationarity test (‖r*‖² to 64 eps)."""
+    m = gs.Newton(globalization=gs.LevenbergMarquardt())
+    opt = gs.Options(max_iterations=200)
+    x, s, nf = orc.solve_mgh(problem, m, opt, x0)
+    assert s["code"] == gs.SUCCESS and s["stationary"] == 1
+    assert s["gradratio"] <= opt.g_tol
+    assert s["fnorm"] ** 2 == pytest.approx(fstar, rel=64 * np.finfo(float).eps)
+    if problem == 11:
+        assert np.allclose(x, -1.0, atol=math.sqrt(np.finfo(float).eps))
+    if problem == 8:
+        assert s["iterations"] >= 2
+
+
+def test_expsum_noise_free_fit_recovers_parameters(gs, orc):
+    """A noise-free single-exponential fit: the residual vanishes and the parameters come back."""
+    m = 40
+    t = 5.0 * np.arange(m) / (m - 1)
+    y = 1.5 * np.exp(-0.7 * t)
+    X8 = np.zeros((1, 8))
+    X8[0, :2] = [1.0, 1.0]
+    x, S = orc.solve_expsum_batch(gs.Newton(globalization=gs.LevenbergMarquardt()), gs.Options(), X8, [1], [m],
+                                  [0], y)
+    assert S[0]["code"] == gs.SUCCESS
+    assert np.allclose(x[0, :2], [1.5, 0.7], rtol=1e-6)
+
+
+def test_exp_family_against_libm_and_exact_values(orc):
+    """The exponential of the exponential-sum family: within 1 ulp of e^x (checked in 40-digit decimal
+    arithmetic), at most 1 ulp from the C library's exp and equal to it almost always, exact at 0,
+    and with the C library's overflow, underflow and NaN behaviour."""
+    from decimal import Decimal, getcontext
+    import math
+
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.uniform(-60.0, 1.0, 200000), rng.uniform(-745.2, 709.8, 200000),
+                        rng.uniform(-1e-6, 1e-6, 10000)])
+    a = orc.exp_family(x)
+    with np.errstate(over="ignore"):
+        b = np.exp(x)
+    d = np.abs(a.view(np.int64) - b.view(np.int64))
+    assert d.max() <= 1
+    assert (d == 0).mean() > 0.9
+    getcontext().prec = 40
+    for xi, ai in zip(x[::400], a[::400]):
+        t = Decimal(float(xi)).exp()
+        ft = float(t)
+        if ft == 0.0 or math.isinf(ft):
+            continue
+        assert abs(Decimal(float(ai)) - t) <= Decimal(math.ulp(ft)), xi
+    s = orc.exp_family(np.array([0.0, -0.0, np.nan, np.inf, -np.inf, 709.78, 709.79, -745.0, -746.0, 1.0]))
+    assert s[0] == 1.0 and s[1] == 1.0 and np.isnan(s[2]) and s[3] == np.inf and s[4] == 0.0
+    assert np.isfinite(s[5]) and s[6] == np.inf and s[7] == 5e-324 and s[8] == 0.0
+    assert s[9] == math.e
