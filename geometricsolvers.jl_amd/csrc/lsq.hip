@@ -278,21 +278,20 @@ template <int N> GS_DEV StepOut lm_step(Fit<N>& f, const LsqParams& q) {
             static_for<j + 1, N>([&](auto ii) {
                 constexpr int i = ii;
                 const double b = R[i][j];
-                if (b != 0.0) {
-                    double cs, sn, r;
-                    givens(R[j][j], b, cs, sn, r);
-                    R[j][j] = r;
-                    R[i][j] = 0.0;
-                    static_for<j + 1, N>([&](auto kk) {
-                        constexpr int k = kk;
-                        const double u = R[j][k], v = R[i][k];
-                        R[j][k] = cs * u + sn * v;
-                        R[i][k] = -sn * u + cs * v;
-                    });
-                    const double ur = c[j], vr = c[i];
-                    c[j] = cs * ur + sn * vr;
-                    c[i] = -sn * ur + cs * vr;
-                }
+                const bool skip = b == 0.0;
+                double cs, sn, r;
+                givens(R[j][j], b, cs, sn, r);
+                R[j][j] = skip ? R[j][j] : r;
+                R[i][j] = 0.0;
+                static_for<j + 1, N>([&](auto kk) {
+                    constexpr int k = kk;
+                    const double u = R[j][k], v = R[i][k];
+                    R[j][k] = skip ? u : cs * u + sn * v;
+                    R[i][k] = skip ? v : -sn * u + cs * v;
+                });
+                const double ur = c[j], vr = c[i];
+                c[j] = skip ? ur : cs * ur + sn * vr;
+                c[i] = skip ? vr : -sn * ur + cs * vr;
             });
         });
         // the remaining rows, streamed in order
@@ -727,10 +726,10 @@ LsqWorkspace g_lsq_ws[64][2];  // per device: [0] bins, cursors and the ordered 
 // The launch shape of each bucket: threads per workgroup and fits per lane. Fewer lanes than fits
 // let a lane that ends early take another fit; more lanes expose more parallel work.
 struct LsqShape {
-    int threads = 256;
+    int threads = 64;
     int fits_per_lane = 1;
 };
-LsqShape g_lsq_shape[4] = {{256, 1}, {256, 1}, {256, 1}, {256, 2}};
+LsqShape g_lsq_shape[4] = {{64, 1}, {64, 1}, {64, 1}, {64, 1}};
 
 // Library streams (default priority) on which the buckets run concurrently, one set per device.
 struct LsqStreams {

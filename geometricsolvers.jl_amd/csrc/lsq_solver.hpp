@@ -30,59 +30,43 @@ struct LsqParams {
 
 // LinearAlgebra's givensAlgorithm (the LAPACK DLARTG scheme): r = √(f² + g²), c = f/r, s = g/r,
 // power-of-two rescaling outside [2^-537, 2^537], c ≥ 0 where |f| > |g|.
+//
+// Written without branches, with the same results bit for bit: for finite f, g the reference's
+// rescaling loops run once (scale ≥ 2^537: one factor 2^-537 brings scale below 2^487; scale ≤
+// 2^-537: one factor 2^537 lifts it above 2^-537 unless scale is 2^-1074 itself, which takes two);
+// every factor is exact, and so is every undo (r · 2^537, or r · 2^-537 once or twice). An infinite
+// scale makes the reference loop 20 times, with the same r = ±Inf, c = NaN and s = ±0 (or NaN) as
+// one factor gives; a NaN scale takes the direct path in both.
 GS_DEV void givens(double f, double g, double& c, double& s, double& r) {
     constexpr double safmn2 = 0x1p-537, safmx2 = 0x1p537;
-    if (g == 0.0) {
-        c = 1.0;
-        s = 0.0;
-        r = f;
-        return;
-    }
-    if (f == 0.0) {
-        c = 0.0;
-        s = 1.0;
-        r = g;
-        return;
-    }
-    double f1 = f, g1 = g;
-    double scale = jmax(__builtin_fabs(f1), __builtin_fabs(g1));
-    // the reference's branch order: a NaN scale fails both comparisons and takes the direct path;
-    // each rescaling loop is bounded (count ≤ 20 above; below, at most four doublings by 2^537)
-    if (__builtin_expect(!(scale >= safmx2) && !(scale <= safmn2), 1)) {
-        r = __builtin_sqrt(f1 * f1 + g1 * g1);
-        c = f1 / r;
-        s = g1 / r;
-    } else if (scale >= safmx2) {
-        int count = 0;
-        do {
-            ++count;
-            f1 *= safmn2;
-            g1 *= safmn2;
-            scale = jmax(__builtin_fabs(f1), __builtin_fabs(g1));
-        } while (!(scale < safmx2 || count >= 20));
-        r = __builtin_sqrt(f1 * f1 + g1 * g1);
-        c = f1 / r;
-        s = g1 / r;
-        for (int i = 0; i < count; ++i) r *= safmx2;
-    } else {
-        int count = 0;
-        do {
-            ++count;
-            f1 *= safmx2;
-            g1 *= safmx2;
-            scale = jmax(__builtin_fabs(f1), __builtin_fabs(g1));
-        } while (!(scale > safmn2) && count < 8);
-        r = __builtin_sqrt(f1 * f1 + g1 * g1);
-        c = f1 / r;
-        s = g1 / r;
-        for (int i = 0; i < count; ++i) r *= safmn2;
-    }
-    if (__builtin_fabs(f) > __builtin_fabs(g) && c < 0.0) {
-        c = -c;
-        s = -s;
-        r = -r;
-    }
+    const double af = __builtin_fabs(f), ag = __builtin_fabs(g);
+    const double scale = jmax(af, ag);
+    const bool big = scale >= safmx2, small = scale <= safmn2;
+    const double m1 = big ? safmn2 : (small ? safmx2 : 1.0);
+    const double m2 = (small && scale == 0x1p-1074) ? safmx2 : 1.0;
+    const double f1 = (f * m1) * m2, g1 = (g * m1) * m2;
+    double rr = __builtin_sqrt(f1 * f1 + g1 * g1);
+    double cc = f1 / rr, ss = g1 / rr;
+    const double u1 = big ? safmx2 : (small ? safmn2 : 1.0);
+    const double u2 = (small && scale == 0x1p-1074) ? safmn2 : 1.0;
+    rr = (rr * u1) * u2;
+    const bool flip = af > ag && cc < 0.0;
+    cc = flip ? -cc : cc;
+    ss = flip ? -ss : ss;
+    rr = flip ? -rr : rr;
+    // g == 0 → (1, 0, f); f == 0 → (0, 1, g)
+    c = g == 0.0 ? 1.0 : (f == 0.0 ? 0.0 : cc);
+    s = g == 0.0 ? 0.0 : (f == 0.0 ? 1.0 : ss);
+    r = g == 0.0 ? f : (f == 0.0 ? g : rr);
 }
+
+// The Taylor coefficients 1/j!, j = 13 … 2, rounded to double. Read from constant memory inside the
+// row loops, they stay in scalar registers (the FMA takes one scalar operand) instead of occupying
+// VGPRs for the life of the kernel.
+__constant__ double kExpTaylor[12] = {
+    0x1.6124613a86d09p-33, 0x1.1eed8eff8d898p-29, 0x1.ae64567f544e4p-26, 0x1.27e4fb7789f5cp-22,
+    0x1.71de3a556c734p-19, 0x1.a01a01a01a01ap-16, 0x1.a01a01a01a01ap-13, 0x1.6c16c16c16c17p-10,
+    0x1.1111111111111p-7,  0x1.5555555555555p-5,  0x1.5555555555555p-3,  0.5};
 
 // The exponential of the exponential-sum family, a fixed algorithm so that the device and the CPU
 // oracle evaluate F and J to the same bits: e^x = 2^k e^r with k = rint(x / ln 2) and
@@ -90,28 +74,21 @@ GS_DEV void givens(double f, double g, double& c, double& s, double& r) {
 // |k| < 2^12, and L2 = ln 2 − L1), e^r from its Taylor polynomial of degree 13 in Horner form with
 // fused multiply-adds (|r| ≤ 0.3466: truncation below 2^-57 relative), then scaled by 2^k. Overflow
 // above ln(2^1024), 0 below ln(2^-1075), NaN unchanged. Within 1 ulp of e^x.
+// Branch-free: the polynomial is evaluated for every x and the special cases selected afterwards (a
+// NaN x propagates through it; the overflow and underflow thresholds are tested on x).
 GS_DEV double expsum_exp(double x) {
-    if (!(x == x)) return x;
-    if (x > 0x1.62e42fefa39efp+9) return __builtin_inf();
-    if (x < -0x1.74910d52d3052p+9) return 0.0;
     const double kd = __builtin_rint(x * 0x1.71547652b82fep+0);
     double r = __builtin_fma(-kd, 0x1.62e42fefa4000p-1, x);
     r = __builtin_fma(-kd, -0x1.8432a1b0e2634p-43, r);
-    double p = 0x1.6124613a86d09p-33;       // 1/13!
-    p = __builtin_fma(p, r, 0x1.1eed8eff8d898p-29);  // 1/12!
-    p = __builtin_fma(p, r, 0x1.ae64567f544e4p-26);  // 1/11!
-    p = __builtin_fma(p, r, 0x1.27e4fb7789f5cp-22);  // 1/10!
-    p = __builtin_fma(p, r, 0x1.71de3a556c734p-19);  // 1/9!
-    p = __builtin_fma(p, r, 0x1.a01a01a01a01ap-16);  // 1/8!
-    p = __builtin_fma(p, r, 0x1.a01a01a01a01ap-13);  // 1/7!
-    p = __builtin_fma(p, r, 0x1.6c16c16c16c17p-10);  // 1/6!
-    p = __builtin_fma(p, r, 0x1.1111111111111p-7);   // 1/5!
-    p = __builtin_fma(p, r, 0x1.5555555555555p-5);   // 1/4!
-    p = __builtin_fma(p, r, 0x1.5555555555555p-3);   // 1/3!
-    p = __builtin_fma(p, r, 0.5);
+    double p = kExpTaylor[0];
+#pragma unroll
+    for (int j = 1; j < 12; ++j) p = __builtin_fma(p, r, kExpTaylor[j]);
     p = __builtin_fma(p, r, 1.0);
     p = __builtin_fma(p, r, 1.0);
-    return __builtin_ldexp(p, int(kd));
+    // |kd| ≤ 1075 wherever the result is used; clamp so the conversion is defined everywhere
+    const double kc = __builtin_fmin(__builtin_fmax(kd, -2000.0), 2000.0);
+    const double e = __builtin_ldexp(p, int(kc));
+    return x > 0x1.62e42fefa39efp+9 ? __builtin_inf() : (x < -0x1.74910d52d3052p+9 ? 0.0 : e);
 }
 
 // One fit of the exponential-sum family, x = (a_1..a_K, λ_1..λ_K), t_i = 5i/(m−1) (i from 0):
@@ -166,22 +143,22 @@ template <int B, int E, class Fn> GS_DEV void static_for(Fn&& fn) {
 template <int K0, int N> GS_DEV void rotate_in(double (&R)[N][N], double (&c)[N], double (&w)[N], double& wr) {
     static_for<K0, N>([&](auto kk) {
         constexpr int k = kk;
+        // a zero entry is skipped (the rotation is computed and not applied), without a branch
         const double b = w[k];
-        if (b != 0.0) {
-            double cs, sn, r;
-            givens(R[k][k], b, cs, sn, r);
-            R[k][k] = r;
-            w[k] = 0.0;
-            static_for<k + 1, N>([&](auto ll) {
-                constexpr int l = ll;
-                const double u = R[k][l], v = w[l];
-                R[k][l] = cs * u + sn * v;
-                w[l] = -sn * u + cs * v;
-            });
-            const double ur = c[k];
-            c[k] = cs * ur + sn * wr;
-            wr = -sn * ur + cs * wr;
-        }
+        const bool skip = b == 0.0;
+        double cs, sn, r;
+        givens(R[k][k], b, cs, sn, r);
+        R[k][k] = skip ? R[k][k] : r;
+        w[k] = 0.0;
+        static_for<k + 1, N>([&](auto ll) {
+            constexpr int l = ll;
+            const double u = R[k][l], v = w[l];
+            R[k][l] = skip ? u : cs * u + sn * v;
+            w[l] = skip ? v : -sn * u + cs * v;
+        });
+        const double ur = c[k];
+        c[k] = skip ? ur : cs * ur + sn * wr;
+        wr = skip ? wr : -sn * ur + cs * wr;
     });
 }
 
