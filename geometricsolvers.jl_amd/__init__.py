@@ -763,3 +763,7 @@ def batch_summary(S) -> BatchSummary:
     _raise_on(lib.gs_batch_summary_f64(ctypes.c_void_p(S.data_ptr()), nsys, ctypes.byref(fn), out,
                                        _stream_ptr(S)), lib)
     return BatchSummary(fn.value, out[0] == 0, int(out[0]), int(out[1]), int(out[2]))
+
+
+from . import learned  # noqa: E402  (the learned start; imports the names above lazily)
+from .learned import LearnedStart, generate_solutions, solve_batch_learned  # noqa: E402

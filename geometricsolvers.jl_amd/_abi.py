@@ -128,6 +128,23 @@ STATUS_DTYPE = np.dtype(
     }
 )
 
+GS_MLP_IN, GS_MLP_HIDDEN, GS_MLP_OUT = 32, 128, 16
+GS_MLP_NPARAM, GS_MLP_NPACKED = 39312, 38912
+
+
+class GsMlp(ctypes.Structure):
+    """gs_mlp: the learned-start network (device parameter and packed-weight pointers, normalisation)."""
+
+    _fields_ = [
+        ("params", ctypes.c_void_p),
+        ("packed", ctypes.c_void_p),
+        ("in_shift", ctypes.c_float * 8),
+        ("in_scale", ctypes.c_float * 8),
+        ("out_shift", ctypes.c_float * 4),
+        ("out_scale", ctypes.c_float * 4),
+    ]
+
+
 # Every symbol that include/gsolve.h declares.
 EXPORTED_SYMBOLS = (
     "gs_solve_batch_f64",
@@ -140,6 +157,10 @@ EXPORTED_SYMBOLS = (
     "gs_last_kernel_ms",
     "gs_solve_expsum_lsq_f64",
     "gs_lsq_set_shape",
+    "gs_mlp_start_henon_heiles",
+    "gs_mlp_grad",
+    "gs_mlp_adam",
+    "gs_mlp_pack",
 )
 
 _LIB = None
@@ -188,6 +209,15 @@ def load_library(path: os.PathLike | None = None):
         ctypes.POINTER(GsMethod), ctypes.c_int64, vp, vp, vp, vp, vp, ctypes.c_int64, vp,
         ctypes.POINTER(GsOptionsF64), vp,
     ]
+    lib.gs_mlp_start_henon_heiles.restype = ctypes.c_int
+    lib.gs_mlp_start_henon_heiles.argtypes = [ctypes.POINTER(GsMlp), ctypes.c_int64, vp, ctypes.c_double, vp, vp]
+    lib.gs_mlp_grad.restype = ctypes.c_int
+    lib.gs_mlp_grad.argtypes = [ctypes.POINTER(GsMlp), vp, vp, ctypes.c_double, vp, ctypes.c_int32, vp, vp, vp]
+    lib.gs_mlp_adam.restype = ctypes.c_int
+    lib.gs_mlp_adam.argtypes = [vp, vp, vp, vp, vp, ctypes.c_int64, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                ctypes.c_float, vp]
+    lib.gs_mlp_pack.restype = ctypes.c_int
+    lib.gs_mlp_pack.argtypes = [vp, vp, vp]
     lib.gs_lsq_set_shape.restype = ctypes.c_int
     lib.gs_lsq_set_shape.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
     if lib.gs_version() != ABI_VERSION:

@@ -1,0 +1,574 @@
+// libgsolve_hip, learned start: a multilayer perceptron that predicts the Newton start of the
+// implicit-midpoint step of Hénon–Heiles, with its inference and training kernels on BF16 MFMA
+// (v_mfma_f32_16x16x32_bf16: BF16 operands, FP32 accumulation).
+//
+// The network g: R^32 → R^16 (5 inputs and 4 outputs used) has three hidden layers of 128 ReLU
+// units. It predicts the correction to the explicit midpoint step x̃ = x + h f(x + (h/2) f(x)),
+// which differs from the implicit midpoint step by O(h³):
+//     x̂₊ = x̃ + h³ (o_j σ_j + μ_j),   o = g(u),  u_j = (x_j − s_j) c_j (j < 4), u_4 = (h − s_4) c_4,
+// so that the network's error (bounded below by the bfloat16 rounding of its inputs) enters the
+// start with a factor h³. Newton in FP64 then polishes x̂₊; the network has no counterpart in the
+// reference.
+//
+// Parameters live in one FP32 array (W1, b1, W2, b2, W3, b3, W4, b4; W_l row-major in × out), and
+// the inference kernel reads a BF16 copy of the weights packed in the MFMA B-operand order, which
+// the Adam kernel writes after each update.
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+
+#include "../../include/gsolve.h"
+#include "device_common.hpp"
+
+namespace gs {
+
+int set_error(int code, const std::string& msg);
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kIn = GS_MLP_IN, kHid = GS_MLP_HIDDEN, kOut = GS_MLP_OUT;
+
+// Offsets of the parameter array.
+constexpr int kW1 = 0, kB1 = kW1 + kIn * kHid, kW2 = kB1 + kHid, kB2 = kW2 + kHid * kHid, kW3 = kB2 + kHid,
+              kB3 = kW3 + kHid * kHid, kW4 = kB3 + kHid, kB4 = kW4 + kHid * kOut, kNParam = kB4 + kOut;
+static_assert(kNParam == GS_MLP_NPARAM, "parameter count");
+// Offsets of the packed BF16 weights (the biases are not packed).
+constexpr int kP1 = 0, kP2 = kP1 + kIn * kHid, kP3 = kP2 + kHid * kHid, kP4 = kP3 + kHid * kHid,
+              kNPacked = kP4 + kHid * kOut;
+static_assert(kNPacked == GS_MLP_NPACKED, "packed weight count");
+
+// The MFMA operand layouts of v_mfma_f32_16x16x32_bf16 (64 lanes):
+//   A (16 × 32): lane l holds row l % 16, columns 8 (l / 16) … 8 (l / 16) + 7;
+//   B (32 × 16): lane l holds column l % 16, rows 8 (l / 16) … 8 (l / 16) + 7;
+//   D (16 × 16): lane l holds column l % 16, rows 4 (l / 16) … 4 (l / 16) + 3.
+// Packed weights: the B operands of layer (in × out) in the order (k-step, column block, lane, 8).
+GS_DEV int packed_index(int k, int col, int out) {
+    const int ks = k >> 5, kk = k & 31, cb = col >> 4, c = col & 15;
+    const int lane = c + 16 * (kk >> 3);
+    return ((ks * (out >> 4) + cb) * 64 + lane) * 8 + (kk & 7);
+}
+
+GS_DEV float relu(float z) { return z > 0.0f ? z : 0.0f; }
+
+// Hénon–Heiles vector field f(q₁, q₂, p₁, p₂) = (p₁, p₂, −q₁ − 2 q₁ q₂, −q₂ − q₁² + q₂²).
+GS_DEV void hh_field(const double (&x)[4], double (&f)[4]) {
+    f[0] = x[2];
+    f[1] = x[3];
+    f[2] = -x[0] - 2.0 * x[0] * x[1];
+    f[3] = (-x[1] - x[0] * x[0]) + x[1] * x[1];
+}
+
+// The explicit midpoint step x̃ = x + h f(x + (h/2) f(x)), the base of the prediction.
+GS_DEV void hh_rk2(const double (&x)[4], double h, double (&xt)[4]) {
+    double f[4], m[4], fm[4];
+    hh_field(x, f);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) m[j] = x[j] + (0.5 * h) * f[j];
+    hh_field(m, fm);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) xt[j] = x[j] + h * fm[j];
+}
+
+struct MlpNorm {
+    float in_shift[8], in_scale[8], out_shift[4], out_scale[4];
+};
+
+MlpNorm norm_of(const gs_mlp& n) {
+    MlpNorm m;
+    for (int j = 0; j < 8; ++j) {
+        m.in_shift[j] = n.in_shift[j];
+        m.in_scale[j] = n.in_scale[j];
+    }
+    for (int j = 0; j < 4; ++j) {
+        m.out_shift[j] = n.out_shift[j];
+        m.out_scale[j] = n.out_scale[j];
+    }
+    return m;
+}
+
+// ---- inference: the start of every system ----------------------------------------------------------
+
+struct StartArgs {
+    const double* Xk;
+    double* X0;
+    int64_t nsys;
+    double h;
+    const __bf16* packed;
+    const float* params;
+    MlpNorm nm;
+};
+
+constexpr int kStartWaves = 8;
+constexpr int kActStride = kHid + 8;  // BF16 row stride of a wave's activation tile (breaks bank conflicts)
+constexpr size_t kStartLds = size_t(kNPacked) * 2 + size_t(3 * kHid + kOut) * 4 + size_t(kStartWaves) * 16 * kActStride * 2 +
+                             size_t(kStartWaves) * 16 * 4 * 4;
+
+// One workgroup of 8 waves per CU holds the packed weights and biases in LDS; each wave runs tiles of
+// 16 systems through the four layers (one 16 × 16 × 32 MFMA per k-step and column block), keeping its
+// tile's activations in a private LDS buffer between layers.
+__global__ __launch_bounds__(64 * kStartWaves) void mlp_start_kernel(const StartArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __bf16* w = reinterpret_cast<__bf16*>(lds);
+    float* bias = reinterpret_cast<float*>(lds + size_t(kNPacked) * 2);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __bf16* act = reinterpret_cast<__bf16*>(lds + size_t(kNPacked) * 2 + size_t(3 * kHid + kOut) * 4) +
+                  size_t(wave) * 16 * kActStride;
+    float* outb = reinterpret_cast<float*>(lds + size_t(kNPacked) * 2 + size_t(3 * kHid + kOut) * 4 +
+                                           size_t(kStartWaves) * 16 * kActStride * 2) +
+                  size_t(wave) * 16 * 4;
+    {
+        const uint4* src = reinterpret_cast<const uint4*>(a.packed);
+        uint4* dst = reinterpret_cast<uint4*>(w);
+        for (int i = threadIdx.x; i < kNPacked / 8; i += blockDim.x) dst[i] = src[i];
+        for (int i = threadIdx.x; i < kHid; i += blockDim.x) {
+            bias[i] = a.params[kB1 + i];
+            bias[kHid + i] = a.params[kB2 + i];
+            bias[2 * kHid + i] = a.params[kB3 + i];
+        }
+        for (int i = threadIdx.x; i < kOut; i += blockDim.x) bias[3 * kHid + i] = a.params[kB4 + i];
+    }
+    __syncthreads();
+    const int r = lane & 15, g = lane >> 4;
+    const int64_t ntile = (a.nsys + 15) / 16;
+    for (int64_t t = int64_t(blockIdx.x) * kStartWaves + wave; t < ntile; t += int64_t(gridDim.x) * kStartWaves) {
+        const int64_t s = t * 16 + r;
+        const bool live = s < a.nsys;
+        double x[4] = {0.0, 0.0, 0.0, 0.0};
+        if (live) {
+            const double2* xp = reinterpret_cast<const double2*>(a.Xk + 4 * s);
+            const double2 u = xp[0], v = xp[1];
+            x[0] = u.x;
+            x[1] = u.y;
+            x[2] = v.x;
+            x[3] = v.y;
+        }
+        // layer 1: A = the features (lanes 0..15 hold row r, columns 0..7; the other lanes zeros)
+        bf16x8 av;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) av[j] = __bf16(0.0f);
+        if (g == 0) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) av[j] = __bf16((float(x[j]) - a.nm.in_shift[j]) * a.nm.in_scale[j]);
+            av[4] = __bf16((float(a.h) - a.nm.in_shift[4]) * a.nm.in_scale[4]);
+        }
+        f32x4 acc[8];
+#pragma unroll
+        for (int cb = 0; cb < 8; ++cb) {
+            acc[cb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+            const bf16x8 bv = *reinterpret_cast<const bf16x8*>(w + kP1 + (cb * 64 + lane) * 8);
+            acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[cb], 0, 0, 0);
+        }
+        // hidden layers 2 and 3 read the previous layer's activations from the wave's tile
+        auto store_act = [&](const float* b) {
+#pragma unroll
+            for (int cb = 0; cb < 8; ++cb) {
+                const int col = cb * 16 + r;
+                const float bc = b[col];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) act[(4 * g + i) * kActStride + col] = __bf16(relu(acc[cb][i] + bc));
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        };
+        auto hidden = [&](int poff) {
+#pragma unroll
+            for (int cb = 0; cb < 8; ++cb) acc[cb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int ks = 0; ks < kHid / 32; ++ks) {
+                const bf16x8 av2 = *reinterpret_cast<const bf16x8*>(act + r * kActStride + ks * 32 + 8 * g);
+#pragma unroll
+                for (int cb = 0; cb < 8; ++cb) {
+                    const bf16x8 bv = *reinterpret_cast<const bf16x8*>(w + poff + ((ks * 8 + cb) * 64 + lane) * 8);
+                    acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av2, bv, acc[cb], 0, 0, 0);
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        };
+        store_act(bias);
+        hidden(kP2);
+        store_act(bias + kHid);
+        hidden(kP3);
+        store_act(bias + 2 * kHid);
+        // output layer (one column block)
+        f32x4 o = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ks = 0; ks < kHid / 32; ++ks) {
+            const bf16x8 av2 = *reinterpret_cast<const bf16x8*>(act + r * kActStride + ks * 32 + 8 * g);
+            const bf16x8 bv = *reinterpret_cast<const bf16x8*>(w + kP4 + (ks * 64 + lane) * 8);
+            o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av2, bv, o, 0, 0, 0);
+        }
+        if (r < 4) {
+            const float bc = bias[3 * kHid + r];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) outb[(4 * g + i) * 4 + r] = o[i] + bc;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (g == 0 && live) {
+            double xt[4];
+            hh_rk2(x, a.h, xt);
+            const double h3 = (a.h * a.h) * a.h;
+            double y[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const double corr = double(outb[r * 4 + j] * a.nm.out_scale[j] + a.nm.out_shift[j]);
+                y[j] = xt[j] + h3 * corr;
+            }
+            double2* yp = reinterpret_cast<double2*>(a.X0 + 4 * s);
+            yp[0] = double2{y[0], y[1]};
+            yp[1] = double2{y[2], y[3]};
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// ---- training ----------------------------------------------------------------------------------------
+
+// C = op(A) · op(B), one wave per 16 × 16 block of C (M and N multiples of 16), K in steps of 32; A(i, k) = A[i as0 + k as1],
+// B(k, j) = B[k bs0 + j bs1], converted to BF16 on load (FP32 master weights or BF16 activations).
+// Epilogues: kPlain C = acc (FP32), kReluAct Z = acc + b → act = relu(Z) (BF16), kBias C = acc + b.
+enum { kPlain = 0, kReluAct = 1, kBias = 2 };
+
+template <class TA, class TB> struct GemmArgs {
+    const TA* A;
+    int64_t as0, as1;
+    const TB* B;
+    int64_t bs0, bs1;
+    int M, N, K;
+    int mode;
+    const float* bias;
+    float* C;
+    __bf16* Cb;
+    int ldc;
+};
+
+template <class T> GS_DEV __bf16 to_bf16(T v) { return __bf16(float(v)); }
+template <> GS_DEV __bf16 to_bf16<__bf16>(__bf16 v) { return v; }
+
+template <class TA, class TB> __global__ __launch_bounds__(256) void gemm_kernel(const GemmArgs<TA, TB> g) {
+    const int64_t wave = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    const int nbn = g.N / 16;
+    if (wave >= int64_t(g.M / 16) * nbn) return;
+    const int bi = int(wave / nbn), bj = int(wave % nbn);
+    const int r = lane & 15, kg = lane >> 4;
+    f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    const TA* ap = g.A + int64_t(bi * 16 + r) * g.as0;
+    const TB* bp = g.B + int64_t(bj * 16 + r) * g.bs1;
+    for (int k0 = 0; k0 < g.K; k0 += 32) {
+        bf16x8 av, bv;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            // K need not be a multiple of 32: the entries past K are zeros, not the next row's
+            const int64_t k = k0 + 8 * kg + j;
+            const bool in = k < g.K;
+            av[j] = in ? to_bf16(ap[in ? k * g.as1 : 0]) : __bf16(0.0f);
+            bv[j] = in ? to_bf16(bp[in ? k * g.bs0 : 0]) : __bf16(0.0f);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
+    }
+    const int col = bj * 16 + r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int row = bi * 16 + 4 * kg + i;
+        float v = acc[i];
+        if (g.mode != kPlain) v += g.bias[col];
+        if (g.mode == kReluAct) {
+            g.C[int64_t(row) * g.ldc + col] = v;
+            g.Cb[int64_t(row) * g.ldc + col] = __bf16(relu(v));
+        } else {
+            g.C[int64_t(row) * g.ldc + col] = v;
+        }
+    }
+}
+
+template <class TA, class TB> void gemm(const GemmArgs<TA, TB>& g, hipStream_t st) {
+    const int64_t waves = int64_t(g.M / 16) * (g.N / 16);
+    hipLaunchKernelGGL((gemm_kernel<TA, TB>), dim3(unsigned((waves + 3) / 4)), dim3(256), 0, st, g);
+}
+
+// The features and normalised targets of a minibatch: row b of U (BF16, kIn wide) and T (FP32, kOut
+// wide) from system idx[b] of (Xk, X1).
+__global__ void features_kernel(const double* Xk, const double* X1, const int64_t* idx, int batch, double h,
+                                MlpNorm nm, __bf16* U, float* T) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= batch) return;
+    const int64_t s = idx[b];
+    double x[4], y[4], xt[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        x[j] = Xk[4 * s + j];
+        y[j] = X1[4 * s + j];
+    }
+    hh_rk2(x, h, xt);
+    const double h3 = (h * h) * h;
+    for (int j = 0; j < kIn; ++j) {
+        float u = 0.0f;
+        if (j < 4) u = (float(x[j]) - nm.in_shift[j]) * nm.in_scale[j];
+        if (j == 4) u = (float(h) - nm.in_shift[4]) * nm.in_scale[4];
+        U[int64_t(b) * kIn + j] = __bf16(u);
+    }
+    for (int j = 0; j < kOut; ++j) {
+        float t = 0.0f;
+        if (j < 4) {
+            const double corr = (y[j] - xt[j]) / h3;
+            t = (float(corr) - nm.out_shift[j]) / nm.out_scale[j];
+        }
+        T[int64_t(b) * kOut + j] = t;
+    }
+}
+
+// Loss ½·mean over the batch and the 4 outputs of (Z4 − T)² (accumulated into *loss) and its
+// gradient dZ4 = (Z4 − T) / (4 · batch) (zero on the padding columns).
+__global__ void loss_kernel(const float* Z4, const float* T, int batch, float* dZ4, float* loss) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    float l = 0.0f;
+    if (i < batch * kOut) {
+        const int j = i % kOut;
+        const float d = j < 4 ? Z4[i] - T[i] : 0.0f;
+        dZ4[i] = d / float(4 * batch);
+        l = 0.5f * d * d / float(4 * batch);
+    }
+    // wave sum, one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) l += __shfl_down(l, off);
+    if ((threadIdx.x & 63) == 0) atomicAdd(loss, l);
+}
+
+// dZ = dA ⊙ [Z > 0] for a ReLU layer (dA FP32 from the GEMM, Z the layer's FP32 pre-activation).
+__global__ void relu_backward_kernel(const float* dA, const float* Z, int64_t n, float* dZ) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) dZ[i] = Z[i] > 0.0f ? dA[i] : 0.0f;
+}
+
+// db_j = Σ_b dZ[b, j] (one workgroup per column).
+__global__ __launch_bounds__(256) void colsum_kernel(const float* dZ, int batch, int width, float* db) {
+    __shared__ float part[256];
+    const int j = blockIdx.x;
+    float s = 0.0f;
+    for (int b = threadIdx.x; b < batch; b += blockDim.x) s += dZ[int64_t(b) * width + j];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) db[j] = part[0];
+}
+
+__global__ void fill_kernel(float* p, int64_t n, float v) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Adam (Kingma & Ba) on the FP32 parameters, then the packed BF16 copy of the weights.
+__global__ void adam_kernel(float* p, const float* grad, float* m, float* v, int64_t t, float lr, float b1, float b2,
+                            float eps, __bf16* packed) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kNParam) return;
+    const float gi = grad[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float mh = mi / (1.0f - __builtin_powf(b1, float(t)));
+    const float vh = vi / (1.0f - __builtin_powf(b2, float(t)));
+    const float pi = p[i] - lr * mh / (__builtin_sqrtf(vh) + eps);
+    p[i] = pi;
+    // the weights' packed BF16 copy
+    int off = -1, k = 0, col = 0, out = kHid;
+    if (i < kB1) { off = kP1; k = i / kHid; col = i % kHid; }
+    else if (i >= kW2 && i < kB2) { off = kP2; k = (i - kW2) / kHid; col = (i - kW2) % kHid; }
+    else if (i >= kW3 && i < kB3) { off = kP3; k = (i - kW3) / kHid; col = (i - kW3) % kHid; }
+    else if (i >= kW4 && i < kB4) { off = kP4; k = (i - kW4) / kOut; col = (i - kW4) % kOut; out = kOut; }
+    if (off >= 0) packed[off + packed_index(k, col, out)] = __bf16(pi);
+}
+
+// Repack all weights (after the caller sets the parameters directly).
+__global__ void pack_kernel(const float* p, __bf16* packed) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kNParam) return;
+    int off = -1, k = 0, col = 0, out = kHid;
+    if (i < kB1) { off = kP1; k = i / kHid; col = i % kHid; }
+    else if (i >= kW2 && i < kB2) { off = kP2; k = (i - kW2) / kHid; col = (i - kW2) % kHid; }
+    else if (i >= kW3 && i < kB3) { off = kP3; k = (i - kW3) / kHid; col = (i - kW3) % kHid; }
+    else if (i >= kW4 && i < kB4) { off = kP4; k = (i - kW4) / kOut; col = (i - kW4) % kOut; out = kOut; }
+    if (off >= 0) packed[off + packed_index(k, col, out)] = __bf16(p[i]);
+}
+
+// Training workspace (per device), sized for the largest batch seen.
+struct MlpWorkspace {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+std::mutex g_mlp_mutex;
+MlpWorkspace g_mlp_ws[64];
+
+int mlp_workspace(size_t bytes, void** out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return set_error(GS_EHIP, "hipGetDevice failed");
+    std::lock_guard<std::mutex> lock(g_mlp_mutex);
+    MlpWorkspace& w = g_mlp_ws[dev];
+    if (w.bytes < bytes) {
+        if (w.ptr) (void)hipFree(w.ptr);
+        w.ptr = nullptr;
+        w.bytes = 0;
+        if (hipMalloc(&w.ptr, bytes) != hipSuccess) return set_error(GS_EHIP, "hipMalloc of the training workspace failed");
+        w.bytes = bytes;
+    }
+    *out = w.ptr;
+    return GS_OK;
+}
+
+int check(hipError_t e, const char* what) {
+    if (e != hipSuccess) return set_error(GS_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+    return GS_OK;
+}
+
+}  // namespace
+}  // namespace gs
+
+using namespace gs;
+
+extern "C" {
+
+int gs_mlp_start_henon_heiles(const gs_mlp* net, int64_t nsys, const double* Xk, double h, double* X0, void* stream) {
+    if (!net || !net->params || !net->packed) return set_error(GS_EINVAL, "the network must not be NULL");
+    if (nsys < 0) return set_error(GS_EINVAL, "nsys must be ≥ 0");
+    if (nsys == 0) return GS_OK;
+    if (!Xk || !X0) return set_error(GS_EINVAL, "Xk and X0 must not be NULL");
+    if ((reinterpret_cast<uintptr_t>(Xk) % 16) || (reinterpret_cast<uintptr_t>(X0) % 16) ||
+        (reinterpret_cast<uintptr_t>(net->packed) % 16))
+        return set_error(GS_EINVAL, "Xk, X0 and the packed weights must be 16-byte aligned");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    static bool attr_set[64] = {};
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return set_error(GS_EHIP, "hipGetDevice failed");
+    if (!attr_set[dev]) {
+        int rc = check(hipFuncSetAttribute(reinterpret_cast<const void*>(&mlp_start_kernel),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(kStartLds)),
+                       "hipFuncSetAttribute");
+        if (rc) return rc;
+        attr_set[dev] = true;
+    }
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    StartArgs a;
+    a.Xk = Xk;
+    a.X0 = X0;
+    a.nsys = nsys;
+    a.h = h;
+    a.packed = reinterpret_cast<const __bf16*>(net->packed);
+    a.params = net->params;
+    a.nm = norm_of(*net);
+    const int64_t ntile = (nsys + 15) / 16;
+    int64_t blocks = (ntile + kStartWaves - 1) / kStartWaves;
+    if (blocks > cus) blocks = cus;
+    hipLaunchKernelGGL(mlp_start_kernel, dim3(unsigned(blocks)), dim3(64 * kStartWaves), kStartLds, st, a);
+    return check(hipGetLastError(), "learned-start launch failed");
+}
+
+int gs_mlp_grad(const gs_mlp* net, const double* Xk, const double* X1, double h, const int64_t* idx, int32_t batch,
+                float* grad, float* loss, void* stream) {
+    if (!net || !net->params || !Xk || !X1 || !idx || !grad || !loss)
+        return set_error(GS_EINVAL, "arguments must not be NULL");
+    if (batch <= 0 || batch % 32 != 0 || batch > (1 << 20)) return set_error(GS_EINVAL, "batch must be a positive multiple of 32 up to 2^20");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const size_t B = size_t(batch);
+    // workspace: U (B×kIn bf16), T (B×kOut f32), Z1..Z3 and A1..A3 (B×kHid f32 / bf16), Z4 (B×kOut f32),
+    // dZ4 (B×kOut f32), dA (B×kHid f32), dZ (B×kHid f32)
+    // (12 regions, each rounded up to 256 bytes)
+    const size_t bytes = B * kIn * 2 + B * kOut * 4 + 3 * B * kHid * 4 + 3 * B * kHid * 2 + 2 * B * kOut * 4 + 2 * B * kHid * 4 + 12 * 256;
+    void* ws = nullptr;
+    int rc = mlp_workspace(bytes, &ws);
+    if (rc) return rc;
+    char* p = static_cast<char*>(ws);
+    auto take = [&](size_t n) {
+        char* q = p;
+        p += (n + 255) & ~size_t(255);
+        return q;
+    };
+    __bf16* U = reinterpret_cast<__bf16*>(take(B * kIn * 2));
+    float* T = reinterpret_cast<float*>(take(B * kOut * 4));
+    float* Z[3];
+    __bf16* A[3];
+    for (int l = 0; l < 3; ++l) Z[l] = reinterpret_cast<float*>(take(B * kHid * 4));
+    for (int l = 0; l < 3; ++l) A[l] = reinterpret_cast<__bf16*>(take(B * kHid * 2));
+    float* Z4 = reinterpret_cast<float*>(take(B * kOut * 4));
+    float* dZ4 = reinterpret_cast<float*>(take(B * kOut * 4));
+    float* dA = reinterpret_cast<float*>(take(B * kHid * 4));
+    float* dZ = reinterpret_cast<float*>(take(B * kHid * 4));
+    const float* P = net->params;
+    MlpNorm nm = norm_of(*net);
+    hipLaunchKernelGGL(features_kernel, dim3(unsigned((B + 255) / 256)), dim3(256), 0, st, Xk, X1, idx, batch, h, nm, U, T);
+    // forward
+    {
+        GemmArgs<__bf16, float> g{U, kIn, 1, P + kW1, kHid, 1, batch, kHid, kIn, kReluAct, P + kB1, Z[0], A[0], kHid};
+        gemm(g, st);
+    }
+    for (int l = 1; l < 3; ++l) {
+        const int wo = l == 1 ? kW2 : kW3, bo = l == 1 ? kB2 : kB3;
+        GemmArgs<__bf16, float> g{A[l - 1], kHid, 1, P + wo, kHid, 1, batch, kHid, kHid, kReluAct, P + bo, Z[l], A[l], kHid};
+        gemm(g, st);
+    }
+    {
+        GemmArgs<__bf16, float> g{A[2], kHid, 1, P + kW4, kOut, 1, batch, kOut, kHid, kBias, P + kB4, Z4, nullptr, kOut};
+        gemm(g, st);
+    }
+    hipLaunchKernelGGL(fill_kernel, dim3(1), dim3(64), 0, st, loss, int64_t(1), 0.0f);
+    hipLaunchKernelGGL(loss_kernel, dim3(unsigned((B * kOut + 255) / 256)), dim3(256), 0, st, Z4, T, batch, dZ4, loss);
+    // backward: the output layer
+    {
+        // dW4 = A3ᵀ dZ4 (kHid × kOut), db4 = Σ dZ4, dA3 = dZ4 W4ᵀ
+        GemmArgs<__bf16, float> g{A[2], 1, kHid, dZ4, kOut, 1, kHid, kOut, batch, kPlain, nullptr, grad + kW4, nullptr, kOut};
+        gemm(g, st);
+        hipLaunchKernelGGL(colsum_kernel, dim3(kOut), dim3(256), 0, st, dZ4, batch, kOut, grad + kB4);
+        GemmArgs<float, float> gd{dZ4, kOut, 1, P + kW4, 1, kOut, batch, kHid, kOut, kPlain, nullptr, dA, nullptr, kHid};
+        gemm(gd, st);
+    }
+    for (int l = 2; l >= 0; --l) {
+        hipLaunchKernelGGL(relu_backward_kernel, dim3(unsigned((B * kHid + 255) / 256)), dim3(256), 0, st, dA, Z[l],
+                           int64_t(B * kHid), dZ);
+        const int wo = l == 0 ? kW1 : l == 1 ? kW2 : kW3, bo = l == 0 ? kB1 : l == 1 ? kB2 : kB3;
+        const int in = l == 0 ? kIn : kHid;
+        if (l == 0) {
+            GemmArgs<__bf16, float> g{U, 1, kIn, dZ, kHid, 1, kIn, kHid, batch, kPlain, nullptr, grad + wo, nullptr, kHid};
+            gemm(g, st);
+        } else {
+            GemmArgs<__bf16, float> g{A[l - 1], 1, kHid, dZ, kHid, 1, in, kHid, batch, kPlain, nullptr, grad + wo, nullptr, kHid};
+            gemm(g, st);
+        }
+        hipLaunchKernelGGL(colsum_kernel, dim3(kHid), dim3(256), 0, st, dZ, batch, kHid, grad + bo);
+        if (l > 0) {
+            GemmArgs<float, float> gd{dZ, kHid, 1, P + wo, 1, kHid, batch, kHid, kHid, kPlain, nullptr, dA, nullptr, kHid};
+            gemm(gd, st);
+        }
+    }
+    return check(hipGetLastError(), "training launch failed");
+}
+
+int gs_mlp_adam(float* params, uint16_t* packed, const float* grad, float* m, float* v, int64_t step, float lr,
+                float beta1, float beta2, float eps, void* stream) {
+    if (!params || !packed || !grad || !m || !v) return set_error(GS_EINVAL, "arguments must not be NULL");
+    if (step < 1) return set_error(GS_EINVAL, "step must be ≥ 1");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(adam_kernel, dim3(unsigned((kNParam + 255) / 256)), dim3(256), 0, st, params, grad, m, v, step, lr,
+                       beta1, beta2, eps, reinterpret_cast<__bf16*>(packed));
+    return check(hipGetLastError(), "Adam launch failed");
+}
+
+int gs_mlp_pack(const float* params, uint16_t* packed, void* stream) {
+    if (!params || !packed) return set_error(GS_EINVAL, "arguments must not be NULL");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(pack_kernel, dim3(unsigned((kNParam + 255) / 256)), dim3(256), 0, st, params,
+                       reinterpret_cast<__bf16*>(packed));
+    return check(hipGetLastError(), "pack launch failed");
+}
+
+}  // extern "C"

@@ -194,6 +194,49 @@ int gs_solve_expsum_lsq_f64(const gs_method* method, int64_t nfit, double* X, co
  * package's autotuner; no counterpart in the reference. */
 int gs_lsq_set_shape(int k, int threads, int fits_per_lane);
 
+/* The learned start, with no counterpart in the reference: a multilayer perceptron (inputs padded to
+ * GS_MLP_IN, three hidden ReLU layers of GS_MLP_HIDDEN, outputs padded to GS_MLP_OUT; BF16 operands,
+ * FP32 accumulation on MFMA) that predicts the Newton start of the implicit-midpoint step of
+ * Hénon–Heiles (GS_FAM_HENON_HEILES_MIDPOINT) as a correction to the explicit midpoint step
+ * x̃ = x + h f(x + (h/2) f(x)):
+ *     x̂₊ = x̃ + h³ (o_j out_scale_j + out_shift_j),
+ *     o = net(u),  u_j = (x_j − in_shift_j) in_scale_j (j < 4),  u_4 = (h − in_shift_4) in_scale_4.
+ * params: GS_MLP_NPARAM floats (W1, b1, W2, b2, W3, b3, W4, b4; each W row-major, inputs × outputs);
+ * packed: GS_MLP_NPACKED bfloat16 words, the weights in the MFMA operand order (gs_mlp_pack,
+ * gs_mlp_adam write them). Newton in FP64 polishes the start; plain Newton from x stays the fallback. */
+#define GS_MLP_IN 32
+#define GS_MLP_HIDDEN 128
+#define GS_MLP_OUT 16
+#define GS_MLP_NPARAM 39312
+#define GS_MLP_NPACKED 38912
+
+typedef struct gs_mlp {
+    const float* params;
+    const uint16_t* packed;
+    float in_shift[8];
+    float in_scale[8];
+    float out_shift[4];
+    float out_scale[4];
+} gs_mlp;
+
+/* X0[i] = x̂₊ of Xk[i] (nsys × 4 doubles each, 16-byte aligned), on `stream` (asynchronous). */
+int gs_mlp_start_henon_heiles(const gs_mlp* net, int64_t nsys, const double* Xk, double h, double* X0,
+                              void* stream);
+
+/* Loss ½ mean_b,j<4 (o_j − t_j)² of a minibatch (systems idx[0..batch−1] of the training pairs
+ * (Xk, X1), t the normalised correction of X1) into *loss (device float), and its gradient with respect
+ * to every parameter into grad (GS_MLP_NPARAM device floats). batch is a multiple of 32. Asynchronous. */
+int gs_mlp_grad(const gs_mlp* net, const double* Xk, const double* X1, double h, const int64_t* idx,
+                int32_t batch, float* grad, float* loss, void* stream);
+
+/* One Adam step (step ≥ 1, the bias corrections use it) on params with grad and the moments m, v
+ * (all GS_MLP_NPARAM device floats), then the packed BF16 weights. Asynchronous. */
+int gs_mlp_adam(float* params, uint16_t* packed, const float* grad, float* m, float* v, int64_t step, float lr,
+                float beta1, float beta2, float eps, void* stream);
+
+/* The packed BF16 weights of params. Asynchronous. */
+int gs_mlp_pack(const float* params, uint16_t* packed, void* stream);
+
 /* Instrumentation, with no counterpart in the reference: when enabled, gs_solve_batch_f64 brackets
  * its kernel with two HIP events on `stream`, and gs_last_kernel_ms returns the elapsed time of the
  * last solve kernel of this thread (ms), or a negative value if none was timed. */
