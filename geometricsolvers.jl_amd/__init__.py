@@ -32,7 +32,7 @@ __all__ = [
     "SINGULAR", "NONFINITE", "LINESEARCH_FAILED", "STATUS_DTYPE", "L2Norm", "MaxNorm", "Options",
     "LUFactorization", "Static", "Backtracking", "LevenbergMarquardt", "NoRefinement", "Refinement", "Newton",
     "NonlinearProblem", "Coupled2", "Hetero2", "ScalarSquare", "TanhLinear3",
-    "HenonHeilesMidpoint", "PerSystem", "BatchParams", "parameters", "solve_batch",
+    "HenonHeilesMidpoint", "FpuGl3", "PerSystem", "BatchParams", "parameters", "solve_batch",
     "LeastSquaresProblem", "ExpSumFits", "ExpSumData",
     "status_buffer", "statuses", "isconverged", "batch_summary", "BatchSummary", "load_library",
 ]
@@ -437,6 +437,29 @@ class HenonHeilesMidpoint(Family):
         h = float(p.shared)
         t = _records(p.per_system, nsys, 4, torch.float64, device, "HenonHeilesMidpoint")
         return t, t.stride(0) * t.element_size(), (h,)
+
+
+class FpuGl3(Family):
+    """The Gauss–Legendre s = 3 stage system of a 16-site FPU-β chain with fixed ends, n = 48.
+
+    Position-only (Nyström) form of the collocation equations, Q_i = q + h c_i p + h² Σ_j (A²)_ij g(Q_j),
+    in the stage increments z_i = Q_i − q (the reference's StageResidual, src/nonlinear/problem.jl:
+    183-189, stacks the 96 first-order increments of (q, p); this form eliminates the momenta).
+    V(r) = r²/2 + β r⁴/4 per bond. Parameter: BatchParams(h, QP) with QP (nsys, 32) = (q, p) per system;
+    the start is z = 0. One wave per system on the device, the LU factor in LDS.
+    """
+
+    id, n, p_len = _abi.GS_FAM_FPU_GL3, 48, 32
+
+    def __init__(self, beta: float = 1.0):
+        self.beta = float(beta)
+
+    def marshal(self, p, nsys, device):
+        torch = _torch()
+        if not isinstance(p, BatchParams):
+            raise ArgumentError("FpuGl3 takes BatchParams(h, QP) with QP the (q, p) record of every system")
+        t = _records(p.per_system, nsys, 32, torch.float64, device, "FpuGl3")
+        return t, t.stride(0) * t.element_size(), (float(p.shared), self.beta)
 
 
 class NonlinearProblem:

@@ -10,6 +10,7 @@
 #include <string>
 
 #include "../../include/gsolve.h"
+#include "mid_solver.hpp"
 #include "small_solver.hpp"
 
 namespace gs {
@@ -132,6 +133,7 @@ static const FamilySpec* family_spec(int id) {
         {GS_FAM_SCALAR_SQUARE, 1, 1, 8, 8},
         {GS_FAM_TANH_LINEAR3, 3, 12, 96, 8},
         {GS_FAM_HENON_HEILES_MIDPOINT, 4, 4, 32, 16},
+        {GS_FAM_FPU_GL3, 48, 32, 256, 8},
     };
     for (const auto& s : specs)
         if (s.id == id) return &s;
@@ -172,6 +174,8 @@ int gs_solve_batch_f64(const gs_method* method, const gs_family* family, int64_t
     if (method->globalization != GS_STATIC && method->globalization != GS_BACKTRACKING)
         return fail(GS_EFAMILY, "globalization " + std::to_string(method->globalization) +
                                     " has no lane-per-system form for this family");
+    if (family->id == GS_FAM_FPU_GL3 && method->globalization != GS_STATIC)
+        return fail(GS_EFAMILY, "the wave-per-system solve of FpuGl3 takes the Static globalization");
     if (method->factor_precision != GS_FACTOR_WORKING && method->factor_precision != GS_FACTOR_F32)
         return fail(GS_EINVAL, "factor_precision must be GS_FACTOR_WORKING or GS_FACTOR_F32");
     if (method->refinement != GS_NO_REFINEMENT && method->refinement != GS_REFINEMENT)
@@ -234,6 +238,33 @@ int gs_solve_batch_f64(const gs_method* method, const gs_family* family, int64_t
         timed = hipEventRecord(g_ev[0], st) == hipSuccess;
     }
     int rc;
+    if (family->id == GS_FAM_FPU_GL3) {
+        // one wave per system (the workgroup size is the kernel's own: 64)
+        MidLaunch L;
+        std::memset(&L, 0, sizeof(L));
+        L.q = q;
+        L.P = static_cast<const char*>(P);
+        L.p_stride = p_stride;
+        for (int k = 0; k < 8; ++k) L.shared[k] = family->shared[k];
+        L.nsys = nsys;
+        L.X = X;
+        L.S = S;
+        L.codes = codes;
+        L.order = order;
+        L.mixed = mixed;
+        L.refine = refine;
+        rc = launch_fpu_gl3(L, st);
+        if (rc != GS_OK) return rc;
+        if (timed) timed = hipEventRecord(g_ev[1], st) == hipSuccess;
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return fail(GS_EHIP, std::string("kernel launch failed: ") + hipGetErrorString(e));
+        e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return fail(GS_EHIP, std::string("hipStreamSynchronize failed: ") + hipGetErrorString(e));
+        g_last_ms = -1.0;
+        float ms = 0.0f;
+        if (timed && hipEventElapsedTime(&ms, g_ev[0], g_ev[1]) == hipSuccess) g_last_ms = ms;
+        return GS_OK;
+    }
     switch (family->id) {
         case GS_FAM_COUPLED2: rc = launch_family<FamCoupled2>(a, glob, mixed, refine, diag, wg, st); break;
         case GS_FAM_HETERO2: rc = launch_family<FamHetero2>(a, glob, mixed, refine, diag, wg, st); break;

@@ -1,0 +1,29 @@
+// The launch of the mid-size solve (one wave per system): the arguments gs_solve_batch_f64 hands to
+// the wave-per-system kernels of mid_solver.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/gsolve.h"
+#include "small_solver.hpp"
+
+namespace gs {
+
+struct MidLaunch {
+    DevParams q;
+    const char* P;        // parameter records
+    int64_t p_stride;     // bytes between records (0: one shared record)
+    double shared[8];     // the family's shared scalars
+    int64_t nsys;
+    double* X;
+    gs_status_f64* S;
+    uint8_t* codes;
+    const int64_t* order;
+    bool mixed, refine;
+};
+
+// The FPU-β Gauss–Legendre s = 3 stage system (GS_FAM_FPU_GL3, n = 48) with the Static
+// globalization; returns a gs_error code (the launch only; the caller synchronises).
+int launch_fpu_gl3(const MidLaunch& L, hipStream_t st);
+
+}  // namespace gs

@@ -76,6 +76,35 @@ def tanh_linear3_residual(x: np.ndarray, rec: np.ndarray) -> np.ndarray:
     return np.einsum("sij,sj->si", A, x) + 0.1 * np.tanh(x) - rec[:, 9:]
 
 
+def fpu_states(nsys: int, seed: int = 0, sigma: float = 0.3) -> np.ndarray:
+    """FPU-β chain states: (q, p) ~ N(0, σ²) entrywise, 32 draws per system (q₁..q₁₆ then p₁..p₁₆).
+    Returns (nsys, 32)."""
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal(size=(nsys, 32)) * sigma
+
+
+def fpu_gl3_residual(z: np.ndarray, qp: np.ndarray, h: float, beta: float = 1.0) -> np.ndarray:
+    """The stage residual of FpuGl3, vectorised over systems (a host recheck; not the oracle)."""
+    import math
+
+    s15 = math.sqrt(15.0)
+    c = np.array([0.5 - s15 / 10, 0.5, 0.5 + s15 / 10])
+    A = np.array([[5 / 36, 2 / 9 - s15 / 15, 5 / 36 - s15 / 30],
+                  [5 / 36 + s15 / 24, 2 / 9, 5 / 36 - s15 / 24],
+                  [5 / 36 + s15 / 30, 2 / 9 + s15 / 15, 5 / 36]])
+    A2 = A @ A
+    q, p = qp[:, :16], qp[:, 16:]
+    n = len(z)
+    Q = q[:, None, :] + z.reshape(n, 3, 16)
+    pad = np.zeros((n, 3, 1))
+    Qe = np.concatenate([pad, Q, pad], axis=2)
+    r = np.diff(Qe, axis=2)                      # 17 bonds
+    dphi = r + beta * r ** 3
+    g = dphi[:, :, 1:] - dphi[:, :, :-1]
+    R = z.reshape(n, 3, 16) - h * c[None, :, None] * p[:, None, :] - h * h * np.einsum("ab,nbm->nam", A2, g)
+    return R.reshape(n, 48)
+
+
 def expsum_fits(nfit: int, seed: int = 0, m_lo: int = 16, m_hi: int = 128, sigma: float = 1e-3):
     """Exponential-sum least-squares fits y(t) = Σ_{k≤K} a_k exp(−λ_k t) + noise.
 

@@ -148,9 +148,42 @@ int orc_solve_batch_f64(const gs_method* method, const gs_family* fam, int64_t n
             });
             return 0;
         }
+        case GS_FAM_FPU_GL3: {
+            const double h = fam->shared[0], beta = fam->shared[1];
+            run_batch<48>(m, o, nsys, X, S, codes, order, nthreads, [&](int64_t idx) {
+                const double* p = reinterpret_cast<const double*>(rec(idx));
+                FpuGl3<double> pr;
+                pr.h = h;
+                pr.beta = beta;
+                for (int k = 0; k < 16; ++k) {
+                    pr.q[k] = p[k];
+                    pr.p[k] = p[16 + k];
+                }
+                return pr;
+            });
+            return 0;
+        }
         default:
             return GS_EFAMILY;
     }
+}
+
+// F and J of the FPU-β Gauss–Legendre stage system at z (48), record qp = (q, p) (32): r (48) and
+// Jc (48×48 column-major), for the family KATs.
+void orc_fpu_gl3_eval(double h, double beta, const double* qp, const double* z, double* r, double* Jc) {
+    FpuGl3<double> pr;
+    pr.h = h;
+    pr.beta = beta;
+    for (int k = 0; k < 16; ++k) {
+        pr.q[k] = qp[k];
+        pr.p[k] = qp[16 + k];
+    }
+    Vec<double, 48> zz;
+    for (int k = 0; k < 48; ++k) zz[k] = z[k];
+    Vec<double, 48> F = pr.F(zz);
+    Mat<double, 48> J = pr.J(zz);
+    for (int k = 0; k < 48; ++k) r[k] = F[k];
+    for (int k = 0; k < 48 * 48; ++k) Jc[k] = J[k];
 }
 
 // --- building blocks for the known-answer tests -------------------------------------------------

@@ -120,6 +120,91 @@ template <class T> struct HenonHeilesMidpoint {
     }
 };
 
+// The Gauss–Legendre s = 3 stage system of a 16-site FPU-β chain with fixed ends, n = 48.
+//
+// The chain: positions q_1..q_16 (q_0 = q_17 = 0), H = Σ p²/2 + Σ_{s=0}^{16} φ(q_{s+1} − q_s),
+// φ(r) = r²/2 + β r⁴/4, force g_m = φ'(q_{m+1} − q_m) − φ'(q_m − q_{m−1}), φ'(r) = r + β r³.
+// The reference's StageResidual (src/nonlinear/problem.jl:183-189) stacks the first-order stage
+// increments of (q, p) (96 unknowns); this family is its position-only (Nyström) form: with
+// P_i = p + h Σ_j a_ij g(Q_j) substituted into Q_i = q + h Σ_j a_ij P_j,
+//     Q_i = q + h c_i p + h² Σ_j (A²)_ij g(Q_j),
+// and the unknowns are the position increments z_i = Q_i − q (start z = 0, the reference's
+// increment convention):
+//     R_{a,m}(z) = (z_{a,m} − (h c_a) p_m) − h² ((A²_{a0} g_{0,m} + A²_{a1} g_{1,m}) + A²_{a2} g_{2,m}),
+//     J_{(a,m),(b,k)} = δ − (h² A²_{ab}) H^b_{mk},  H^b the tridiagonal Hessian of −V at Q_b:
+//     H^b_{mm} = −(K_{b,m} + K_{b,m+1}), H^b_{m,m−1} = K_{b,m}, H^b_{m,m+1} = K_{b,m+1},
+//     K_{b,s} = φ''(r_{b,s}) = 1 + (3β) r², r_{b,s} = Q_{b,s} − Q_{b,s−1} (bond s, ends at 0).
+// c and A² are the Gauss–Legendre tableau (c = ½ ∓ √15/10, ½; A the collocation matrix of
+// test/helpers/stage_problems.jl:148-160) computed in 50-digit arithmetic and rounded once.
+constexpr double kGlC[3] = {0x1.cda042f0236e1p-4, 0x1.0000000000000p-1, 0x1.c64bf7a1fb924p-1};
+constexpr double kGlA2[3][3] = {{0x1.6c16c16c16c17p-7, -0x1.0f9922995a9d1p-7, 0x1.ce741a1dacb5ep-9},
+                                {0x1.a370da4233f7dp-4, 0x1.c71c71c71c71cp-6, -0x1.537f6b3fb1445p-8},
+                                {0x1.bfe2a15ea5befp-3, 0x1.4f8d7b6829931p-3, 0x1.6c16c16c16c17p-7}};
+
+template <class T> struct FpuGl3 {
+    static constexpr int N = 48;
+    T h, beta;
+    Vec<T, 16> q, p;
+    long nf = 0;
+    // Q_b,m = q_m + z_{b,m}
+    void stages(const Vec<T, 48>& z, T (&Q)[3][16]) const {
+        for (int b = 0; b < 3; ++b)
+            for (int m = 0; m < 16; ++m) Q[b][m] = q[m] + z[b * 16 + m];
+    }
+    static T bond(const T (&Qb)[16], int s) {  // r_s = Q_s − Q_{s−1}, the ends at 0
+        T hi = s < 16 ? Qb[s] : T(0);
+        T lo = s > 0 ? Qb[s - 1] : T(0);
+        return hi - lo;
+    }
+    Vec<T, 48> F(const Vec<T, 48>& z) {
+        ++nf;
+        T Q[3][16], g[3][16];
+        stages(z, Q);
+        for (int b = 0; b < 3; ++b)
+            for (int m = 0; m < 16; ++m) {
+                T rl = bond(Q[b], m), rr = bond(Q[b], m + 1);
+                T dl = rl + beta * ((rl * rl) * rl), dr = rr + beta * ((rr * rr) * rr);
+                g[b][m] = dr - dl;
+            }
+        const T h2 = h * h;
+        Vec<T, 48> R;
+        for (int a = 0; a < 3; ++a) {
+            const T hc = h * T(kGlC[a]);
+            for (int m = 0; m < 16; ++m) {
+                T s = T(kGlA2[a][0]) * g[0][m];
+                s = s + T(kGlA2[a][1]) * g[1][m];
+                s = s + T(kGlA2[a][2]) * g[2][m];
+                R[a * 16 + m] = (z[a * 16 + m] - hc * p[m]) - h2 * s;
+            }
+        }
+        return R;
+    }
+    Mat<T, 48> J(const Vec<T, 48>& z) const {
+        T Q[3][16], K[3][17];
+        stages(z, Q);
+        for (int b = 0; b < 3; ++b)
+            for (int s = 0; s <= 16; ++s) {
+                T r = bond(Q[b], s);
+                K[b][s] = T(1) + (T(3) * beta) * (r * r);
+            }
+        const T h2 = h * h;
+        Mat<T, 48> J;
+        for (int j = 0; j < 48; ++j) {
+            const int b = j / 16, k = j % 16;
+            for (int i = 0; i < 48; ++i) {
+                const int a = i / 16, m = i % 16;
+                T H = T(0);
+                if (k == m) H = -(K[b][m] + K[b][m + 1]);
+                else if (k + 1 == m) H = K[b][m];
+                else if (k == m + 1) H = K[b][m + 1];
+                const T coef = h2 * T(kGlA2[a][b]);
+                J[i + 48 * j] = (i == j ? T(1) : T(0)) - coef * H;
+            }
+        }
+        return J;
+    }
+};
+
 // Fixtures of the reference's loop tests (test/nonlinear/solver.jl:183-290), for the oracle's KATs:
 //   101: y.^2 − q, J = Diagonal(2y), n = 2         (solver.jl:242-246)
 //   102: 2y − q,  J = 2I, n = 2                    (solver.jl:279-281)

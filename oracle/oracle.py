@@ -53,6 +53,8 @@ def lib():
         L.orc_solve_expsum_batch.argtypes = [vp, vp, ctypes.c_int64, vp, vp, vp, vp, vp, vp, ctypes.c_int]
         L.orc_solve_mgh.restype = ctypes.c_int
         L.orc_solve_mgh.argtypes = [ctypes.c_int, vp, vp, vp, vp, vp]
+        L.orc_fpu_gl3_eval.restype = None
+        L.orc_fpu_gl3_eval.argtypes = [ctypes.c_double, ctypes.c_double, vp, vp, vp, vp]
         L.orc_exp_family.restype = None
         L.orc_exp_family.argtypes = [vp, vp, ctypes.c_int64]
         L.orc_givens.restype = None
@@ -70,6 +72,9 @@ def _ptr(a: np.ndarray):
 def host_records(family, p, nsys: int):
     """The parameter records of a batch on the host: (array, byte stride, shared scalars)."""
     name = type(family).__name__
+    if name == "FpuGl3":
+        rec = np.ascontiguousarray(np.asarray(p.per_system, dtype=np.float64).reshape(nsys, 32))
+        return rec, rec.strides[0], (float(p.shared), float(family.beta))
     if name == "HenonHeilesMidpoint":
         rec = np.ascontiguousarray(np.asarray(p.per_system, dtype=np.float64).reshape(nsys, 4))
         return rec, rec.strides[0], (float(p.shared),)
@@ -204,6 +209,16 @@ def solve_mgh(problem: int, method, options, x0):
     cm, co = method.to_c(), options.to_c()
     assert lib().orc_solve_mgh(problem, ctypes.byref(cm), ctypes.byref(co), _ptr(x), _ptr(S), ctypes.byref(nf)) == 0
     return x, S[0], nf.value
+
+
+def fpu_gl3_eval(h: float, beta: float, qp, z):
+    """(F, J) of the FPU-β Gauss–Legendre s = 3 stage system (families.hpp FpuGl3)."""
+    qp = np.ascontiguousarray(qp, dtype=np.float64)
+    z = np.ascontiguousarray(z, dtype=np.float64)
+    r = np.empty(48)
+    J = np.empty(48 * 48)
+    lib().orc_fpu_gl3_eval(h, beta, _ptr(qp), _ptr(z), _ptr(r), _ptr(J))
+    return r, J.reshape(48, 48, order="F")
 
 
 def exp_family(x):
