@@ -1,16 +1,24 @@
-"""Throughput of the batched FP64 Newton solve on one MI355X.
+"""Throughput of the batched Newton solve on one MI355X: the learned start with the mixed-precision
+method against the plain FP64 method.
 
 Workload (the configuration the headline metric is quoted on): one implicit-midpoint step of the
 Hénon–Heiles system for 2^20 independent states, n = 4 unknowns each, h = 0.1, states on energy
-shells E ~ U[0.01, 0.15]; Newton + Static + static LU in FP64 from x₀ = x_k, f_abstol = 1e-12,
-f_reltol = 0, status form of solve_batch (72-byte SolverStatus per system). A "step" is one
-solve_batch call over a fresh copy of the batch; every copy is made before the clock starts, as the
-reference's device benchmark does (benchmark/devices.jl:372-404).
+shells E ~ U[0.01, 0.15] (seed 0); f_abstol = 1e-12, f_reltol = 0, status form of solve_batch
+(72-byte SolverStatus per system).
 
-value = converged systems/s: systems with code SUCCESS whose FP64 residual, rechecked on the host
-after the timed region, satisfies ‖F(x)‖₂ ≤ 1e-12; divided by the wall time of the K timed steps.
+  plain          Newton + Static + static LU in FP64 from x₀ = x_k.
+  learned+mixed  the learned start (a 3 × 128 BF16 MLP, trained before the clock starts on 10^6
+                 solutions of the plain method, from states of another seed) followed by Newton with
+                 LUFactorization(Float32) + Refinement, and plain Newton from x_k for every system
+                 that does not converge from the learned start.
 
-It prints one JSON line (rank 0, one GPU).
+A "step" is one solve of the whole batch in the given form. value = converged systems/s of the
+learned+mixed form: systems with code SUCCESS whose FP64 residual, rechecked on the host after the
+timed region, satisfies ‖F(x)‖₂ ≤ 1e-12, divided by the wall time of the K timed steps; the plain
+form's figure and the ratio sit beside it in `config`.
+
+`--workload fpu_gl3` and `--workload expsum_lsq` print the same kind of line for the 65536-system
+FPU-β Gauss–Legendre stage systems (n = 48) and the 262144 exponential-sum fits.
 """
 from __future__ import annotations
 
@@ -29,14 +37,19 @@ sys.path.insert(0, str(ROOT))
 NSYS = 1 << 20
 H = 0.1
 TOL = 1e-12
-# algorithmic HBM bytes per system of the status form: x_k 32 (record) + X in 32 + X out 32 + S 72
-BYTES_PER_SYSTEM = 32 + 32 + 32 + 72
+NTRAIN = 1_000_000
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
+FP64_PEAK_TFLOPS = 78.6
+# algorithmic BF16 flops per system of the learned start: 2 × (5·128 + 128·128 + 128·128 + 128·4)
+MLP_FLOPS_PER_SYSTEM = 2 * (5 * 128 + 128 * 128 + 128 * 128 + 128 * 4)
+# algorithmic HBM bytes per system of the plain solve: x_k 32 (record) + X in 32 + X out 32 + S 72
+BYTES_PER_SYSTEM = 32 + 32 + 32 + 72
 
 
-def cpu_baseline(gs, workloads, xk, min_seconds: float = 8.0):
+def cpu_baseline(gs, xk, min_seconds: float = 8.0):
     """The oracle (C++ restatement of the reference's R1 loop, OpenMP over systems) on the host cores:
-    a bounded sample of the same workload, repeated until `min_seconds` of wall time."""
+    the plain method on a bounded sample of the same workload, repeated until `min_seconds`."""
     from oracle import oracle as orc
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
@@ -53,13 +66,257 @@ def cpu_baseline(gs, workloads, xk, min_seconds: float = 8.0):
         if el >= min_seconds:
             break
     conv = int((S["code"] == 0).sum())
+    # the same port on one thread, on a smaller sample
+    one = xk[: 1 << 14]
+    reps1, t1 = 0, time.perf_counter()
+    while True:
+        _, S1 = orc.solve_batch(gs.Newton(), gs.HenonHeilesMidpoint(), one.copy(), gs.BatchParams(H, one), opts,
+                                nthreads=1)
+        reps1 += 1
+        el1 = time.perf_counter() - t1
+        if el1 >= min_seconds / 4:
+            break
     return {
         "value": conv * reps / el,
         "unit": "converged systems/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{len(sample)} systems of the same workload (seed 0) x {reps} repetitions, {el:.1f} s wall, "
-                  f"OpenMP schedule(static) over systems",
+        "sample": f"plain method on {len(sample)} systems of the same workload (seed 0) x {reps} repetitions, "
+                  f"{el:.1f} s wall, OpenMP schedule(static) over systems",
+        "single_thread": {"value": int((S1["code"] == 0).sum()) * reps1 / el1, "cores": 1,
+                          "sample": f"{len(one)} systems x {reps1} repetitions, {el1:.1f} s wall"},
+    }
+
+
+def _event_ms(events):
+    import torch
+
+    torch.cuda.synchronize()
+    return [a.elapsed_time(b) for a, b in events]
+
+
+def bench_henon_heiles(args, gs, workloads, torch):
+    dev = torch.device("cuda:0")
+    lib = gs.load_library()
+    K, W = args.steps, args.warmup
+    xk = workloads.henon_heiles_states(NSYS, seed=0)
+    Xk = torch.as_tensor(xk, device=dev)
+    prob = gs.NonlinearProblem(gs.HenonHeilesMidpoint())
+    opts = gs.Options(f_abstol=TOL, f_reltol=0.0)
+    p = gs.BatchParams(H, Xk)
+    plain = gs.Newton()
+    mixed = gs.Newton(linsolve=gs.LUFactorization(np.float32), refinement=gs.Refinement())
+
+    # the network, trained before any clock starts
+    t0 = time.perf_counter()
+    Xt = torch.as_tensor(workloads.henon_heiles_states(NTRAIN, seed=100), device=dev)
+    X1, _ = gs.generate_solutions(gs, Xt, H, opts)
+    net = gs.LearnedStart(dev, seed=0)
+    net.fit_normalization(Xt, X1, H)
+    hist = net.train(Xt, X1, H, epochs=args.epochs, batch=4096, lr=3e-3, lr_final=3e-5, seed=0)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    del Xt, X1
+
+    # plain: K steps on fresh copies made before the clock
+    starts = [Xk.clone() for _ in range(K + W)]
+    Sp = [gs.status_buffer(NSYS, dev) for _ in range(K)]
+    Sw = gs.status_buffer(NSYS, dev)
+    lib.gs_set_kernel_timing(1)
+    for i in range(W):
+        gs.solve_batch(starts[K + i], Sw, prob, plain, p, opts)
+    torch.cuda.synchronize()
+    solve_ms = []
+    t0 = time.perf_counter()
+    for i in range(K):
+        gs.solve_batch(starts[i], Sp[i], prob, plain, p, opts)
+        solve_ms.append(lib.gs_last_kernel_ms())
+    torch.cuda.synchronize()
+    plain_el = time.perf_counter() - t0
+    lib.gs_set_kernel_timing(0)
+
+    # learned + mixed: the start kernel writes every step's X, so no copies are needed
+    Xl = [torch.empty_like(Xk) for _ in range(K)]
+    Sl = [gs.status_buffer(NSYS, dev) for _ in range(K)]
+    Xw = torch.empty_like(Xk)
+    for i in range(W):
+        gs.solve_batch_learned(gs, Xw, Sw, prob, mixed, p, net, opts)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(K)]
+    fallbacks = 0
+    t0 = time.perf_counter()
+    for i in range(K):
+        # the start kernel alone, bracketed on the stream it runs on, for the roofline
+        ev[i][0].record()
+        net.start(Xk, H, out=Xl[i])
+        ev[i][1].record()
+        gs.solve_batch(Xl[i], Sl[i], prob, mixed, p, opts)
+        bad = torch.nonzero(Sl[i][:, 0] != 0).flatten()
+        nb = int(bad.numel())
+        if nb:
+            fallbacks += nb
+            Xb = Xk.index_select(0, bad).contiguous()
+            Sb = gs.status_buffer(nb, dev)
+            gs.solve_batch(Xb, Sb, prob, plain, gs.BatchParams(H, Xb.clone()), opts)
+            Xl[i].index_copy_(0, bad, Xb)
+            Sl[i].index_copy_(0, bad, Sb)
+    torch.cuda.synchronize()
+    learned_el = time.perf_counter() - t0
+    start_ms = float(np.mean(_event_ms(ev)))
+
+    # host FP64 recheck of every step's result, outside the timed regions
+    def converged(Xs, Ss):
+        n = 0
+        for X, S in zip(Xs, Ss):
+            st = gs.statuses(S)
+            x = X.cpu().numpy()
+            r = np.sqrt((workloads.henon_heiles_midpoint_residual(x, xk, H) ** 2).sum(axis=1))
+            n += int(((st["code"] == 0) & (r <= TOL)).sum())
+        return n
+
+    conv_plain = converged(starts[:K], Sp)
+    conv_learned = converged(Xl, Sl)
+    it_plain = float(gs.statuses(Sp[-1])["iterations"].mean())
+    it_learned = float(gs.statuses(Sl[-1])["iterations"].mean())
+    value = conv_learned / learned_el
+    plain_value = conv_plain / plain_el
+    achieved = MLP_FLOPS_PER_SYSTEM * NSYS / (start_ms * 1e-3) / 1e12
+    traffic = None
+    tfile = ROOT / "profiles" / "traffic_mlp_start_henon_heiles_2p20.json"
+    if tfile.exists():
+        traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
+    solve_kms = float(np.mean(solve_ms))
+    return {
+        "metric": "converged systems/s at residual ≤1e-12 (FP64 check), learned+mixed vs plain, 1 GPU",
+        "value": value,
+        "unit": "converged systems/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": learned_el / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (numpy PCG64, Hénon–Heiles energy shells; batch seed 0, training seed 100)",
+        "config": {
+            "workload": "henon_heiles_implicit_midpoint n=4, batch 2^20, h=0.1, abstol 1e-12, reltol 0, status form",
+            "method": "learned start (3x128 BF16 MLP) + Newton(LUFactorization(Float32), Refinement), "
+                      "plain Newton fallback",
+            "batch": NSYS,
+            "learned_mixed_value": value,
+            "plain_value": plain_value,
+            "learned_mixed_vs_plain": value / plain_value,
+            "plain_ms_per_step": plain_el / K * 1e3,
+            "plain_solve_kernel_ms": solve_kms,
+            "plain_solve_kernel_hbm_frac": BYTES_PER_SYSTEM * NSYS / (solve_kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "learned_start_kernel_ms": start_ms,
+            "mean_iterations_plain": it_plain,
+            "mean_iterations_learned_mixed": it_learned,
+            "fallback_systems": fallbacks,
+            "training": {"pairs": NTRAIN, "epochs": args.epochs, "seconds": train_s, "final_loss": hist[-1]},
+        },
+        "roofline": {
+            "bound": "mfma",
+            "kernel": "mlp_start_kernel (BF16 MFMA learned start)",
+            "achieved": achieved,
+            "peak": BF16_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / BF16_PEAK_TFLOPS,
+            "traffic": traffic,
+        },
+        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(gs, xk),
+    }
+
+
+def bench_fpu_gl3(args, gs, workloads, torch):
+    """65536 FPU-β Gauss–Legendre s = 3 stage systems (n = 48): plain FP64 and mixed FP32 + refinement."""
+    dev = torch.device("cuda:0")
+    n = 65536
+    qp = workloads.fpu_states(n, seed=0)
+    QP = torch.as_tensor(qp, device=dev)
+    prob = gs.NonlinearProblem(gs.FpuGl3())
+    p = gs.BatchParams(0.05, QP)
+    opts = gs.Options(f_abstol=TOL, f_reltol=0.0)
+    out = {}
+    for name, m in (("plain", gs.Newton()),
+                    ("mixed", gs.Newton(linsolve=gs.LUFactorization(np.float32), refinement=gs.Refinement()))):
+        Zs = [torch.zeros((n, 48), dtype=torch.float64, device=dev) for _ in range(args.steps + args.warmup)]
+        S = [gs.status_buffer(n, dev) for _ in range(args.steps)]
+        Sw = gs.status_buffer(n, dev)
+        for i in range(args.warmup):
+            gs.solve_batch(Zs[args.steps + i], Sw, prob, m, p, opts)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            gs.solve_batch(Zs[i], S[i], prob, m, p, opts)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        conv = 0
+        for i in range(args.steps):
+            st = gs.statuses(S[i])
+            r = np.sqrt((workloads.fpu_gl3_residual(Zs[i].cpu().numpy(), qp, 0.05) ** 2).sum(axis=1))
+            conv += int(((st["code"] == 0) & (r <= TOL)).sum())
+        out[name] = (conv / el, el / args.steps * 1e3, float(gs.statuses(S[-1])["iterations"].mean()))
+    return {
+        "metric": "converged systems/s at residual ≤1e-12 (FP64 check), 1 GPU",
+        "value": out["mixed"][0],
+        "unit": "converged systems/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": out["mixed"][1],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic ((q, p) ~ N(0, 0.3^2), numpy PCG64 seed 0)",
+        "config": {"workload": "fpu_beta_gl3 n=48, batch 65536, h=0.05, beta=1, abstol 1e-12, reltol 0",
+                   "method": "Newton(LUFactorization(Float32) blocked with MFMA trailing updates, Refinement)",
+                   "plain_value": out["plain"][0], "plain_ms_per_step": out["plain"][1],
+                   "mean_iterations": out["mixed"][2]},
+    }
+
+
+def bench_expsum(args, gs, workloads, torch):
+    """262144 exponential-sum least-squares fits (K ~ U{1..4}, m ~ U{16..128}), LevenbergMarquardt."""
+    dev = torch.device("cuda:0")
+    n = 262144
+    K_, m_, off, Y, X0, _ = workloads.expsum_fits(n, seed=0)
+    data = gs.ExpSumData(*(torch.as_tensor(v, device=dev) for v in (K_, m_, off, Y)))
+    prob = gs.LeastSquaresProblem(gs.ExpSumFits())
+    method = gs.Newton(globalization=gs.LevenbergMarquardt())
+    opts = gs.Options()
+    X0d = torch.as_tensor(X0, device=dev)
+    Xs = [X0d.clone() for _ in range(args.steps + args.warmup)]
+    S = [gs.status_buffer(n, dev) for _ in range(args.steps)]
+    Sw = gs.status_buffer(n, dev)
+    for i in range(args.warmup):
+        gs.solve_batch(Xs[args.steps + i], Sw, prob, method, data, opts)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        gs.solve_batch(Xs[i], S[i], prob, method, data, opts)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    st = gs.statuses(S[-1])
+    codes = np.bincount(st["code"], minlength=6)
+    return {
+        "metric": "fits/s (every fit run to its LevenbergMarquardt exit), 1 GPU",
+        "value": n * args.steps / el,
+        "unit": "fits/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (numpy PCG64 seed 0)",
+        "config": {"workload": "expsum_lsq 262144 fits, K~U{1..4}, m~U{16..128}, sigma 1e-3",
+                   "method": "Newton + LevenbergMarquardt (stacked Givens), defaults",
+                   "codes": codes.tolist(), "total_iterations": int(st["iterations"].sum())},
     }
 
 
@@ -68,6 +325,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--workload", choices=("henon_heiles", "fpu_gl3", "expsum_lsq"), default="henon_heiles")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -75,88 +334,18 @@ def main() -> int:
               "MI355X; --gpus above 1 is not supported", file=sys.stderr)
         return 2
 
+    import importlib
+
     import torch
 
     import __graft_entry__
 
     gs = __graft_entry__.load_package()
-    import importlib
-
     workloads = importlib.import_module(gs.__name__ + ".workloads")
-    lib = gs.load_library()
-    dev = torch.device("cuda:0")
-    xk = workloads.henon_heiles_states(NSYS, seed=0)
-    Xk = torch.as_tensor(xk, device=dev)
-    K, W = args.steps, args.warmup
-    starts = [Xk.clone() for _ in range(K + W)]
-    S = [gs.status_buffer(NSYS, dev) for _ in range(K)]
-    S_warm = gs.status_buffer(NSYS, dev)
-    prob = gs.NonlinearProblem(gs.HenonHeilesMidpoint())
-    method = gs.Newton()
-    opts = gs.Options(f_abstol=TOL, f_reltol=0.0)
-    p = gs.BatchParams(H, Xk)
-
-    lib.gs_set_kernel_timing(1)
-    for i in range(W):
-        gs.solve_batch(starts[K + i], S_warm, prob, method, p, opts)
-    torch.cuda.synchronize()
-    kernel_ms = []
-    t0 = time.perf_counter()
-    for i in range(K):
-        gs.solve_batch(starts[i], S[i], prob, method, p, opts)
-        kernel_ms.append(lib.gs_last_kernel_ms())
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    lib.gs_set_kernel_timing(0)
-
-    # host FP64 recheck of every step's result, outside the timed region
-    converged = 0
-    for i in range(K):
-        st = gs.statuses(S[i])
-        x = starts[i].cpu().numpy()
-        r = np.sqrt((workloads.henon_heiles_midpoint_residual(x, xk, H) ** 2).sum(axis=1))
-        converged += int(((st["code"] == 0) & (r <= TOL)).sum())
-    iters = gs.statuses(S[-1])["iterations"]
-
-    value = converged / elapsed
-    kms = float(np.mean(kernel_ms))
-    achieved = BYTES_PER_SYSTEM * NSYS / (kms * 1e-3) / 1e9
-    traffic = None
-    tfile = ROOT / "profiles" / "traffic_henon_heiles_midpoint_2p20.json"
-    if tfile.exists():
-        traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
-    line = {
-        "metric": "converged systems/s at residual ≤1e-12 (FP64 check), learned+mixed vs plain, 1 GPU",
-        "value": value,
-        "unit": "converged systems/s",
-        "n_gpus": 1,
-        "steps": K,
-        "warmup": W,
-        "ms_per_step": elapsed / K * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (numpy PCG64 seed 0, Hénon–Heiles energy shells)",
-        "config": {
-            "workload": "henon_heiles_implicit_midpoint n=4 FP64, batch 2^20, h=0.1, "
-                        "Newton+Static+LU, abstol 1e-12, reltol 0, status form",
-            "method": "plain FP64 Newton (lane per system)",
-            "batch": NSYS,
-            "mean_iterations": float(iters.mean()),
-            "kernel_ms": kms,
-        },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-        },
-        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(gs, workloads, xk),
-    }
-    print(json.dumps(line), flush=True)
+    fn = {"henon_heiles": bench_henon_heiles, "fpu_gl3": bench_fpu_gl3, "expsum_lsq": bench_expsum}[args.workload]
+    if args.workload != "henon_heiles" and "--steps" not in sys.argv:
+        args.steps, args.warmup = 5, 1
+    print(json.dumps(fn(args, gs, workloads, torch)), flush=True)
     return 0
 
 
