@@ -158,6 +158,7 @@ EXPORTED_SYMBOLS = (
     "gs_solve_expsum_lsq_f64",
     "gs_lsq_set_shape",
     "gs_mlp_start_henon_heiles",
+    "gs_solve_learned_henon_heiles_f64",
     "gs_mlp_grad",
     "gs_mlp_adam",
     "gs_mlp_pack",
@@ -211,6 +212,11 @@ def load_library(path: os.PathLike | None = None):
     ]
     lib.gs_mlp_start_henon_heiles.restype = ctypes.c_int
     lib.gs_mlp_start_henon_heiles.argtypes = [ctypes.POINTER(GsMlp), ctypes.c_int64, vp, ctypes.c_double, vp, vp]
+    lib.gs_solve_learned_henon_heiles_f64.restype = ctypes.c_int
+    lib.gs_solve_learned_henon_heiles_f64.argtypes = [
+        ctypes.POINTER(GsMlp), ctypes.POINTER(GsMethod), ctypes.c_int64, vp, ctypes.c_double, vp, vp,
+        ctypes.POINTER(GsOptionsF64), ctypes.POINTER(ctypes.c_int64), vp,
+    ]
     lib.gs_mlp_grad.restype = ctypes.c_int
     lib.gs_mlp_grad.argtypes = [ctypes.POINTER(GsMlp), vp, vp, ctypes.c_double, vp, ctypes.c_int32, vp, vp, vp]
     lib.gs_mlp_adam.restype = ctypes.c_int

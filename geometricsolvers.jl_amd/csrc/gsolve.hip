@@ -25,10 +25,54 @@ int set_error(int code, const std::string& msg) {
 
 static int fail(int code, const std::string& msg) { return set_error(code, msg); }
 
+// The kernels' view of a method and its options (options.jl:58-71; newton.jl:138-250).
+DevParams dev_params(const gs_method& method, const gs_options_f64& options) {
+    DevParams q;
+    std::memset(&q, 0, sizeof(q));
+    q.f_abstol = options.f_abstol;
+    q.f_reltol = options.f_reltol;
+    q.x_abstol = options.x_abstol;
+    q.x_reltol = options.x_reltol;
+    q.kappa = options.kappa;
+    q.min_iterations = options.min_iterations;
+    q.max_iterations = options.max_iterations;
+    q.max_stalls = options.max_stalls;
+    q.norm_max = options.norm == GS_NORM_MAX;
+    q.roundoff = options.roundoff != 0;
+    q.rate_test = options.rate_test != 0;
+    q.alpha = method.static_alpha;
+    q.c1 = method.bt_c1;
+    q.p = method.bt_p;
+    q.bt_maxiter = method.bt_maxiter;
+    q.ref_maxiter = method.ref_maxiter;
+    q.ref_eta_eff = method.ref_eta > 1.4901161193847656e-8 ? method.ref_eta : 1.4901161193847656e-8;
+    q.mu = method.regularization;
+    q.best = method.best != 0;
+    return q;
+}
+
 // Kernel timing (gs_set_kernel_timing): events are created once per thread and reused.
 static thread_local bool g_timing = false;
 static thread_local double g_last_ms = -1.0;
 static thread_local hipEvent_t g_ev[2] = {nullptr, nullptr};
+
+// The bracket of gs_set_kernel_timing around one kernel launch on `st` (every entry point that
+// times its kernel uses it): begin records the first event when timing is on.
+bool timing_begin(hipStream_t st) {
+    g_last_ms = -1.0;
+    if (!g_timing) return false;
+    if (!g_ev[0] && (hipEventCreate(&g_ev[0]) != hipSuccess || hipEventCreate(&g_ev[1]) != hipSuccess)) return false;
+    return hipEventRecord(g_ev[0], st) == hipSuccess;
+}
+
+// Record the second event after the launch (before the caller synchronises).
+bool timing_mark(hipStream_t st, bool timed) { return timed && hipEventRecord(g_ev[1], st) == hipSuccess; }
+
+// After the synchronisation: the elapsed time for gs_last_kernel_ms.
+void timing_end(bool timed) {
+    float ms = 0.0f;
+    g_last_ms = (timed && hipEventElapsedTime(&ms, g_ev[0], g_ev[1]) == hipSuccess) ? double(ms) : -1.0;
+}
 
 struct KernelArgs {
     FamArgs fam;
@@ -200,25 +244,7 @@ int gs_solve_batch_f64(const gs_method* method, const gs_family* family, int64_t
     a.codes = codes;
     a.order = order;
     DevParams& q = a.q;
-    q.f_abstol = options->f_abstol;
-    q.f_reltol = options->f_reltol;
-    q.x_abstol = options->x_abstol;
-    q.x_reltol = options->x_reltol;
-    q.kappa = options->kappa;
-    q.min_iterations = options->min_iterations;
-    q.max_iterations = options->max_iterations;
-    q.max_stalls = options->max_stalls;
-    q.norm_max = options->norm == GS_NORM_MAX;
-    q.roundoff = options->roundoff != 0;
-    q.rate_test = options->rate_test != 0;
-    q.alpha = method->static_alpha;
-    q.c1 = method->bt_c1;
-    q.p = method->bt_p;
-    q.bt_maxiter = method->bt_maxiter;
-    q.ref_maxiter = method->ref_maxiter;
-    q.ref_eta_eff = method->ref_eta > 1.4901161193847656e-8 ? method->ref_eta : 1.4901161193847656e-8;
-    q.mu = method->regularization;
-    q.best = method->best != 0;
+    q = dev_params(*method, *options);
 
     const bool diag = S != nullptr;
     if (!diag && !codes) return fail(GS_EINVAL, "codes must not be NULL");
@@ -231,12 +257,7 @@ int gs_solve_batch_f64(const gs_method* method, const gs_family* family, int64_t
     const bool mixed = method->factor_precision == GS_FACTOR_F32;
     const bool refine = method->refinement == GS_REFINEMENT;
     const int glob = method->globalization;
-    bool timed = false;
-    if (g_timing) {
-        if (!g_ev[0] && (hipEventCreate(&g_ev[0]) != hipSuccess || hipEventCreate(&g_ev[1]) != hipSuccess))
-            return fail(GS_EHIP, "hipEventCreate failed");
-        timed = hipEventRecord(g_ev[0], st) == hipSuccess;
-    }
+    bool timed = timing_begin(st);
     int rc;
     if (family->id == GS_FAM_FPU_GL3) {
         // one wave per system (the workgroup size is the kernel's own: 64)
@@ -255,14 +276,12 @@ int gs_solve_batch_f64(const gs_method* method, const gs_family* family, int64_t
         L.refine = refine;
         rc = launch_fpu_gl3(L, st);
         if (rc != GS_OK) return rc;
-        if (timed) timed = hipEventRecord(g_ev[1], st) == hipSuccess;
+        timed = timing_mark(st, timed);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return fail(GS_EHIP, std::string("kernel launch failed: ") + hipGetErrorString(e));
         e = hipStreamSynchronize(st);
         if (e != hipSuccess) return fail(GS_EHIP, std::string("hipStreamSynchronize failed: ") + hipGetErrorString(e));
-        g_last_ms = -1.0;
-        float ms = 0.0f;
-        if (timed && hipEventElapsedTime(&ms, g_ev[0], g_ev[1]) == hipSuccess) g_last_ms = ms;
+        timing_end(timed);
         return GS_OK;
     }
     switch (family->id) {
@@ -276,14 +295,12 @@ int gs_solve_batch_f64(const gs_method* method, const gs_family* family, int64_t
         default: return fail(GS_EFAMILY, "problem family is not registered");
     }
     if (rc != GS_OK) return rc;
-    if (timed) timed = hipEventRecord(g_ev[1], st) == hipSuccess;
+    timed = timing_mark(st, timed);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(GS_EHIP, std::string("kernel launch failed: ") + hipGetErrorString(e));
     e = hipStreamSynchronize(st);
     if (e != hipSuccess) return fail(GS_EHIP, std::string("hipStreamSynchronize failed: ") + hipGetErrorString(e));
-    g_last_ms = -1.0;
-    float ms = 0.0f;
-    if (timed && hipEventElapsedTime(&ms, g_ev[0], g_ev[1]) == hipSuccess) g_last_ms = ms;
+    timing_end(timed);
     return GS_OK;
 }
 

@@ -9,6 +9,14 @@
 
 namespace gs {
 
+// The kernels' view of a method and its options (defined with gs_solve_batch_f64).
+DevParams dev_params(const gs_method& method, const gs_options_f64& options);
+
+// The gs_set_kernel_timing bracket around one launch (defined with gs_solve_batch_f64).
+bool timing_begin(hipStream_t st);
+bool timing_mark(hipStream_t st, bool timed);
+void timing_end(bool timed);
+
 struct MidLaunch {
     DevParams q;
     const char* P;        // parameter records

@@ -15,11 +15,13 @@
 // the Adam kernel writes after each update.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
 #include <mutex>
 #include <string>
 
 #include "../../include/gsolve.h"
-#include "device_common.hpp"
+#include "families.hpp"
+#include "mid_solver.hpp"
 
 namespace gs {
 
@@ -104,129 +106,241 @@ struct StartArgs {
 
 constexpr int kStartWaves = 8;
 constexpr int kActStride = kHid + 8;  // BF16 row stride of a wave's activation tile (breaks bank conflicts)
-constexpr size_t kStartLds = size_t(kNPacked) * 2 + size_t(3 * kHid + kOut) * 4 + size_t(kStartWaves) * 16 * kActStride * 2 +
-                             size_t(kStartWaves) * 16 * 4 * 4;
+// LDS of a workgroup: packed weights, biases, then per wave an activation tile, the output rows of a
+// tile, and (fused solve) the corrections of its 64 systems.
+constexpr size_t kLdsWeights = size_t(kNPacked) * 2;
+constexpr size_t kLdsBias = size_t(3 * kHid + kOut) * 4;
+constexpr size_t kLdsAct = size_t(16) * kActStride * 2;
+constexpr size_t kLdsOut = size_t(16) * 4 * 4;
+constexpr size_t kLdsCorr = size_t(64) * 4 * 4;
+constexpr size_t kLdsPerWave = kLdsAct + kLdsOut + kLdsCorr;
+constexpr size_t kStartLds = kLdsWeights + kLdsBias + size_t(kStartWaves) * kLdsPerWave;
+
+// The workgroup's view of its LDS.
+struct MlpLds {
+    const __bf16* w;
+    const float* bias;
+    __bf16* act;   // this wave's activation tile
+    float* outb;   // this wave's 16 × 4 outputs
+    float* corr;   // this wave's 64 × 4 corrections
+};
+
+// Load the packed weights and biases into LDS (the whole workgroup) and return this wave's view.
+GS_DEV MlpLds mlp_lds_setup(unsigned char* lds, const __bf16* packed, const float* params) {
+    __bf16* w = reinterpret_cast<__bf16*>(lds);
+    float* bias = reinterpret_cast<float*>(lds + kLdsWeights);
+    const uint4* src = reinterpret_cast<const uint4*>(packed);
+    uint4* dst = reinterpret_cast<uint4*>(w);
+    for (int i = threadIdx.x; i < kNPacked / 8; i += blockDim.x) dst[i] = src[i];
+    for (int i = threadIdx.x; i < kHid; i += blockDim.x) {
+        bias[i] = params[kB1 + i];
+        bias[kHid + i] = params[kB2 + i];
+        bias[2 * kHid + i] = params[kB3 + i];
+    }
+    for (int i = threadIdx.x; i < kOut; i += blockDim.x) bias[3 * kHid + i] = params[kB4 + i];
+    __syncthreads();
+    const int wave = threadIdx.x >> 6;
+    unsigned char* base = lds + kLdsWeights + kLdsBias + size_t(wave) * kLdsPerWave;
+    MlpLds v;
+    v.w = w;
+    v.bias = bias;
+    v.act = reinterpret_cast<__bf16*>(base);
+    v.outb = reinterpret_cast<float*>(base + kLdsAct);
+    v.corr = reinterpret_cast<float*>(base + kLdsAct + kLdsOut);
+    return v;
+}
+
+GS_DEV void wave_lds_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The network on one tile of 16 systems (one 16 × 16 × 32 MFMA per k-step and column block): lanes
+// 0..15 pass the state x of system r = lane (the other lanes' x is not read); the outputs o_j + b4_j
+// of row r land in L.outb[r·4 + j].
+GS_DEV void mlp_tile(const MlpLds& L, const MlpNorm& nm, double h, const double (&x)[4]) {
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    bf16x8 av;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) av[j] = __bf16(0.0f);
+    if (g == 0) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) av[j] = __bf16((float(x[j]) - nm.in_shift[j]) * nm.in_scale[j]);
+        av[4] = __bf16((float(h) - nm.in_shift[4]) * nm.in_scale[4]);
+    }
+    f32x4 acc[8];
+#pragma unroll
+    for (int cb = 0; cb < 8; ++cb) {
+        acc[cb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        const bf16x8 bv = *reinterpret_cast<const bf16x8*>(L.w + kP1 + (cb * 64 + lane) * 8);
+        acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[cb], 0, 0, 0);
+    }
+    auto store_act = [&](const float* b) {
+#pragma unroll
+        for (int cb = 0; cb < 8; ++cb) {
+            const int col = cb * 16 + r;
+            const float bc = b[col];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) L.act[(4 * g + i) * kActStride + col] = __bf16(relu(acc[cb][i] + bc));
+        }
+        wave_lds_fence();
+    };
+    auto hidden = [&](int poff) {
+#pragma unroll
+        for (int cb = 0; cb < 8; ++cb) acc[cb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ks = 0; ks < kHid / 32; ++ks) {
+            const bf16x8 av2 = *reinterpret_cast<const bf16x8*>(L.act + r * kActStride + ks * 32 + 8 * g);
+#pragma unroll
+            for (int cb = 0; cb < 8; ++cb) {
+                const bf16x8 bv = *reinterpret_cast<const bf16x8*>(L.w + poff + ((ks * 8 + cb) * 64 + lane) * 8);
+                acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av2, bv, acc[cb], 0, 0, 0);
+            }
+        }
+        wave_lds_fence();
+    };
+    store_act(L.bias);
+    hidden(kP2);
+    store_act(L.bias + kHid);
+    hidden(kP3);
+    store_act(L.bias + 2 * kHid);
+    f32x4 o = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int ks = 0; ks < kHid / 32; ++ks) {
+        const bf16x8 av2 = *reinterpret_cast<const bf16x8*>(L.act + r * kActStride + ks * 32 + 8 * g);
+        const bf16x8 bv = *reinterpret_cast<const bf16x8*>(L.w + kP4 + (ks * 64 + lane) * 8);
+        o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av2, bv, o, 0, 0, 0);
+    }
+    if (r < 4) {
+        const float bc = L.bias[3 * kHid + r];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) L.outb[(4 * g + i) * 4 + r] = o[i] + bc;
+    }
+    wave_lds_fence();
+}
+
+GS_DEV void load_state(const double* X, int64_t s, double (&x)[4]) {
+    const double2* xp = reinterpret_cast<const double2*>(X + 4 * s);
+    const double2 u = xp[0], v = xp[1];
+    x[0] = u.x;
+    x[1] = u.y;
+    x[2] = v.x;
+    x[3] = v.y;
+}
+
+// x̂ = x̃ + h³ (o σ + μ) from a row of outputs.
+GS_DEV void predicted_start(const double (&x)[4], double h, const float* o, const MlpNorm& nm, double (&y)[4]) {
+    double xt[4];
+    hh_rk2(x, h, xt);
+    const double h3 = (h * h) * h;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const double corr = double(o[j] * nm.out_scale[j] + nm.out_shift[j]);
+        y[j] = xt[j] + h3 * corr;
+    }
+}
 
 // One workgroup of 8 waves per CU holds the packed weights and biases in LDS; each wave runs tiles of
-// 16 systems through the four layers (one 16 × 16 × 32 MFMA per k-step and column block), keeping its
-// tile's activations in a private LDS buffer between layers.
+// 16 systems through the four layers, keeping its tile's activations in a private LDS buffer.
 __global__ __launch_bounds__(64 * kStartWaves) void mlp_start_kernel(const StartArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
-    __bf16* w = reinterpret_cast<__bf16*>(lds);
-    float* bias = reinterpret_cast<float*>(lds + size_t(kNPacked) * 2);
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    __bf16* act = reinterpret_cast<__bf16*>(lds + size_t(kNPacked) * 2 + size_t(3 * kHid + kOut) * 4) +
-                  size_t(wave) * 16 * kActStride;
-    float* outb = reinterpret_cast<float*>(lds + size_t(kNPacked) * 2 + size_t(3 * kHid + kOut) * 4 +
-                                           size_t(kStartWaves) * 16 * kActStride * 2) +
-                  size_t(wave) * 16 * 4;
-    {
-        const uint4* src = reinterpret_cast<const uint4*>(a.packed);
-        uint4* dst = reinterpret_cast<uint4*>(w);
-        for (int i = threadIdx.x; i < kNPacked / 8; i += blockDim.x) dst[i] = src[i];
-        for (int i = threadIdx.x; i < kHid; i += blockDim.x) {
-            bias[i] = a.params[kB1 + i];
-            bias[kHid + i] = a.params[kB2 + i];
-            bias[2 * kHid + i] = a.params[kB3 + i];
-        }
-        for (int i = threadIdx.x; i < kOut; i += blockDim.x) bias[3 * kHid + i] = a.params[kB4 + i];
-    }
-    __syncthreads();
-    const int r = lane & 15, g = lane >> 4;
+    const MlpLds L = mlp_lds_setup(lds, a.packed, a.params);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
     const int64_t ntile = (a.nsys + 15) / 16;
     for (int64_t t = int64_t(blockIdx.x) * kStartWaves + wave; t < ntile; t += int64_t(gridDim.x) * kStartWaves) {
         const int64_t s = t * 16 + r;
         const bool live = s < a.nsys;
         double x[4] = {0.0, 0.0, 0.0, 0.0};
-        if (live) {
-            const double2* xp = reinterpret_cast<const double2*>(a.Xk + 4 * s);
-            const double2 u = xp[0], v = xp[1];
-            x[0] = u.x;
-            x[1] = u.y;
-            x[2] = v.x;
-            x[3] = v.y;
-        }
-        // layer 1: A = the features (lanes 0..15 hold row r, columns 0..7; the other lanes zeros)
-        bf16x8 av;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) av[j] = __bf16(0.0f);
-        if (g == 0) {
-#pragma unroll
-            for (int j = 0; j < 4; ++j) av[j] = __bf16((float(x[j]) - a.nm.in_shift[j]) * a.nm.in_scale[j]);
-            av[4] = __bf16((float(a.h) - a.nm.in_shift[4]) * a.nm.in_scale[4]);
-        }
-        f32x4 acc[8];
-#pragma unroll
-        for (int cb = 0; cb < 8; ++cb) {
-            acc[cb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-            const bf16x8 bv = *reinterpret_cast<const bf16x8*>(w + kP1 + (cb * 64 + lane) * 8);
-            acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[cb], 0, 0, 0);
-        }
-        // hidden layers 2 and 3 read the previous layer's activations from the wave's tile
-        auto store_act = [&](const float* b) {
-#pragma unroll
-            for (int cb = 0; cb < 8; ++cb) {
-                const int col = cb * 16 + r;
-                const float bc = b[col];
-#pragma unroll
-                for (int i = 0; i < 4; ++i) act[(4 * g + i) * kActStride + col] = __bf16(relu(acc[cb][i] + bc));
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        };
-        auto hidden = [&](int poff) {
-#pragma unroll
-            for (int cb = 0; cb < 8; ++cb) acc[cb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int ks = 0; ks < kHid / 32; ++ks) {
-                const bf16x8 av2 = *reinterpret_cast<const bf16x8*>(act + r * kActStride + ks * 32 + 8 * g);
-#pragma unroll
-                for (int cb = 0; cb < 8; ++cb) {
-                    const bf16x8 bv = *reinterpret_cast<const bf16x8*>(w + poff + ((ks * 8 + cb) * 64 + lane) * 8);
-                    acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av2, bv, acc[cb], 0, 0, 0);
-                }
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        };
-        store_act(bias);
-        hidden(kP2);
-        store_act(bias + kHid);
-        hidden(kP3);
-        store_act(bias + 2 * kHid);
-        // output layer (one column block)
-        f32x4 o = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int ks = 0; ks < kHid / 32; ++ks) {
-            const bf16x8 av2 = *reinterpret_cast<const bf16x8*>(act + r * kActStride + ks * 32 + 8 * g);
-            const bf16x8 bv = *reinterpret_cast<const bf16x8*>(w + kP4 + (ks * 64 + lane) * 8);
-            o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av2, bv, o, 0, 0, 0);
-        }
-        if (r < 4) {
-            const float bc = bias[3 * kHid + r];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) outb[(4 * g + i) * 4 + r] = o[i] + bc;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (live) load_state(a.Xk, s, x);
+        mlp_tile(L, a.nm, a.h, x);
         if (g == 0 && live) {
-            double xt[4];
-            hh_rk2(x, a.h, xt);
-            const double h3 = (a.h * a.h) * a.h;
             double y[4];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const double corr = double(outb[r * 4 + j] * a.nm.out_scale[j] + a.nm.out_shift[j]);
-                y[j] = xt[j] + h3 * corr;
-            }
+            predicted_start(x, a.h, L.outb + r * 4, a.nm, y);
             double2* yp = reinterpret_cast<double2*>(a.X0 + 4 * s);
             yp[0] = double2{y[0], y[1]};
             yp[1] = double2{y[2], y[3]};
         }
         __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// ---- the fused solve: learned start, Newton from it, plain Newton where that fails ---------------
+
+struct LearnedArgs {
+    StartArgs st;
+    DevParams q;    // the method of the solve from the learned start
+    DevParams qf;   // the fallback: plain Newton from x_k
+    gs_status_f64* S;
+    uint32_t* fallbacks;
+};
+
+GS_DEV void store_status_learned(gs_status_f64* dst, uint8_t code, int32_t iterations, double fnorm, double stepnorm,
+                                 int32_t failures, bool promoted, double gradratio, double rate, bool contracted) {
+    uint64_t* w = reinterpret_cast<uint64_t*>(dst);
+    w[0] = uint64_t(code) | (uint64_t(uint32_t(iterations)) << 32);
+    w[1] = __builtin_bit_cast(uint64_t, fnorm);
+    w[2] = __builtin_bit_cast(uint64_t, stepnorm);
+    w[3] = uint64_t(uint32_t(failures)) | (uint64_t(promoted ? 1 : 0) << 32);
+    w[4] = uint64_t(uint32_t(-1));
+    w[5] = __builtin_bit_cast(uint64_t, dnan());
+    w[6] = __builtin_bit_cast(uint64_t, gradratio);
+    w[7] = __builtin_bit_cast(uint64_t, rate);
+    w[8] = uint64_t(contracted ? 1 : 0);
+}
+
+// Each wave takes 64 systems at a time: four 16-system tiles through the network (corrections into
+// LDS), then one lane per system runs the method from its predicted start; a lane whose solve does
+// not end in SUCCESS solves again from x_k with plain Newton, and that status is the system's.
+template <bool MIXED, bool REFINE>
+__global__ __launch_bounds__(64 * kStartWaves) void learned_solve_kernel(const LearnedArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const MlpLds L = mlp_lds_setup(lds, a.st.packed, a.st.params);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    const int64_t nchunk = (a.st.nsys + 63) / 64;
+    FamArgs fa;
+    fa.P = reinterpret_cast<const char*>(a.st.Xk);
+    fa.p_stride = 32;
+    fa.shared[0] = a.st.h;
+    for (int k = 1; k < 8; ++k) fa.shared[k] = 0.0;
+    for (int64_t c = int64_t(blockIdx.x) * kStartWaves + wave; c < nchunk; c += int64_t(gridDim.x) * kStartWaves) {
+        for (int t = 0; t < 4; ++t) {
+            const int64_t s = c * 64 + t * 16 + r;
+            double x[4] = {0.0, 0.0, 0.0, 0.0};
+            if (s < a.st.nsys) load_state(a.st.Xk, s, x);
+            mlp_tile(L, a.st.nm, a.st.h, x);
+            if (g == 0) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) L.corr[(t * 16 + r) * 4 + j] = L.outb[r * 4 + j];
+            }
+            wave_lds_fence();
+        }
+        const int64_t s = c * 64 + lane;
+        if (s < a.st.nsys) {
+            double xk[4], x[4];
+            load_state(a.st.Xk, s, xk);
+            predicted_start(xk, a.st.h, L.corr + lane * 4, a.st.nm, x);
+            FamHenonHeilesMidpoint fam;
+            fam.load(fa, s);
+            SmallSolve<FamHenonHeilesMidpoint, kStatic, MIXED, REFINE, true> sol(fam, a.q);
+            const auto R = sol.solve(x);
+            double2* xp = reinterpret_cast<double2*>(a.st.X0 + 4 * s);
+            if (R.code == kSuccess) {
+                store_status_learned(a.S + s, R.code, R.iterations, R.fnorm, R.stepnorm, R.failures, R.promoted,
+                                     R.gradratio, R.rate, R.contracted);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) x[j] = xk[j];
+                SmallSolve<FamHenonHeilesMidpoint, kStatic, false, false, true> fb(fam, a.qf);
+                const auto Rf = fb.solve(x);
+                atomicAdd(a.fallbacks, 1u);
+                store_status_learned(a.S + s, Rf.code, Rf.iterations, Rf.fnorm, Rf.stepnorm, Rf.failures, Rf.promoted,
+                                     Rf.gradratio, Rf.rate, Rf.contracted);
+            }
+            xp[0] = double2{x[0], x[1]};
+            xp[1] = double2{x[2], x[3]};
+        }
+        wave_lds_fence();
     }
 }
 
@@ -411,11 +525,13 @@ struct MlpWorkspace {
 std::mutex g_mlp_mutex;
 MlpWorkspace g_mlp_ws[64];
 
-int mlp_workspace(size_t bytes, void** out) {
+MlpWorkspace g_mlp_counter[64];  // the fused solve's fallback counter (its own, so training can overlap)
+
+int mlp_workspace(size_t bytes, void** out, bool counter = false) {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return set_error(GS_EHIP, "hipGetDevice failed");
     std::lock_guard<std::mutex> lock(g_mlp_mutex);
-    MlpWorkspace& w = g_mlp_ws[dev];
+    MlpWorkspace& w = counter ? g_mlp_counter[dev] : g_mlp_ws[dev];
     if (w.bytes < bytes) {
         if (w.ptr) (void)hipFree(w.ptr);
         w.ptr = nullptr;
@@ -472,6 +588,88 @@ int gs_mlp_start_henon_heiles(const gs_mlp* net, int64_t nsys, const double* Xk,
     if (blocks > cus) blocks = cus;
     hipLaunchKernelGGL(mlp_start_kernel, dim3(unsigned(blocks)), dim3(64 * kStartWaves), kStartLds, st, a);
     return check(hipGetLastError(), "learned-start launch failed");
+}
+
+int gs_solve_learned_henon_heiles_f64(const gs_mlp* net, const gs_method* method, int64_t nsys, const double* Xk,
+                                      double h, double* X, gs_status_f64* S, const gs_options_f64* options,
+                                      int64_t* fallbacks, void* stream) {
+    if (!net || !net->params || !net->packed || !method || !options)
+        return set_error(GS_EINVAL, "net, method and options must not be NULL");
+    if (nsys < 0) return set_error(GS_EINVAL, "nsys must be ≥ 0");
+    if (method->linsolve != GS_LU || method->globalization != GS_STATIC)
+        return set_error(GS_EFAMILY, "the fused learned solve takes Newton with LU and the Static globalization");
+    if (method->factor_precision != GS_FACTOR_WORKING && method->factor_precision != GS_FACTOR_F32)
+        return set_error(GS_EINVAL, "factor_precision must be GS_FACTOR_WORKING or GS_FACTOR_F32");
+    if (fallbacks) *fallbacks = 0;
+    if (nsys == 0) return GS_OK;
+    if (!Xk || !X || !S) return set_error(GS_EINVAL, "Xk, X and S must not be NULL");
+    if ((reinterpret_cast<uintptr_t>(Xk) % 16) || (reinterpret_cast<uintptr_t>(X) % 16) ||
+        (reinterpret_cast<uintptr_t>(S) % 8) || (reinterpret_cast<uintptr_t>(net->packed) % 16))
+        return set_error(GS_EINVAL, "Xk, X and the packed weights must be 16-byte aligned, S 8-byte aligned");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return set_error(GS_EHIP, "hipGetDevice failed");
+    const int v = method->factor_precision == GS_FACTOR_F32 ? 1 : 0;
+    const void* kern = v ? (method->refinement == GS_REFINEMENT ? reinterpret_cast<const void*>(&learned_solve_kernel<true, true>)
+                                                                 : reinterpret_cast<const void*>(&learned_solve_kernel<true, false>))
+                         : (method->refinement == GS_REFINEMENT ? reinterpret_cast<const void*>(&learned_solve_kernel<false, true>)
+                                                                 : reinterpret_cast<const void*>(&learned_solve_kernel<false, false>));
+    // one attribute per kernel variant
+    static bool attr_kern[64][4] = {};
+    const int ki = v * 2 + (method->refinement == GS_REFINEMENT ? 1 : 0);
+    if (!attr_kern[dev][ki]) {
+        int rc = check(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(kStartLds)),
+                       "hipFuncSetAttribute");
+        if (rc) return rc;
+        attr_kern[dev][ki] = true;
+    }
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    void* ws = nullptr;
+    int rc = mlp_workspace(256, &ws, true);
+    if (rc) return rc;
+    uint32_t* counter = static_cast<uint32_t*>(ws);
+    hipLaunchKernelGGL(fill_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<float*>(counter), int64_t(1), 0.0f);
+    LearnedArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.st.Xk = Xk;
+    a.st.X0 = X;
+    a.st.nsys = nsys;
+    a.st.h = h;
+    a.st.packed = reinterpret_cast<const __bf16*>(net->packed);
+    a.st.params = net->params;
+    a.st.nm = norm_of(*net);
+    a.q = dev_params(*method, *options);
+    gs_method plain;
+    std::memset(&plain, 0, sizeof(plain));
+    plain.linsolve = GS_LU;
+    plain.factor_precision = GS_FACTOR_WORKING;
+    plain.globalization = GS_STATIC;
+    plain.refinement = GS_NO_REFINEMENT;
+    plain.static_alpha = 1.0;
+    a.qf = dev_params(plain, *options);
+    a.S = S;
+    a.fallbacks = counter;
+    const int64_t nchunk = (nsys + 63) / 64;
+    int64_t blocks = (nchunk + kStartWaves - 1) / kStartWaves;
+    if (blocks > cus) blocks = cus;
+    const dim3 grid{unsigned(blocks), 1, 1}, block{64 * kStartWaves, 1, 1};
+    const bool refine = method->refinement == GS_REFINEMENT;
+    bool timed = timing_begin(st);
+    if (v) {
+        if (refine) hipLaunchKernelGGL((learned_solve_kernel<true, true>), grid, block, kStartLds, st, a);
+        else hipLaunchKernelGGL((learned_solve_kernel<true, false>), grid, block, kStartLds, st, a);
+    } else {
+        if (refine) hipLaunchKernelGGL((learned_solve_kernel<false, true>), grid, block, kStartLds, st, a);
+        else hipLaunchKernelGGL((learned_solve_kernel<false, false>), grid, block, kStartLds, st, a);
+    }
+    timed = timing_mark(st, timed);
+    if ((rc = check(hipGetLastError(), "fused learned solve launch failed"))) return rc;
+    uint32_t host = 0;
+    if ((rc = check(hipMemcpyAsync(&host, counter, 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync"))) return rc;
+    if ((rc = check(hipStreamSynchronize(st), "hipStreamSynchronize"))) return rc;
+    timing_end(timed);
+    if (fallbacks) *fallbacks = int64_t(host);
+    return GS_OK;
 }
 
 int gs_mlp_grad(const gs_mlp* net, const double* Xk, const double* X1, double h, const int64_t* idx, int32_t batch,

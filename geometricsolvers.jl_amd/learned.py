@@ -204,25 +204,41 @@ def generate_solutions(gs, Xk, h: float, options=None):
     return X, S
 
 
-def solve_batch_learned(gs, X, S, prob, method, p, net: LearnedStart, options=None, fallback=None):
+def solve_batch_learned(gs, X, S, prob, method, p, net: LearnedStart, options=None, fallback=None, fused: bool = True):
     """solve_batch from the learned start: X is overwritten with the solutions, S with the statuses.
     Every system whose solve from the learned start does not return SUCCESS is solved again from the
     default start x₀ = x_k with `fallback` (plain Newton by default), and its status is that solve's.
-    Returns the number of systems that fell back."""
+    Returns the number of systems that fell back.
+
+    With `fused` (the default, for plain Newton as the fallback) one kernel does all three parts per
+    system — the start, the solve from it and the fallback — and leaves the start in registers;
+    otherwise the start kernel, the batched solve and the fallback solve run one after the other."""
     torch = _torch()
     if not isinstance(prob.F, gs.HenonHeilesMidpoint) or not isinstance(p, gs.BatchParams):
         raise gs.ArgumentError("the learned start is trained for HenonHeilesMidpoint with BatchParams(h, X_k)")
     h = float(p.shared)
     Xk = p.per_system
+    opts = gs.resolve_options(options, {})
+    if fused and fallback is None and isinstance(method.globalization, gs.Static):
+        lib = _lib()
+        cm = method.to_c()
+        co = opts.to_c()
+        net_c = net.to_c()
+        nb = ctypes.c_int64(0)
+        rc = lib.gs_solve_learned_henon_heiles_f64(
+            ctypes.byref(net_c), ctypes.byref(cm), Xk.shape[0], ctypes.c_void_p(Xk.data_ptr()), h,
+            ctypes.c_void_p(X.data_ptr()), ctypes.c_void_p(S.data_ptr()), ctypes.byref(co), ctypes.byref(nb), _stream(X))
+        _raise_on(rc, lib)
+        return int(nb.value)
     net.start(Xk, h, out=X)
-    gs.solve_batch(X, S, prob, method, p, options)
+    gs.solve_batch(X, S, prob, method, p, opts)
     bad = torch.nonzero(S[:, 0] != 0).flatten()
     nb = int(bad.numel())
     if nb:
         fb = fallback if fallback is not None else gs.Newton()
         Xb = Xk.index_select(0, bad).contiguous()
         Sb = gs.status_buffer(nb, X.device)
-        gs.solve_batch(Xb, Sb, prob, fb, gs.BatchParams(h, Xb.clone()), options)
+        gs.solve_batch(Xb, Sb, prob, fb, gs.BatchParams(h, Xb.clone()), opts)
         X.index_copy_(0, bad, Xb)
         S.index_copy_(0, bad, Sb)
     return nb

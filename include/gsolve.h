@@ -223,6 +223,16 @@ typedef struct gs_mlp {
 int gs_mlp_start_henon_heiles(const gs_mlp* net, int64_t nsys, const double* Xk, double h, double* X0,
                               void* stream);
 
+/* The fused learned solve: for each system, the start x̂₊ (as gs_mlp_start_henon_heiles), `method`
+ * (Newton with LU, Static, FP64 or FP32 factors, with or without Refinement) from x̂₊, and plain
+ * Newton from x₀ = x_k for every system whose solve from x̂₊ does not return SUCCESS; X and S receive
+ * the final iterate and status of each system (the fallback's where it ran), *fallbacks the number of
+ * systems that fell back. Synchronous. The reference has no learned start; per system this is its R1
+ * solve!! (solver.jl:277-378) twice at most. */
+int gs_solve_learned_henon_heiles_f64(const gs_mlp* net, const gs_method* method, int64_t nsys, const double* Xk,
+                                      double h, double* X, gs_status_f64* S, const gs_options_f64* options,
+                                      int64_t* fallbacks, void* stream);
+
 /* Loss ½ mean_b,j<4 (o_j − t_j)² of a minibatch (systems idx[0..batch−1] of the training pairs
  * (Xk, X1), t the normalised correction of X1) into *loss (device float), and its gradient with respect
  * to every parameter into grad (GS_MLP_NPARAM device floats). batch is a multiple of 32. Asynchronous. */
@@ -237,9 +247,10 @@ int gs_mlp_adam(float* params, uint16_t* packed, const float* grad, float* m, fl
 /* The packed BF16 weights of params. Asynchronous. */
 int gs_mlp_pack(const float* params, uint16_t* packed, void* stream);
 
-/* Instrumentation, with no counterpart in the reference: when enabled, gs_solve_batch_f64 brackets
- * its kernel with two HIP events on `stream`, and gs_last_kernel_ms returns the elapsed time of the
- * last solve kernel of this thread (ms), or a negative value if none was timed. */
+/* Instrumentation, with no counterpart in the reference: when enabled, gs_solve_batch_f64 and
+ * gs_solve_learned_henon_heiles_f64 bracket their kernel with two HIP events on `stream`, and
+ * gs_last_kernel_ms returns the elapsed time of the last such kernel of this thread (ms), or a
+ * negative value if none was timed. */
 void gs_set_kernel_timing(int enable);
 double gs_last_kernel_ms(void);
 

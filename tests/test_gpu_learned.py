@@ -130,3 +130,32 @@ def test_training_gains_over_plain_and_falls_back(gs, workloads, device):
     assert nb == len(xt)
     assert np.array_equal(Sb.cpu().numpy(), Sp.cpu().numpy())
     assert torch.equal(Xb, Xp)
+
+
+def test_fused_solve_matches_the_three_step_path(gs, workloads, device):
+    """The fused kernel (start, solve from it, fallback in one launch) gives the same statuses and
+    iterates, bit for bit, as the start kernel followed by the batched solve and the fallback solve —
+    for the plain and the mixed method, with a trained network and with a NaN network (every system
+    falls back)."""
+    Xk, X1 = _pairs(gs, workloads, device, 1 << 15, seed=41)
+    net = gs.LearnedStart(device, seed=0)
+    net.fit_normalization(Xk, X1, H)
+    net.train(Xk, X1, H, epochs=4, batch=2048, lr=3e-3, lr_final=3e-5, seed=0)
+    xt = torch.as_tensor(workloads.henon_heiles_states(20000 + 3, seed=42), device=device)
+    prob = gs.NonlinearProblem(gs.HenonHeilesMidpoint())
+    p = gs.BatchParams(H, xt)
+    opts = gs.Options(**TOL)
+    mixed = gs.Newton(linsolve=gs.LUFactorization(np.float32), refinement=gs.Refinement())
+    bad = gs.LearnedStart(device, seed=0)
+    bad.norm = dict(net.norm)
+    bad.set_parameters(np.full(M.NPARAM, np.nan, dtype=np.float32))
+    for n in (net, bad):
+        for method in (gs.Newton(), mixed):
+            Xa, Sa = torch.empty_like(xt), gs.status_buffer(len(xt), device)
+            Xb, Sb = torch.empty_like(xt), gs.status_buffer(len(xt), device)
+            na = gs.solve_batch_learned(gs, Xa, Sa, prob, method, p, n, opts, fused=True)
+            nb = gs.solve_batch_learned(gs, Xb, Sb, prob, method, p, n, opts, fused=False)
+            assert na == nb
+            assert torch.equal(Sa, Sb)
+            assert torch.equal(Xa.view(torch.int64), Xb.view(torch.int64))
+        assert (gs.statuses(Sa)["code"] == 0).all()
