@@ -11,8 +11,8 @@
 // reference.
 //
 // Parameters live in one FP32 array (W1, b1, W2, b2, W3, b3, W4, b4; W_l row-major in × out), and
-// the inference kernel reads a BF16 copy of the weights packed in the MFMA B-operand order, which
-// the Adam kernel writes after each update.
+// the inference kernel reads a BF16 copy of the weights packed as the MFMA A operands of the
+// transposed products Wᵀ aᵀ, which the Adam kernel writes after each update.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -30,7 +30,11 @@ int set_error(int code, const std::string& msg);
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef short s16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kIn = GS_MLP_IN, kHid = GS_MLP_HIDDEN, kOut = GS_MLP_OUT;
 
@@ -43,18 +47,33 @@ constexpr int kP1 = 0, kP2 = kP1 + kIn * kHid, kP3 = kP2 + kHid * kHid, kP4 = kP
               kNPacked = kP4 + kHid * kOut;
 static_assert(kNPacked == GS_MLP_NPACKED, "packed weight count");
 
-// The MFMA operand layouts of v_mfma_f32_16x16x32_bf16 (64 lanes):
-//   A (16 × 32): lane l holds row l % 16, columns 8 (l / 16) … 8 (l / 16) + 7;
-//   B (32 × 16): lane l holds column l % 16, rows 8 (l / 16) … 8 (l / 16) + 7;
-//   D (16 × 16): lane l holds column l % 16, rows 4 (l / 16) … 4 (l / 16) + 3.
-// Packed weights: the B operands of layer (in × out) in the order (k-step, column block, lane, 8).
+// The MFMA operand layouts of v_mfma_f32_16x16x32_bf16 (64 lanes, g = l / 16):
+//   A (16 × 32): lane l holds row l % 16, columns 8 g … 8 g + 7;
+//   B (32 × 16): lane l holds column l % 16, rows 8 g … 8 g + 7;
+//   D (16 × 16): lane l holds column l % 16, rows 4 g … 4 g + 3.
+// Inference computes each layer transposed, zᵀ = Wᵀ aᵀ over 16 systems at a time: A is a 16-unit
+// block of Wᵀ, B the previous activations (columns = systems) and D the new ones. D's rows 4 g … 4 g + 3
+// of unit blocks 2 s and 2 s + 1 are then exactly the eight B rows of lane group g in k-step s, once the
+// k index of every k-step is permuted: slot j of lane group g stands for unit
+//     32 s + 4 g + j (j < 4),   32 s + 16 + 4 g + (j − 4) (j ≥ 4),
+// so that activations pass from layer to layer in registers. The packed weights are the A operands
+// under that permutation, in the order (k-step, unit block, lane, slot).
 GS_DEV int packed_index(int k, int col, int out) {
-    const int ks = k >> 5, kk = k & 31, cb = col >> 4, c = col & 15;
-    const int lane = c + 16 * (kk >> 3);
-    return ((ks * (out >> 4) + cb) * 64 + lane) * 8 + (kk & 7);
+    const int ks = k >> 5, kk = k & 31, ob = col >> 4;
+    const int g = (kk & 15) >> 2, slot = (kk & 3) | ((kk & 16) >> 2);
+    const int lane = (col & 15) + 16 * g;
+    return ((ks * (out >> 4) + ob) * 64 + lane) * 8 + slot;
 }
 
 GS_DEV float relu(float z) { return z > 0.0f ? z : 0.0f; }
+
+// ReLU of two FP32 values into a packed BF16 pair: round each to BF16, then clear every value whose
+// sign bit is set, as an int16 maximum with 0 (one packed integer instruction for the pair).
+GS_DEV uint32_t relu_bf16x2(float a, float b) {
+    const bf16x2 p = __builtin_convertvector(f32x2{a, b}, bf16x2);
+    const s16x2 v = __builtin_elementwise_max(__builtin_bit_cast(s16x2, p), s16x2{0, 0});
+    return __builtin_bit_cast(uint32_t, v);
+}
 
 // Hénon–Heiles vector field f(q₁, q₂, p₁, p₂) = (p₁, p₂, −q₁ − 2 q₁ q₂, −q₂ − q₁² + q₂²).
 GS_DEV void hh_field(const double (&x)[4], double (&f)[4]) {
@@ -105,24 +124,20 @@ struct StartArgs {
 };
 
 constexpr int kStartWaves = 8;
-constexpr int kActStride = kHid + 8;  // BF16 row stride of a wave's activation tile (breaks bank conflicts)
-// LDS of a workgroup: packed weights, biases, then per wave an activation tile, the output rows of a
-// tile, and (fused solve) the corrections of its 64 systems.
+constexpr int kTiles = 4;  // 16-system tiles per wave pass: each weight fragment read feeds kTiles MFMAs
+constexpr int kChunk = 16 * kTiles;
+static_assert(kChunk == 64, "one lane per system of a chunk");
+// LDS of a workgroup: packed weights, biases, then per wave the corrections of its kChunk systems.
 constexpr size_t kLdsWeights = size_t(kNPacked) * 2;
 constexpr size_t kLdsBias = size_t(3 * kHid + kOut) * 4;
-constexpr size_t kLdsAct = size_t(16) * kActStride * 2;
-constexpr size_t kLdsOut = size_t(16) * 4 * 4;
-constexpr size_t kLdsCorr = size_t(64) * 4 * 4;
-constexpr size_t kLdsPerWave = kLdsAct + kLdsOut + kLdsCorr;
-constexpr size_t kStartLds = kLdsWeights + kLdsBias + size_t(kStartWaves) * kLdsPerWave;
+constexpr size_t kLdsCorr = size_t(kChunk) * 4 * 4;
+constexpr size_t kStartLds = kLdsWeights + kLdsBias + size_t(kStartWaves) * kLdsCorr;
 
 // The workgroup's view of its LDS.
 struct MlpLds {
     const __bf16* w;
     const float* bias;
-    __bf16* act;   // this wave's activation tile
-    float* outb;   // this wave's 16 × 4 outputs
-    float* corr;   // this wave's 64 × 4 corrections
+    float* corr;   // this wave's kChunk × 4 outputs
 };
 
 // Load the packed weights and biases into LDS (the whole workgroup) and return this wave's view.
@@ -140,13 +155,10 @@ GS_DEV MlpLds mlp_lds_setup(unsigned char* lds, const __bf16* packed, const floa
     for (int i = threadIdx.x; i < kOut; i += blockDim.x) bias[3 * kHid + i] = params[kB4 + i];
     __syncthreads();
     const int wave = threadIdx.x >> 6;
-    unsigned char* base = lds + kLdsWeights + kLdsBias + size_t(wave) * kLdsPerWave;
     MlpLds v;
     v.w = w;
     v.bias = bias;
-    v.act = reinterpret_cast<__bf16*>(base);
-    v.outb = reinterpret_cast<float*>(base + kLdsAct);
-    v.corr = reinterpret_cast<float*>(base + kLdsAct + kLdsOut);
+    v.corr = reinterpret_cast<float*>(lds + kLdsWeights + kLdsBias + size_t(wave) * kLdsCorr);
     return v;
 }
 
@@ -156,66 +168,95 @@ GS_DEV void wave_lds_fence() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// The network on one tile of 16 systems (one 16 × 16 × 32 MFMA per k-step and column block): lanes
-// 0..15 pass the state x of system r = lane (the other lanes' x is not read); the outputs o_j + b4_j
-// of row r land in L.outb[r·4 + j].
-GS_DEV void mlp_tile(const MlpLds& L, const MlpNorm& nm, double h, const double (&x)[4]) {
+GS_DEV double lane_value(double v, int src) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const int lo = __shfl(int(uint32_t(u)), src), hi = __shfl(int(uint32_t(u >> 32)), src);
+    return __builtin_bit_cast(double, (uint64_t(uint32_t(hi)) << 32) | uint32_t(lo));
+}
+
+// The network on the kChunk systems of one wave pass. Lane l passes the state x of system l of the
+// chunk (any finite value past the batch); the outputs o + b4 of system l land in L.corr[4 l + j].
+// Tile t holds systems 16 t … 16 t + 15 as the B columns; every weight fragment is read from LDS once
+// and multiplies all kTiles tiles.
+GS_DEV void mlp_chunk(const MlpLds& L, const MlpNorm& nm, double h, const double (&x)[4]) {
     const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-    bf16x8 av;
+    constexpr int kKs = kHid / 32, kOb = kHid / 16;
+    // activations as packed BF16 pairs: nxt[t][s] is the B operand of tile t in k-step s
+    u32x4 act[kTiles][kKs], nxt[kTiles][kKs];
+    const float uh = (float(h) - nm.in_shift[4]) * nm.in_scale[4];
+    // layer 1 inputs: slots 0..3 of lane group 0 are the features of x, slot 0 of group 1 is h's
+    bf16x8 in[kTiles];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) av[j] = __bf16(0.0f);
+    for (int t = 0; t < kTiles; ++t) {
+        double xt[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) xt[j] = lane_value(x[j], 16 * t + r);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) in[t][j] = __bf16(0.0f);
+        if (g == 0) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) in[t][j] = __bf16((float(xt[j]) - nm.in_shift[j]) * nm.in_scale[j]);
+        } else if (g == 1) {
+            in[t][0] = __bf16(uh);
+        }
+    }
+    // The bias of unit block ob, rows 4 g … 4 g + 3: the C operand of the block's first MFMA.
+    auto bias_of = [&](const float* b, int ob) { return *reinterpret_cast<const f32x4*>(b + ob * 16 + 4 * g); };
+    // ReLU of unit block ob into the B operands of the next layer (blocks 2 s, 2 s + 1 → k-step s)
+    auto activate = [&](const f32x4 (&acc)[kTiles], int ob) {
+#pragma unroll
+        for (int t = 0; t < kTiles; ++t) {
+            nxt[t][ob >> 1][(ob & 1) * 2] = relu_bf16x2(acc[t][0], acc[t][1]);
+            nxt[t][ob >> 1][(ob & 1) * 2 + 1] = relu_bf16x2(acc[t][2], acc[t][3]);
+        }
+    };
+#pragma unroll
+    for (int ob = 0; ob < kOb; ++ob) {
+        const bf16x8 wv = *reinterpret_cast<const bf16x8*>(L.w + kP1 + (ob * 64 + lane) * 8);
+        const f32x4 bv = bias_of(L.bias, ob);
+        f32x4 acc[kTiles];
+#pragma unroll
+        for (int t = 0; t < kTiles; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wv, in[t], bv, 0, 0, 0);
+        activate(acc, ob);
+    }
+    // a hidden layer, one unit block at a time: only that block's accumulators are live
+    auto hidden = [&](int poff, const float* b) {
+#pragma unroll
+        for (int t = 0; t < kTiles; ++t) {
+#pragma unroll
+            for (int ks = 0; ks < kKs; ++ks) act[t][ks] = nxt[t][ks];
+        }
+#pragma unroll
+        for (int ob = 0; ob < kOb; ++ob) {
+            const f32x4 bv = bias_of(b, ob);
+            f32x4 acc[kTiles];
+#pragma unroll
+            for (int ks = 0; ks < kKs; ++ks) {
+                const bf16x8 wv = *reinterpret_cast<const bf16x8*>(L.w + poff + ((ks * kOb + ob) * 64 + lane) * 8);
+#pragma unroll
+                for (int t = 0; t < kTiles; ++t)
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wv, __builtin_bit_cast(bf16x8, act[t][ks]),
+                                                                      ks == 0 ? bv : acc[t], 0, 0, 0);
+            }
+            activate(acc, ob);
+        }
+    };
+    hidden(kP2, L.bias + kHid);
+    hidden(kP3, L.bias + 2 * kHid);
+    f32x4 o[kTiles];
+    const f32x4 b4 = bias_of(L.bias + 3 * kHid, 0);
+#pragma unroll
+    for (int ks = 0; ks < kKs; ++ks) {
+        const bf16x8 wv = *reinterpret_cast<const bf16x8*>(L.w + kP4 + (ks * 64 + lane) * 8);
+#pragma unroll
+        for (int t = 0; t < kTiles; ++t)
+            o[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wv, __builtin_bit_cast(bf16x8, nxt[t][ks]), ks == 0 ? b4 : o[t],
+                                                           0, 0, 0);
+    }
+    // outputs 0..3 are rows 0..3 of D: lane group 0
     if (g == 0) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) av[j] = __bf16((float(x[j]) - nm.in_shift[j]) * nm.in_scale[j]);
-        av[4] = __bf16((float(h) - nm.in_shift[4]) * nm.in_scale[4]);
-    }
-    f32x4 acc[8];
-#pragma unroll
-    for (int cb = 0; cb < 8; ++cb) {
-        acc[cb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-        const bf16x8 bv = *reinterpret_cast<const bf16x8*>(L.w + kP1 + (cb * 64 + lane) * 8);
-        acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[cb], 0, 0, 0);
-    }
-    auto store_act = [&](const float* b) {
-#pragma unroll
-        for (int cb = 0; cb < 8; ++cb) {
-            const int col = cb * 16 + r;
-            const float bc = b[col];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) L.act[(4 * g + i) * kActStride + col] = __bf16(relu(acc[cb][i] + bc));
-        }
-        wave_lds_fence();
-    };
-    auto hidden = [&](int poff) {
-#pragma unroll
-        for (int cb = 0; cb < 8; ++cb) acc[cb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int ks = 0; ks < kHid / 32; ++ks) {
-            const bf16x8 av2 = *reinterpret_cast<const bf16x8*>(L.act + r * kActStride + ks * 32 + 8 * g);
-#pragma unroll
-            for (int cb = 0; cb < 8; ++cb) {
-                const bf16x8 bv = *reinterpret_cast<const bf16x8*>(L.w + poff + ((ks * 8 + cb) * 64 + lane) * 8);
-                acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av2, bv, acc[cb], 0, 0, 0);
-            }
-        }
-        wave_lds_fence();
-    };
-    store_act(L.bias);
-    hidden(kP2);
-    store_act(L.bias + kHid);
-    hidden(kP3);
-    store_act(L.bias + 2 * kHid);
-    f32x4 o = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-    for (int ks = 0; ks < kHid / 32; ++ks) {
-        const bf16x8 av2 = *reinterpret_cast<const bf16x8*>(L.act + r * kActStride + ks * 32 + 8 * g);
-        const bf16x8 bv = *reinterpret_cast<const bf16x8*>(L.w + kP4 + (ks * 64 + lane) * 8);
-        o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av2, bv, o, 0, 0, 0);
-    }
-    if (r < 4) {
-        const float bc = L.bias[3 * kHid + r];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) L.outb[(4 * g + i) * 4 + r] = o[i] + bc;
+        for (int t = 0; t < kTiles; ++t) *reinterpret_cast<f32x4*>(L.corr + (16 * t + r) * 4) = o[t];
     }
     wave_lds_fence();
 }
@@ -241,27 +282,27 @@ GS_DEV void predicted_start(const double (&x)[4], double h, const float* o, cons
     }
 }
 
-// One workgroup of 8 waves per CU holds the packed weights and biases in LDS; each wave runs tiles of
-// 16 systems through the four layers, keeping its tile's activations in a private LDS buffer.
+// One workgroup of 8 waves per CU holds the packed weights and biases in LDS; each wave runs chunks
+// of kChunk systems through the four layers, its activations in registers.
 __global__ __launch_bounds__(64 * kStartWaves) void mlp_start_kernel(const StartArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     const MlpLds L = mlp_lds_setup(lds, a.packed, a.params);
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-    const int64_t ntile = (a.nsys + 15) / 16;
-    for (int64_t t = int64_t(blockIdx.x) * kStartWaves + wave; t < ntile; t += int64_t(gridDim.x) * kStartWaves) {
-        const int64_t s = t * 16 + r;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t nchunk = (a.nsys + kChunk - 1) / kChunk;
+    for (int64_t c = int64_t(blockIdx.x) * kStartWaves + wave; c < nchunk; c += int64_t(gridDim.x) * kStartWaves) {
+        const int64_t s = c * kChunk + lane;
         const bool live = s < a.nsys;
         double x[4] = {0.0, 0.0, 0.0, 0.0};
         if (live) load_state(a.Xk, s, x);
-        mlp_tile(L, a.nm, a.h, x);
-        if (g == 0 && live) {
+        mlp_chunk(L, a.nm, a.h, x);
+        if (live) {
             double y[4];
-            predicted_start(x, a.h, L.outb + r * 4, a.nm, y);
+            predicted_start(x, a.h, L.corr + lane * 4, a.nm, y);
             double2* yp = reinterpret_cast<double2*>(a.X0 + 4 * s);
             yp[0] = double2{y[0], y[1]};
             yp[1] = double2{y[2], y[3]};
         }
-        __builtin_amdgcn_wave_barrier();
+        wave_lds_fence();
     }
 }
 
@@ -289,36 +330,27 @@ GS_DEV void store_status_learned(gs_status_f64* dst, uint8_t code, int32_t itera
     w[8] = uint64_t(contracted ? 1 : 0);
 }
 
-// Each wave takes 64 systems at a time: four 16-system tiles through the network (corrections into
-// LDS), then one lane per system runs the method from its predicted start; a lane whose solve does
-// not end in SUCCESS solves again from x_k with plain Newton, and that status is the system's.
+// Each wave takes 64 systems at a time: the network on all of them (corrections into LDS), then one
+// lane per system runs the method from its predicted start; a lane whose solve does not end in
+// SUCCESS solves again from x_k with plain Newton, and that status is the system's.
 template <bool MIXED, bool REFINE>
 __global__ __launch_bounds__(64 * kStartWaves) void learned_solve_kernel(const LearnedArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     const MlpLds L = mlp_lds_setup(lds, a.st.packed, a.st.params);
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-    const int64_t nchunk = (a.st.nsys + 63) / 64;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t nchunk = (a.st.nsys + kChunk - 1) / kChunk;
     FamArgs fa;
     fa.P = reinterpret_cast<const char*>(a.st.Xk);
     fa.p_stride = 32;
     fa.shared[0] = a.st.h;
     for (int k = 1; k < 8; ++k) fa.shared[k] = 0.0;
     for (int64_t c = int64_t(blockIdx.x) * kStartWaves + wave; c < nchunk; c += int64_t(gridDim.x) * kStartWaves) {
-        for (int t = 0; t < 4; ++t) {
-            const int64_t s = c * 64 + t * 16 + r;
-            double x[4] = {0.0, 0.0, 0.0, 0.0};
-            if (s < a.st.nsys) load_state(a.st.Xk, s, x);
-            mlp_tile(L, a.st.nm, a.st.h, x);
-            if (g == 0) {
-#pragma unroll
-                for (int j = 0; j < 4; ++j) L.corr[(t * 16 + r) * 4 + j] = L.outb[r * 4 + j];
-            }
-            wave_lds_fence();
-        }
-        const int64_t s = c * 64 + lane;
+        const int64_t s = c * kChunk + lane;
+        double xk[4] = {0.0, 0.0, 0.0, 0.0};
+        if (s < a.st.nsys) load_state(a.st.Xk, s, xk);
+        mlp_chunk(L, a.st.nm, a.st.h, xk);
         if (s < a.st.nsys) {
-            double xk[4], x[4];
-            load_state(a.st.Xk, s, xk);
+            double x[4];
             predicted_start(xk, a.st.h, L.corr + lane * 4, a.st.nm, x);
             FamHenonHeilesMidpoint fam;
             fam.load(fa, s);
@@ -583,8 +615,8 @@ int gs_mlp_start_henon_heiles(const gs_mlp* net, int64_t nsys, const double* Xk,
     a.packed = reinterpret_cast<const __bf16*>(net->packed);
     a.params = net->params;
     a.nm = norm_of(*net);
-    const int64_t ntile = (nsys + 15) / 16;
-    int64_t blocks = (ntile + kStartWaves - 1) / kStartWaves;
+    const int64_t nchunk = (nsys + kChunk - 1) / kChunk;
+    int64_t blocks = (nchunk + kStartWaves - 1) / kStartWaves;
     if (blocks > cus) blocks = cus;
     hipLaunchKernelGGL(mlp_start_kernel, dim3(unsigned(blocks)), dim3(64 * kStartWaves), kStartLds, st, a);
     return check(hipGetLastError(), "learned-start launch failed");
@@ -596,8 +628,8 @@ int gs_solve_learned_henon_heiles_f64(const gs_mlp* net, const gs_method* method
     if (!net || !net->params || !net->packed || !method || !options)
         return set_error(GS_EINVAL, "net, method and options must not be NULL");
     if (nsys < 0) return set_error(GS_EINVAL, "nsys must be ≥ 0");
-    if (method->linsolve != GS_LU || method->globalization != GS_STATIC)
-        return set_error(GS_EFAMILY, "the fused learned solve takes Newton with LU and the Static globalization");
+    if (method->linsolve != GS_LU || method->globalization != GS_STATIC || method->reuse != 0)
+        return set_error(GS_EFAMILY, "the fused learned solve takes Newton with LU, the Static globalization and no reuse");
     if (method->factor_precision != GS_FACTOR_WORKING && method->factor_precision != GS_FACTOR_F32)
         return set_error(GS_EINVAL, "factor_precision must be GS_FACTOR_WORKING or GS_FACTOR_F32");
     if (fallbacks) *fallbacks = 0;
@@ -649,7 +681,7 @@ int gs_solve_learned_henon_heiles_f64(const gs_mlp* net, const gs_method* method
     a.qf = dev_params(plain, *options);
     a.S = S;
     a.fallbacks = counter;
-    const int64_t nchunk = (nsys + 63) / 64;
+    const int64_t nchunk = (nsys + kChunk - 1) / kChunk;
     int64_t blocks = (nchunk + kStartWaves - 1) / kStartWaves;
     if (blocks > cus) blocks = cus;
     const dim3 grid{unsigned(blocks), 1, 1}, block{64 * kStartWaves, 1, 1};
