@@ -60,7 +60,7 @@ struct GsMethod
     regularization::Float64
     lm::NTuple{6, Float64}
     best::Int32
-    pad::Int32
+    reuse::Int32
 end
 
 struct GsFamily
@@ -178,18 +178,21 @@ end
 # ---- the method and options as ABI structs ----------------------------------------------------
 
 device_method(m) = nothing  # not served: the generic method runs
-function device_method(m::Newton{<:LUFactorization{F}, G, Nothing, RF, Float64, Nothing}) where {F, G, RF}
+# `reuse = k` is served (the held factor of newton.jl:1781-1874); `keep_factor` keeps a factor across
+# the solves of one state, and each system of a batch solves once from a fresh state.
+function device_method(m::Newton{<:LUFactorization{F}, G, Nothing, RF, Float64, K}) where {
+        F, G, RF, K <: Union{Nothing, Int32}}
     F === Nothing || F === Float32 || return nothing
     G <: Union{Static, Backtracking} || return nothing
     RF <: Union{NoRefinement, Refinement} || return nothing
-    m.keep_factor && return nothing
     g = m.globalization
     bt = g isa Backtracking ? g : Backtracking()
     rf = m.refinement isa Refinement ? m.refinement : Refinement()
     return GsMethod(Int32(0), F === Float32 ? Int32(1) : Int32(0), g isa Backtracking ? Int32(1) : Int32(0),
         m.refinement isa Refinement ? Int32(1) : Int32(0), g isa Static ? Float64(g.α) : 1.0,
         Float64(bt.c₁), Float64(bt.p), Int32(bt.maxiter), Int32(rf.maxiter), Float64(rf.η),
-        Float64(m.regularization), ntuple(_ -> 0.0, 6), m.best === Val(true) ? Int32(1) : Int32(0), Int32(0))
+        Float64(m.regularization), ntuple(_ -> 0.0, 6), m.best === Val(true) ? Int32(1) : Int32(0),
+        m.reuse === nothing ? Int32(0) : m.reuse)
 end
 
 function device_options(o::Options{Float64})

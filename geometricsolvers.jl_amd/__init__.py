@@ -30,7 +30,7 @@ from ._abi import STATUS_DTYPE, load_library
 __all__ = [
     "ArgumentError", "DimensionMismatch", "ReturnCode", "SUCCESS", "STALLED", "MAXITERS",
     "SINGULAR", "NONFINITE", "LINESEARCH_FAILED", "STATUS_DTYPE", "L2Norm", "MaxNorm", "Options",
-    "LUFactorization", "Static", "Backtracking", "LevenbergMarquardt", "NoRefinement", "Refinement", "Newton",
+    "LUFactorization", "Static", "Backtracking", "DogLeg", "LevenbergMarquardt", "NoRefinement", "Refinement", "Newton",
     "NonlinearProblem", "Coupled2", "Hetero2", "ScalarSquare", "TanhLinear3",
     "HenonHeilesMidpoint", "FpuGl3", "PerSystem", "BatchParams", "parameters", "solve_batch",
     "LeastSquaresProblem", "ExpSumFits", "ExpSumData",
@@ -193,6 +193,22 @@ class Backtracking:
 
 
 @dataclass(frozen=True)
+class DogLeg:
+    """DogLeg(; radius = None) (src/globalization/dogleg.jl:1-70): the trust-region step of Procedure
+    4.1 of Pawlowski, Simonis, Walker and Shadid (2008). `radius` is the initial radius, stored
+    clamped to [1e-6, 1e10]; None starts it at the first Newton step's norm."""
+
+    radius: Optional[float] = None
+
+    def __post_init__(self):
+        if self.radius is not None:
+            r = float(self.radius)
+            if not (r > 0 and math.isfinite(r)):
+                raise ArgumentError("the radius of DogLeg must be positive and finite")
+            object.__setattr__(self, "radius", min(max(r, 1e-6), 1e10))
+
+
+@dataclass(frozen=True)
 class LevenbergMarquardt:
     """LevenbergMarquardt(; μ₀, μ_min, μ_max, μ_decrease, μ_increase, ρ_accept)
     (src/globalization/levenbergmarquardt.jl:48-98), `scaling = :none`, the stacked Givens step."""
@@ -239,10 +255,13 @@ class Refinement:
 
 @dataclass(frozen=True)
 class Newton:
-    """Newton(; linsolve, globalization, refinement, regularization, best) (newton.jl:138-250).
+    """Newton(; linsolve, globalization, refinement, regularization, best, reuse, keep_factor)
+    (newton.jl:138-250).
 
-    `reuse`, `keep_factor` and a `jacobian` selector have no device form: the families carry their
-    analytic Jacobians, and a factor kept across steps is not part of this kernel.
+    `reuse = k` holds the factor for up to k consecutive steps (newton.jl:1781-1874) in the
+    lane-per-system kernels. `keep_factor` keeps the factor across the solves of one state; a batch
+    solves each system once from a fresh state, so it changes no result here. A `jacobian` selector
+    has no device form: the families carry their analytic Jacobians.
     """
 
     linsolve: LUFactorization = field(default_factory=LUFactorization)
@@ -265,8 +284,12 @@ class Newton:
             raise ArgumentError("Newton reuse must be ≥ 1 or nothing")
         if not isinstance(self.linsolve, LUFactorization):
             raise ArgumentError("the device solve takes LUFactorization() or LUFactorization(Float32)")
-        if not isinstance(self.globalization, (Static, Backtracking, LevenbergMarquardt)):
-            raise ArgumentError("the device solve takes Static(), Backtracking() or LevenbergMarquardt() with LU")
+        if not isinstance(self.globalization, (Static, Backtracking, LevenbergMarquardt, DogLeg)):
+            raise ArgumentError(
+                "the device solve takes Static(), Backtracking(), DogLeg() or LevenbergMarquardt() with LU")
+        if isinstance(self.globalization, DogLeg) and self.reuse is not None:
+            raise ArgumentError("Newton with a DogLeg globalization takes no reuse: its model needs the "
+                                "Jacobian of the current iterate")
         if not isinstance(self.refinement, (NoRefinement, Refinement)):
             raise ArgumentError("refinement must be NoRefinement() or Refinement()")
         if isinstance(self.globalization, LevenbergMarquardt):
@@ -278,8 +301,6 @@ class Newton:
                     "the solve of J d = -F, which the damped step does not take; use refinement = NoRefinement()")
 
     def to_c(self) -> _abi.GsMethod:
-        if self.reuse is not None:
-            raise ArgumentError("Newton(; reuse = k) has no device form in libgsolve_hip")
         c = self.__dict__.get("_c")
         if c is not None:
             return c
@@ -288,7 +309,9 @@ class Newton:
         m.factor_precision = _abi.GS_FACTOR_F32 if self.linsolve.precision is not None else _abi.GS_FACTOR_WORKING
         g = self.globalization
         m.globalization = (_abi.GS_BACKTRACKING if isinstance(g, Backtracking) else
-                           _abi.GS_LEVENBERG_MARQUARDT if isinstance(g, LevenbergMarquardt) else _abi.GS_STATIC)
+                           _abi.GS_LEVENBERG_MARQUARDT if isinstance(g, LevenbergMarquardt) else
+                           _abi.GS_DOGLEG if isinstance(g, DogLeg) else _abi.GS_STATIC)
+        m.dl_radius = float(g.radius) if isinstance(g, DogLeg) and g.radius is not None else 0.0
         lm = g if isinstance(g, LevenbergMarquardt) else LevenbergMarquardt()
         m.lm_mu0, m.lm_mu_min, m.lm_mu_max = lm.mu0, lm.mu_min, lm.mu_max
         m.lm_mu_decrease, m.lm_mu_increase, m.lm_rho_accept = lm.mu_decrease, lm.mu_increase, lm.rho_accept
@@ -301,6 +324,7 @@ class Newton:
         m.ref_eta, m.ref_maxiter = rr.eta, rr.maxiter
         m.regularization = self.regularization
         m.best = int(bool(self.best))
+        m.reuse = int(self.reuse) if self.reuse is not None else 0
         object.__setattr__(self, "_c", m)  # the method is frozen: build the struct once
         return m
 

@@ -19,7 +19,7 @@ LIB_PATH = LIB_DIR / "libgsolve_hip.so"
 # gs_error
 GS_OK, GS_EINVAL, GS_EDIM, GS_EFAMILY, GS_EHIP = 0, -1, -2, -3, -4
 # gs_globalization / gs_factor_precision / gs_refinement / gs_norm / gs_linsolve
-GS_STATIC, GS_BACKTRACKING, GS_LEVENBERG_MARQUARDT = 0, 1, 2
+GS_STATIC, GS_BACKTRACKING, GS_LEVENBERG_MARQUARDT, GS_DOGLEG = 0, 1, 2, 3
 GS_FACTOR_WORKING, GS_FACTOR_F32 = 0, 1
 GS_NO_REFINEMENT, GS_REFINEMENT = 0, 1
 GS_NORM_L2, GS_NORM_MAX = 0, 1
@@ -97,7 +97,8 @@ class GsMethod(ctypes.Structure):
         ("lm_mu_increase", ctypes.c_double),
         ("lm_rho_accept", ctypes.c_double),
         ("best", ctypes.c_int32),
-        ("pad_", ctypes.c_int32),
+        ("reuse", ctypes.c_int32),
+        ("dl_radius", ctypes.c_double),
     ]
 
 

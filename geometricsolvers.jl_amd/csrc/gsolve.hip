@@ -11,7 +11,7 @@
 
 #include "../../include/gsolve.h"
 #include "mid_solver.hpp"
-#include "small_solver.hpp"
+#include "small_launch.hpp"
 
 namespace gs {
 
@@ -48,6 +48,7 @@ DevParams dev_params(const gs_method& method, const gs_options_f64& options) {
     q.ref_eta_eff = method.ref_eta > 1.4901161193847656e-8 ? method.ref_eta : 1.4901161193847656e-8;
     q.mu = method.regularization;
     q.best = method.best != 0;
+    q.reuse = method.reuse;
     return q;
 }
 
@@ -72,97 +73,6 @@ bool timing_mark(hipStream_t st, bool timed) { return timed && hipEventRecord(g_
 void timing_end(bool timed) {
     float ms = 0.0f;
     g_last_ms = (timed && hipEventElapsedTime(&ms, g_ev[0], g_ev[1]) == hipSuccess) ? double(ms) : -1.0;
-}
-
-struct KernelArgs {
-    FamArgs fam;
-    DevParams q;
-    int64_t nsys;
-    double* X;
-    gs_status_f64* S;
-    uint8_t* codes;
-    const int64_t* order;
-};
-
-// The status of one system in the byte layout of SolverStatus{Float64}, as nine 8-byte words.
-GS_DEV void store_status(gs_status_f64* dst, uint8_t code, int32_t iterations, double fnorm, double stepnorm,
-                         int32_t failures, bool promoted, double gradratio, double rate, bool contracted) {
-    uint64_t* w = reinterpret_cast<uint64_t*>(dst);
-    uint64_t w0 = uint64_t(code) | (uint64_t(uint32_t(iterations)) << 32);
-    uint64_t w3 = uint64_t(uint32_t(failures)) | (uint64_t(promoted ? 1 : 0) << 32);
-    uint64_t w4 = uint64_t(uint32_t(-1));  // rank = -1: an LU factor reports no rank
-    w[0] = w0;
-    w[1] = __builtin_bit_cast(uint64_t, fnorm);
-    w[2] = __builtin_bit_cast(uint64_t, stepnorm);
-    w[3] = w3;
-    w[4] = w4;
-    w[5] = __builtin_bit_cast(uint64_t, dnan());  // nullnorm
-    w[6] = __builtin_bit_cast(uint64_t, gradratio);
-    w[7] = __builtin_bit_cast(uint64_t, rate);
-    w[8] = uint64_t(contracted ? 1 : 0);
-}
-
-// The largest workgroup of the lane-per-system kernels (their launch bound) and their default.
-constexpr int kSmallMaxWorkgroup = 256;
-
-// One work-item per system (the launch shape of `_solve_kernel!`, batched.jl:171-179).
-// Waves per SIMD the register allocator targets: 4 (≤ 128 VGPRs) for the plain FP64 Static form,
-// which fit without scratch; the heavier forms keep the allocator's own choice.
-template <int GLOB, bool MIXED, bool REFINE> constexpr int small_waves_per_eu() {
-    return (GLOB == kStatic && !MIXED && !REFINE) ? 4 : 1;
-}
-
-template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG>
-__global__ __launch_bounds__(kSmallMaxWorkgroup)
-__attribute__((amdgpu_waves_per_eu(small_waves_per_eu<GLOB, MIXED, REFINE>())))
-void solve_small_kernel(const KernelArgs a) {
-    constexpr int N = Fam::N;
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= a.nsys) return;
-    const int64_t idx = a.order ? a.order[i] - 1 : i;
-    Fam fam;
-    fam.load(a.fam, idx);
-    double x[N];
-    double* xp = a.X + idx * N;
-#pragma unroll
-    for (int k = 0; k < N; ++k) x[k] = xp[k];
-    SmallSolve<Fam, GLOB, MIXED, REFINE, DIAG> solver(fam, a.q);
-    auto R = solver.solve(x);
-#pragma unroll
-    for (int k = 0; k < N; ++k) xp[k] = x[k];
-    if (DIAG) {
-        store_status(a.S + idx, R.code, R.iterations, R.fnorm, R.stepnorm, R.failures, R.promoted, R.gradratio,
-                     R.rate, R.contracted);
-    } else {
-        a.codes[idx] = R.code;
-    }
-}
-
-template <class Fam, int GLOB, bool MIXED, bool REFINE>
-static int launch_diag(const KernelArgs& a, bool diag, int wg, hipStream_t st) {
-    const int64_t blocks = (a.nsys + wg - 1) / wg;
-    if (blocks > 0x7fffffffLL) return fail(GS_EINVAL, "batch too large for one launch");
-    if (diag)
-        hipLaunchKernelGGL((solve_small_kernel<Fam, GLOB, MIXED, REFINE, true>), dim3(unsigned(blocks)), dim3(wg), 0, st, a);
-    else
-        hipLaunchKernelGGL((solve_small_kernel<Fam, GLOB, MIXED, REFINE, false>), dim3(unsigned(blocks)), dim3(wg), 0, st, a);
-    return GS_OK;
-}
-
-template <class Fam>
-static int launch_family(const KernelArgs& a, int glob, bool mixed, bool refine, bool diag, int wg, hipStream_t st) {
-#define GS_L(G, M, R) return launch_diag<Fam, G, M, R>(a, diag, wg, st)
-    if (glob == GS_STATIC) {
-        if (!mixed && !refine) GS_L(kStatic, false, false);
-        if (!mixed && refine) GS_L(kStatic, false, true);
-        if (mixed && !refine) GS_L(kStatic, true, false);
-        GS_L(kStatic, true, true);
-    }
-    if (!mixed && !refine) GS_L(kBacktracking, false, false);
-    if (!mixed && refine) GS_L(kBacktracking, false, true);
-    if (mixed && !refine) GS_L(kBacktracking, true, false);
-    GS_L(kBacktracking, true, true);
-#undef GS_L
 }
 
 struct FamilySpec {
@@ -220,6 +130,9 @@ int gs_solve_batch_f64(const gs_method* method, const gs_family* family, int64_t
                                     " has no lane-per-system form for this family");
     if (family->id == GS_FAM_FPU_GL3 && method->globalization != GS_STATIC)
         return fail(GS_EFAMILY, "the wave-per-system solve of FpuGl3 takes the Static globalization");
+    if (method->reuse < 0) return fail(GS_EINVAL, "reuse must be ≥ 1, or 0 for nothing");
+    if (family->id == GS_FAM_FPU_GL3 && method->reuse > 0)
+        return fail(GS_EFAMILY, "the wave-per-system solve of FpuGl3 takes no reuse");
     if (method->factor_precision != GS_FACTOR_WORKING && method->factor_precision != GS_FACTOR_F32)
         return fail(GS_EINVAL, "factor_precision must be GS_FACTOR_WORKING or GS_FACTOR_F32");
     if (method->refinement != GS_NO_REFINEMENT && method->refinement != GS_REFINEMENT)
@@ -284,16 +197,12 @@ int gs_solve_batch_f64(const gs_method* method, const gs_family* family, int64_t
         timing_end(timed);
         return GS_OK;
     }
-    switch (family->id) {
-        case GS_FAM_COUPLED2: rc = launch_family<FamCoupled2>(a, glob, mixed, refine, diag, wg, st); break;
-        case GS_FAM_HETERO2: rc = launch_family<FamHetero2>(a, glob, mixed, refine, diag, wg, st); break;
-        case GS_FAM_SCALAR_SQUARE: rc = launch_family<FamScalarSquare>(a, glob, mixed, refine, diag, wg, st); break;
-        case GS_FAM_TANH_LINEAR3: rc = launch_family<FamTanhLinear3>(a, glob, mixed, refine, diag, wg, st); break;
-        case GS_FAM_HENON_HEILES_MIDPOINT:
-            rc = launch_family<FamHenonHeilesMidpoint>(a, glob, mixed, refine, diag, wg, st);
-            break;
-        default: return fail(GS_EFAMILY, "problem family is not registered");
-    }
+    if (method->reuse > 0)
+        rc = launch_small_reuse(family->id, glob, a, mixed, refine, diag, wg, st);
+    else if (glob == GS_STATIC)
+        rc = launch_small_family<kStatic, false>(family->id, a, mixed, refine, diag, wg, st);
+    else
+        rc = launch_small_family<kBacktracking, false>(family->id, a, mixed, refine, diag, wg, st);
     if (rc != GS_OK) return rc;
     timed = timing_mark(st, timed);
     hipError_t e = hipGetLastError();

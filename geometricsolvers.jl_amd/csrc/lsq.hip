@@ -797,6 +797,7 @@ int gs_solve_expsum_lsq_f64(const gs_method* method, int64_t nfit, double* X, co
     if (!method || !options) return set_error(GS_EINVAL, "method and options must not be NULL");
     if (method->globalization != GS_LEVENBERG_MARQUARDT)
         return set_error(GS_EFAMILY, "the least-squares batch takes Newton(; globalization = LevenbergMarquardt())");
+    if (method->reuse != 0) return set_error(GS_EINVAL, "Newton with a LevenbergMarquardt globalization takes no reuse");
     // the LevenbergMarquardt constructor's checks (levenbergmarquardt.jl:60-95); they also bound the
     // damping loop of a step, which ends once μ passes μ_max
     {

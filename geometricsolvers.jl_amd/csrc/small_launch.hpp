@@ -1,0 +1,125 @@
+// The lane-per-system kernel of gs_solve_batch_f64 and its launch, one globalization at a time, so
+// that the instantiations of each form (plain, reuse = k, DogLeg) compile in translation units of
+// their own: gsolve.hip (Static and Backtracking), small_reuse.hip (reuse = k), small_dogleg.hip
+// (DogLeg).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/gsolve.h"
+#include "families.hpp"
+#include "small_solver.hpp"
+
+namespace gs {
+
+int set_error(int code, const std::string& msg);
+
+struct KernelArgs {
+    FamArgs fam;
+    DevParams q;
+    int64_t nsys;
+    double* X;
+    gs_status_f64* S;
+    uint8_t* codes;
+    const int64_t* order;
+};
+
+// The status of one system in the byte layout of SolverStatus{Float64}, as nine 8-byte words.
+GS_DEV void store_status(gs_status_f64* dst, uint8_t code, int32_t iterations, double fnorm, double stepnorm,
+                         int32_t failures, bool promoted, double gradratio, double rate, bool contracted,
+                         bool reused) {
+    uint64_t* w = reinterpret_cast<uint64_t*>(dst);
+    uint64_t w0 = uint64_t(code) | (uint64_t(uint32_t(iterations)) << 32);
+    uint64_t w3 = uint64_t(uint32_t(failures)) | (uint64_t(promoted ? 1 : 0) << 32);
+    uint64_t w4 = uint64_t(uint32_t(-1));  // rank = -1: an LU factor reports no rank
+    w[0] = w0;
+    w[1] = __builtin_bit_cast(uint64_t, fnorm);
+    w[2] = __builtin_bit_cast(uint64_t, stepnorm);
+    w[3] = w3;
+    w[4] = w4;
+    w[5] = __builtin_bit_cast(uint64_t, dnan());  // nullnorm
+    w[6] = __builtin_bit_cast(uint64_t, gradratio);
+    w[7] = __builtin_bit_cast(uint64_t, rate);
+    w[8] = uint64_t(contracted ? 1 : 0) | (uint64_t(reused ? 1 : 0) << 8);
+}
+
+// The largest workgroup of the lane-per-system kernels (their launch bound) and their default.
+constexpr int kSmallMaxWorkgroup = 256;
+
+// Waves per SIMD the register allocator targets: 4 (≤ 128 VGPRs) for the plain FP64 Static form,
+// which fit without scratch; the heavier forms keep the allocator's own choice.
+template <int GLOB, bool MIXED, bool REFINE, bool REUSE> constexpr int small_waves_per_eu() {
+    return (GLOB == kStatic && !MIXED && !REFINE && !REUSE) ? 4 : 1;
+}
+
+// One work-item per system (the launch shape of `_solve_kernel!`, batched.jl:171-179).
+template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE>
+__global__ __launch_bounds__(kSmallMaxWorkgroup)
+__attribute__((amdgpu_waves_per_eu(small_waves_per_eu<GLOB, MIXED, REFINE, REUSE>())))
+void solve_small_kernel(const KernelArgs a) {
+    constexpr int N = Fam::N;
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= a.nsys) return;
+    const int64_t idx = a.order ? a.order[i] - 1 : i;
+    Fam fam;
+    fam.load(a.fam, idx);
+    double x[N];
+    double* xp = a.X + idx * N;
+#pragma unroll
+    for (int k = 0; k < N; ++k) x[k] = xp[k];
+    SmallSolve<Fam, GLOB, MIXED, REFINE, DIAG, REUSE> solver(fam, a.q);
+    auto R = solver.solve(x);
+#pragma unroll
+    for (int k = 0; k < N; ++k) xp[k] = x[k];
+    if (DIAG) {
+        store_status(a.S + idx, R.code, R.iterations, R.fnorm, R.stepnorm, R.failures, R.promoted, R.gradratio,
+                     R.rate, R.contracted, R.reused);
+    } else {
+        a.codes[idx] = R.code;
+    }
+}
+
+template <class Fam, int GLOB, bool MIXED, bool REFINE, bool REUSE>
+int launch_diag(const KernelArgs& a, bool diag, int wg, hipStream_t st) {
+    const int64_t blocks = (a.nsys + wg - 1) / wg;
+    if (blocks > 0x7fffffffLL) return set_error(GS_EINVAL, "batch too large for one launch");
+    if (diag)
+        hipLaunchKernelGGL((solve_small_kernel<Fam, GLOB, MIXED, REFINE, true, REUSE>), dim3(unsigned(blocks)), dim3(wg),
+                           0, st, a);
+    else
+        hipLaunchKernelGGL((solve_small_kernel<Fam, GLOB, MIXED, REFINE, false, REUSE>), dim3(unsigned(blocks)),
+                           dim3(wg), 0, st, a);
+    return GS_OK;
+}
+
+// The launch of one globalization's kernels for family Fam.
+template <class Fam, int GLOB, bool REUSE>
+int launch_glob(const KernelArgs& a, bool mixed, bool refine, bool diag, int wg, hipStream_t st) {
+    if (!mixed && !refine) return launch_diag<Fam, GLOB, false, false, REUSE>(a, diag, wg, st);
+    if (!mixed && refine) return launch_diag<Fam, GLOB, false, true, REUSE>(a, diag, wg, st);
+    if (mixed && !refine) return launch_diag<Fam, GLOB, true, false, REUSE>(a, diag, wg, st);
+    return launch_diag<Fam, GLOB, true, true, REUSE>(a, diag, wg, st);
+}
+
+// The family dispatch of one globalization (GS_FAM_* → Fam); EFAMILY for a family without a
+// lane-per-system form.
+template <int GLOB, bool REUSE>
+int launch_small_family(int family, const KernelArgs& a, bool mixed, bool refine, bool diag, int wg, hipStream_t st) {
+    switch (family) {
+        case GS_FAM_COUPLED2: return launch_glob<FamCoupled2, GLOB, REUSE>(a, mixed, refine, diag, wg, st);
+        case GS_FAM_HETERO2: return launch_glob<FamHetero2, GLOB, REUSE>(a, mixed, refine, diag, wg, st);
+        case GS_FAM_SCALAR_SQUARE: return launch_glob<FamScalarSquare, GLOB, REUSE>(a, mixed, refine, diag, wg, st);
+        case GS_FAM_TANH_LINEAR3: return launch_glob<FamTanhLinear3, GLOB, REUSE>(a, mixed, refine, diag, wg, st);
+        case GS_FAM_HENON_HEILES_MIDPOINT:
+            return launch_glob<FamHenonHeilesMidpoint, GLOB, REUSE>(a, mixed, refine, diag, wg, st);
+        default: return set_error(GS_EFAMILY, "problem family is not registered");
+    }
+}
+
+// Defined in small_reuse.hip: Static and Backtracking with reuse = k.
+int launch_small_reuse(int family, int glob, const KernelArgs& a, bool mixed, bool refine, bool diag, int wg,
+                       hipStream_t st);
+
+}  // namespace gs

@@ -12,6 +12,8 @@
 // method and the options is a runtime value.
 #pragma once
 
+#include <type_traits>
+
 #include "device_common.hpp"
 #include "families.hpp"
 
@@ -34,6 +36,7 @@ struct DevParams {
     double ref_eta_eff;    // max(η, √eps)
     double mu;             // regularization
     int32_t best;
+    int32_t reuse;         // Newton(; reuse = k): k ≥ 1 (the REUSE kernels only)
 };
 
 enum { kStatic = 0, kBacktracking = 1 };
@@ -253,8 +256,9 @@ template <int N> GS_DEV double grad_ratio(const double (&J)[N][N], const double 
 // One system
 // ------------------------------------------------------------------------------------------------
 
-template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG> struct SmallSolve {
+template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = false> struct SmallSolve {
     static constexpr int N = Fam::N;
+    using TF = typename std::conditional<MIXED, float, double>::type;
     const Fam& fam;
     const DevParams& q;
 
@@ -264,6 +268,12 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG> struct SmallS
     double gradratio;   // diagnostics of the last factorisation
     bool promoted;      // mixed precision: factorising in FP64
     int slow_steps;
+    // reuse = k: the factor the state holds, in the factor precision, and the rule state FactorAge
+    // (newton.jl:621-633): its age, whether the last step stalled, whether it stepped along a reused
+    // factor
+    Factor<TF, N> kept;
+    int age;
+    bool stalled, reused;
 
     GS_DEV SmallSolve(const Fam& f, const DevParams& p) : fam(f), q(p) {}
 
@@ -406,9 +416,20 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG> struct SmallS
 
     // --- refinement -----------------------------------------------------------------------------
 
-    // refine_passes!! (refinement.jl:211-244) with the held Jacobian J.
-    template <class TF>
-    GS_DEV void refine(double (&d)[N], const Factor<TF, N>& f, const double (&J)[N][N], const double (&Fx)[N], bool& failed) const {
+    // The product of a refinement (newton.jl:1344-1384): the held regularised Jacobian J of a fresh
+    // factor, or, along a reused factor (JVP), the JVP at x with J = J(x): 1·J(x) v + μ v where μ > 0
+    // (RegularizedJVP).
+    template <bool JVP> GS_DEV void product(const double (&J)[N][N], const double (&v)[N], double (&Jv)[N]) const {
+        matvec<N>(J, v, Jv);
+        if (JVP && q.mu != 0.0) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) Jv[i] = 1.0 * Jv[i] + q.mu * v[i];
+        }
+    }
+
+    // refine_passes!! (refinement.jl:211-244) with the product of `product<JVP>(J, ·)`.
+    template <class TFR, bool JVP = false>
+    GS_DEV void refine(double (&d)[N], const Factor<TFR, N>& f, const double (&J)[N][N], const double (&Fx)[N], bool& failed) const {
         failed = false;
         double fnorm = rnorm<N>(Fx);
         if (fnorm == 0.0) return;
@@ -417,7 +438,7 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG> struct SmallS
             return;
         }
         double Jv[N], r[N];
-        matvec<N>(J, d, Jv);
+        product<JVP>(J, d, Jv);
 #pragma unroll
         for (int i = 0; i < N; ++i) r[i] = -Fx[i] + -Jv[i];
         double rprev = rnorm<N>(r);
@@ -431,7 +452,7 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG> struct SmallS
             ldiv<N>(f, r, dd);
 #pragma unroll
             for (int i = 0; i < N; ++i) dc[i] = d[i] + dd[i];
-            matvec<N>(J, dc, Jv);
+            product<JVP>(J, dc, Jv);
 #pragma unroll
             for (int i = 0; i < N; ++i) r[i] = -Fx[i] + -Jv[i];
             double rcur = rnorm<N>(r);
@@ -577,11 +598,224 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG> struct SmallS
         return s;
     }
 
+    // --- reuse = k (newton.jl:1781-1874) ---------------------------------------------------------
+
+    GS_DEV int fresh_age() const { return q.reuse == 1 ? 0 : 1; }                // newton.jl:629
+    GS_DEV int next_age() const { return age + 1 >= q.reuse ? 0 : age + 1; }      // newton.jl:632
+
+    // reuse_result (newton.jl:1814-1820): a SINGULAR step resets the age, a STALLED one refreshes
+    // the factor at the next step.
+    GS_DEV void set_rule(uint8_t code, int age_next, bool along_reused) {
+        age = code == kSingular ? 0 : age_next;
+        stalled = code != kSingular && code == kStalled;
+        reused = along_reused;
+    }
+
+    // newton_step!! with reuse in the working precision (newton.jl:1781-1810): a step along the held
+    // factor refines with the JVP at x and takes the measured slope; a failed refinement (one
+    // step-rule failure) or a line search that fails without a step refactorises at x in the step.
+    GS_DEV Step plain_step_reuse(double (&x)[N], double (&Fx)[N]) {
+        int failures = 0;
+        if (!(age == 0 || stalled)) {
+            const int age_next = next_age();
+            double d[N];
+            ldiv<N>(kept, Fx, d);
+#pragma unroll
+            for (int i = 0; i < N; ++i) d[i] = -d[i];
+            bool ref_failed = false;
+            if (REFINE) {
+                double Jx[N][N];
+                fam.J(x, Jx);
+                refine<TF, true>(d, kept, Jx, Fx, ref_failed);
+            }
+            if (ref_failed) {
+                failures = 1;
+            } else {
+                // a failed search without a step leaves x, Fx and the merit as they were
+                Step s = globalize(x, Fx, d, false);
+                if (!(s.code == kLsFailed && s.stepnorm == 0.0)) {
+                    sfnorm = s.fnorm;
+                    set_rule(s.code, age_next, true);
+                    return s;
+                }
+            }
+        }
+        double J[N][N];
+        jacobian(x, Fx, J);
+        factorize<TF, N>(J, kept);
+        Step s;
+        if (kept.info > 0) {
+            s = Step{kSingular, false, false, 0, rnorm<N>(Fx), 0.0};
+        } else {
+            double d[N];
+            ldiv<N>(kept, Fx, d);
+#pragma unroll
+            for (int i = 0; i < N; ++i) d[i] = -d[i];
+            if (REFINE) {
+                bool failed;
+                refine<TF>(d, kept, J, Fx, failed);
+            }
+            s = globalize(x, Fx, d, q.mu == 0.0 && !REFINE);
+        }
+        s.failures += failures;
+        sfnorm = s.fnorm;
+        set_rule(s.code, fresh_age(), false);
+        return s;
+    }
+
+    // mixed_step!! with reuse (newton.jl:1823-1874) over mixed_newton!! with store = true
+    // (newton.jl:1604-1666): a promoted state factorises in FP64 at every step; otherwise the step
+    // runs along the held FP32 factor until its age or a stall refreshes it, and a reused direction
+    // whose refinement fails is refactorised in FP32 at x (one failure) before any promotion. A line
+    // search along the held factor that fails without a step is retried once from a fresh factor.
+    GS_DEV Step mixed_step_reuse(double (&x)[N], double (&Fx)[N]) {
+        const double fnorm_old = residual_norm(Fx);
+        const bool state_promoted = promoted;
+        bool fresh = promoted || age == 0 || stalled;
+        const int age_fresh = fresh_age();
+        int age_next = fresh ? age_fresh : next_age();
+        while (true) {
+            const bool was_fresh = fresh;
+            double d[N];
+            bool ok = true, md_fresh = true, have = false;
+            int failures = 0;
+            double lin_rel = 0.0;
+            if (promoted) {
+                double J[N][N];
+                jacobian(x, Fx, J);
+                Factor<double, N> f;
+                factorize<double, N>(J, f);
+                if (f.info > 0) {
+                    ok = false;
+                } else {
+                    ldiv<N>(f, Fx, d);
+#pragma unroll
+                    for (int i = 0; i < N; ++i) d[i] = -d[i];
+                    if (REFINE) {
+                        bool failed;
+                        refine<double>(d, f, J, Fx, failed);
+                    }
+                }
+                have = true;
+            } else if (!fresh) {
+                ldiv<N>(kept, Fx, d);
+#pragma unroll
+                for (int i = 0; i < N; ++i) d[i] = -d[i];
+                bool ref_failed = false;
+                if (REFINE) {
+                    double Jx[N][N];
+                    fam.J(x, Jx);
+                    refine<TF, true>(d, kept, Jx, Fx, ref_failed);
+                }
+                if (!ref_failed) {
+                    md_fresh = false;
+                    have = true;
+                }
+            }
+            if (!have) {
+                double J[N][N];
+                jacobian(x, Fx, J);
+                Factor<float, N> f;
+                factorize<float, N>(J, f);
+                if (!fresh) failures += 1;
+                bool ref_failed = false;
+                if (f.info > 0) {
+                    ref_failed = true;
+                } else {
+                    ldiv<N>(f, Fx, d);
+#pragma unroll
+                    for (int i = 0; i < N; ++i) d[i] = -d[i];
+                    if (REFINE) refine<float>(d, f, J, Fx, ref_failed);
+                }
+                if (ref_failed) {
+                    // promote_factor!! (newton.jl:1332-1339): refactorise the held Jacobian in FP64
+                    failures += 1;
+                    promoted = true;
+                    Factor<double, N> g;
+                    factorize<double, N>(J, g);
+                    if (g.info > 0) {
+                        ok = false;
+                    } else {
+                        ldiv<N>(g, Fx, d);
+#pragma unroll
+                        for (int i = 0; i < N; ++i) d[i] = -d[i];
+                    }
+                } else {
+                    copy_factor(f);
+                    if (fnorm_old != 0.0 && isfin(fnorm_old)) {
+                        // linear_residual_ratio (newton.jl:1422-1430)
+                        double Jd[N], lin[N];
+                        matvec<N>(J, d, Jd);
+#pragma unroll
+                        for (int i = 0; i < N; ++i) lin[i] = 1.0 * Fx[i] + 1.0 * Jd[i];
+                        lin_rel = rnorm<N>(lin) / fnorm_old;
+                    }
+                }
+            }
+            fresh = md_fresh;
+            if (!ok) {
+                set_rule(kSingular, 0, !md_fresh);
+                return Step{kSingular, promoted, false, failures, rnorm<N>(Fx), 0.0};
+            }
+            const bool promoted_step = promoted && !state_promoted;
+            if (promoted_step) {
+                age_next = 0;
+            } else if (md_fresh && !was_fresh) {
+                age_next = age_fresh;
+            }
+            // newton_stepkind(prob, m, T, md.fresh, promoted) (newton.jl:1244-1254)
+            Step s = globalize(x, Fx, d, md_fresh && q.mu == 0.0 && promoted);
+            if (!md_fresh && s.code == kLsFailed && s.stepnorm == 0.0) {
+                fresh = true;
+                age_next = age_fresh;
+                continue;
+            }
+            s.failures += failures;
+            if (promoted_step) {
+                s.promoted = true;
+                slow_steps = 0;
+            } else if (md_fresh && !promoted) {
+                // read_mixed_stall (newton.jl:1437-1451), η_lin = max(√eps, 1e-4)
+                if (s.fnorm > 0.5 * fnorm_old && lin_rel > 1e-4) {
+                    int slow = slow_steps + 1;
+                    if (slow >= 2) {
+                        promoted = true;
+                        slow_steps = 0;
+                        s.failures += 1;
+                        s.promoted = true;
+                    } else {
+                        slow_steps = slow;
+                    }
+                } else {
+                    slow_steps = 0;
+                }
+            }
+            sfnorm = s.fnorm;
+            set_rule(s.code, age_next, !md_fresh);
+            return s;
+        }
+    }
+
+    // The held factor of a mixed solve is the FP32 one (`kept` is Factor<float, N> when MIXED).
+    GS_DEV void copy_factor(const Factor<float, N>& f) {
+        if constexpr (MIXED) kept = f;
+    }
+
+    GS_DEV Step step(double (&x)[N], double (&Fx)[N]) {
+        if constexpr (REUSE) {
+            if constexpr (MIXED) return mixed_step_reuse(x, Fx);
+            else return plain_step_reuse(x, Fx);
+        } else {
+            if constexpr (MIXED) return mixed_step(x, Fx);
+            else return plain_step(x, Fx);
+        }
+    }
+
     // --- the loop -------------------------------------------------------------------------------
 
     struct Result {
         uint8_t code;
-        bool contracted, promoted;
+        bool contracted, promoted, reused;
         int32_t iterations, failures;
         double fnorm, stepnorm, rate, gradratio;
     };
@@ -597,13 +831,16 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG> struct SmallS
         gradratio = dnan();
         promoted = false;
         slow_steps = 0;
+        age = 0;
+        stalled = false;
+        reused = false;
         double best_x[N];
         double best_fn = fnorm0;
         if (q.best) {
 #pragma unroll
             for (int i = 0; i < N; ++i) best_x[i] = x[i];
         }
-        Result R{kSuccess, false, false, 0, 0, fnorm0, 0.0, dnan(), dnan()};
+        Result R{kSuccess, false, false, false, 0, 0, fnorm0, 0.0, dnan(), dnan()};
         if (!isfin(fnorm0)) {
             R.code = kNonfinite;
         } else {
@@ -617,7 +854,7 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG> struct SmallS
                 int stalls = 0;
                 while (true) {
                     const double rate_k = R.rate;
-                    Step s = MIXED ? mixed_step(x, Fx) : plain_step(x, Fx);
+                    Step s = step(x, Fx);
                     if (q.best && s.fnorm < best_fn) {
                         best_fn = s.fnorm;
 #pragma unroll
@@ -631,6 +868,9 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG> struct SmallS
                     R.stepnorm = s.stepnorm;
                     R.failures += s.failures;
                     R.promoted |= s.promoted;
+                    // observe_rate!! (newton.jl:1205-1210): a step of rate θ ≥ 1 along a reused
+                    // factor refreshes the factor at the next step
+                    if (REUSE && reused && R.rate >= 1.0) stalled = true;
                     if (s.code == kSingular || s.code == kNonfinite || s.code == kMaxIters) break;
                     if (s.code == kLsFailed && s.stepnorm == 0.0) break;
                     const double ftest = q.norm_max ? rmaxnorm<N>(Fx) : R.fnorm;
@@ -672,6 +912,7 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG> struct SmallS
             }
         }
         R.gradratio = gradratio;
+        R.reused = REUSE && reused && R.iterations > 0;  // reused_factor (newton.jl:1188-1191)
         // restore_best (newton.jl:792-799)
         if (q.best && R.code != kSuccess && isfin(best_fn) && best_fn < R.fnorm) {
 #pragma unroll

@@ -93,12 +93,15 @@ typedef struct gs_options_f64 {
     uint8_t pad_[6];
 } gs_options_f64;
 
-/* The method: Newton(; linsolve, globalization, refinement, regularization, best)
+/* The method: Newton(; linsolve, globalization, refinement, regularization, best, reuse)
  * (src/nonlinear/newton.jl:138-250) with LUFactorization{Nothing|Float32} (src/linear/methods.jl:37-47),
  * Static (src/globalization/static.jl:8-30), Backtracking (src/globalization/backtracking.jl:28-50),
  * LevenbergMarquardt (src/globalization/levenbergmarquardt.jl:48-98) and Refinement
- * (src/nonlinear/refinement.jl:27-49). `reuse` and `keep_factor` have no device form here. */
-enum gs_globalization { GS_STATIC = 0, GS_BACKTRACKING = 1, GS_LEVENBERG_MARQUARDT = 2 };
+ * (src/nonlinear/refinement.jl:27-49), and DogLeg (src/globalization/dogleg.jl:1-80, the trust-region
+ * step of newton.jl:1725-1780). `reuse = k` (newton.jl:1781-1874) is the period of the held
+ * factor of the lane-per-system solve with Static or Backtracking. `keep_factor` keeps a factor
+ * across the solves of one state; a batch solves each system once, so it has no field here. */
+enum gs_globalization { GS_STATIC = 0, GS_BACKTRACKING = 1, GS_LEVENBERG_MARQUARDT = 2, GS_DOGLEG = 3 };
 enum gs_factor_precision { GS_FACTOR_WORKING = 0 /* LUFactorization() */, GS_FACTOR_F32 = 1 };
 enum gs_refinement { GS_NO_REFINEMENT = 0, GS_REFINEMENT = 1 };
 enum gs_linsolve { GS_LU = 0 };
@@ -122,7 +125,8 @@ typedef struct gs_method {
     double lm_mu_increase;
     double lm_rho_accept;
     int32_t best;              /* best = true                                            */
-    int32_t pad_;
+    int32_t reuse;             /* reuse = k ≥ 1; 0 = nothing (factorise at every step)   */
+    double dl_radius;          /* DogLeg(; radius) in [1e-6, 1e10]; 0 = nothing          */
 } gs_method;
 
 /* The registered problem families. Each has its residual F(x, p) and its analytic Jacobian J(x, p)

@@ -45,6 +45,8 @@ template <class R> static Method<R> method_from(const gs_method& g) {
     m.ref_maxiter = g.ref_maxiter;
     m.regularization = R(g.regularization);
     m.best = g.best != 0;
+    m.reuse = g.reuse;
+    m.dl_radius = R(g.dl_radius);
     m.lm_mu0 = R(g.lm_mu0);
     m.lm_mu_min = R(g.lm_mu_min);
     m.lm_mu_max = R(g.lm_mu_max);
@@ -283,8 +285,9 @@ double orc_backtrack_step(double phi0, double d0, double a, double pa, double ap
 int orc_max_threads(void) { return omp_get_max_threads(); }
 
 // One solve of a loop-test fixture (families.hpp, 101-103) with parameters p; x is overwritten.
-int orc_solve_fixture(int fixture, const gs_method* method, const gs_options_f64* options, double* x,
-                      const double* p, gs_status_f64* S) {
+// counts (may be NULL) receives the residual and the Jacobian calls of the solve.
+int orc_solve_fixture_counts(int fixture, const gs_method* method, const gs_options_f64* options, double* x,
+                             const double* p, gs_status_f64* S, long* counts) {
     Method<double> m = method_from<double>(*method);
     Options<double> o = options_from<double>(*options);
     auto run = [&](auto prob) {
@@ -295,6 +298,10 @@ int orc_solve_fixture(int fixture, const gs_method* method, const gs_options_f64
         Status<double> st = solver.solve(v);
         for (int k = 0; k < N; ++k) x[k] = v[k];
         write_status(S, st);
+        if (counts) {
+            counts[0] = prob.nf;
+            counts[1] = prob.nj;
+        }
         return 0;
     };
     switch (fixture) {
@@ -303,6 +310,33 @@ int orc_solve_fixture(int fixture, const gs_method* method, const gs_options_f64
         case 103: return run(InfJacobian1<double>{});
     }
     return GS_EINVAL;
+}
+
+// One DogLeg step!! (newton.jl:1725-1737) from x0 on a DogLegPw residual (families.hpp), as the
+// reference's one_step (test/globalization/dogleg.jl:60-66): out = {x₊, radius after the step, code,
+// failures, stepnorm, residual calls (F(x₀) included), limited}.
+int orc_dogleg_pw_step(int kind, const gs_method* method, double p0, double p1, double x0, double* out) {
+    Method<double> m = method_from<double>(*method);
+    Options<double> o;
+    DogLegPw<double> prob{kind, p0, p1};
+    Solver<double, 1, DogLegPw<double>> solver(prob, m, o);
+    Vec<double, 1> x{x0};
+    Vec<double, 1> Fx = prob.F(x);
+    NState<double, 1> st = solver.start(x, Fx, rnorm(Fx));
+    StepInfo<double> info = solver.step(x, Fx, st);
+    out[0] = x[0];
+    out[1] = st.radius;
+    out[2] = info.code;
+    out[3] = info.failures;
+    out[4] = info.stepnorm;
+    out[5] = double(prob.nf);
+    out[6] = info.limited ? 1.0 : 0.0;
+    return 0;
+}
+
+int orc_solve_fixture(int fixture, const gs_method* method, const gs_options_f64* options, double* x,
+                      const double* p, gs_status_f64* S) {
+    return orc_solve_fixture_counts(fixture, method, options, x, p, S, nullptr);
 }
 
 }  // extern "C"

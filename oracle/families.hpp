@@ -209,15 +209,21 @@ template <class T> struct FpuGl3 {
 //   101: y.^2 − q, J = Diagonal(2y), n = 2         (solver.jl:242-246)
 //   102: 2y − q,  J = 2I, n = 2                    (solver.jl:279-281)
 //   103: x·x − 4, J = Inf, n = 1 (a zero step)     (solver.jl:185)
+// `nj` counts the calls of J, the factorisations' and the JVPs' (J(x) v) alike, as the counting
+// Jacobian closures of the reference's tests do (test/nonlinear/newton.jl:486-545).
 template <class T> struct DiagSquare2 {
     static constexpr int N = 2;
     T q0, q1;
     long nf = 0;
+    mutable long nj = 0;
     Vec<T, 2> F(const Vec<T, 2>& y) {
         ++nf;
         return {y[0] * y[0] - q0, y[1] * y[1] - q1};
     }
-    Mat<T, 2> J(const Vec<T, 2>& y) const { return {T(2) * y[0], T(0), T(0), T(2) * y[1]}; }
+    Mat<T, 2> J(const Vec<T, 2>& y) const {
+        ++nj;
+        return {y[0] + y[0], T(0), T(0), y[1] + y[1]};
+    }
 };
 template <class T> struct Affine2 {
     static constexpr int N = 2;
@@ -227,7 +233,11 @@ template <class T> struct Affine2 {
         ++nf;
         return {T(2) * y[0] - q0, T(2) * y[1] - q1};
     }
-    Mat<T, 2> J(const Vec<T, 2>&) const { return {T(2), T(0), T(0), T(2)}; }
+    mutable long nj = 0;
+    Mat<T, 2> J(const Vec<T, 2>&) const {
+        ++nj;
+        return {T(2), T(0), T(0), T(2)};
+    }
 };
 template <class T> struct InfJacobian1 {
     static constexpr int N = 1;
@@ -236,7 +246,35 @@ template <class T> struct InfJacobian1 {
         ++nf;
         return {x[0] * x[0] - T(4)};
     }
-    Mat<T, 1> J(const Vec<T, 1>&) const { return {std::numeric_limits<T>::infinity()}; }
+    mutable long nj = 0;
+    Mat<T, 1> J(const Vec<T, 1>&) const {
+        ++nj;
+        return {std::numeric_limits<T>::infinity()};
+    }
+};
+
+// The one-dimensional residuals of the reference's DogLeg tests (test/globalization/dogleg.jl:
+// 48-51, 316-340), all with J = 1, so that each test sets the ratio ρ = ared/pred exactly:
+//   kind 0 (pw_F):  p0 at x = 0, p1 at any other x
+//   kind 1 (F5):    −10 at x = 0, −10.5 at x ≥ 5, x − 10 otherwise
+//   kind 2:         x − 10
+template <class T> struct DogLegPw {
+    static constexpr int N = 1;
+    int kind;
+    T p0, p1;
+    long nf = 0;
+    mutable long nj = 0;
+    Vec<T, 1> F(const Vec<T, 1>& x) {
+        ++nf;
+        const T v = x[0];
+        if (kind == 0) return {v == T(0) ? p0 : p1};
+        if (kind == 1) return {v == T(0) ? T(-10) : (v >= T(5) ? T(-10.5) : v - T(10))};
+        return {v - T(10)};
+    }
+    Mat<T, 1> J(const Vec<T, 1>&) const {
+        ++nj;
+        return {T(1)};
+    }
 };
 
 }  // namespace orc

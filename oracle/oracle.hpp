@@ -19,6 +19,7 @@
 #include <cstring>
 #include <limits>
 #include <type_traits>
+#include <utility>
 
 namespace orc {
 
@@ -405,6 +406,18 @@ template <class T, int N> inline Vec<T, N> matvec_t(const Mat<T, N>& A, const Ve
     return y;
 }
 
+// jvp!! (src/ad/jvp.jl:87-100): a problem with an analytic Jacobian takes J(x) * v; one without
+// (the standard test set) defines Jv(x, v), a dual evaluation of F seeded along v.
+template <class P, class R, int N, class = void> struct HasJv : std::false_type {};
+template <class P, class R, int N>
+struct HasJv<P, R, N, std::void_t<decltype(std::declval<P&>().Jv(std::declval<const Vec<R, N>&>(),
+                                                                std::declval<const Vec<R, N>&>()))>>
+    : std::true_type {};
+template <class R, int N, class P> inline Vec<R, N> jvp(P& prob, const Vec<R, N>& x, const Vec<R, N>& v) {
+    if constexpr (HasJv<P, R, N>::value) return prob.Jv(x, v);
+    else return matvec<R, N>(prob.J(x), v);
+}
+
 // grad_ratio_and_norm (src/nonlinear/newton.jl:1287-1298): ‖Jᴴ F‖/(‖J‖_F ‖F‖) and ‖J‖_F.
 template <class T, int N>
 inline void grad_ratio_and_norm(const Mat<T, N>& J, const Vec<T, N>& Fx, T fnorm, T& gr, T& jnorm) {
@@ -437,7 +450,7 @@ template <class T, int N> inline Mat<T, N> regularize(Mat<T, N> J, T mu) {
 // Method description
 // ---------------------------------------------------------------------------------------------
 
-enum Glob { G_STATIC = 0, G_BACKTRACKING = 1, G_LM = 2 };
+enum Glob { G_STATIC = 0, G_BACKTRACKING = 1, G_LM = 2, G_DOGLEG = 3 };
 
 template <class R> struct Method {
     bool factor_f32 = false;   // LUFactorization(Float32)
@@ -451,6 +464,8 @@ template <class R> struct Method {
     int32_t ref_maxiter = 5;
     R regularization = 0;
     bool best = false;
+    int32_t reuse = 0;         // Newton(; reuse = k); 0 = nothing
+    R dl_radius = 0;           // DogLeg(; radius), clamped to [1e-6, 1e10]; 0 = nothing
     // LevenbergMarquardt
     R lm_mu0 = R(1e-6);
     R lm_mu_min = std::max(R(1e-15), 4 * eps_<R>());
@@ -535,7 +550,7 @@ inline LSResult<R, N> backtracking(P& prob, const Method<R>& m, const Vec<R, N>&
         Vec<R, N> xa;
         for (int i = 0; i < N; ++i) xa[i] = x[i] + R(0) * d[i];
         Vec<R, N> Fa = prob.F(xa);
-        Vec<R, N> Jv = matvec<R, N>(prob.J(xa), d);
+        Vec<R, N> Jv = jvp<R, N>(prob, xa, d);
         d0 = 2 * rdot(Fa, Jv);
         n = 1;
     }
@@ -607,6 +622,15 @@ template <class R, int N> struct NState {
     R jnorm;
     bool promoted;  // mixed precision: promoted to the working precision
     int32_t slow_steps;
+    // reuse = k: the factor the state holds (in the factor precision) and its rule state
+    // FactorAge (newton.jl:621-633)
+    StaticLU<R, N> facR;
+    StaticLU<float, N> facF;
+    int32_t age = 0;
+    bool stalled = false;
+    bool reused = false;
+    // DogLeg: the trust radius TrustRadius (newton.jl:637-640); 0 until the first step sets it
+    R radius = 0;
     // best iterate (newton.jl:760-799)
     Vec<R, N> best_x;
     R best_fnorm;
@@ -674,10 +698,28 @@ template <class R, int N, class P> struct Solver {
         return step_info<R>(NONFINITE, rnorm(Fx), R(0), 0, false, false);
     }
 
+    // The product a refinement takes (newton.jl:1344-1384): the held regularised Jacobian of a
+    // fresh factor (HeldJacobianJVP), or, along a reused factor (`held` = nullptr), the JVP at x,
+    // which is RegularizedJVP's 1·(J(x) v) + μ v where μ > 0.
+    Vec<R, N> product(const Mat<R, N>* held, const Vec<R, N>& x, const Vec<R, N>& v) {
+        if (held) return matvec<R, N>(*held, v);
+        Vec<R, N> Jv = jvp<R, N>(prob, x, v);
+        if (m.regularization == R(0)) return Jv;
+        for (int i = 0; i < N; ++i) Jv[i] = R(1) * Jv[i] + m.regularization * v[i];
+        return Jv;
+    }
+
     // refine_passes!! (src/nonlinear/refinement.jl:211-244) with the held Jacobian J.
     template <class TF>
     Vec<R, N> refine(Vec<R, N> d, const StaticLU<TF, N>& fac, const Mat<R, N>& J, const Vec<R, N>& Fx,
                      bool& failed) {
+        return refine_with<TF>(d, fac, &J, Fx, Fx, failed);
+    }
+
+    // refine_passes!! with the product of `product(held, x, ·)`.
+    template <class TF>
+    Vec<R, N> refine_with(Vec<R, N> d, const StaticLU<TF, N>& fac, const Mat<R, N>* held, const Vec<R, N>& x,
+                          const Vec<R, N>& Fx, bool& failed) {
         failed = false;
         if (!m.refinement) return d;
         R fnorm = rnorm(Fx);
@@ -687,7 +729,7 @@ template <class R, int N, class P> struct Solver {
             return d;
         }
         R eta = std::max(m.ref_eta, std::sqrt(eps_<R>()));
-        Vec<R, N> Jv = matvec<R, N>(J, d);
+        Vec<R, N> Jv = product(held, x, d);
         Vec<R, N> r;
         for (int i = 0; i < N; ++i) r[i] = -Fx[i] + -Jv[i];
         R rprev = rnorm(r);
@@ -700,7 +742,7 @@ template <class R, int N, class P> struct Solver {
             Vec<R, N> dd = ldiv<TF, R, N>(fac, r);
             Vec<R, N> dc;
             for (int i = 0; i < N; ++i) dc[i] = d[i] + dd[i];
-            Jv = matvec<R, N>(J, dc);
+            Jv = product(held, x, dc);
             for (int i = 0; i < N; ++i) r[i] = -Fx[i] + -Jv[i];
             R rcur = rnorm(r);
             if (!(rcur < rprev)) {
@@ -767,23 +809,28 @@ template <class R, int N, class P> struct Solver {
         return rnorm(lin) / fnorm_old;
     }
 
-    // mixed_step!! + mixed_newton!! with fresh factors and no reuse (newton.jl:1604-1666, 1692-1720).
-    StepInfo<R> mixed_step(Vec<R, N>& x, Vec<R, N>& Fx, NState<R, N>& st) {
-        R fnorm_old = residual_norm(Fx, st);
-        bool was_promoted = st.promoted;
-        // --- mixed_newton!! ---
+    // The result of mixed_newton!! (newton.jl:1471-1478): the direction, the regularised Jacobian
+    // the factorisation used, the linear-model ratio, whether the direction exists, the failures.
+    struct MixedDir {
         Vec<R, N> d{};
-        bool ok = true, exact = true;
-        int32_t failures = 0;
-        R lin_rel = 0;
-        bool md_promoted = st.promoted;
         Mat<R, N> Jreg;
+        R lin_rel = 0;
+        bool ok = true;
+        int32_t failures = 0;
+    };
+
+    // mixed_newton!! with a fresh factor and no store (newton.jl:1604-1666): FP32 factor and
+    // refinement, promotion to T where either fails; st.promoted and the diagnostics are updated.
+    MixedDir mixed_newton(const Vec<R, N>& x, const Vec<R, N>& Fx, NState<R, N>& st, R fnorm_old) {
+        MixedDir md;
+        Vec<R, N>& d = md.d;
+        Mat<R, N>& Jreg = md.Jreg;
         R gr, jn;
         if (st.promoted) {
             StaticLU<R, N> fac = factor<R>(x, Fx, st, Jreg, gr, jn);
             set_diag(st, gr, jn);
             if (fac.info > 0) {
-                ok = false;
+                md.ok = false;
             } else {
                 d = ldiv<R, R, N>(fac, Fx);
                 for (int i = 0; i < N; ++i) d[i] = -d[i];
@@ -802,30 +849,59 @@ template <class R, int N, class P> struct Solver {
             }
             set_diag(st, gr, jn);
             if (ref_failed || fac.info > 0) {
-                failures += 1;
+                md.failures += 1;
                 // promote_factor!! (newton.jl:1332-1339): refactorise Jreg in T.
                 StaticLU<R, N> facT = lu_factorize<R, R, N>(Jreg);
-                md_promoted = true;
                 st.promoted = true;
                 if (facT.info > 0) {
-                    ok = false;
+                    md.ok = false;
                 } else {
                     d = ldiv<R, R, N>(facT, Fx);
                     for (int i = 0; i < N; ++i) d[i] = -d[i];
                 }
             } else {
-                lin_rel = linear_residual_ratio(Jreg, Fx, d, fnorm_old);
+                md.lin_rel = linear_residual_ratio(Jreg, Fx, d, fnorm_old);
             }
         }
-        if (!ok) {
-            return step_info<R>(SINGULAR, rnorm(Fx), R(0), failures, st.promoted, false);
+        return md;
+    }
+
+    // read_mixed_stall (newton.jl:1437-1451) on the state and the step info.
+    void read_mixed_stall(NState<R, N>& st, StepInfo<R>& info, R fnorm_old, R lin_rel) const {
+        R eta_lin = std::max(std::sqrt(eps_<R>()), R(1e-4));
+        if (info.fnorm > R(0.5) * fnorm_old && lin_rel > eta_lin) {
+            int32_t slow = st.slow_steps + 1;
+            if (slow >= 2) {
+                st.promoted = true;
+                st.slow_steps = 0;
+                info.failures += 1;
+                info.promoted = true;
+            } else {
+                st.promoted = false;
+                st.slow_steps = slow;
+            }
+        } else {
+            st.promoted = false;
+            st.slow_steps = 0;
         }
+    }
+
+    // mixed_step!! + mixed_newton!! with fresh factors and no reuse (newton.jl:1604-1666, 1692-1720).
+    StepInfo<R> mixed_step(Vec<R, N>& x, Vec<R, N>& Fx, NState<R, N>& st) {
+        R fnorm_old = residual_norm(Fx, st);
+        bool was_promoted = st.promoted;
+        MixedDir md = mixed_newton(x, Fx, st, fnorm_old);
+        if (!md.ok) {
+            return step_info<R>(SINGULAR, rnorm(Fx), R(0), md.failures, st.promoted, false);
+        }
+        const bool md_promoted = st.promoted;
         bool promoted_step = md_promoted && !was_promoted;
         // newton_stepkind (newton.jl:1244-1254) with in_working_T = promoted.
-        exact = m.regularization == R(0) && md_promoted;
+        bool exact = m.regularization == R(0) && md_promoted;
         R merit = st.merit;
-        StepInfo<R> info = globalize(x, Fx, d, exact, merit);
-        info.failures += failures;  // bump_failures (newton.jl:1480-1485)
+        StepInfo<R> info = globalize(x, Fx, md.d, exact, merit);
+        info.failures += md.failures;  // bump_failures (newton.jl:1480-1485)
+        const R lin_rel = md.lin_rel;
         if (promoted_step) {
             info.promoted = true;
             st.promoted = true;
@@ -833,31 +909,349 @@ template <class R, int N, class P> struct Solver {
         } else if (was_promoted) {
             // the promoted state keeps stepping in T
         } else {
-            // read_mixed_stall (newton.jl:1437-1451)
-            R eta_lin = std::max(std::sqrt(eps_<R>()), R(1e-4));
-            if (info.fnorm > R(0.5) * fnorm_old && lin_rel > eta_lin) {
-                int32_t slow = st.slow_steps + 1;
-                if (slow >= 2) {
-                    st.promoted = true;
-                    st.slow_steps = 0;
-                    info.failures += 1;
-                    info.promoted = true;
-                } else {
-                    st.promoted = false;
-                    st.slow_steps = slow;
-                }
-            } else {
-                st.promoted = false;
-                st.slow_steps = 0;
-            }
+            read_mixed_stall(st, info, fnorm_old, lin_rel);
         }
         st.merit = merit;
         st.fnorm = info.fnorm;
         return info;
     }
 
+    // ---- DogLeg (src/globalization/dogleg.jl:116-206, newton.jl:1725-1780) ----
+
+    static R dogleg_lo() { return R(1e-6); }
+    static R dogleg_hi() { return R(1e10); }
+    static R dogleg_shrink(R d) { return jmax(R(0.25) * d, dogleg_lo()); }
+    static R dogleg_grow(R d) { return jmin(R(4) * d, dogleg_hi()); }
+    static R dogleg_clamp(R d) { return std::min(std::max(d, dogleg_lo()), dogleg_hi()); }
+
+    // dogleg_step (dogleg.jl:116-206): the trial loop on the model matrix J (the regularised
+    // Jacobian that was factorised) with the Newton step sN (read where `newton`) and the radius
+    // `radius` (0 when unset); x, Fx and the radius are updated.
+    StepInfo<R> dogleg_step(Vec<R, N>& x, Vec<R, N>& Fx, const Mat<R, N>& J, const Vec<R, N>& sN, bool newton,
+                            R& radius) {
+        const R lo = dogleg_lo();
+        const R fnorm = rnorm(Fx);
+        if (fnorm == R(0)) return step_info<R>(SUCCESS, fnorm, R(0), 0, false, false);
+        const R nN = newton ? rnorm(sN) : std::numeric_limits<R>::infinity();
+        Vec<R, N> u, gt, gh;
+        const R finv = R(1) / fnorm;
+        for (int i = 0; i < N; ++i) u[i] = finv * Fx[i];
+        gt = matvec_t<R, N>(J, u);
+        const R gnorm = rnorm(gt);
+        const R ginv = R(1) / gnorm;
+        for (int i = 0; i < N; ++i) gh[i] = ginv * gt[i];
+        const Vec<R, N> Jg = matvec<R, N>(J, gh);
+        const R jnorm = rnorm(Jg);
+        const bool cauchy = gnorm != R(0) && std::isfinite(gnorm) && jnorm != R(0) && std::isfinite(jnorm);
+        const R nC = fnorm * ((gnorm / jnorm) / jnorm);
+        R delta = radius;
+        if (delta == R(0) && (newton || cauchy)) {
+            const R n0 = newton ? nN : nC;
+            delta = n0 < lo ? R(2) * lo : dogleg_clamp(n0);
+        }
+        while (true) {
+            Vec<R, N> sv;
+            bool boundary;
+            if (newton && nN <= delta) {
+                sv = sN;
+                boundary = nN >= delta;
+            } else if (!cauchy) {
+                radius = delta;
+                return step_info<R>(STALLED, fnorm, R(0), 0, false, false);
+            } else if (!newton || nC >= delta) {
+                const R a = -jmin(nC, delta);
+                for (int i = 0; i < N; ++i) sv[i] = a * gh[i];
+                boundary = nC >= delta;
+            } else {
+                Vec<R, N> av, b, bh;
+                const R ac = -(nC / delta);
+                for (int i = 0; i < N; ++i) av[i] = ac * gh[i];
+                for (int i = 0; i < N; ++i) b[i] = R(1) * sN[i] + nC * gh[i];
+                const R binv = R(1) / rnorm(b);
+                for (int i = 0; i < N; ++i) bh[i] = binv * b[i];
+                const R ab = rdot(av, bh);
+                const R c = R(1) - norm2(av);
+                const R ab2 = ab * ab;
+                const R t = c / (ab + std::sqrt(ab2 + c));
+                const R dt = delta * t;
+                for (int i = 0; i < N; ++i) sv[i] = delta * av[i] + dt * bh[i];
+                boundary = true;
+            }
+            const Vec<R, N> Js = matvec<R, N>(J, sv);
+            Vec<R, N> Flin;
+            for (int i = 0; i < N; ++i) Flin[i] = R(1) * Fx[i] + R(1) * Js[i];
+            const R pred = fnorm - rnorm(Flin);
+            Vec<R, N> xp;
+            for (int i = 0; i < N; ++i) xp[i] = x[i] + R(1) * sv[i];
+            const Vec<R, N> Fp = F(xp);
+            const R fnp = rnorm(Fp);
+            const R ared = fnorm - fnp;
+            if (pred > R(0) && ared >= R(1e-4) * pred) {
+                const bool limited = !(newton && nN <= delta);
+                const R rho = ared / pred;
+                if (rho < R(0.1)) {
+                    delta = nN < delta ? jmax(nN, lo) : dogleg_shrink(delta);
+                } else if (rho > R(0.75) && boundary) {
+                    delta = dogleg_grow(delta);
+                }
+                const R stepnorm = rnorm(sv);
+                x = xp;
+                Fx = Fp;
+                radius = delta;
+                return step_info<R>(SUCCESS, fnp, stepnorm, 0, false, limited);
+            }
+            if (delta == lo) {
+                radius = delta;
+                return step_info<R>(LINESEARCH_FAILED, fnorm, R(0), 1, false, false);
+            }
+            delta = dogleg_shrink(delta);
+        }
+    }
+
+    // newton_step!! with DogLeg in the working precision (newton.jl:1725-1737) and
+    // dogleg_direction (newton.jl:1741-1746): a failed factor, or a Newton step of zero or
+    // non-finite norm, leaves the Cauchy step.
+    StepInfo<R> dogleg_plain_step(Vec<R, N>& x, Vec<R, N>& Fx, NState<R, N>& st) {
+        Mat<R, N> Jreg;
+        R gr, jn;
+        StaticLU<R, N> fac = factor<R>(x, Fx, st, Jreg, gr, jn);
+        Vec<R, N> d = Fx;
+        bool newton = false;
+        if (fac.info <= 0) {
+            d = ldiv<R, R, N>(fac, Fx);
+            for (int i = 0; i < N; ++i) d[i] = -d[i];
+            bool failed;
+            d = refine<R>(d, fac, Jreg, Fx, failed);
+            const R nd = rnorm(d);
+            newton = std::isfinite(nd) && nd != R(0);
+        }
+        StepInfo<R> info = dogleg_step(x, Fx, Jreg, d, newton, st.radius);
+        set_diag(st, gr, jn);
+        st.fnorm = info.fnorm;
+        return info;
+    }
+
+    // mixed_step!! with DogLeg (newton.jl:1757-1780): the direction of mixed_newton!!, the trial loop
+    // on the Jacobian it factorised, and the mixed-precision reading of the step.
+    StepInfo<R> dogleg_mixed_step(Vec<R, N>& x, Vec<R, N>& Fx, NState<R, N>& st) {
+        const R fnorm_old = residual_norm(Fx, st);
+        const bool was_promoted = st.promoted;
+        MixedDir md = mixed_newton(x, Fx, st, fnorm_old);
+        const bool promoted_step = st.promoted && !was_promoted;
+        Vec<R, N> d = Fx;
+        bool newton = false;
+        if (md.ok) {
+            d = md.d;
+            const R nd = rnorm(md.d);
+            newton = std::isfinite(nd) && nd != R(0);
+        }
+        StepInfo<R> info = dogleg_step(x, Fx, md.Jreg, d, newton, st.radius);
+        info.failures += md.failures;
+        if (promoted_step) {
+            info.promoted = true;
+            st.promoted = true;
+            st.slow_steps = 0;
+        } else if (!was_promoted) {
+            read_mixed_stall(st, info, fnorm_old, md.lin_rel);
+        }
+        st.fnorm = info.fnorm;
+        return info;
+    }
+
+    // ---- reuse = k (newton.jl:1781-1874) ----
+
+    // fresh_age / next_age (newton.jl:629-633).
+    int32_t fresh_age() const { return m.reuse == 1 ? 0 : 1; }
+    int32_t next_age(const NState<R, N>& st) const { return st.age + 1 >= m.reuse ? 0 : st.age + 1; }
+
+    // reuse_result (newton.jl:1814-1820): the rule state after a step along a reused or fresh factor.
+    void set_rule(NState<R, N>& st, uint8_t code, int32_t age_next, bool reused) const {
+        st.age = code == SINGULAR ? 0 : age_next;
+        st.stalled = code == SINGULAR ? false : code == STALLED;
+        st.reused = reused;
+    }
+
+    // newton_step!! with reuse in the working precision (newton.jl:1781-1810). A step along the held
+    // factor refines with the JVP at x; a failed refinement (one step-rule failure) or a line search
+    // that fails without a step falls back to a fresh factor within the step.
+    StepInfo<R> plain_step_reuse(Vec<R, N>& x, Vec<R, N>& Fx, NState<R, N>& st) {
+        int32_t failures = 0;
+        if (!(st.age == 0 || st.stalled)) {
+            const int32_t age_next = next_age(st);
+            Vec<R, N> d = ldiv<R, R, N>(st.facR, Fx);
+            for (int i = 0; i < N; ++i) d[i] = -d[i];
+            bool ref_failed = false;
+            if (m.refinement) d = refine_with<R>(d, st.facR, nullptr, x, Fx, ref_failed);
+            if (ref_failed) {
+                failures = 1;
+            } else {
+                Vec<R, N> x1 = x, F1 = Fx;
+                R merit = st.merit;
+                // newton_line with fresh = false: MeasuredSlope (newton.jl:1406-1415, 1244-1254)
+                StepInfo<R> info = globalize(x1, F1, d, false, merit);
+                if (!(info.code == LINESEARCH_FAILED && info.stepnorm == R(0))) {
+                    x = x1;
+                    Fx = F1;
+                    st.merit = merit;
+                    st.fnorm = info.fnorm;
+                    set_rule(st, info.code, age_next, true);
+                    return info;
+                }
+            }
+        }
+        Mat<R, N> Jreg;
+        R gr, jn;
+        StaticLU<R, N> fac = factor<R>(x, Fx, st, Jreg, gr, jn);
+        StepInfo<R> info;
+        R merit = st.merit;
+        if (fac.info > 0) {
+            info = step_info<R>(SINGULAR, rnorm(Fx), R(0), 0, false, false);
+        } else {
+            Vec<R, N> d = ldiv<R, R, N>(fac, Fx);
+            for (int i = 0; i < N; ++i) d[i] = -d[i];
+            bool failed;
+            d = refine<R>(d, fac, Jreg, Fx, failed);
+            bool exact = m.regularization == R(0) && !m.refinement;
+            info = globalize(x, Fx, d, exact, merit);
+        }
+        info.failures += failures;  // bump_failures (newton.jl:1480-1485)
+        set_diag(st, gr, jn);
+        st.facR = fac;
+        st.merit = merit;
+        st.fnorm = info.fnorm;
+        set_rule(st, info.code, fresh_age(), false);
+        return info;
+    }
+
+    // mixed_step!! with reuse (newton.jl:1823-1874) and mixed_newton!! with store = true
+    // (newton.jl:1604-1666): a promoted state factorises in T at every step; otherwise the step runs
+    // along the held FP32 factor until its age or a stall refreshes it, and a reused direction whose
+    // refinement fails is refactorised in FP32 at x (one failure) before any promotion.
+    StepInfo<R> mixed_step_reuse(Vec<R, N>& x, Vec<R, N>& Fx, NState<R, N>& st) {
+        const R fnorm_old = residual_norm(Fx, st);
+        const bool state_promoted = st.promoted;
+        bool fresh = st.promoted || st.age == 0 || st.stalled;
+        const int32_t age_fresh = fresh_age();
+        int32_t age_next = fresh ? age_fresh : next_age(st);
+        while (true) {
+            const bool was_fresh = fresh;
+            // --- mixed_newton!! ---
+            Vec<R, N> d{};
+            bool ok = true, md_fresh = true;
+            int32_t failures = 0;
+            R lin_rel = 0;
+            bool have = false;
+            Mat<R, N> Jreg;
+            R gr, jn;
+            if (st.promoted) {
+                StaticLU<R, N> fac = factor<R>(x, Fx, st, Jreg, gr, jn);
+                set_diag(st, gr, jn);
+                if (fac.info > 0) {
+                    ok = false;
+                } else {
+                    d = ldiv<R, R, N>(fac, Fx);
+                    for (int i = 0; i < N; ++i) d[i] = -d[i];
+                    bool failed;
+                    d = refine<R>(d, fac, Jreg, Fx, failed);
+                }
+                have = true;
+            } else if (!fresh) {
+                d = ldiv<float, R, N>(st.facF, Fx);
+                for (int i = 0; i < N; ++i) d[i] = -d[i];
+                bool ref_failed = false;
+                if (m.refinement)
+                    d = refine_with<float>(d, st.facF, nullptr, x, Fx, ref_failed);
+                if (!ref_failed) {
+                    md_fresh = false;
+                    have = true;
+                }
+            }
+            if (!have) {
+                StaticLU<float, N> fac = factor<float>(x, Fx, st, Jreg, gr, jn);
+                if (!fresh) failures += 1;
+                bool ref_failed = false;
+                if (fac.info > 0) {
+                    ref_failed = true;
+                } else {
+                    d = ldiv<float, R, N>(fac, Fx);
+                    for (int i = 0; i < N; ++i) d[i] = -d[i];
+                    d = refine<float>(d, fac, Jreg, Fx, ref_failed);
+                }
+                set_diag(st, gr, jn);
+                if (ref_failed || fac.info > 0) {
+                    failures += 1;
+                    // promote_factor!! (newton.jl:1332-1339): refactorise Jreg in T
+                    StaticLU<R, N> facT = lu_factorize<R, R, N>(Jreg);
+                    st.promoted = true;
+                    if (facT.info > 0) {
+                        ok = false;
+                    } else {
+                        d = ldiv<R, R, N>(facT, Fx);
+                        for (int i = 0; i < N; ++i) d[i] = -d[i];
+                    }
+                } else {
+                    st.facF = fac;
+                    lin_rel = linear_residual_ratio(Jreg, Fx, d, fnorm_old);
+                }
+            }
+            fresh = md_fresh;
+            if (!ok) {
+                StepInfo<R> info = step_info<R>(SINGULAR, rnorm(Fx), R(0), failures, st.promoted, false);
+                set_rule(st, SINGULAR, 0, !md_fresh);
+                return info;
+            }
+            const bool promoted_step = st.promoted && !state_promoted;
+            if (promoted_step) {
+                age_next = 0;
+            } else if (md_fresh && !was_fresh) {
+                age_next = age_fresh;
+            }
+            // newton_stepkind(prob, m, T, md.fresh, promoted) (newton.jl:1244-1254)
+            const bool exact = md_fresh && m.regularization == R(0) && st.promoted;
+            Vec<R, N> x1 = x, F1 = Fx;
+            R merit = st.merit;
+            StepInfo<R> info = globalize(x1, F1, d, exact, merit);
+            if (!md_fresh && info.code == LINESEARCH_FAILED && info.stepnorm == R(0)) {
+                fresh = true;
+                age_next = age_fresh;
+                continue;
+            }
+            x = x1;
+            Fx = F1;
+            info.failures += failures;
+            if (promoted_step) {
+                info.promoted = true;
+                st.promoted = true;
+                st.slow_steps = 0;
+            } else if (md_fresh && !st.promoted) {
+                // read_mixed_stall (newton.jl:1437-1451)
+                R eta_lin = std::max(std::sqrt(eps_<R>()), R(1e-4));
+                if (info.fnorm > R(0.5) * fnorm_old && lin_rel > eta_lin) {
+                    int32_t slow = st.slow_steps + 1;
+                    if (slow >= 2) {
+                        st.promoted = true;
+                        st.slow_steps = 0;
+                        info.failures += 1;
+                        info.promoted = true;
+                    } else {
+                        st.slow_steps = slow;
+                    }
+                } else {
+                    st.slow_steps = 0;
+                }
+            }
+            st.merit = merit;
+            st.fnorm = info.fnorm;
+            set_rule(st, info.code, age_next, !md_fresh);
+            return info;
+        }
+    }
+
     StepInfo<R> step(Vec<R, N>& x, Vec<R, N>& Fx, NState<R, N>& st) {
-        StepInfo<R> info = m.factor_f32 ? mixed_step(x, Fx, st) : plain_step(x, Fx, st);
+        StepInfo<R> info = m.glob == G_DOGLEG
+                               ? (m.factor_f32 ? dogleg_mixed_step(x, Fx, st) : dogleg_plain_step(x, Fx, st))
+                           : m.reuse > 0 ? (m.factor_f32 ? mixed_step_reuse(x, Fx, st) : plain_step_reuse(x, Fx, st))
+                                         : (m.factor_f32 ? mixed_step(x, Fx, st) : plain_step(x, Fx, st));
         // step!! keeps the best iterate (newton.jl:1671-1675, take_best :770-773)
         if (m.best && info.fnorm < st.best_fnorm) {
             st.best_x = x;
@@ -874,7 +1268,8 @@ template <class R, int N, class P> struct Solver {
         r.rank = st.rank;
         r.nullnorm = st.nullnorm;
         r.gradratio = st.gradratio;
-        r.reused = false;
+        // reused_factor (newton.jl:1188-1191)
+        r.reused = m.reuse > 0 && st.reused && s.iterations > 0;
         // restore_best / should_restore_best (newton.jl:792-799, composition/anderson.jl:503-505)
         if (m.best && r.code != SUCCESS && std::isfinite(st.best_fnorm) && st.best_fnorm < r.fnorm) {
             x = st.best_x;
@@ -884,11 +1279,9 @@ template <class R, int N, class P> struct Solver {
     }
 
     // solve!! (src/nonlinear/solver.jl:277-378) for a square NonlinearProblem.
-    Status<R> solve(Vec<R, N>& x) {
-        Vec<R, N> Fx = F(x);
-        R fnorm0 = rnorm(Fx);
+    // newton_start!! (newton.jl:1135-1144) and reset_rule (newton.jl:675-686).
+    NState<R, N> start(const Vec<R, N>& x, const Vec<R, N>& Fx, R fnorm0) const {
         NState<R, N> st;
-        // newton_start!! (newton.jl:1135-1144)
         st.merit = m.glob == G_BACKTRACKING ? norm2(Fx) : nan_<R>();
         st.fnorm = fnorm0;
         st.promoted = false;
@@ -899,6 +1292,14 @@ template <class R, int N, class P> struct Solver {
         st.jnorm = 0;
         st.best_x = x;
         st.best_fnorm = fnorm0;
+        st.radius = m.dl_radius;  // the radius each solve starts from
+        return st;
+    }
+
+    Status<R> solve(Vec<R, N>& x) {
+        Vec<R, N> Fx = F(x);
+        R fnorm0 = rnorm(Fx);
+        NState<R, N> st = start(x, Fx, fnorm0);
         Status<R> status0;
         status0.code = SUCCESS;
         status0.iterations = 0;
@@ -919,6 +1320,9 @@ template <class R, int N, class P> struct Solver {
             StepInfo<R> info = step(x, Fx, st);
             Status<R> recorded = record(status, info);
             status = recorded;
+            // observe_rate!! (newton.jl:1205-1210): a step of rate θ ≥ 1 along a reused factor
+            // refreshes the factor at the next step
+            if (m.reuse > 0 && st.reused && recorded.rate >= R(1)) st.stalled = true;
             if (info.code == SINGULAR || info.code == NONFINITE || info.code == MAXITERS)
                 return finish(x, st, status);
             if (info.code == LINESEARCH_FAILED && info.stepnorm == R(0)) return finish(x, st, status);

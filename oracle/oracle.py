@@ -61,6 +61,10 @@ def lib():
         L.orc_givens.argtypes = [ctypes.c_double, ctypes.c_double, vp]
         L.orc_solve_fixture.restype = ctypes.c_int
         L.orc_solve_fixture.argtypes = [ctypes.c_int, vp, vp, vp, vp, vp]
+        L.orc_dogleg_pw_step.restype = ctypes.c_int
+        L.orc_dogleg_pw_step.argtypes = [ctypes.c_int, vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, vp]
+        L.orc_solve_fixture_counts.restype = ctypes.c_int
+        L.orc_solve_fixture_counts.argtypes = [ctypes.c_int, vp, vp, vp, vp, vp, vp]
         _LIB = L
     return _LIB
 
@@ -173,15 +177,20 @@ def rnorm(a) -> float:
     return lib().orc_rnorm(_ptr(a), len(a))
 
 
-def solve_fixture(fixture: int, method, options, x, p=(0.0, 0.0)):
-    """One solve of a loop-test fixture (101 y.^2 − q, 102 2y − q, 103 x² − 4 with J = Inf)."""
+def solve_fixture(fixture: int, method, options, x, p=(0.0, 0.0), counts: bool = False):
+    """One solve of a loop-test fixture (101 y.^2 − q, 102 2y − q, 103 x² − 4 with J = Inf); with
+    `counts`, also (residual calls, Jacobian calls)."""
     x = np.ascontiguousarray(np.atleast_1d(np.asarray(x, dtype=np.float64))).copy()
     pp = np.ascontiguousarray(np.asarray(p, dtype=np.float64))
     S = np.zeros(1, dtype=_status_dtype())
+    c = np.zeros(2, dtype=np.int64)
     cm, co = method.to_c(), options.to_c()
-    rc = lib().orc_solve_fixture(fixture, ctypes.byref(cm), ctypes.byref(co), _ptr(x), _ptr(pp), _ptr(S))
+    rc = lib().orc_solve_fixture_counts(fixture, ctypes.byref(cm), ctypes.byref(co), _ptr(x), _ptr(pp), _ptr(S),
+                                        _ptr(c))
     if rc != 0:
         raise RuntimeError(f"oracle fixture failed with {rc}")
+    if counts:
+        return x, S[0], (int(c[0]), int(c[1]))
     return x, S[0]
 
 
@@ -237,3 +246,15 @@ def givens(f: float, g: float):
 
 def max_threads() -> int:
     return lib().orc_max_threads()
+
+
+def dogleg_pw_step(kind: int, method, p0: float = 0.0, p1: float = 0.0, x0: float = 0.0) -> dict:
+    """One DogLeg step from x0 on the one-dimensional residual `kind` of the reference's DogLeg tests
+    (0: p0 at 0 and p1 elsewhere; 1: −10 at 0, −10.5 at x ≥ 5, x − 10 otherwise; 2: x − 10), J = 1."""
+    out = np.zeros(7)
+    cm = method.to_c()
+    rc = lib().orc_dogleg_pw_step(kind, ctypes.byref(cm), float(p0), float(p1), float(x0), _ptr(out))
+    if rc != 0:
+        raise RuntimeError(f"oracle DogLeg step failed with {rc}")
+    return dict(x=out[0], radius=out[1], code=int(out[2]), failures=int(out[3]), stepnorm=out[4], f_calls=int(out[5]),
+                limited=bool(out[6]))

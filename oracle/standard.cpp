@@ -2,9 +2,9 @@
 //
 // The 23 problems of the reference's standard test set (test/helpers/standard_problems.jl:55-386,
 // after Moré, Garbow and Hillstrom 1981) restated as templates over the scalar type, with a
-// forward-mode dual number for the Jacobian, as the reference's cells take it from ForwardDiff
-// (test/helpers/standard_cells.jl:128-148). `orc_standard_cell` runs one cell of the columns
-// "Newton Static LU" and "Newton Backtracking LU" with `max_iterations = 200` and default Options
+// forward-mode dual number for the Jacobian and the JVP, as the reference's cells take them from
+// ForwardDiff (test/helpers/standard_cells.jl:128-148). `orc_standard_cell` runs one cell of the
+// Newton/LU columns of the table with `max_iterations = 200` and default Options
 // (standard_cells.jl:21, 33-35), so that tests/ can compare the oracle's solve!! loop with the
 // golden table test/verification/standard_reference.csv (code, iterations, f_evals, fnorm).
 #include <cmath>
@@ -89,6 +89,20 @@ template <class T> inline T val(T x) { return x; }
 template <class T, int NP> inline T val(const Dual<T, NP>& a) { return a.v; }
 template <class S> inline S sq(const S& x) { return x * x; }
 template <class S> inline S cube(const S& x) { return (x * x) * x; }
+// x^2 and x^3 of a dual: ForwardDiff's rule for an integer power, `partials(x) * y * v^(y-1)`
+// evaluated left to right (zero partials stay zero), with the value v^y of Base (v·v, (v·v)·v).
+template <class T, int NP> inline Dual<T, NP> int_power(const Dual<T, NP>& a, T value, T y, T vpow) {
+    Dual<T, NP> r(value);
+    bool zero = true;
+    for (int i = 0; i < NP; ++i) zero = zero && a.p[i] == T(0);
+    if (!zero)
+        for (int i = 0; i < NP; ++i) r.p[i] = (a.p[i] * y) * vpow;
+    return r;
+}
+template <class T, int NP> inline Dual<T, NP> sq(const Dual<T, NP>& a) { return int_power(a, a.v * a.v, T(2), a.v); }
+template <class T, int NP> inline Dual<T, NP> cube(const Dual<T, NP>& a) {
+    return int_power(a, (a.v * a.v) * a.v, T(3), a.v * a.v);
+}
 
 // Scalar constants in the element type: C(c) converts a double literal, as T(c) in the reference.
 #define C(c) S(T(c))
@@ -352,6 +366,20 @@ template <class T, int N> struct StandardProblem {
         standard_F<T, T>(prob, N, x.data(), out.data());
         return out;
     }
+    // jvp!! with AutoForwardDiff (src/ad/jvp.jl:96-100): F at a dual iterate with one partial,
+    // seeded along v; the partials of the residual are J(x) v. No primal F call is counted.
+    Vec<T, N> Jv(const Vec<T, N>& x, const Vec<T, N>& v) const {
+        using D = Dual<T, 1>;
+        D xd[N], od[N];
+        for (int j = 0; j < N; ++j) {
+            xd[j] = D(x[j]);
+            xd[j].p[0] = v[j];
+        }
+        standard_F<T, D>(prob, N, xd, od);
+        Vec<T, N> out;
+        for (int i = 0; i < N; ++i) out[i] = od[i].p[0];
+        return out;
+    }
     // The Jacobian by one forward pass of N partials (src/ad/forwarddiff_static.jl:50-55).
     Mat<T, N> J(const Vec<T, N>& x) const {
         using D = Dual<T, N>;
@@ -376,7 +404,10 @@ static int run_cell(int prob, int scale, int column, int* code, int* iterations,
     for (int i = 0; i < N; ++i) x[i] = T(scale) * u[i];
     StandardProblem<T, N> p{prob};
     Method<T> m;
-    m.glob = column == 0 ? G_STATIC : G_BACKTRACKING;
+    // columns: 0 "Newton Static LU", 1 "Newton Backtracking LU", 2 "Newton reuse 3 Backtracking LU"
+    // and 3 "Newton DogLeg LU" (standard_cells.jl:33-49)
+    m.glob = column == 0 ? G_STATIC : column == 3 ? G_DOGLEG : G_BACKTRACKING;
+    if (column == 2) m.reuse = 3;
     Options<T> o;
     o.max_iterations = 200;
     Solver<T, N, StandardProblem<T, N>> s(p, m, o);
@@ -409,7 +440,8 @@ static int dispatch_cell(int prob, int scale, int column, int* code, int* iterat
 extern "C" {
 
 // One cell: problem 1..23, start scale 1/10/100, column 0 = "Newton Static LU",
-// 1 = "Newton Backtracking LU", eltype 32 or 64. Writes the code, iterations, primal F calls and
+// 1 = "Newton Backtracking LU", 2 = "Newton reuse 3 Backtracking LU", 3 = "Newton DogLeg LU",
+// eltype 32 or 64. Writes the code, iterations, primal F calls and
 // ‖F‖; xout (length n, may be NULL) receives the final iterate.
 int orc_standard_cell(int prob, int scale, int column, int eltype, int* code, int* iterations,
                       long* f_evals, double* fnorm, double* xout) {

@@ -1,15 +1,16 @@
 """Extract the golden rows of the reference's standard table used to pin the oracle.
 
 Reads test/verification/standard_reference.csv of the reference (a data file of its own tests,
-written by scripts/standard/update_reference.jl) and keeps the columns "Newton Static LU" and
-"Newton Backtracking LU", the two Newton/LU columns whose solve the oracle restates. Run from the
-repository root: python tests/golden/extract_standard.py /root/reference
+written by scripts/standard/update_reference.jl) and keeps the Newton/LU columns whose solve the
+oracle restates: "Newton Static LU", "Newton Backtracking LU", "Newton reuse 3 Backtracking LU"
+and "Newton DogLeg LU". Run from the repository root:
+python tests/golden/extract_standard.py /root/reference
 """
 import csv
 import sys
 from pathlib import Path
 
-COLUMNS = ("Newton Static LU", "Newton Backtracking LU")
+COLUMNS = ("Newton Static LU", "Newton Backtracking LU", "Newton reuse 3 Backtracking LU", "Newton DogLeg LU")
 
 
 def main(ref_root: str) -> None:

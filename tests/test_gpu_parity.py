@@ -105,6 +105,17 @@ HH_METHODS = {
     "mixed-static-norefine": lambda gs: gs.Newton(linsolve=gs.LUFactorization("float32")),
     "plain-refine": lambda gs: gs.Newton(refinement=gs.Refinement()),
     "best-regularized": lambda gs: gs.Newton(best=True, regularization=1e-3),
+    # reuse = k: the held factor (newton.jl:1781-1874)
+    "reuse1-static": lambda gs: gs.Newton(reuse=1),
+    "reuse2-static": lambda gs: gs.Newton(reuse=2),
+    "reuse3-backtracking": lambda gs: gs.Newton(reuse=3, globalization=gs.Backtracking()),
+    "reuse3-refine-regularized": lambda gs: gs.Newton(reuse=3, refinement=gs.Refinement(), regularization=1e-3),
+    "reuse3-mixed-static-refine": lambda gs: gs.Newton(reuse=3, linsolve=gs.LUFactorization("float32"),
+                                                       refinement=gs.Refinement()),
+    "reuse2-mixed-backtracking-refine": lambda gs: gs.Newton(reuse=2, linsolve=gs.LUFactorization("float32"),
+                                                             globalization=gs.Backtracking(),
+                                                             refinement=gs.Refinement()),
+    "reuse4-mixed-static-norefine": lambda gs: gs.Newton(reuse=4, linsolve=gs.LUFactorization("float32")),
 }
 
 
@@ -130,6 +141,40 @@ def test_henon_heiles_hard_steps(gs, orc, workloads, device):
             x, st, xo, so = run_both(gs, orc, device, m, gs.HenonHeilesMidpoint(), xk * 3.0, gs.BatchParams(2.0, xk * 3.0), opts)
             assert_parity(st, x, so, xo, bitwise=True)
             assert len(np.unique(st["code"])) >= 2
+
+
+def test_reuse_hard_steps(gs, orc, workloads, device):
+    """Newton(; reuse = k) where the held factor matters: large steps (h = 2) from far-out starts, so
+    that steps along a stale factor stall, refresh on θ ≥ 1, or fail their search and refactorise."""
+    n = 16384
+    xk = workloads.henon_heiles_states(n, seed=7, e_lo=0.1, e_hi=0.166)
+    f32 = gs.LUFactorization("float32")
+    methods = (gs.Newton(reuse=3), gs.Newton(reuse=3, globalization=gs.Backtracking()),
+               gs.Newton(reuse=2, linsolve=f32, refinement=gs.Refinement()),
+               gs.Newton(reuse=3, linsolve=f32, globalization=gs.Backtracking(), refinement=gs.Refinement()),
+               gs.Newton(reuse=5, refinement=gs.Refinement(), regularization=1e-2, globalization=gs.Backtracking()))
+    for m in methods:
+        for opts in (gs.Options(f_abstol=1e-12, f_reltol=0.0, max_iterations=30),
+                     gs.Options(norm=gs.MaxNorm(), min_iterations=1, x_reltol=1e-6, roundoff=True)):
+            x, st, xo, so = run_both(gs, orc, device, m, gs.HenonHeilesMidpoint(), xk * 3.0, gs.BatchParams(2.0, xk * 3.0),
+                                     opts)
+            assert_parity(st, x, so, xo, bitwise=True)
+            assert st["reused"].any() and len(np.unique(st["code"])) >= 2
+
+
+def test_reuse_coupled_batch(gs, orc, device):
+    """test/nonlinear/batched.jl:263-352 with Newton(; reuse = 2): bit-identical to the oracle, the
+    exact root in 0 steps, and more steps than a fresh factor per step on systems 3 and 4."""
+    P = np.array([[2.125, 1.5], [1.25, 2.0], [1.5, 2.5], [1.75, 3.0]])
+    X0 = np.array([[2.0, 1.0], [2.5, 1.5], [3.5, 2.5], [4.0, 3.0]])
+    opts = gs.Options(max_iterations=150)
+    _, sn = run_gpu(gs, device, gs.Newton(), gs.Coupled2(), X0.copy(), gs.PerSystem(P), opts)
+    x, st, xo, so = run_both(gs, orc, device, gs.Newton(reuse=2), gs.Coupled2(), X0.copy(), gs.PerSystem(P), opts)
+    assert_parity(st, x, so, xo, bitwise=True)
+    assert (st["code"] == gs.SUCCESS).all() and st["iterations"][0] == 0 and np.array_equal(x[0], X0[0])
+    assert st["iterations"][2] > sn["iterations"][2] and st["iterations"][3] > sn["iterations"][3]
+    _, counts = run_gpu(gs, device, gs.Newton(reuse=2), gs.Coupled2(), X0.copy(), gs.PerSystem(P), opts, counts=True)
+    assert counts == dict(SUCCESS=4, STALLED=0, MAXITERS=0, SINGULAR=0, NONFINITE=0, LINESEARCH_FAILED=0)
 
 
 def test_full_size_henon_heiles(gs, orc, workloads, device):

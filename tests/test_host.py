@@ -189,3 +189,33 @@ def test_least_squares_entry_point_checks_method_without_gpu(gs):
     assert rc == abi.GS_EFAMILY
     rc = lib.gs_lsq_set_shape(5, 256, 1)
     assert rc == abi.GS_EINVAL
+
+
+def test_reuse_method_and_entry_checks(gs):
+    """Newton(; reuse = k) crosses the ABI as gs_method.reuse; the constructor's refusals
+    (newton.jl:60-90) and the entry points' refusal of reuse where no held factor exists (the
+    least-squares batch, the wave-per-system FpuGl3 solve) happen before any device call."""
+    assert gs.Newton(reuse=3).to_c().reuse == 3 and gs.Newton().to_c().reuse == 0
+    assert gs.Newton(reuse=2, keep_factor=True).to_c().reuse == 2
+    with pytest.raises(gs.ArgumentError, match="reuse must be ≥ 1"):
+        gs.Newton(reuse=0)
+    with pytest.raises(gs.ArgumentError, match="LevenbergMarquardt globalization takes no reuse"):
+        gs.Newton(globalization=gs.LevenbergMarquardt(), reuse=2)
+    lib = gs.load_library()
+    from importlib import import_module
+
+    abi = import_module(gs.__name__ + "._abi")
+    m = abi.GsMethod.from_buffer_copy(gs.Newton(globalization=gs.LevenbergMarquardt()).to_c())
+    m.reuse = 2
+    o = gs.Options().to_c()
+    rc = lib.gs_solve_expsum_lsq_f64(ctypes.byref(m), 1, None, None, None, None, None, 0, None, ctypes.byref(o), None)
+    assert rc == abi.GS_EINVAL and "no reuse" in abi.last_error(lib)
+    fam = abi.GsFamily()
+    fam.id, fam.n, fam.m, fam.p_len = abi.GS_FAM_FPU_GL3, 48, 48, 32
+    rc = lib.gs_solve_batch_f64(ctypes.byref(gs.Newton(reuse=2).to_c()), ctypes.byref(fam), 1, None, None, 0, None,
+                                None, None, ctypes.byref(o), 0, None)
+    assert rc == abi.GS_EINVAL  # neither S nor codes: refused first, before the family checks
+    codes = (ctypes.c_uint8 * 1)()
+    rc = lib.gs_solve_batch_f64(ctypes.byref(gs.Newton(reuse=2).to_c()), ctypes.byref(fam), 1, None, None, 0, None,
+                                codes, None, ctypes.byref(o), 0, None)
+    assert rc == abi.GS_EFAMILY and "no reuse" in abi.last_error(lib)
