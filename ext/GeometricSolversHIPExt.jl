@@ -19,7 +19,7 @@ module GeometricSolversHIPExt
 
 using AMDGPU: ROCVector, ROCArray, stream
 using GeometricSolvers
-using GeometricSolvers: Options, SolverStatus, ReturnCode, L2Norm, MaxNorm, Static, Backtracking,
+using GeometricSolvers: Options, SolverStatus, ReturnCode, L2Norm, MaxNorm, Static, Backtracking, DogLeg,
                         Newton, NoRefinement, Refinement, LUFactorization, NonlinearProblem, PerSystem,
                         BatchParams, _validate_batch, _validate_batch_problem, _validate_workgroupsize,
                         resolve_options, _tally_return_codes, ToReal
@@ -61,6 +61,7 @@ struct GsMethod
     lm::NTuple{6, Float64}
     best::Int32
     reuse::Int32
+    dl_radius::Float64
 end
 
 struct GsFamily
@@ -183,16 +184,18 @@ device_method(m) = nothing  # not served: the generic method runs
 function device_method(m::Newton{<:LUFactorization{F}, G, Nothing, RF, Float64, K}) where {
         F, G, RF, K <: Union{Nothing, Int32}}
     F === Nothing || F === Float32 || return nothing
-    G <: Union{Static, Backtracking} || return nothing
+    G <: Union{Static, Backtracking, DogLeg} || return nothing
     RF <: Union{NoRefinement, Refinement} || return nothing
     g = m.globalization
     bt = g isa Backtracking ? g : Backtracking()
     rf = m.refinement isa Refinement ? m.refinement : Refinement()
-    return GsMethod(Int32(0), F === Float32 ? Int32(1) : Int32(0), g isa Backtracking ? Int32(1) : Int32(0),
+    glob = g isa Backtracking ? Int32(1) : g isa DogLeg ? Int32(3) : Int32(0)
+    radius = g isa DogLeg && g.radius !== nothing ? Float64(g.radius) : 0.0
+    return GsMethod(Int32(0), F === Float32 ? Int32(1) : Int32(0), glob,
         m.refinement isa Refinement ? Int32(1) : Int32(0), g isa Static ? Float64(g.α) : 1.0,
         Float64(bt.c₁), Float64(bt.p), Int32(bt.maxiter), Int32(rf.maxiter), Float64(rf.η),
         Float64(m.regularization), ntuple(_ -> 0.0, 6), m.best === Val(true) ? Int32(1) : Int32(0),
-        m.reuse === nothing ? Int32(0) : m.reuse)
+        m.reuse === nothing ? Int32(0) : m.reuse, radius)
 end
 
 function device_options(o::Options{Float64})

@@ -49,6 +49,7 @@ DevParams dev_params(const gs_method& method, const gs_options_f64& options) {
     q.mu = method.regularization;
     q.best = method.best != 0;
     q.reuse = method.reuse;
+    q.dl_radius = method.dl_radius;
     return q;
 }
 
@@ -125,9 +126,17 @@ int gs_solve_batch_f64(const gs_method* method, const gs_family* family, int64_t
     if (family->n != fs->n || family->m != fs->n)
         return fail(GS_EINVAL, "family " + std::to_string(family->id) + " has n = m = " + std::to_string(fs->n));
     if (method->linsolve != GS_LU) return fail(GS_EFAMILY, "only LUFactorization has a device form");
-    if (method->globalization != GS_STATIC && method->globalization != GS_BACKTRACKING)
+    if (method->globalization != GS_STATIC && method->globalization != GS_BACKTRACKING &&
+        method->globalization != GS_DOGLEG)
         return fail(GS_EFAMILY, "globalization " + std::to_string(method->globalization) +
                                     " has no lane-per-system form for this family");
+    if (method->globalization == GS_DOGLEG) {
+        // the DogLeg constructor's checks (dogleg.jl:44-60): a radius stored clamped, or 0 (nothing)
+        const double r = method->dl_radius;
+        if (!(r == 0.0 || (r >= 1e-6 && r <= 1e10)))
+            return fail(GS_EINVAL, "the DogLeg radius must be 0 (nothing) or in [1e-6, 1e10]");
+        if (method->reuse != 0) return fail(GS_EINVAL, "Newton with a DogLeg globalization takes no reuse");
+    }
     if (family->id == GS_FAM_FPU_GL3 && method->globalization != GS_STATIC)
         return fail(GS_EFAMILY, "the wave-per-system solve of FpuGl3 takes the Static globalization");
     if (method->reuse < 0) return fail(GS_EINVAL, "reuse must be ≥ 1, or 0 for nothing");
@@ -197,7 +206,9 @@ int gs_solve_batch_f64(const gs_method* method, const gs_family* family, int64_t
         timing_end(timed);
         return GS_OK;
     }
-    if (method->reuse > 0)
+    if (glob == GS_DOGLEG)
+        rc = launch_small_dogleg(family->id, a, mixed, refine, diag, wg, st);
+    else if (method->reuse > 0)
         rc = launch_small_reuse(family->id, glob, a, mixed, refine, diag, wg, st);
     else if (glob == GS_STATIC)
         rc = launch_small_family<kStatic, false>(family->id, a, mixed, refine, diag, wg, st);

@@ -122,4 +122,7 @@ int launch_small_family(int family, const KernelArgs& a, bool mixed, bool refine
 int launch_small_reuse(int family, int glob, const KernelArgs& a, bool mixed, bool refine, bool diag, int wg,
                        hipStream_t st);
 
+// Defined in small_dogleg.hip: DogLeg.
+int launch_small_dogleg(int family, const KernelArgs& a, bool mixed, bool refine, bool diag, int wg, hipStream_t st);
+
 }  // namespace gs

@@ -37,9 +37,10 @@ struct DevParams {
     double mu;             // regularization
     int32_t best;
     int32_t reuse;         // Newton(; reuse = k): k ≥ 1 (the REUSE kernels only)
+    double dl_radius;      // DogLeg(; radius): the radius each solve starts from, 0 = unset
 };
 
-enum { kStatic = 0, kBacktracking = 1 };
+enum { kStatic = 0, kBacktracking = 1, kDogLeg = 2 };
 
 // The outcome of one step (StepInfo, src/base/status.jl:220-234; `limited` is always false here).
 struct Step {
@@ -47,6 +48,7 @@ struct Step {
     bool promoted, shortened;
     int32_t failures;
     double fnorm, stepnorm;
+    bool limited;  // a trust-region radius, not the distance to a root, set the step's norm
 };
 
 // ------------------------------------------------------------------------------------------------
@@ -274,6 +276,8 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
     Factor<TF, N> kept;
     int age;
     bool stalled, reused;
+    // DogLeg: the trust radius (TrustRadius, newton.jl:637-640), 0 until the first step sets it
+    double radius;
 
     GS_DEV SmallSolve(const Fam& f, const DevParams& p) : fam(f), q(p) {}
 
@@ -509,16 +513,16 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
         return s;
     }
 
-    // mixed_step!! / mixed_newton!! with fresh factors (newton.jl:1604-1666, 1692-1720).
-    GS_DEV Step mixed_step(double (&x)[N], double (&Fx)[N]) {
-        double fnorm_old = residual_norm(Fx);
-        const bool was_promoted = promoted;
-        double J[N][N];
+    // mixed_newton!! with a fresh factor and no store (newton.jl:1604-1666): the FP32 factor and its
+    // refinement, or the FP64 one of a promoted state; a failed FP32 factor or refinement promotes
+    // and refactorises the held Jacobian in FP64. J receives the regularised Jacobian; returns
+    // whether a direction exists (md.ok).
+    GS_DEV bool mixed_newton(const double (&x)[N], const double (&Fx)[N], double fnorm_old, double (&J)[N][N],
+                             double (&d)[N], int& failures, double& lin_rel) {
         jacobian(x, Fx, J);
-        double d[N];
         bool ok = true;
-        int failures = 0;
-        double lin_rel = 0.0;
+        failures = 0;
+        lin_rel = 0.0;
         if (promoted) {
             Factor<double, N> f;
             factorize<double, N>(J, f);
@@ -567,7 +571,35 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
                 lin_rel = rnorm<N>(lin) / fnorm_old;
             }
         }
-        if (!ok) {
+        return ok;
+    }
+
+    // read_mixed_stall (newton.jl:1437-1451), η_lin = max(√eps, 1e-4): two slow steps whose FP32
+    // linear model is inaccurate promote.
+    GS_DEV void read_mixed_stall(Step& s, double fnorm_old, double lin_rel) {
+        if (s.fnorm > 0.5 * fnorm_old && lin_rel > 1e-4) {
+            int slow = slow_steps + 1;
+            if (slow >= 2) {
+                promoted = true;
+                slow_steps = 0;
+                s.failures += 1;
+                s.promoted = true;
+            } else {
+                slow_steps = slow;
+            }
+        } else {
+            slow_steps = 0;
+        }
+    }
+
+    // mixed_step!! / mixed_newton!! with fresh factors (newton.jl:1604-1666, 1692-1720).
+    GS_DEV Step mixed_step(double (&x)[N], double (&Fx)[N]) {
+        double fnorm_old = residual_norm(Fx);
+        const bool was_promoted = promoted;
+        double J[N][N], d[N];
+        int failures;
+        double lin_rel;
+        if (!mixed_newton(x, Fx, fnorm_old, J, d, failures, lin_rel)) {
             Step s{kSingular, promoted, false, failures, rnorm<N>(Fx), 0.0};
             return s;
         }
@@ -579,20 +611,175 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
             s.promoted = true;
             slow_steps = 0;
         } else if (!was_promoted) {
-            // read_mixed_stall (newton.jl:1437-1451), η_lin = max(√eps, 1e-4)
-            if (s.fnorm > 0.5 * fnorm_old && lin_rel > 1e-4) {
-                int slow = slow_steps + 1;
-                if (slow >= 2) {
-                    promoted = true;
-                    slow_steps = 0;
-                    s.failures += 1;
-                    s.promoted = true;
-                } else {
-                    slow_steps = slow;
-                }
+            read_mixed_stall(s, fnorm_old, lin_rel);
+        }
+        sfnorm = s.fnorm;
+        return s;
+    }
+
+    // --- DogLeg (dogleg.jl:116-206, newton.jl:1725-1780) ----------------------------------------
+
+    GS_DEV static double dl_clamp(double v) { return v > 1e10 ? 1e10 : (v < 1e-6 ? 1e-6 : v); }
+    GS_DEV static double dl_shrink(double v) { return jmax(0.25 * v, 1e-6); }
+    GS_DEV static double dl_grow(double v) { return jmin(4.0 * v, 1e10); }
+
+    // The trial loop of dogleg_step on the model matrix J (the regularised Jacobian that was
+    // factorised) with the Newton step sN (read where `newton`): the Newton step inside the radius,
+    // else the Cauchy point cut to the radius, else the point of norm δ on the segment between them;
+    // a trial is accepted when pred > 0 and ared ≥ 1e-4 pred (unsquared norms), a rejection shrinks δ
+    // to max(δ/4, 1e-6), and a rejection at 1e-6 fails. x, Fx and `radius` are updated.
+    GS_DEV Step dogleg_trial(double (&x)[N], double (&Fx)[N], const double (&J)[N][N], const double (&sN)[N],
+                             bool newton) {
+        const double lo = 1e-6;
+        const double fnorm = rnorm<N>(Fx);
+        if (fnorm == 0.0) return Step{kSuccess, false, false, 0, fnorm, 0.0};
+        const double nN = newton ? rnorm<N>(sN) : __builtin_inf();
+        // the Cauchy direction from the unit residual: g̃ = Jᵀ (F/‖F‖), ĝ = g̃/‖g̃‖
+        double gh[N];
+        const double finv = 1.0 / fnorm;
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            double acc = J[0][k] * (finv * Fx[0]);
+#pragma unroll
+            for (int j = 1; j < N; ++j) acc = __builtin_fma(J[j][k], finv * Fx[j], acc);
+            gh[k] = acc;
+        }
+        const double gnorm = rnorm<N>(gh);
+        const double ginv = 1.0 / gnorm;
+#pragma unroll
+        for (int i = 0; i < N; ++i) gh[i] = ginv * gh[i];
+        double Jg[N];
+        matvec<N>(J, gh, Jg);
+        const double jnorm = rnorm<N>(Jg);
+        const bool cauchy = gnorm != 0.0 && isfin(gnorm) && jnorm != 0.0 && isfin(jnorm);
+        const double nC = fnorm * ((gnorm / jnorm) / jnorm);
+        double delta = radius;
+        if (delta == 0.0 && (newton || cauchy)) {
+            const double n0 = newton ? nN : nC;
+            delta = n0 < lo ? 2.0 * lo : dl_clamp(n0);
+        }
+        while (true) {
+            double sv[N];
+            bool boundary;
+            if (newton && nN <= delta) {
+#pragma unroll
+                for (int i = 0; i < N; ++i) sv[i] = sN[i];
+                boundary = nN >= delta;
+            } else if (!cauchy) {
+                radius = delta;
+                return Step{kStalled, false, false, 0, fnorm, 0.0};
+            } else if (!newton || nC >= delta) {
+                const double a = -jmin(nC, delta);
+#pragma unroll
+                for (int i = 0; i < N; ++i) sv[i] = a * gh[i];
+                boundary = nC >= delta;
             } else {
-                slow_steps = 0;
+                // the root t ≥ 0 of ‖a + t b̂‖ = 1 in units of δ, a = −(nC/δ) ĝ, b = sN + nC ĝ
+                double av[N], bh[N];
+                const double ac = -(nC / delta);
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    av[i] = ac * gh[i];
+                    bh[i] = 1.0 * sN[i] + nC * gh[i];
+                }
+                const double binv = 1.0 / rnorm<N>(bh);
+#pragma unroll
+                for (int i = 0; i < N; ++i) bh[i] = binv * bh[i];
+                const double ab = rdot<N>(av, bh);
+                const double c = 1.0 - norm2<N>(av);
+                const double ab2 = ab * ab;
+                const double t = c / (ab + __builtin_sqrt(ab2 + c));
+                const double dt = delta * t;
+#pragma unroll
+                for (int i = 0; i < N; ++i) sv[i] = delta * av[i] + dt * bh[i];
+                boundary = true;
             }
+            double Js[N], Flin[N], xp[N], Fp[N];
+            matvec<N>(J, sv, Js);
+#pragma unroll
+            for (int i = 0; i < N; ++i) Flin[i] = 1.0 * Fx[i] + 1.0 * Js[i];
+            const double pred = fnorm - rnorm<N>(Flin);
+#pragma unroll
+            for (int i = 0; i < N; ++i) xp[i] = x[i] + 1.0 * sv[i];
+            fam.F(xp, Fp);
+            const double fnp = rnorm<N>(Fp);
+            const double ared = fnorm - fnp;
+            if (pred > 0.0 && ared >= 1e-4 * pred) {
+                const bool limited = !(newton && nN <= delta);
+                const double rho = ared / pred;
+                if (rho < 0.1) {
+                    delta = nN < delta ? jmax(nN, lo) : dl_shrink(delta);
+                } else if (rho > 0.75 && boundary) {
+                    delta = dl_grow(delta);
+                }
+                const double stepnorm = rnorm<N>(sv);
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    x[i] = xp[i];
+                    Fx[i] = Fp[i];
+                }
+                radius = delta;
+                return Step{kSuccess, false, false, 0, fnp, stepnorm, limited};
+            }
+            if (delta == lo) {
+                radius = delta;
+                return Step{kLsFailed, false, false, 1, fnorm, 0.0};
+            }
+            delta = dl_shrink(delta);
+        }
+    }
+
+    // newton_step!! with DogLeg (newton.jl:1725-1737) and dogleg_direction (newton.jl:1741-1746): a
+    // failed factor, or a Newton step of zero or non-finite norm, leaves the Cauchy step.
+    GS_DEV Step plain_step_dogleg(double (&x)[N], double (&Fx)[N]) {
+        double J[N][N];
+        jacobian(x, Fx, J);
+        Factor<double, N> f;
+        factorize<double, N>(J, f);
+        double d[N];
+        bool newton = false;
+#pragma unroll
+        for (int i = 0; i < N; ++i) d[i] = Fx[i];
+        if (f.info <= 0) {
+            ldiv<N>(f, Fx, d);
+#pragma unroll
+            for (int i = 0; i < N; ++i) d[i] = -d[i];
+            if (REFINE) {
+                bool failed;
+                refine<double>(d, f, J, Fx, failed);
+            }
+            const double nd = rnorm<N>(d);
+            newton = isfin(nd) && nd != 0.0;
+        }
+        Step s = dogleg_trial(x, Fx, J, d, newton);
+        sfnorm = s.fnorm;
+        return s;
+    }
+
+    // mixed_step!! with DogLeg (newton.jl:1757-1780).
+    GS_DEV Step mixed_step_dogleg(double (&x)[N], double (&Fx)[N]) {
+        const double fnorm_old = residual_norm(Fx);
+        const bool was_promoted = promoted;
+        double J[N][N], d[N];
+        int failures;
+        double lin_rel;
+        const bool ok = mixed_newton(x, Fx, fnorm_old, J, d, failures, lin_rel);
+        const bool promoted_step = promoted && !was_promoted;
+        bool newton = false;
+        if (ok) {
+            const double nd = rnorm<N>(d);
+            newton = isfin(nd) && nd != 0.0;
+        } else {
+#pragma unroll
+            for (int i = 0; i < N; ++i) d[i] = Fx[i];
+        }
+        Step s = dogleg_trial(x, Fx, J, d, newton);
+        s.failures += failures;
+        if (promoted_step) {
+            s.promoted = true;
+            slow_steps = 0;
+        } else if (!was_promoted) {
+            read_mixed_stall(s, fnorm_old, lin_rel);
         }
         sfnorm = s.fnorm;
         return s;
@@ -802,7 +989,10 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
     }
 
     GS_DEV Step step(double (&x)[N], double (&Fx)[N]) {
-        if constexpr (REUSE) {
+        if constexpr (GLOB == kDogLeg) {
+            if constexpr (MIXED) return mixed_step_dogleg(x, Fx);
+            else return plain_step_dogleg(x, Fx);
+        } else if constexpr (REUSE) {
             if constexpr (MIXED) return mixed_step_reuse(x, Fx);
             else return plain_step_reuse(x, Fx);
         } else {
@@ -834,6 +1024,7 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
         age = 0;
         stalled = false;
         reused = false;
+        radius = q.dl_radius;  // reset_rule (newton.jl:677)
         double best_x[N];
         double best_fn = fnorm0;
         if (q.best) {
@@ -878,7 +1069,8 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
                         R.code = kSuccess;
                         break;
                     }
-                    const bool steptested = s.code != kStalled;
+                    // a step that a trust radius limited passes only the residual test
+                    const bool steptested = s.code != kStalled && !s.limited;
                     const double xnorm = steptested ? rnorm<N>(x) : dnan();
                     const double xtol = jmax(q.x_abstol, q.x_reltol * xnorm);
                     const bool past_min = R.iterations >= q.min_iterations;

@@ -922,7 +922,8 @@ template <class R, int N, class P> struct Solver {
     static R dogleg_hi() { return R(1e10); }
     static R dogleg_shrink(R d) { return jmax(R(0.25) * d, dogleg_lo()); }
     static R dogleg_grow(R d) { return jmin(R(4) * d, dogleg_hi()); }
-    static R dogleg_clamp(R d) { return std::min(std::max(d, dogleg_lo()), dogleg_hi()); }
+    // clamp(δ, lo, hi) in Julia's form: ifelse(δ > hi, hi, ifelse(δ < lo, lo, δ))
+    static R dogleg_clamp(R d) { return d > dogleg_hi() ? dogleg_hi() : (d < dogleg_lo() ? dogleg_lo() : d); }
 
     // dogleg_step (dogleg.jl:116-206): the trial loop on the model matrix J (the regularised
     // Jacobian that was factorised) with the Newton step sN (read where `newton`) and the radius

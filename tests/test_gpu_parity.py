@@ -116,6 +116,13 @@ HH_METHODS = {
                                                              globalization=gs.Backtracking(),
                                                              refinement=gs.Refinement()),
     "reuse4-mixed-static-norefine": lambda gs: gs.Newton(reuse=4, linsolve=gs.LUFactorization("float32")),
+    # DogLeg: the trust-region trial loop on the factorised Jacobian (dogleg.jl:116-206)
+    "dogleg": lambda gs: gs.Newton(globalization=gs.DogLeg()),
+    "dogleg-radius": lambda gs: gs.Newton(globalization=gs.DogLeg(radius=0.05)),
+    "dogleg-mixed-refine": lambda gs: gs.Newton(globalization=gs.DogLeg(), linsolve=gs.LUFactorization("float32"),
+                                                refinement=gs.Refinement()),
+    "dogleg-refine-regularized": lambda gs: gs.Newton(globalization=gs.DogLeg(radius=1e-3), refinement=gs.Refinement(),
+                                                      regularization=1e-3),
 }
 
 
@@ -160,6 +167,42 @@ def test_reuse_hard_steps(gs, orc, workloads, device):
                                      opts)
             assert_parity(st, x, so, xo, bitwise=True)
             assert st["reused"].any() and len(np.unique(st["code"])) >= 2
+
+
+def test_dogleg_hard_steps(gs, orc, workloads, device):
+    """DogLeg where the radius matters: large steps (h = 2) from far-out starts give Cauchy and
+    segment steps, radius floors (LINESEARCH_FAILED), stalls and mixed-precision promotions."""
+    n = 16384
+    xk = workloads.henon_heiles_states(n, seed=7, e_lo=0.1, e_hi=0.166)
+    f32 = gs.LUFactorization("float32")
+    methods = (gs.Newton(globalization=gs.DogLeg()),
+               gs.Newton(globalization=gs.DogLeg(), linsolve=f32, refinement=gs.Refinement()),
+               gs.Newton(globalization=gs.DogLeg(radius=0.5), regularization=1e-2))
+    for m in methods:
+        for opts in (gs.Options(f_abstol=1e-12, f_reltol=0.0, max_iterations=30),
+                     gs.Options(norm=gs.MaxNorm(), min_iterations=1, x_reltol=1e-6, roundoff=True)):
+            x, st, xo, so = run_both(gs, orc, device, m, gs.HenonHeilesMidpoint(), xk * 3.0, gs.BatchParams(2.0, xk * 3.0),
+                                     opts)
+            assert_parity(st, x, so, xo, bitwise=True)
+            assert (st["code"] == gs.LINESEARCH_FAILED).any() and (st["code"] == gs.SUCCESS).any()
+
+
+def test_dogleg_coupled_batch_and_tanh(gs, orc, workloads, device):
+    """test/nonlinear/batched.jl:263-352 with Newton(; globalization = DogLeg()): bit-identical to
+    the oracle, the exact root in 0 steps, the counts form; and the n = 3 tanh family."""
+    P = np.array([[2.125, 1.5], [1.25, 2.0], [1.5, 2.5], [1.75, 3.0]])
+    X0 = np.array([[2.0, 1.0], [2.5, 1.5], [3.5, 2.5], [4.0, 3.0]])
+    opts = gs.Options(max_iterations=150)
+    m = gs.Newton(globalization=gs.DogLeg())
+    x, st, xo, so = run_both(gs, orc, device, m, gs.Coupled2(), X0.copy(), gs.PerSystem(P), opts)
+    assert_parity(st, x, so, xo, bitwise=True)
+    assert (st["code"] == gs.SUCCESS).all() and st["iterations"][0] == 0 and np.array_equal(x[0], X0[0])
+    _, counts = run_gpu(gs, device, m, gs.Coupled2(), X0.copy(), gs.PerSystem(P), opts, counts=True)
+    assert counts == dict(SUCCESS=4, STALLED=0, MAXITERS=0, SINGULAR=0, NONFINITE=0, LINESEARCH_FAILED=0)
+    rec, x0 = workloads.tanh_linear3_batch(4096, 0)
+    opts = gs.Options(f_abstol=1e-12, f_reltol=0.0, max_iterations=50)
+    x, st, xo, so = run_both(gs, orc, device, m, gs.TanhLinear3(), x0.copy(), gs.PerSystem(rec), opts)
+    assert_parity(st, x, so, xo, near_tol=True, tol=1e-12)
 
 
 def test_reuse_coupled_batch(gs, orc, device):

@@ -219,3 +219,32 @@ def test_reuse_method_and_entry_checks(gs):
     rc = lib.gs_solve_batch_f64(ctypes.byref(gs.Newton(reuse=2).to_c()), ctypes.byref(fam), 1, None, None, 0, None,
                                 codes, None, ctypes.byref(o), 0, None)
     assert rc == abi.GS_EFAMILY and "no reuse" in abi.last_error(lib)
+
+
+def test_dogleg_entry_checks(gs):
+    """gs_solve_batch_f64 refuses a DogLeg radius outside the constructor's clamp and DogLeg with
+    reuse (dogleg.jl:44-60, newton.jl:66-70) before any device call; FpuGl3 takes Static only."""
+    lib = gs.load_library()
+    from importlib import import_module
+
+    abi = import_module(gs.__name__ + "._abi")
+    fam = abi.GsFamily()
+    fam.id, fam.n, fam.m, fam.p_len = abi.GS_FAM_COUPLED2, 2, 2, 2
+    o = gs.Options().to_c()
+    codes = (ctypes.c_uint8 * 1)()
+    m = abi.GsMethod.from_buffer_copy(gs.Newton(globalization=gs.DogLeg()).to_c())
+    assert m.globalization == abi.GS_DOGLEG and m.dl_radius == 0.0
+    m.dl_radius = 1e-9
+    rc = lib.gs_solve_batch_f64(ctypes.byref(m), ctypes.byref(fam), 1, None, None, 0, None, codes, None,
+                                ctypes.byref(o), 0, None)
+    assert rc == abi.GS_EINVAL and "radius" in abi.last_error(lib)
+    m.dl_radius, m.reuse = 0.0, 2
+    rc = lib.gs_solve_batch_f64(ctypes.byref(m), ctypes.byref(fam), 1, None, None, 0, None, codes, None,
+                                ctypes.byref(o), 0, None)
+    assert rc == abi.GS_EINVAL and "no reuse" in abi.last_error(lib)
+    fam.id, fam.n, fam.m, fam.p_len = abi.GS_FAM_FPU_GL3, 48, 48, 32
+    m.reuse = 0
+    rc = lib.gs_solve_batch_f64(ctypes.byref(m), ctypes.byref(fam), 1, None, None, 0, None, codes, None,
+                                ctypes.byref(o), 0, None)
+    assert rc == abi.GS_EFAMILY
+
