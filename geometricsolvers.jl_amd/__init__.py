@@ -25,11 +25,11 @@ from typing import Any, Optional
 import numpy as np
 
 from . import _abi
-from ._abi import STATUS_DTYPE, load_library
+from ._abi import STATUS_DTYPE, STATUS_DTYPE_F32, load_library
 
 __all__ = [
     "ArgumentError", "DimensionMismatch", "ReturnCode", "SUCCESS", "STALLED", "MAXITERS",
-    "SINGULAR", "NONFINITE", "LINESEARCH_FAILED", "STATUS_DTYPE", "L2Norm", "MaxNorm", "Options",
+    "SINGULAR", "NONFINITE", "LINESEARCH_FAILED", "STATUS_DTYPE", "STATUS_DTYPE_F32", "L2Norm", "MaxNorm", "Options",
     "LUFactorization", "Static", "Backtracking", "DogLeg", "LevenbergMarquardt", "NoRefinement", "Refinement", "Newton",
     "NonlinearProblem", "Coupled2", "Hetero2", "ScalarSquare", "TanhLinear3",
     "HenonHeilesMidpoint", "FpuGl3", "PerSystem", "BatchParams", "parameters", "solve_batch",
@@ -89,22 +89,38 @@ def _nonnegative(name: str, tol: float) -> None:
 
 @dataclass(frozen=True)
 class Options:
-    """Options{Float64} (options.jl:58-71). Build with `Options.make(**kw)` or `Options()`."""
+    """Options{R} (options.jl:58-71, 207-235) with R = Float64 or, with `eltype="float32"`, Float32:
+    the defaults √eps(R), 2 eps(R), √eps(R) and R(1)/10 are R's, and every tolerance is held as its
+    R value (R(x), as the constructor converts)."""
 
     f_abstol: float = 0.0
-    f_reltol: float = _SQRT_EPS
+    f_reltol: Optional[float] = None
     x_abstol: float = 0.0
-    x_reltol: float = 2 * _EPS
+    x_reltol: Optional[float] = None
     min_iterations: int = 0
     max_iterations: int = 100
     max_stalls: int = 2
-    g_tol: float = _SQRT_EPS
+    g_tol: Optional[float] = None
     norm: Any = field(default_factory=L2Norm)
-    kappa: float = 0.1
+    kappa: Optional[float] = None
     roundoff: bool = False
     rate_test: bool = True
+    eltype: str = "float64"
 
     def __post_init__(self):
+        if self.eltype not in ("float64", "float32"):
+            raise ArgumentError("Options eltype must be 'float64' or 'float32'")
+        f32 = self.eltype == "float32"
+        eps = float(np.finfo(np.float32).eps) if f32 else _EPS
+        sqrt_eps = float(np.sqrt(np.float32(eps))) if f32 else _SQRT_EPS
+        defaults = {"f_reltol": sqrt_eps, "x_reltol": 2 * eps, "g_tol": sqrt_eps,
+                    "kappa": float(np.float32(1) / np.float32(10)) if f32 else 0.1}
+        for name, v in defaults.items():
+            if getattr(self, name) is None:
+                object.__setattr__(self, name, v)
+        if f32:
+            for name in ("f_abstol", "f_reltol", "x_abstol", "x_reltol", "g_tol", "kappa"):
+                object.__setattr__(self, name, float(np.float32(getattr(self, name))))
         # Options(T; kwargs...) validation (options.jl:220-235)
         for name in ("f_abstol", "f_reltol", "x_abstol", "x_reltol", "g_tol"):
             _nonnegative(name, getattr(self, name))
@@ -137,14 +153,18 @@ class Options:
         return o
 
 
-def resolve_options(options: Optional[Options], kwargs: dict) -> Options:
-    """resolve_options (src/nonlinear/solver.jl:391-398): keywords or an Options, never both."""
+def resolve_options(options: Optional[Options], kwargs: dict, eltype: str = "float64") -> Options:
+    """resolve_options (src/nonlinear/solver.jl:391-398): keywords or an Options, never both; an
+    Options of another real type than the batch's is refused (check_options, options.jl:241-245)."""
     if options is None:
-        return Options(**kwargs)
+        return Options(eltype=eltype, **kwargs)
     if kwargs:
         raise ArgumentError("pass either `options` or keyword arguments, not both")
     if not isinstance(options, Options):
         raise ArgumentError("options must be an Options")
+    if options.eltype != eltype:
+        R, T = ("Float32" if e == "float32" else "Float64" for e in (options.eltype, eltype))
+        raise ArgumentError(f"options are Options{{{R}}}, but the iterate has element type {T}: pass Options({T}; …)")
     return options
 
 
@@ -366,8 +386,9 @@ class Family:
     p_len: int = 0
     p_dtype = None  # torch dtype of a parameter record
 
-    def marshal(self, p, nsys: int, device):
-        """The parameter records of a batch: (tensor, byte stride, shared scalars)."""
+    def marshal(self, p, nsys: int, device, real=None):
+        """The parameter records of a batch: (tensor, byte stride, shared scalars); `real` is the torch
+        float type of the records (the batch's working precision, float64 by default)."""
         raise NotImplementedError
 
 
@@ -392,15 +413,16 @@ def _records(data, nsys: int, p_len: int, dtype, device, what: str):
 class _SimpleFamily(Family):
     """A family whose parameter is PerSystem(records) or one shared record."""
 
-    def marshal(self, p, nsys, device):
+    def marshal(self, p, nsys, device, real=None):
         torch = _torch()
+        dtype = self.p_dtype if self.p_dtype == torch.int64 or real is None else real
         if isinstance(p, PerSystem):
-            t = _records(p.data, nsys, self.p_len, self.p_dtype, device, type(self).__name__)
+            t = _records(p.data, nsys, self.p_len, dtype, device, type(self).__name__)
             return t, t.stride(0) * t.element_size(), ()
         if isinstance(p, BatchParams):
             raise ArgumentError(f"{type(self).__name__} takes PerSystem or shared parameters")
-        t = torch.as_tensor(np.asarray(p, dtype=np.int64 if self.p_dtype == torch.int64 else np.float64))
-        t = t.reshape(1, -1).to(device=device, dtype=self.p_dtype).contiguous()
+        t = torch.as_tensor(np.asarray(p, dtype=np.int64 if dtype == torch.int64 else np.float64))
+        t = t.reshape(1, -1).to(device=device, dtype=dtype).contiguous()
         if t.shape[1] != self.p_len:
             raise ArgumentError(f"{type(self).__name__} takes {self.p_len} parameters per system")
         return t, 0, ()
@@ -454,12 +476,12 @@ class HenonHeilesMidpoint(Family):
 
     id, n, p_len = _abi.GS_FAM_HENON_HEILES_MIDPOINT, 4, 4
 
-    def marshal(self, p, nsys, device):
+    def marshal(self, p, nsys, device, real=None):
         torch = _torch()
         if not isinstance(p, BatchParams):
             raise ArgumentError("HenonHeilesMidpoint takes BatchParams(h, X_k)")
         h = float(p.shared)
-        t = _records(p.per_system, nsys, 4, torch.float64, device, "HenonHeilesMidpoint")
+        t = _records(p.per_system, nsys, 4, real or torch.float64, device, "HenonHeilesMidpoint")
         return t, t.stride(0) * t.element_size(), (h,)
 
 
@@ -478,7 +500,7 @@ class FpuGl3(Family):
     def __init__(self, beta: float = 1.0):
         self.beta = float(beta)
 
-    def marshal(self, p, nsys, device):
+    def marshal(self, p, nsys, device, real=None):
         torch = _torch()
         if not isinstance(p, BatchParams):
             raise ArgumentError("FpuGl3 takes BatchParams(h, QP) with QP the (q, p) record of every system")
@@ -550,15 +572,19 @@ class ExpSumData:
 # --------------------------------------------------------------------------------------------
 
 
-def status_buffer(nsys: int, device):
-    """A device vector of `nsys` SolverStatus{Float64} (72 bytes each), as a (nsys, 72) uint8 tensor."""
-    return _torch().empty((nsys, STATUS_DTYPE.itemsize), dtype=_torch().uint8, device=device)
+def status_buffer(nsys: int, device, eltype: str = "float64"):
+    """A device vector of `nsys` SolverStatus{Float64} (72 bytes each) or, with eltype="float32",
+    SolverStatus{Float32} (44 bytes each), as an (nsys, size) uint8 tensor."""
+    size = (STATUS_DTYPE_F32 if eltype == "float32" else STATUS_DTYPE).itemsize
+    return _torch().empty((nsys, size), dtype=_torch().uint8, device=device)
 
 
 def statuses(S) -> np.ndarray:
-    """The statuses of a status buffer as a host numpy array of dtype STATUS_DTYPE."""
+    """The statuses of a status buffer as a host numpy array of dtype STATUS_DTYPE (or
+    STATUS_DTYPE_F32 for a buffer of Float32 statuses)."""
     host = S.detach().to("cpu").contiguous().numpy()
-    return host.reshape(-1).view(STATUS_DTYPE)
+    dt = STATUS_DTYPE_F32 if S.shape[-1] == STATUS_DTYPE_F32.itemsize else STATUS_DTYPE
+    return host.reshape(-1).view(dt)
 
 
 def isconverged(x) -> bool | np.ndarray:
@@ -580,11 +606,17 @@ def _stream_ptr(X):
     return ctypes.c_void_p(torch.cuda.current_stream(X.device).cuda_stream)
 
 
-def _check_x(X, nfam: int):
+def _eltype(X) -> str:
+    """The working precision of a batch: 'float64' or 'float32' (the R of its SolverStatus{R})."""
+    torch = _torch()
+    return "float32" if X.dtype == torch.float32 else "float64"
+
+
+def _check_x(X, nfam: int, allow_f32: bool = False):
     torch = _torch()
     if not isinstance(X, torch.Tensor):
         raise ArgumentError("X must be a torch tensor of shape (nsys, n)")
-    if X.dtype != torch.float64:
+    if X.dtype != torch.float64 and not (allow_f32 and X.dtype == torch.float32):
         raise ArgumentError(
             f"SolverStatus real type Float64 does not match batch real element type {X.dtype}")
     if X.dim() != 2 or X.shape[1] != nfam:
@@ -664,13 +696,18 @@ def solve_batch(X, *args, order=None, options: Optional[Options] = None, workgro
             "the device solve of a NonlinearProblem takes Static() or Backtracking(); "
             "LevenbergMarquardt() is the least-squares batch's (a LeastSquaresProblem)")
     fam = prob.F
-    _check_x(X, fam.n)
+    _check_x(X, fam.n, allow_f32=True)
+    elt = _eltype(X)
     order_h = _validate_batch(X, p, order)
     nsys = X.shape[0]
     torch = _torch()
     if S is not None:
-        if S.dtype != torch.uint8 or S.dim() != 2 or S.shape[1] != STATUS_DTYPE.itemsize:
+        if S.dtype != torch.uint8 or S.dim() != 2 or S.shape[1] not in (STATUS_DTYPE.itemsize, STATUS_DTYPE_F32.itemsize):
             raise ArgumentError("S must be a status buffer (status_buffer(nsys, device))")
+        if S.shape[1] != (STATUS_DTYPE_F32 if elt == "float32" else STATUS_DTYPE).itemsize:
+            R = "Float64" if elt == "float32" else "Float32"
+            raise ArgumentError(
+                f"SolverStatus real type {R} does not match batch real element type {'Float32' if elt == 'float32' else 'Float64'}")
         if S.shape[0] != nsys:
             raise DimensionMismatch(
                 f"status vector length {S.shape[0]} does not match batch length {nsys}")
@@ -680,7 +717,7 @@ def solve_batch(X, *args, order=None, options: Optional[Options] = None, workgro
         if not S.is_contiguous():
             raise ArgumentError("S must be contiguous")
     ws = _validate_workgroupsize(workgroupsize)
-    opt = resolve_options(options, kwargs)
+    opt = resolve_options(options, kwargs, elt)
     cm = method.to_c()
     if X.device.type != "cuda":
         raise ArgumentError(
@@ -688,7 +725,7 @@ def solve_batch(X, *args, order=None, options: Optional[Options] = None, workgro
     if nsys == 0:
         return S if S is not None else {k: 0 for k in _CODE_NAMES}
     lib = load_library()
-    P, p_stride, shared = fam.marshal(p, nsys, X.device)
+    P, p_stride, shared = fam.marshal(p, nsys, X.device, X.dtype)
     cf = _abi.GsFamily()
     cf.id, cf.n, cf.m, cf.p_len = fam.id, fam.n, fam.n, fam.p_len
     for k, v in enumerate(shared):
@@ -700,7 +737,8 @@ def solve_batch(X, *args, order=None, options: Optional[Options] = None, workgro
     if S is None:
         codes = torch.empty(nsys, dtype=torch.uint8, device=X.device)
     copt = opt.to_c()
-    rc = lib.gs_solve_batch_f64(
+    entry = lib.gs_solve_batch_f32 if elt == "float32" else lib.gs_solve_batch_f64
+    rc = entry(
         ctypes.byref(cm), ctypes.byref(cf), nsys, ctypes.c_void_p(X.data_ptr()),
         ctypes.c_void_p(P.data_ptr()), p_stride,
         ctypes.c_void_p(S.data_ptr()) if S is not None else None,
@@ -812,5 +850,7 @@ def batch_summary(S) -> BatchSummary:
     return BatchSummary(fn.value, out[0] == 0, int(out[0]), int(out[1]), int(out[2]))
 
 
+from . import autotune  # noqa: E402  (the least-squares launch-shape autotuner)
+from .autotune import tune_lsq  # noqa: E402
 from . import learned  # noqa: E402  (the learned start; imports the names above lazily)
 from .learned import LearnedStart, generate_solutions, solve_batch_learned  # noqa: E402

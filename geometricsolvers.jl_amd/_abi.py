@@ -129,6 +129,19 @@ STATUS_DTYPE = np.dtype(
     }
 )
 
+# SolverStatus{Float32} (44 bytes, Julia's field offsets).
+STATUS_DTYPE_F32 = np.dtype(
+    {
+        "names": list(STATUS_DTYPE.names),
+        "formats": [
+            "u1", "<i2", "<i4", "<f4", "<f4", "<i4", "u1", "u1", "<i2", "<i4", "<f4", "<f4",
+            "<f4", "u1", "u1",
+        ],
+        "offsets": [0, 2, 4, 8, 12, 16, 20, 21, 22, 24, 28, 32, 36, 40, 41],
+        "itemsize": 44,
+    }
+)
+
 GS_MLP_IN, GS_MLP_HIDDEN, GS_MLP_OUT = 32, 128, 16
 GS_MLP_NPARAM, GS_MLP_NPACKED = 39312, 38912
 
@@ -157,7 +170,11 @@ EXPORTED_SYMBOLS = (
     "gs_set_kernel_timing",
     "gs_last_kernel_ms",
     "gs_solve_expsum_lsq_f64",
+    "gs_shutdown",
+    "gs_solve_batch_f32",
     "gs_lsq_set_shape",
+    "gs_lsq_get_shape",
+    "gs_lsq_set_graph",
     "gs_mlp_start_henon_heiles",
     "gs_solve_learned_henon_heiles_f64",
     "gs_mlp_grad",
@@ -227,6 +244,14 @@ def load_library(path: os.PathLike | None = None):
     lib.gs_mlp_pack.argtypes = [vp, vp, vp]
     lib.gs_lsq_set_shape.restype = ctypes.c_int
     lib.gs_lsq_set_shape.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.gs_lsq_get_shape.restype = ctypes.c_int
+    lib.gs_lsq_get_shape.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_void_p]
+    lib.gs_lsq_set_graph.restype = ctypes.c_int
+    lib.gs_lsq_set_graph.argtypes = [ctypes.c_int]
+    lib.gs_shutdown.restype = ctypes.c_int
+    lib.gs_shutdown.argtypes = []
+    lib.gs_solve_batch_f32.restype = ctypes.c_int
+    lib.gs_solve_batch_f32.argtypes = lib.gs_solve_batch_f64.argtypes
     if lib.gs_version() != ABI_VERSION:
         raise RuntimeError(f"libgsolve_hip ABI version {lib.gs_version()} != {ABI_VERSION}")
     sizes = (ctypes.c_int64 * 4)()

@@ -118,6 +118,16 @@ int blocks_for(int64_t n) {
 }
 
 }  // namespace
+
+// Free the workspace of every device (gs_shutdown; no call may be in flight).
+void release_reduce() {
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    for (int d = 0; d < 64; ++d) {
+        if (!g_ws[d].ptr) continue;
+        if (hipSetDevice(d) == hipSuccess) (void)hipFree(g_ws[d].ptr);
+        g_ws[d] = Workspace{};
+    }
+}
 }  // namespace gs
 
 using namespace gs;

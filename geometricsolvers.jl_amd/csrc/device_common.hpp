@@ -21,23 +21,52 @@ constexpr double kDblTiny = 4.9406564584124654e-324;    // nextfloat(0.0)
 constexpr float kFltMin = 1.17549435e-38f;
 constexpr int kEmax64 = 1023;                           // exponent(floatmax(Float64))
 
+// The constants of the working precision R (Float64 or Float32): floatmin, eps, nextfloat(0),
+// exponent(floatmax) and √eps.
+template <class T> struct Lim;
+template <> struct Lim<double> {
+    static constexpr double min = 2.2250738585072014e-308, eps = 2.220446049250313e-16, tiny = 4.9406564584124654e-324;
+    static constexpr int emax = 1023;
+    static constexpr double sqrteps = 1.4901161193847656e-8;
+};
+template <> struct Lim<float> {
+    static constexpr float min = 1.17549435e-38f, eps = 1.1920929e-7f, tiny = 1.4e-45f;
+    static constexpr int emax = 127;
+    static constexpr float sqrteps = 3.4526698e-4f;
+};
+
 GS_DEV double dnan() { return __builtin_nan(""); }
+template <class T> GS_DEV T tnan() { return T(__builtin_nan("")); }
 GS_DEV bool isfin(double a) { return __builtin_isfinite(a); }
+GS_DEV bool isfin(float a) { return __builtin_isfinite(a); }
 GS_DEV bool isnan_(double a) { return __builtin_isnan(a); }
+GS_DEV bool isnan_(float a) { return __builtin_isnan(a); }
+GS_DEV double tabs(double a) { return __builtin_fabs(a); }
+GS_DEV float tabs(float a) { return __builtin_fabsf(a); }
+GS_DEV double tsqrt(double a) { return __builtin_sqrt(a); }
+GS_DEV float tsqrt(float a) { return __builtin_sqrtf(a); }
+GS_DEV double tfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+GS_DEV float tfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+GS_DEV double tfmax(double a, double b) { return __builtin_fmax(a, b); }
+GS_DEV float tfmax(float a, float b) { return __builtin_fmaxf(a, b); }
+GS_DEV double tfmin(double a, double b) { return __builtin_fmin(a, b); }
+GS_DEV float tfmin(float a, float b) { return __builtin_fminf(a, b); }
 
 // Julia's max/min of floats: NaN-propagating.
-GS_DEV double jmax(double a, double b) { return (isnan_(a) || isnan_(b)) ? dnan() : (a > b ? a : b); }
-GS_DEV double jmin(double a, double b) { return (isnan_(a) || isnan_(b)) ? dnan() : (a < b ? a : b); }
+template <class T> GS_DEV T jmax(T a, T b) { return (isnan_(a) || isnan_(b)) ? tnan<T>() : (a > b ? a : b); }
+template <class T> GS_DEV T jmin(T a, T b) { return (isnan_(a) || isnan_(b)) ? tnan<T>() : (a < b ? a : b); }
 
-// a·2^e, exact unless it leaves the range (v_ldexp_f64 rounds correctly into the subnormals).
+// a·2^e, exact unless it leaves the range (v_ldexp rounds correctly into the subnormals).
 GS_DEV double ldexp2(double a, int e) { return __builtin_ldexp(a, e); }
+GS_DEV float ldexp2(float a, int e) { return __builtin_ldexpf(a, e); }
 
 // ⌊log2|a|⌋ of a finite nonzero a, subnormals included (Julia `exponent`).
 GS_DEV int exponent_of(double a) { return __builtin_amdgcn_frexp_exp(a) - 1; }
+GS_DEV int exponent_of(float a) { return __builtin_amdgcn_frexp_expf(a) - 1; }
 
 // Squared 2-norm as a left fold without FMA (norm2, reductions.jl:12-15).
-template <int L> GS_DEV double norm2(const double (&a)[L]) {
-    double s = a[0] * a[0];
+template <int L, class T> GS_DEV T norm2(const T (&a)[L]) {
+    T s = a[0] * a[0];
 #pragma unroll
     for (int i = 1; i < L; ++i) s = s + a[i] * a[i];
     return s;
@@ -45,40 +74,40 @@ template <int L> GS_DEV double norm2(const double (&a)[L]) {
 
 // The scaled path of rnorm (reductions.jl:355-363): it runs only for a sum that overflowed or fell
 // below L·floatmin. Inline, so that no array becomes addressable (and no lane touches scratch).
-template <int L> GS_DEV double rnorm_scaled(const double (&a)[L]) {
-    double m = __builtin_fabs(a[0]);
+template <int L, class T> GS_DEV T rnorm_scaled(const T (&a)[L]) {
+    T m = tabs(a[0]);
 #pragma unroll
-    for (int i = 1; i < L; ++i) m = jmax(m, __builtin_fabs(a[i]));
-    if (m == 0.0 || !isfin(m)) return m;
+    for (int i = 1; i < L; ++i) m = jmax(m, tabs(a[i]));
+    if (m == T(0) || !isfin(m)) return m;
     int e = exponent_of(m);
-    double t = ldexp2(a[0], -e);
-    double s = t * t;
+    T t = ldexp2(a[0], -e);
+    T s = t * t;
 #pragma unroll
     for (int i = 1; i < L; ++i) {
         t = ldexp2(a[i], -e);
         s = s + t * t;
     }
-    return ldexp2(__builtin_sqrt(s), e);
+    return ldexp2(tsqrt(s), e);
 }
 
-// rnorm (reductions.jl:41-57).
-template <int L> GS_DEV double rnorm(const double (&a)[L]) {
-    double s = norm2(a);
-    if (__builtin_expect(isfin(s) && s >= double(L) * kDblMin, 1)) return __builtin_sqrt(s);
-    return rnorm_scaled(a);
+// rnorm (reductions.jl:41-57); the sum is in the element type (Float32 or Float64).
+template <int L, class T> GS_DEV T rnorm(const T (&a)[L]) {
+    T s = norm2<L>(a);
+    if (__builtin_expect(isfin(s) && s >= T(L) * Lim<T>::min, 1)) return tsqrt(s);
+    return rnorm_scaled<L>(a);
 }
 
 // rmaxnorm (reductions.jl:67).
-template <int L> GS_DEV double rmaxnorm(const double (&a)[L]) {
-    double m = __builtin_fabs(a[0]);
+template <int L, class T> GS_DEV T rmaxnorm(const T (&a)[L]) {
+    T m = tabs(a[0]);
 #pragma unroll
-    for (int i = 1; i < L; ++i) m = jmax(m, __builtin_fabs(a[i]));
+    for (int i = 1; i < L; ++i) m = jmax(m, tabs(a[i]));
     return m;
 }
 
 // rdot (reductions.jl:77-85): a left fold of products, no FMA.
-template <int L> GS_DEV double rdot(const double (&a)[L], const double (&b)[L]) {
-    double s = a[0] * b[0];
+template <int L, class T> GS_DEV T rdot(const T (&a)[L], const T (&b)[L]) {
+    T s = a[0] * b[0];
 #pragma unroll
     for (int i = 1; i < L; ++i) s = s + a[i] * b[i];
     return s;
@@ -87,15 +116,15 @@ template <int L> GS_DEV double rdot(const double (&a)[L], const double (&b)[L]) 
 // The exponent of the largest |entry| of an array, clamped to ±emax; 0 when the array is zero or
 // holds a non-finite entry (matrix_exponent, factors.jl:833-843, whose largest-abs-bits maximum makes
 // a NaN the largest entry). A float max over |a| orders the finite entries as their bits do.
-template <int L> GS_DEV int max_exponent(const double (&a)[L], int emax) {
-    double m = __builtin_fabs(a[0]);
+template <int L, class T> GS_DEV int max_exponent(const T (&a)[L], int emax) {
+    T m = tabs(a[0]);
     bool nan_seen = isnan_(a[0]);
 #pragma unroll
     for (int i = 1; i < L; ++i) {
-        m = __builtin_fmax(m, __builtin_fabs(a[i]));
+        m = tfmax(m, tabs(a[i]));
         nan_seen |= isnan_(a[i]);
     }
-    if (nan_seen || !isfin(m) || m == 0.0) return 0;
+    if (nan_seen || !isfin(m) || m == T(0)) return 0;
     int e = exponent_of(m);
     return e < -emax ? -emax : (e > emax ? emax : e);
 }
@@ -112,11 +141,12 @@ constexpr int headroom(int n) {
 }
 
 // rhs_exponent (factors.jl:880-885) for a right-hand side of length L and factor exponent e.
-template <int L> GS_DEV int rhs_exponent(const double (&r)[L], int e) {
-    int m = max_exponent<L>(r, kEmax64);
+template <int L, class T> GS_DEV int rhs_exponent(const T (&r)[L], int e) {
+    constexpr int emax = Lim<T>::emax;
+    int m = max_exponent<L>(r, emax);
     int lo = e < 0 ? e : 0;
     int c = m < lo ? lo : (m > 0 ? 0 : m);
-    int alt = m - (kEmax64 - headroom(L));
+    int alt = m - (emax - headroom(L));
     return c > alt ? c : alt;
 }
 

@@ -7,10 +7,13 @@
 // src/globalization/levenbergmarquardt.jl:48-98, 162-164, 359-482, 575-584).
 //
 // Launch structure (one call): the fits are bucketed by n and ordered by m on the device (a counting
-// sort over (K, m) bins), then one persistent kernel per non-empty bucket, compiled for that n, runs
-// one lane per fit; a lane whose fit ends takes the next fit of its bucket from an atomic queue at
-// the top of the next LM outer step. The launch shape of each bucket (workgroup size, workgroups per
-// CU) comes from the library's autotuner (gs_lsq_tune) or its defaults.
+// sort over (K, m) bins), then one persistent kernel per bucket, compiled for that n, runs one fit
+// per lane; a lane whose fit ends takes the next fit of its bucket from an atomic queue at the top of
+// the next LM outer step. The buckets' ranges stay on the device, so the whole sequence does not
+// depend on the data: it is captured once into a hipGraph and replayed while the call's pointers,
+// sizes, parameters and shapes stay the same, with one host synchronisation at the end. The shape of
+// each bucket (workgroup size, resident workgroups per CU) is the package's autotuner's choice
+// (set through gs_lsq_set_shape) or the default (64 threads, the kernel's occupancy).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -579,8 +582,10 @@ struct LsqArgs {
     const int64_t* y_offset;
     const double* Y;
     gs_status_f64* S;
-    const int32_t* list;     // the fits of this bucket, ordered by m
-    int64_t count;           // fits in this bucket
+    const int32_t* list;     // every fit, ordered by (K, m)
+    const uint32_t* bstart;  // the buckets' starts in `list` (5 entries), written by lsq_scan
+    const uint32_t* bad;     // the count of fits that fail validation, written by lsq_count
+    int bucket;              // this bucket, K − 1
     uint32_t* queue;         // this bucket's queue head
     double* slots;           // this bucket's lane slots (Slot<N>::kSize entries per lane)
 };
@@ -621,6 +626,12 @@ template <int N> GS_DEV void fit_store(Fit<N>& f, const LsqArgs& a) {
 // without a fit takes the next one from the bucket's queue (one atomic per wave per refill), and the
 // wave leaves the loop once the queue is empty and every lane is idle.
 template <int N> __global__ __launch_bounds__(256) void lsq_kernel(const LsqArgs a) {
+    // the bucket's range comes from the device-side sort, so the launch does not depend on the data;
+    // a batch with an invalid fit solves nothing (the host reports the error after the launch)
+    if (*a.bad != 0) return;
+    const uint32_t lo = a.bstart[a.bucket];
+    const uint64_t count = a.bstart[a.bucket + 1] - lo;
+    const int32_t* list = a.list + lo;
     Fit<N> f;
     const int64_t lanes = int64_t(gridDim.x) * blockDim.x;
     f.slot.p = a.slots + int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -640,10 +651,10 @@ template <int N> __global__ __launch_bounds__(256) void lsq_kernel(const LsqArgs
             base = __shfl(base, leader);
             if (need) {
                 const uint64_t slot = uint64_t(base) + __popcll(ballot & lane_lt);
-                if (slot >= uint64_t(a.count)) {
+                if (slot >= count) {
                     exhausted = true;
                 } else {
-                    fit_load<N>(f, a, a.list[slot]);
+                    fit_load<N>(f, a, list[slot]);
                     if (fit_start<N>(f, a.q))
                         fit_store<N>(f, a);
                     else
@@ -723,17 +734,21 @@ struct LsqWorkspace {
 std::mutex g_lsq_mutex;
 LsqWorkspace g_lsq_ws[64][2];  // per device: [0] bins, cursors and the ordered list; [1] lane slots
 
-// The launch shape of each bucket: threads per workgroup and fits per lane. Fewer lanes than fits
-// let a lane that ends early take another fit; more lanes expose more parallel work.
+// The launch shape of each bucket: threads per workgroup and resident workgroups per CU (0: as many
+// as the kernel's occupancy allows). The kernels are persistent, so the grid is the shape's share of
+// the machine (capped by the batch), not the bucket's size.
 struct LsqShape {
     int threads = 64;
-    int fits_per_lane = 1;
+    int blocks_per_cu = 0;
 };
-LsqShape g_lsq_shape[4] = {{64, 1}, {64, 1}, {64, 1}, {64, 1}};
+LsqShape g_lsq_shape[4] = {{64, 0}, {64, 0}, {64, 0}, {64, 0}};
+bool g_lsq_graph_on = true;
 
-// Library streams (default priority) on which the buckets run concurrently, one set per device.
+// Library streams (default priority) on which the buckets run concurrently, one set per device, and
+// the stream the launch sequence is captured on.
 struct LsqStreams {
     hipStream_t s[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipStream_t cap = nullptr;
     hipEvent_t start = nullptr;
     hipEvent_t done[4] = {nullptr, nullptr, nullptr, nullptr};
 };
@@ -750,7 +765,8 @@ int lsq_streams(LsqStreams** out) {
                 hipEventCreateWithFlags(&L.done[b], hipEventDisableTiming) != hipSuccess)
                 return set_error(GS_EHIP, "stream creation failed");
         }
-        if (hipEventCreateWithFlags(&L.start, hipEventDisableTiming) != hipSuccess)
+        if (hipStreamCreateWithFlags(&L.cap, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&L.start, hipEventDisableTiming) != hipSuccess)
             return set_error(GS_EHIP, "event creation failed");
     }
     *out = &L;
@@ -773,18 +789,108 @@ int lsq_workspace(int which, size_t bytes, void** out) {
     return GS_OK;
 }
 
-// Workgroups of a bucket of `count` fits under shape `sh`.
-int64_t bucket_blocks(int64_t count, const LsqShape& sh) {
-    const int64_t fits_per_block = int64_t(sh.threads) * sh.fits_per_lane;
-    const int64_t blocks = (count + fits_per_block - 1) / fits_per_block;
-    return blocks < 1 ? 1 : blocks;
+template <int N> const void* bucket_kernel() { return reinterpret_cast<const void*>(&lsq_kernel<N>); }
+const void* bucket_kernel_of(int b) {
+    switch (b) {
+        case 0: return bucket_kernel<2>();
+        case 1: return bucket_kernel<4>();
+        case 2: return bucket_kernel<6>();
+        default: return bucket_kernel<8>();
+    }
 }
 
-template <int N> void launch_bucket(const LsqArgs& a, const LsqShape& sh, hipStream_t st) {
-    hipLaunchKernelGGL(lsq_kernel<N>, dim3(unsigned(bucket_blocks(a.count, sh))), dim3(sh.threads), 0, st, a);
+// Workgroups of bucket b under its shape for a batch of nfit fits: the resident workgroups the shape
+// asks for (the occupancy's when blocks_per_cu is 0), and no more lanes than fits.
+int lsq_bucket_blocks(int b, int64_t nfit, int64_t* out) {
+    const LsqShape& sh = g_lsq_shape[b];
+    int dev = 0, cus = 256, per_cu = sh.blocks_per_cu;
+    if (hipGetDevice(&dev) != hipSuccess) return set_error(GS_EHIP, "hipGetDevice failed");
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (per_cu == 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bucket_kernel_of(b), sh.threads, 0) !=
+                            hipSuccess || per_cu < 1))
+        per_cu = 1;
+    int64_t blocks = int64_t(per_cu) * cus;
+    const int64_t cap = (nfit + sh.threads - 1) / sh.threads;
+    *out = blocks < cap ? blocks : cap;
+    return GS_OK;
+}
+
+// The captured launch sequence of one device, and what it was captured for.
+struct LsqGraph {
+    hipGraphExec_t exec = nullptr;
+    std::string key;
+};
+LsqGraph g_lsq_graph[64];
+
+// The whole launch sequence on `st`: clear, count, scan, scatter, then the four bucket kernels on the
+// library's streams, forked from and joined back to `st` with events.
+int lsq_enqueue(hipStream_t st, const LsqStreams& L, const int32_t* K, const int32_t* m, const int64_t* y_offset,
+                int64_t ny, int64_t nfit, uint32_t* bins, int64_t words, LsqArgs a, const int64_t (&blocks)[4],
+                double* const (&slots)[4]) {
+    const size_t nb = size_t(kBins);
+    uint32_t* cursor = bins + nb;
+    uint32_t* bstart = cursor + nb;
+    uint32_t* queues = bstart + 5;
+    uint32_t* bad = queues + 4;
+    int32_t* list = reinterpret_cast<int32_t*>(bad + 1);
+    hipLaunchKernelGGL(lsq_clear, dim3(64), dim3(256), 0, st, bins, words);
+    const int gb = int((nfit + 255) / 256 < 4096 ? (nfit + 255) / 256 : 4096);
+    hipLaunchKernelGGL(lsq_count, dim3(gb), dim3(256), 0, st, K, m, y_offset, ny, nfit, bins, bad);
+    hipLaunchKernelGGL(lsq_scan, dim3(1), dim3(1024), 0, st, bins, cursor, bstart);
+    hipLaunchKernelGGL(lsq_scatter, dim3(gb), dim3(256), 0, st, K, m, nfit, cursor, list);
+    a.list = list;
+    a.bstart = bstart;
+    a.bad = bad;
+    if (hipEventRecord(L.start, st) != hipSuccess) return set_error(GS_EHIP, "hipEventRecord failed");
+    for (int b = 0; b < 4; ++b) {
+        a.bucket = b;
+        a.queue = queues + b;
+        a.slots = slots[b];
+        hipStream_t sb = L.s[b];
+        if (hipStreamWaitEvent(sb, L.start, 0) != hipSuccess) return set_error(GS_EHIP, "hipStreamWaitEvent failed");
+        const dim3 grid{unsigned(blocks[b]), 1, 1}, block{unsigned(g_lsq_shape[b].threads), 1, 1};
+        switch (b) {
+            case 0: hipLaunchKernelGGL(lsq_kernel<2>, grid, block, 0, sb, a); break;
+            case 1: hipLaunchKernelGGL(lsq_kernel<4>, grid, block, 0, sb, a); break;
+            case 2: hipLaunchKernelGGL(lsq_kernel<6>, grid, block, 0, sb, a); break;
+            case 3: hipLaunchKernelGGL(lsq_kernel<8>, grid, block, 0, sb, a); break;
+        }
+        if (hipEventRecord(L.done[b], sb) != hipSuccess || hipStreamWaitEvent(st, L.done[b], 0) != hipSuccess)
+            return set_error(GS_EHIP, "joining the bucket streams failed");
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(GS_EHIP, std::string("least-squares launch failed: ") + hipGetErrorString(e));
+    return GS_OK;
 }
 
 }  // namespace
+
+// Free the captured graphs, the library streams and events, and the workspaces of every device
+// (gs_shutdown; no call may be in flight). The next call creates them again.
+void release_lsq() {
+    std::lock_guard<std::mutex> lock(g_lsq_mutex);
+    for (int d = 0; d < 64; ++d) {
+        LsqGraph& G = g_lsq_graph[d];
+        LsqStreams& L = g_lsq_streams[d];
+        const bool used = G.exec || L.start || g_lsq_ws[d][0].ptr || g_lsq_ws[d][1].ptr;
+        if (!used || hipSetDevice(d) != hipSuccess) continue;
+        if (G.exec) (void)hipGraphExecDestroy(G.exec);
+        G.exec = nullptr;
+        G.key.clear();
+        for (int b = 0; b < 4; ++b) {
+            if (L.s[b]) (void)hipStreamDestroy(L.s[b]);
+            if (L.done[b]) (void)hipEventDestroy(L.done[b]);
+        }
+        if (L.cap) (void)hipStreamDestroy(L.cap);
+        if (L.start) (void)hipEventDestroy(L.start);
+        L = LsqStreams{};
+        for (auto& w : g_lsq_ws[d]) {
+            if (w.ptr) (void)hipFree(w.ptr);
+            w = LsqWorkspace{};
+        }
+    }
+}
+
 }  // namespace gs
 
 using namespace gs;
@@ -823,38 +929,23 @@ int gs_solve_expsum_lsq_f64(const gs_method* method, int64_t nfit, double* X, co
     hipStream_t st = static_cast<hipStream_t>(stream);
     // workspace: bins, cursors, 5 bucket starts, 4 queues, 1 error count, the ordered list
     const size_t nb = size_t(kBins);
-    const size_t words = 2 * nb + 5 + 4 + 1;
+    const int64_t words = int64_t(2 * nb + 5 + 4 + 1);
     void* ws = nullptr;
-    int rc = lsq_workspace(0, words * 4 + size_t(nfit) * 4, &ws);
+    int rc = lsq_workspace(0, size_t(words) * 4 + size_t(nfit) * 4, &ws);
     if (rc) return rc;
-    uint32_t* bins = static_cast<uint32_t*>(ws);
-    uint32_t* cursor = bins + nb;
-    uint32_t* bstart = cursor + nb;
-    uint32_t* queues = bstart + 5;
-    uint32_t* bad = queues + 4;
-    int32_t* list = reinterpret_cast<int32_t*>(bad + 1);
-    hipLaunchKernelGGL(lsq_clear, dim3(64), dim3(256), 0, st, bins, int64_t(words));
-    const int gb = int((nfit + 255) / 256 < 4096 ? (nfit + 255) / 256 : 4096);
-    hipLaunchKernelGGL(lsq_count, dim3(gb), dim3(256), 0, st, K, m, y_offset, ny, nfit, bins, bad);
-    hipLaunchKernelGGL(lsq_scan, dim3(1), dim3(1024), 0, st, bins, cursor, bstart);
-    hipLaunchKernelGGL(lsq_scatter, dim3(gb), dim3(256), 0, st, K, m, nfit, cursor, list);
-    uint32_t host[6];
-    if (hipMemcpyAsync(host, bstart, 5 * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipMemcpyAsync(host + 5, bad, 4, hipMemcpyDeviceToHost, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
-        return set_error(GS_EHIP, "least-squares bucketing failed");
-    if (host[5] != 0)
-        return set_error(GS_EINVAL, std::to_string(host[5]) + " fits have K outside 1:4, m outside 2K:" +
-                                        std::to_string(kMaxM) + " or data outside Y");
     // the lane slots of the four buckets (Slot<N>::kSize doubles per lane)
+    int64_t blocks[4];
     size_t slot_off[4], slot_total = 0;
     for (int b = 0; b < 4; ++b) {
         const int n = 2 * (b + 1);
-        const int64_t cnt = int64_t(host[b + 1] - host[b]);
+        if ((rc = lsq_bucket_blocks(b, nfit, &blocks[b]))) return rc;
         slot_off[b] = slot_total;
-        if (cnt) slot_total += size_t(bucket_blocks(cnt, g_lsq_shape[b]) * g_lsq_shape[b].threads) * size_t(n * (n + 1) / 2 + 2 * n);
+        slot_total += size_t(blocks[b] * g_lsq_shape[b].threads) * size_t(n * (n + 1) / 2 + 2 * n);
     }
     void* slot_ws = nullptr;
     if ((rc = lsq_workspace(1, slot_total * 8, &slot_ws))) return rc;
+    double* slots[4];
+    for (int b = 0; b < 4; ++b) slots[b] = static_cast<double*>(slot_ws) + slot_off[b];
     LsqArgs a;
     std::memset(&a, 0, sizeof(a));
     LsqParams& q = a.q;
@@ -881,43 +972,91 @@ int gs_solve_expsum_lsq_f64(const gs_method* method, int64_t nfit, double* X, co
     a.y_offset = y_offset;
     a.Y = Y;
     a.S = S;
-    // the buckets run concurrently on the library's streams, joined to the caller's stream
     LsqStreams* L = nullptr;
     if ((rc = lsq_streams(&L))) return rc;
-    if (hipEventRecord(L->start, st) != hipSuccess) return set_error(GS_EHIP, "hipEventRecord failed");
-    for (int b = 0; b < 4; ++b) {
-        const uint32_t cnt = host[b + 1] - host[b];
-        if (cnt == 0) continue;
-        a.list = list + host[b];
-        a.count = cnt;
-        a.queue = queues + b;
-        a.slots = static_cast<double*>(slot_ws) + slot_off[b];
-        hipStream_t sb = L->s[b];
-        if (hipStreamWaitEvent(sb, L->start, 0) != hipSuccess) return set_error(GS_EHIP, "hipStreamWaitEvent failed");
-        switch (b) {
-            case 0: launch_bucket<2>(a, g_lsq_shape[0], sb); break;
-            case 1: launch_bucket<4>(a, g_lsq_shape[1], sb); break;
-            case 2: launch_bucket<6>(a, g_lsq_shape[2], sb); break;
-            case 3: launch_bucket<8>(a, g_lsq_shape[3], sb); break;
+    uint32_t* bins = static_cast<uint32_t*>(ws);
+    const uint32_t* bad = bins + 2 * nb + 5 + 4;
+    if (g_lsq_graph_on) {
+        // one captured graph per device, replayed while the call's pointers, sizes, parameters and
+        // shapes are those it was captured with
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return set_error(GS_EHIP, "hipGetDevice failed");
+        const void* ptrs[8] = {X, K, m, y_offset, Y, S, ws, slot_ws};
+        std::string key;
+        key.append(reinterpret_cast<const char*>(&nfit), sizeof(nfit));
+        key.append(reinterpret_cast<const char*>(&ny), sizeof(ny));
+        key.append(reinterpret_cast<const char*>(ptrs), sizeof(ptrs));
+        key.append(reinterpret_cast<const char*>(&q), sizeof(q));
+        key.append(reinterpret_cast<const char*>(blocks), sizeof(blocks));
+        key.append(reinterpret_cast<const char*>(g_lsq_shape), sizeof(g_lsq_shape));
+        std::lock_guard<std::mutex> lock(g_lsq_mutex);
+        LsqGraph& G = g_lsq_graph[dev];
+        if (!G.exec || G.key != key) {
+            if (G.exec) (void)hipGraphExecDestroy(G.exec);
+            G.exec = nullptr;
+            G.key.clear();
+            if (hipStreamBeginCapture(L->cap, hipStreamCaptureModeThreadLocal) != hipSuccess)
+                return set_error(GS_EHIP, "hipStreamBeginCapture failed");
+            rc = lsq_enqueue(L->cap, *L, K, m, y_offset, ny, nfit, bins, words, a, blocks, slots);
+            hipGraph_t graph = nullptr;
+            const hipError_t ec = hipStreamEndCapture(L->cap, &graph);
+            if (rc) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return rc;
+            }
+            if (ec != hipSuccess || !graph) return set_error(GS_EHIP, "hipStreamEndCapture failed");
+            const hipError_t ei = hipGraphInstantiate(&G.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ei != hipSuccess) {
+                G.exec = nullptr;
+                return set_error(GS_EHIP, std::string("hipGraphInstantiate failed: ") + hipGetErrorString(ei));
+            }
+            G.key = key;
         }
-        if (hipEventRecord(L->done[b], sb) != hipSuccess || hipStreamWaitEvent(st, L->done[b], 0) != hipSuccess)
-            return set_error(GS_EHIP, "joining the bucket streams failed");
+        if (hipGraphLaunch(G.exec, st) != hipSuccess) return set_error(GS_EHIP, "hipGraphLaunch failed");
+    } else {
+        if ((rc = lsq_enqueue(st, *L, K, m, y_offset, ny, nfit, bins, words, a, blocks, slots))) return rc;
     }
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return set_error(GS_EHIP, std::string("least-squares launch failed: ") + hipGetErrorString(e));
-    e = hipStreamSynchronize(st);
+    uint32_t host_bad = 0;
+    if (hipMemcpyAsync(&host_bad, bad, 4, hipMemcpyDeviceToHost, st) != hipSuccess)
+        return set_error(GS_EHIP, "least-squares status copy failed");
+    hipError_t e = hipStreamSynchronize(st);
     if (e != hipSuccess) return set_error(GS_EHIP, std::string("hipStreamSynchronize failed: ") + hipGetErrorString(e));
+    if (host_bad != 0)
+        return set_error(GS_EINVAL, std::to_string(host_bad) + " fits have K outside 1:4, m outside 2K:" +
+                                        std::to_string(kMaxM) + " or data outside Y");
     return GS_OK;
 }
 
-// The launch shape of bucket k (n = 2k): threads per workgroup (64, 128, 192 or 256) and fits per
-// lane (1..64), as the autotuner of the Python package sets it.
-int gs_lsq_set_shape(int k, int threads, int fits_per_lane) {
+// The launch shape of bucket k (n = 2k): threads per workgroup (64, 128, 192 or 256) and resident
+// workgroups per CU (0: the kernel's occupancy, else 1..64), as the package's autotuner sets it.
+int gs_lsq_set_shape(int k, int threads, int blocks_per_cu) {
     if (k < 1 || k > 4) return set_error(GS_EINVAL, "k must be in 1:4");
     if (threads < 64 || threads > 256 || threads % 64) return set_error(GS_EINVAL, "threads must be 64, 128, 192 or 256");
-    if (fits_per_lane < 1 || fits_per_lane > 64) return set_error(GS_EINVAL, "fits_per_lane must be in 1:64");
+    if (blocks_per_cu < 0 || blocks_per_cu > 64) return set_error(GS_EINVAL, "blocks_per_cu must be in 0:64");
+    std::lock_guard<std::mutex> lock(g_lsq_mutex);
     g_lsq_shape[k - 1].threads = threads;
-    g_lsq_shape[k - 1].fits_per_lane = fits_per_lane;
+    g_lsq_shape[k - 1].blocks_per_cu = blocks_per_cu;
+    return GS_OK;
+}
+
+// The shape of bucket k and, for a batch of nfit fits, its workgroups (out[0..2]).
+int gs_lsq_get_shape(int k, int64_t nfit, int64_t out[3]) {
+    if (k < 1 || k > 4 || !out) return set_error(GS_EINVAL, "k must be in 1:4 and out not NULL");
+    int64_t blocks = 0;
+    const int rc = lsq_bucket_blocks(k - 1, nfit < 1 ? 1 : nfit, &blocks);
+    if (rc) return rc;
+    out[0] = g_lsq_shape[k - 1].threads;
+    out[1] = g_lsq_shape[k - 1].blocks_per_cu;
+    out[2] = blocks;
+    return GS_OK;
+}
+
+// Capture the least-squares launch sequence into a hipGraph and replay it (1, the default) or
+// launch it eagerly (0).
+int gs_lsq_set_graph(int enable) {
+    std::lock_guard<std::mutex> lock(g_lsq_mutex);
+    g_lsq_graph_on = enable != 0;
     return GS_OK;
 }
 

@@ -581,6 +581,20 @@ int check(hipError_t e, const char* what) {
 }
 
 }  // namespace
+
+// Free the training workspace and the fallback counter of every device (gs_shutdown; no call may be
+// in flight).
+void release_mlp() {
+    std::lock_guard<std::mutex> lock(g_mlp_mutex);
+    for (int d = 0; d < 64; ++d) {
+        for (MlpWorkspace* w : {&g_mlp_ws[d], &g_mlp_counter[d]}) {
+            if (!w->ptr) continue;
+            if (hipSetDevice(d) == hipSuccess) (void)hipFree(w->ptr);
+            *w = MlpWorkspace{};
+        }
+    }
+}
+
 }  // namespace gs
 
 using namespace gs;

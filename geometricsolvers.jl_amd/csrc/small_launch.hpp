@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <type_traits>
 
 #include "../../include/gsolve.h"
 #include "families.hpp"
@@ -16,17 +17,43 @@ namespace gs {
 
 int set_error(int code, const std::string& msg);
 
-struct KernelArgs {
+template <class T> struct KernelArgsT {
     FamArgs fam;
-    DevParams q;
+    DevParamsT<T> q;
     int64_t nsys;
-    double* X;
-    gs_status_f64* S;
+    T* X;
+    void* S;  // gs_status_f64 or gs_status_f32
     uint8_t* codes;
     const int64_t* order;
 };
+using KernelArgs = KernelArgsT<double>;
 
 // The status of one system in the byte layout of SolverStatus{Float64}, as nine 8-byte words.
+GS_DEV void store_status(gs_status_f64* dst, uint8_t code, int32_t iterations, double fnorm, double stepnorm,
+                         int32_t failures, bool promoted, double gradratio, double rate, bool contracted,
+                         bool reused);
+
+// The status of one system in the byte layout of SolverStatus{Float32} (44 bytes), as eleven 4-byte
+// words: code @0, f_calls @2, iterations @4, fnorm @8, stepnorm @12, step_failures @16, promoted @20,
+// stationary @21, jvp_calls @22, rank @24, nullnorm @28, gradratio @32, rate @36, contracted @40,
+// reused @41.
+GS_DEV void store_status(gs_status_f32* dst, uint8_t code, int32_t iterations, float fnorm, float stepnorm,
+                         int32_t failures, bool promoted, float gradratio, float rate, bool contracted,
+                         bool reused) {
+    uint32_t* w = reinterpret_cast<uint32_t*>(dst);
+    w[0] = uint32_t(code);
+    w[1] = uint32_t(iterations);
+    w[2] = __builtin_bit_cast(uint32_t, fnorm);
+    w[3] = __builtin_bit_cast(uint32_t, stepnorm);
+    w[4] = uint32_t(failures);
+    w[5] = uint32_t(promoted ? 1 : 0);
+    w[6] = uint32_t(-1);  // rank = -1: an LU factor reports no rank
+    w[7] = __builtin_bit_cast(uint32_t, tnan<float>());  // nullnorm
+    w[8] = __builtin_bit_cast(uint32_t, gradratio);
+    w[9] = __builtin_bit_cast(uint32_t, rate);
+    w[10] = uint32_t(contracted ? 1 : 0) | (uint32_t(reused ? 1 : 0) << 8);
+}
+
 GS_DEV void store_status(gs_status_f64* dst, uint8_t code, int32_t iterations, double fnorm, double stepnorm,
                          int32_t failures, bool promoted, double gradratio, double rate, bool contracted,
                          bool reused) {
@@ -58,15 +85,17 @@ template <int GLOB, bool MIXED, bool REFINE, bool REUSE> constexpr int small_wav
 template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE>
 __global__ __launch_bounds__(kSmallMaxWorkgroup)
 __attribute__((amdgpu_waves_per_eu(small_waves_per_eu<GLOB, MIXED, REFINE, REUSE>())))
-void solve_small_kernel(const KernelArgs a) {
+void solve_small_kernel(const KernelArgsT<typename Fam::T> a) {
+    using T = typename Fam::T;
+    using Status = typename std::conditional<sizeof(T) == 8, gs_status_f64, gs_status_f32>::type;
     constexpr int N = Fam::N;
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= a.nsys) return;
     const int64_t idx = a.order ? a.order[i] - 1 : i;
     Fam fam;
     fam.load(a.fam, idx);
-    double x[N];
-    double* xp = a.X + idx * N;
+    T x[N];
+    T* xp = a.X + idx * N;
 #pragma unroll
     for (int k = 0; k < N; ++k) x[k] = xp[k];
     SmallSolve<Fam, GLOB, MIXED, REFINE, DIAG, REUSE> solver(fam, a.q);
@@ -74,15 +103,15 @@ void solve_small_kernel(const KernelArgs a) {
 #pragma unroll
     for (int k = 0; k < N; ++k) xp[k] = x[k];
     if (DIAG) {
-        store_status(a.S + idx, R.code, R.iterations, R.fnorm, R.stepnorm, R.failures, R.promoted, R.gradratio,
-                     R.rate, R.contracted, R.reused);
+        store_status(static_cast<Status*>(a.S) + idx, R.code, R.iterations, R.fnorm, R.stepnorm, R.failures, R.promoted,
+                     R.gradratio, R.rate, R.contracted, R.reused);
     } else {
         a.codes[idx] = R.code;
     }
 }
 
 template <class Fam, int GLOB, bool MIXED, bool REFINE, bool REUSE>
-int launch_diag(const KernelArgs& a, bool diag, int wg, hipStream_t st) {
+int launch_diag(const KernelArgsT<typename Fam::T>& a, bool diag, int wg, hipStream_t st) {
     const int64_t blocks = (a.nsys + wg - 1) / wg;
     if (blocks > 0x7fffffffLL) return set_error(GS_EINVAL, "batch too large for one launch");
     if (diag)
@@ -94,26 +123,30 @@ int launch_diag(const KernelArgs& a, bool diag, int wg, hipStream_t st) {
     return GS_OK;
 }
 
-// The launch of one globalization's kernels for family Fam.
+// The launch of one globalization's kernels for family Fam. A Float32 batch has no mixed form: its
+// factor is in its working precision already (factor_eltype(LUFactorization{Float32}, Float32)).
 template <class Fam, int GLOB, bool REUSE>
-int launch_glob(const KernelArgs& a, bool mixed, bool refine, bool diag, int wg, hipStream_t st) {
-    if (!mixed && !refine) return launch_diag<Fam, GLOB, false, false, REUSE>(a, diag, wg, st);
-    if (!mixed && refine) return launch_diag<Fam, GLOB, false, true, REUSE>(a, diag, wg, st);
-    if (mixed && !refine) return launch_diag<Fam, GLOB, true, false, REUSE>(a, diag, wg, st);
-    return launch_diag<Fam, GLOB, true, true, REUSE>(a, diag, wg, st);
+int launch_glob(const KernelArgsT<typename Fam::T>& a, bool mixed, bool refine, bool diag, int wg, hipStream_t st) {
+    if constexpr (sizeof(typename Fam::T) == 8) {
+        if (mixed && !refine) return launch_diag<Fam, GLOB, true, false, REUSE>(a, diag, wg, st);
+        if (mixed) return launch_diag<Fam, GLOB, true, true, REUSE>(a, diag, wg, st);
+    }
+    if (!refine) return launch_diag<Fam, GLOB, false, false, REUSE>(a, diag, wg, st);
+    return launch_diag<Fam, GLOB, false, true, REUSE>(a, diag, wg, st);
 }
 
 // The family dispatch of one globalization (GS_FAM_* → Fam); EFAMILY for a family without a
 // lane-per-system form.
-template <int GLOB, bool REUSE>
-int launch_small_family(int family, const KernelArgs& a, bool mixed, bool refine, bool diag, int wg, hipStream_t st) {
+template <int GLOB, bool REUSE, class T>
+int launch_small_family(int family, const KernelArgsT<T>& a, bool mixed, bool refine, bool diag, int wg,
+                        hipStream_t st) {
     switch (family) {
-        case GS_FAM_COUPLED2: return launch_glob<FamCoupled2, GLOB, REUSE>(a, mixed, refine, diag, wg, st);
-        case GS_FAM_HETERO2: return launch_glob<FamHetero2, GLOB, REUSE>(a, mixed, refine, diag, wg, st);
-        case GS_FAM_SCALAR_SQUARE: return launch_glob<FamScalarSquare, GLOB, REUSE>(a, mixed, refine, diag, wg, st);
-        case GS_FAM_TANH_LINEAR3: return launch_glob<FamTanhLinear3, GLOB, REUSE>(a, mixed, refine, diag, wg, st);
+        case GS_FAM_COUPLED2: return launch_glob<FamCoupled2T<T>, GLOB, REUSE>(a, mixed, refine, diag, wg, st);
+        case GS_FAM_HETERO2: return launch_glob<FamHetero2T<T>, GLOB, REUSE>(a, mixed, refine, diag, wg, st);
+        case GS_FAM_SCALAR_SQUARE: return launch_glob<FamScalarSquareT<T>, GLOB, REUSE>(a, mixed, refine, diag, wg, st);
+        case GS_FAM_TANH_LINEAR3: return launch_glob<FamTanhLinear3T<T>, GLOB, REUSE>(a, mixed, refine, diag, wg, st);
         case GS_FAM_HENON_HEILES_MIDPOINT:
-            return launch_glob<FamHenonHeilesMidpoint, GLOB, REUSE>(a, mixed, refine, diag, wg, st);
+            return launch_glob<FamHenonHeilesMidpointT<T>, GLOB, REUSE>(a, mixed, refine, diag, wg, st);
         default: return set_error(GS_EFAMILY, "problem family is not registered");
     }
 }
@@ -124,5 +157,9 @@ int launch_small_reuse(int family, int glob, const KernelArgs& a, bool mixed, bo
 
 // Defined in small_dogleg.hip: DogLeg.
 int launch_small_dogleg(int family, const KernelArgs& a, bool mixed, bool refine, bool diag, int wg, hipStream_t st);
+
+// Defined in small_f32.hip: every form of a Float32 batch.
+int launch_small_f32(int family, int glob, bool reuse, const KernelArgsT<float>& a, bool refine, bool diag, int wg,
+                     hipStream_t st);
 
 }  // namespace gs

@@ -21,33 +21,35 @@ namespace gs {
 
 enum : uint8_t { kSuccess = 0, kStalled = 1, kMaxIters = 2, kSingular = 3, kNonfinite = 4, kLsFailed = 5 };
 
-// The runtime part of the method and of Options, in the form the kernel reads.
-struct DevParams {
+// The runtime part of the method and of Options, in the working precision T (Options{R} and the
+// method adapted with ToReal{R}, whose floats are R(x)).
+template <class T> struct DevParamsT {
     // Options (src/base/options.jl:58-71)
-    double f_abstol, f_reltol, x_abstol, x_reltol, kappa;
+    T f_abstol, f_reltol, x_abstol, x_reltol, kappa;
     int32_t min_iterations, max_iterations, max_stalls;
     int32_t norm_max;      // 1 = MaxNorm
     int32_t roundoff, rate_test;
     // method
-    double alpha;          // Static(α)
-    double c1, p;          // Backtracking
+    T alpha;               // Static(α)
+    T c1, p;               // Backtracking
     int32_t bt_maxiter;
     int32_t ref_maxiter;
-    double ref_eta_eff;    // max(η, √eps)
-    double mu;             // regularization
+    T ref_eta_eff;         // max(η, √eps)
+    T mu;                  // regularization
     int32_t best;
     int32_t reuse;         // Newton(; reuse = k): k ≥ 1 (the REUSE kernels only)
-    double dl_radius;      // DogLeg(; radius): the radius each solve starts from, 0 = unset
+    T dl_radius;           // DogLeg(; radius): the radius each solve starts from, 0 = unset
 };
+using DevParams = DevParamsT<double>;
 
 enum { kStatic = 0, kBacktracking = 1, kDogLeg = 2 };
 
-// The outcome of one step (StepInfo, src/base/status.jl:220-234; `limited` is always false here).
-struct Step {
+// The outcome of one step (StepInfo, src/base/status.jl:220-234).
+template <class T> struct StepT {
     uint8_t code;
     bool promoted, shortened;
     int32_t failures;
-    double fnorm, stepnorm;
+    T fnorm, stepnorm;
     bool limited;  // a trust-region radius, not the distance to a root, set the step's norm
 };
 
@@ -63,10 +65,10 @@ template <class TF, int N> GS_DEV int lu_inplace(TF (&A)[N][N], int (&piv)[N]) {
 #pragma unroll
     for (int k = 0; k < N; ++k) {
         int p = k;
-        TF amax = __builtin_fabs(A[k][k]);
+        TF amax = tabs(A[k][k]);
 #pragma unroll
         for (int i = k + 1; i < N; ++i) {
-            TF a = __builtin_fabs(A[i][k]);
+            TF a = tabs(A[i][k]);
             if (a > amax) {
                 amax = a;
                 p = i;
@@ -140,14 +142,14 @@ template <class TF, int N> struct Factor {
 
 // factorize!! (lu.jl:24-64) after factor_matrix (factors.jl:802-811): in the working precision only
 // e < 0 divides; a narrower TF divides for every e ≠ 0.
-template <class TF, int N> GS_DEV void factorize(const double (&J)[N][N], Factor<TF, N>& f) {
-    double flat[N * N];
+template <class TF, int N, class T> GS_DEV void factorize(const T (&J)[N][N], Factor<TF, N>& f) {
+    T flat[N * N];
 #pragma unroll
     for (int i = 0; i < N; ++i)
 #pragma unroll
         for (int j = 0; j < N; ++j) flat[i * N + j] = J[i][j];
-    int e = max_exponent<N * N>(flat, kEmax64);
-    constexpr bool same = sizeof(TF) == sizeof(double);
+    int e = max_exponent<N * N>(flat, Lim<T>::emax);
+    constexpr bool same = sizeof(TF) == sizeof(T);
     bool scale = e != 0 && (!same || e < 0);
     f.e = scale ? e : 0;
     // the prescale is exact, and a wave in which no lane scales skips it
@@ -167,7 +169,7 @@ template <class TF, int N> GS_DEV void factorize(const double (&J)[N][N], Factor
 }
 
 // ldiv!! in the working precision (factors.jl:931-936).
-template <int N> GS_DEV void ldiv_same(const Factor<double, N>& f, const double (&r)[N], double (&d)[N]) {
+template <int N, class T> GS_DEV void ldiv_same(const Factor<T, N>& f, const T (&r)[N], T (&d)[N]) {
     const int k = rhs_exponent<N>(r, f.e);
     if (__any(k != 0)) {
 #pragma unroll
@@ -176,7 +178,7 @@ template <int N> GS_DEV void ldiv_same(const Factor<double, N>& f, const double 
 #pragma unroll
         for (int i = 0; i < N; ++i) d[i] = r[i];
     }
-    lu_backsolve<double, N>(f.A, f.piv, d);
+    lu_backsolve<T, N>(f.A, f.piv, d);
     const int u = k - f.e;
     if (__any(u != 0)) {
 #pragma unroll
@@ -204,52 +206,52 @@ template <int N> GS_DEV void ldiv_mixed(const Factor<float, N>& f, const double 
     for (int i = 0; i < N; ++i) d[i] = ldexp2(double(t[i]) * frac, ex - f.e);
 }
 
-template <int N> GS_DEV void ldiv(const Factor<double, N>& f, const double (&r)[N], double (&d)[N]) { ldiv_same<N>(f, r, d); }
+template <int N, class T> GS_DEV void ldiv(const Factor<T, N>& f, const T (&r)[N], T (&d)[N]) { ldiv_same<N>(f, r, d); }
 template <int N> GS_DEV void ldiv(const Factor<float, N>& f, const double (&r)[N], double (&d)[N]) { ldiv_mixed<N>(f, r, d); }
 
 // StaticArrays' SMatrix × SVector: entry i = fma-fold of row i (combine_products' muladd chain).
-template <int N> GS_DEV void matvec(const double (&A)[N][N], const double (&v)[N], double (&y)[N]) {
+template <int N, class T> GS_DEV void matvec(const T (&A)[N][N], const T (&v)[N], T (&y)[N]) {
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-        double acc = A[i][0] * v[0];
+        T acc = A[i][0] * v[0];
 #pragma unroll
-        for (int j = 1; j < N; ++j) acc = __builtin_fma(A[i][j], v[j], acc);
+        for (int j = 1; j < N; ++j) acc = tfma(A[i][j], v[j], acc);
         y[i] = acc;
     }
 }
 
 // ‖Jᴴ F‖/(‖J‖_F ‖F‖) (grad_ratio_and_norm, newton.jl:1287-1298).
-template <int N> GS_DEV double grad_ratio_scaled(const double (&J)[N][N], const double (&Fx)[N], double jnorm, double fnorm) {
+template <int N, class T> GS_DEV T grad_ratio_scaled(const T (&J)[N][N], const T (&Fx)[N], T jnorm, T fnorm) {
     int e = exponent_of(fnorm);
-    double sc = ldexp2(1.0, -e);
-    double g[N];
+    T sc = ldexp2(T(1), -e);
+    T g[N];
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-        double acc = J[0][k] * (Fx[0] * sc);
+        T acc = J[0][k] * (Fx[0] * sc);
 #pragma unroll
-        for (int j = 1; j < N; ++j) acc = __builtin_fma(J[j][k], Fx[j] * sc, acc);
+        for (int j = 1; j < N; ++j) acc = tfma(J[j][k], Fx[j] * sc, acc);
         g[k] = acc;
     }
     return rnorm<N>(g) / jnorm / ldexp2(fnorm, -e);
 }
 
-template <int N> GS_DEV double grad_ratio(const double (&J)[N][N], const double (&Fx)[N], double fnorm) {
-    double flat[N * N];
+template <int N, class T> GS_DEV T grad_ratio(const T (&J)[N][N], const T (&Fx)[N], T fnorm) {
+    T flat[N * N];
 #pragma unroll
     for (int j = 0; j < N; ++j)
 #pragma unroll
         for (int i = 0; i < N; ++i) flat[i + j * N] = J[i][j];
-    double jnorm = rnorm<N * N>(flat);
-    if (jnorm == 0.0 || fnorm == 0.0) return 0.0;
-    double g[N];
+    T jnorm = rnorm<N * N>(flat);
+    if (jnorm == T(0) || fnorm == T(0)) return T(0);
+    T g[N];
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-        double acc = J[0][k] * Fx[0];
+        T acc = J[0][k] * Fx[0];
 #pragma unroll
-        for (int j = 1; j < N; ++j) acc = __builtin_fma(J[j][k], Fx[j], acc);
+        for (int j = 1; j < N; ++j) acc = tfma(J[j][k], Fx[j], acc);
         g[k] = acc;
     }
-    double gnorm = rnorm<N>(g);
+    T gnorm = rnorm<N>(g);
     if (__builtin_expect(isfin(gnorm) || !(isfin(jnorm) && isfin(fnorm)), 1)) return gnorm / jnorm / fnorm;
     return grad_ratio_scaled<N>(J, Fx, jnorm, fnorm);
 }
@@ -260,14 +262,16 @@ template <int N> GS_DEV double grad_ratio(const double (&J)[N][N], const double 
 
 template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = false> struct SmallSolve {
     static constexpr int N = Fam::N;
-    using TF = typename std::conditional<MIXED, float, double>::type;
+    using T = typename Fam::T;  // the working precision
+    using TF = typename std::conditional<MIXED, float, T>::type;
+    using Step = StepT<T>;
     const Fam& fam;
-    const DevParams& q;
+    const DevParamsT<T>& q;
 
     // the per-solve state (NewtonState, newton.jl:715-744, as the R1 value state holds it)
-    double merit;       // norm2(Fx) for Backtracking
-    double sfnorm;      // ‖Fx‖ that the state carries
-    double gradratio;   // diagnostics of the last factorisation
+    T merit;       // norm2(Fx) for Backtracking
+    T sfnorm;      // ‖Fx‖ that the state carries
+    T gradratio;   // diagnostics of the last factorisation
     bool promoted;      // mixed precision: factorising in FP64
     int slow_steps;
     // reuse = k: the factor the state holds, in the factor precision, and the rule state FactorAge
@@ -277,14 +281,14 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
     int age;
     bool stalled, reused;
     // DogLeg: the trust radius (TrustRadius, newton.jl:637-640), 0 until the first step sets it
-    double radius;
+    T radius;
 
-    GS_DEV SmallSolve(const Fam& f, const DevParams& p) : fam(f), q(p) {}
+    GS_DEV SmallSolve(const Fam& f, const DevParamsT<T>& p) : fam(f), q(p) {}
 
     // residual_norm (newton.jl:1274-1278)
-    GS_DEV double residual_norm(const double (&Fx)[N]) const {
+    GS_DEV T residual_norm(const T (&Fx)[N]) const {
         if (GLOB == kBacktracking) {
-            if (isfin(merit) && merit >= double(N) * kDblMin) return __builtin_sqrt(merit);
+            if (isfin(merit) && merit >= T(N) * Lim<T>::min) return tsqrt(merit);
             return rnorm<N>(Fx);
         }
         return isnan_(sfnorm) ? rnorm<N>(Fx) : sfnorm;
@@ -293,16 +297,16 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
     // --- globalization --------------------------------------------------------------------------
 
     // Static (linefunction.jl:114-131).
-    GS_DEV Step step_static(double (&x)[N], double (&Fx)[N], const double (&d)[N]) const {
-        double s = q.alpha;
-        double dn = rnorm<N>(d);
-        double stepnorm = __builtin_fabs(s) * dn;
+    GS_DEV Step step_static(T (&x)[N], T (&Fx)[N], const T (&d)[N]) const {
+        T s = q.alpha;
+        T dn = rnorm<N>(d);
+        T stepnorm = tabs(s) * dn;
         if (!isfin(stepnorm)) return Step{kNonfinite, false, false, 0, rnorm<N>(Fx), stepnorm};
-        double xp[N], Fp[N];
+        T xp[N], Fp[N];
 #pragma unroll
         for (int i = 0; i < N; ++i) xp[i] = x[i] + s * d[i];
         fam.F(xp, Fp);
-        double fn = rnorm<N>(Fp);
+        T fn = rnorm<N>(Fp);
         if (!isfin(fn)) return Step{kNonfinite, false, false, 0, rnorm<N>(Fx), stepnorm};
 #pragma unroll
         for (int i = 0; i < N; ++i) {
@@ -313,78 +317,78 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
     }
 
     // backtrack_step (backtracking.jl:320-340)
-    GS_DEV static double backtrack_step(double phi0, double d0, double a, double pa, double ap, double pp, double p) {
-        if (!isfin(pa)) return 0.1 * a;
-        double an = dnan();
-        double s = __builtin_fabs(d0);
-        double r1 = (pa - phi0) / s + a;
-        if (!(ap > 0.0)) {
-            if (r1 > 0.0) an = (a * a) / (2.0 * r1);
+    GS_DEV static T backtrack_step(T phi0, T d0, T a, T pa, T ap, T pp, T p) {
+        if (!isfin(pa)) return T(0.1) * a;
+        T an = tnan<T>();
+        T s = tabs(d0);
+        T r1 = (pa - phi0) / s + a;
+        if (!(ap > T(0.0))) {
+            if (r1 > T(0.0)) an = (a * a) / (T(2.0) * r1);
         } else if (isfin(pp) && a != ap) {
-            double q1 = r1 / (a * a);
-            double q2 = ((pp - phi0) / s + ap) / (ap * ap);
-            double A = (q1 - q2) / (a - ap);
-            double B = (a * q2 - ap * q1) / (a - ap);
-            double disc = B * B + 3.0 * A;
-            if (disc >= 0.0) an = B > 0.0 ? 1.0 / (B + __builtin_sqrt(disc)) : (-B + __builtin_sqrt(disc)) / (3.0 * A);
+            T q1 = r1 / (a * a);
+            T q2 = ((pp - phi0) / s + ap) / (ap * ap);
+            T A = (q1 - q2) / (a - ap);
+            T B = (a * q2 - ap * q1) / (a - ap);
+            T disc = B * B + T(3.0) * A;
+            if (disc >= T(0.0)) an = B > T(0.0) ? T(1.0) / (B + tsqrt(disc)) : (-B + tsqrt(disc)) / (T(3.0) * A);
         }
-        if (!(isfin(an) && an > 0.0)) an = p * a;
-        return jmin(jmax(an, 0.1 * a), p * a);
+        if (!(isfin(an) && an > T(0.0))) an = p * a;
+        return jmin(jmax(an, T(0.1) * a), p * a);
     }
 
     // globalize_merit!! for Backtracking (linefunction.jl:159-199 → linesearch.jl:154-171 →
     // backtracking.jl:354-390). `exact`: ExactStep (slope −2φ₀) or MeasuredSlope (φ′(0) = 2⟨F, J d⟩).
-    GS_DEV Step step_backtracking(double (&x)[N], double (&Fx)[N], const double (&d)[N], bool exact) {
-        const double phi0 = merit;
-        double dn = rnorm<N>(d);
+    GS_DEV Step step_backtracking(T (&x)[N], T (&Fx)[N], const T (&d)[N], bool exact) {
+        const T phi0 = merit;
+        T dn = rnorm<N>(d);
         if (!isfin(dn)) return Step{kNonfinite, false, false, 0, rnorm<N>(Fx), dn};
-        double d0;
+        T d0;
         if (exact) {
-            d0 = -2.0 * phi0;
+            d0 = -T(2.0) * phi0;
         } else {
-            double xa[N], Fa[N], Ja[N][N], Jv[N];
+            T xa[N], Fa[N], Ja[N][N], Jv[N];
 #pragma unroll
-            for (int i = 0; i < N; ++i) xa[i] = x[i] + 0.0 * d[i];
+            for (int i = 0; i < N; ++i) xa[i] = x[i] + T(0.0) * d[i];
             fam.F(xa, Fa);
             fam.J(xa, Ja);
             matvec<N>(Ja, d, Jv);
-            d0 = 2.0 * rdot<N>(Fa, Jv);
+            d0 = T(2.0) * rdot<N>(Fa, Jv);
         }
-        if (!(isfin(phi0) && isfin(d0) && d0 < 0.0)) {
-            uint8_t c = !(isfin(phi0) && isfin(d0)) ? kNonfinite : (d0 > 0.0 ? kLsFailed : kStalled);
+        if (!(isfin(phi0) && isfin(d0) && d0 < T(0.0))) {
+            uint8_t c = !(isfin(phi0) && isfin(d0)) ? kNonfinite : (d0 > T(0.0) ? kLsFailed : kStalled);
             // a LINESEARCH_FAILED anchor keeps x with φ = NaN: the failure takes no step
-            if (c == kLsFailed) return Step{kLsFailed, false, false, 1, rnorm<N>(Fx), 0.0};
-            return Step{c, false, false, 0, rnorm<N>(Fx), 0.0};
+            if (c == kLsFailed) return Step{kLsFailed, false, false, 1, rnorm<N>(Fx), T(0.0)};
+            return Step{c, false, false, 0, rnorm<N>(Fx), T(0.0)};
         }
-        const double tau = __builtin_fmax(4.0 * kDblEps * __builtin_fabs(phi0), 4.0 * kDblTiny);
-        const double amin = jmax(tau / __builtin_fabs(d0), kDblMin);
-        double alpha = __builtin_fmin(__builtin_fmax(1.0, amin), __builtin_inf());
-        double aa = alpha, pa = dnan();
-        double Fa[N];
-        double ap = dnan(), pp = dnan();
+        const T tau = tfmax(T(4.0) * Lim<T>::eps * tabs(phi0), T(4.0) * Lim<T>::tiny);
+        const T amin = jmax(tau / tabs(d0), Lim<T>::min);
+        T alpha = tfmin(tfmax(T(1.0), amin), T(__builtin_inf()));
+        T aa = alpha, pa = tnan<T>();
+        T Fa[N];
+        T ap = tnan<T>(), pp = tnan<T>();
         int frozen = 0;
         uint8_t code = kLsFailed;
-        const double sqrteps = 1.4901161193847656e-8;
+        const T sqrteps = Lim<T>::sqrteps;
         for (int trips = 0; trips < q.bt_maxiter; ++trips) {
             aa = alpha;
-            double xa[N];
+            T xa[N];
 #pragma unroll
             for (int i = 0; i < N; ++i) xa[i] = x[i] + alpha * d[i];
             fam.F(xa, Fa);
             pa = norm2<N>(Fa);
             frozen = (pa == phi0 && alpha <= sqrteps) ? frozen + 1 : 0;
-            if (pa - phi0 <= jmin(0.0, q.c1 * alpha * d0 + tau)) {
-                code = (isfin(pa) && pa - phi0 < -tau) ? kSuccess : (__builtin_fabs(pa - phi0) <= tau ? kStalled : kLsFailed);
+            if (pa - phi0 <= jmin(T(0.0), q.c1 * alpha * d0 + tau)) {
+                code = (isfin(pa) && pa - phi0 < -tau) ? kSuccess : (tabs(pa - phi0) <= tau ? kStalled : kLsFailed);
                 break;
             } else if (frozen >= 2) {
                 code = kStalled;
                 break;
             } else if (alpha <= amin) {
-                code = __builtin_fabs(pa - phi0) <= tau ? kStalled : kLsFailed;
+                code = tabs(pa - phi0) <= tau ? kStalled : kLsFailed;
                 break;
             }
-            double an = isfin(pa) ? backtrack_step(phi0, d0, alpha, pa, ap, pp, q.p)
-                                  : jmin(0.1 * alpha, __builtin_sqrt(alpha) * __builtin_sqrt(amin));
+            T an = isfin(pa) ? backtrack_step(phi0, d0, alpha, pa, ap, pp, q.p)
+                                  : jmin(T(0.1) * alpha, tsqrt(alpha) * tsqrt(amin));
             ap = alpha;
             pp = pa;
             alpha = jmax(an, amin);
@@ -396,7 +400,7 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
                 Fx[i] = Fa[i];
             }
             merit = pa;
-            return Step{kSuccess, false, aa < 1.0, 0, __builtin_sqrt(pa), aa * dn};
+            return Step{kSuccess, false, aa < T(1.0), 0, tsqrt(pa), aa * dn};
         }
         if (code == kLsFailed) {
             if (isfin(pa) && pa < phi0) {
@@ -406,14 +410,14 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
                     Fx[i] = Fa[i];
                 }
                 merit = pa;
-                return Step{kLsFailed, false, aa < 1.0, 1, __builtin_sqrt(pa), aa * dn};
+                return Step{kLsFailed, false, aa < T(1.0), 1, tsqrt(pa), aa * dn};
             }
-            return Step{kLsFailed, false, false, 1, rnorm<N>(Fx), 0.0};
+            return Step{kLsFailed, false, false, 1, rnorm<N>(Fx), T(0.0)};
         }
-        return Step{code == kStalled ? kStalled : kNonfinite, false, false, 0, rnorm<N>(Fx), 0.0};
+        return Step{code == kStalled ? kStalled : kNonfinite, false, false, 0, rnorm<N>(Fx), T(0.0)};
     }
 
-    GS_DEV Step globalize(double (&x)[N], double (&Fx)[N], const double (&d)[N], bool exact) {
+    GS_DEV Step globalize(T (&x)[N], T (&Fx)[N], const T (&d)[N], bool exact) {
         if (GLOB == kStatic) return step_static(x, Fx, d);
         return step_backtracking(x, Fx, d, exact);
     }
@@ -423,43 +427,43 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
     // The product of a refinement (newton.jl:1344-1384): the held regularised Jacobian J of a fresh
     // factor, or, along a reused factor (JVP), the JVP at x with J = J(x): 1·J(x) v + μ v where μ > 0
     // (RegularizedJVP).
-    template <bool JVP> GS_DEV void product(const double (&J)[N][N], const double (&v)[N], double (&Jv)[N]) const {
+    template <bool JVP> GS_DEV void product(const T (&J)[N][N], const T (&v)[N], T (&Jv)[N]) const {
         matvec<N>(J, v, Jv);
-        if (JVP && q.mu != 0.0) {
+        if (JVP && q.mu != T(0.0)) {
 #pragma unroll
-            for (int i = 0; i < N; ++i) Jv[i] = 1.0 * Jv[i] + q.mu * v[i];
+            for (int i = 0; i < N; ++i) Jv[i] = T(1.0) * Jv[i] + q.mu * v[i];
         }
     }
 
     // refine_passes!! (refinement.jl:211-244) with the product of `product<JVP>(J, ·)`.
     template <class TFR, bool JVP = false>
-    GS_DEV void refine(double (&d)[N], const Factor<TFR, N>& f, const double (&J)[N][N], const double (&Fx)[N], bool& failed) const {
+    GS_DEV void refine(T (&d)[N], const Factor<TFR, N>& f, const T (&J)[N][N], const T (&Fx)[N], bool& failed) const {
         failed = false;
-        double fnorm = rnorm<N>(Fx);
-        if (fnorm == 0.0) return;
+        T fnorm = rnorm<N>(Fx);
+        if (fnorm == T(0.0)) return;
         if (!isfin(fnorm)) {
             failed = true;
             return;
         }
-        double Jv[N], r[N];
+        T Jv[N], r[N];
         product<JVP>(J, d, Jv);
 #pragma unroll
         for (int i = 0; i < N; ++i) r[i] = -Fx[i] + -Jv[i];
-        double rprev = rnorm<N>(r);
+        T rprev = rnorm<N>(r);
         if (!isfin(rprev)) {
             failed = true;
             return;
         }
         if (rprev <= q.ref_eta_eff * fnorm) return;
         for (int it = 1; it <= q.ref_maxiter; ++it) {
-            double dd[N], dc[N];
+            T dd[N], dc[N];
             ldiv<N>(f, r, dd);
 #pragma unroll
             for (int i = 0; i < N; ++i) dc[i] = d[i] + dd[i];
             product<JVP>(J, dc, Jv);
 #pragma unroll
             for (int i = 0; i < N; ++i) r[i] = -Fx[i] + -Jv[i];
-            double rcur = rnorm<N>(r);
+            T rcur = rnorm<N>(r);
             if (!(rcur < rprev)) {
                 failed = true;
                 return;
@@ -474,39 +478,39 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
 
     // --- steps ----------------------------------------------------------------------------------
 
-    GS_DEV void jacobian(const double (&x)[N], const double (&Fx)[N], double (&J)[N][N]) {
+    GS_DEV void jacobian(const T (&x)[N], const T (&Fx)[N], T (&J)[N][N]) {
         if (DIAG) {
-            double fnorm = residual_norm(Fx);
+            T fnorm = residual_norm(Fx);
             fam.J(x, J);
             gradratio = grad_ratio<N>(J, Fx, fnorm);
         } else {
             fam.J(x, J);
         }
-        if (q.mu != 0.0) {
+        if (q.mu != T(0.0)) {
 #pragma unroll
             for (int i = 0; i < N; ++i) J[i][i] = J[i][i] + q.mu;
         }
     }
 
     // newton_step!! in the working precision (newton.jl:1679-1688, 1302-1311, 1396-1416).
-    GS_DEV Step plain_step(double (&x)[N], double (&Fx)[N]) {
-        double J[N][N];
+    GS_DEV Step plain_step(T (&x)[N], T (&Fx)[N]) {
+        T J[N][N];
         jacobian(x, Fx, J);
-        Factor<double, N> f;
-        factorize<double, N>(J, f);
+        Factor<T, N> f;
+        factorize<T, N>(J, f);
         Step s;
         if (f.info > 0) {
-            s = Step{kSingular, false, false, 0, rnorm<N>(Fx), 0.0};
+            s = Step{kSingular, false, false, 0, rnorm<N>(Fx), T(0.0)};
         } else {
-            double d[N];
+            T d[N];
             ldiv<N>(f, Fx, d);
 #pragma unroll
             for (int i = 0; i < N; ++i) d[i] = -d[i];
             if (REFINE) {
                 bool failed;
-                refine<double>(d, f, J, Fx, failed);
+                refine<T>(d, f, J, Fx, failed);
             }
-            bool exact = q.mu == 0.0 && !REFINE;
+            bool exact = q.mu == T(0.0) && !REFINE;
             s = globalize(x, Fx, d, exact);
         }
         sfnorm = s.fnorm;
@@ -517,15 +521,15 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
     // refinement, or the FP64 one of a promoted state; a failed FP32 factor or refinement promotes
     // and refactorises the held Jacobian in FP64. J receives the regularised Jacobian; returns
     // whether a direction exists (md.ok).
-    GS_DEV bool mixed_newton(const double (&x)[N], const double (&Fx)[N], double fnorm_old, double (&J)[N][N],
-                             double (&d)[N], int& failures, double& lin_rel) {
+    GS_DEV bool mixed_newton(const T (&x)[N], const T (&Fx)[N], T fnorm_old, T (&J)[N][N],
+                             T (&d)[N], int& failures, T& lin_rel) {
         jacobian(x, Fx, J);
         bool ok = true;
         failures = 0;
-        lin_rel = 0.0;
+        lin_rel = T(0.0);
         if (promoted) {
-            Factor<double, N> f;
-            factorize<double, N>(J, f);
+            Factor<T, N> f;
+            factorize<T, N>(J, f);
             if (f.info > 0) {
                 ok = false;
             } else {
@@ -534,7 +538,7 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
                 for (int i = 0; i < N; ++i) d[i] = -d[i];
                 if (REFINE) {
                     bool failed;
-                    refine<double>(d, f, J, Fx, failed);
+                    refine<T>(d, f, J, Fx, failed);
                 }
             }
         } else {
@@ -553,8 +557,8 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
                 // promote_factor!! (newton.jl:1332-1339): refactorise the held Jacobian in FP64
                 failures += 1;
                 promoted = true;
-                Factor<double, N> g;
-                factorize<double, N>(J, g);
+                Factor<T, N> g;
+                factorize<T, N>(J, g);
                 if (g.info > 0) {
                     ok = false;
                 } else {
@@ -562,22 +566,22 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
 #pragma unroll
                     for (int i = 0; i < N; ++i) d[i] = -d[i];
                 }
-            } else if (fnorm_old != 0.0 && isfin(fnorm_old)) {
+            } else if (fnorm_old != T(0.0) && isfin(fnorm_old)) {
                 // linear_residual_ratio (newton.jl:1422-1430)
-                double Jd[N], lin[N];
+                T Jd[N], lin[N];
                 matvec<N>(J, d, Jd);
 #pragma unroll
-                for (int i = 0; i < N; ++i) lin[i] = 1.0 * Fx[i] + 1.0 * Jd[i];
+                for (int i = 0; i < N; ++i) lin[i] = T(1.0) * Fx[i] + T(1.0) * Jd[i];
                 lin_rel = rnorm<N>(lin) / fnorm_old;
             }
         }
         return ok;
     }
 
-    // read_mixed_stall (newton.jl:1437-1451), η_lin = max(√eps, 1e-4): two slow steps whose FP32
+    // read_mixed_stall (newton.jl:1437-1451), η_lin = max(√eps, T(1e-4)): two slow steps whose FP32
     // linear model is inaccurate promote.
-    GS_DEV void read_mixed_stall(Step& s, double fnorm_old, double lin_rel) {
-        if (s.fnorm > 0.5 * fnorm_old && lin_rel > 1e-4) {
+    GS_DEV void read_mixed_stall(Step& s, T fnorm_old, T lin_rel) {
+        if (s.fnorm > T(0.5) * fnorm_old && lin_rel > T(1e-4)) {
             int slow = slow_steps + 1;
             if (slow >= 2) {
                 promoted = true;
@@ -593,18 +597,18 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
     }
 
     // mixed_step!! / mixed_newton!! with fresh factors (newton.jl:1604-1666, 1692-1720).
-    GS_DEV Step mixed_step(double (&x)[N], double (&Fx)[N]) {
-        double fnorm_old = residual_norm(Fx);
+    GS_DEV Step mixed_step(T (&x)[N], T (&Fx)[N]) {
+        T fnorm_old = residual_norm(Fx);
         const bool was_promoted = promoted;
-        double J[N][N], d[N];
+        T J[N][N], d[N];
         int failures;
-        double lin_rel;
+        T lin_rel;
         if (!mixed_newton(x, Fx, fnorm_old, J, d, failures, lin_rel)) {
-            Step s{kSingular, promoted, false, failures, rnorm<N>(Fx), 0.0};
+            Step s{kSingular, promoted, false, failures, rnorm<N>(Fx), T(0.0)};
             return s;
         }
         const bool promoted_step = promoted && !was_promoted;
-        const bool exact = q.mu == 0.0 && promoted;
+        const bool exact = q.mu == T(0.0) && promoted;
         Step s = globalize(x, Fx, d, exact);
         s.failures += failures;
         if (promoted_step) {
@@ -619,47 +623,47 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
 
     // --- DogLeg (dogleg.jl:116-206, newton.jl:1725-1780) ----------------------------------------
 
-    GS_DEV static double dl_clamp(double v) { return v > 1e10 ? 1e10 : (v < 1e-6 ? 1e-6 : v); }
-    GS_DEV static double dl_shrink(double v) { return jmax(0.25 * v, 1e-6); }
-    GS_DEV static double dl_grow(double v) { return jmin(4.0 * v, 1e10); }
+    GS_DEV static T dl_clamp(T v) { return v > T(1e10) ? T(1e10) : (v < T(1e-6) ? T(1e-6) : v); }
+    GS_DEV static T dl_shrink(T v) { return jmax(T(0.25) * v, T(1e-6)); }
+    GS_DEV static T dl_grow(T v) { return jmin(T(4.0) * v, T(1e10)); }
 
     // The trial loop of dogleg_step on the model matrix J (the regularised Jacobian that was
     // factorised) with the Newton step sN (read where `newton`): the Newton step inside the radius,
     // else the Cauchy point cut to the radius, else the point of norm δ on the segment between them;
-    // a trial is accepted when pred > 0 and ared ≥ 1e-4 pred (unsquared norms), a rejection shrinks δ
-    // to max(δ/4, 1e-6), and a rejection at 1e-6 fails. x, Fx and `radius` are updated.
-    GS_DEV Step dogleg_trial(double (&x)[N], double (&Fx)[N], const double (&J)[N][N], const double (&sN)[N],
+    // a trial is accepted when pred > 0 and ared ≥ T(1e-4) pred (unsquared norms), a rejection shrinks δ
+    // to max(δ/4, T(1e-6)), and a rejection at T(1e-6) fails. x, Fx and `radius` are updated.
+    GS_DEV Step dogleg_trial(T (&x)[N], T (&Fx)[N], const T (&J)[N][N], const T (&sN)[N],
                              bool newton) {
-        const double lo = 1e-6;
-        const double fnorm = rnorm<N>(Fx);
-        if (fnorm == 0.0) return Step{kSuccess, false, false, 0, fnorm, 0.0};
-        const double nN = newton ? rnorm<N>(sN) : __builtin_inf();
+        const T lo = T(1e-6);
+        const T fnorm = rnorm<N>(Fx);
+        if (fnorm == T(0.0)) return Step{kSuccess, false, false, 0, fnorm, T(0.0)};
+        const T nN = newton ? rnorm<N>(sN) : T(__builtin_inf());
         // the Cauchy direction from the unit residual: g̃ = Jᵀ (F/‖F‖), ĝ = g̃/‖g̃‖
-        double gh[N];
-        const double finv = 1.0 / fnorm;
+        T gh[N];
+        const T finv = T(1.0) / fnorm;
 #pragma unroll
         for (int k = 0; k < N; ++k) {
-            double acc = J[0][k] * (finv * Fx[0]);
+            T acc = J[0][k] * (finv * Fx[0]);
 #pragma unroll
-            for (int j = 1; j < N; ++j) acc = __builtin_fma(J[j][k], finv * Fx[j], acc);
+            for (int j = 1; j < N; ++j) acc = tfma(J[j][k], finv * Fx[j], acc);
             gh[k] = acc;
         }
-        const double gnorm = rnorm<N>(gh);
-        const double ginv = 1.0 / gnorm;
+        const T gnorm = rnorm<N>(gh);
+        const T ginv = T(1.0) / gnorm;
 #pragma unroll
         for (int i = 0; i < N; ++i) gh[i] = ginv * gh[i];
-        double Jg[N];
+        T Jg[N];
         matvec<N>(J, gh, Jg);
-        const double jnorm = rnorm<N>(Jg);
-        const bool cauchy = gnorm != 0.0 && isfin(gnorm) && jnorm != 0.0 && isfin(jnorm);
-        const double nC = fnorm * ((gnorm / jnorm) / jnorm);
-        double delta = radius;
-        if (delta == 0.0 && (newton || cauchy)) {
-            const double n0 = newton ? nN : nC;
-            delta = n0 < lo ? 2.0 * lo : dl_clamp(n0);
+        const T jnorm = rnorm<N>(Jg);
+        const bool cauchy = gnorm != T(0.0) && isfin(gnorm) && jnorm != T(0.0) && isfin(jnorm);
+        const T nC = fnorm * ((gnorm / jnorm) / jnorm);
+        T delta = radius;
+        if (delta == T(0.0) && (newton || cauchy)) {
+            const T n0 = newton ? nN : nC;
+            delta = n0 < lo ? T(2.0) * lo : dl_clamp(n0);
         }
         while (true) {
-            double sv[N];
+            T sv[N];
             bool boundary;
             if (newton && nN <= delta) {
 #pragma unroll
@@ -667,52 +671,52 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
                 boundary = nN >= delta;
             } else if (!cauchy) {
                 radius = delta;
-                return Step{kStalled, false, false, 0, fnorm, 0.0};
+                return Step{kStalled, false, false, 0, fnorm, T(0.0)};
             } else if (!newton || nC >= delta) {
-                const double a = -jmin(nC, delta);
+                const T a = -jmin(nC, delta);
 #pragma unroll
                 for (int i = 0; i < N; ++i) sv[i] = a * gh[i];
                 boundary = nC >= delta;
             } else {
                 // the root t ≥ 0 of ‖a + t b̂‖ = 1 in units of δ, a = −(nC/δ) ĝ, b = sN + nC ĝ
-                double av[N], bh[N];
-                const double ac = -(nC / delta);
+                T av[N], bh[N];
+                const T ac = -(nC / delta);
 #pragma unroll
                 for (int i = 0; i < N; ++i) {
                     av[i] = ac * gh[i];
-                    bh[i] = 1.0 * sN[i] + nC * gh[i];
+                    bh[i] = T(1.0) * sN[i] + nC * gh[i];
                 }
-                const double binv = 1.0 / rnorm<N>(bh);
+                const T binv = T(1.0) / rnorm<N>(bh);
 #pragma unroll
                 for (int i = 0; i < N; ++i) bh[i] = binv * bh[i];
-                const double ab = rdot<N>(av, bh);
-                const double c = 1.0 - norm2<N>(av);
-                const double ab2 = ab * ab;
-                const double t = c / (ab + __builtin_sqrt(ab2 + c));
-                const double dt = delta * t;
+                const T ab = rdot<N>(av, bh);
+                const T c = T(1.0) - norm2<N>(av);
+                const T ab2 = ab * ab;
+                const T t = c / (ab + tsqrt(ab2 + c));
+                const T dt = delta * t;
 #pragma unroll
                 for (int i = 0; i < N; ++i) sv[i] = delta * av[i] + dt * bh[i];
                 boundary = true;
             }
-            double Js[N], Flin[N], xp[N], Fp[N];
+            T Js[N], Flin[N], xp[N], Fp[N];
             matvec<N>(J, sv, Js);
 #pragma unroll
-            for (int i = 0; i < N; ++i) Flin[i] = 1.0 * Fx[i] + 1.0 * Js[i];
-            const double pred = fnorm - rnorm<N>(Flin);
+            for (int i = 0; i < N; ++i) Flin[i] = T(1.0) * Fx[i] + T(1.0) * Js[i];
+            const T pred = fnorm - rnorm<N>(Flin);
 #pragma unroll
-            for (int i = 0; i < N; ++i) xp[i] = x[i] + 1.0 * sv[i];
+            for (int i = 0; i < N; ++i) xp[i] = x[i] + T(1.0) * sv[i];
             fam.F(xp, Fp);
-            const double fnp = rnorm<N>(Fp);
-            const double ared = fnorm - fnp;
-            if (pred > 0.0 && ared >= 1e-4 * pred) {
+            const T fnp = rnorm<N>(Fp);
+            const T ared = fnorm - fnp;
+            if (pred > T(0.0) && ared >= T(1e-4) * pred) {
                 const bool limited = !(newton && nN <= delta);
-                const double rho = ared / pred;
-                if (rho < 0.1) {
+                const T rho = ared / pred;
+                if (rho < T(0.1)) {
                     delta = nN < delta ? jmax(nN, lo) : dl_shrink(delta);
-                } else if (rho > 0.75 && boundary) {
+                } else if (rho > T(0.75) && boundary) {
                     delta = dl_grow(delta);
                 }
-                const double stepnorm = rnorm<N>(sv);
+                const T stepnorm = rnorm<N>(sv);
 #pragma unroll
                 for (int i = 0; i < N; ++i) {
                     x[i] = xp[i];
@@ -723,7 +727,7 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
             }
             if (delta == lo) {
                 radius = delta;
-                return Step{kLsFailed, false, false, 1, fnorm, 0.0};
+                return Step{kLsFailed, false, false, 1, fnorm, T(0.0)};
             }
             delta = dl_shrink(delta);
         }
@@ -731,12 +735,12 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
 
     // newton_step!! with DogLeg (newton.jl:1725-1737) and dogleg_direction (newton.jl:1741-1746): a
     // failed factor, or a Newton step of zero or non-finite norm, leaves the Cauchy step.
-    GS_DEV Step plain_step_dogleg(double (&x)[N], double (&Fx)[N]) {
-        double J[N][N];
+    GS_DEV Step plain_step_dogleg(T (&x)[N], T (&Fx)[N]) {
+        T J[N][N];
         jacobian(x, Fx, J);
-        Factor<double, N> f;
-        factorize<double, N>(J, f);
-        double d[N];
+        Factor<T, N> f;
+        factorize<T, N>(J, f);
+        T d[N];
         bool newton = false;
 #pragma unroll
         for (int i = 0; i < N; ++i) d[i] = Fx[i];
@@ -746,10 +750,10 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
             for (int i = 0; i < N; ++i) d[i] = -d[i];
             if (REFINE) {
                 bool failed;
-                refine<double>(d, f, J, Fx, failed);
+                refine<T>(d, f, J, Fx, failed);
             }
-            const double nd = rnorm<N>(d);
-            newton = isfin(nd) && nd != 0.0;
+            const T nd = rnorm<N>(d);
+            newton = isfin(nd) && nd != T(0.0);
         }
         Step s = dogleg_trial(x, Fx, J, d, newton);
         sfnorm = s.fnorm;
@@ -757,18 +761,18 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
     }
 
     // mixed_step!! with DogLeg (newton.jl:1757-1780).
-    GS_DEV Step mixed_step_dogleg(double (&x)[N], double (&Fx)[N]) {
-        const double fnorm_old = residual_norm(Fx);
+    GS_DEV Step mixed_step_dogleg(T (&x)[N], T (&Fx)[N]) {
+        const T fnorm_old = residual_norm(Fx);
         const bool was_promoted = promoted;
-        double J[N][N], d[N];
+        T J[N][N], d[N];
         int failures;
-        double lin_rel;
+        T lin_rel;
         const bool ok = mixed_newton(x, Fx, fnorm_old, J, d, failures, lin_rel);
         const bool promoted_step = promoted && !was_promoted;
         bool newton = false;
         if (ok) {
-            const double nd = rnorm<N>(d);
-            newton = isfin(nd) && nd != 0.0;
+            const T nd = rnorm<N>(d);
+            newton = isfin(nd) && nd != T(0.0);
         } else {
 #pragma unroll
             for (int i = 0; i < N; ++i) d[i] = Fx[i];
@@ -801,17 +805,17 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
     // newton_step!! with reuse in the working precision (newton.jl:1781-1810): a step along the held
     // factor refines with the JVP at x and takes the measured slope; a failed refinement (one
     // step-rule failure) or a line search that fails without a step refactorises at x in the step.
-    GS_DEV Step plain_step_reuse(double (&x)[N], double (&Fx)[N]) {
+    GS_DEV Step plain_step_reuse(T (&x)[N], T (&Fx)[N]) {
         int failures = 0;
         if (!(age == 0 || stalled)) {
             const int age_next = next_age();
-            double d[N];
+            T d[N];
             ldiv<N>(kept, Fx, d);
 #pragma unroll
             for (int i = 0; i < N; ++i) d[i] = -d[i];
             bool ref_failed = false;
             if (REFINE) {
-                double Jx[N][N];
+                T Jx[N][N];
                 fam.J(x, Jx);
                 refine<TF, true>(d, kept, Jx, Fx, ref_failed);
             }
@@ -820,21 +824,21 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
             } else {
                 // a failed search without a step leaves x, Fx and the merit as they were
                 Step s = globalize(x, Fx, d, false);
-                if (!(s.code == kLsFailed && s.stepnorm == 0.0)) {
+                if (!(s.code == kLsFailed && s.stepnorm == T(0.0))) {
                     sfnorm = s.fnorm;
                     set_rule(s.code, age_next, true);
                     return s;
                 }
             }
         }
-        double J[N][N];
+        T J[N][N];
         jacobian(x, Fx, J);
         factorize<TF, N>(J, kept);
         Step s;
         if (kept.info > 0) {
-            s = Step{kSingular, false, false, 0, rnorm<N>(Fx), 0.0};
+            s = Step{kSingular, false, false, 0, rnorm<N>(Fx), T(0.0)};
         } else {
-            double d[N];
+            T d[N];
             ldiv<N>(kept, Fx, d);
 #pragma unroll
             for (int i = 0; i < N; ++i) d[i] = -d[i];
@@ -842,7 +846,7 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
                 bool failed;
                 refine<TF>(d, kept, J, Fx, failed);
             }
-            s = globalize(x, Fx, d, q.mu == 0.0 && !REFINE);
+            s = globalize(x, Fx, d, q.mu == T(0.0) && !REFINE);
         }
         s.failures += failures;
         sfnorm = s.fnorm;
@@ -855,23 +859,23 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
     // runs along the held FP32 factor until its age or a stall refreshes it, and a reused direction
     // whose refinement fails is refactorised in FP32 at x (one failure) before any promotion. A line
     // search along the held factor that fails without a step is retried once from a fresh factor.
-    GS_DEV Step mixed_step_reuse(double (&x)[N], double (&Fx)[N]) {
-        const double fnorm_old = residual_norm(Fx);
+    GS_DEV Step mixed_step_reuse(T (&x)[N], T (&Fx)[N]) {
+        const T fnorm_old = residual_norm(Fx);
         const bool state_promoted = promoted;
         bool fresh = promoted || age == 0 || stalled;
         const int age_fresh = fresh_age();
         int age_next = fresh ? age_fresh : next_age();
         while (true) {
             const bool was_fresh = fresh;
-            double d[N];
+            T d[N];
             bool ok = true, md_fresh = true, have = false;
             int failures = 0;
-            double lin_rel = 0.0;
+            T lin_rel = T(0.0);
             if (promoted) {
-                double J[N][N];
+                T J[N][N];
                 jacobian(x, Fx, J);
-                Factor<double, N> f;
-                factorize<double, N>(J, f);
+                Factor<T, N> f;
+                factorize<T, N>(J, f);
                 if (f.info > 0) {
                     ok = false;
                 } else {
@@ -880,7 +884,7 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
                     for (int i = 0; i < N; ++i) d[i] = -d[i];
                     if (REFINE) {
                         bool failed;
-                        refine<double>(d, f, J, Fx, failed);
+                        refine<T>(d, f, J, Fx, failed);
                     }
                 }
                 have = true;
@@ -890,7 +894,7 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
                 for (int i = 0; i < N; ++i) d[i] = -d[i];
                 bool ref_failed = false;
                 if (REFINE) {
-                    double Jx[N][N];
+                    T Jx[N][N];
                     fam.J(x, Jx);
                     refine<TF, true>(d, kept, Jx, Fx, ref_failed);
                 }
@@ -900,7 +904,7 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
                 }
             }
             if (!have) {
-                double J[N][N];
+                T J[N][N];
                 jacobian(x, Fx, J);
                 Factor<float, N> f;
                 factorize<float, N>(J, f);
@@ -918,8 +922,8 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
                     // promote_factor!! (newton.jl:1332-1339): refactorise the held Jacobian in FP64
                     failures += 1;
                     promoted = true;
-                    Factor<double, N> g;
-                    factorize<double, N>(J, g);
+                    Factor<T, N> g;
+                    factorize<T, N>(J, g);
                     if (g.info > 0) {
                         ok = false;
                     } else {
@@ -929,12 +933,12 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
                     }
                 } else {
                     copy_factor(f);
-                    if (fnorm_old != 0.0 && isfin(fnorm_old)) {
+                    if (fnorm_old != T(0.0) && isfin(fnorm_old)) {
                         // linear_residual_ratio (newton.jl:1422-1430)
-                        double Jd[N], lin[N];
+                        T Jd[N], lin[N];
                         matvec<N>(J, d, Jd);
 #pragma unroll
-                        for (int i = 0; i < N; ++i) lin[i] = 1.0 * Fx[i] + 1.0 * Jd[i];
+                        for (int i = 0; i < N; ++i) lin[i] = T(1.0) * Fx[i] + T(1.0) * Jd[i];
                         lin_rel = rnorm<N>(lin) / fnorm_old;
                     }
                 }
@@ -942,7 +946,7 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
             fresh = md_fresh;
             if (!ok) {
                 set_rule(kSingular, 0, !md_fresh);
-                return Step{kSingular, promoted, false, failures, rnorm<N>(Fx), 0.0};
+                return Step{kSingular, promoted, false, failures, rnorm<N>(Fx), T(0.0)};
             }
             const bool promoted_step = promoted && !state_promoted;
             if (promoted_step) {
@@ -951,8 +955,8 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
                 age_next = age_fresh;
             }
             // newton_stepkind(prob, m, T, md.fresh, promoted) (newton.jl:1244-1254)
-            Step s = globalize(x, Fx, d, md_fresh && q.mu == 0.0 && promoted);
-            if (!md_fresh && s.code == kLsFailed && s.stepnorm == 0.0) {
+            Step s = globalize(x, Fx, d, md_fresh && q.mu == T(0.0) && promoted);
+            if (!md_fresh && s.code == kLsFailed && s.stepnorm == T(0.0)) {
                 fresh = true;
                 age_next = age_fresh;
                 continue;
@@ -962,8 +966,8 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
                 s.promoted = true;
                 slow_steps = 0;
             } else if (md_fresh && !promoted) {
-                // read_mixed_stall (newton.jl:1437-1451), η_lin = max(√eps, 1e-4)
-                if (s.fnorm > 0.5 * fnorm_old && lin_rel > 1e-4) {
+                // read_mixed_stall (newton.jl:1437-1451), η_lin = max(√eps, T(1e-4))
+                if (s.fnorm > T(0.5) * fnorm_old && lin_rel > T(1e-4)) {
                     int slow = slow_steps + 1;
                     if (slow >= 2) {
                         promoted = true;
@@ -988,7 +992,7 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
         if constexpr (MIXED) kept = f;
     }
 
-    GS_DEV Step step(double (&x)[N], double (&Fx)[N]) {
+    GS_DEV Step step(T (&x)[N], T (&Fx)[N]) {
         if constexpr (GLOB == kDogLeg) {
             if constexpr (MIXED) return mixed_step_dogleg(x, Fx);
             else return plain_step_dogleg(x, Fx);
@@ -1007,36 +1011,36 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
         uint8_t code;
         bool contracted, promoted, reused;
         int32_t iterations, failures;
-        double fnorm, stepnorm, rate, gradratio;
+        T fnorm, stepnorm, rate, gradratio;
     };
 
     // solve!! (solver.jl:277-378) for a square NonlinearProblem with a Newton method.
-    GS_DEV Result solve(double (&x)[N]) {
-        double Fx[N];
+    GS_DEV Result solve(T (&x)[N]) {
+        T Fx[N];
         fam.F(x, Fx);
-        const double fnorm0 = rnorm<N>(Fx);
+        const T fnorm0 = rnorm<N>(Fx);
         // newton_start!! (newton.jl:1135-1144)
-        merit = GLOB == kBacktracking ? norm2<N>(Fx) : dnan();
+        merit = GLOB == kBacktracking ? norm2<N>(Fx) : tnan<T>();
         sfnorm = fnorm0;
-        gradratio = dnan();
+        gradratio = tnan<T>();
         promoted = false;
         slow_steps = 0;
         age = 0;
         stalled = false;
         reused = false;
         radius = q.dl_radius;  // reset_rule (newton.jl:677)
-        double best_x[N];
-        double best_fn = fnorm0;
+        T best_x[N];
+        T best_fn = fnorm0;
         if (q.best) {
 #pragma unroll
             for (int i = 0; i < N; ++i) best_x[i] = x[i];
         }
-        Result R{kSuccess, false, false, false, 0, 0, fnorm0, 0.0, dnan(), dnan()};
+        Result R{kSuccess, false, false, false, 0, 0, fnorm0, T(0.0), tnan<T>(), tnan<T>()};
         if (!isfin(fnorm0)) {
             R.code = kNonfinite;
         } else {
-            const double ftest0 = q.norm_max ? rmaxnorm<N>(Fx) : fnorm0;
-            const double ftol = jmax(q.f_abstol, q.f_reltol * ftest0);
+            const T ftest0 = q.norm_max ? rmaxnorm<N>(Fx) : fnorm0;
+            const T ftol = jmax(q.f_abstol, q.f_reltol * ftest0);
             if (0 >= q.min_iterations && ftest0 <= ftol) {
                 R.code = kSuccess;
             } else if (q.max_iterations == 0) {
@@ -1044,7 +1048,7 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
             } else {
                 int stalls = 0;
                 while (true) {
-                    const double rate_k = R.rate;
+                    const T rate_k = R.rate;
                     Step s = step(x, Fx);
                     if (q.best && s.fnorm < best_fn) {
                         best_fn = s.fnorm;
@@ -1052,7 +1056,7 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
                         for (int i = 0; i < N; ++i) best_x[i] = x[i];
                     }
                     // record (status.jl:245-262)
-                    R.rate = R.stepnorm > 0.0 ? s.stepnorm / R.stepnorm : dnan();
+                    R.rate = R.stepnorm > T(0.0) ? s.stepnorm / R.stepnorm : tnan<T>();
                     R.code = s.code;
                     R.iterations += 1;
                     R.fnorm = s.fnorm;
@@ -1061,27 +1065,27 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
                     R.promoted |= s.promoted;
                     // observe_rate!! (newton.jl:1205-1210): a step of rate θ ≥ 1 along a reused
                     // factor refreshes the factor at the next step
-                    if (REUSE && reused && R.rate >= 1.0) stalled = true;
+                    if (REUSE && reused && R.rate >= T(1.0)) stalled = true;
                     if (s.code == kSingular || s.code == kNonfinite || s.code == kMaxIters) break;
-                    if (s.code == kLsFailed && s.stepnorm == 0.0) break;
-                    const double ftest = q.norm_max ? rmaxnorm<N>(Fx) : R.fnorm;
+                    if (s.code == kLsFailed && s.stepnorm == T(0.0)) break;
+                    const T ftest = q.norm_max ? rmaxnorm<N>(Fx) : R.fnorm;
                     if (R.iterations >= q.min_iterations && ftest <= ftol) {
                         R.code = kSuccess;
                         break;
                     }
                     // a step that a trust radius limited passes only the residual test
                     const bool steptested = s.code != kStalled && !s.limited;
-                    const double xnorm = steptested ? rnorm<N>(x) : dnan();
-                    const double xtol = jmax(q.x_abstol, q.x_reltol * xnorm);
+                    const T xnorm = steptested ? rnorm<N>(x) : tnan<T>();
+                    const T xtol = jmax(q.x_abstol, q.x_reltol * xnorm);
                     const bool past_min = R.iterations >= q.min_iterations;
                     bool contracted = false;
                     if (!s.shortened && q.rate_test && past_min) {
-                        double th = jmax(R.rate, rate_k);
-                        contracted = R.stepnorm > 0.0 && th < 1.0 && th / (1.0 - th) * R.stepnorm <= q.kappa * xtol;
+                        T th = jmax(R.rate, rate_k);
+                        contracted = R.stepnorm > T(0.0) && th < T(1.0) && th / (T(1.0) - th) * R.stepnorm <= q.kappa * xtol;
                     }
-                    bool step_conv = past_min && R.stepnorm > 0.0 && R.stepnorm <= xtol;
-                    bool zero_step = s.code == kSuccess && s.stepnorm == 0.0 && past_min;
-                    bool roundoff = q.roundoff && past_min && R.rate >= 1.0 && rate_k < 1.0;
+                    bool step_conv = past_min && R.stepnorm > T(0.0) && R.stepnorm <= xtol;
+                    bool zero_step = s.code == kSuccess && s.stepnorm == T(0.0) && past_min;
+                    bool roundoff = q.roundoff && past_min && R.rate >= T(1.0) && rate_k < T(1.0);
                     if (contracted || (steptested && (step_conv || zero_step || roundoff))) {
                         R.code = contracted ? kSuccess : kStalled;
                         R.contracted = contracted;

@@ -73,6 +73,29 @@ typedef struct gs_status_f64 {
     uint8_t pad2_[6];
 } gs_status_f64;
 
+/* SolverStatus{Float32}, byte-identical to Julia's C layout: code u8 @0, f_calls i16 @2, iterations
+ * i32 @4, fnorm @8, stepnorm @12, step_failures i32 @16, promoted @20, stationary @21, jvp_calls i16
+ * @22, rank i32 @24, nullnorm @28, gradratio @32, rate @36, contracted @40, reused @41; size 44. */
+typedef struct gs_status_f32 {
+    uint8_t code;
+    uint8_t pad0_;
+    int16_t f_calls;
+    int32_t iterations;
+    float fnorm;
+    float stepnorm;
+    int32_t step_failures;
+    uint8_t promoted;
+    uint8_t stationary;
+    int16_t jvp_calls;
+    int32_t rank;
+    float nullnorm;
+    float gradratio;
+    float rate;
+    uint8_t contracted;
+    uint8_t reused;
+    uint8_t pad1_[2];
+} gs_status_f32;
+
 /* The stopping norm of Options (src/base/options.jl:8-25). */
 enum gs_norm { GS_NORM_L2 = 0, GS_NORM_MAX = 1 };
 
@@ -166,6 +189,24 @@ int gs_solve_batch_f64(const gs_method* method, const gs_family* family, int64_t
                        uint8_t* codes, const int64_t* order, const gs_options_f64* options,
                        int32_t workgroupsize, void* stream);
 
+/* Free every device resource the library caches (the workspaces of the reductions, the
+ * least-squares batch and the learned start, the least-squares graphs, streams and events, and the
+ * calling thread's timing events) on every device, and restore the current device. No call may be
+ * in flight; a later call creates what it needs again. No counterpart in the reference, whose KA
+ * path (batched.jl:171-195) leaves device memory to Julia's garbage collector. */
+int gs_shutdown(void);
+
+/* The Float32 twin of gs_solve_batch_f64: X holds nsys × n floats (Vector{SVector{n,Float32}}), the
+ * parameter records hold floats (Int64 codes for family 2), S receives SolverStatus{Float32}. The
+ * options and the method's floats arrive as doubles and are used as Float32 (R(x), as Options{Float32}
+ * and ToReal{Float32} hold them; an Options{Float32} passes through exactly). A Float32 batch has no
+ * mixed-precision form (its factor is in its working precision already: factor_precision is ignored).
+ * The families of the lane-per-system kernels (1–5) are served; family 6 is refused (GS_EFAMILY). */
+int gs_solve_batch_f32(const gs_method* method, const gs_family* family, int64_t nsys,
+                       float* X, const void* P, int64_t p_stride, gs_status_f32* S,
+                       uint8_t* codes, const int64_t* order, const gs_options_f64* options,
+                       int32_t workgroupsize, void* stream);
+
 /* The six-count tally of a codes buffer (batched.jl:197-199), on the device; counts[6] is host memory. */
 int gs_tally_codes(const uint8_t* codes, int64_t nsys, int64_t counts[6], void* stream);
 
@@ -194,9 +235,17 @@ int gs_solve_expsum_lsq_f64(const gs_method* method, int64_t nfit, double* X, co
                             gs_status_f64* S, const gs_options_f64* options, void* stream);
 
 /* The launch shape of the least-squares bucket K = k (1..4): `threads` per workgroup (64, 128, 192 or
- * 256) and `fits_per_lane` (1..64; a lane whose fit ends takes the next fit of its bucket). Set by the
- * package's autotuner; no counterpart in the reference. */
-int gs_lsq_set_shape(int k, int threads, int fits_per_lane);
+ * 256) and resident workgroups per CU (`blocks_per_cu`, 1..64, or 0 for the kernel's occupancy) of its
+ * persistent kernel (a lane whose fit ends takes the next fit of its bucket). Set by the package's
+ * autotuner; no counterpart in the reference. gs_lsq_get_shape reports (threads, blocks_per_cu,
+ * workgroups for a batch of nfit fits). */
+int gs_lsq_set_shape(int k, int threads, int blocks_per_cu);
+int gs_lsq_get_shape(int k, int64_t nfit, int64_t out[3]);
+
+/* The least-squares launch sequence (the device sort and the four bucket kernels) is captured into
+ * a hipGraph and replayed while a call's pointers, sizes, method, options and shapes are those of the
+ * capture (enable = 1, the default), or launched eagerly (0). No counterpart in the reference. */
+int gs_lsq_set_graph(int enable);
 
 /* The learned start, with no counterpart in the reference: a multilayer perceptron (inputs padded to
  * GS_MLP_IN, three hidden ReLU layers of GS_MLP_HIDDEN, outputs padded to GS_MLP_OUT; BF16 operands,

@@ -77,22 +77,92 @@ static void write_status(gs_status_f64* out, const Status<double>& s) {
     *out = t;
 }
 
-template <int N, class MakeProb>
-static void run_batch(const Method<double>& m, const Options<double>& o, int64_t nsys, double* X,
-                      gs_status_f64* S, uint8_t* codes, const int64_t* order, int nthreads,
-                      MakeProb make) {
+static void write_status(gs_status_f32* out, const Status<float>& s) {
+    gs_status_f32 t;
+    std::memset(&t, 0, sizeof(t));
+    t.code = s.code;
+    t.f_calls = s.f_calls;
+    t.iterations = s.iterations;
+    t.fnorm = s.fnorm;
+    t.stepnorm = s.stepnorm;
+    t.step_failures = s.step_failures;
+    t.promoted = s.promoted;
+    t.stationary = s.stationary;
+    t.jvp_calls = s.jvp_calls;
+    t.rank = s.rank;
+    t.nullnorm = s.nullnorm;
+    t.gradratio = s.gradratio;
+    t.rate = s.rate;
+    t.contracted = s.contracted;
+    t.reused = s.reused;
+    *out = t;
+}
+
+template <int N, class R, class SS, class MakeProb>
+static void run_batch(const Method<R>& m, const Options<R>& o, int64_t nsys, R* X, SS* S, uint8_t* codes,
+                      const int64_t* order, int nthreads, MakeProb make) {
 #pragma omp parallel for schedule(static) num_threads(nthreads)
     for (int64_t i = 0; i < nsys; ++i) {
         int64_t idx = order ? order[i] - 1 : i;
         auto prob = make(idx);
-        Solver<double, N, decltype(prob)> solver(prob, m, o);
-        Vec<double, N> x;
+        Solver<R, N, decltype(prob)> solver(prob, m, o);
+        Vec<R, N> x;
         for (int k = 0; k < N; ++k) x[k] = X[idx * N + k];
-        Status<double> st = solver.solve(x);
+        Status<R> st = solver.solve(x);
         for (int k = 0; k < N; ++k) X[idx * N + k] = x[k];
         if (S) write_status(S + idx, st);
         if (codes) codes[idx] = st.code;
     }
+}
+
+// The lane-per-system families (1-5) in the working precision R, records of R (Int64 codes for 2).
+template <class R, class SS>
+static int solve_small_batch(const Method<R>& m, const Options<R>& o, const gs_family* fam, int64_t nsys, R* X,
+                             const void* P, int64_t p_stride, SS* S, uint8_t* codes, const int64_t* order,
+                             int nthreads) {
+    const char* Pb = static_cast<const char*>(P);
+    auto rec = [&](int64_t idx) -> const char* { return Pb + (p_stride ? idx * p_stride : 0); };
+    switch (fam->id) {
+        case GS_FAM_COUPLED2:
+            run_batch<2>(m, o, nsys, X, S, codes, order, nthreads, [&](int64_t idx) {
+                const R* p = reinterpret_cast<const R*>(rec(idx));
+                return Coupled2<R>{p[0], p[1]};
+            });
+            return 0;
+        case GS_FAM_HETERO2:
+            run_batch<2>(m, o, nsys, X, S, codes, order, nthreads, [&](int64_t idx) {
+                const int64_t* p = reinterpret_cast<const int64_t*>(rec(idx));
+                return Hetero2<R>{p[0]};
+            });
+            return 0;
+        case GS_FAM_SCALAR_SQUARE:
+            run_batch<1>(m, o, nsys, X, S, codes, order, nthreads, [&](int64_t idx) {
+                const R* p = reinterpret_cast<const R*>(rec(idx));
+                return ScalarSquare<R>{p[0]};
+            });
+            return 0;
+        case GS_FAM_TANH_LINEAR3:
+            run_batch<3>(m, o, nsys, X, S, codes, order, nthreads, [&](int64_t idx) {
+                const R* p = reinterpret_cast<const R*>(rec(idx));
+                TanhLinear3<R> pr;
+                for (int k = 0; k < 9; ++k) pr.A[k] = p[k];
+                for (int k = 0; k < 3; ++k) pr.b[k] = p[9 + k];
+                return pr;
+            });
+            return 0;
+        case GS_FAM_HENON_HEILES_MIDPOINT: {
+            const R h = R(fam->shared[0]);
+            run_batch<4>(m, o, nsys, X, S, codes, order, nthreads, [&](int64_t idx) {
+                const R* p = reinterpret_cast<const R*>(rec(idx));
+                HenonHeilesMidpoint<R> pr;
+                pr.h = h;
+                for (int k = 0; k < 4; ++k) pr.xk[k] = p[k];
+                return pr;
+            });
+            return 0;
+        }
+    }
+    return GS_EFAMILY;
 }
 
 }  // namespace orc
@@ -112,44 +182,6 @@ int orc_solve_batch_f64(const gs_method* method, const gs_family* fam, int64_t n
     const char* Pb = static_cast<const char*>(P);
     auto rec = [&](int64_t idx) -> const char* { return Pb + (p_stride ? idx * p_stride : 0); };
     switch (fam->id) {
-        case GS_FAM_COUPLED2:
-            run_batch<2>(m, o, nsys, X, S, codes, order, nthreads, [&](int64_t idx) {
-                const double* p = reinterpret_cast<const double*>(rec(idx));
-                return Coupled2<double>{p[0], p[1]};
-            });
-            return 0;
-        case GS_FAM_HETERO2:
-            run_batch<2>(m, o, nsys, X, S, codes, order, nthreads, [&](int64_t idx) {
-                const int64_t* p = reinterpret_cast<const int64_t*>(rec(idx));
-                return Hetero2<double>{p[0]};
-            });
-            return 0;
-        case GS_FAM_SCALAR_SQUARE:
-            run_batch<1>(m, o, nsys, X, S, codes, order, nthreads, [&](int64_t idx) {
-                const double* p = reinterpret_cast<const double*>(rec(idx));
-                return ScalarSquare<double>{p[0]};
-            });
-            return 0;
-        case GS_FAM_TANH_LINEAR3:
-            run_batch<3>(m, o, nsys, X, S, codes, order, nthreads, [&](int64_t idx) {
-                const double* p = reinterpret_cast<const double*>(rec(idx));
-                TanhLinear3<double> pr;
-                for (int k = 0; k < 9; ++k) pr.A[k] = p[k];
-                for (int k = 0; k < 3; ++k) pr.b[k] = p[9 + k];
-                return pr;
-            });
-            return 0;
-        case GS_FAM_HENON_HEILES_MIDPOINT: {
-            double h = fam->shared[0];
-            run_batch<4>(m, o, nsys, X, S, codes, order, nthreads, [&](int64_t idx) {
-                const double* p = reinterpret_cast<const double*>(rec(idx));
-                HenonHeilesMidpoint<double> pr;
-                pr.h = h;
-                for (int k = 0; k < 4; ++k) pr.xk[k] = p[k];
-                return pr;
-            });
-            return 0;
-        }
         case GS_FAM_FPU_GL3: {
             const double h = fam->shared[0], beta = fam->shared[1];
             run_batch<48>(m, o, nsys, X, S, codes, order, nthreads, [&](int64_t idx) {
@@ -166,8 +198,22 @@ int orc_solve_batch_f64(const gs_method* method, const gs_family* fam, int64_t n
             return 0;
         }
         default:
-            return GS_EFAMILY;
+            return solve_small_batch<double>(m, o, fam, nsys, X, P, p_stride, S, codes, order, nthreads);
     }
+}
+
+// The host restatement of gs_solve_batch_f32 (include/gsolve.h): the same solve in Float32, the
+// method's floats and the options converted to Float32 (ToReal{Float32}, Options{Float32}).
+int orc_solve_batch_f32(const gs_method* method, const gs_family* fam, int64_t nsys, float* X, const void* P,
+                        int64_t p_stride, gs_status_f32* S, uint8_t* codes, const int64_t* order,
+                        const gs_options_f64* options, int nthreads) {
+    if (!method || !fam || !options || nsys < 0) return GS_EINVAL;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    Method<float> m = method_from<float>(*method);
+    m.factor_f32 = false;  // factor_eltype(LUFactorization{Float32}, Float32) is the working type
+    if (m.dl_radius != 0.0f) m.dl_radius = m.dl_radius > 1e10f ? 1e10f : (m.dl_radius < 1e-6f ? 1e-6f : m.dl_radius);
+    Options<float> o = options_from<float>(*options);
+    return solve_small_batch<float>(m, o, fam, nsys, X, P, p_stride, S, codes, order, nthreads);
 }
 
 // F and J of the FPU-β Gauss–Legendre stage system at z (48), record qp = (q, p) (32): r (48) and

@@ -32,6 +32,8 @@ def lib():
         vp = ctypes.c_void_p
         L.orc_solve_batch_f64.restype = ctypes.c_int
         L.orc_solve_batch_f64.argtypes = [vp, vp, ctypes.c_int64, vp, vp, ctypes.c_int64, vp, vp, vp, vp, ctypes.c_int]
+        L.orc_solve_batch_f32.restype = ctypes.c_int
+        L.orc_solve_batch_f32.argtypes = L.orc_solve_batch_f64.argtypes
         L.orc_standard_cell.restype = ctypes.c_int
         L.orc_standard_cell.argtypes = [ctypes.c_int] * 4 + [vp] * 5
         L.orc_standard_u0.restype = ctypes.c_int
@@ -73,16 +75,17 @@ def _ptr(a: np.ndarray):
     return ctypes.c_void_p(a.ctypes.data)
 
 
-def host_records(family, p, nsys: int):
-    """The parameter records of a batch on the host: (array, byte stride, shared scalars)."""
+def host_records(family, p, nsys: int, real=np.float64):
+    """The parameter records of a batch on the host: (array, byte stride, shared scalars); `real` is
+    the records' float type (the batch's working precision)."""
     name = type(family).__name__
     if name == "FpuGl3":
-        rec = np.ascontiguousarray(np.asarray(p.per_system, dtype=np.float64).reshape(nsys, 32))
+        rec = np.ascontiguousarray(np.asarray(p.per_system, dtype=real).reshape(nsys, 32))
         return rec, rec.strides[0], (float(p.shared), float(family.beta))
     if name == "HenonHeilesMidpoint":
-        rec = np.ascontiguousarray(np.asarray(p.per_system, dtype=np.float64).reshape(nsys, 4))
+        rec = np.ascontiguousarray(np.asarray(p.per_system, dtype=real).reshape(nsys, 4))
         return rec, rec.strides[0], (float(p.shared),)
-    dtype = np.int64 if name == "Hetero2" else np.float64
+    dtype = np.int64 if name == "Hetero2" else real
     if hasattr(p, "data"):
         rec = np.ascontiguousarray(np.asarray(p.data, dtype=dtype).reshape(nsys, family.p_len))
         return rec, rec.strides[0], ()
@@ -96,16 +99,18 @@ def solve_batch(method, family, X: np.ndarray, p, options, order=None, nthreads:
     `method`, `options`: objects with `to_c()` giving the ABI structs of include/gsolve.h;
     `family`: a registered family (id, n, p_len); `order`: 1-based permutation or None.
     """
-    from_pkg_status = _status_dtype()
-    X = np.ascontiguousarray(X, dtype=np.float64)
+    f32 = np.asarray(X).dtype == np.float32
+    real = np.float32 if f32 else np.float64
+    X = np.ascontiguousarray(X, dtype=real)
     nsys = X.shape[0]
-    rec, stride, shared = host_records(family, p, nsys)
+    rec, stride, shared = host_records(family, p, nsys, real)
     cm = method.to_c()
     co = options.to_c()
     fam = _family_struct(family, shared)
-    S = np.zeros(nsys, dtype=from_pkg_status)
+    S = np.zeros(nsys, dtype=_status_dtype_f32() if f32 else _status_dtype())
     o = None if order is None else np.ascontiguousarray(np.asarray(order, dtype=np.int64))
-    rc = lib().orc_solve_batch_f64(
+    entry = lib().orc_solve_batch_f32 if f32 else lib().orc_solve_batch_f64
+    rc = entry(
         ctypes.byref(cm), ctypes.byref(fam), nsys, _ptr(X), _ptr(rec), stride, _ptr(S), None,
         None if o is None else _ptr(o), ctypes.byref(co), nthreads)
     if rc != 0:
@@ -120,6 +125,16 @@ def _status_dtype():
         "formats": ["u1", "<i2", "<i4", "<f8", "<f8", "<i4", "u1", "u1", "<i2", "<i4", "<f8", "<f8", "<f8", "u1", "u1"],
         "offsets": [0, 2, 4, 8, 16, 24, 28, 29, 30, 32, 40, 48, 56, 64, 65],
         "itemsize": 72,
+    })
+
+
+def _status_dtype_f32():
+    return np.dtype({
+        "names": ["code", "f_calls", "iterations", "fnorm", "stepnorm", "step_failures", "promoted",
+                  "stationary", "jvp_calls", "rank", "nullnorm", "gradratio", "rate", "contracted", "reused"],
+        "formats": ["u1", "<i2", "<i4", "<f4", "<f4", "<i4", "u1", "u1", "<i2", "<i4", "<f4", "<f4", "<f4", "u1", "u1"],
+        "offsets": [0, 2, 4, 8, 12, 16, 20, 21, 22, 24, 28, 32, 36, 40, 41],
+        "itemsize": 44,
     })
 
 

@@ -186,3 +186,77 @@ def test_expsum_full_batch(gs, orc, workloads, device):
     assert np.allclose(host, st["fnorm"][idx], rtol=1e-9, atol=0.0)
     codes = np.bincount(st["code"], minlength=6)
     print("codes", codes.tolist())
+
+
+def test_expsum_graph_replays_match_eager(gs, workloads, device):
+    """The captured launch sequence, replayed on the same buffers with eager work and
+    synchronisations between the replays, gives the eager launch's results bit for bit every time."""
+    lib = gs.load_library()
+    K, m, off, Y, X0, _ = workloads.expsum_fits(6000, seed=12)
+    method = _lm(gs)
+    prob = gs.LeastSquaresProblem(gs.ExpSumFits())
+    data = _device_data(gs, device, K, m, off, Y)
+    try:
+        assert lib.gs_lsq_set_graph(0) == 0
+        X = torch.as_tensor(X0, device=device).clone()
+        S = gs.status_buffer(len(X0), device)
+        gs.solve_batch(X, S, prob, method, data)
+        torch.cuda.synchronize()
+        xe, se = X.cpu().numpy(), gs.statuses(S)
+        assert lib.gs_lsq_set_graph(1) == 0
+        Xg = torch.empty_like(X)
+        Sg = gs.status_buffer(len(X0), device)
+        for r in range(4):
+            Xg.copy_(torch.as_tensor(X0, device=device))
+            Sg.zero_()
+            junk = (torch.arange(1 << 16, device=device, dtype=torch.float64) * r).sum()  # eager work
+            torch.cuda.synchronize()
+            gs.solve_batch(Xg, Sg, prob, method, data)
+            torch.cuda.synchronize()
+            _assert_bitwise(Xg.cpu().numpy(), gs.statuses(Sg), xe, se)
+            assert float(junk) == float(r) * (1 << 16) * ((1 << 16) - 1) / 2
+    finally:
+        lib.gs_lsq_set_graph(1)
+
+
+def test_tune_lsq_changes_no_result(gs, workloads, device):
+    """The autotuner times candidate shapes per bucket on the batch and sets the fastest; results do
+    not depend on the shape (every fit is one lane's, whichever lane takes it)."""
+    K, m, off, Y, X0, _ = workloads.expsum_fits(20000, seed=13)
+    method = _lm(gs)
+    x0, s0 = _solve_gpu(gs, device, K, m, off, Y, X0, method, gs.Options())
+    data = _device_data(gs, device, K, m, off, Y)
+    try:
+        best = gs.tune_lsq(torch.as_tensor(X0, device=device), data, candidates=((64, 0), (128, 1), (256, 0)),
+                           repeats=1)
+        assert sorted(best) == [1, 2, 3, 4]
+        for k, (threads, per_cu, secs) in best.items():
+            assert (threads, per_cu) == gs.autotune.get_shape(k)[:2] and secs > 0
+        print("tuned", best)
+        x1, s1 = _solve_gpu(gs, device, K, m, off, Y, X0, method, gs.Options())
+        _assert_bitwise(x1, s1, x0, s0)
+        # a shape with one resident workgroup per CU: fewer lanes than fits, every lane loops
+        for k in (1, 2, 3, 4):
+            gs.autotune.set_shape(k, 64, 1)
+        x2, s2 = _solve_gpu(gs, device, K, m, off, Y, X0, method, gs.Options())
+        _assert_bitwise(x2, s2, x0, s0)
+    finally:
+        for k in (1, 2, 3, 4):
+            gs.autotune.set_shape(k, 64, 0)
+
+
+def test_shutdown_releases_and_recreates(gs, workloads, device):
+    """gs_shutdown frees the library's cached workspaces, graphs, streams and events; the next call
+    creates them again and gives the same results bit for bit."""
+    lib = gs.load_library()
+    K, m, off, Y, X0, _ = workloads.expsum_fits(3000, seed=14)
+    method = _lm(gs)
+    x0, s0 = _solve_gpu(gs, device, K, m, off, Y, X0, method, gs.Options())
+    assert lib.gs_shutdown() == 0
+    x1, s1 = _solve_gpu(gs, device, K, m, off, Y, X0, method, gs.Options())
+    _assert_bitwise(x1, s1, x0, s0)
+    assert lib.gs_shutdown() == 0 and lib.gs_shutdown() == 0  # idempotent
+    S = gs.status_buffer(len(X0), device)
+    S.copy_(torch.as_tensor(s1.view(np.uint8).reshape(len(X0), 72), device=device))
+    summ = gs.batch_summary(S)
+    assert summ.failure_count == int((s1["code"] != gs.SUCCESS).sum())

@@ -85,7 +85,7 @@ def run_gpu(gs, device, method, family, X0, p_host, opts, order=None, workgroups
     if counts:
         c = gs.solve_batch(X, prob, method, p, opts, order=order, workgroupsize=workgroupsize)
         return X.cpu().numpy(), c
-    S = gs.status_buffer(X.shape[0], device)
+    S = gs.status_buffer(X.shape[0], device, "float32" if X.dtype == torch.float32 else "float64")
     gs.solve_batch(X, S, prob, method, p, opts, order=order, workgroupsize=workgroupsize)
     return X.cpu().numpy(), gs.statuses(S)
 

@@ -97,7 +97,14 @@ def test_validation_order_and_messages(gs):
     with pytest.raises(gs.ArgumentError, match="pass either `options` or keyword arguments, not both"):
         gs.solve_batch(X, prob, gs.Newton(), (2.125, 1.5), options=gs.Options(), max_iterations=3)
     with pytest.raises(gs.ArgumentError, match="SolverStatus real type Float64"):
-        gs.solve_batch(X.float(), prob, gs.Newton(), (2.125, 1.5))
+        gs.solve_batch(X.half(), prob, gs.Newton(), (2.125, 1.5))
+    # a Float32 batch takes SolverStatus{Float32} and Options{Float32} (batched.jl:268-272, options.jl:241-245)
+    with pytest.raises(gs.ArgumentError, match="SolverStatus real type Float64 does not match batch real element type Float32"):
+        gs.solve_batch(X.float(), gs.status_buffer(3, "cpu"), prob, gs.Newton(), (2.125, 1.5))
+    with pytest.raises(gs.ArgumentError, match=r"options are Options\{Float64\}, but the iterate has element type Float32"):
+        gs.solve_batch(X.float(), prob, gs.Newton(), (2.125, 1.5), options=gs.Options())
+    with pytest.raises(gs.ArgumentError, match="device array"):
+        gs.solve_batch(X.float(), prob, gs.Newton(), (2.125, 1.5), options=gs.Options(eltype="float32"))
 
 
 def test_host_tensor_is_refused_not_solved(gs):
@@ -189,6 +196,7 @@ def test_least_squares_entry_point_checks_method_without_gpu(gs):
     assert rc == abi.GS_EFAMILY
     rc = lib.gs_lsq_set_shape(5, 256, 1)
     assert rc == abi.GS_EINVAL
+    assert lib.gs_lsq_set_shape(1, 64, 65) == abi.GS_EINVAL and lib.gs_lsq_set_shape(1, 96, 0) == abi.GS_EINVAL
 
 
 def test_reuse_method_and_entry_checks(gs):
@@ -247,4 +255,21 @@ def test_dogleg_entry_checks(gs):
     rc = lib.gs_solve_batch_f64(ctypes.byref(m), ctypes.byref(fam), 1, None, None, 0, None, codes, None,
                                 ctypes.byref(o), 0, None)
     assert rc == abi.GS_EFAMILY
+
+
+def test_float32_options_and_status_layout(gs):
+    """Options(Float32) takes √eps, 2 eps and 1/10 of Float32 and holds every tolerance as its Float32
+    value (options.jl:207-235); SolverStatus{Float32} is 44 bytes at Julia's offsets."""
+    o = gs.Options(eltype="float32")
+    f32 = np.float32
+    assert o.f_reltol == float(np.sqrt(f32(np.finfo(f32).eps))) and o.x_reltol == 2 * float(np.finfo(f32).eps)
+    assert o.kappa == float(f32(0.1)) and o.g_tol == o.f_reltol
+    assert gs.Options(eltype="float32", f_abstol=1e-12).f_abstol == float(f32(1e-12))
+    d = gs.STATUS_DTYPE_F32
+    assert d.itemsize == 44
+    assert [d.fields[n][1] for n in ("code", "f_calls", "iterations", "fnorm", "stepnorm", "step_failures",
+                                     "promoted", "stationary", "jvp_calls", "rank", "nullnorm", "gradratio",
+                                     "rate", "contracted", "reused")] == [0, 2, 4, 8, 12, 16, 20, 21, 22, 24, 28,
+                                                                           32, 36, 40, 41]
+    assert gs.status_buffer(5, "cpu", "float32").shape == (5, 44)
 

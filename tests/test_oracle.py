@@ -468,3 +468,22 @@ def test_dogleg_radius_validation(gs):
     with pytest.raises(gs.ArgumentError, match="DogLeg globalization takes no reuse"):
         gs.Newton(globalization=gs.DogLeg(), reuse=2)
     assert gs.Newton(globalization=gs.DogLeg(radius=3.0)).to_c().dl_radius == 3.0
+
+
+def test_float32_batch_oracle(gs, orc, workloads):
+    """The Float32 batch of the oracle (the Float32 cells of the golden table pin its solve): the exact
+    dyadic root takes no step in Float32 too, and the Hénon–Heiles batch in Float32 converges to the
+    Float64 solutions within Float32 accuracy."""
+    P = np.array([[2.125, 1.5], [1.25, 2.0]], dtype=np.float32)
+    X0 = np.array([[2.0, 1.0], [2.5, 1.5]], dtype=np.float32)
+    x, st = orc.solve_batch(gs.Newton(), gs.Coupled2(), X0.copy(), gs.PerSystem(P), gs.Options(eltype="float32"))
+    assert x.dtype == np.float32 and st.dtype.itemsize == 44
+    assert st["iterations"][0] == 0 and np.array_equal(x[0], X0[0]) and (st["code"] == gs.SUCCESS).all()
+    xk = workloads.henon_heiles_states(2048, seed=5)
+    x32, s32 = orc.solve_batch(gs.Newton(), gs.HenonHeilesMidpoint(), xk.astype(np.float32),
+                               gs.BatchParams(0.1, xk.astype(np.float32)),
+                               gs.Options(eltype="float32", f_abstol=1e-6, f_reltol=0.0))
+    x64, _ = orc.solve_batch(gs.Newton(), gs.HenonHeilesMidpoint(), xk.copy(), gs.BatchParams(0.1, xk),
+                             gs.Options(f_abstol=1e-12, f_reltol=0.0))
+    assert (s32["code"] == gs.SUCCESS).all()
+    assert np.abs(x32 - x64).max() <= 1e-5 * np.abs(x64).max()
