@@ -66,6 +66,7 @@ DevParams dev_params(const gs_method& method, const gs_options_f64& options) {
 void release_lsq();
 void release_mlp();
 void release_reduce();
+void release_mid();
 
 static thread_local bool g_timing = false;
 static thread_local double g_last_ms = -1.0;
@@ -282,6 +283,7 @@ int gs_shutdown(void) {
     release_lsq();
     release_mlp();
     release_reduce();
+    release_mid();
     if (g_ev[0]) (void)hipEventDestroy(g_ev[0]);
     if (g_ev[1]) (void)hipEventDestroy(g_ev[1]);
     g_ev[0] = g_ev[1] = nullptr;

@@ -1,4 +1,4 @@
-// libgsolve_hip, mid-size systems: one wave per system, the factor in LDS.
+// libgsolve_hip, mid-size systems: one wave per system, each lane's factor row in VGPRs.
 //
 // The FPU-β Gauss–Legendre s = 3 stage system (n = 48, GS_FAM_FPU_GL3) under Newton with the Static
 // globalization, in the working precision (static LU in FP64) or mixed (LUFactorization(Float32)
@@ -6,20 +6,27 @@
 // 277-378) with newton_step!! (src/nonlinear/newton.jl:1604-1720) per system, as the lane-per-system
 // kernels run it for n ≤ 8.
 //
-// Lane i < 48 owns entry i of every vector (in LDS) and row i of the factor; the scalar state of the
-// solve is uniform across the wave. Sums that the reference folds left to right (norms, the
-// matrix-vector products) are folded in that order: the norms by every lane over the LDS vector
-// (broadcast reads), the products of the tridiagonal-block Jacobian by the lane of their row or
-// column over its nonzero entries only (a structural zero adds an exact zero to the fold). The
-// Jacobian itself is never stored: its entries are recomputed from the bond stiffnesses K.
+// Lane i < 48 owns entry i of every vector (in LDS) and row i of the factor (in registers: 96 VGPRs
+// in FP64, 48 in FP32); the scalar state of the solve is uniform across the wave. The factorisation
+// is the reference's unblocked LU in the factor's precision — first-largest pivot, a/p multipliers,
+// rank-1 updates without FMA — so both the FP64 and the FP32 factor match the oracle bit for bit:
+//   - the pivot search is a DPP max over the wave, then a ballot of the lanes that hold it;
+//   - rows never move: each lane keeps the logical row index `pos` of the row it holds, and a row
+//     swap exchanges two indices (ties go to the smallest logical index, as the reference's scan);
+//   - the pivot row's entries reach the other lanes by v_readlane, one broadcast per entry, which
+//     every lane below the pivot uses in one multiply and one subtract.
+// In this layout a step of the rank-1 update costs one instruction per column for all 48 rows at
+// once, so a blocked form with the trailing update on MFMA saves no VALU work and adds LDS staging
+// (measured: the LDS-resident blocked FP32 LU took 128k cycles per factorisation).
 //
-// The FP64 factorisation is the reference's unblocked LU (first-largest pivot by a wave reduction,
-// a/p multipliers, rank-1 updates without FMA), so the working-precision solve matches the oracle
-// bit for bit. The FP32 factor of the mixed method is blocked (16-column panels): the panels and
-// U12 are factored as the reference does, and the trailing update A22 −= L21 U12 runs on
-// v_mfma_f32_16x16x4f32, whose sums round differently; the FP64 refinement corrects the step.
+// Sums that the reference folds left to right (norms, the matrix-vector products, ‖J‖_F) are folded
+// in that order: the norms by every lane over the LDS vector (broadcast reads), the products of the
+// tridiagonal-block Jacobian by the lane of their row or column over its nonzero entries only (a
+// structural zero adds an exact zero to the fold). The Jacobian itself is never stored: its entries
+// are recomputed from the bond stiffnesses K.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 
 #include "mid_solver.hpp"
@@ -42,35 +49,116 @@ __constant__ double kGlA2[9] = {0x1.6c16c16c16c17p-7, -0x1.0f9922995a9d1p-7, 0x1
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 struct alignas(16) MidShared {
-    double A[kN * kN];  // the factor: FP64, or FP32 in its first half
     double x[kN], Fx[kN], d[kN], r[kN], t1[kN], t2[kN], t3[kN], bx[kN];
-    float t32[kN];
     double qp[32];      // q, p of the system
     double Qv[kN];      // stage positions
     double g[kN];       // stage forces
     double K[3 * 17];   // bond stiffnesses at the Jacobian's point
-    int piv[kN];
 };
 
 GS_DEV void sync() { __syncthreads(); }
 
 GS_DEV int lane_id() { return int(threadIdx.x); }
 
-GS_DEV double wave_fmax(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = __builtin_fmax(v, __shfl_xor(v, o));
-    return v;
+// ---- wave primitives (called with all 64 lanes active) ---------------------------------------------
+
+GS_DEV int readlane(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+GS_DEV float readlane(float v, int lane) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+}
+GS_DEV double readlane(double v, int lane) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(b)), lane));
+    const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(b >> 32)), lane));
+    return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
 }
 
-// Left folds over an LDS vector of 48, evaluated by every lane (uniform results).
+// The value a DPP move (CTRL) brings into this lane; lanes outside the row mask ROWS keep their own.
+template <int CTRL, int ROWS> GS_DEV float dpp_move(float v) {
+    const int b = __builtin_bit_cast(int, v);
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(b, b, CTRL, ROWS, 0xF, false));
+}
+template <int CTRL, int ROWS> GS_DEV double dpp_move(double v) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const int lo = int(uint32_t(b)), hi = int(uint32_t(b >> 32));
+    const uint32_t mlo = uint32_t(__builtin_amdgcn_update_dpp(lo, lo, CTRL, ROWS, 0xF, false));
+    const uint32_t mhi = uint32_t(__builtin_amdgcn_update_dpp(hi, hi, CTRL, ROWS, 0xF, false));
+    return __builtin_bit_cast(double, (uint64_t(mhi) << 32) | mlo);
+}
+
+// The maximum over the wave of a value that is nowhere NaN, in every lane: quad swaps, half-row and
+// row mirrors give each row of 16 its maximum, the row_bcast:15 / row_bcast:31 steps carry it into
+// lane 63.
+template <class T> GS_DEV T wave_max(T v) {
+    v = tfmax(v, dpp_move<0xB1, 0xF>(v));   // quad_perm [1,0,3,2]
+    v = tfmax(v, dpp_move<0x4E, 0xF>(v));   // quad_perm [2,3,0,1]
+    v = tfmax(v, dpp_move<0x141, 0xF>(v));  // row_half_mirror
+    v = tfmax(v, dpp_move<0x140, 0xF>(v));  // row_mirror
+    v = tfmax(v, dpp_move<0x142, 0xA>(v));  // row_bcast:15 into rows 1 and 3
+    v = tfmax(v, dpp_move<0x143, 0xC>(v));  // row_bcast:31 into rows 2 and 3
+    return readlane(v, 63);
+}
+
+// The signed maximum of a 32-bit integer over the wave, in every lane (the same DPP pattern on
+// v_max_i32).
+template <int CTRL, int ROWS> GS_DEV int dpp_move(int v) {
+    // lanes outside ROWS receive INT_MIN, the identity of the max (so the move folds into v_max_i32_dpp)
+    return __builtin_amdgcn_update_dpp(int(0x80000000u), v, CTRL, ROWS, 0xF, false);
+}
+GS_DEV int imax(int a, int b) { return a > b ? a : b; }
+GS_DEV int wave_max_i32(int v) {
+    v = imax(v, dpp_move<0xB1, 0xF>(v));
+    v = imax(v, dpp_move<0x4E, 0xF>(v));
+    v = imax(v, dpp_move<0x141, 0xF>(v));
+    v = imax(v, dpp_move<0x140, 0xF>(v));
+    v = imax(v, dpp_move<0x142, 0xA>(v));
+    v = imax(v, dpp_move<0x143, 0xC>(v));
+    return readlane(v, 63);
+}
+
+GS_DEV int lowest_lane(uint64_t mask) { return __builtin_ctzll(mask); }
+
+// The lanes holding the largest of c over the lanes with `cand`, where c is +0, a positive float,
+// +inf or −inf (a non-candidate or a NaN): as IEEE bit patterns these order as signed integers, so
+// the maximum is an integer reduction — in FP64 over the high words, then over the low words of the
+// lanes that share the largest high word. (Each reduction runs before any condition that depends on
+// the lane, so that every lane takes part.)
+GS_DEV uint64_t lanes_of_max(float c, bool cand) {
+    const int key = __builtin_bit_cast(int, c);
+    const int top = wave_max_i32(key);
+    return __ballot(cand && key == top);
+}
+GS_DEV uint64_t lanes_of_max(double c, bool cand) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, c);
+    const int hi = int(uint32_t(b >> 32));
+    const int mhi = wave_max_i32(hi);
+    const bool top = cand && hi == mhi;
+    uint64_t hit = __ballot(top);
+    if (hit & (hit - 1)) {
+        const int lo = top ? int(uint32_t(b) ^ 0x80000000u) : int(0x80000000u);
+        const int mlo = wave_max_i32(lo);
+        hit = __ballot(top && lo == mlo);
+    }
+    return hit;
+}
+
+// Materialises v in its VGPRs here: an empty asm that neither reads memory nor emits an instruction.
+// It stops the compiler from sinking a per-lane select to the select's far use, which would keep
+// both inputs and the lane mask of every LU step live across the whole factorisation.
+template <class T> GS_DEV void keep(T& v) { asm volatile("" : "+v"(v)); }
+
+// Left folds over an LDS vector of 48, evaluated by every lane (uniform results). The loops unroll
+// by 8 only: a full unroll would hoist all 48 broadcast reads into 96 VGPRs beside the factor rows.
 GS_DEV double fold_norm2(const double* v) {
     double s = v[0] * v[0];
+#pragma unroll 8
     for (int i = 1; i < kN; ++i) s = s + v[i] * v[i];
     return s;
 }
 
 GS_DEV double fold_rmax(const double* v) {
     double m = __builtin_fabs(v[0]);
+#pragma unroll 8
     for (int i = 1; i < kN; ++i) m = jmax(m, __builtin_fabs(v[i]));
     return m;
 }
@@ -84,6 +172,7 @@ GS_DEV double fold_rnorm(const double* v) {
     const int e = exponent_of(m);
     double t = ldexp2(v[0], -e);
     double acc = t * t;
+#pragma unroll 8
     for (int i = 1; i < kN; ++i) {
         t = ldexp2(v[i], -e);
         acc = acc + t * t;
@@ -96,8 +185,8 @@ GS_DEV double fold_rnorm(const double* v) {
 GS_DEV int vec_exponent(const double* v) {
     const int l = lane_id();
     const double a = l < kN ? __builtin_fabs(v[l]) : 0.0;
-    const bool nan_any = __any(l < kN && isnan_(a));
-    const double m = wave_fmax(a);
+    const bool nan_any = __ballot(l < kN && isnan_(a)) != 0;
+    const double m = wave_max(isnan_(a) ? 0.0 : a);
     if (nan_any || !isfin(m) || m == 0.0) return 0;
     const int e = exponent_of(m);
     return e < -kEmax64 ? -kEmax64 : (e > kEmax64 ? kEmax64 : e);
@@ -214,41 +303,45 @@ GS_DEV void jmatvec(MidShared& s, const Fpu& f, double mu, const double* v, doub
     sync();
 }
 
-// ‖J‖_F: the reference's left fold over the entries in column-major order (zeros add nothing).
+// ‖J‖_F: the reference's left fold over the entries in column-major order (rnorm, reductions.jl:
+// 41-57). Lane j forms the nine entries of column j at rows a·16 + k−1, k, k+1 (k = j mod 16; zero
+// where the chain ends, and a zero square adds nothing to the nonnegative sum), then the fold takes
+// their squares column by column through v_readlane.
 GS_DEV double jnorm_fold(const MidShared& s, const Fpu& f) {
-    double acc = 0.0, mx = 0.0;
-    bool first = true;
-    for (int j = 0; j < kN; ++j) {
-        const int k = j % kSites;
-        for (int a = 0; a < 3; ++a) {
-            for (int dm = -1; dm <= 1; ++dm) {
-                const int m = k + dm;
-                if (m < 0 || m >= kSites) continue;
-                const double v = jentry(s, f, a * kSites + m, j);
-                acc = first ? v * v : acc + v * v;
-                mx = first ? __builtin_fabs(v) : jmax(mx, __builtin_fabs(v));
-                first = false;
-            }
-        }
+    const int l = lane_id();
+    const int j = l < kN ? l : 0;
+    const int k = j % kSites;
+    double v[9], sq[9];
+    double mx = 0.0;
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        const int m = k + t % 3 - 1;
+        v[t] = (l < kN && m >= 0 && m < kSites) ? jentry(s, f, (t / 3) * kSites + m, j) : 0.0;
+        sq[t] = v[t] * v[t];
+        mx = jmax(mx, __builtin_fabs(v[t]));
+    }
+    double acc = 0.0;
+    for (int c = 0; c < kN; ++c) {
+#pragma unroll
+        for (int t = 0; t < 9; ++t) acc = acc + readlane(sq[t], c);
     }
     if (isfin(acc) && acc >= double(kN * kN) * kDblMin) return __builtin_sqrt(acc);
+    // the scaled path: the NaN-propagating maximum of |J_ij|, then the fold of (J_ij/2^e)²
+    const bool nan_any = __ballot(isnan_(mx)) != 0;
+    mx = nan_any ? dnan() : wave_max(mx);
     if (mx == 0.0 || !isfin(mx)) return mx;
     const int e = exponent_of(mx);
-    double t = 0.0;
-    first = true;
-    for (int j = 0; j < kN; ++j) {
-        const int k = j % kSites;
-        for (int a = 0; a < 3; ++a) {
-            for (int dm = -1; dm <= 1; ++dm) {
-                const int m = k + dm;
-                if (m < 0 || m >= kSites) continue;
-                const double v = ldexp2(jentry(s, f, a * kSites + m, j), -e);
-                t = first ? v * v : t + v * v;
-                first = false;
-            }
-        }
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        const double w = ldexp2(v[t], -e);
+        sq[t] = w * w;
     }
-    return ldexp2(__builtin_sqrt(t), e);
+    double acc2 = 0.0;
+    for (int c = 0; c < kN; ++c) {
+#pragma unroll
+        for (int t = 0; t < 9; ++t) acc2 = acc2 + readlane(sq[t], c);
+    }
+    return ldexp2(__builtin_sqrt(acc2), e);
 }
 
 // Jᴴ (sc F) into out (column folds over the nonzero rows).
@@ -310,181 +403,157 @@ GS_DEV int jreg_exponent(const MidShared& s, const Fpu& f, double mu) {
                 nan_seen |= isnan_(v);
             }
     }
-    const bool nan_any = __any(nan_seen);
-    m = wave_fmax(m);
+    const bool nan_any = __ballot(nan_seen) != 0;
+    m = wave_max(isnan_(m) ? 0.0 : m);
     if (nan_any || !isfin(m) || m == 0.0) return 0;
     const int e = exponent_of(m);
     return e < -kEmax64 ? -kEmax64 : (e > kEmax64 ? kEmax64 : e);
 }
 
-// factor_matrix (factors.jl:802-811) into s.A: Jreg/2^e in FP64 (e < 0 only) or FP32 (any e ≠ 0).
-template <bool F32> GS_DEV int build_factor_matrix(MidShared& s, const Fpu& f, double mu) {
+// factor_matrix (factors.jl:802-811) as rows: lane l < 48 receives row l of Jreg/2^e in T, with e ≠ 0
+// applied in FP32 and only e < 0 in FP64; returns e. The entries are jreg's, formed from the six
+// stiffnesses and three coefficients of the lane's row. `tame` reports, uniformly, that every entry
+// is finite, no entry is −0 and every |entry| < 2^(emax−48): then no step of the LU meets a
+// non-finite value (growth ≤ 2^47 under partial pivoting, multipliers ≤ 1) nor makes a −0, and a
+// zero multiplier leaves a row exactly as it is.
+template <class T> GS_DEV int factor_rows(const MidShared& s, const Fpu& f, double mu, T (&a)[kN], bool& tame) {
+    constexpr bool kF32 = sizeof(T) == 4;
     const int e0 = jreg_exponent(s, f, mu);
-    const int e = F32 ? e0 : (e0 < 0 ? e0 : 0);
-    const int l = lane_id();
-    if (l < kN) {
-        for (int j = 0; j < kN; ++j) {
-            const double v = jreg(s, f, mu, l, j);
-            const double w = e != 0 ? ldexp2(v, -e) : v;
-            if (F32)
-                reinterpret_cast<float*>(s.A)[l + kN * j] = float(w);
-            else
-                s.A[l + kN * j] = w;
-        }
+    const int e = kF32 ? e0 : (e0 < 0 ? e0 : 0);
+    const int i = lane_id() < kN ? lane_id() : kN - 1;
+    const int ab = i / kSites, m = i % kSites;
+    double km[3], kp[3], cf[3];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+        km[b] = s.K[b * 17 + m];
+        kp[b] = s.K[b * 17 + m + 1];
+        cf[b] = f.coef(ab, b);
     }
-    sync();
+    const T big = ldexp2(T(1), Lim<T>::emax - kN);
+    bool wild = false;
+#pragma unroll
+    for (int j = 0; j < kN; ++j) {
+        const int b = j / kSites, k = j % kSites;
+        double H = 0.0;
+        if (k == m) H = -(km[b] + kp[b]);
+        else if (k + 1 == m) H = km[b];
+        else if (k == m + 1) H = kp[b];
+        double v = (i == j ? 1.0 : 0.0) - cf[b] * H;
+        if (i == j && mu != 0.0) v = v + mu;
+        if (e != 0) v = ldexp2(v, -e);
+        a[j] = T(v);
+        wild |= !(tabs(a[j]) < big) || (a[j] == T(0) && __builtin_signbit(a[j]));
+    }
+    tame = __ballot(wild) == 0;
     return e;
 }
 
-// The pivot of column k over rows k..47: the first largest |a|, or k when |a_kk| is NaN (the
-// reference's strict > scan never moves off a NaN and never picks one).
-template <class T> GS_DEV int pivot_of(const T* A, int k) {
+// factorize!! StaticLU (lu.jl:24-64) on register rows; returns info (the first zero pivot, 1-based,
+// or 0). `pos` receives the logical row each lane's row ends in (−1 in lanes 48..63). Step k: the
+// pivot is the first largest |a_ik| over the logical rows i ≥ k, or row k itself when |a_kk| is NaN
+// (the reference's strict > scan never leaves a NaN and never picks one); the swap of rows k and p
+// exchanges two `pos` values; then every lane below the pivot forms its multiplier a_ik/a_kk and
+// subtracts multiplier × pivot-row entry (no FMA) from each entry right of column k.
+// TAME: the rows satisfy factor_rows' `tame`, and the other lanes run the same update with a zero
+// multiplier (an exact no-op there), which keeps every entry in its register; otherwise each entry
+// is selected per lane.
+template <class T, bool TAME> GS_DEV int lu_rows(T (&a)[kN], int& pos) {
     const int l = lane_id();
-    const double akk = __builtin_fabs(double(A[k + kN * k]));
-    double c = -__builtin_inf();
-    if (l >= k && l < kN) {
-        const double a = __builtin_fabs(double(A[l + kN * k]));
-        c = isnan_(a) ? -__builtin_inf() : a;
-    }
-    const double m = wave_fmax(c);
-    if (isnan_(akk)) return k;
-    const uint64_t hit = __ballot(l >= k && l < kN && c == m);
-    return hit ? __ffsll(static_cast<long long>(hit)) - 1 : k;
-}
-
-// Swap rows k and p across every column.
-template <class T> GS_DEV void swap_rows(T* A, int k, int p) {
-    const int l = lane_id();
-    if (p != k && l < kN) {
-        const T u = A[k + kN * l];
-        A[k + kN * l] = A[p + kN * l];
-        A[p + kN * l] = u;
-    }
-    sync();
-}
-
-// Elimination step k on columns k+1 .. jend−1: multipliers a_ik / a_kk, then a_ij − a_ik a_kj.
-template <class T> GS_DEV void eliminate(T* A, int k, int jend, int& info) {
-    const T akk = A[k + kN * k];
-    const int l = lane_id();
-    if (akk == T(0)) {
-        if (info == 0) info = k + 1;
-        return;
-    }
-    if (l > k && l < kN) {
-        const T lik = A[l + kN * k] / akk;
-        A[l + kN * k] = lik;
-        for (int j = k + 1; j < jend; ++j) A[l + kN * j] = A[l + kN * j] - lik * A[k + kN * j];
-    }
-    sync();
-}
-
-// factorize!! StaticLU (lu.jl:24-64) in FP64, unblocked.
-GS_DEV int lu64(MidShared& s) {
+    const bool live = l < kN;
+    pos = live ? l : -1;
     int info = 0;
+#pragma unroll
     for (int k = 0; k < kN; ++k) {
-        const int p = pivot_of(s.A, k);
-        if (lane_id() == 0) s.piv[k] = p;
-        swap_rows(s.A, k, p);
-        eliminate(s.A, k, kN, info);
-    }
-    sync();
-    return info;
-}
-
-// The same factorisation in FP32, blocked by 16 columns: each panel as the unblocked algorithm
-// (its row swaps across all columns), U12 by the panel's unit-lower forward substitution, and the
-// trailing update A22 −= L21 U12 on MFMA (v_mfma_f32_16x16x4f32, −L21 as the A operand).
-GS_DEV int lu32_blocked(MidShared& s) {
-    float* A = reinterpret_cast<float*>(s.A);
-    const int l = lane_id();
-    int info = 0;
-    for (int p0 = 0; p0 < kN; p0 += 16) {
-        const int pend = p0 + 16;
-        for (int k = p0; k < pend; ++k) {
-            const int p = pivot_of(A, k);
-            if (l == 0) s.piv[k] = p;
-            swap_rows(A, k, p);
-            eliminate(A, k, pend, info);
-        }
-        if (pend >= kN) break;
-        // U12: rows p0..pend−1, columns pend..47 (one lane per column)
-        for (int k = p0; k < pend; ++k) {
-            const int j = pend + l;
-            if (j < kN) {
-                const float akj = A[k + kN * j];
-                for (int i = k + 1; i < pend; ++i) A[i + kN * j] = A[i + kN * j] - A[i + kN * k] * akj;
-            }
-            sync();
-        }
-        // A22 −= L21 U12, 16 × 16 blocks, K = 16 in four MFMA steps of 4
-        const int nb = (kN - pend) / 16;
-        const int rr = l & 15, gg = l >> 4;
-        for (int bi = 0; bi < nb; ++bi) {
-            for (int bj = 0; bj < nb; ++bj) {
-                const int R0 = pend + 16 * bi, C0 = pend + 16 * bj;
-                f32x4 acc;
-#pragma unroll
-                for (int i = 0; i < 4; ++i) acc[i] = A[(R0 + 4 * gg + i) + kN * (C0 + rr)];
-#pragma unroll
-                for (int kk = 0; kk < 4; ++kk) {
-                    const int kc = p0 + 4 * kk + gg;
-                    const float av = -A[(R0 + rr) + kN * kc];
-                    const float bv = A[kc + kN * (C0 + rr)];
-                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+        const T ak = a[k];
+        const bool cand = live && pos >= k;
+        const T mag = tabs(ak);
+        const T c = (cand && !isnan_(mag)) ? mag : -T(__builtin_inf());
+        const uint64_t hit = lanes_of_max(c, cand);
+        const int owner = lowest_lane(__ballot(live && pos == k));
+        int piv = owner;
+        if (!isnan_(readlane(ak, owner))) {
+            piv = lowest_lane(hit);
+            if (hit & (hit - 1)) {
+                // a tie: the smallest logical row among the lanes holding the maximum
+                int best = readlane(pos, piv);
+                for (uint64_t h = hit & (hit - 1); h; h &= h - 1) {
+                    const int ln = lowest_lane(h);
+                    const int pl = readlane(pos, ln);
+                    if (pl < best) {
+                        best = pl;
+                        piv = ln;
+                    }
                 }
-                sync();
+            }
+        }
+        const int pp = readlane(pos, piv);
+        if (l == owner) pos = pp;
+        if (l == piv) pos = k;
+        // a zero pivot records info and eliminates nothing (no branch: the update runs with no row
+        // below, so that no entry changes register)
+        const T ukk = readlane(ak, piv);
+        const bool zero = ukk == T(0);
+        if (zero && info == 0) info = k + 1;
+        const bool below = live && pos > k && !zero;
+        const T lik = ak / ukk;
+        a[k] = below ? lik : ak;
+        keep(a[k]);
+        if (TAME) {
+            const T mk = below ? lik : T(0);
 #pragma unroll
-                for (int i = 0; i < 4; ++i) A[(R0 + 4 * gg + i) + kN * (C0 + rr)] = acc[i];
-                sync();
+            for (int j = k + 1; j < kN; ++j) {
+                a[j] = a[j] - mk * readlane(a[j], piv);
+                // keep the broadcasts of a few columns in flight, not the whole row's in SGPRs
+                if ((j - k) % 8 == 0) __builtin_amdgcn_sched_barrier(0);
+            }
+        } else {
+#pragma unroll
+            for (int j = k + 1; j < kN; ++j) {
+                const T t = a[j] - lik * readlane(a[j], piv);
+                a[j] = below ? t : a[j];
+                keep(a[j]);
             }
         }
     }
-    sync();
     return info;
 }
 
-// backsolve (lu.jl:67-92) of the factor in place on x (T = double or float).
-template <class T> GS_DEV void backsolve(MidShared& s, const T* A, T* x) {
+// backsolve (lu.jl:67-92) on the register rows: y is entry l of the right-hand side in lane l (the
+// row permutation is the `pos` map); returns, in each lane, entry pos of the solution. Forward over
+// the unit lower factor, then backward over U, each step broadcasting the finished entry.
+template <class T> GS_DEV T solve_rows(const T (&a)[kN], int pos, T y) {
     const int l = lane_id();
-    if (l == 0) {
-        for (int k = 0; k < kN; ++k) {
-            const int p = s.piv[k];
-            if (p != k) {
-                const T u = x[k];
-                x[k] = x[p];
-                x[p] = u;
-            }
-        }
+    const bool live = l < kN;
+#pragma unroll
+    for (int k = 0; k < kN - 1; ++k) {
+        const T yk = readlane(y, lowest_lane(__ballot(live && pos == k)));
+        const T t = y - a[k] * yk;
+        y = live && pos > k ? t : y;
     }
-    sync();
-    for (int k = 0; k < kN; ++k) {
-        const T xk = x[k];
-        if (l > k && l < kN) x[l] = x[l] - A[l + kN * k] * xk;
-        sync();
-    }
+#pragma unroll
     for (int k = kN - 1; k >= 0; --k) {
-        const T xk = x[k] / A[k + kN * k];
-        if (l == k) x[k] = xk;
-        if (l < k) x[l] = x[l] - A[l + kN * k] * xk;
-        sync();
+        const int o = lowest_lane(__ballot(live && pos == k));
+        const T xk = readlane(y, o) / readlane(a[k], o);
+        const T t = y - a[k] * xk;
+        y = l == o ? xk : (live && pos < k ? t : y);
     }
+    return y;
 }
 
 // ldiv!! with FP64 factors (factors.jl:931-936): out = J \ r (out ≠ r).
-GS_DEV void ldiv64(MidShared& s, int fe, const double* r, double* out) {
+GS_DEV void ldiv_rows(MidShared& s, const double (&a)[kN], int pos, int fe, const double* r, double* out) {
     const int k = vec_rhs_exponent(r, fe);
     const int l = lane_id();
-    if (l < kN) out[l] = k != 0 ? ldexp2(r[l], -k) : r[l];
-    sync();
-    backsolve<double>(s, s.A, out);
+    double y = 0.0;
+    if (l < kN) y = k != 0 ? ldexp2(r[l], -k) : r[l];
+    y = solve_rows<double>(a, pos, y);
     const int e = k - fe;
-    if (e != 0 && l < kN) out[l] = ldexp2(out[l], e);
+    if (l < kN) out[pos] = e != 0 ? ldexp2(y, e) : y;
     sync();
 }
 
 // ldiv!! with FP32 factors (factors.jl:943-949): r/s in FP32, then a·frac·2^(ex−e) in FP64.
-GS_DEV void ldiv32(MidShared& s, int fe, const double* r, double* out) {
+GS_DEV void ldiv_rows(MidShared& s, const float (&a)[kN], int pos, int fe, const double* r, double* out) {
     const int l = lane_id();
     const double sm = fold_rmax(r);
     if (sm == 0.0) {
@@ -492,12 +561,12 @@ GS_DEV void ldiv32(MidShared& s, int fe, const double* r, double* out) {
         sync();
         return;
     }
-    if (l < kN) s.t32[l] = float(r[l] / sm);
-    sync();
-    backsolve<float>(s, reinterpret_cast<const float*>(s.A), s.t32);
+    float y = 0.0f;
+    if (l < kN) y = float(r[l] / sm);
+    y = solve_rows<float>(a, pos, y);
     const int ex = isfin(sm) ? __builtin_amdgcn_frexp_exp(sm) : 0;
     const double frac = isfin(sm) ? __builtin_amdgcn_frexp_mant(sm) : sm;
-    if (l < kN) out[l] = ldexp2(double(s.t32[l]) * frac, ex - fe);
+    if (l < kN) out[pos] = ldexp2(double(y) * frac, ex - fe);
     sync();
 }
 
@@ -538,60 +607,70 @@ GS_DEV Info globalize_static(MidShared& s, const Fpu& f, const DevParams& q) {
     return Info{kSuccess, fnp, stepnorm, 0, false};
 }
 
-// refine_passes!! (refinement.jl:211-244) on s.d with the held Jreg; `failed` as the reference's.
-// Without Refinement() it leaves d as it is and does not fail.
-template <bool F32, bool REFINE>
-GS_DEV void refine(MidShared& s, const Fpu& f, const DevParams& q, int fe, bool& failed) {
-    failed = false;
-    if (!REFINE) return;
+// The step's direction d = −(J \ F) into s.d from a register factor, then — when `refine` holds —
+// refine_passes!! (refinement.jl:211-244) on it with the held Jreg: one loop whose pass 0 solves for F
+// and whose later passes solve for the residual r = −F − J d, so that each factor type has one
+// solve in the kernel. Returns refine_passes!!' `failed` (false without refinement).
+template <class T, bool REFINE>
+GS_DEV bool direction_refined(MidShared& s, const Fpu& f, const DevParams& q, const T (&a)[kN], int pos, int fe,
+                              bool refine) {
     const int l = lane_id();
-    const double fnorm = fold_rnorm(s.Fx);
-    if (fnorm == 0.0) return;
-    if (!isfin(fnorm)) {
-        failed = true;
-        return;
-    }
-    jmatvec(s, f, q.mu, s.d, s.t1);
-    if (l < kN) s.r[l] = -s.Fx[l] + -s.t1[l];
-    sync();
-    double rprev = fold_rnorm(s.r);
-    if (!isfin(rprev)) {
-        failed = true;
-        return;
-    }
-    if (rprev <= q.ref_eta_eff * fnorm) return;
-    for (int it = 1; it <= q.ref_maxiter; ++it) {
-        if (F32)
-            ldiv32(s, fe, s.r, s.t2);
-        else
-            ldiv64(s, fe, s.r, s.t2);
-        if (l < kN) s.t3[l] = s.d[l] + s.t2[l];
-        sync();
-        jmatvec(s, f, q.mu, s.t3, s.t1);
-        if (l < kN) s.r[l] = -s.Fx[l] + -s.t1[l];
-        sync();
-        const double rcur = fold_rnorm(s.r);
-        if (!(rcur < rprev)) {
-            failed = true;
-            return;
+    double fnorm = 0.0, rprev = 0.0;
+    for (int it = 0;; ++it) {
+        ldiv_rows(s, a, pos, fe, it == 0 ? s.Fx : s.r, it == 0 ? s.d : s.t2);
+        if (it == 0) {
+            if (l < kN) s.d[l] = -s.d[l];
+            sync();
+            if (!REFINE || !refine) return false;
+            fnorm = fold_rnorm(s.Fx);
+            if (fnorm == 0.0) return false;
+            if (!isfin(fnorm)) return true;
+            jmatvec(s, f, q.mu, s.d, s.t1);
+            if (l < kN) s.r[l] = -s.Fx[l] + -s.t1[l];
+            sync();
+            rprev = fold_rnorm(s.r);
+            if (!isfin(rprev)) return true;
+            if (rprev <= q.ref_eta_eff * fnorm) return false;
+            if (q.ref_maxiter < 1) return true;
+        } else {
+            if (l < kN) s.t3[l] = s.d[l] + s.t2[l];
+            sync();
+            jmatvec(s, f, q.mu, s.t3, s.t1);
+            if (l < kN) s.r[l] = -s.Fx[l] + -s.t1[l];
+            sync();
+            const double rcur = fold_rnorm(s.r);
+            if (!(rcur < rprev)) return true;
+            if (l < kN) s.d[l] = s.t3[l];
+            sync();
+            rprev = rcur;
+            if (rcur <= q.ref_eta_eff * fnorm) return false;
+            if (it >= q.ref_maxiter) return true;
         }
-        if (l < kN) s.d[l] = s.t3[l];
-        sync();
-        rprev = rcur;
-        if (rcur <= q.ref_eta_eff * fnorm) return;
     }
-    failed = true;
 }
 
-// The step's direction d = −(J \ F) into s.d from the factor in s.A.
-template <bool F32> GS_DEV void direction(MidShared& s, int fe) {
-    if (F32)
-        ldiv32(s, fe, s.Fx, s.d);
-    else
-        ldiv64(s, fe, s.Fx, s.d);
-    const int l = lane_id();
-    if (l < kN) s.d[l] = -s.d[l];
-    sync();
+// The value factor_and_direction returns, and the code a step reports, when the rows are not tame
+// and the kernel factors only tame rows: the system is left to the EXACT kernel.
+constexpr int kRedoInfo = -1;
+constexpr uint8_t kRedo = 0xFF;
+
+// newton_factor's factorisation and the direction with T factors: Jreg at the state's point into
+// register rows, the LU, then direction_refined. Returns the LU's info (no direction when > 0) or
+// kRedoInfo; `failed` is direction_refined's. EXACT selects the LU with per-lane selects, exact for
+// any rows; otherwise the zero-multiplier LU runs on tame rows and anything else is handed over.
+// (One LU form per kernel: two inlined forms side by side cost the register allocator the factor's
+// registers twice over.)
+template <class T, bool REFINE, bool EXACT>
+GS_DEV int factor_and_direction(MidShared& s, const Fpu& f, const DevParams& q, bool refine, bool& failed) {
+    T a[kN];
+    bool tame;
+    const int fe = factor_rows<T>(s, f, q.mu, a, tame);
+    failed = false;
+    if (!EXACT && !tame) return kRedoInfo;
+    int pos;
+    const int info = lu_rows<T, !EXACT>(a, pos);
+    failed = info == 0 && direction_refined<T, REFINE>(s, f, q, a, pos, fe, refine);
+    return info;
 }
 
 // newton_factor (newton.jl:1302-1311) up to the factorisation: fnorm, J at x, its diagnostics.
@@ -602,19 +681,13 @@ GS_DEV void factor_diag(MidShared& s, const Fpu& f, NewtonState& st) {
 }
 
 // newton_step!! plain (newton.jl:1679-1688).
-template <bool REFINE> GS_DEV Info plain_step(MidShared& s, const Fpu& f, const DevParams& q, NewtonState& st) {
+template <bool REFINE, bool EXACT>
+GS_DEV Info plain_step(MidShared& s, const Fpu& f, const DevParams& q, NewtonState& st) {
     factor_diag(s, f, st);
-    const int fe = build_factor_matrix<false>(s, f, q.mu);
-    const int info = lu64(s);
-    Info r;
-    if (info > 0) {
-        r = Info{kSingular, fold_rnorm(s.Fx), 0.0, 0, false};
-    } else {
-        direction<false>(s, fe);
-        bool failed;
-        refine<false, REFINE>(s, f, q, fe, failed);
-        r = globalize_static(s, f, q);
-    }
+    bool failed;
+    const int info = factor_and_direction<double, REFINE, EXACT>(s, f, q, true, failed);
+    if (info == kRedoInfo) return Info{kRedo, 0.0, 0.0, 0, false};
+    const Info r = info > 0 ? Info{kSingular, fold_rnorm(s.Fx), 0.0, 0, false} : globalize_static(s, f, q);
     st.fnorm = r.fnorm;
     return r;
 }
@@ -629,49 +702,36 @@ GS_DEV double linear_residual_ratio(MidShared& s, const Fpu& f, const DevParams&
     return fold_rnorm(s.t2) / fnorm_old;
 }
 
-// mixed_step!! + mixed_newton!! with fresh factors (newton.jl:1604-1666, 1692-1720).
-template <bool REFINE> GS_DEV Info mixed_step(MidShared& s, const Fpu& f, const DevParams& q, NewtonState& st) {
+// mixed_step!! + mixed_newton!! with fresh factors (newton.jl:1604-1666, 1692-1720). The FP64
+// factor serves both a state promoted before the step (with refinement) and the promotion after a
+// failed FP32 step (promote_factor!!, newton.jl:1332-1339: no refinement), from one site.
+template <bool REFINE, bool EXACT>
+GS_DEV Info mixed_step(MidShared& s, const Fpu& f, const DevParams& q, NewtonState& st) {
     const double fnorm_old = isnan_(st.fnorm) ? fold_rnorm(s.Fx) : st.fnorm;
     const bool was_promoted = st.promoted;
-    bool ok = true, md_promoted = st.promoted;
+    bool ok = true;
     int32_t failures = 0;
     double lin_rel = 0.0;
-    if (st.promoted) {
-        factor_diag(s, f, st);
-        const int fe = build_factor_matrix<false>(s, f, q.mu);
-        if (lu64(s) > 0) {
-            ok = false;
-        } else {
-            direction<false>(s, fe);
-            bool failed;
-            refine<false, REFINE>(s, f, q, fe, failed);
-        }
-    } else {
-        factor_diag(s, f, st);
-        const int fe = build_factor_matrix<true>(s, f, q.mu);
-        const int info = lu32_blocked(s);
-        bool ref_failed = false;
-        if (info > 0) {
-            ref_failed = true;
-        } else {
-            direction<true>(s, fe);
-            refine<true, REFINE>(s, f, q, fe, ref_failed);
-        }
-        if (ref_failed || info > 0) {
+    factor_diag(s, f, st);
+    if (!was_promoted) {
+        bool dir_failed;
+        const int info32 = factor_and_direction<float, REFINE, EXACT>(s, f, q, true, dir_failed);
+        if (info32 == kRedoInfo) return Info{kRedo, 0.0, 0.0, 0, false};
+        const bool ref_failed = info32 > 0 || dir_failed;
+        if (ref_failed) {
             failures += 1;
-            // promote_factor!! (newton.jl:1332-1339): Jreg in FP64
-            const int fe64 = build_factor_matrix<false>(s, f, q.mu);
-            md_promoted = true;
             st.promoted = true;
-            if (lu64(s) > 0) {
-                ok = false;
-            } else {
-                direction<false>(s, fe64);
-            }
         } else {
             lin_rel = linear_residual_ratio(s, f, q, fnorm_old);
         }
     }
+    if (st.promoted) {
+        bool failed;
+        const int info64 = factor_and_direction<double, REFINE, EXACT>(s, f, q, was_promoted, failed);
+        if (info64 == kRedoInfo) return Info{kRedo, 0.0, 0.0, 0, false};
+        ok = info64 == 0;
+    }
+    const bool md_promoted = st.promoted;
     if (!ok) {
         Info r{kSingular, fold_rnorm(s.Fx), 0.0, failures, st.promoted};
         st.fnorm = r.fnorm;
@@ -723,12 +783,16 @@ GS_DEV void store_status_mid(gs_status_f64* dst, uint8_t code, int32_t iteration
     w[8] = uint64_t(contracted ? 1 : 0);
 }
 
-template <bool MIXED, bool REFINE> __global__ __launch_bounds__(64) void fpu_gl3_kernel(const MidLaunch L) {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    MidShared& s = *reinterpret_cast<MidShared*>(lds_raw);
+// One wave per system. The kernel with EXACT = false factors with the zero-multiplier LU and hands a
+// system whose rows are ever not tame to the EXACT kernel, launched after it over the same grid:
+// that system keeps its input untouched and redo[idx] = 1; every other system writes redo[idx] = 0,
+// and the EXACT kernel solves only the systems marked 1, from the start.
+template <bool MIXED, bool REFINE, bool EXACT> __global__ __launch_bounds__(64) void fpu_gl3_kernel(const MidLaunch L) {
+    __shared__ MidShared s;
     const int l = lane_id();
     const int64_t i = blockIdx.x;
     const int64_t idx = L.order ? L.order[i] - 1 : i;
+    if (EXACT && L.redo[idx] == 0) return;
     const DevParams& q = L.q;
     const Fpu f = fpu_of(L.shared);
     {
@@ -763,7 +827,11 @@ template <bool MIXED, bool REFINE> __global__ __launch_bounds__(64) void fpu_gl3
     int32_t stalls = 0;
     while (!done) {
         const double rate_k = rate;
-        const Info info = MIXED ? mixed_step<REFINE>(s, f, q, st) : plain_step<REFINE>(s, f, q, st);
+        const Info info = MIXED ? mixed_step<REFINE, EXACT>(s, f, q, st) : plain_step<REFINE, EXACT>(s, f, q, st);
+        if (!EXACT && info.code == kRedo) {
+            if (l == 0) L.redo[idx] = 1;
+            return;
+        }
         if (q.best && info.fnorm < st.best_fnorm) {
             if (l < kN) s.bx[l] = s.x[l];
             st.best_fnorm = info.fnorm;
@@ -818,28 +886,62 @@ template <bool MIXED, bool REFINE> __global__ __launch_bounds__(64) void fpu_gl3
     }
     if (l < kN) L.X[idx * kN + l] = s.x[l];
     if (l == 0) {
+        if (!EXACT) L.redo[idx] = 0;
         if (L.S) store_status_mid(L.S + idx, code, it, fnorm, stepnorm, failures, promoted, st.gradratio, rate, contracted);
         if (L.codes) L.codes[idx] = code;
     }
 }
 
-}  // namespace
+// The per-system hand-over flags of the two kernels, one byte per system, kept per device until
+// gs_shutdown (the entry point synchronises before it returns, so no launch is in flight when the
+// buffer grows).
+struct RedoFlags {
+    uint8_t* ptr = nullptr;
+    size_t bytes = 0;
+};
+std::mutex g_redo_mutex;
+RedoFlags g_redo[64];
 
-template <bool MIXED, bool REFINE> int launch_variant(const MidLaunch& L, hipStream_t st) {
-    const size_t lds = sizeof(MidShared);
-    static bool attr_set = false;
-    if (!attr_set) {
-        if (hipFuncSetAttribute(reinterpret_cast<const void*>(&fpu_gl3_kernel<MIXED, REFINE>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)) != hipSuccess)
-            return set_error(GS_EHIP, "hipFuncSetAttribute failed");
-        attr_set = true;
+int redo_flags(size_t bytes, uint8_t** out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return set_error(GS_EHIP, "hipGetDevice failed");
+    std::lock_guard<std::mutex> lock(g_redo_mutex);
+    RedoFlags& w = g_redo[dev];
+    if (w.bytes < bytes) {
+        if (w.ptr) (void)hipFree(w.ptr);
+        w.ptr = nullptr;
+        w.bytes = 0;
+        if (hipMalloc(reinterpret_cast<void**>(&w.ptr), bytes) != hipSuccess)
+            return set_error(GS_EHIP, "hipMalloc of the wave-per-system flags failed");
+        w.bytes = bytes;
     }
-    hipLaunchKernelGGL((fpu_gl3_kernel<MIXED, REFINE>), dim3(unsigned(L.nsys)), dim3(64), lds, st, L);
+    *out = w.ptr;
     return GS_OK;
 }
 
-int launch_fpu_gl3(const MidLaunch& L, hipStream_t st) {
-    if (L.nsys > 0x7fffffffLL) return set_error(GS_EINVAL, "a wave-per-system batch holds at most 2^31-1 systems");
+template <bool MIXED, bool REFINE> int launch_variant(const MidLaunch& L, hipStream_t st) {
+    hipLaunchKernelGGL((fpu_gl3_kernel<MIXED, REFINE, false>), dim3(unsigned(L.nsys)), dim3(64), 0, st, L);
+    hipLaunchKernelGGL((fpu_gl3_kernel<MIXED, REFINE, true>), dim3(unsigned(L.nsys)), dim3(64), 0, st, L);
+    return GS_OK;
+}
+
+}  // namespace
+
+// Free the flags of every device (gs_shutdown; no call may be in flight).
+void release_mid() {
+    std::lock_guard<std::mutex> lock(g_redo_mutex);
+    for (int d = 0; d < 64; ++d) {
+        if (!g_redo[d].ptr) continue;
+        if (hipSetDevice(d) == hipSuccess) (void)hipFree(g_redo[d].ptr);
+        g_redo[d] = RedoFlags{};
+    }
+}
+
+int launch_fpu_gl3(const MidLaunch& L0, hipStream_t st) {
+    if (L0.nsys > 0x7fffffffLL) return set_error(GS_EINVAL, "a wave-per-system batch holds at most 2^31-1 systems");
+    MidLaunch L = L0;
+    const int rc = redo_flags(size_t(L.nsys), &L.redo);
+    if (rc != GS_OK) return rc;
     if (L.mixed) return L.refine ? launch_variant<true, true>(L, st) : launch_variant<true, false>(L, st);
     return L.refine ? launch_variant<false, true>(L, st) : launch_variant<false, false>(L, st);
 }

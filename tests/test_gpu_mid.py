@@ -1,12 +1,12 @@
 """Parity of the wave-per-system solve (the FPU-β Gauss–Legendre s = 3 stage system, n = 48) with the
 CPU oracle, through the C ABI.
 
-In the working precision the device follows the oracle's operation order (the unblocked LU, the
-left folds, the Jacobian products over the same entries in the same order), and the results are
-compared bit for bit. The mixed method factors in FP32 with the trailing updates on MFMA, which
-round differently from the oracle's unblocked FP32 LU; after FP64 refinement the bar is the one
-stated for floating point: return codes, iterations and promotions identical, iterates within
-1e-10 relative, status norms within 1e-6 relative above the round-off floor.
+The device follows the oracle's operation order in both factor precisions (the unblocked LU in
+FP64 or FP32, the left folds, the Jacobian products over the same entries in the same order), and
+the results are compared bit for bit; the mixed method is also held to the floating-point bar
+(codes, iterations and promotions identical, iterates within 1e-10 relative). Rows that are not
+tame (a non-finite or −0 entry, or one beyond 2^(emax−48)) go to the second kernel, whose LU
+selects every entry per lane; the last tests force that path.
 """
 import numpy as np
 import pytest
@@ -100,6 +100,7 @@ def test_mixed_against_oracle(gs, orc, workloads, device, refine):
     opts = gs.Options(**TOL)
     z, st = _run(gs, device, method, QP, Z0, opts)
     zo, so = _oracle(gs, orc, method, QP, Z0, opts)
+    _assert_bitwise(z, st, zo, so)
     for f in ("code", "iterations", "promoted", "step_failures"):
         assert (st[f] == so[f]).all(), (f, np.nonzero(st[f] != so[f])[0][:8])
     assert np.abs(z - zo).max() <= 1e-10 * np.abs(zo).max()
@@ -129,3 +130,29 @@ def test_full_batch_and_residual_recheck(gs, orc, workloads, device):
     sample = np.random.default_rng(3).choice(65536, 1024, replace=False)
     zo, so = _oracle(gs, orc, gs.Newton(), QP[sample], Z0[sample], opts)
     _assert_bitwise(z64[sample], s64[sample], zo, so)
+
+
+@pytest.mark.parametrize("case", ["huge_regularization", "f32_negative_zero"])
+def test_exact_kernel_bitwise(gs, orc, workloads, device, case):
+    """Systems whose rows are not tame are solved by the second kernel: a regularisation of 1e300
+    puts entries beyond 2^975 on the diagonal (FP64 rows), and h = 1e-25 rounds the off-diagonal
+    entries of the FP32 rows to −0. Both are compared with the oracle bit for bit."""
+    QP = workloads.fpu_states(128, seed=6)
+    Z0 = np.random.default_rng(1).standard_normal((128, 48)) * 0.1
+    opts = gs.Options(max_iterations=5, **TOL)
+    h = H
+    if case == "huge_regularization":
+        method = gs.Newton(regularization=1e300)
+    else:
+        method = gs.Newton(linsolve=gs.LUFactorization(np.float32), refinement=gs.Refinement())
+        h = 1e-25
+    Z = torch.as_tensor(Z0, device=device).clone()
+    S = gs.status_buffer(len(Z0), device)
+    prob = gs.NonlinearProblem(gs.FpuGl3())
+    gs.solve_batch(Z, S, prob, method, gs.BatchParams(h, torch.as_tensor(QP, device=device)), opts)
+    torch.cuda.synchronize()
+    z, st = Z.cpu().numpy(), gs.statuses(S)
+    zo, so = orc.solve_batch(method, gs.FpuGl3(), np.array(Z0), gs.BatchParams(h, QP), opts)
+    _assert_bitwise(z, st, zo, so)
+    assert (st["iterations"] > 0).all()
+    print(case, np.bincount(st["code"], minlength=6).tolist(), np.bincount(st["iterations"]).tolist())
