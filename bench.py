@@ -313,16 +313,17 @@ def bench_fpu_gl3(args, gs, workloads, torch):
         "dtype": "f64",
         "data": "synthetic ((q, p) ~ N(0, 0.3^2), numpy PCG64 seed 0)",
         "config": {"workload": "fpu_beta_gl3 n=48, batch 65536, h=0.05, beta=1, abstol 1e-12, reltol 0",
-                   "method": "Newton(LUFactorization(Float32) blocked with MFMA trailing updates, Refinement)",
+                   "method": "Newton(LUFactorization(Float32), Refinement); the factor's rows in VGPRs",
                    "plain_value": out["plain"][0], "plain_ms_per_step": out["plain"][1],
                    "mean_iterations": out["mixed"][2], "mixed_kernel_ms": out["mixed"][3],
                    "plain_kernel_ms": out["plain"][3]},
         "roofline": {
             "bound": "mfma",
-            "kernel": "fpu_gl3_kernel<mixed, refine> (one wave per system; FP32 LU with MFMA trailing updates, "
-                      "FP64 refinement); achieved = "
+            "kernel": "fpu_gl3_kernel<mixed, refine> (one wave per system, each lane's factor row in VGPRs; "
+                      "FP32 LU, FP64 refinement; VALU-bound, no MFMA); achieved = "
                       f"{FPU_FLOPS_PER_ITER} flop per Newton iteration x the batch's iterations / kernel time; "
-                      "peak = the FP64 dense peak, the precision of the residual, refinement and update",
+                      "peak = the FP64 dense peak (equal to the FP64 vector rate), the precision of the residual, "
+                      "refinement and update",
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",

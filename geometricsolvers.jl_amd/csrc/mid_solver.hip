@@ -387,64 +387,70 @@ GS_DEV void grad_ratio(MidShared& s, const Fpu& f, double fnorm, double& gr, dou
 
 // ---- factorisations ------------------------------------------------------------------------------
 
-// The exponent of the largest |Jreg entry| (matrix_exponent, factors.jl:826-843).
-GS_DEV int jreg_exponent(const MidShared& s, const Fpu& f, double mu) {
-    const int l = lane_id();
-    double m = 0.0;
-    bool nan_seen = false;
-    if (l < kN) {
-        const int mm = l % kSites;
-        for (int b = 0; b < 3; ++b)
-            for (int dk = -1; dk <= 1; ++dk) {
-                const int k = mm + dk;
-                if (k < 0 || k >= kSites) continue;
-                const double v = __builtin_fabs(jreg(s, f, mu, l, b * kSites + k));
-                m = __builtin_fmax(m, v);
-                nan_seen |= isnan_(v);
-            }
-    }
-    const bool nan_any = __ballot(nan_seen) != 0;
-    m = wave_max(isnan_(m) ? 0.0 : m);
-    if (nan_any || !isfin(m) || m == 0.0) return 0;
-    const int e = exponent_of(m);
-    return e < -kEmax64 ? -kEmax64 : (e > kEmax64 ? kEmax64 : e);
-}
-
-// factor_matrix (factors.jl:802-811) as rows: lane l < 48 receives row l of Jreg/2^e in T, with e ≠ 0
-// applied in FP32 and only e < 0 in FP64; returns e. The entries are jreg's, formed from the six
-// stiffnesses and three coefficients of the lane's row. `tame` reports, uniformly, that every entry
-// is finite, no entry is −0 and every |entry| < 2^(emax−48): then no step of the LU meets a
-// non-finite value (growth ≤ 2^47 under partial pivoting, multipliers ≤ 1) nor makes a −0, and a
-// zero multiplier leaves a row exactly as it is.
+// factor_matrix (factors.jl:802-811) as rows: lane l < 48 receives row l of Jreg/2^e in T, with e
+// the exponent of the largest |Jreg entry| (matrix_exponent, factors.jl:826-843) applied when ≠ 0
+// in FP32 and only when < 0 in FP64; returns e. Row i = a·16 + m has its nonzeros at columns
+// b·16 + m + d (d = −1, 0, 1, within the chain): the lane forms those nine entries (and the
+// structural zero 0 − c_ab·0 of each block) once, then places them with three compares per column.
+// `tame` reports, uniformly, that every entry is finite, no entry is −0 and every |entry| <
+// 2^(emax−48): then no step of the LU meets a non-finite value (growth ≤ 2^47 under partial
+// pivoting, multipliers ≤ 1) nor makes a −0, and a zero multiplier leaves a row exactly as it is.
 template <class T> GS_DEV int factor_rows(const MidShared& s, const Fpu& f, double mu, T (&a)[kN], bool& tame) {
     constexpr bool kF32 = sizeof(T) == 4;
-    const int e0 = jreg_exponent(s, f, mu);
-    const int e = kF32 ? e0 : (e0 < 0 ? e0 : 0);
-    const int i = lane_id() < kN ? lane_id() : kN - 1;
+    const int l = lane_id();
+    const bool live = l < kN;
+    const int i = live ? l : kN - 1;
     const int ab = i / kSites, m = i % kSites;
-    double km[3], kp[3], cf[3];
+    double band[3][3], zero[3];
+    double mx = 0.0;
+    bool nan_seen = false;
 #pragma unroll
     for (int b = 0; b < 3; ++b) {
-        km[b] = s.K[b * 17 + m];
-        kp[b] = s.K[b * 17 + m + 1];
-        cf[b] = f.coef(ab, b);
+        const double km = s.K[b * 17 + m], kp = s.K[b * 17 + m + 1], cf = f.coef(ab, b);
+        zero[b] = 0.0 - cf * 0.0;
+        nan_seen |= isnan_(zero[b]);
+#pragma unroll
+        for (int d = -1; d <= 1; ++d) {
+            const double H = d == 0 ? -(km + kp) : (d < 0 ? km : kp);
+            const bool diag = d == 0 && b == ab;
+            double v = (diag ? 1.0 : 0.0) - cf * H;
+            if (diag && mu != 0.0) v = v + mu;
+            band[b][d + 1] = v;
+            const bool in_row = live && m + d >= 0 && m + d < kSites;
+            if (in_row) {
+                mx = __builtin_fmax(mx, __builtin_fabs(v));
+                nan_seen |= isnan_(v);
+            }
+        }
     }
+    const bool nan_any = __ballot(live && nan_seen) != 0;
+    const double wmx = wave_max(isnan_(mx) ? 0.0 : mx);
+    int e0 = 0;
+    if (!nan_any && isfin(wmx) && wmx != 0.0) {
+        const int ex = exponent_of(wmx);
+        e0 = ex < -kEmax64 ? -kEmax64 : (ex > kEmax64 ? kEmax64 : ex);
+    }
+    const int e = kF32 ? e0 : (e0 < 0 ? e0 : 0);
     const T big = ldexp2(T(1), Lim<T>::emax - kN);
     bool wild = false;
+    T tb[3][3], tz[3];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+        tz[b] = T(e != 0 ? ldexp2(zero[b], -e) : zero[b]);
+        wild |= !(tabs(tz[b]) < big) || (tz[b] == T(0) && __builtin_signbit(tz[b]));
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            tb[b][d] = T(e != 0 ? ldexp2(band[b][d], -e) : band[b][d]);
+            const bool in_row = m + d - 1 >= 0 && m + d - 1 < kSites;
+            wild |= in_row && (!(tabs(tb[b][d]) < big) || (tb[b][d] == T(0) && __builtin_signbit(tb[b][d])));
+        }
+    }
 #pragma unroll
     for (int j = 0; j < kN; ++j) {
         const int b = j / kSites, k = j % kSites;
-        double H = 0.0;
-        if (k == m) H = -(km[b] + kp[b]);
-        else if (k + 1 == m) H = km[b];
-        else if (k == m + 1) H = kp[b];
-        double v = (i == j ? 1.0 : 0.0) - cf[b] * H;
-        if (i == j && mu != 0.0) v = v + mu;
-        if (e != 0) v = ldexp2(v, -e);
-        a[j] = T(v);
-        wild |= !(tabs(a[j]) < big) || (a[j] == T(0) && __builtin_signbit(a[j]));
+        a[j] = k + 1 == m ? tb[b][0] : (k == m ? tb[b][1] : (k == m + 1 ? tb[b][2] : tz[b]));
     }
-    tame = __ballot(wild) == 0;
+    tame = __ballot(live && wild) == 0;
     return e;
 }
 
