@@ -62,6 +62,8 @@ struct GsMethod
     best::Int32
     reuse::Int32
     dl_radius::Float64
+    lm_scaling::Int32
+    pad::Int32
 end
 
 struct GsFamily
@@ -195,7 +197,7 @@ function device_method(m::Newton{<:LUFactorization{F}, G, Nothing, RF, Float64, 
         m.refinement isa Refinement ? Int32(1) : Int32(0), g isa Static ? Float64(g.α) : 1.0,
         Float64(bt.c₁), Float64(bt.p), Int32(bt.maxiter), Int32(rf.maxiter), Float64(rf.η),
         Float64(m.regularization), ntuple(_ -> 0.0, 6), m.best === Val(true) ? Int32(1) : Int32(0),
-        m.reuse === nothing ? Int32(0) : m.reuse, radius)
+        m.reuse === nothing ? Int32(0) : m.reuse, radius, Int32(0), Int32(0))
 end
 
 function device_options(o::Options{Float64})

@@ -230,8 +230,10 @@ class DogLeg:
 
 @dataclass(frozen=True)
 class LevenbergMarquardt:
-    """LevenbergMarquardt(; μ₀, μ_min, μ_max, μ_decrease, μ_increase, ρ_accept)
-    (src/globalization/levenbergmarquardt.jl:48-98), `scaling = :none`, the stacked Givens step."""
+    """LevenbergMarquardt(; μ₀, μ_min, μ_max, μ_decrease, μ_increase, ρ_accept, scaling)
+    (src/globalization/levenbergmarquardt.jl:48-98) on the stacked Givens step; `scaling` is
+    "none" (D = I) or "diagonal" (Marquardt's column norms of J, floored at √eps of the largest,
+    levenbergmarquardt.jl:318-347)."""
 
     mu0: float = 1e-6
     mu_min: float = max(1e-15, 4 * _EPS)
@@ -239,6 +241,7 @@ class LevenbergMarquardt:
     mu_decrease: float = 1.0 / 3.0
     mu_increase: float = 4.0
     rho_accept: float = 0.0
+    scaling: str = "none"
 
     def __post_init__(self):
         if not (all(math.isfinite(v) for v in (self.mu0, self.mu_min, self.mu_max))
@@ -252,6 +255,9 @@ class LevenbergMarquardt:
             raise ArgumentError(f"LevenbergMarquardt requires μ_increase > 1; got μ_increase = {self.mu_increase}")
         if not (math.isfinite(self.rho_accept) and 0 <= self.rho_accept < 1):
             raise ArgumentError(f"LevenbergMarquardt requires 0 ≤ ρ_accept < 1; got ρ_accept = {self.rho_accept}")
+        if self.scaling not in ("none", "diagonal"):
+            raise ArgumentError(
+                f"LevenbergMarquardt requires scaling ∈ (:none, :diagonal); got scaling = :{self.scaling}")
 
 
 @dataclass(frozen=True)
@@ -335,6 +341,7 @@ class Newton:
         lm = g if isinstance(g, LevenbergMarquardt) else LevenbergMarquardt()
         m.lm_mu0, m.lm_mu_min, m.lm_mu_max = lm.mu0, lm.mu_min, lm.mu_max
         m.lm_mu_decrease, m.lm_mu_increase, m.lm_rho_accept = lm.mu_decrease, lm.mu_increase, lm.rho_accept
+        m.lm_scaling = _abi.GS_LM_SCALING_DIAGONAL if lm.scaling == "diagonal" else _abi.GS_LM_SCALING_NONE
         m.static_alpha = g.alpha if isinstance(g, Static) else 1.0
         bt = g if isinstance(g, Backtracking) else Backtracking()
         m.bt_c1, m.bt_p, m.bt_maxiter = bt.c1, bt.p, bt.maxiter

@@ -23,6 +23,7 @@ GS_STATIC, GS_BACKTRACKING, GS_LEVENBERG_MARQUARDT, GS_DOGLEG = 0, 1, 2, 3
 GS_FACTOR_WORKING, GS_FACTOR_F32 = 0, 1
 GS_NO_REFINEMENT, GS_REFINEMENT = 0, 1
 GS_NORM_L2, GS_NORM_MAX = 0, 1
+GS_LM_SCALING_NONE, GS_LM_SCALING_DIAGONAL = 0, 1
 GS_LU = 0
 # gs_family_id
 GS_FAM_COUPLED2 = 1
@@ -99,6 +100,8 @@ class GsMethod(ctypes.Structure):
         ("best", ctypes.c_int32),
         ("reuse", ctypes.c_int32),
         ("dl_radius", ctypes.c_double),
+        ("lm_scaling", ctypes.c_int32),
+        ("pad_", ctypes.c_int32),
     ]
 
 

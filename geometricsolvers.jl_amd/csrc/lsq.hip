@@ -255,13 +255,53 @@ template <int N> GS_DEV double linear_norm(const ExpSumRow<N>& row, const double
     return ldexp2(__builtin_sqrt(t), e);
 }
 
-// The LevenbergMarquardt newton_step!! (newton.jl:1897-1971) of one fit.
-template <int N> GS_DEV StepOut lm_step(Fit<N>& f, const LsqParams& q) {
+// lm_damping_diagonal (levenbergmarquardt.jl:318-341) from the column sums of squares col2 (each a
+// left fold over the rows, as rnorm's): Marquardt's column norms floored at √eps of the largest, or
+// at one where every column is zero. A column sum that overflows or underflows takes rnorm's scaled
+// second pass over that column.
+template <int N>
+GS_DEV void damping_diagonal(const ExpSumRow<N>& row, const double (&x)[N], const double (&col2)[N], double (&dd)[N]) {
+    double dmax = 0.0;
+    static_for<0, N>([&](auto kk) {
+        constexpr int k = kk;
+        const double s = col2[k];
+        double d;
+        if (__builtin_expect(isfin(s) && s >= double(row.m) * kDblMin, 1)) {
+            d = __builtin_sqrt(s);
+        } else {
+            double mx = 0.0;
+            for (int i = 0; i < row.m; ++i) {
+                const double v = __builtin_fabs(row.J(x, i, k));
+                mx = i == 0 ? v : jmax(mx, v);
+            }
+            if (mx == 0.0 || !isfin(mx)) {
+                d = mx;
+            } else {
+                const int e = exponent_of(mx);
+                double t = 0.0;
+                for (int i = 0; i < row.m; ++i) {
+                    const double v = ldexp2(row.J(x, i, k), -e);
+                    t = i == 0 ? v * v : t + v * v;
+                }
+                d = ldexp2(__builtin_sqrt(t), e);
+            }
+        }
+        dd[k] = d;
+        dmax = jmax(dmax, d);
+    });
+    const double floor = dmax == 0.0 ? 1.0 : 1.4901161193847656e-8 * dmax;  // √eps(Float64) = 2^-26
+    static_for<0, N>([&](auto kk) { dd[int(kk)] = jmax(dd[int(kk)], floor); });
+}
+
+// The LevenbergMarquardt newton_step!! (newton.jl:1897-1971) of one fit; SCALED: scaling = :diagonal.
+template <int N, bool SCALED> GS_DEV StepOut lm_step(Fit<N>& f, const LsqParams& q) {
     const double fnorm = f.fnorm;
     if (!isfin(f.mu)) return StepOut{kLsqNonfinite, fnorm, 0.0, 0, false};
     if (fnorm == 0.0) return StepOut{kLsqSuccess, 0.0, 0.0, 0, false};
-    // the reduction of [J | −F] (lm_qr_factor!, levenbergmarquardt.jl:359-421) into the lane's slot
+    // the reduction of [J | −F] (lm_qr_factor!, levenbergmarquardt.jl:359-421) into the lane's slot,
+    // and with SCALED the column sums of J∘J in the rows' order
     bool jfinite = true;
+    double col2[N];
     {
         double R[N][N], c[N];
         static_for<0, N>([&](auto ii) {
@@ -272,6 +312,7 @@ template <int N> GS_DEV StepOut lm_step(Fit<N>& f, const LsqParams& q) {
                 constexpr int k = kk;
                 jfinite &= isfin(Ji[k]);
                 R[i][k] = Ji[k];
+                if (SCALED) col2[k] = i == 0 ? Ji[k] * Ji[k] : col2[k] + Ji[k] * Ji[k];
             });
             c[i] = -Fi;
         });
@@ -301,7 +342,11 @@ template <int N> GS_DEV StepOut lm_step(Fit<N>& f, const LsqParams& q) {
         for (int i = N; i < f.row.m; ++i) {
             double w[N];
             const double Fi = f.row.FJ(f.x, i, w);
-            static_for<0, N>([&](auto kk) { jfinite &= isfin(w[int(kk)]); });
+            static_for<0, N>([&](auto kk) {
+                constexpr int k = kk;
+                jfinite &= isfin(w[k]);
+                if (SCALED) col2[k] = col2[k] + w[k] * w[k];
+            });
             double wr = -Fi;
             rotate_in<0>(R, c, w, wr);
         }
@@ -315,13 +360,17 @@ template <int N> GS_DEV StepOut lm_step(Fit<N>& f, const LsqParams& q) {
         diag_pass<N>(f);
         return StepOut{kLsqNonfinite, fnorm, 0.0, 0, false};
     }
+    double dd[N];
+    if (SCALED) damping_diagonal<N>(f.row, f.x, col2, dd);
     int32_t failures = 0;
     double mu = f.mu;
     bool diag_done = false;
     while (true) {
-        // lm_trial (:575-584): the damping rows √μ e_jᵀ (D = I) eliminated against a copy of (R, c)
+        // lm_trial (:575-584): the damping rows √μ d_j e_jᵀ (d_j = 1 without SCALED) eliminated
+        // against a copy of (R, c); a damping entry that is not finite fails the trial (:425-431)
         const double smu = __builtin_sqrt(mu < 4.0 * kDblEps ? 0.0 : mu);
         bool ok = isfin(smu);
+        if (SCALED) static_for<0, N>([&](auto kk) { ok &= isfin(smu * dd[int(kk)]); });
         double d[N];
         static_for<0, N>([&](auto kk) { d[int(kk)] = 0.0; });
         if (ok) {
@@ -339,7 +388,7 @@ template <int N> GS_DEV StepOut lm_step(Fit<N>& f, const LsqParams& q) {
                 constexpr int j = jj;
                 double w[N];
                 static_for<0, N>([&](auto kk) { w[int(kk)] = 0.0; });
-                w[j] = smu * 1.0;
+                w[j] = SCALED ? smu * dd[j] : smu * 1.0;
                 double wr = 0.0;
                 rotate_in<j>(Rt, ct, w, wr);
             });
@@ -478,13 +527,13 @@ GS_DEV bool stationary_dg(const Fit<N>& f, const LsqParams& q, int32_t it, doubl
 
 // One pass of the solve!! loop on a LeastSquaresProblem (solver.jl:301-377): a step, its record, and
 // every exit. Returns true when the fit's solve has ended.
-template <int N> GS_DEV bool fit_iterate(Fit<N>& f, const LsqParams& q) {
+template <int N, bool SCALED> GS_DEV bool fit_iterate(Fit<N>& f, const LsqParams& q) {
     static_for<0, N>([&](auto kk) { f.slot[Slot<N>::kX + int(kk)] = f.x[int(kk)]; });
     f.it_k = f.it;
     f.fnorm_k = f.fnorm;
     f.stepnorm_k = f.stepnorm;
     f.rate_k = f.rate;
-    StepOut s = lm_step<N>(f, q);
+    StepOut s = lm_step<N, SCALED>(f, q);
     // record (status.jl:245-262)
     f.rate = f.stepnorm > 0.0 ? s.stepnorm / f.stepnorm : dnan();
     f.code = s.code;
@@ -625,7 +674,7 @@ template <int N> GS_DEV void fit_store(Fit<N>& f, const LsqArgs& a) {
 // The persistent kernel of one bucket (fits with n = N). Every lane loops over LM outer steps; a lane
 // without a fit takes the next one from the bucket's queue (one atomic per wave per refill), and the
 // wave leaves the loop once the queue is empty and every lane is idle.
-template <int N> __global__ __launch_bounds__(256) void lsq_kernel(const LsqArgs a) {
+template <int N, bool SCALED> __global__ __launch_bounds__(256) void lsq_kernel(const LsqArgs a) {
     // the bucket's range comes from the device-side sort, so the launch does not depend on the data;
     // a batch with an invalid fit solves nothing (the host reports the error after the launch)
     if (*a.bad != 0) return;
@@ -663,7 +712,7 @@ template <int N> __global__ __launch_bounds__(256) void lsq_kernel(const LsqArgs
             }
         }
         if (!__any(active)) break;
-        if (active && fit_iterate<N>(f, a.q)) {
+        if (active && fit_iterate<N, SCALED>(f, a.q)) {
             fit_store<N>(f, a);
             active = false;
         }
@@ -789,24 +838,27 @@ int lsq_workspace(int which, size_t bytes, void** out) {
     return GS_OK;
 }
 
-template <int N> const void* bucket_kernel() { return reinterpret_cast<const void*>(&lsq_kernel<N>); }
-const void* bucket_kernel_of(int b) {
+template <int N, bool SCALED> const void* bucket_kernel() {
+    return reinterpret_cast<const void*>(&lsq_kernel<N, SCALED>);
+}
+template <bool SCALED> const void* bucket_kernel_of(int b) {
     switch (b) {
-        case 0: return bucket_kernel<2>();
-        case 1: return bucket_kernel<4>();
-        case 2: return bucket_kernel<6>();
-        default: return bucket_kernel<8>();
+        case 0: return bucket_kernel<2, SCALED>();
+        case 1: return bucket_kernel<4, SCALED>();
+        case 2: return bucket_kernel<6, SCALED>();
+        default: return bucket_kernel<8, SCALED>();
     }
 }
+const void* bucket_kernel_of(int b, bool scaled) { return scaled ? bucket_kernel_of<true>(b) : bucket_kernel_of<false>(b); }
 
 // Workgroups of bucket b under its shape for a batch of nfit fits: the resident workgroups the shape
 // asks for (the occupancy's when blocks_per_cu is 0), and no more lanes than fits.
-int lsq_bucket_blocks(int b, int64_t nfit, int64_t* out) {
+int lsq_bucket_blocks(int b, int64_t nfit, bool scaled, int64_t* out) {
     const LsqShape& sh = g_lsq_shape[b];
     int dev = 0, cus = 256, per_cu = sh.blocks_per_cu;
     if (hipGetDevice(&dev) != hipSuccess) return set_error(GS_EHIP, "hipGetDevice failed");
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (per_cu == 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bucket_kernel_of(b), sh.threads, 0) !=
+    if (per_cu == 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bucket_kernel_of(b, scaled), sh.threads, 0) !=
                             hipSuccess || per_cu < 1))
         per_cu = 1;
     int64_t blocks = int64_t(per_cu) * cus;
@@ -849,11 +901,20 @@ int lsq_enqueue(hipStream_t st, const LsqStreams& L, const int32_t* K, const int
         hipStream_t sb = L.s[b];
         if (hipStreamWaitEvent(sb, L.start, 0) != hipSuccess) return set_error(GS_EHIP, "hipStreamWaitEvent failed");
         const dim3 grid{unsigned(blocks[b]), 1, 1}, block{unsigned(g_lsq_shape[b].threads), 1, 1};
-        switch (b) {
-            case 0: hipLaunchKernelGGL(lsq_kernel<2>, grid, block, 0, sb, a); break;
-            case 1: hipLaunchKernelGGL(lsq_kernel<4>, grid, block, 0, sb, a); break;
-            case 2: hipLaunchKernelGGL(lsq_kernel<6>, grid, block, 0, sb, a); break;
-            case 3: hipLaunchKernelGGL(lsq_kernel<8>, grid, block, 0, sb, a); break;
+        if (a.q.scaled) {
+            switch (b) {
+                case 0: hipLaunchKernelGGL((lsq_kernel<2, true>), grid, block, 0, sb, a); break;
+                case 1: hipLaunchKernelGGL((lsq_kernel<4, true>), grid, block, 0, sb, a); break;
+                case 2: hipLaunchKernelGGL((lsq_kernel<6, true>), grid, block, 0, sb, a); break;
+                case 3: hipLaunchKernelGGL((lsq_kernel<8, true>), grid, block, 0, sb, a); break;
+            }
+        } else {
+            switch (b) {
+                case 0: hipLaunchKernelGGL((lsq_kernel<2, false>), grid, block, 0, sb, a); break;
+                case 1: hipLaunchKernelGGL((lsq_kernel<4, false>), grid, block, 0, sb, a); break;
+                case 2: hipLaunchKernelGGL((lsq_kernel<6, false>), grid, block, 0, sb, a); break;
+                case 3: hipLaunchKernelGGL((lsq_kernel<8, false>), grid, block, 0, sb, a); break;
+            }
         }
         if (hipEventRecord(L.done[b], sb) != hipSuccess || hipStreamWaitEvent(st, L.done[b], 0) != hipSuccess)
             return set_error(GS_EHIP, "joining the bucket streams failed");
@@ -917,6 +978,8 @@ int gs_solve_expsum_lsq_f64(const gs_method* method, int64_t nfit, double* X, co
             return set_error(GS_EINVAL, "LevenbergMarquardt requires μ_increase > 1");
         if (!(std::isfinite(method->lm_rho_accept) && 0.0 <= method->lm_rho_accept && method->lm_rho_accept < 1.0))
             return set_error(GS_EINVAL, "LevenbergMarquardt requires 0 ≤ ρ_accept < 1");
+        if (method->lm_scaling != GS_LM_SCALING_NONE && method->lm_scaling != GS_LM_SCALING_DIAGONAL)
+            return set_error(GS_EINVAL, "LevenbergMarquardt requires scaling ∈ (:none, :diagonal)");
         if (options->max_iterations < 0 || options->min_iterations < 0 || options->max_stalls < 1)
             return set_error(GS_EINVAL, "max_iterations and min_iterations must be ≥ 0 and max_stalls ≥ 1");
     }
@@ -938,7 +1001,7 @@ int gs_solve_expsum_lsq_f64(const gs_method* method, int64_t nfit, double* X, co
     size_t slot_off[4], slot_total = 0;
     for (int b = 0; b < 4; ++b) {
         const int n = 2 * (b + 1);
-        if ((rc = lsq_bucket_blocks(b, nfit, &blocks[b]))) return rc;
+        if ((rc = lsq_bucket_blocks(b, nfit, method->lm_scaling == GS_LM_SCALING_DIAGONAL, &blocks[b]))) return rc;
         slot_off[b] = slot_total;
         slot_total += size_t(blocks[b] * g_lsq_shape[b].threads) * size_t(n * (n + 1) / 2 + 2 * n);
     }
@@ -967,6 +1030,7 @@ int gs_solve_expsum_lsq_f64(const gs_method* method, int64_t nfit, double* X, co
     q.mu_dec = method->lm_mu_decrease;
     q.mu_inc = method->lm_mu_increase;
     q.rho_accept = method->lm_rho_accept;
+    q.scaled = method->lm_scaling == GS_LM_SCALING_DIAGONAL;
     a.X = X;
     a.m = m;
     a.y_offset = y_offset;
@@ -1040,11 +1104,12 @@ int gs_lsq_set_shape(int k, int threads, int blocks_per_cu) {
     return GS_OK;
 }
 
-// The shape of bucket k and, for a batch of nfit fits, its workgroups (out[0..2]).
+// The shape of bucket k and, for a batch of nfit fits, its workgroups (out[0..2]; the occupancy of
+// the scaling = :none kernels where the shape leaves it to the occupancy).
 int gs_lsq_get_shape(int k, int64_t nfit, int64_t out[3]) {
     if (k < 1 || k > 4 || !out) return set_error(GS_EINVAL, "k must be in 1:4 and out not NULL");
     int64_t blocks = 0;
-    const int rc = lsq_bucket_blocks(k - 1, nfit < 1 ? 1 : nfit, &blocks);
+    const int rc = lsq_bucket_blocks(k - 1, nfit < 1 ? 1 : nfit, false, &blocks);
     if (rc) return rc;
     out[0] = g_lsq_shape[k - 1].threads;
     out[1] = g_lsq_shape[k - 1].blocks_per_cu;

@@ -26,6 +26,8 @@ struct LsqParams {
     int32_t min_iterations, max_iterations, max_stalls, norm_max, roundoff, rate_test;
     // LevenbergMarquardt
     double mu0, mu_min_eff, mu_max, mu_dec, mu_inc, rho_accept;
+    int32_t scaled;  // scaling = :diagonal (the kernels of lsq_kernel<N, true>)
+    int32_t pad_;
 };
 
 // LinearAlgebra's givensAlgorithm (the LAPACK DLARTG scheme): r = √(f² + g²), c = f/r, s = g/r,

@@ -150,7 +150,14 @@ typedef struct gs_method {
     int32_t best;              /* best = true                                            */
     int32_t reuse;             /* reuse = k ≥ 1; 0 = nothing (factorise at every step)   */
     double dl_radius;          /* DogLeg(; radius) in [1e-6, 1e10]; 0 = nothing          */
+    int32_t lm_scaling;        /* LevenbergMarquardt(; scaling): gs_lm_scaling           */
+    int32_t pad_;
 } gs_method;
+
+/* LevenbergMarquardt(; scaling) (src/globalization/levenbergmarquardt.jl:318-347): the damping
+ * diagonal D of [J; √μ D] δ = [−F; 0] is I, or Marquardt's column norms of J floored at
+ * √eps · their largest. */
+enum gs_lm_scaling { GS_LM_SCALING_NONE = 0, GS_LM_SCALING_DIAGONAL = 1 };
 
 /* The registered problem families. Each has its residual F(x, p) and its analytic Jacobian J(x, p)
  * as HIP device functions; the Julia extension carries the same F and J as Julia functions, which

@@ -53,6 +53,7 @@ template <class R> static Method<R> method_from(const gs_method& g) {
     m.lm_mu_decrease = R(g.lm_mu_decrease);
     m.lm_mu_increase = R(g.lm_mu_increase);
     m.lm_rho_accept = R(g.lm_rho_accept);
+    m.lm_diagonal = g.lm_scaling == GS_LM_SCALING_DIAGONAL;
     return m;
 }
 

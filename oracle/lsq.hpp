@@ -203,19 +203,41 @@ template <int N, class P> struct LsqSolver {
         }
     }
 
-    // lm_trial (levenbergmarquardt.jl:575-584) → lm_qr_solve! (:438-482), D = I (scaling = :none).
-    bool trial(const Mat<double, N>& R0, const Vec<double, N>& c0, double mu, Vec<double, N>& d) const {
+    // lm_damping_diagonal! (levenbergmarquardt.jl:318-341) of the m×N Jacobian J: with scaling =
+    // :diagonal the column norms (rnorm over the m rows) floored at √eps(R) of the largest (at one
+    // where every column is zero); with :none, ones.
+    Vec<double, N> damping_diagonal(const double* J) const {
+        Vec<double, N> dd;
+        if (!mt.lm_diagonal) {
+            dd.fill(1.0);
+            return dd;
+        }
+        double dmax = 0.0;
+        for (int j = 0; j < N; ++j) {
+            dd[j] = rnorm_m(J + size_t(j) * m);
+            dmax = jmax(dmax, dd[j]);
+        }
+        const double floor = dmax == 0.0 ? 1.0 : std::sqrt(eps_<double>()) * dmax;
+        for (int j = 0; j < N; ++j) dd[j] = jmax(dd[j], floor);
+        return dd;
+    }
+
+    // lm_trial (levenbergmarquardt.jl:575-584) → lm_qr_solve! (:438-482) with the damping rows
+    // √μ d_j e_jᵀ.
+    bool trial(const Mat<double, N>& R0, const Vec<double, N>& c0, double mu, const Vec<double, N>& dd,
+               Vec<double, N>& d) const {
         double smu = std::sqrt(mu < 4 * eps_<double>() ? 0.0 : mu);  // lm_step_damping (:162-164)
         d.fill(0.0);
-        if (!std::isfinite(smu)) return false;  // lm_damping_ok (:425-431) with d_j = 1
-        if (!std::isfinite(smu * 1.0)) return false;
+        if (!std::isfinite(smu)) return false;  // lm_damping_ok (:425-431)
+        for (int j = 0; j < N; ++j)
+            if (!std::isfinite(smu * dd[j])) return false;
         Mat<double, N> Rm = R0;
         Vec<double, N> c = c0;
         auto R = [&](int i, int j) -> double& { return Rm[i + j * N]; };
         for (int j = 0; j < N; ++j) {
             double w[N];
             for (int k = 0; k < N; ++k) w[k] = 0.0;
-            w[j] = smu * 1.0;
+            w[j] = smu * dd[j];
             double wr = 0.0;
             for (int k = j; k < N; ++k) {
                 double b = w[k];
@@ -277,10 +299,11 @@ template <int N, class P> struct LsqSolver {
         Mat<double, N> R0;
         Vec<double, N> c0;
         qr_factor(Jb.data(), Fx.data(), R0, c0);
+        const Vec<double, N> dd = damping_diagonal(Jb.data());
         int32_t failures = 0;
         while (true) {
             Vec<double, N> d;
-            bool ok = trial(R0, c0, mu, d);
+            bool ok = trial(R0, c0, mu, dd, d);
             if (ok) {
                 // Jδ as StaticArrays' muladd fold over the N columns of each row
                 for (int i = 0; i < m; ++i) {

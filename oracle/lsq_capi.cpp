@@ -151,6 +151,7 @@ Method<double> lm_method_from(const gs_method& g) {
     m.lm_mu_decrease = g.lm_mu_decrease;
     m.lm_mu_increase = g.lm_mu_increase;
     m.lm_rho_accept = g.lm_rho_accept;
+    m.lm_diagonal = g.lm_scaling == GS_LM_SCALING_DIAGONAL;
     return m;
 }
 
@@ -183,6 +184,29 @@ void run_expsum(const Method<double>& m, const Options<double>& o, double* x8, i
     for (int k = 0; k < N; ++k) x8[k] = x[k];
     write_status_lsq(S, st);
 }
+
+// F(x) = A x − b (A m×N column-major), with its constant Jacobian: the linear problems of the
+// reference's LevenbergMarquardt closed-form tests (test/globalization/levenbergmarquardt.jl:
+// 280-330).
+template <int N> struct Linear {
+    int mm;
+    const double* A;
+    const double* b;
+    long nf = 0;
+    int m() const { return mm; }
+    void F(const double* x, double* r) {
+        ++nf;
+        for (int i = 0; i < mm; ++i) {
+            double s = A[i] * x[0];
+            for (int k = 1; k < N; ++k) s = std::fma(A[i + size_t(k) * mm], x[k], s);
+            r[i] = s - b[i];
+        }
+    }
+    void J(const double* x, double* Jc) const {
+        (void)x;
+        for (size_t i = 0; i < size_t(mm) * N; ++i) Jc[i] = A[i];
+    }
+};
 
 }  // namespace orc
 
@@ -236,6 +260,57 @@ int orc_solve_mgh(int id, const gs_method* method, const gs_options_f64* options
         case 12: return run(std::integral_constant<int, 5>{});
         case 8: return run(std::integral_constant<int, 3>{});
         case 6: return run(std::integral_constant<int, 2>{});
+    }
+    return GS_EINVAL;
+}
+
+// The linear problem A x = b (n ≤ 4 unknowns) solved from x by Newton + LevenbergMarquardt.
+int orc_solve_linear_lsq(int n, int mm, const double* A, const double* b, const gs_method* method,
+                         const gs_options_f64* options, double* x, gs_status_f64* S) {
+    Method<double> mt = lm_method_from(*method);
+    Options<double> o = options_from_lsq<double>(*options);
+    auto run = [&](auto tag) {
+        constexpr int N = decltype(tag)::value;
+        Linear<N> prob{mm, A, b};
+        LsqSolver<N, Linear<N>> s(prob, mt, o);
+        Vec<double, N> v;
+        for (int k = 0; k < N; ++k) v[k] = x[k];
+        Status<double> st = s.solve(v);
+        for (int k = 0; k < N; ++k) x[k] = v[k];
+        write_status_lsq(S, st);
+        return 0;
+    };
+    if (mm < n) return GS_EINVAL;
+    switch (n) {
+        case 1: return run(std::integral_constant<int, 1>{});
+        case 2: return run(std::integral_constant<int, 2>{});
+        case 3: return run(std::integral_constant<int, 3>{});
+        case 4: return run(std::integral_constant<int, 4>{});
+    }
+    return GS_EINVAL;
+}
+
+// lm_damping_diagonal (levenbergmarquardt.jl:318-347) of an m×n Jacobian (n ≤ 4, column-major).
+int orc_lm_damping_diagonal(int n, int mm, const double* J, int diagonal, double* out) {
+    gs_method g;
+    std::memset(&g, 0, sizeof(g));
+    g.lm_mu0 = 1e-6;
+    g.lm_scaling = diagonal ? GS_LM_SCALING_DIAGONAL : GS_LM_SCALING_NONE;
+    Method<double> mt = lm_method_from(g);
+    Options<double> o;
+    auto run = [&](auto tag) {
+        constexpr int N = decltype(tag)::value;
+        Linear<N> prob{mm, J, J};
+        LsqSolver<N, Linear<N>> s(prob, mt, o);
+        Vec<double, N> dd = s.damping_diagonal(J);
+        for (int k = 0; k < N; ++k) out[k] = dd[k];
+        return 0;
+    };
+    switch (n) {
+        case 1: return run(std::integral_constant<int, 1>{});
+        case 2: return run(std::integral_constant<int, 2>{});
+        case 3: return run(std::integral_constant<int, 3>{});
+        case 4: return run(std::integral_constant<int, 4>{});
     }
     return GS_EINVAL;
 }

@@ -473,6 +473,7 @@ template <class R> struct Method {
     R lm_mu_decrease = R(1) / R(3);
     R lm_mu_increase = R(4);
     R lm_rho_accept = 0;
+    bool lm_diagonal = false;  // LevenbergMarquardt(; scaling = :diagonal)
 };
 
 // A problem restated for the oracle: F(x) and J(x) of one system, with its parameters bound, and a

@@ -55,6 +55,10 @@ def lib():
         L.orc_solve_expsum_batch.argtypes = [vp, vp, ctypes.c_int64, vp, vp, vp, vp, vp, vp, ctypes.c_int]
         L.orc_solve_mgh.restype = ctypes.c_int
         L.orc_solve_mgh.argtypes = [ctypes.c_int, vp, vp, vp, vp, vp]
+        L.orc_solve_linear_lsq.restype = ctypes.c_int
+        L.orc_solve_linear_lsq.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, vp, vp]
+        L.orc_lm_damping_diagonal.restype = ctypes.c_int
+        L.orc_lm_damping_diagonal.argtypes = [ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, vp]
         L.orc_fpu_gl3_eval.restype = None
         L.orc_fpu_gl3_eval.argtypes = [ctypes.c_double, ctypes.c_double, vp, vp, vp, vp]
         L.orc_exp_family.restype = None
@@ -233,6 +237,30 @@ def solve_mgh(problem: int, method, options, x0):
     cm, co = method.to_c(), options.to_c()
     assert lib().orc_solve_mgh(problem, ctypes.byref(cm), ctypes.byref(co), _ptr(x), _ptr(S), ctypes.byref(nf)) == 0
     return x, S[0], nf.value
+
+
+def solve_linear_lsq(A, b, method, options, x0):
+    """Newton + LevenbergMarquardt on F(x) = A x − b (A m × n, n ≤ 4) from x0: (x, status)."""
+    A = np.asarray(A, dtype=np.float64)
+    m, n = A.shape
+    Ac = np.ascontiguousarray(A.T).reshape(-1)  # column-major
+    bb = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.array(x0, dtype=np.float64)
+    S = np.zeros(1, dtype=_status_dtype())
+    cm, co = method.to_c(), options.to_c()
+    assert lib().orc_solve_linear_lsq(n, m, _ptr(Ac), _ptr(bb), ctypes.byref(cm), ctypes.byref(co), _ptr(x),
+                                      _ptr(S)) == 0
+    return x, S[0]
+
+
+def lm_damping_diagonal(J, diagonal: bool = True):
+    """lm_damping_diagonal (levenbergmarquardt.jl:318-347) of an m × n matrix (n ≤ 4)."""
+    J = np.asarray(J, dtype=np.float64)
+    m, n = J.shape
+    Jc = np.ascontiguousarray(J.T).reshape(-1)
+    out = np.empty(n)
+    assert lib().orc_lm_damping_diagonal(n, m, _ptr(Jc), 1 if diagonal else 0, _ptr(out)) == 0
+    return out
 
 
 def fpu_gl3_eval(h: float, beta: float, qp, z):

@@ -77,6 +77,8 @@ def test_expsum_bitwise_default(gs, orc, workloads, device):
     dict(opts=dict(x_reltol=1e-6, kappa=0.5, max_stalls=1, roundoff=True)),
     dict(lm=dict(mu0=1e-2, rho_accept=0.25, mu_decrease=0.5, mu_increase=10.0, mu_max=1e6)),
     dict(lm=dict(mu0=1e-12, mu_min=1e-14, mu_max=1e-3)),
+    dict(lm=dict(scaling="diagonal")),
+    dict(lm=dict(scaling="diagonal", mu0=1e-2, mu_increase=10.0)),
 ])
 def test_expsum_bitwise_options(gs, orc, workloads, device, case):
     K, m, off, Y, X0, _ = workloads.expsum_fits(1024, seed=11)
@@ -102,9 +104,11 @@ def test_expsum_noise_free_and_truth_start(gs, orc, workloads, device):
         _assert_bitwise(x, st, xo, so)
 
 
-def test_expsum_hostile_starts(gs, orc, workloads, device):
+@pytest.mark.parametrize("scaling", ["none", "diagonal"])
+def test_expsum_hostile_starts(gs, orc, workloads, device, scaling):
     """Starts whose exponentials overflow (λ ≪ 0), NaN and Inf entries, zero amplitudes (zero
-    columns of J), huge and tiny scales, and m = 2K (square fits)."""
+    columns of J), huge and tiny scales, and m = 2K (square fits); with scaling = :diagonal the zero
+    columns take the floor and the tiny ones the scaled column norm."""
     K, m, off, Y, X0, _ = workloads.expsum_fits(512, seed=7)
     X = X0.copy()
     rng = np.random.default_rng(1)
@@ -127,7 +131,7 @@ def test_expsum_hostile_starts(gs, orc, workloads, device):
     sq = rng.random(len(X)) < 0.2
     m2[sq] = 2 * K[sq]
     opts = gs.Options()
-    method = _lm(gs)
+    method = _lm(gs, scaling=scaling)
     x, st = _solve_gpu(gs, device, K, m2, off, Y, X, method, opts)
     xo, so = orc.solve_expsum_batch(method, opts, X, K, m2, off, Y)
     _assert_bitwise(x, st, xo, so)

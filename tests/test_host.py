@@ -175,6 +175,8 @@ def test_least_squares_batch_validation(gs, workloads):
         gs.LeastSquaresProblem(gs.HenonHeilesMidpoint())
     with pytest.raises(gs.ArgumentError, match="μ_increase > 1"):
         gs.LevenbergMarquardt(mu_increase=1.0)
+    with pytest.raises(gs.ArgumentError, match=r"scaling ∈ \(:none, :diagonal\); got scaling = :bad"):
+        gs.LevenbergMarquardt(scaling="bad")
 
 
 def test_least_squares_entry_point_checks_method_without_gpu(gs):
@@ -190,6 +192,12 @@ def test_least_squares_entry_point_checks_method_without_gpu(gs):
     o = gs.Options().to_c()
     rc = lib.gs_solve_expsum_lsq_f64(ctypes.byref(bad), 1, None, None, None, None, None, 0, None, ctypes.byref(o), None)
     assert rc == abi.GS_EINVAL and "μ_increase" in abi.last_error(lib)
+    assert m.lm_scaling == abi.GS_LM_SCALING_NONE
+    assert gs.Newton(globalization=gs.LevenbergMarquardt(scaling="diagonal")).to_c().lm_scaling == abi.GS_LM_SCALING_DIAGONAL
+    bad = abi.GsMethod.from_buffer_copy(m)
+    bad.lm_scaling = 2
+    rc = lib.gs_solve_expsum_lsq_f64(ctypes.byref(bad), 1, None, None, None, None, None, 0, None, ctypes.byref(o), None)
+    assert rc == abi.GS_EINVAL and "scaling" in abi.last_error(lib)
     bad = abi.GsMethod.from_buffer_copy(m)
     bad.globalization = abi.GS_STATIC
     rc = lib.gs_solve_expsum_lsq_f64(ctypes.byref(bad), 1, None, None, None, None, None, 0, None, ctypes.byref(o), None)

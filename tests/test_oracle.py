@@ -345,11 +345,13 @@ MGH_CASES = [
 ]
 
 
+@pytest.mark.parametrize("scaling", ["none", "diagonal"])
 @pytest.mark.parametrize("problem,x0,fstar", MGH_CASES)
-def test_mgh_minima_levenberg_marquardt(gs, orc, problem, x0, fstar):
+def test_mgh_minima_levenberg_marquardt(gs, orc, problem, x0, fstar, scaling):
     """test/nonlinear/leastsquares.jl:601-740: Newton + LevenbergMarquardt reaches the published
-    nonzero-residual minima, ending on the stationarity test (‖r*‖² to 64 eps)."""
-    m = gs.Newton(globalization=gs.LevenbergMarquardt())
+    nonzero-residual minima, ending on the stationarity test (‖r*‖² to 64 eps); the minimum does not
+    depend on the damping's scaling."""
+    m = gs.Newton(globalization=gs.LevenbergMarquardt(scaling=scaling))
     opt = gs.Options(max_iterations=200)
     x, s, nf = orc.solve_mgh(problem, m, opt, x0)
     assert s["code"] == gs.SUCCESS and s["stationary"] == 1
@@ -359,6 +361,46 @@ def test_mgh_minima_levenberg_marquardt(gs, orc, problem, x0, fstar):
         assert np.allclose(x, -1.0, atol=math.sqrt(np.finfo(float).eps))
     if problem == 8:
         assert s["iterations"] >= 2
+
+
+def _damped_reference(A, b, mu, diagonal):
+    """damped_reference(_diagonal) of test/globalization/levenbergmarquardt.jl:214-229: the LM step
+    from 0 of F = A x − b, −(AᵀA + μ D²)⁻¹ Aᵀ(−b), D = I or the floored column norms."""
+    d = np.linalg.norm(A, axis=0)
+    floor = 1.0 if d.max() == 0 else math.sqrt(np.finfo(float).eps) * d.max()
+    D2 = np.diag(np.maximum(d, floor) ** 2) if diagonal else np.eye(A.shape[1])
+    return -np.linalg.solve(A.T @ A + mu * D2, A.T @ (-b))
+
+
+def test_lm_damping_diagonal_kats(orc):
+    """test/globalization/levenbergmarquardt.jl:331-337: a zero Jacobian floors every column norm at
+    one; the floor is √eps(R) · maxⱼ dⱼ (a column norm of 1e-12 takes it, exactly); without scaling
+    D = I."""
+    assert orc.lm_damping_diagonal(np.zeros((2, 2))).tolist() == [1.0, 1.0]
+    assert orc.lm_damping_diagonal(np.array([[1.0, 0.0], [0.0, 1e-12]])).tolist() == [1.0, math.sqrt(np.finfo(float).eps)]
+    J = np.array([[3.0, 1.0], [4.0, 2.0], [0.0, 2.0]])
+    assert orc.lm_damping_diagonal(J).tolist() == [5.0, 3.0]
+    assert orc.lm_damping_diagonal(J, diagonal=False).tolist() == [1.0, 1.0]
+
+
+@pytest.mark.parametrize("diagonal", [False, True])
+def test_lm_step_against_damped_normal_equations(gs, orc, diagonal):
+    """test/globalization/levenbergmarquardt.jl:270-330: one LM step from 0 on F = A x − b matches
+    the damped normal equations with D = I or the floored column norms (100 eps), and a zero column
+    of A still receives positive damping under :diagonal (1e-10)."""
+    lm = gs.LevenbergMarquardt(mu0=1e-2, mu_min=1e-2, scaling="diagonal" if diagonal else "none")
+    method = gs.Newton(globalization=lm)
+    opts = gs.Options(f_abstol=1e-30, f_reltol=0.0, max_iterations=1)
+    A = np.array([[3.0, 1.0], [1.0, 2.0]])
+    b = np.array([4.0, 3.0])
+    x, s = orc.solve_linear_lsq(A, b, method, opts, np.zeros(2))
+    assert np.allclose(x, _damped_reference(A, b, 1e-2, diagonal), rtol=100 * np.finfo(float).eps, atol=0)
+    assert s["iterations"] == 1
+    if diagonal:
+        A0 = np.array([[3.0, 0.0], [1.0, 0.0]])
+        x0, _ = orc.solve_linear_lsq(A0, b, method, opts, np.zeros(2))
+        assert np.isfinite(x0).all()
+        assert np.allclose(x0, _damped_reference(A0, b, 1e-2, True), rtol=1e-10, atol=0)
 
 
 def test_expsum_noise_free_fit_recovers_parameters(gs, orc):
