@@ -21,8 +21,9 @@ using AMDGPU: ROCVector, ROCArray, stream
 using GeometricSolvers
 using GeometricSolvers: Options, SolverStatus, ReturnCode, L2Norm, MaxNorm, Static, Backtracking, DogLeg,
                         Newton, NoRefinement, Refinement, LUFactorization, NonlinearProblem, PerSystem,
-                        BatchParams, _validate_batch, _validate_batch_problem, _validate_workgroupsize,
-                        resolve_options, _tally_return_codes, ToReal
+                        BatchParams, LevenbergMarquardt, LeastSquaresProblem, _validate_batch,
+                        _validate_batch_problem, _validate_workgroupsize, resolve_options,
+                        _tally_return_codes, ToReal
 using StaticArrays: SVector, SMatrix, @SMatrix
 import GeometricSolvers: solve_batch!
 
@@ -140,51 +141,123 @@ struct ScalarSquare <: HIPFamily end
 (::ScalarSquare)(x::T, p) where {T <: Number} = x^2 - T(p)
 scalar_square_jacobian(x, _) = 2 * x
 
+"""
+    FpuGl3(β = 1.0)
+
+The Gauss–Legendre s = 3 stage system of a 16-site FPU-β chain with fixed ends, `n = 48`, in the
+stage increments `z = (z₁, z₂, z₃)` (16 each) of the position-only (Nyström) collocation form:
+`F_a = z_a − h c_a p − h² Σ_b (A²)_ab g(q + z_b)`, `g(Q)_m = V′(Q_{m+1} − Q_m) − V′(Q_m − Q_{m−1})`,
+`V′(r) = r + β r³`, `Q_0 = Q_17 = 0`. With `p = (h, qp)` (a `BatchParams(h, QP)`, `qp = (q, p)`
+of 32 entries). The tableau constants are the device's, rounded once from 50-digit values.
+"""
+struct FpuGl3 <: HIPFamily
+    β::Float64
+end
+FpuGl3() = FpuGl3(1.0)
+
+const GL3_C = (0x1.cda042f0236e1p-4, 0x1.0000000000000p-1, 0x1.c64bf7a1fb924p-1)
+const GL3_A2 = (0x1.6c16c16c16c17p-7, -0x1.0f9922995a9d1p-7, 0x1.ce741a1dacb5ep-9,
+                0x1.a370da4233f7dp-4, 0x1.c71c71c71c71cp-6, -0x1.537f6b3fb1445p-8,
+                0x1.bfe2a15ea5befp-3, 0x1.4f8d7b6829931p-3, 0x1.6c16c16c16c17p-7)  # (A²)_ab, row-major
+
+# the bond r_s = Q_s − Q_{s−1} of stage b (b = 0..2, s = 0..16; the ends at zero)
+@inline function gl3_bond(Q, b, s, ::Type{T}) where {T}
+    hi = s < 16 ? Q[16b + s + 1] : zero(T)
+    lo = s > 0 ? Q[16b + s] : zero(T)
+    return hi - lo
+end
+
+function (f::FpuGl3)(z::SVector{48, T}, p) where {T}
+    h, qp = p
+    β = T(f.β)
+    Q = SVector{48, T}(ntuple(l -> qp[(l - 1) % 16 + 1] + z[l], 48))
+    dφ(r) = r + β * ((r * r) * r)
+    g = SVector{48, T}(ntuple(l -> dφ(gl3_bond(Q, (l - 1) ÷ 16, (l - 1) % 16 + 1, T)) -
+                                   dφ(gl3_bond(Q, (l - 1) ÷ 16, (l - 1) % 16, T)), 48))
+    h2 = h * h
+    return SVector{48, T}(ntuple(48) do l
+        a, m = (l - 1) ÷ 16, (l - 1) % 16
+        t = T(GL3_A2[3a + 1]) * g[m + 1]
+        t = t + T(GL3_A2[3a + 2]) * g[16 + m + 1]
+        t = t + T(GL3_A2[3a + 3]) * g[32 + m + 1]
+        (z[l] - (h * T(GL3_C[a + 1])) * qp[16 + m + 1]) - h2 * t
+    end)
+end
+
+# J = I − h² (A² ⊗ H), H the tridiagonal force Jacobian of each stage with stiffnesses
+# K_s = 1 + 3β r_s²: the entry (a·16 + m, b·16 + k) is δ − (h² (A²)_ab) H_b(m, k).
+function fpu_gl3_jacobian(f::FpuGl3, z::SVector{48, T}, p) where {T}
+    h, qp = p
+    Q = SVector{48, T}(ntuple(l -> qp[(l - 1) % 16 + 1] + z[l], 48))
+    K(b, s) = one(T) + (3 * T(f.β)) * (gl3_bond(Q, b, s, T)^2)
+    h2 = h * h
+    return SMatrix{48, 48, T}(ntuple(48 * 48) do idx
+        i, j = (idx - 1) % 48, (idx - 1) ÷ 48
+        a, m, b, k = i ÷ 16, i % 16, j ÷ 16, j % 16
+        H = k == m ? -(K(b, m) + K(b, m + 1)) : k + 1 == m ? K(b, m) : k == m + 1 ? K(b, m + 1) : zero(T)
+        (i == j ? one(T) : zero(T)) - (h2 * T(GL3_A2[3a + b + 1])) * H
+    end)
+end
+
 """The `NonlinearProblem` of a registered family, with its analytic Jacobian."""
 family_problem(f::HenonHeilesMidpoint) = NonlinearProblem(f, hh_midpoint_jacobian)
 family_problem(f::TanhLinear3) = NonlinearProblem(f, tanh_linear3_jacobian)
 family_problem(f::Coupled2) = NonlinearProblem(f, coupled2_jacobian)
 family_problem(f::ScalarSquare) = NonlinearProblem(f, scalar_square_jacobian)
+family_problem(f::FpuGl3) = NonlinearProblem(f, (z, p) -> fpu_gl3_jacobian(f, z, p))
 
 family_id(::HenonHeilesMidpoint) = Int32(5)
 family_id(::TanhLinear3) = Int32(4)
 family_id(::Coupled2) = Int32(1)
 family_id(::ScalarSquare) = Int32(3)
+family_id(::FpuGl3) = Int32(6)
 family_n(::HenonHeilesMidpoint) = 4
 family_n(::TanhLinear3) = 3
 family_n(::Coupled2) = 2
 family_n(::ScalarSquare) = 1
+family_n(::FpuGl3) = 48
 
-# The parameter records of a batch: (device pointer, byte stride, shared scalars, roots to keep).
-function family_records(::HenonHeilesMidpoint, p::BatchParams, n)
-    Xk = p.per_system::ROCVector{SVector{4, Float64}}
-    return pointer(Xk), Int64(32), (Float64(p.shared),), Xk
+# The parameter records of a batch in the working precision R: (device pointer, byte stride, shared
+# scalars, roots to keep). A record is the isbits element of the device array (its size is the
+# stride); a shared record travels as a one-element array at stride 0.
+function family_records(::HenonHeilesMidpoint, p::BatchParams, ::Type{R}) where {R}
+    Xk = p.per_system::ROCVector{SVector{4, R}}
+    return pointer(Xk), Int64(sizeof(SVector{4, R})), (Float64(p.shared),), Xk
 end
-function family_records(::TanhLinear3, p::PerSystem, n)
-    # records of (SMatrix{3,3} column-major, SVector{3}) are 96 bytes each
-    d = p.data::ROCVector{Tuple{SMatrix{3, 3, Float64, 9}, SVector{3, Float64}}}
-    return pointer(d), Int64(96), (), d
+function family_records(::TanhLinear3, p::PerSystem, ::Type{R}) where {R}
+    # records of (SMatrix{3,3} column-major, SVector{3}): 12 entries each
+    d = p.data::ROCVector{Tuple{SMatrix{3, 3, R, 9}, SVector{3, R}}}
+    return pointer(d), Int64(12 * sizeof(R)), (), d
 end
-function family_records(::Coupled2, p::PerSystem, n)
-    d = p.data::ROCVector{SVector{2, Float64}}
-    return pointer(d), Int64(16), (), d
+function family_records(::Coupled2, p::PerSystem, ::Type{R}) where {R}
+    d = p.data::ROCVector{SVector{2, R}}
+    return pointer(d), Int64(2 * sizeof(R)), (), d
 end
-function family_records(::Coupled2, p::SVector{2, Float64}, n)
-    d = ROCArray([p])
+function family_records(::Coupled2, p::SVector{2}, ::Type{R}) where {R}
+    d = ROCArray([SVector{2, R}(p)])
     return pointer(d), Int64(0), (), d
 end
-function family_records(::ScalarSquare, p::PerSystem, n)
-    d = p.data::ROCVector{Float64}
-    return pointer(d), Int64(8), (), d
+function family_records(::ScalarSquare, p::PerSystem, ::Type{R}) where {R}
+    d = p.data::ROCVector{R}
+    return pointer(d), Int64(sizeof(R)), (), d
 end
+function family_records(f::FpuGl3, p::BatchParams, ::Type{Float64})
+    QP = p.per_system::ROCVector{SVector{32, Float64}}
+    return pointer(QP), Int64(256), (Float64(p.shared), f.β), QP
+end
+
+# What the library serves of a family beyond the method checks: the n = 48 kernel takes the Static
+# globalization without `reuse`, in Float64 only; anything else falls through to the generic path.
+served(::HIPFamily, cm, ::Type{R}) where {R} = true
+served(::FpuGl3, cm, ::Type{R}) where {R} = R === Float64 && cm.globalization == 0 && cm.reuse == 0
 
 # ---- the method and options as ABI structs ----------------------------------------------------
 
 device_method(m) = nothing  # not served: the generic method runs
 # `reuse = k` is served (the held factor of newton.jl:1781-1874); `keep_factor` keeps a factor across
 # the solves of one state, and each system of a batch solves once from a fresh state.
-function device_method(m::Newton{<:LUFactorization{F}, G, Nothing, RF, Float64, K}) where {
-        F, G, RF, K <: Union{Nothing, Int32}}
+function device_method(m::Newton{<:LUFactorization{F}, G, Nothing, RF, R, K}) where {
+        F, G, RF, R <: Union{Float32, Float64}, K <: Union{Nothing, Int32}}
     F === Nothing || F === Float32 || return nothing
     G <: Union{Static, Backtracking, DogLeg} || return nothing
     RF <: Union{NoRefinement, Refinement} || return nothing
@@ -200,10 +273,31 @@ function device_method(m::Newton{<:LUFactorization{F}, G, Nothing, RF, Float64, 
         m.reuse === nothing ? Int32(0) : m.reuse, radius, Int32(0), Int32(0))
 end
 
-function device_options(o::Options{Float64})
-    return GsOptionsF64(o.f_abstol, o.f_reltol, o.x_abstol, o.x_reltol, o.min_iterations,
-        o.max_iterations, o.max_stalls, o.norm isa MaxNorm ? Int32(1) : Int32(0), o.g_tol, o.kappa,
-        UInt8(o.roundoff), UInt8(o.rate_test), ntuple(_ -> 0x00, 6))
+# Newton(; globalization = LevenbergMarquardt(...)) for the least-squares batch: the stacked Givens
+# step (levenbergmarquardt.jl:359-482), which the reference takes for every linear method except an
+# SVD factor without scaling (its one-SVD step rounds differently, and there is no other R2 form of
+# a LeastSquaresProblem to fall through to).
+function device_lm_method(m::Newton)
+    g = m.globalization
+    g isa LevenbergMarquardt || throw(ArgumentError(
+        "the device least-squares batch takes Newton(; globalization = LevenbergMarquardt())"))
+    m.reuse === nothing || throw(ArgumentError("Newton with a LevenbergMarquardt globalization takes no reuse"))
+    m.jacobian === nothing || throw(ArgumentError(
+        "the device least-squares batch uses the family's analytic Jacobian: pass Newton without a jacobian"))
+    m.linsolve isa GeometricSolvers.SVDFactorization && g.scaling === Val(:none) && throw(ArgumentError(
+        "the device least-squares batch takes the stacked Givens step: use a linear method other than SVDFactorization() or scaling = :diagonal"))
+    lm = (Float64(g.μ₀), Float64(g.μ_min), Float64(g.μ_max), Float64(g.μ_decrease), Float64(g.μ_increase),
+        Float64(g.ρ_accept))
+    return GsMethod(Int32(0), Int32(0), Int32(2), Int32(0), 1.0, 1e-4, 0.5, Int32(100), Int32(5),
+        sqrt(eps(Float64)), 0.0, lm, Int32(0), Int32(0), 0.0,
+        g.scaling === Val(:diagonal) ? Int32(1) : Int32(0), Int32(0))
+end
+
+# Options{R} as the ABI's doubles (an Options{Float32} passes through exactly).
+function device_options(o::Options{R}) where {R}
+    return GsOptionsF64(Float64(o.f_abstol), Float64(o.f_reltol), Float64(o.x_abstol), Float64(o.x_reltol),
+        o.min_iterations, o.max_iterations, o.max_stalls, o.norm isa MaxNorm ? Int32(1) : Int32(0),
+        Float64(o.g_tol), Float64(o.kappa), UInt8(o.roundoff), UInt8(o.rate_test), ntuple(_ -> 0x00, 6))
 end
 
 # ---- the launch -------------------------------------------------------------------------------
@@ -215,67 +309,134 @@ function gs_error(rc)
     error("libgsolve_hip: $msg")
 end
 
-function hip_launch!(X, S, codes, fam::HIPFamily, cm::GsMethod, p, opt::Options{Float64}, order, ws)
-    P, stride, shared, root = family_records(fam, p, length(X))
+function hip_launch!(X, S, codes, fam::HIPFamily, cm::GsMethod, p, opt::Options{R}, order, ws) where {R}
+    P, stride, shared, root = family_records(fam, p, R)
     cf = GsFamily(family_id(fam), Int32(family_n(fam)), Int32(family_n(fam)), Int32(0),
         ntuple(k -> k <= length(shared) ? shared[k] : 0.0, 8))
     co = device_options(opt)
     order_dev = order === nothing ? nothing : ROCArray(Int64.(collect(order)))
+    Sp = S === nothing ? C_NULL : pointer(S)
+    Cp = codes === nothing ? C_NULL : pointer(codes)
+    Op = order_dev === nothing ? C_NULL : pointer(order_dev)
     GC.@preserve root order_dev begin
-        rc = ccall((:gs_solve_batch_f64, libgsolve), Cint,
-            (Ref{GsMethod}, Ref{GsFamily}, Int64, Ptr{Cvoid}, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Ptr{Cvoid},
-                Ptr{Cvoid}, Ref{GsOptionsF64}, Int32, Ptr{Cvoid}),
-            cm, cf, length(X), pointer(X), P, stride, S === nothing ? C_NULL : pointer(S),
-            codes === nothing ? C_NULL : pointer(codes),
-            order_dev === nothing ? C_NULL : pointer(order_dev), co, Int32(something(ws, 0)),
-            stream().stream)
+        rc = if R === Float32
+            # the Float32 twin (SolverStatus{Float32}, Float32 records and iterates)
+            ccall((:gs_solve_batch_f32, libgsolve), Cint,
+                (Ref{GsMethod}, Ref{GsFamily}, Int64, Ptr{Cvoid}, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Ptr{Cvoid},
+                    Ptr{Cvoid}, Ref{GsOptionsF64}, Int32, Ptr{Cvoid}),
+                cm, cf, length(X), pointer(X), P, stride, Sp, Cp, Op, co, Int32(something(ws, 0)),
+                stream().stream)
+        else
+            ccall((:gs_solve_batch_f64, libgsolve), Cint,
+                (Ref{GsMethod}, Ref{GsFamily}, Int64, Ptr{Cvoid}, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Ptr{Cvoid},
+                    Ptr{Cvoid}, Ref{GsOptionsF64}, Int32, Ptr{Cvoid}),
+                cm, cf, length(X), pointer(X), P, stride, Sp, Cp, Op, co, Int32(something(ws, 0)),
+                stream().stream)
+        end
     end
     rc == 0 || gs_error(rc)
     return nothing
 end
 
-# The status form (batched.jl:257-287) for a device batch of a registered family.
-function solve_batch!(X::ROCVector{SVector{N, Float64}}, S::ROCVector{SolverStatus{Float64}},
+# The status form (batched.jl:257-287) for a device batch of a registered family, in Float64 or
+# Float32 (the working precision of the iterates; S holds SolverStatus of the same precision).
+function solve_batch!(X::ROCVector{SVector{N, T}}, S::ROCVector{SolverStatus{T}},
         prob::NonlinearProblem{<:HIPFamily}, method::Newton, p = nothing;
         order::Union{Nothing, AbstractVector{<:Integer}} = nothing,
         options::Union{Nothing, Options} = nothing,
-        workgroupsize::Union{Nothing, Integer} = nothing, kwargs...) where {N}
-    cm = device_method(Adapt_real(method))
-    cm === nothing && return invoke(solve_batch!,
-        Tuple{AbstractVector, AbstractVector{SolverStatus{Float64}}, NonlinearProblem, Any, Any},
+        workgroupsize::Union{Nothing, Integer} = nothing, kwargs...) where {N, T <: Union{Float32, Float64}}
+    cm = device_method(Adapt_real(method, T))
+    (cm === nothing || !served(prob.F, cm, T)) && return invoke(solve_batch!,
+        Tuple{AbstractVector, AbstractVector{SolverStatus{T}}, NonlinearProblem, Any, Any},
         X, S, prob, method, p; order, options, workgroupsize, kwargs...)
     _validate_batch(X, p, order)
     _validate_batch_problem(prob)
     length(S) == length(X) || throw(DimensionMismatch(
         "status vector length $(length(S)) does not match batch length $(length(X))"))
     ws = _validate_workgroupsize(workgroupsize)
-    opt = resolve_options(Float64, options, kwargs)
+    opt = resolve_options(T, options, kwargs)
     isempty(X) && return S
     hip_launch!(X, S, nothing, prob.F, cm, p, opt, order, ws)
     return S
 end
 
 # The counts form (batched.jl:289-307).
-function solve_batch!(X::ROCVector{SVector{N, Float64}}, prob::NonlinearProblem{<:HIPFamily},
+function solve_batch!(X::ROCVector{SVector{N, T}}, prob::NonlinearProblem{<:HIPFamily},
         method::Newton, p = nothing;
         order::Union{Nothing, AbstractVector{<:Integer}} = nothing,
         options::Union{Nothing, Options} = nothing,
-        workgroupsize::Union{Nothing, Integer} = nothing, kwargs...) where {N}
-    cm = device_method(Adapt_real(method))
-    cm === nothing && return invoke(solve_batch!,
+        workgroupsize::Union{Nothing, Integer} = nothing, kwargs...) where {N, T <: Union{Float32, Float64}}
+    cm = device_method(Adapt_real(method, T))
+    (cm === nothing || !served(prob.F, cm, T)) && return invoke(solve_batch!,
         Tuple{AbstractVector, NonlinearProblem, Any, Any},
         X, prob, method, p; order, options, workgroupsize, kwargs...)
     _validate_batch(X, p, order)
     _validate_batch_problem(prob)
     ws = _validate_workgroupsize(workgroupsize)
-    opt = resolve_options(Float64, options, kwargs)
+    opt = resolve_options(T, options, kwargs)
     isempty(X) && return _tally_return_codes(ReturnCode[])
     codes = ROCArray{ReturnCode}(undef, length(X))
     hip_launch!(X, nothing, codes, prob.F, cm, p, opt, order, ws)
     return _tally_return_codes(Array(codes))
 end
 
-# Every float of the method in Float64, as `_prepare_batch` adapts it (batched.jl:162).
-Adapt_real(m) = GeometricSolvers.Adapt.adapt(ToReal{Float64}(), m)
+# Every float of the method in the working precision T, as `_prepare_batch` adapts it (batched.jl:162).
+Adapt_real(m, ::Type{T} = Float64) where {T} = GeometricSolvers.Adapt.adapt(ToReal{T}(), m)
+
+# ---- the least-squares batch (an R2 form the reference does not have) ------------------------
+
+"""
+    ExpSumFits()
+
+Exponential-sum least-squares fits `y(t) ≈ Σ_{k≤K} a_k exp(−λ_k t)` at `t_j = 5j/(m−1)`, with
+`x = (a_1..a_K, λ_1..λ_K)` in the first `2K` of the 8 entries of a fit's `SVector{8, Float64}`;
+`LeastSquaresProblem(ExpSumFits())` is the problem of `gs_solve_expsum_lsq_f64`.
+"""
+struct ExpSumFits end
+
+"""
+    ExpSumData(K, m, y_offset, Y)
+
+The per-fit data of an `ExpSumFits` batch on the device: `K::ROCVector{Int32}` (1..4),
+`m::ROCVector{Int32}` (samples, ≥ 2K), `y_offset::ROCVector{Int64}` and the concatenated samples
+`Y::ROCVector{Float64}`; fit i's samples are `Y[y_offset[i] + 1 : y_offset[i] + m[i]]`.
+"""
+struct ExpSumData{VK <: ROCVector{Int32}, VM <: ROCVector{Int32}, VO <: ROCVector{Int64}, VY <: ROCVector{Float64}}
+    K::VK
+    m::VM
+    y_offset::VO
+    Y::VY
+end
+Base.length(d::ExpSumData) = length(d.K)
+
+# The status form: every fit's R1 solve (solver.jl:148-189, 277-378) with Newton + LevenbergMarquardt,
+# in one call; the fits are bucketed by size on the device, so there is no `order` or workgroup size.
+function solve_batch!(X::ROCVector{SVector{8, Float64}}, S::ROCVector{SolverStatus{Float64}},
+        prob::LeastSquaresProblem{ExpSumFits}, method::Newton, data::ExpSumData;
+        options::Union{Nothing, Options} = nothing, kwargs...)
+    cm = device_lm_method(Adapt_real(method))
+    length(data) == length(X) || throw(DimensionMismatch(
+        "ExpSumData length $(length(data)) does not match batch length $(length(X))"))
+    length(S) == length(X) || throw(DimensionMismatch(
+        "status vector length $(length(S)) does not match batch length $(length(X))"))
+    opt = resolve_options(Float64, options, kwargs)
+    isempty(X) && return S
+    co = device_options(opt)
+    rc = ccall((:gs_solve_expsum_lsq_f64, libgsolve), Cint,
+        (Ref{GsMethod}, Int64, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int64,
+            Ptr{Cvoid}, Ref{GsOptionsF64}, Ptr{Cvoid}),
+        cm, length(X), pointer(X), pointer(data.K), pointer(data.m), pointer(data.y_offset),
+        pointer(data.Y), length(data.Y), pointer(S), co, stream().stream)
+    rc == 0 || gs_error(rc)
+    return S
+end
+
+# The counts form.
+function solve_batch!(X::ROCVector{SVector{8, Float64}}, prob::LeastSquaresProblem{ExpSumFits},
+        method::Newton, data::ExpSumData; kwargs...)
+    S = ROCArray{SolverStatus{Float64}}(undef, length(X))
+    solve_batch!(X, S, prob, method, data; kwargs...)
+    return _tally_return_codes([s.code for s in Array(S)])
+end
 
 end # module

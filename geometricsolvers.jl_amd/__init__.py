@@ -499,7 +499,7 @@ class FpuGl3(Family):
     in the stage increments z_i = Q_i − q (the reference's StageResidual, src/nonlinear/problem.jl:
     183-189, stacks the 96 first-order increments of (q, p); this form eliminates the momenta).
     V(r) = r²/2 + β r⁴/4 per bond. Parameter: BatchParams(h, QP) with QP (nsys, 32) = (q, p) per system;
-    the start is z = 0. One wave per system on the device, the LU factor in LDS.
+    the start is z = 0. One wave per system on the device, each lane holding one row of the factor.
     """
 
     id, n, p_len = _abi.GS_FAM_FPU_GL3, 48, 32
