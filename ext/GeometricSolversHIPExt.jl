@@ -21,7 +21,7 @@ using AMDGPU: ROCVector, ROCArray, stream
 using GeometricSolvers
 using GeometricSolvers: Options, SolverStatus, ReturnCode, L2Norm, MaxNorm, Static, Backtracking, DogLeg,
                         Newton, NoRefinement, Refinement, LUFactorization, NonlinearProblem, PerSystem,
-                        BatchParams, LevenbergMarquardt, LeastSquaresProblem, _validate_batch,
+                        BatchParams, LevenbergMarquardt, LeastSquaresProblem, Deflated, _validate_batch,
                         _validate_batch_problem, _validate_workgroupsize, resolve_options,
                         _tally_return_codes, ToReal
 using StaticArrays: SVector, SMatrix, @SMatrix
@@ -382,6 +382,96 @@ end
 
 # Every float of the method in the working precision T, as `_prepare_batch` adapts it (batched.jl:162).
 Adapt_real(m, ::Type{T} = Float64) where {T} = GeometricSolvers.Adapt.adapt(ToReal{T}(), m)
+
+# ---- the Deflated batch (deflation.jl:714-760) ------------------------------------------------
+
+struct GsDeflation
+    nroots::Int32
+    pad::Int32
+    power::Float64
+    shift::Float64
+    roots::NTuple{32, Float64}
+end
+
+# The roots as the flat record of gs_deflation, or `nothing` where the device form does not hold
+# them (more than 4 roots or 32 entries, a power that is not an integer in 1:64); such a Deflated,
+# and every method the device form does not take (DogLeg, reuse, best = true, a Float32 batch), runs
+# the reference's generic batch, which also raises the reference's own errors.
+function device_deflation(def::Deflated, n::Int)
+    nr = length(def.roots)
+    (1 <= nr <= 4 && nr * n <= 32) || return nothing
+    p = Float64(def.power)
+    (isinteger(p) && 1 <= p <= 64) || return nothing
+    flat = Float64[]
+    for r in def.roots
+        length(r) == n || return nothing
+        append!(flat, Float64.(collect(r)))
+    end
+    return GsDeflation(Int32(nr), Int32(0), p, Float64(def.shift),
+        ntuple(k -> k <= length(flat) ? flat[k] : 0.0, 32))
+end
+
+served_deflated(fam::HIPFamily, cm, ::Type{T}) where {T} = !(fam isa FpuGl3) && T === Float64 &&
+    cm !== nothing && cm.globalization in (0, 1) && cm.reuse == 0 && cm.best == 0
+
+function hip_launch_deflated!(X, S, codes, def::Deflated, cd::GsDeflation, cm::GsMethod, p, opt, order, ws)
+    fam = def.prob.F
+    P, stride, shared, root = family_records(fam, p, Float64)
+    cf = GsFamily(family_id(fam), Int32(family_n(fam)), Int32(family_n(fam)), Int32(0),
+        ntuple(k -> k <= length(shared) ? shared[k] : 0.0, 8))
+    order_dev = order === nothing ? nothing : ROCArray(Int64.(collect(order)))
+    GC.@preserve root order_dev begin
+        rc = ccall((:gs_solve_batch_deflated_f64, libgsolve), Cint,
+            (Ref{GsMethod}, Ref{GsFamily}, Ref{GsDeflation}, Int64, Ptr{Cvoid}, Ptr{Cvoid}, Int64,
+                Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ref{GsOptionsF64}, Int32, Ptr{Cvoid}),
+            cm, cf, cd, length(X), pointer(X), P, stride, S === nothing ? C_NULL : pointer(S),
+            codes === nothing ? C_NULL : pointer(codes), order_dev === nothing ? C_NULL : pointer(order_dev),
+            device_options(opt), Int32(something(ws, 0)), stream().stream)
+    end
+    rc == 0 || gs_error(rc)
+    return nothing
+end
+
+# The status form of the Deflated batch.
+function solve_batch!(X::ROCVector{SVector{N, T}}, S::ROCVector{SolverStatus{T}},
+        def::Deflated{<:NonlinearProblem{<:HIPFamily}}, method::Newton, p = nothing;
+        order::Union{Nothing, AbstractVector{<:Integer}} = nothing,
+        options::Union{Nothing, Options} = nothing,
+        workgroupsize::Union{Nothing, Integer} = nothing, kwargs...) where {N, T <: Union{Float32, Float64}}
+    cm = device_method(Adapt_real(method, T))
+    cd = device_deflation(def, N)
+    (cd === nothing || !served_deflated(def.prob.F, cm, T)) && return invoke(solve_batch!,
+        Tuple{AbstractVector, AbstractVector{SolverStatus{T}}, Deflated{<:NonlinearProblem}, Any, Any},
+        X, S, def, method, p; order, options, workgroupsize, kwargs...)
+    _validate_batch(X, p, order)
+    length(S) == length(X) || throw(DimensionMismatch(
+        "status vector length $(length(S)) does not match batch length $(length(X))"))
+    ws = _validate_workgroupsize(workgroupsize)
+    opt = resolve_options(T, options, kwargs)
+    isempty(X) && return S
+    hip_launch_deflated!(X, S, nothing, def, cd, cm, p, opt, order, ws)
+    return S
+end
+
+# The counts form of the Deflated batch.
+function solve_batch!(X::ROCVector{SVector{N, T}}, def::Deflated{<:NonlinearProblem{<:HIPFamily}},
+        method::Newton, p = nothing;
+        order::Union{Nothing, AbstractVector{<:Integer}} = nothing,
+        options::Union{Nothing, Options} = nothing,
+        workgroupsize::Union{Nothing, Integer} = nothing, kwargs...) where {N, T <: Union{Float32, Float64}}
+    cm = device_method(Adapt_real(method, T))
+    cd = device_deflation(def, N)
+    (cd === nothing || !served_deflated(def.prob.F, cm, T)) && return invoke(solve_batch!,
+        Tuple{AbstractVector, Deflated{<:NonlinearProblem}, Any, Any},
+        X, def, method, p; order, options, workgroupsize, kwargs...)
+    _validate_batch(X, p, order)
+    ws = _validate_workgroupsize(workgroupsize)
+    opt = resolve_options(T, options, kwargs)
+    isempty(X) && return _tally_return_codes(ReturnCode[])
+    codes = ROCArray{ReturnCode}(undef, length(X))
+    hip_launch_deflated!(X, nothing, codes, def, cd, cm, p, opt, order, ws)
+    return _tally_return_codes(Array(codes))
+end
 
 # ---- the least-squares batch (an R2 form the reference does not have) ------------------------
 

@@ -17,6 +17,7 @@ always the HIP kernel: a missing library raises, there is no CPU fallback.
 from __future__ import annotations
 
 import ctypes
+import functools
 import enum
 import math
 from dataclasses import dataclass, field
@@ -31,7 +32,7 @@ __all__ = [
     "ArgumentError", "DimensionMismatch", "ReturnCode", "SUCCESS", "STALLED", "MAXITERS",
     "SINGULAR", "NONFINITE", "LINESEARCH_FAILED", "STATUS_DTYPE", "STATUS_DTYPE_F32", "L2Norm", "MaxNorm", "Options",
     "LUFactorization", "Static", "Backtracking", "DogLeg", "LevenbergMarquardt", "NoRefinement", "Refinement", "Newton",
-    "NonlinearProblem", "Coupled2", "Hetero2", "ScalarSquare", "TanhLinear3",
+    "NonlinearProblem", "Deflated", "Coupled2", "Hetero2", "ScalarSquare", "TanhLinear3",
     "HenonHeilesMidpoint", "FpuGl3", "PerSystem", "BatchParams", "parameters", "solve_batch",
     "LeastSquaresProblem", "ExpSumFits", "ExpSumData",
     "status_buffer", "statuses", "isconverged", "batch_summary", "BatchSummary", "load_library",
@@ -515,6 +516,58 @@ class FpuGl3(Family):
         return t, t.stride(0) * t.element_size(), (float(p.shared), self.beta)
 
 
+class Deflated:
+    """Deflated(prob, roots; power = 2, shift = 1) (src/nonlinear/deflation.jl:15-235): the problem whose
+    residual is M(x) F(x), M(x) = ∏ᵢ (‖x − rᵢ‖^(−p) + σ), with the product-rule Jacobian
+    M J + F ∇Mᵀ, so that the known roots repel Newton; the status reports, and the residual test
+    reads, the inner ‖F‖. `prob` is a NonlinearProblem of a lane-per-system family; `roots` a
+    non-empty tuple of roots (numbers for n = 1, sequences of n reals otherwise), shared by every
+    system of a batch. On the device: at most 4 roots and an integer power."""
+
+    def __init__(self, prob, roots, power: float = 2, shift: float = 1):
+        if not isinstance(prob, NonlinearProblem):
+            raise ArgumentError("Deflated needs a NonlinearProblem or a StationarityProblem")
+        power, shift = float(power), float(shift)
+        if not (power > 0.5 and math.isfinite(power)):
+            raise ArgumentError(
+                "Deflated power must be finite and > 1/2: at or below 1/2 the known roots do not repel Newton")
+        if not (shift > 0 and math.isfinite(shift)):
+            raise ArgumentError("Deflated shift must be finite and > 0")
+        if not isinstance(roots, (tuple, list)):
+            raise ArgumentError(
+                "Deflated roots must be a non-empty Tuple of roots or an AbstractVector of real vectors")
+        if len(roots) == 0:
+            raise ArgumentError("Deflated needs at least one root")
+        rs = []
+        for r in roots:
+            v = np.atleast_1d(np.asarray(r, dtype=np.float64))
+            if v.ndim != 1 or v.size == 0:
+                raise ArgumentError("Deflated roots must be real numbers or real vectors")
+            rs.append(v)
+        if any(v.size != rs[0].size for v in rs):
+            raise ArgumentError("Deflated needs roots of one length")
+        if not all(np.isfinite(v).all() for v in rs):
+            raise ArgumentError("Deflated needs roots with finite real entries")
+        self.prob, self.roots, self.power, self.shift = prob, tuple(rs), power, shift
+
+    def to_c(self, n: int):
+        """gs_deflation for an iterate of n entries (the checks of a solve's start, deflation.jl:551-575)."""
+        if any(v.size != n for v in self.roots):
+            raise ArgumentError(
+                f"Deflated needs roots of the length of the iterate: the start has {n} entries and a root has "
+                f"{next(v.size for v in self.roots if v.size != n)}")
+        if len(self.roots) > 4:
+            raise ArgumentError("the device deflation takes at most 4 roots")
+        if self.power != math.floor(self.power) or self.power > 64:
+            raise ArgumentError("the device deflation takes an integer power in 1:64")
+        d = _abi.GsDeflation()
+        d.nroots, d.power, d.shift = len(self.roots), self.power, self.shift
+        for r, v in enumerate(self.roots):
+            for i, x in enumerate(v):
+                d.roots[r * n + i] = float(x)
+        return d
+
+
 class NonlinearProblem:
     """NonlinearProblem(F) (src/nonlinear/problem.jl:81-110) over a registered device family."""
 
@@ -677,7 +730,7 @@ def solve_batch(X, *args, order=None, options: Optional[Options] = None, workgro
     `order` is a permutation of 1:nsys (1-based, as Julia's): work-item i solves system order[i].
     The call is synchronous, as `_launch!` is (batched.jl:193).
     """
-    if args and isinstance(args[0], (NonlinearProblem, LeastSquaresProblem)):
+    if args and isinstance(args[0], (NonlinearProblem, LeastSquaresProblem, Deflated)):
         S = None
         rest = args
     else:
@@ -694,6 +747,13 @@ def solve_batch(X, *args, order=None, options: Optional[Options] = None, workgro
         raise ArgumentError("too many positional arguments")
     if isinstance(prob, LeastSquaresProblem):
         return _solve_lsq(X, S, prob, method, p, order, options, workgroupsize, kwargs)
+    deflation = None
+    if isinstance(prob, Deflated):
+        deflation, prob = prob, prob.prob
+        if isinstance(method, Newton) and method.best:
+            raise ArgumentError(
+                "Deflated takes no Newton with best = true: the best iterate would be ranked by the deflated "
+                "residual, not by the inner residual that status.fnorm reports")
     if not isinstance(prob, NonlinearProblem):
         raise ArgumentError("prob must be a NonlinearProblem over a registered family")
     if not isinstance(method, Newton):
@@ -705,6 +765,13 @@ def solve_batch(X, *args, order=None, options: Optional[Options] = None, workgro
     fam = prob.F
     _check_x(X, fam.n, allow_f32=True)
     elt = _eltype(X)
+    cdefl = None
+    if deflation is not None:
+        if elt != "float64":
+            raise ArgumentError(
+                "Deflated needs an iterate in the element type of its roots: the roots give Float64 and the "
+                "start is Float32")
+        cdefl = deflation.to_c(fam.n)
     order_h = _validate_batch(X, p, order)
     nsys = X.shape[0]
     torch = _torch()
@@ -744,9 +811,14 @@ def solve_batch(X, *args, order=None, options: Optional[Options] = None, workgro
     if S is None:
         codes = torch.empty(nsys, dtype=torch.uint8, device=X.device)
     copt = opt.to_c()
-    entry = lib.gs_solve_batch_f32 if elt == "float32" else lib.gs_solve_batch_f64
+    if cdefl is not None:
+        entry = functools.partial(lib.gs_solve_batch_deflated_f64, ctypes.byref(cm), ctypes.byref(cf),
+                                  ctypes.byref(cdefl))
+    else:
+        entry = functools.partial(lib.gs_solve_batch_f32 if elt == "float32" else lib.gs_solve_batch_f64,
+                                  ctypes.byref(cm), ctypes.byref(cf))
     rc = entry(
-        ctypes.byref(cm), ctypes.byref(cf), nsys, ctypes.c_void_p(X.data_ptr()),
+        nsys, ctypes.c_void_p(X.data_ptr()),
         ctypes.c_void_p(P.data_ptr()), p_stride,
         ctypes.c_void_p(S.data_ptr()) if S is not None else None,
         ctypes.c_void_p(codes.data_ptr()) if codes is not None else None,

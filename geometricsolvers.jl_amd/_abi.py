@@ -115,6 +115,17 @@ class GsFamily(ctypes.Structure):
     ]
 
 
+class GsDeflation(ctypes.Structure):
+    """gs_deflation: the shared roots, power and shift of a Deflated batch."""
+    _fields_ = [
+        ("nroots", ctypes.c_int32),
+        ("pad_", ctypes.c_int32),
+        ("power", ctypes.c_double),
+        ("shift", ctypes.c_double),
+        ("roots", ctypes.c_double * 32),
+    ]
+
+
 # SolverStatus{Float64} as a numpy structured dtype (72 bytes, Julia's field offsets).
 STATUS_DTYPE = np.dtype(
     {
@@ -175,6 +186,7 @@ EXPORTED_SYMBOLS = (
     "gs_solve_expsum_lsq_f64",
     "gs_shutdown",
     "gs_solve_batch_f32",
+    "gs_solve_batch_deflated_f64",
     "gs_lsq_set_shape",
     "gs_lsq_get_shape",
     "gs_lsq_set_graph",
@@ -255,6 +267,11 @@ def load_library(path: os.PathLike | None = None):
     lib.gs_shutdown.argtypes = []
     lib.gs_solve_batch_f32.restype = ctypes.c_int
     lib.gs_solve_batch_f32.argtypes = lib.gs_solve_batch_f64.argtypes
+    lib.gs_solve_batch_deflated_f64.restype = ctypes.c_int
+    lib.gs_solve_batch_deflated_f64.argtypes = [
+        ctypes.POINTER(GsMethod), ctypes.POINTER(GsFamily), ctypes.POINTER(GsDeflation), ctypes.c_int64, vp, vp,
+        ctypes.c_int64, vp, vp, vp, ctypes.POINTER(GsOptionsF64), ctypes.c_int32, vp,
+    ]
     if lib.gs_version() != ABI_VERSION:
         raise RuntimeError(f"libgsolve_hip ABI version {lib.gs_version()} != {ABI_VERSION}")
     sizes = (ctypes.c_int64 * 4)()

@@ -8,15 +8,26 @@
 // entry (i, j).
 #pragma once
 
+#include <type_traits>
+
 #include "device_common.hpp"
 
 namespace gs {
 
 // Launch arguments shared by every family.
+// The roots, power and shift of a Deflated batch (gs_deflation), read by FamDeflatedT only.
+struct DeflArgs {
+    int32_t nroots;      // 1..4
+    int32_t power;       // p, an integer ≥ 1
+    double shift;        // σ
+    double roots[32];    // root r at roots[r·N ..]
+};
+
 struct FamArgs {
     const char* P;       // parameter records (device)
     int64_t p_stride;    // bytes between records; 0 = one shared record
     double shared[8];    // scalar parameters shared by every system
+    DeflArgs defl;       // the deflation of gs_solve_batch_deflated_f64
 };
 
 GS_DEV const char* record(const FamArgs& a, int64_t idx) { return a.P + (a.p_stride ? idx * a.p_stride : 0); }
@@ -180,5 +191,136 @@ using FamHetero2 = FamHetero2T<double>;
 using FamScalarSquare = FamScalarSquareT<double>;
 using FamTanhLinear3 = FamTanhLinear3T<double>;
 using FamHenonHeilesMidpoint = FamHenonHeilesMidpointT<double>;
+
+// ---- Deflated (src/nonlinear/deflation.jl) --------------------------------------------------------
+
+// Julia's x^n for a Float64 base and an integer n (Base.Math.pow_body, Julia 1.13; not under the
+// reference tree), which x^y with an integer-valued Float64 y runs: power by squaring of the
+// reciprocal (for n < 0) with a compensated running product, x = 1 giving 1 exactly.
+GS_DEV double pow_int(double x, int n) {
+    if (x == 1.0 || n == 0) return 1.0;
+    if (n == 3) return x * x * x;
+    double y = 1.0, xnlo = 0.0, ynlo = 0.0;
+    if (n < 0) {
+        const double rx = 1.0 / x;
+        if (n == -1) return rx;
+        if (isfin(x)) xnlo = -__builtin_fma(x, rx, -1.0) * rx;
+        x = rx;
+        n = -n;
+    }
+    while (n > 1) {
+        if (n & 1) {
+            const double err = __builtin_fma(y, xnlo, x * ynlo);
+            const double hi = x * y;
+            ynlo = __builtin_fma(x, y, -hi) + err;
+            y = hi;
+        }
+        const double err = (x * 2.0) * xnlo;
+        const double hi = x * x;
+        xnlo = __builtin_fma(x, x, -hi) + err;
+        x = hi;
+        n >>= 1;
+    }
+    const double err = __builtin_fma(y, xnlo, x * ynlo);
+    return (isfin(x) && isfin(err)) ? __builtin_fma(x, y, err) : x * y;
+}
+
+// Deflated(prob, roots; power = p, shift = σ) around family Inner: the residual M(x) F(x) with
+// M = ∏ᵢ (‖x − rᵢ‖^(−p) + σ) (_deflate_scale, deflation.jl:251-261), the Jacobian M J + F ∇Mᵀ
+// (_deflated_jacobian, :339-343) with (M, ∇M) by the division-free recurrence g ← mᵢ g + M ∇mᵢ,
+// M ← M mᵢ and the unit-scaled coordinates of ∇mᵢ (_deflate_factors, :264-278; :239-241), and the
+// product M (J v) + ⟨∇M, v⟩ F (_deflated_jvp, :352-357). Roots beyond nroots are never read.
+template <class Inner> struct FamDeflatedT {
+    using T = typename Inner::T;
+    static constexpr int N = Inner::N;
+    static constexpr bool kDeflated = true;
+    static_assert(sizeof(T) == 8, "the device deflation is Float64");
+    Inner in;
+    int nroots, power;
+    T shift;
+    T roots[4][N];
+    GS_DEV void load(const FamArgs& a, int64_t idx) {
+        in.load(a, idx);
+        nroots = a.defl.nroots;
+        power = a.defl.power;
+        shift = T(a.defl.shift);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int i = 0; i < N; ++i) roots[r][i] = T(a.defl.roots[r * N + i]);
+    }
+    // M(x)
+    GS_DEV T scale(const T (&x)[N]) const {
+        T M = T(1);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            if (r >= nroots) break;
+            T u[N];
+#pragma unroll
+            for (int i = 0; i < N; ++i) u[i] = x[i] - roots[r][i];
+            M = M * (pow_int(rnorm<N>(u), -power) + shift);
+        }
+        return M;
+    }
+    // (M(x), ∇M(x))
+    GS_DEV void factors(const T (&x)[N], T& M, T (&g)[N]) const {
+        M = T(1);
+#pragma unroll
+        for (int i = 0; i < N; ++i) g[i] = T(0);
+        const T pw = T(power);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            if (r >= nroots) break;
+            T u[N];
+#pragma unroll
+            for (int i = 0; i < N; ++i) u[i] = x[i] - roots[r][i];
+            const T d = rnorm<N>(u);
+            const T phi = pow_int(d, -power);
+            const T m = phi + shift;
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                const T gi = u[i] == T(0) ? T(0) : -((pw * ((u[i] / d) * phi)) / d);
+                g[i] = m * g[i] + M * gi;
+            }
+            M = M * m;
+        }
+    }
+    GS_DEV void F(const T (&x)[N], T (&r)[N]) const {
+        const T M = scale(x);
+        in.F(x, r);
+#pragma unroll
+        for (int i = 0; i < N; ++i) r[i] = M * r[i];
+    }
+    GS_DEV void J(const T (&x)[N], T (&Jm)[N][N]) const {
+        T Fi[N], M, g[N];
+        in.F(x, Fi);
+        in.J(x, Jm);
+        factors(x, M, g);
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+#pragma unroll
+            for (int j = 0; j < N; ++j) Jm[i][j] = M * Jm[i][j] + Fi[i] * g[j];
+    }
+    GS_DEV void Jv(const T (&x)[N], const T (&v)[N], T (&out)[N]) const {
+        T Fi[N], Ji[N][N], jv[N], M, g[N];
+        in.F(x, Fi);
+        in.J(x, Ji);
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            T acc = Ji[i][0] * v[0];
+#pragma unroll
+            for (int k = 1; k < N; ++k) acc = __builtin_fma(Ji[i][k], v[k], acc);
+            jv[i] = acc;
+        }
+        factors(x, M, g);
+        const T gv = rdot<N>(g, v);
+#pragma unroll
+        for (int i = 0; i < N; ++i) out[i] = M * jv[i] + gv * Fi[i];
+    }
+};
+
+// Whether a family is a Deflated wrapper (its status reports the inner residual).
+template <class F, class = void> struct is_deflated : std::false_type {};
+template <class F> struct is_deflated<F, std::void_t<decltype(F::kDeflated)>> : std::true_type {};
 
 }  // namespace gs

@@ -7,6 +7,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <cmath>
 #include <string>
 
 #include "../../include/gsolve.h"
@@ -246,6 +247,65 @@ int gs_solve_batch_f64(const gs_method* method, const gs_family* family, int64_t
         rc = launch_small_family<kStatic, false>(family->id, a, mixed, refine, diag, wg, st);
     else
         rc = launch_small_family<kBacktracking, false>(family->id, a, mixed, refine, diag, wg, st);
+    return finish_launch(rc, st, timed);
+}
+
+int gs_solve_batch_deflated_f64(const gs_method* method, const gs_family* family, const gs_deflation* deflation,
+                                int64_t nsys, double* X, const void* P, int64_t p_stride, gs_status_f64* S,
+                                uint8_t* codes, const int64_t* order, const gs_options_f64* options,
+                                int32_t workgroupsize, void* stream) {
+    g_last_error.clear();
+    const FamilySpec* fs = nullptr;
+    int rc = validate_batch<double>(method, family, nsys, X, P, p_stride, S, codes, options, workgroupsize, &fs);
+    if (rc != GS_OK) return rc;
+    // the Deflated constructor's checks (deflation.jl:120-132, 162-210) and _check_deflated_method
+    // (:594-604), then the device's own limits
+    if (!deflation) return set_error(GS_EINVAL, "deflation must not be NULL");
+    const gs_deflation& d = *deflation;
+    if (d.nroots < 1) return set_error(GS_EINVAL, "Deflated needs at least one root");
+    if (d.nroots > 4) return set_error(GS_EINVAL, "the device deflation takes at most 4 roots");
+    if (!(d.power > 0.5 && std::isfinite(d.power)))
+        return set_error(GS_EINVAL,
+                         "Deflated power must be finite and > 1/2: at or below 1/2 the known roots do not repel Newton");
+    if (!(d.shift > 0.0 && std::isfinite(d.shift))) return set_error(GS_EINVAL, "Deflated shift must be finite and > 0");
+    if (d.power != std::floor(d.power) || d.power > 64.0)
+        return set_error(GS_EINVAL, "the device deflation takes an integer power in 1:64");
+    for (int r = 0; r < d.nroots; ++r)
+        for (int i = 0; i < fs->n; ++i)
+            if (!std::isfinite(d.roots[r * fs->n + i]))
+                return set_error(GS_EINVAL, "Deflated needs roots with finite real entries");
+    if (method->best != 0)
+        return set_error(GS_EINVAL,
+                         "Deflated takes no Newton with best = true: the best iterate would be ranked by the deflated "
+                         "residual, not by the inner residual that status.fnorm reports");
+    if (family->id == GS_FAM_FPU_GL3)
+        return set_error(GS_EFAMILY, "a Deflated batch wraps one of the lane-per-system families (1-5)");
+    if (method->globalization == GS_DOGLEG || method->reuse > 0)
+        return set_error(GS_EFAMILY, "a Deflated batch takes the Static or Backtracking globalization without reuse");
+    if (nsys == 0) return GS_OK;
+    KernelArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.fam.P = static_cast<const char*>(P);
+    a.fam.p_stride = p_stride;
+    for (int k = 0; k < 8; ++k) a.fam.shared[k] = family->shared[k];
+    a.fam.defl.nroots = d.nroots;
+    a.fam.defl.power = int32_t(d.power);
+    a.fam.defl.shift = d.shift;
+    for (int k = 0; k < d.nroots * fs->n; ++k) a.fam.defl.roots[k] = d.roots[k];
+    a.nsys = nsys;
+    a.X = X;
+    a.S = S;
+    a.codes = codes;
+    a.order = order;
+    a.q = dev_params(*method, *options);
+    const bool diag = S != nullptr;
+    const int wg = workgroupsize > 0 ? workgroupsize : kSmallMaxWorkgroup;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const bool mixed = method->factor_precision == GS_FACTOR_F32;
+    const bool refine = method->refinement == GS_REFINEMENT;
+    bool timed = timing_begin(st);
+    const int glob = method->globalization == GS_BACKTRACKING ? kBacktracking : kStatic;
+    rc = launch_small_deflated(family->id, glob, a, mixed, refine, diag, wg, st);
     return finish_launch(rc, st, timed);
 }
 

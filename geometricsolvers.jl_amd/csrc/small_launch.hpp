@@ -158,6 +158,10 @@ int launch_small_reuse(int family, int glob, const KernelArgs& a, bool mixed, bo
 // Defined in small_dogleg.hip: DogLeg.
 int launch_small_dogleg(int family, const KernelArgs& a, bool mixed, bool refine, bool diag, int wg, hipStream_t st);
 
+// Defined in small_deflated.hip: Static and Backtracking on a Deflated batch (FamDeflatedT).
+int launch_small_deflated(int family, int glob, const KernelArgs& a, bool mixed, bool refine, bool diag, int wg,
+                          hipStream_t st);
+
 // Defined in small_f32.hip: every form of a Float32 batch.
 int launch_small_f32(int family, int glob, bool reuse, const KernelArgsT<float>& a, bool refine, bool diag, int wg,
                      hipStream_t st);

@@ -346,12 +346,17 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
         if (exact) {
             d0 = -T(2.0) * phi0;
         } else {
-            T xa[N], Fa[N], Ja[N][N], Jv[N];
+            T xa[N], Fa[N], Jv[N];
 #pragma unroll
             for (int i = 0; i < N; ++i) xa[i] = x[i] + T(0.0) * d[i];
             fam.F(xa, Fa);
-            fam.J(xa, Ja);
-            matvec<N>(Ja, d, Jv);
+            if constexpr (is_deflated<Fam>::value) {
+                fam.Jv(xa, d, Jv);  // the deflated product M (J d) + ⟨∇M, d⟩ F (deflation.jl:352-357)
+            } else {
+                T Ja[N][N];
+                fam.J(xa, Ja);
+                matvec<N>(Ja, d, Jv);
+            }
             d0 = T(2.0) * rdot<N>(Fa, Jv);
         }
         if (!(isfin(phi0) && isfin(d0) && d0 < T(0.0))) {
@@ -1015,6 +1020,25 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
     };
 
     // solve!! (solver.jl:277-378) for a square NonlinearProblem with a Newton method.
+    // converged_status (solver.jl:122-125): the status fnorm of a step's end at x. A Deflated family
+    // reports its inner ‖F‖ = ‖M F‖ / M where the norm is finite (_inner_status, deflation.jl:514-520).
+    GS_DEV T status_fnorm(const T (&x)[N], T fn) const {
+        if constexpr (is_deflated<Fam>::value) return isfin(fn) ? fn / fam.scale(x) : fn;
+        else return fn;
+    }
+    // test_status (solver.jl:138-146): the norm the residual test reads, from the step's raw fnorm,
+    // its status fnorm and the residual Fx at x. Under MaxNorm a Deflated family scales the status
+    // fnorm by ‖M F‖∞ / ‖M F‖₂ (deflation.jl:540-548).
+    GS_DEV T test_fnorm(const T (&Fx)[N], T raw, T st) const {
+        if (!q.norm_max) return st;
+        if constexpr (is_deflated<Fam>::value) {
+            const T mx = rmaxnorm<N>(Fx);
+            return raw == T(0.0) ? mx : (mx / raw) * st;
+        } else {
+            return rmaxnorm<N>(Fx);
+        }
+    }
+
     GS_DEV Result solve(T (&x)[N]) {
         T Fx[N];
         fam.F(x, Fx);
@@ -1039,7 +1063,8 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
         if (!isfin(fnorm0)) {
             R.code = kNonfinite;
         } else {
-            const T ftest0 = q.norm_max ? rmaxnorm<N>(Fx) : fnorm0;
+            R.fnorm = status_fnorm(x, fnorm0);
+            const T ftest0 = test_fnorm(Fx, fnorm0, R.fnorm);
             const T ftol = jmax(q.f_abstol, q.f_reltol * ftest0);
             if (0 >= q.min_iterations && ftest0 <= ftol) {
                 R.code = kSuccess;
@@ -1059,7 +1084,7 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
                     R.rate = R.stepnorm > T(0.0) ? s.stepnorm / R.stepnorm : tnan<T>();
                     R.code = s.code;
                     R.iterations += 1;
-                    R.fnorm = s.fnorm;
+                    R.fnorm = status_fnorm(x, s.fnorm);
                     R.stepnorm = s.stepnorm;
                     R.failures += s.failures;
                     R.promoted |= s.promoted;
@@ -1068,7 +1093,7 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
                     if (REUSE && reused && R.rate >= T(1.0)) stalled = true;
                     if (s.code == kSingular || s.code == kNonfinite || s.code == kMaxIters) break;
                     if (s.code == kLsFailed && s.stepnorm == T(0.0)) break;
-                    const T ftest = q.norm_max ? rmaxnorm<N>(Fx) : R.fnorm;
+                    const T ftest = test_fnorm(Fx, s.fnorm, R.fnorm);
                     if (R.iterations >= q.min_iterations && ftest <= ftol) {
                         R.code = kSuccess;
                         break;

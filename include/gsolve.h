@@ -203,6 +203,29 @@ int gs_solve_batch_f64(const gs_method* method, const gs_family* family, int64_t
  * path (batched.jl:171-195) leaves device memory to Julia's garbage collector. */
 int gs_shutdown(void);
 
+/* Deflated(prob, roots; power, shift) (src/nonlinear/deflation.jl): the roots shared by every system
+ * of a batch (the isbits Tuple of static roots that solve_batch! takes, deflation.jl:681-688). Root
+ * r occupies roots[r·n .. r·n + n − 1]. */
+typedef struct gs_deflation {
+    int32_t nroots;     /* 1..4                                                               */
+    int32_t pad_;
+    double power;       /* p > 1/2, finite; the device takes integer-valued p (Julia's x^y then
+                         * runs its integer power, Base.Math.pow_body)                       */
+    double shift;       /* σ > 0, finite                                                      */
+    double roots[32];   /* finite entries                                                     */
+} gs_deflation;
+
+/* gs_solve_batch_f64 on Deflated(NonlinearProblem(family), roots; power, shift): the steps and the
+ * line search take the deflated residual M(x) F(x) with the product-rule Jacobian M J + F ∇Mᵀ
+ * (deflation.jl:236-399), and the status reports, and the residual test reads, the inner
+ * ‖F‖ = ‖M F‖ / M (converged_status / test_status, deflation.jl:514-548). Families 1–5, Static and
+ * Backtracking, plain, mixed and refined; a Newton with best = true is refused as the reference
+ * refuses it (deflation.jl:594-604); DogLeg and reuse = k have no deflated device form (GS_EFAMILY). */
+int gs_solve_batch_deflated_f64(const gs_method* method, const gs_family* family, const gs_deflation* deflation,
+                                int64_t nsys, double* X, const void* P, int64_t p_stride, gs_status_f64* S,
+                                uint8_t* codes, const int64_t* order, const gs_options_f64* options,
+                                int32_t workgroupsize, void* stream);
+
 /* The Float32 twin of gs_solve_batch_f64: X holds nsys × n floats (Vector{SVector{n,Float32}}), the
  * parameter records hold floats (Int64 codes for family 2), S receives SolverStatus{Float32}. The
  * options and the method's floats arrive as doubles and are used as Float32 (R(x), as Options{Float32}

@@ -116,34 +116,55 @@ static void run_batch(const Method<R>& m, const Options<R>& o, int64_t nsys, R* 
     }
 }
 
-// The lane-per-system families (1-5) in the working precision R, records of R (Int64 codes for 2).
+// run_batch on the problems of `make`, or, with a deflation, on their Deflated wrappers.
+template <int N, class R, class SS, class MakeProb>
+static void run_batch_d(const gs_deflation* d, const Method<R>& m, const Options<R>& o, int64_t nsys, R* X, SS* S,
+                        uint8_t* codes, const int64_t* order, int nthreads, MakeProb make) {
+    if (!d) {
+        run_batch<N>(m, o, nsys, X, S, codes, order, nthreads, make);
+        return;
+    }
+    run_batch<N>(m, o, nsys, X, S, codes, order, nthreads, [&](int64_t idx) {
+        using Inner = decltype(make(idx));
+        DeflatedProb<R, N, Inner> w{make(idx)};
+        w.nroots = d->nroots;
+        w.power = int64_t(d->power);
+        w.shift = R(d->shift);
+        for (int r = 0; r < 4; ++r)
+            for (int i = 0; i < N; ++i) w.roots[r][i] = r < d->nroots ? R(d->roots[r * N + i]) : R(0);
+        return w;
+    });
+}
+
+// The lane-per-system families (1-5) in the working precision R, records of R (Int64 codes for 2),
+// deflated where `defl` is given.
 template <class R, class SS>
 static int solve_small_batch(const Method<R>& m, const Options<R>& o, const gs_family* fam, int64_t nsys, R* X,
                              const void* P, int64_t p_stride, SS* S, uint8_t* codes, const int64_t* order,
-                             int nthreads) {
+                             int nthreads, const gs_deflation* defl = nullptr) {
     const char* Pb = static_cast<const char*>(P);
     auto rec = [&](int64_t idx) -> const char* { return Pb + (p_stride ? idx * p_stride : 0); };
     switch (fam->id) {
         case GS_FAM_COUPLED2:
-            run_batch<2>(m, o, nsys, X, S, codes, order, nthreads, [&](int64_t idx) {
+            run_batch_d<2>(defl, m, o, nsys, X, S, codes, order, nthreads, [&](int64_t idx) {
                 const R* p = reinterpret_cast<const R*>(rec(idx));
                 return Coupled2<R>{p[0], p[1]};
             });
             return 0;
         case GS_FAM_HETERO2:
-            run_batch<2>(m, o, nsys, X, S, codes, order, nthreads, [&](int64_t idx) {
+            run_batch_d<2>(defl, m, o, nsys, X, S, codes, order, nthreads, [&](int64_t idx) {
                 const int64_t* p = reinterpret_cast<const int64_t*>(rec(idx));
                 return Hetero2<R>{p[0]};
             });
             return 0;
         case GS_FAM_SCALAR_SQUARE:
-            run_batch<1>(m, o, nsys, X, S, codes, order, nthreads, [&](int64_t idx) {
+            run_batch_d<1>(defl, m, o, nsys, X, S, codes, order, nthreads, [&](int64_t idx) {
                 const R* p = reinterpret_cast<const R*>(rec(idx));
                 return ScalarSquare<R>{p[0]};
             });
             return 0;
         case GS_FAM_TANH_LINEAR3:
-            run_batch<3>(m, o, nsys, X, S, codes, order, nthreads, [&](int64_t idx) {
+            run_batch_d<3>(defl, m, o, nsys, X, S, codes, order, nthreads, [&](int64_t idx) {
                 const R* p = reinterpret_cast<const R*>(rec(idx));
                 TanhLinear3<R> pr;
                 for (int k = 0; k < 9; ++k) pr.A[k] = p[k];
@@ -153,7 +174,7 @@ static int solve_small_batch(const Method<R>& m, const Options<R>& o, const gs_f
             return 0;
         case GS_FAM_HENON_HEILES_MIDPOINT: {
             const R h = R(fam->shared[0]);
-            run_batch<4>(m, o, nsys, X, S, codes, order, nthreads, [&](int64_t idx) {
+            run_batch_d<4>(defl, m, o, nsys, X, S, codes, order, nthreads, [&](int64_t idx) {
                 const R* p = reinterpret_cast<const R*>(rec(idx));
                 HenonHeilesMidpoint<R> pr;
                 pr.h = h;
@@ -201,6 +222,22 @@ int orc_solve_batch_f64(const gs_method* method, const gs_family* fam, int64_t n
         default:
             return solve_small_batch<double>(m, o, fam, nsys, X, P, p_stride, S, codes, order, nthreads);
     }
+}
+
+// The host restatement of gs_solve_batch_deflated_f64 (include/gsolve.h).
+int orc_solve_batch_deflated_f64(const gs_method* method, const gs_family* fam, const gs_deflation* defl,
+                                 int64_t nsys, double* X, const void* P, int64_t p_stride, gs_status_f64* S,
+                                 uint8_t* codes, const int64_t* order, const gs_options_f64* options, int nthreads) {
+    if (!method || !fam || !options || !defl || nsys < 0 || defl->nroots < 1 || defl->nroots > 4) return GS_EINVAL;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    Method<double> m = method_from<double>(*method);
+    Options<double> o = options_from<double>(*options);
+    return solve_small_batch<double>(m, o, fam, nsys, X, P, p_stride, S, codes, order, nthreads, defl);
+}
+
+// Julia's Float64 x^n for an integer n (the deflation's power), for the KATs.
+void orc_pow_int(const double* x, int64_t n, double* out, int64_t len) {
+    for (int64_t i = 0; i < len; ++i) out[i] = pow_int(x[i], n);
 }
 
 // The host restatement of gs_solve_batch_f32 (include/gsolve.h): the same solve in Float32, the

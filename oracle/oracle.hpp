@@ -418,6 +418,110 @@ template <class R, int N, class P> inline Vec<R, N> jvp(P& prob, const Vec<R, N>
     else return matvec<R, N>(prob.J(x), v);
 }
 
+// Julia's x^n for a Float64 base and an integer n (Base.Math.pow_body, Julia 1.13; a dependency not
+// under the reference tree), which x^y with an integer-valued Float64 y runs (deflation.jl's
+// d^(-power)): power by squaring of the reciprocal for n < 0, with a compensated running product
+// (two_mul = (x·y, fma(x, y, −x·y)), muladd = fma); x = 1 gives 1 exactly.
+inline double pow_int(double x, int64_t n) {
+    if (x == 1.0 || n == 0) return 1.0;
+    if (n == 3) return x * x * x;
+    double y = 1.0, xnlo = 0.0, ynlo = 0.0;
+    if (n < 0) {
+        const double rx = 1.0 / x;
+        if (n == -1) return rx;
+        if (std::isfinite(x)) xnlo = -std::fma(x, rx, -1.0) * rx;
+        x = rx;
+        n = -n;
+    }
+    while (n > 1) {
+        if (n & 1) {
+            const double err = std::fma(y, xnlo, x * ynlo);
+            const double hi = x * y;
+            ynlo = std::fma(x, y, -hi) + err;
+            y = hi;
+        }
+        const double err = (x * 2.0) * xnlo;
+        const double hi = x * x;
+        xnlo = std::fma(x, x, -hi) + err;
+        x = hi;
+        n >>= 1;
+    }
+    const double err = std::fma(y, xnlo, x * ynlo);
+    return (std::isfinite(x) && std::isfinite(err)) ? std::fma(x, y, err) : x * y;
+}
+
+// Deflated(prob, roots; power = p, shift = σ) (src/nonlinear/deflation.jl) around problem P: the
+// residual M(x) F(x), M = ∏ᵢ (‖x − rᵢ‖^(−p) + σ) (_deflate_scale, :251-261); the Jacobian
+// M J + F ∇Mᵀ (:339-343) with (M, ∇M) from the recurrence g ← mᵢ g + M ∇mᵢ, M ← M mᵢ and the
+// unit-scaled ∇mᵢ coordinates −(p((uⱼ/d)ϕ)/d), zero where uⱼ = 0 (_deflate_factors, :239-278);
+// the product M (J v) + ⟨∇M, v⟩ F (_deflated_jvp, :352-357).
+template <class R, int N, class P> struct DeflatedProb {
+    static constexpr bool kDeflated = true;
+    P in;
+    int nroots;
+    int64_t power;
+    R shift;
+    Vec<R, N> roots[4];
+    long nf = 0;
+    R scale(const Vec<R, N>& x) const {
+        R M = R(1);
+        for (int r = 0; r < nroots; ++r) {
+            Vec<R, N> u;
+            for (int i = 0; i < N; ++i) u[i] = x[i] - roots[r][i];
+            M = M * (R(pow_int(double(rnorm(u)), -power)) + shift);
+        }
+        return M;
+    }
+    void factors(const Vec<R, N>& x, R& M, Vec<R, N>& g) const {
+        M = R(1);
+        g.fill(R(0));
+        const R pw = R(power);
+        for (int r = 0; r < nroots; ++r) {
+            Vec<R, N> u;
+            for (int i = 0; i < N; ++i) u[i] = x[i] - roots[r][i];
+            const R d = rnorm(u);
+            const R phi = R(pow_int(double(d), -power));
+            const R m = phi + shift;
+            for (int i = 0; i < N; ++i) {
+                const R gi = u[i] == R(0) ? R(0) : -((pw * ((u[i] / d) * phi)) / d);
+                g[i] = m * g[i] + M * gi;
+            }
+            M = M * m;
+        }
+    }
+    Vec<R, N> F(const Vec<R, N>& x) {
+        ++nf;
+        const R M = scale(x);
+        Vec<R, N> f = in.F(x);
+        for (int i = 0; i < N; ++i) f[i] = M * f[i];
+        return f;
+    }
+    Mat<R, N> J(const Vec<R, N>& x) {
+        Vec<R, N> f = in.F(x);
+        Mat<R, N> Jm = in.J(x);
+        R M;
+        Vec<R, N> g;
+        factors(x, M, g);
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < N; ++j) Jm[i + j * N] = M * Jm[i + j * N] + f[i] * g[j];
+        return Jm;
+    }
+    Vec<R, N> Jv(const Vec<R, N>& x, const Vec<R, N>& v) {
+        Vec<R, N> f = in.F(x);
+        Vec<R, N> jv = jvp<R, N>(in, x, v);
+        R M;
+        Vec<R, N> g;
+        factors(x, M, g);
+        const R gv = rdot(g, v);
+        Vec<R, N> out;
+        for (int i = 0; i < N; ++i) out[i] = M * jv[i] + gv * f[i];
+        return out;
+    }
+};
+
+template <class P, class = void> struct is_deflated_prob : std::false_type {};
+template <class P> struct is_deflated_prob<P, std::void_t<decltype(P::kDeflated)>> : std::true_type {};
+
 // grad_ratio_and_norm (src/nonlinear/newton.jl:1287-1298): ‖Jᴴ F‖/(‖J‖_F ‖F‖) and ‖J‖_F.
 template <class T, int N>
 inline void grad_ratio_and_norm(const Mat<T, N>& J, const Vec<T, N>& Fx, T fnorm, T& gr, T& jnorm) {
@@ -1298,6 +1402,23 @@ template <class R, int N, class P> struct Solver {
         return st;
     }
 
+    // converged_status (solver.jl:122-125): a Deflated problem reports its inner ‖F‖ = ‖M F‖ / M where
+    // the norm is finite (_inner_status, deflation.jl:514-528).
+    R status_fnorm(const Vec<R, N>& x, R fn) const {
+        if constexpr (is_deflated_prob<P>::value) return std::isfinite(fn) ? fn / prob.scale(x) : fn;
+        else return fn;
+    }
+    // test_status (solver.jl:138-146; deflation.jl:540-548 under MaxNorm).
+    R test_fnorm(const Vec<R, N>& Fx, R raw, R st) const {
+        if (o.norm != MAXNORM) return st;
+        if constexpr (is_deflated_prob<P>::value) {
+            const R mx = rmaxnorm(Fx);
+            return raw == R(0) ? mx : (mx / raw) * st;
+        } else {
+            return rmaxnorm(Fx);
+        }
+    }
+
     Status<R> solve(Vec<R, N>& x) {
         Vec<R, N> Fx = F(x);
         R fnorm0 = rnorm(Fx);
@@ -1309,7 +1430,8 @@ template <class R, int N, class P> struct Solver {
         status0.stepnorm = 0;
         if (!std::isfinite(fnorm0)) return finish(x, st, closing_status(status0, NONFINITE, false, false));
         Status<R> status = status0;
-        R ftest0 = o.norm == MAXNORM ? rmaxnorm(Fx) : status.fnorm;
+        status.fnorm = status_fnorm(x, fnorm0);
+        R ftest0 = test_fnorm(Fx, fnorm0, status.fnorm);
         {
             Status<R> tested = status;
             tested.fnorm = ftest0;
@@ -1322,14 +1444,15 @@ template <class R, int N, class P> struct Solver {
             StepInfo<R> info = step(x, Fx, st);
             Status<R> recorded = record(status, info);
             status = recorded;
+            status.fnorm = status_fnorm(x, recorded.fnorm);
             // observe_rate!! (newton.jl:1205-1210): a step of rate θ ≥ 1 along a reused factor
             // refreshes the factor at the next step
             if (m.reuse > 0 && st.reused && recorded.rate >= R(1)) st.stalled = true;
             if (info.code == SINGULAR || info.code == NONFINITE || info.code == MAXITERS)
                 return finish(x, st, status);
             if (info.code == LINESEARCH_FAILED && info.stepnorm == R(0)) return finish(x, st, status);
-            Status<R> tested = recorded;
-            if (o.norm == MAXNORM) tested.fnorm = rmaxnorm(Fx);
+            Status<R> tested = status;
+            tested.fnorm = test_fnorm(Fx, recorded.fnorm, status.fnorm);
             if (residual_converged(o, tested, ftest0))
                 return finish(x, st, closing_status(status, SUCCESS, false, false));
             bool steptested = !(info.code == STALLED || info.limited);

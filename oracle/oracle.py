@@ -34,6 +34,11 @@ def lib():
         L.orc_solve_batch_f64.argtypes = [vp, vp, ctypes.c_int64, vp, vp, ctypes.c_int64, vp, vp, vp, vp, ctypes.c_int]
         L.orc_solve_batch_f32.restype = ctypes.c_int
         L.orc_solve_batch_f32.argtypes = L.orc_solve_batch_f64.argtypes
+        L.orc_solve_batch_deflated_f64.restype = ctypes.c_int
+        L.orc_solve_batch_deflated_f64.argtypes = [vp, vp, vp, ctypes.c_int64, vp, vp, ctypes.c_int64, vp, vp, vp, vp,
+                                                   ctypes.c_int]
+        L.orc_pow_int.restype = None
+        L.orc_pow_int.argtypes = [vp, ctypes.c_int64, vp, ctypes.c_int64]
         L.orc_standard_cell.restype = ctypes.c_int
         L.orc_standard_cell.argtypes = [ctypes.c_int] * 4 + [vp] * 5
         L.orc_standard_u0.restype = ctypes.c_int
@@ -97,11 +102,12 @@ def host_records(family, p, nsys: int, real=np.float64):
     return rec, 0, ()
 
 
-def solve_batch(method, family, X: np.ndarray, p, options, order=None, nthreads: int = 0):
+def solve_batch(method, family, X: np.ndarray, p, options, order=None, nthreads: int = 0, deflated=None):
     """The oracle's batched solve on host arrays: returns (X, statuses) with X overwritten.
 
     `method`, `options`: objects with `to_c()` giving the ABI structs of include/gsolve.h;
-    `family`: a registered family (id, n, p_len); `order`: 1-based permutation or None.
+    `family`: a registered family (id, n, p_len); `order`: 1-based permutation or None;
+    `deflated`: a Deflated (its roots, power and shift; its problem is `family`'s) or None.
     """
     f32 = np.asarray(X).dtype == np.float32
     real = np.float32 if f32 else np.float64
@@ -113,10 +119,16 @@ def solve_batch(method, family, X: np.ndarray, p, options, order=None, nthreads:
     fam = _family_struct(family, shared)
     S = np.zeros(nsys, dtype=_status_dtype_f32() if f32 else _status_dtype())
     o = None if order is None else np.ascontiguousarray(np.asarray(order, dtype=np.int64))
-    entry = lib().orc_solve_batch_f32 if f32 else lib().orc_solve_batch_f64
-    rc = entry(
-        ctypes.byref(cm), ctypes.byref(fam), nsys, _ptr(X), _ptr(rec), stride, _ptr(S), None,
-        None if o is None else _ptr(o), ctypes.byref(co), nthreads)
+    if deflated is not None:
+        cd = deflated.to_c(family.n)
+        rc = lib().orc_solve_batch_deflated_f64(
+            ctypes.byref(cm), ctypes.byref(fam), ctypes.byref(cd), nsys, _ptr(X), _ptr(rec), stride, _ptr(S), None,
+            None if o is None else _ptr(o), ctypes.byref(co), nthreads)
+    else:
+        entry = lib().orc_solve_batch_f32 if f32 else lib().orc_solve_batch_f64
+        rc = entry(
+            ctypes.byref(cm), ctypes.byref(fam), nsys, _ptr(X), _ptr(rec), stride, _ptr(S), None,
+            None if o is None else _ptr(o), ctypes.byref(co), nthreads)
     if rc != 0:
         raise RuntimeError(f"oracle failed with {rc}")
     return X, S
@@ -301,3 +313,11 @@ def dogleg_pw_step(kind: int, method, p0: float = 0.0, p1: float = 0.0, x0: floa
         raise RuntimeError(f"oracle DogLeg step failed with {rc}")
     return dict(x=out[0], radius=out[1], code=int(out[2]), failures=int(out[3]), stepnorm=out[4], f_calls=int(out[5]),
                 limited=bool(out[6]))
+
+
+def pow_int(x, n: int):
+    """Julia's Float64 x^n for an integer n (Base.Math.pow_body), elementwise."""
+    x = np.ascontiguousarray(np.atleast_1d(np.asarray(x, dtype=np.float64)))
+    out = np.empty_like(x)
+    lib().orc_pow_int(_ptr(x), int(n), _ptr(out), x.size)
+    return out

@@ -529,3 +529,98 @@ def test_float32_batch_oracle(gs, orc, workloads):
                              gs.Options(f_abstol=1e-12, f_reltol=0.0))
     assert (s32["code"] == gs.SUCCESS).all()
     assert np.abs(x32 - x64).max() <= 1e-5 * np.abs(x64).max()
+
+
+# ---- Deflated (src/nonlinear/deflation.jl) -----------------------------------------------------
+
+def test_pow_int_closed_forms(orc):
+    """Julia's Float64 x^n for an integer n (Base.Math.pow_body, the deflation's d^(−p)): 1/x
+    exactly for n = −1, powers of two exactly, 1^n = 1, 0^(−2) = Inf, and within one ulp of the
+    long-double power elsewhere (a compensated product, so correctly rounded almost always)."""
+    x = np.array([0.3, 1.7, 1e-3, 123.456, 0.9999999, 3.0e150, 7.1e-154])
+    assert (orc.pow_int(x, -1) == 1.0 / x).all()
+    assert orc.pow_int(np.array([0.25, 8.0]), -2).tolist() == [16.0, 1.0 / 64.0]
+    assert orc.pow_int(np.array([1.0]), -7)[0] == 1.0
+    assert orc.pow_int(np.array([0.0]), -2)[0] == np.inf
+    for n in (-2, -3, -4, -5, 2, 4):
+        v = orc.pow_int(x, n)
+        ref = x.astype(np.longdouble) ** n
+        ok = (np.abs(ref) < np.finfo(float).max) & (np.abs(ref) > np.finfo(float).tiny)  # in the normal range
+        err = np.abs((v[ok] - ref[ok]) / ref[ok]) / np.finfo(float).eps
+        assert err.max() <= 1.0, (n, err.max())
+
+
+def _defl_solve(gs, orc, fam, prob, roots, X0, p, method=None, opts=None, **kw):
+    d = gs.Deflated(prob, roots, **kw)
+    return orc.solve_batch(method or gs.Newton(), fam, np.array(X0, dtype=float), p,
+                           opts or gs.Options(f_abstol=1e-12, f_reltol=0.0), deflated=d)
+
+
+def test_deflated_scalar_square_leaves_the_known_root(gs, orc):
+    """x² − p deflated at +√p from 1.2√p converges to −√p; the status reports the inner |x² − p|
+    (‖M F‖ / M, within a few ulps of the inner residual); a start on the deflated root is NONFINITE
+    at 0 iterations (deflation.jl:101-104); an undeflated solve from the same start finds +√p."""
+    fam = gs.ScalarSquare()
+    prob = gs.NonlinearProblem(fam)
+    p = 2.0
+    X0 = [[1.2 * math.sqrt(p)]]
+    x, s = _defl_solve(gs, orc, fam, prob, (math.sqrt(p),), X0, gs.PerSystem(np.array([p])))
+    assert s["code"][0] == gs.SUCCESS and x[0, 0] == pytest.approx(-math.sqrt(p), rel=1e-12)
+    inner = abs(x[0, 0] ** 2 - p)
+    assert s["fnorm"][0] <= 1e-12 and s["fnorm"][0] == pytest.approx(inner, rel=1e-9, abs=4e-16)
+    x1, s1 = _defl_solve(gs, orc, fam, prob, (math.sqrt(p),), [[math.sqrt(p)]], gs.PerSystem(np.array([p])))
+    assert s1["code"][0] == gs.NONFINITE and s1["iterations"][0] == 0
+    xu, su = orc.solve_batch(gs.Newton(), fam, np.array(X0), gs.PerSystem(np.array([p])),
+                             gs.Options(f_abstol=1e-12, f_reltol=0.0))
+    assert su["code"][0] == gs.SUCCESS and xu[0, 0] == pytest.approx(math.sqrt(p), rel=1e-12)
+
+
+def test_deflated_finds_distinct_roots_from_one_start(gs, orc):
+    """test/nonlinear/deflation.jl:159-173 on the coupled family x + y²/8 − p₀, y + x²/8 − p₁ at
+    p = (10, 10), whose four real roots are (5.798, 5.798), (−13.798, −13.798), (9.657, −1.657) and
+    (−1.657, 9.657): under full Newton steps the solve from one start finds r₁, the solve deflated
+    at r₁ another root, at (r₁, r₂) a third and at (r₁, r₂, r₃) the fourth, each converged with its
+    inner residual at the tolerance."""
+    fam = gs.Coupled2()
+    prob = gs.NonlinearProblem(fam)
+    p = gs.PerSystem(np.array([[10.0, 10.0]]))
+    m = gs.Newton()
+    opts = gs.Options(f_abstol=1e-12, f_reltol=0.0, max_iterations=200)
+    x0 = [[0.5, -0.3]]
+    roots, found = [], []
+    for k in range(4):
+        if k == 0:
+            r, s = orc.solve_batch(m, fam, np.array(x0), p, opts)
+        else:
+            r, s = _defl_solve(gs, orc, fam, prob, tuple(roots), x0, p, m, opts)
+        assert s["code"][0] == gs.SUCCESS
+        F = np.array([r[0, 0] + r[0, 1] ** 2 / 8 - 10.0, r[0, 1] + r[0, 0] ** 2 / 8 - 10.0])
+        assert np.linalg.norm(F) <= 1e-11 and s["fnorm"][0] <= 1e-12
+        roots.append(r[0])
+    for i in range(4):
+        for j in range(i + 1, 4):
+            assert np.linalg.norm(roots[i] - roots[j]) > 0.5
+    s32 = math.sqrt(32.0)  # the roots of x² + 8x − 80 = 0 on the diagonal, and of x + y = 8 off it
+    want = sorted([(-4 + math.sqrt(96), -4 + math.sqrt(96)), (-4 - math.sqrt(96), -4 - math.sqrt(96)),
+                   (4 + s32, 4 - s32), (4 - s32, 4 + s32)])
+    got = sorted(tuple(r) for r in roots)
+    assert np.allclose(got, want, rtol=1e-12, atol=1e-12)
+
+
+def test_deflated_checks(gs):
+    """The Deflated constructor's checks (deflation.jl:120-210) and the device's limits."""
+    prob = gs.NonlinearProblem(gs.Coupled2())
+    with pytest.raises(gs.ArgumentError, match="at least one root"):
+        gs.Deflated(prob, ())
+    with pytest.raises(gs.ArgumentError, match="power must be finite and > 1/2"):
+        gs.Deflated(prob, ((1.0, 2.0),), power=0.5)
+    with pytest.raises(gs.ArgumentError, match="shift must be finite and > 0"):
+        gs.Deflated(prob, ((1.0, 2.0),), shift=0.0)
+    with pytest.raises(gs.ArgumentError, match="roots of one length"):
+        gs.Deflated(prob, ((1.0, 2.0), (1.0,)))
+    with pytest.raises(gs.ArgumentError, match="finite real entries"):
+        gs.Deflated(prob, ((1.0, np.nan),))
+    with pytest.raises(gs.ArgumentError, match="integer power"):
+        gs.Deflated(prob, ((1.0, 2.0),), power=1.5).to_c(2)
+    with pytest.raises(gs.ArgumentError, match="length of the iterate"):
+        gs.Deflated(prob, ((1.0, 2.0, 3.0),)).to_c(2)
