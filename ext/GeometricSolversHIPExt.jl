@@ -21,7 +21,8 @@ using AMDGPU: ROCVector, ROCArray, stream
 using GeometricSolvers
 using GeometricSolvers: Options, SolverStatus, ReturnCode, L2Norm, MaxNorm, Static, Backtracking, DogLeg,
                         Newton, NoRefinement, Refinement, LUFactorization, NonlinearProblem, PerSystem,
-                        BatchParams, LevenbergMarquardt, LeastSquaresProblem, Deflated, _validate_batch,
+                        BatchParams, LevenbergMarquardt, LeastSquaresProblem, Deflated, StationarityProblem,
+                        _validate_batch,
                         _validate_batch_problem, _validate_workgroupsize, resolve_options,
                         _tally_return_codes, ToReal
 using StaticArrays: SVector, SMatrix, @SMatrix
@@ -470,6 +471,131 @@ function solve_batch!(X::ROCVector{SVector{N, T}}, def::Deflated{<:NonlinearProb
     isempty(X) && return _tally_return_codes(ReturnCode[])
     codes = ROCArray{ReturnCode}(undef, length(X))
     hip_launch_deflated!(X, nothing, codes, def, cd, cm, p, opt, order, ws)
+    return _tally_return_codes(Array(codes))
+end
+
+# ---- the StationarityProblem batch (stationarity.jl:642-698) ----------------------------------
+
+"""
+    HenonHeilesAction()
+
+The two-step discrete action of the variational midpoint rule for Hénon–Heiles on the full state
+`x = (q₋, q, q₊)`: `S(x, (h, λ)) = L_d(q₋, q) + L_d(q, q₊)`, `L_d(a, b) = |b − a|²/(2h) −
+h V((a + b)/2)`, `V(q) = (q₁² + q₂²)/2 + λ (q₁² q₂ − q₂³/3)`. `StationarityProblem(HenonHeilesAction();
+E = (3, 4), U = (5, 6))` gives the discrete Euler–Lagrange equations for `q₊`. The device uses the
+closed forms of ∇S and ∇²S; on the host the reference differentiates this `S` itself.
+"""
+struct HenonHeilesAction end
+
+function (::HenonHeilesAction)(x::SVector{6, T}, p) where {T}
+    h, λ = p
+    V(q1, q2) = (q1^2 + q2^2) / 2 + λ * (q1^2 * q2 - q2^3 / 3)
+    Ld(a1, a2, b1, b2) = ((b1 - a1)^2 + (b2 - a2)^2) / (2h) - h * V((a1 + b1) / 2, (a2 + b2) / 2)
+    return Ld(x[1], x[2], x[3], x[4]) + Ld(x[3], x[4], x[5], x[6])
+end
+
+struct GsStationarity
+    n_eq::Int32
+    b_kind::Int32
+    e::NTuple{8, Int32}
+    u::NTuple{8, Int32}
+    b_shared::NTuple{8, Float64}
+    b_data::Ptr{Cvoid}
+    b_stride::Int64
+end
+
+_stat_indices(::Colon) = ()
+_stat_indices(idx) = Tuple(Int32.(collect(idx)))
+
+# The index sets and the datum as gs_stationarity, or `nothing` where the device form does not hold
+# them (|E| other than 2 or 6, a callable datum): those run the reference's generic batch.
+function device_stationarity(prob::StationarityProblem, b_src, m::Int)
+    m in (2, 6) || return nothing
+    e, u = _stat_indices(prob.E), _stat_indices(prob.U)
+    pad(t) = ntuple(k -> k <= length(t) ? t[k] : Int32(0), 8)
+    kind, shared, data, stride = Int32(0), ntuple(_ -> 0.0, 8), C_NULL, Int64(0)
+    if b_src === nothing
+    elseif b_src isa Number
+        kind, shared = Int32(1), ntuple(k -> k <= m ? Float64(b_src) : 0.0, 8)
+    elseif b_src isa AbstractVector{<:Real}
+        length(b_src) == m || throw(ArgumentError("datum b must have length $m, got $(length(b_src))"))
+        v = Float64.(collect(b_src))
+        kind, shared = Int32(1), ntuple(k -> k <= m ? v[k] : 0.0, 8)
+    elseif b_src isa PerSystem && b_src.data isa ROCVector{<:SVector{M, Float64} where {M}}
+        length(eltype(b_src.data)) == m || throw(ArgumentError(
+            "datum b must have length $m, got $(length(eltype(b_src.data)))"))
+        kind, data, stride = Int32(2), Ptr{Cvoid}(pointer(b_src.data)), Int64(8m)
+    else
+        return nothing
+    end
+    return GsStationarity(Int32(length(e)), kind, pad(e), pad(u), shared, data, stride)
+end
+
+_action_records(p::Tuple{Real, Real}) = (ROCArray([Float64(p[1]), Float64(p[2])]), Int64(0))
+_action_records(p::SVector{2}) = _action_records((p[1], p[2]))
+_action_records(p::PerSystem) = (p.data, Int64(16))
+
+function hip_launch_stationarity!(X, S, codes, cs::GsStationarity, cm::GsMethod, p, opt, order, ws)
+    P, stride = _action_records(p)
+    cf = GsFamily(Int32(8), Int32(6), Int32(6), Int32(2), ntuple(_ -> 0.0, 8))
+    order_dev = order === nothing ? nothing : ROCArray(Int64.(collect(order)))
+    GC.@preserve P order_dev begin
+        rc = ccall((:gs_solve_batch_stationarity_f64, libgsolve), Cint,
+            (Ref{GsMethod}, Ref{GsFamily}, Ref{GsStationarity}, Int64, Ptr{Cvoid}, Ptr{Cvoid}, Int64,
+                Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ref{GsOptionsF64}, Int32, Ptr{Cvoid}),
+            cm, cf, cs, length(X), pointer(X), pointer(P), stride, S === nothing ? C_NULL : pointer(S),
+            codes === nothing ? C_NULL : pointer(codes), order_dev === nothing ? C_NULL : pointer(order_dev),
+            device_options(opt), Int32(something(ws, 0)), stream().stream)
+    end
+    rc == 0 || gs_error(rc)
+    return nothing
+end
+
+served_stationarity(cm) = cm !== nothing && cm.globalization in (0, 1) && cm.reuse == 0
+
+# The status form: X holds each system's full state (SVector{6, Float64}); p = (h, λ) shared or
+# PerSystem of SVector{2, Float64}; the datum from the keyword `b` or the problem's own.
+function solve_batch!(X::ROCVector{SVector{6, Float64}}, S::ROCVector{SolverStatus{Float64}},
+        prob::StationarityProblem{HenonHeilesAction}, method::Newton = Newton(), p = nothing;
+        order::Union{Nothing, AbstractVector{<:Integer}} = nothing,
+        options::Union{Nothing, Options} = nothing, b = nothing,
+        workgroupsize::Union{Nothing, Integer} = nothing, kwargs...)
+    b_src = b === nothing ? prob.b : b
+    cm = device_method(Adapt_real(method))
+    cs = device_stationarity(prob, b_src, prob.E isa Colon ? 6 : length(prob.E))
+    (cs === nothing || !served_stationarity(cm)) && return invoke(solve_batch!,
+        Tuple{AbstractArray, AbstractArray{SolverStatus{Float64}}, StationarityProblem, Any, Any},
+        X, S, prob, method, p; order, options, b, workgroupsize, kwargs...)
+    _validate_batch(X, p, order)
+    b_src isa PerSystem && _validate_batch(X, b_src, nothing)
+    length(S) == length(X) || throw(DimensionMismatch(
+        "status vector S has length $(length(S)), expected $(length(X)) to match the batch X"))
+    ws = _validate_workgroupsize(workgroupsize)
+    opt = resolve_options(Float64, options, kwargs)
+    isempty(X) && return S
+    GC.@preserve b_src hip_launch_stationarity!(X, S, nothing, cs, cm, p, opt, order, ws)
+    return S
+end
+
+# The counts form.
+function solve_batch!(X::ROCVector{SVector{6, Float64}}, prob::StationarityProblem{HenonHeilesAction},
+        method::Newton = Newton(), p = nothing;
+        order::Union{Nothing, AbstractVector{<:Integer}} = nothing,
+        options::Union{Nothing, Options} = nothing, b = nothing,
+        workgroupsize::Union{Nothing, Integer} = nothing, kwargs...)
+    b_src = b === nothing ? prob.b : b
+    cm = device_method(Adapt_real(method))
+    cs = device_stationarity(prob, b_src, prob.E isa Colon ? 6 : length(prob.E))
+    (cs === nothing || !served_stationarity(cm)) && return invoke(solve_batch!,
+        Tuple{AbstractArray, StationarityProblem, Any, Any},
+        X, prob, method, p; order, options, b, workgroupsize, kwargs...)
+    _validate_batch(X, p, order)
+    b_src isa PerSystem && _validate_batch(X, b_src, nothing)
+    ws = _validate_workgroupsize(workgroupsize)
+    opt = resolve_options(Float64, options, kwargs)
+    isempty(X) && return _tally_return_codes(ReturnCode[])
+    codes = ROCArray{ReturnCode}(undef, length(X))
+    GC.@preserve b_src hip_launch_stationarity!(X, nothing, codes, cs, cm, p, opt, order, ws)
     return _tally_return_codes(Array(codes))
 end
 

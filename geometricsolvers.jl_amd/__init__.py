@@ -32,7 +32,7 @@ __all__ = [
     "ArgumentError", "DimensionMismatch", "ReturnCode", "SUCCESS", "STALLED", "MAXITERS",
     "SINGULAR", "NONFINITE", "LINESEARCH_FAILED", "STATUS_DTYPE", "STATUS_DTYPE_F32", "L2Norm", "MaxNorm", "Options",
     "LUFactorization", "Static", "Backtracking", "DogLeg", "LevenbergMarquardt", "NoRefinement", "Refinement", "Newton",
-    "NonlinearProblem", "Deflated", "Coupled2", "Hetero2", "ScalarSquare", "TanhLinear3",
+    "NonlinearProblem", "Deflated", "StationarityProblem", "HenonHeilesAction", "Coupled2", "Hetero2", "ScalarSquare", "TanhLinear3",
     "HenonHeilesMidpoint", "FpuGl3", "PerSystem", "BatchParams", "parameters", "solve_batch",
     "LeastSquaresProblem", "ExpSumFits", "ExpSumData",
     "status_buffer", "statuses", "isconverged", "batch_summary", "BatchSummary", "load_library",
@@ -516,6 +516,115 @@ class FpuGl3(Family):
         return t, t.stride(0) * t.element_size(), (float(p.shared), self.beta)
 
 
+class HenonHeilesAction(_SimpleFamily):
+    """The two-step discrete action of the variational midpoint rule for Hénon–Heiles, an action for
+    StationarityProblem: on the full state x = (q₋, q, q₊) (6 entries),
+    S(x) = L_d(q₋, q) + L_d(q, q₊), L_d(a, b) = |b − a|²/(2h) − h V((a + b)/2),
+    V(q) = (q₁² + q₂²)/2 + λ (q₁² q₂ − q₂³/3). With E = (3, 4) and U = (5, 6) the stationarity
+    equations are the discrete Euler–Lagrange equations for q₊. Parameter p = (h, λ), shared or
+    PerSystem; ∇S and ∇²S are closed forms on the device."""
+
+    id, n, p_len = _abi.GS_ACT_HENON_HEILES, 6, 2
+
+    @property
+    def p_dtype(self):
+        return _torch().float64
+
+
+def _index_set(idx, name):
+    """_normalize_indices (stationarity.jl:60-72): None or ':' is Colon; a tuple of integers is kept."""
+    if idx is None or (isinstance(idx, str) and idx == ":"):
+        return None
+    if isinstance(idx, (tuple, list)):
+        if len(idx) == 0:
+            raise ArgumentError("index set must be non-empty, got ()")
+        if not all(isinstance(i, (int, np.integer)) and not isinstance(i, bool) for i in idx):
+            raise ArgumentError(f"invalid index set type for {name}")
+        return tuple(int(i) for i in idx)
+    raise ArgumentError(f"invalid index set type {type(idx).__name__}")
+
+
+class StationarityProblem:
+    """StationarityProblem(S; E = :, U = :, b = nothing) (src/nonlinear/stationarity.jl:1-118): the
+    residual (∇S(x, p))_E + b on the unknowns x[U] of each system's full state, with the Jacobian
+    (∇²S)_{E,U}. `S` is a registered action (HenonHeilesAction); E and U are None (:) or tuples of
+    1-based indices of one length; b is None, a number, a sequence of |E| numbers, or a PerSystem
+    of those. A batch (solve_batch with this problem) overwrites the entries U of each row of X and
+    keeps the others."""
+
+    def __init__(self, S, E=None, U=None, b=None):
+        if not isinstance(S, HenonHeilesAction):
+            raise ArgumentError("the device StationarityProblem takes a registered action (HenonHeilesAction)")
+        En, Un = _index_set(E, "E"), _index_set(U, "U")
+        if (En is None) != (Un is None):
+            raise ArgumentError("StationarityProblem requires both E and U to be :")
+        if En is not None:
+            if len(En) != len(Un):
+                raise ArgumentError(f"length(E) == length(U) required, got {len(En)} and {len(Un)}")
+            if not all(i >= 1 for i in En):
+                raise ArgumentError(f"indices in E must be >= 1, got {En}")
+            if not all(i >= 1 for i in Un):
+                raise ArgumentError(f"indices in U must be >= 1, got {Un}")
+            if len(set(En)) != len(En):
+                raise ArgumentError("index set E has duplicate indices")
+            if len(set(Un)) != len(Un):
+                raise ArgumentError("index set U has duplicate indices")
+        if callable(b) and not isinstance(b, PerSystem):
+            raise ArgumentError("a callable datum b(p) has no device form: pass its values as PerSystem")
+        self.S, self.E, self.U, self.b = S, En, Un, b
+
+    def n_equations(self, M: int) -> int:
+        return M if self.E is None else len(self.E)
+
+    def to_c(self, M: int, b_kw, nsys: int, device=None):
+        """gs_stationarity for full states of M entries and the datum `b_kw` (or the problem's own):
+        the checks of _validate_stationarity_solve / _validate_stationarity_batch
+        (stationarity.jl:133-157, 600-618). Returns (struct, the datum array to keep alive)."""
+        m = self.n_equations(M)
+        for name, idx in (("E", self.E), ("U", self.U)):
+            if idx is not None and not all(1 <= i <= M for i in idx):
+                raise ArgumentError(f"index in {name} exceeds {M}")
+        if callable(b_kw) and not isinstance(b_kw, PerSystem):
+            raise ArgumentError("a callable datum b(p) has no device form: pass its values as PerSystem")
+        bsrc = self.b if b_kw is None else b_kw
+        c = _abi.GsStationarity()
+        c.n_eq = 0 if self.E is None else len(self.E)
+        for k in range(c.n_eq):
+            c.e[k], c.u[k] = self.E[k], self.U[k]
+        keep = None
+        if bsrc is None:
+            c.b_kind = _abi.GS_DATUM_NONE
+        elif isinstance(bsrc, PerSystem):
+            data = bsrc.data
+            torch = _torch()
+            if device is None:
+                arr = np.ascontiguousarray(np.asarray(data.cpu() if isinstance(data, torch.Tensor) else data,
+                                                      dtype=np.float64).reshape(len(data), -1))
+                ptr, stride, rows, width = arr.ctypes.data, arr.strides[0], arr.shape[0], arr.shape[1]
+                keep = arr
+            else:
+                t = data if isinstance(data, torch.Tensor) else torch.as_tensor(np.asarray(data, dtype=np.float64))
+                t = t.to(device=device, dtype=torch.float64)
+                t = (t.reshape(len(t), -1) if t.dim() > 0 else t).contiguous()
+                ptr, stride, rows, width = t.data_ptr(), t.stride(0) * 8, t.shape[0], t.shape[1]
+                keep = t
+            if rows != nsys:
+                raise DimensionMismatch(f"PerSystem datum has {rows} entries, expected {nsys} to match the batch")
+            if width != m:
+                raise ArgumentError(f"datum b must have length {m}, got {width}")
+            c.b_kind, c.b_data, c.b_stride = _abi.GS_DATUM_PER_SYSTEM, ptr, stride
+        else:
+            v = np.atleast_1d(np.asarray(bsrc, dtype=np.float64))
+            if np.ndim(bsrc) == 0:
+                v = np.full(m, float(bsrc))
+            elif v.size != m:
+                raise ArgumentError(f"datum b must have length {m}, got {v.size}")
+            c.b_kind = _abi.GS_DATUM_SHARED
+            for k in range(m):
+                c.b_shared[k] = float(v[k])
+        return c, keep
+
+
 class Deflated:
     """Deflated(prob, roots; power = 2, shift = 1) (src/nonlinear/deflation.jl:15-235): the problem whose
     residual is M(x) F(x), M(x) = ∏ᵢ (‖x − rᵢ‖^(−p) + σ), with the product-rule Jacobian
@@ -730,7 +839,7 @@ def solve_batch(X, *args, order=None, options: Optional[Options] = None, workgro
     `order` is a permutation of 1:nsys (1-based, as Julia's): work-item i solves system order[i].
     The call is synchronous, as `_launch!` is (batched.jl:193).
     """
-    if args and isinstance(args[0], (NonlinearProblem, LeastSquaresProblem, Deflated)):
+    if args and isinstance(args[0], (NonlinearProblem, LeastSquaresProblem, Deflated, StationarityProblem)):
         S = None
         rest = args
     else:
@@ -747,6 +856,8 @@ def solve_batch(X, *args, order=None, options: Optional[Options] = None, workgro
         raise ArgumentError("too many positional arguments")
     if isinstance(prob, LeastSquaresProblem):
         return _solve_lsq(X, S, prob, method, p, order, options, workgroupsize, kwargs)
+    if isinstance(prob, StationarityProblem):
+        return _solve_stationarity(X, S, prob, method, p, order, options, workgroupsize, kwargs)
     deflation = None
     if isinstance(prob, Deflated):
         deflation, prob = prob, prob.prob
@@ -824,6 +935,55 @@ def solve_batch(X, *args, order=None, options: Optional[Options] = None, workgro
         ctypes.c_void_p(codes.data_ptr()) if codes is not None else None,
         ctypes.c_void_p(order_dev.data_ptr()) if order_dev is not None else None,
         ctypes.byref(copt), ws, _stream_ptr(X))
+    _raise_on(rc, lib)
+    if S is not None:
+        return S
+    counts = (ctypes.c_int64 * 6)()
+    _raise_on(lib.gs_tally_codes(ctypes.c_void_p(codes.data_ptr()), nsys, counts, _stream_ptr(X)), lib)
+    return {k: int(counts[i]) for i, k in enumerate(_CODE_NAMES)}
+
+
+def _solve_stationarity(X, S, prob, method, p, order, options, workgroupsize, kwargs):
+    """solve_batch on a StationarityProblem (stationarity.jl:642-698): each row of X is a full state;
+    the datum comes from the keyword `b` or the problem's own."""
+    b_kw = kwargs.pop("b", None)
+    if not isinstance(method, Newton):
+        raise ArgumentError("method must be a Newton")
+    if isinstance(method.globalization, LevenbergMarquardt):
+        raise ArgumentError("the device StationarityProblem batch takes Static() or Backtracking()")
+    act = prob.S
+    _check_x(X, act.n)
+    order_h = _validate_batch(X, p, order)
+    if isinstance(b_kw if b_kw is not None else prob.b, PerSystem):
+        _validate_batch(X, b_kw if b_kw is not None else prob.b, None)
+    nsys = X.shape[0]
+    if S is not None:
+        _check_status_buffer(S, nsys, X)
+    ws = _validate_workgroupsize(workgroupsize)
+    opt = resolve_options(options, kwargs, "float64")
+    cm = method.to_c()
+    if X.device.type != "cuda":
+        raise ArgumentError(
+            "X must be a device array (a HIP tensor): libgsolve_hip solves on the GPU and has no host path")
+    cs, keep = prob.to_c(act.n, b_kw, nsys, X.device)
+    if nsys == 0:
+        return S if S is not None else {k: 0 for k in _CODE_NAMES}
+    lib = load_library()
+    torch = _torch()
+    P, p_stride, shared = act.marshal(p, nsys, X.device, X.dtype)
+    cf = _abi.GsFamily()
+    cf.id, cf.n, cf.m, cf.p_len = act.id, act.n, act.n, act.p_len
+    order_dev = None if order_h is None else torch.as_tensor(order_h, dtype=torch.int64).to(X.device)
+    codes = torch.empty(nsys, dtype=torch.uint8, device=X.device) if S is None else None
+    copt = opt.to_c()
+    rc = lib.gs_solve_batch_stationarity_f64(
+        ctypes.byref(cm), ctypes.byref(cf), ctypes.byref(cs), nsys, ctypes.c_void_p(X.data_ptr()),
+        ctypes.c_void_p(P.data_ptr()), p_stride,
+        ctypes.c_void_p(S.data_ptr()) if S is not None else None,
+        ctypes.c_void_p(codes.data_ptr()) if codes is not None else None,
+        ctypes.c_void_p(order_dev.data_ptr()) if order_dev is not None else None,
+        ctypes.byref(copt), ws, _stream_ptr(X))
+    del keep
     _raise_on(rc, lib)
     if S is not None:
         return S

@@ -33,6 +33,9 @@ GS_FAM_TANH_LINEAR3 = 4
 GS_FAM_HENON_HEILES_MIDPOINT = 5
 GS_FAM_FPU_GL3 = 6
 GS_FAM_EXPSUM_LSQ = 7
+GS_ACT_HENON_HEILES = 8
+
+GS_DATUM_NONE, GS_DATUM_SHARED, GS_DATUM_PER_SYSTEM = 0, 1, 2
 
 
 class GsStatusF64(ctypes.Structure):
@@ -126,6 +129,20 @@ class GsDeflation(ctypes.Structure):
     ]
 
 
+class GsStationarity(ctypes.Structure):
+    """gs_stationarity: the index sets (1-based; n_eq = 0 for E = U = :) and the datum of a
+    StationarityProblem batch."""
+    _fields_ = [
+        ("n_eq", ctypes.c_int32),
+        ("b_kind", ctypes.c_int32),
+        ("e", ctypes.c_int32 * 8),
+        ("u", ctypes.c_int32 * 8),
+        ("b_shared", ctypes.c_double * 8),
+        ("b_data", ctypes.c_void_p),
+        ("b_stride", ctypes.c_int64),
+    ]
+
+
 # SolverStatus{Float64} as a numpy structured dtype (72 bytes, Julia's field offsets).
 STATUS_DTYPE = np.dtype(
     {
@@ -187,6 +204,7 @@ EXPORTED_SYMBOLS = (
     "gs_shutdown",
     "gs_solve_batch_f32",
     "gs_solve_batch_deflated_f64",
+    "gs_solve_batch_stationarity_f64",
     "gs_lsq_set_shape",
     "gs_lsq_get_shape",
     "gs_lsq_set_graph",
@@ -270,6 +288,11 @@ def load_library(path: os.PathLike | None = None):
     lib.gs_solve_batch_deflated_f64.restype = ctypes.c_int
     lib.gs_solve_batch_deflated_f64.argtypes = [
         ctypes.POINTER(GsMethod), ctypes.POINTER(GsFamily), ctypes.POINTER(GsDeflation), ctypes.c_int64, vp, vp,
+        ctypes.c_int64, vp, vp, vp, ctypes.POINTER(GsOptionsF64), ctypes.c_int32, vp,
+    ]
+    lib.gs_solve_batch_stationarity_f64.restype = ctypes.c_int
+    lib.gs_solve_batch_stationarity_f64.argtypes = [
+        ctypes.POINTER(GsMethod), ctypes.POINTER(GsFamily), ctypes.POINTER(GsStationarity), ctypes.c_int64, vp, vp,
         ctypes.c_int64, vp, vp, vp, ctypes.POINTER(GsOptionsF64), ctypes.c_int32, vp,
     ]
     if lib.gs_version() != ABI_VERSION:

@@ -23,11 +23,24 @@ struct DeflArgs {
     double roots[32];    // root r at roots[r·N ..]
 };
 
+// The index sets and datum of a StationarityProblem batch (gs_stationarity), read by
+// FamStationarityT only: E and U 0-based, the datum none, shared (|E| values) or one record per system.
+struct StatArgs {
+    int32_t e[8];
+    int32_t u[8];
+    int32_t b_kind;        // 0 none, 1 shared, 2 per system
+    int32_t pad_;
+    double b_shared[8];
+    const double* b_data;  // per-system datum records (device)
+    int64_t b_stride;      // bytes between datum records
+};
+
 struct FamArgs {
     const char* P;       // parameter records (device)
     int64_t p_stride;    // bytes between records; 0 = one shared record
     double shared[8];    // scalar parameters shared by every system
     DeflArgs defl;       // the deflation of gs_solve_batch_deflated_f64
+    StatArgs stat;       // the index sets and datum of gs_solve_batch_stationarity_f64
 };
 
 GS_DEV const char* record(const FamArgs& a, int64_t idx) { return a.P + (a.p_stride ? idx * a.p_stride : 0); }
@@ -318,6 +331,187 @@ template <class Inner> struct FamDeflatedT {
         for (int i = 0; i < N; ++i) out[i] = M * jv[i] + gv * Fi[i];
     }
 };
+
+// The discrete action of the variational midpoint rule for Hénon–Heiles over two steps, on the full
+// state x = (q₋, q, q₊) (6 entries): S(x) = L_d(q₋, q) + L_d(q, q₊), L_d(a, b) = |b − a|²/(2h) −
+// h V((a + b)/2), V(q) = (q₁² + q₂²)/2 + λ (q₁² q₂ − q₂³/3) (λ = 1: the Hénon–Heiles potential of
+// the midpoint family). A registered action of StationarityProblem(S; E, U, b): `grad` is ∇S and
+// `hess` ∇²S, in closed form. The record is (h, λ).
+template <class R> struct ActHenonHeilesT {
+    using T = R;
+    static constexpr int M = 6;
+    T h, lam;
+    GS_DEV void load(const FamArgs& a, int64_t idx) {
+        const T* p = reinterpret_cast<const T*>(record(a, idx));
+        h = p[0];
+        lam = p[1];
+    }
+    // ∇V(m)
+    GS_DEV void dV(const T (&m)[2], T (&d)[2]) const {
+        const T l2 = T(2) * lam;
+        d[0] = m[0] + l2 * (m[0] * m[1]);
+        d[1] = m[1] + lam * ((m[0] * m[0]) - (m[1] * m[1]));
+    }
+    GS_DEV void grad(const T (&x)[6], T (&g)[6]) const {
+        const T hh = h / T(2);
+        T m1[2], m2[2], d1[2], d2[2], v1[2], v2[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            v1[i] = x[2 + i] - x[i];
+            v2[i] = x[4 + i] - x[2 + i];
+            m1[i] = (x[i] + x[2 + i]) / T(2);
+            m2[i] = (x[2 + i] + x[4 + i]) / T(2);
+        }
+        dV(m1, d1);
+        dV(m2, d2);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            g[i] = -(v1[i] / h) - hh * d1[i];
+            g[2 + i] = ((v1[i] / h) - hh * d1[i]) - ((v2[i] / h) + hh * d2[i]);
+            g[4 + i] = (v2[i] / h) - hh * d2[i];
+        }
+    }
+    GS_DEV void hess(const T (&x)[6], T (&H)[6][6]) const {
+        const T ih = T(1) / h, hq = h / T(4), l2 = T(2) * lam;
+        T w[2][3];  // ∇²V at the two midpoints: (w00, w01, w11)
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const T m0 = (x[2 * k] + x[2 * k + 2]) / T(2);
+            const T m1 = (x[2 * k + 1] + x[2 * k + 3]) / T(2);
+            w[k][0] = T(1) + l2 * m1;
+            w[k][1] = l2 * m0;
+            w[k][2] = T(1) - l2 * m1;
+        }
+        // P = I/h − (h/4) W, Q = −I/h − (h/4) W, per step k
+        T P[2][3], Q[2][3];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            P[k][0] = ih - hq * w[k][0];
+            P[k][1] = -(hq * w[k][1]);
+            P[k][2] = ih - hq * w[k][2];
+            Q[k][0] = -ih - hq * w[k][0];
+            Q[k][1] = -(hq * w[k][1]);
+            Q[k][2] = -ih - hq * w[k][2];
+        }
+#pragma unroll
+        for (int i = 0; i < 6; ++i)
+#pragma unroll
+            for (int j = 0; j < 6; ++j) H[i][j] = T(0);
+        auto put = [&](int bi, int bj, T a00, T a01, T a11) {
+            H[2 * bi][2 * bj] = a00;
+            H[2 * bi][2 * bj + 1] = a01;
+            H[2 * bi + 1][2 * bj] = a01;
+            H[2 * bi + 1][2 * bj + 1] = a11;
+        };
+        put(0, 0, P[0][0], P[0][1], P[0][2]);
+        put(0, 1, Q[0][0], Q[0][1], Q[0][2]);
+        put(1, 0, Q[0][0], Q[0][1], Q[0][2]);
+        put(1, 1, P[0][0] + P[1][0], P[0][1] + P[1][1], P[0][2] + P[1][2]);
+        put(1, 2, Q[1][0], Q[1][1], Q[1][2]);
+        put(2, 1, Q[1][0], Q[1][1], Q[1][2]);
+        put(2, 2, P[1][0], P[1][1], P[1][2]);
+    }
+};
+
+// StationarityProblem(S; E, U, b) (src/nonlinear/stationarity.jl:1-60) on the unknowns u = x[U] of
+// the full state x of an action Act: F_a(u) = (∇S)(x(u))_{E_a} (+ b_a), J_ac = (∇²S)(x(u))_{E_a, U_c},
+// with x(u)_i = u_c for i = U_c and the start's x_i elsewhere (_embed_unknowns, :209-226); NE = |E|.
+// The kernel loads the full state (`gather`) and writes back only the entries of U (`scatter`), so
+// the others keep their bits. Indices are runtime values: every pick is a chain of selects, so no
+// register array is indexed dynamically.
+template <class Act, int NE> struct FamStationarityT {
+    using T = typename Act::T;
+    static constexpr int N = NE;
+    static constexpr int kStateLen = Act::M;
+    static constexpr int M = Act::M;
+    Act act;
+    T xref[M];
+    // em[k][i]: E_k = i; um[c][i]: U_c = i. Held as flags, not indices: a chain of selects on
+    // `index == i` is rewritten by the compiler into an indexed stack array (scratch memory).
+    bool em[NE][M], um[NE][M];
+    bool has_b;
+    T b[NE];
+    GS_DEV void load(const FamArgs& a, int64_t idx) {
+        act.load(a, idx);
+#pragma unroll
+        for (int k = 0; k < NE; ++k)
+#pragma unroll
+            for (int i = 0; i < M; ++i) {
+                em[k][i] = a.stat.e[k] == i;
+                um[k][i] = a.stat.u[k] == i;
+            }
+        has_b = a.stat.b_kind != 0;
+        if (a.stat.b_kind == 2) {
+            const double* bp =
+                reinterpret_cast<const double*>(reinterpret_cast<const char*>(a.stat.b_data) + idx * a.stat.b_stride);
+#pragma unroll
+            for (int k = 0; k < NE; ++k) b[k] = T(bp[k]);
+        } else {
+            // the kernel argument read at constant offsets (a pointer into it would copy it to scratch)
+#pragma unroll
+            for (int k = 0; k < NE; ++k) b[k] = has_b ? T(a.stat.b_shared[k]) : T(0);
+        }
+    }
+    GS_DEV static T pick(const T (&v)[M], const bool (&m)[M]) {
+        T r = v[0];
+#pragma unroll
+        for (int i = 1; i < M; ++i) r = m[i] ? v[i] : r;
+        return r;
+    }
+    GS_DEV void gather(const T* xp, T (&uu)[NE]) {
+#pragma unroll
+        for (int i = 0; i < M; ++i) xref[i] = xp[i];
+#pragma unroll
+        for (int c = 0; c < NE; ++c) uu[c] = pick(xref, um[c]);
+    }
+    GS_DEV void scatter(T* xp, const T (&uu)[NE]) const {
+        T x[M];
+        embed(uu, x);
+#pragma unroll
+        for (int i = 0; i < M; ++i) xp[i] = x[i];
+    }
+    GS_DEV void embed(const T (&uu)[NE], T (&x)[M]) const {
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            T xi = xref[i];
+#pragma unroll
+            for (int c = 0; c < NE; ++c) xi = um[c][i] ? uu[c] : xi;
+            x[i] = xi;
+        }
+    }
+    GS_DEV void F(const T (&uu)[NE], T (&r)[NE]) const {
+        T x[M], g[M];
+        embed(uu, x);
+        act.grad(x, g);
+#pragma unroll
+        for (int k = 0; k < NE; ++k) {
+            const T gk = pick(g, em[k]);
+            r[k] = has_b ? gk + b[k] : gk;
+        }
+    }
+    GS_DEV void J(const T (&uu)[NE], T (&Jm)[NE][NE]) const {
+        T x[M], H[M][M];
+        embed(uu, x);
+        act.hess(x, H);
+#pragma unroll
+        for (int k = 0; k < NE; ++k) {
+            T row[M];
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                T r = H[0][j];
+#pragma unroll
+                for (int i = 1; i < M; ++i) r = em[k][i] ? H[i][j] : r;
+                row[j] = r;
+            }
+#pragma unroll
+            for (int c = 0; c < NE; ++c) Jm[k][c] = pick(row, um[c]);
+        }
+    }
+};
+
+// Whether a family holds a full state beyond its unknowns (X rows of kStateLen entries).
+template <class F, class = void> struct holds_state : std::false_type {};
+template <class F> struct holds_state<F, std::void_t<decltype(F::kStateLen)>> : std::true_type {};
 
 // Whether a family is a Deflated wrapper (its status reports the inner residual).
 template <class F, class = void> struct is_deflated : std::false_type {};

@@ -105,6 +105,7 @@ static const FamilySpec* family_spec(int id) {
         {GS_FAM_TANH_LINEAR3, 3, 12, 96, 8, 48, 4},
         {GS_FAM_HENON_HEILES_MIDPOINT, 4, 4, 32, 16, 16, 16},
         {GS_FAM_FPU_GL3, 48, 32, 256, 8, 0, 0},
+        {GS_ACT_HENON_HEILES, 6, 2, 16, 8, 0, 0},
     };
     for (const auto& s : specs)
         if (s.id == id) return &s;
@@ -203,7 +204,10 @@ int gs_solve_batch_f64(const gs_method* method, const gs_family* family, int64_t
     g_last_error.clear();
     const FamilySpec* fs = nullptr;
     int rc = validate_batch<double>(method, family, nsys, X, P, p_stride, S, codes, options, workgroupsize, &fs);
-    if (rc != GS_OK || nsys == 0) return rc;
+    if (rc != GS_OK) return rc;
+    if (family->id == GS_ACT_HENON_HEILES)
+        return set_error(GS_EFAMILY, "an action is solved as a StationarityProblem (gs_solve_batch_stationarity_f64)");
+    if (nsys == 0) return rc;
     KernelArgs a;
     std::memset(&a, 0, sizeof(a));
     a.fam.P = static_cast<const char*>(P);
@@ -278,7 +282,7 @@ int gs_solve_batch_deflated_f64(const gs_method* method, const gs_family* family
         return set_error(GS_EINVAL,
                          "Deflated takes no Newton with best = true: the best iterate would be ranked by the deflated "
                          "residual, not by the inner residual that status.fnorm reports");
-    if (family->id == GS_FAM_FPU_GL3)
+    if (family->id == GS_FAM_FPU_GL3 || family->id == GS_ACT_HENON_HEILES)
         return set_error(GS_EFAMILY, "a Deflated batch wraps one of the lane-per-system families (1-5)");
     if (method->globalization == GS_DOGLEG || method->reuse > 0)
         return set_error(GS_EFAMILY, "a Deflated batch takes the Static or Backtracking globalization without reuse");
@@ -306,6 +310,79 @@ int gs_solve_batch_deflated_f64(const gs_method* method, const gs_family* family
     bool timed = timing_begin(st);
     const int glob = method->globalization == GS_BACKTRACKING ? kBacktracking : kStatic;
     rc = launch_small_deflated(family->id, glob, a, mixed, refine, diag, wg, st);
+    return finish_launch(rc, st, timed);
+}
+
+int gs_solve_batch_stationarity_f64(const gs_method* method, const gs_family* action, const gs_stationarity* stat,
+                                    int64_t nsys, double* X, const void* P, int64_t p_stride, gs_status_f64* S,
+                                    uint8_t* codes, const int64_t* order, const gs_options_f64* options,
+                                    int32_t workgroupsize, void* stream) {
+    g_last_error.clear();
+    if (!stat) return set_error(GS_EINVAL, "stat must not be NULL");
+    if (action && action->id != GS_ACT_HENON_HEILES)
+        return set_error(GS_EFAMILY, "a StationarityProblem batch takes a registered action (GS_ACT_HENON_HEILES)");
+    const FamilySpec* fs = nullptr;
+    int rc = validate_batch<double>(method, action, nsys, X, P, p_stride, S, codes, options, workgroupsize, &fs);
+    if (rc != GS_OK) return rc;
+    // the constructor's index-set checks (stationarity.jl:87-103) and the solve's
+    // (_validate_stationarity_solve, :133-157), then the device's own limits
+    const gs_stationarity& t = *stat;
+    const int M = fs->n;
+    if (t.n_eq < 0 || t.n_eq > 8) return set_error(GS_EINVAL, "n_eq must be 0 (E = U = :) or in 1:8");
+    const int ne = t.n_eq == 0 ? M : t.n_eq;
+    for (int k = 0; k < t.n_eq; ++k) {
+        if (t.e[k] < 1) return set_error(GS_EINVAL, "indices in E must be >= 1");
+        if (t.u[k] < 1) return set_error(GS_EINVAL, "indices in U must be >= 1");
+    }
+    for (int k = 0; k < t.n_eq; ++k)
+        for (int j = k + 1; j < t.n_eq; ++j) {
+            if (t.e[k] == t.e[j]) return set_error(GS_EINVAL, "index set E has duplicate indices");
+            if (t.u[k] == t.u[j]) return set_error(GS_EINVAL, "index set U has duplicate indices");
+        }
+    for (int k = 0; k < t.n_eq; ++k)
+        if (t.e[k] > M) return set_error(GS_EINVAL, "index in E exceeds " + std::to_string(M));
+    for (int k = 0; k < t.n_eq; ++k)
+        if (t.u[k] > M) return set_error(GS_EINVAL, "index in U exceeds " + std::to_string(M));
+    if (t.b_kind < GS_DATUM_NONE || t.b_kind > GS_DATUM_PER_SYSTEM) return set_error(GS_EINVAL, "b_kind must be a gs_datum_kind");
+    if (ne != 2 && ne != 6)
+        return set_error(GS_EFAMILY, "the device StationarityProblem batch takes |E| = 2 or 6 equations, got " +
+                                         std::to_string(ne));
+    if (method->globalization == GS_DOGLEG || method->reuse > 0)
+        return set_error(GS_EFAMILY,
+                         "a StationarityProblem batch takes the Static or Backtracking globalization without reuse");
+    if (nsys == 0) return GS_OK;
+    if (t.b_kind == GS_DATUM_PER_SYSTEM) {
+        if (!t.b_data) return set_error(GS_EINVAL, "a per-system datum needs b_data");
+        if ((reinterpret_cast<uintptr_t>(t.b_data) % 8) != 0 || (t.b_stride % 8) != 0 || t.b_stride < 8 * ne)
+            return set_error(GS_EINVAL, "b_data and b_stride must be 8-byte aligned, b_stride at least 8·|E| bytes");
+    }
+    KernelArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.fam.P = static_cast<const char*>(P);
+    a.fam.p_stride = p_stride;
+    for (int k = 0; k < 8; ++k) a.fam.shared[k] = action->shared[k];
+    for (int k = 0; k < ne; ++k) {
+        a.fam.stat.e[k] = t.n_eq == 0 ? k : t.e[k] - 1;
+        a.fam.stat.u[k] = t.n_eq == 0 ? k : t.u[k] - 1;
+        a.fam.stat.b_shared[k] = t.b_shared[k];
+    }
+    a.fam.stat.b_kind = t.b_kind;
+    a.fam.stat.b_data = t.b_data;
+    a.fam.stat.b_stride = t.b_stride;
+    a.nsys = nsys;
+    a.X = X;
+    a.S = S;
+    a.codes = codes;
+    a.order = order;
+    a.q = dev_params(*method, *options);
+    const bool diag = S != nullptr;
+    const int wg = workgroupsize > 0 ? workgroupsize : kSmallMaxWorkgroup;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const bool mixed = method->factor_precision == GS_FACTOR_F32;
+    const bool refine = method->refinement == GS_REFINEMENT;
+    bool timed = timing_begin(st);
+    const int glob = method->globalization == GS_BACKTRACKING ? kBacktracking : kStatic;
+    rc = launch_small_stationarity(action->id, ne, glob, a, mixed, refine, diag, wg, st);
     return finish_launch(rc, st, timed);
 }
 

@@ -75,16 +75,17 @@ GS_DEV void store_status(gs_status_f64* dst, uint8_t code, int32_t iterations, d
 // The largest workgroup of the lane-per-system kernels (their launch bound) and their default.
 constexpr int kSmallMaxWorkgroup = 256;
 
-// Waves per SIMD the register allocator targets: 4 (≤ 128 VGPRs) for the plain FP64 Static form,
-// which fit without scratch; the heavier forms keep the allocator's own choice.
-template <int GLOB, bool MIXED, bool REFINE, bool REUSE> constexpr int small_waves_per_eu() {
-    return (GLOB == kStatic && !MIXED && !REFINE && !REUSE) ? 4 : 1;
+// Waves per SIMD the register allocator targets: 4 (≤ 128 VGPRs) for the plain FP64 Static form of
+// the residual families, which fit without scratch; the heavier forms, and the families that hold a
+// full state (a StationarityProblem's action), keep the allocator's own choice.
+template <class Fam, int GLOB, bool MIXED, bool REFINE, bool REUSE> constexpr int small_waves_per_eu() {
+    return (GLOB == kStatic && !MIXED && !REFINE && !REUSE && !holds_state<Fam>::value) ? 4 : 1;
 }
 
 // One work-item per system (the launch shape of `_solve_kernel!`, batched.jl:171-179).
 template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE>
 __global__ __launch_bounds__(kSmallMaxWorkgroup)
-__attribute__((amdgpu_waves_per_eu(small_waves_per_eu<GLOB, MIXED, REFINE, REUSE>())))
+__attribute__((amdgpu_waves_per_eu(small_waves_per_eu<Fam, GLOB, MIXED, REFINE, REUSE>())))
 void solve_small_kernel(const KernelArgsT<typename Fam::T> a) {
     using T = typename Fam::T;
     using Status = typename std::conditional<sizeof(T) == 8, gs_status_f64, gs_status_f32>::type;
@@ -95,6 +96,21 @@ void solve_small_kernel(const KernelArgsT<typename Fam::T> a) {
     Fam fam;
     fam.load(a.fam, idx);
     T x[N];
+    if constexpr (holds_state<Fam>::value) {
+        // a row of X is the full state; the unknowns are its entries U
+        T* xp = a.X + idx * Fam::kStateLen;
+        fam.gather(xp, x);
+        SmallSolve<Fam, GLOB, MIXED, REFINE, DIAG, REUSE> solver(fam, a.q);
+        auto R = solver.solve(x);
+        fam.scatter(xp, x);
+        if (DIAG) {
+            store_status(static_cast<Status*>(a.S) + idx, R.code, R.iterations, R.fnorm, R.stepnorm, R.failures,
+                         R.promoted, R.gradratio, R.rate, R.contracted, R.reused);
+        } else {
+            a.codes[idx] = R.code;
+        }
+        return;
+    }
     T* xp = a.X + idx * N;
 #pragma unroll
     for (int k = 0; k < N; ++k) x[k] = xp[k];
@@ -161,6 +177,11 @@ int launch_small_dogleg(int family, const KernelArgs& a, bool mixed, bool refine
 // Defined in small_deflated.hip: Static and Backtracking on a Deflated batch (FamDeflatedT).
 int launch_small_deflated(int family, int glob, const KernelArgs& a, bool mixed, bool refine, bool diag, int wg,
                           hipStream_t st);
+
+// Defined in small_stationarity.hip: Static and Backtracking on a StationarityProblem batch of a
+// registered action (FamStationarityT), |E| = 2 or 6.
+int launch_small_stationarity(int action, int ne, int glob, const KernelArgs& a, bool mixed, bool refine, bool diag,
+                              int wg, hipStream_t st);
 
 // Defined in small_f32.hip: every form of a Float32 batch.
 int launch_small_f32(int family, int glob, bool reuse, const KernelArgsT<float>& a, bool refine, bool diag, int wg,

@@ -170,7 +170,10 @@ enum gs_family_id {
     GS_FAM_TANH_LINEAR3 = 4,     /* A_p x + 0.1 tanh(x) − b_p : n = 3, P = A (9, column-major) + b (3)           */
     GS_FAM_HENON_HEILES_MIDPOINT = 5, /* implicit-midpoint step of Hénon–Heiles : n = 4, P = x_k (4), shared[0] = h */
     GS_FAM_FPU_GL3 = 6,          /* Gauss–Legendre s = 3 stage system of a 16-site FPU-β chain : n = 48           */
-    GS_FAM_EXPSUM_LSQ = 7        /* exponential-sum least-squares fit (LevenbergMarquardt)                       */
+    GS_FAM_EXPSUM_LSQ = 7,       /* exponential-sum least-squares fit (LevenbergMarquardt)                       */
+    GS_ACT_HENON_HEILES = 8      /* an action S(x, p), not a residual: the two-step discrete action of the       *
+                                  * variational midpoint rule for Hénon–Heiles, x = (q₋, q, q₊) (6 entries),     *
+                                  * P = (h, λ); taken by gs_solve_batch_stationarity_f64 only                   */
 };
 
 typedef struct gs_family {
@@ -225,6 +228,35 @@ int gs_solve_batch_deflated_f64(const gs_method* method, const gs_family* family
                                 int64_t nsys, double* X, const void* P, int64_t p_stride, gs_status_f64* S,
                                 uint8_t* codes, const int64_t* order, const gs_options_f64* options,
                                 int32_t workgroupsize, void* stream);
+
+/* StationarityProblem(S; E, U, b) (src/nonlinear/stationarity.jl:1-60): the residual (∇S)_E + b on
+ * the unknowns x[U] of each system's full state, the Jacobian (∇²S)_{E,U}. Indices are 1-based, as
+ * the reference's; n_eq = 0 stands for E = U = : (every entry). */
+enum gs_datum_kind {
+    GS_DATUM_NONE = 0,          /* b = nothing                                                       */
+    GS_DATUM_SHARED = 1,        /* one datum for every system: b_shared[0 .. |E|-1] (a Number repeated) */
+    GS_DATUM_PER_SYSTEM = 2     /* PerSystem(b): system i reads |E| doubles at b_data + i·b_stride bytes */
+};
+
+typedef struct gs_stationarity {
+    int32_t n_eq;               /* |E| = |U|, or 0 for E = U = :                                      */
+    int32_t b_kind;             /* gs_datum_kind                                                      */
+    int32_t e[8];               /* E (1-based), n_eq entries                                          */
+    int32_t u[8];               /* U (1-based), n_eq entries                                          */
+    double b_shared[8];
+    const double* b_data;       /* device                                                             */
+    int64_t b_stride;           /* bytes between the datum records of two systems                     */
+} gs_stationarity;
+
+/* solve_batch! on a StationarityProblem (stationarity.jl:642-698, both forms): X holds each system's
+ * full state (M doubles, M = 6 for GS_ACT_HENON_HEILES); the solve iterates the entries U and writes
+ * them back, the others keep their bits; S or codes as for gs_solve_batch_f64. Served: the registered
+ * actions, |E| = 2 or 6 (GS_EFAMILY otherwise), LUFactorization (Float64 or Float32 factor),
+ * Static and Backtracking, refinement, regularization, best. */
+int gs_solve_batch_stationarity_f64(const gs_method* method, const gs_family* action, const gs_stationarity* stat,
+                                    int64_t nsys, double* X, const void* P, int64_t p_stride, gs_status_f64* S,
+                                    uint8_t* codes, const int64_t* order, const gs_options_f64* options,
+                                    int32_t workgroupsize, void* stream);
 
 /* The Float32 twin of gs_solve_batch_f64: X holds nsys × n floats (Vector{SVector{n,Float32}}), the
  * parameter records hold floats (Int64 codes for family 2), S receives SolverStatus{Float32}. The

@@ -187,6 +187,41 @@ static int solve_small_batch(const Method<R>& m, const Options<R>& o, const gs_f
     return GS_EFAMILY;
 }
 
+// A StationarityProblem batch over the action of `make`: each system's row of X is its full state
+// (A::M entries); the solve runs on the unknowns x[U] and writes back the full state x(u*)
+// (stationarity.jl:530-540).
+template <int N, class SS, class MakeAct>
+static void run_batch_stationarity(const Method<double>& m, const Options<double>& o, const gs_stationarity& t,
+                                   int64_t nsys, double* X, SS* S, uint8_t* codes, const int64_t* order, int nthreads,
+                                   MakeAct make) {
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+    for (int64_t i = 0; i < nsys; ++i) {
+        const int64_t idx = order ? order[i] - 1 : i;
+        using Act = decltype(make(idx));
+        constexpr int M = Act::M;
+        StationarityProb<double, N, Act> prob{make(idx)};
+        for (int k = 0; k < M; ++k) prob.xref[k] = X[idx * M + k];
+        for (int k = 0; k < N; ++k) {
+            prob.e[k] = t.n_eq == 0 ? k : t.e[k] - 1;
+            prob.u[k] = t.n_eq == 0 ? k : t.u[k] - 1;
+        }
+        prob.has_b = t.b_kind != GS_DATUM_NONE;
+        if (prob.has_b) {
+            const double* bp = t.b_kind == GS_DATUM_PER_SYSTEM
+                                   ? reinterpret_cast<const double*>(reinterpret_cast<const char*>(t.b_data) + idx * t.b_stride)
+                                   : t.b_shared;
+            for (int k = 0; k < N; ++k) prob.b[k] = bp[k];
+        }
+        Vec<double, N> u = prob.extract(prob.xref);
+        Solver<double, N, decltype(prob)> solver(prob, m, o);
+        Status<double> st = solver.solve(u);
+        const Vec<double, M> x = solver.prob.embed(u);
+        for (int k = 0; k < M; ++k) X[idx * M + k] = x[k];
+        if (S) write_status(S + idx, st);
+        if (codes) codes[idx] = st.code;
+    }
+}
+
 }  // namespace orc
 
 using namespace orc;
@@ -233,6 +268,44 @@ int orc_solve_batch_deflated_f64(const gs_method* method, const gs_family* fam, 
     Method<double> m = method_from<double>(*method);
     Options<double> o = options_from<double>(*options);
     return solve_small_batch<double>(m, o, fam, nsys, X, P, p_stride, S, codes, order, nthreads, defl);
+}
+
+// The host restatement of gs_solve_batch_stationarity_f64 (include/gsolve.h), |E| = 2 or 6.
+int orc_solve_batch_stationarity_f64(const gs_method* method, const gs_family* act, const gs_stationarity* stat,
+                                     int64_t nsys, double* X, const void* P, int64_t p_stride, gs_status_f64* S,
+                                     uint8_t* codes, const int64_t* order, const gs_options_f64* options,
+                                     int nthreads) {
+    if (!method || !act || !options || !stat || nsys < 0 || act->id != GS_ACT_HENON_HEILES) return GS_EINVAL;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    Method<double> m = method_from<double>(*method);
+    Options<double> o = options_from<double>(*options);
+    const char* Pb = static_cast<const char*>(P);
+    auto make = [&](int64_t idx) {
+        const double* p = reinterpret_cast<const double*>(Pb + (p_stride ? idx * p_stride : 0));
+        return HenonHeilesAction<double>{p[0], p[1]};
+    };
+    const int ne = stat->n_eq == 0 ? 6 : stat->n_eq;
+    if (ne == 2)
+        run_batch_stationarity<2>(m, o, *stat, nsys, X, S, codes, order, nthreads, make);
+    else if (ne == 6)
+        run_batch_stationarity<6>(m, o, *stat, nsys, X, S, codes, order, nthreads, make);
+    else
+        return GS_EFAMILY;
+    return 0;
+}
+
+// ∇S (6) and ∇²S (6×6, row-major) of the Hénon–Heiles discrete action at x, for the KATs.
+void orc_hh_action_eval(double h, double lam, const double* x, double* g, double* H) {
+    HenonHeilesAction<double> a{h, lam};
+    Vec<double, 6> xv;
+    for (int k = 0; k < 6; ++k) xv[k] = x[k];
+    const Vec<double, 6> gv = a.grad(xv);
+    double Hm[6][6];
+    a.hess(xv, Hm);
+    for (int i = 0; i < 6; ++i) {
+        g[i] = gv[i];
+        for (int j = 0; j < 6; ++j) H[6 * i + j] = Hm[i][j];
+    }
 }
 
 // Julia's Float64 x^n for an integer n (the deflation's power), for the KATs.

@@ -277,4 +277,110 @@ template <class T> struct DogLegPw {
     }
 };
 
+// The two-step discrete action of the variational midpoint rule for Hénon–Heiles, a registered
+// action of StationarityProblem: x = (q₋, q, q₊) (6 entries), S(x) = L_d(q₋, q) + L_d(q, q₊),
+// L_d(a, b) = |b − a|²/(2h) − h V((a + b)/2), V(q) = (q₁² + q₂²)/2 + λ (q₁² q₂ − q₂³/3). ∇S and ∇²S
+// in closed form (the reference differentiates S by nested ForwardDiff duals, hessian.jl:74-99; the
+// closed forms agree with those to rounding, and the device evaluates these same expressions).
+template <class T> struct HenonHeilesAction {
+    static constexpr int M = 6;
+    T h, lam;
+    void dV(const T (&m)[2], T (&d)[2]) const {
+        const T l2 = T(2) * lam;
+        d[0] = m[0] + l2 * (m[0] * m[1]);
+        d[1] = m[1] + lam * ((m[0] * m[0]) - (m[1] * m[1]));
+    }
+    Vec<T, 6> grad(const Vec<T, 6>& x) const {
+        const T hh = h / T(2);
+        T m1[2], m2[2], d1[2], d2[2], v1[2], v2[2];
+        for (int i = 0; i < 2; ++i) {
+            v1[i] = x[2 + i] - x[i];
+            v2[i] = x[4 + i] - x[2 + i];
+            m1[i] = (x[i] + x[2 + i]) / T(2);
+            m2[i] = (x[2 + i] + x[4 + i]) / T(2);
+        }
+        dV(m1, d1);
+        dV(m2, d2);
+        Vec<T, 6> g;
+        for (int i = 0; i < 2; ++i) {
+            g[i] = -(v1[i] / h) - hh * d1[i];
+            g[2 + i] = ((v1[i] / h) - hh * d1[i]) - ((v2[i] / h) + hh * d2[i]);
+            g[4 + i] = (v2[i] / h) - hh * d2[i];
+        }
+        return g;
+    }
+    // H[i][j] = ∂²S/∂xᵢ∂xⱼ: blocks P = I/h − (h/4) W and Q = −I/h − (h/4) W of each step, W = ∇²V
+    // at the step's midpoint; H = [P₁ Q₁ 0; Q₁ P₁+P₂ Q₂; 0 Q₂ P₂].
+    void hess(const Vec<T, 6>& x, T (&H)[6][6]) const {
+        const T ih = T(1) / h, hq = h / T(4), l2 = T(2) * lam;
+        T P[2][3], Q[2][3];
+        for (int k = 0; k < 2; ++k) {
+            const T m0 = (x[2 * k] + x[2 * k + 2]) / T(2);
+            const T m1 = (x[2 * k + 1] + x[2 * k + 3]) / T(2);
+            const T w0 = T(1) + l2 * m1, w1 = l2 * m0, w2 = T(1) - l2 * m1;
+            P[k][0] = ih - hq * w0;
+            P[k][1] = -(hq * w1);
+            P[k][2] = ih - hq * w2;
+            Q[k][0] = -ih - hq * w0;
+            Q[k][1] = -(hq * w1);
+            Q[k][2] = -ih - hq * w2;
+        }
+        for (int i = 0; i < 6; ++i)
+            for (int j = 0; j < 6; ++j) H[i][j] = T(0);
+        auto put = [&](int bi, int bj, T a00, T a01, T a11) {
+            H[2 * bi][2 * bj] = a00;
+            H[2 * bi][2 * bj + 1] = a01;
+            H[2 * bi + 1][2 * bj] = a01;
+            H[2 * bi + 1][2 * bj + 1] = a11;
+        };
+        put(0, 0, P[0][0], P[0][1], P[0][2]);
+        put(0, 1, Q[0][0], Q[0][1], Q[0][2]);
+        put(1, 0, Q[0][0], Q[0][1], Q[0][2]);
+        put(1, 1, P[0][0] + P[1][0], P[0][1] + P[1][1], P[0][2] + P[1][2]);
+        put(1, 2, Q[1][0], Q[1][1], Q[1][2]);
+        put(2, 1, Q[1][0], Q[1][1], Q[1][2]);
+        put(2, 2, P[1][0], P[1][1], P[1][2]);
+    }
+};
+
+// The reduced problem of a StationarityProblem (_ReducedStationarityProblem, stationarity.jl:305-
+// 324) over the action A: the unknowns u = x[U] of the start x₀ (_extract_unknowns, :209-212), the
+// full state x(u) with x_{U_c} = u_c and x₀ elsewhere (_embed_unknowns, :221-226), the residual
+// (∇S)_E + b, b added only where a datum is given (_stationarity_residual_r1, hessian.jl:47-63),
+// and the Jacobian (∇²S)_{E,U} (hessian.jl:74-99). e and u are 0-based.
+template <class R, int N, class A> struct StationarityProb {
+    static constexpr int M = A::M;
+    A act;
+    Vec<R, M> xref;
+    int e[N], u[N];
+    bool has_b = false;
+    Vec<R, N> b{};
+    long nf = 0;
+    Vec<R, M> embed(const Vec<R, N>& uu) const {
+        Vec<R, M> x = xref;
+        for (int c = 0; c < N; ++c) x[u[c]] = uu[c];
+        return x;
+    }
+    Vec<R, N> extract(const Vec<R, M>& x) const {
+        Vec<R, N> uu;
+        for (int c = 0; c < N; ++c) uu[c] = x[u[c]];
+        return uu;
+    }
+    Vec<R, N> F(const Vec<R, N>& uu) {
+        ++nf;
+        const Vec<R, M> g = act.grad(embed(uu));
+        Vec<R, N> f;
+        for (int a = 0; a < N; ++a) f[a] = has_b ? g[e[a]] + b[a] : g[e[a]];
+        return f;
+    }
+    Mat<R, N> J(const Vec<R, N>& uu) const {
+        R H[M][M];
+        act.hess(embed(uu), H);
+        Mat<R, N> Jm;
+        for (int a = 0; a < N; ++a)
+            for (int c = 0; c < N; ++c) Jm[a + c * N] = H[e[a]][u[c]];
+        return Jm;
+    }
+};
+
 }  // namespace orc

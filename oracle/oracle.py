@@ -37,6 +37,11 @@ def lib():
         L.orc_solve_batch_deflated_f64.restype = ctypes.c_int
         L.orc_solve_batch_deflated_f64.argtypes = [vp, vp, vp, ctypes.c_int64, vp, vp, ctypes.c_int64, vp, vp, vp, vp,
                                                    ctypes.c_int]
+        L.orc_solve_batch_stationarity_f64.restype = ctypes.c_int
+        L.orc_solve_batch_stationarity_f64.argtypes = [vp, vp, vp, ctypes.c_int64, vp, vp, ctypes.c_int64, vp, vp, vp,
+                                                       vp, ctypes.c_int]
+        L.orc_hh_action_eval.restype = None
+        L.orc_hh_action_eval.argtypes = [ctypes.c_double, ctypes.c_double, vp, vp, vp]
         L.orc_pow_int.restype = None
         L.orc_pow_int.argtypes = [vp, ctypes.c_int64, vp, ctypes.c_int64]
         L.orc_standard_cell.restype = ctypes.c_int
@@ -102,13 +107,18 @@ def host_records(family, p, nsys: int, real=np.float64):
     return rec, 0, ()
 
 
-def solve_batch(method, family, X: np.ndarray, p, options, order=None, nthreads: int = 0, deflated=None):
+def solve_batch(method, family, X: np.ndarray, p, options, order=None, nthreads: int = 0, deflated=None,
+                stationarity=None, b=None):
     """The oracle's batched solve on host arrays: returns (X, statuses) with X overwritten.
 
     `method`, `options`: objects with `to_c()` giving the ABI structs of include/gsolve.h;
     `family`: a registered family (id, n, p_len); `order`: 1-based permutation or None;
-    `deflated`: a Deflated (its roots, power and shift; its problem is `family`'s) or None.
+    `deflated`: a Deflated (its roots, power and shift; its problem is `family`'s) or None;
+    `stationarity`: a StationarityProblem over the action `family` (X rows are full states), with
+    the datum `b` overriding its own, or None.
     """
+    if stationarity is not None:
+        return _solve_stationarity(method, family, X, p, options, order, nthreads, stationarity, b)
     f32 = np.asarray(X).dtype == np.float32
     real = np.float32 if f32 else np.float64
     X = np.ascontiguousarray(X, dtype=real)
@@ -132,6 +142,32 @@ def solve_batch(method, family, X: np.ndarray, p, options, order=None, nthreads:
     if rc != 0:
         raise RuntimeError(f"oracle failed with {rc}")
     return X, S
+
+
+def _solve_stationarity(method, family, X, p, options, order, nthreads, prob, b):
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    nsys = X.shape[0]
+    rec, stride, shared = host_records(family, p, nsys, np.float64)
+    cm, co = method.to_c(), options.to_c()
+    fam = _family_struct(family, shared)
+    cs, keep = prob.to_c(family.n, b, nsys)
+    S = np.zeros(nsys, dtype=_status_dtype())
+    o = None if order is None else np.ascontiguousarray(np.asarray(order, dtype=np.int64))
+    rc = lib().orc_solve_batch_stationarity_f64(
+        ctypes.byref(cm), ctypes.byref(fam), ctypes.byref(cs), nsys, _ptr(X), _ptr(rec), stride, _ptr(S), None,
+        None if o is None else _ptr(o), ctypes.byref(co), nthreads)
+    del keep
+    if rc != 0:
+        raise RuntimeError(f"oracle failed with {rc}")
+    return X, S
+
+
+def hh_action_eval(h: float, lam: float, x):
+    """∇S and ∇²S of the Hénon–Heiles two-step discrete action at x (6 entries)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    g, H = np.zeros(6), np.zeros((6, 6))
+    lib().orc_hh_action_eval(h, lam, _ptr(x), _ptr(g), _ptr(H))
+    return g, H
 
 
 def _status_dtype():
