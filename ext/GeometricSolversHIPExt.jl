@@ -22,6 +22,7 @@ using GeometricSolvers
 using GeometricSolvers: Options, SolverStatus, ReturnCode, L2Norm, MaxNorm, Static, Backtracking, DogLeg,
                         Newton, NoRefinement, Refinement, LUFactorization, NonlinearProblem, PerSystem,
                         BatchParams, LevenbergMarquardt, LeastSquaresProblem, Deflated, StationarityProblem,
+                        Reparametrized,
                         _validate_batch,
                         _validate_batch_problem, _validate_workgroupsize, resolve_options,
                         _tally_return_codes, ToReal
@@ -471,6 +472,99 @@ function solve_batch!(X::ROCVector{SVector{N, T}}, def::Deflated{<:NonlinearProb
     isempty(X) && return _tally_return_codes(ReturnCode[])
     codes = ROCArray{ReturnCode}(undef, length(X))
     hip_launch_deflated!(X, nothing, codes, def, cd, cm, p, opt, order, ws)
+    return _tally_return_codes(Array(codes))
+end
+
+# ---- the Reparametrized batch (reparametrized.jl:981-1031) -------------------------------------
+
+"""
+    LinearChart(B::SMatrix{N, N, Float64}, c::SVector{N, Float64})
+    PolarChart()
+
+The registered charts of a device `Reparametrized` batch: `x = B y + c` (`∂g = B`), and the polar
+chart `x = (r cos θ, r sin θ)` of the reference's tests (n = 2). Both are callables `g(y, p)`, so a
+`Reparametrized(prob, LinearChart(B, c))` is also what the reference's generic path runs.
+"""
+struct LinearChart{N, L}
+    B::SMatrix{N, N, Float64, L}
+    c::SVector{N, Float64}
+end
+(g::LinearChart)(y, p) = g.B * y + g.c
+struct PolarChart end
+(::PolarChart)(y, p) = SVector(y[1] * cos(y[2]), y[1] * sin(y[2]))
+
+struct GsChart
+    kind::Int32
+    pad::Int32
+    B::NTuple{16, Float64}
+    c::NTuple{4, Float64}
+end
+
+device_chart(g::LinearChart{N}) where {N} = GsChart(Int32(1), Int32(0),
+    ntuple(k -> k <= N * N ? g.B[(k - 1) ÷ N + 1, (k - 1) % N + 1] : 0.0, 16),
+    ntuple(k -> k <= N ? g.c[k] : 0.0, 4))
+device_chart(::PolarChart) = GsChart(Int32(2), Int32(0), ntuple(_ -> 0.0, 16), ntuple(_ -> 0.0, 4))
+
+const HIPReparam = Reparametrized{<:NonlinearProblem{<:HIPFamily}, <:Union{LinearChart, PolarChart}}
+
+served_reparam(rep, cm) = !rep.covariant && !(rep.prob.F isa FpuGl3) && cm !== nothing &&
+    cm.globalization in (0, 1) && cm.reuse == 0 && (!(rep.g isa PolarChart) || family_n(rep.prob.F) == 2)
+
+function hip_launch_reparam!(Y, S, codes, rep, cm::GsMethod, p, opt, order, ws)
+    fam = rep.prob.F
+    P, stride, shared, root = family_records(fam, p, Float64)
+    cf = GsFamily(family_id(fam), Int32(family_n(fam)), Int32(family_n(fam)), Int32(0),
+        ntuple(k -> k <= length(shared) ? shared[k] : 0.0, 8))
+    order_dev = order === nothing ? nothing : ROCArray(Int64.(collect(order)))
+    GC.@preserve root order_dev begin
+        rc = ccall((:gs_solve_batch_reparametrized_f64, libgsolve), Cint,
+            (Ref{GsMethod}, Ref{GsFamily}, Ref{GsChart}, Int64, Ptr{Cvoid}, Ptr{Cvoid}, Int64,
+                Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ref{GsOptionsF64}, Int32, Ptr{Cvoid}),
+            cm, cf, device_chart(rep.g), length(Y), pointer(Y), P, stride, S === nothing ? C_NULL : pointer(S),
+            codes === nothing ? C_NULL : pointer(codes), order_dev === nothing ? C_NULL : pointer(order_dev),
+            device_options(opt), Int32(something(ws, 0)), stream().stream)
+    end
+    rc == 0 || gs_error(rc)
+    return nothing
+end
+
+# The status form: Y holds y₀ and receives x* = g(y*).
+function solve_batch!(Y::ROCVector{SVector{N, Float64}}, S::ROCVector{SolverStatus{Float64}},
+        rep::HIPReparam, method::Newton = Newton(), p = nothing;
+        order::Union{Nothing, AbstractVector{<:Integer}} = nothing,
+        options::Union{Nothing, Options} = nothing,
+        workgroupsize::Union{Nothing, Integer} = nothing, kwargs...) where {N}
+    cm = device_method(Adapt_real(method))
+    served_reparam(rep, cm) || return invoke(solve_batch!,
+        Tuple{AbstractVector{<:Union{SVector, Number}}, AbstractVector{SolverStatus{Float64}}, Reparametrized, Any, Any},
+        Y, S, rep, method, p; order, options, workgroupsize, kwargs...)
+    length(S) == length(Y) || throw(DimensionMismatch(
+        "status vector length $(length(S)) does not match batch length $(length(Y))"))
+    _validate_batch(Y, p, order)
+    _validate_batch_problem(rep)
+    ws = _validate_workgroupsize(workgroupsize)
+    opt = resolve_options(Float64, options, kwargs)
+    isempty(Y) && return S
+    hip_launch_reparam!(Y, S, nothing, rep, cm, p, opt, order, ws)
+    return S
+end
+
+# The counts form.
+function solve_batch!(Y::ROCVector{SVector{N, Float64}}, rep::HIPReparam, method::Newton = Newton(), p = nothing;
+        order::Union{Nothing, AbstractVector{<:Integer}} = nothing,
+        options::Union{Nothing, Options} = nothing,
+        workgroupsize::Union{Nothing, Integer} = nothing, kwargs...) where {N}
+    cm = device_method(Adapt_real(method))
+    served_reparam(rep, cm) || return invoke(solve_batch!,
+        Tuple{AbstractVector{<:Union{SVector, Number}}, Reparametrized, Any, Any},
+        Y, rep, method, p; order, options, workgroupsize, kwargs...)
+    _validate_batch(Y, p, order)
+    _validate_batch_problem(rep)
+    ws = _validate_workgroupsize(workgroupsize)
+    opt = resolve_options(Float64, options, kwargs)
+    isempty(Y) && return _tally_return_codes(ReturnCode[])
+    codes = ROCArray{ReturnCode}(undef, length(Y))
+    hip_launch_reparam!(Y, nothing, codes, rep, cm, p, opt, order, ws)
     return _tally_return_codes(Array(codes))
 end
 

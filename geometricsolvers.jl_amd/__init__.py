@@ -32,7 +32,8 @@ __all__ = [
     "ArgumentError", "DimensionMismatch", "ReturnCode", "SUCCESS", "STALLED", "MAXITERS",
     "SINGULAR", "NONFINITE", "LINESEARCH_FAILED", "STATUS_DTYPE", "STATUS_DTYPE_F32", "L2Norm", "MaxNorm", "Options",
     "LUFactorization", "Static", "Backtracking", "DogLeg", "LevenbergMarquardt", "NoRefinement", "Refinement", "Newton",
-    "NonlinearProblem", "Deflated", "StationarityProblem", "HenonHeilesAction", "Coupled2", "Hetero2", "ScalarSquare", "TanhLinear3",
+    "NonlinearProblem", "Deflated", "StationarityProblem", "HenonHeilesAction", "Reparametrized", "LinearChart",
+    "PolarChart", "Coupled2", "Hetero2", "ScalarSquare", "TanhLinear3",
     "HenonHeilesMidpoint", "FpuGl3", "PerSystem", "BatchParams", "parameters", "solve_batch",
     "LeastSquaresProblem", "ExpSumFits", "ExpSumData",
     "status_buffer", "statuses", "isconverged", "batch_summary", "BatchSummary", "load_library",
@@ -625,6 +626,68 @@ class StationarityProblem:
         return c, keep
 
 
+class LinearChart:
+    """The chart x = B y + c (B an n × n matrix, c an n-vector, shared by the batch), ∂g = B."""
+
+    def __init__(self, B, c=None):
+        B = np.atleast_2d(np.asarray(B, dtype=np.float64))
+        if B.ndim != 2 or B.shape[0] != B.shape[1] or B.shape[0] > 4:
+            raise ArgumentError("a LinearChart takes a square B of order 1 to 4")
+        c = np.zeros(B.shape[0]) if c is None else np.atleast_1d(np.asarray(c, dtype=np.float64))
+        if c.shape != (B.shape[0],):
+            raise ArgumentError(f"a LinearChart takes c of length {B.shape[0]}")
+        if not (np.isfinite(B).all() and np.isfinite(c).all()):
+            raise ArgumentError("the linear chart needs a finite B and c")
+        self.B, self.c = B, c
+
+    def __call__(self, y):
+        return self.B @ np.asarray(y, dtype=np.float64) + self.c
+
+
+class PolarChart:
+    """The polar chart of the reference's tests (test/nonlinear/reparametrized.jl:31-38):
+    x = (r cos θ, r sin θ), y = (r, θ), ∂g = [cos θ, −r sin θ; sin θ, r cos θ]."""
+
+    def __call__(self, y):
+        return np.array([y[0] * math.cos(y[1]), y[0] * math.sin(y[1])])
+
+
+class Reparametrized:
+    """Reparametrized(prob, g, ∂g = nothing; covariant = false) (src/nonlinear/reparametrized.jl:1-60)
+    over a registered chart: Newton iterates the chart coordinates y with F̃(y) = F(g(y)) and
+    J̃ = J(g(y)) ∂g(y); a batch takes y₀ in X and receives x* = g(y*). `g` is a LinearChart or the
+    PolarChart (n = 2), each carrying its Jacobian; `covariant = True` has no device form."""
+
+    def __init__(self, prob, g, dg=None, covariant: bool = False, curvature: bool = True):
+        if isinstance(prob, Reparametrized):
+            raise ArgumentError(
+                "Reparametrized cannot wrap a Reparametrized; compose the charts into one Reparametrized, with "
+                "the chart g₁∘g₂ and the chart Jacobian ∂g₁(g₂)·∂g₂")
+        if not isinstance(prob, NonlinearProblem):
+            raise ArgumentError("the device Reparametrized wraps a NonlinearProblem of a registered family")
+        if not isinstance(g, (LinearChart, PolarChart)):
+            raise ArgumentError("the device Reparametrized takes a registered chart (LinearChart or PolarChart)")
+        if covariant:
+            raise ArgumentError("the device Reparametrized takes covariant = false (F̃ = F∘g)")
+        self.prob, self.g, self.dg, self.covariant, self.curvature = prob, g, dg, False, curvature
+
+    def to_c(self, n: int):
+        c = _abi.GsChart()
+        if isinstance(self.g, PolarChart):
+            if n != 2:
+                raise ArgumentError("the polar chart maps 2 coordinates: families 1 and 2")
+            c.kind = _abi.GS_CHART_POLAR
+            return c
+        if self.g.B.shape[0] != n:
+            raise DimensionMismatch(f"the chart maps {self.g.B.shape[0]} coordinates, the family has {n}")
+        c.kind = _abi.GS_CHART_LINEAR
+        for i in range(n):
+            c.c[i] = float(self.g.c[i])
+            for j in range(n):
+                c.B[i * n + j] = float(self.g.B[i, j])
+        return c
+
+
 class Deflated:
     """Deflated(prob, roots; power = 2, shift = 1) (src/nonlinear/deflation.jl:15-235): the problem whose
     residual is M(x) F(x), M(x) = ∏ᵢ (‖x − rᵢ‖^(−p) + σ), with the product-rule Jacobian
@@ -839,7 +902,8 @@ def solve_batch(X, *args, order=None, options: Optional[Options] = None, workgro
     `order` is a permutation of 1:nsys (1-based, as Julia's): work-item i solves system order[i].
     The call is synchronous, as `_launch!` is (batched.jl:193).
     """
-    if args and isinstance(args[0], (NonlinearProblem, LeastSquaresProblem, Deflated, StationarityProblem)):
+    if args and isinstance(args[0], (NonlinearProblem, LeastSquaresProblem, Deflated, StationarityProblem,
+                                     Reparametrized)):
         S = None
         rest = args
     else:
@@ -858,7 +922,9 @@ def solve_batch(X, *args, order=None, options: Optional[Options] = None, workgro
         return _solve_lsq(X, S, prob, method, p, order, options, workgroupsize, kwargs)
     if isinstance(prob, StationarityProblem):
         return _solve_stationarity(X, S, prob, method, p, order, options, workgroupsize, kwargs)
-    deflation = None
+    deflation = reparam = None
+    if isinstance(prob, Reparametrized):
+        reparam, prob = prob, prob.prob
     if isinstance(prob, Deflated):
         deflation, prob = prob, prob.prob
         if isinstance(method, Newton) and method.best:
@@ -876,7 +942,11 @@ def solve_batch(X, *args, order=None, options: Optional[Options] = None, workgro
     fam = prob.F
     _check_x(X, fam.n, allow_f32=True)
     elt = _eltype(X)
-    cdefl = None
+    cdefl = cchart = None
+    if reparam is not None:
+        if elt != "float64":
+            raise ArgumentError("the device Reparametrized batch is Float64")
+        cchart = reparam.to_c(fam.n)
     if deflation is not None:
         if elt != "float64":
             raise ArgumentError(
@@ -922,7 +992,10 @@ def solve_batch(X, *args, order=None, options: Optional[Options] = None, workgro
     if S is None:
         codes = torch.empty(nsys, dtype=torch.uint8, device=X.device)
     copt = opt.to_c()
-    if cdefl is not None:
+    if cchart is not None:
+        entry = functools.partial(lib.gs_solve_batch_reparametrized_f64, ctypes.byref(cm), ctypes.byref(cf),
+                                  ctypes.byref(cchart))
+    elif cdefl is not None:
         entry = functools.partial(lib.gs_solve_batch_deflated_f64, ctypes.byref(cm), ctypes.byref(cf),
                                   ctypes.byref(cdefl))
     else:

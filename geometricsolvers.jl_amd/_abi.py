@@ -36,6 +36,7 @@ GS_FAM_EXPSUM_LSQ = 7
 GS_ACT_HENON_HEILES = 8
 
 GS_DATUM_NONE, GS_DATUM_SHARED, GS_DATUM_PER_SYSTEM = 0, 1, 2
+GS_CHART_LINEAR, GS_CHART_POLAR = 1, 2
 
 
 class GsStatusF64(ctypes.Structure):
@@ -143,6 +144,16 @@ class GsStationarity(ctypes.Structure):
     ]
 
 
+class GsChart(ctypes.Structure):
+    """gs_chart: a registered chart of a Reparametrized batch (linear: B row-major and c)."""
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("pad_", ctypes.c_int32),
+        ("B", ctypes.c_double * 16),
+        ("c", ctypes.c_double * 4),
+    ]
+
+
 # SolverStatus{Float64} as a numpy structured dtype (72 bytes, Julia's field offsets).
 STATUS_DTYPE = np.dtype(
     {
@@ -205,6 +216,7 @@ EXPORTED_SYMBOLS = (
     "gs_solve_batch_f32",
     "gs_solve_batch_deflated_f64",
     "gs_solve_batch_stationarity_f64",
+    "gs_solve_batch_reparametrized_f64",
     "gs_lsq_set_shape",
     "gs_lsq_get_shape",
     "gs_lsq_set_graph",
@@ -293,6 +305,11 @@ def load_library(path: os.PathLike | None = None):
     lib.gs_solve_batch_stationarity_f64.restype = ctypes.c_int
     lib.gs_solve_batch_stationarity_f64.argtypes = [
         ctypes.POINTER(GsMethod), ctypes.POINTER(GsFamily), ctypes.POINTER(GsStationarity), ctypes.c_int64, vp, vp,
+        ctypes.c_int64, vp, vp, vp, ctypes.POINTER(GsOptionsF64), ctypes.c_int32, vp,
+    ]
+    lib.gs_solve_batch_reparametrized_f64.restype = ctypes.c_int
+    lib.gs_solve_batch_reparametrized_f64.argtypes = [
+        ctypes.POINTER(GsMethod), ctypes.POINTER(GsFamily), ctypes.POINTER(GsChart), ctypes.c_int64, vp, vp,
         ctypes.c_int64, vp, vp, vp, ctypes.POINTER(GsOptionsF64), ctypes.c_int32, vp,
     ]
     if lib.gs_version() != ABI_VERSION:

@@ -35,12 +35,21 @@ struct StatArgs {
     int64_t b_stride;      // bytes between datum records
 };
 
+// The chart of a Reparametrized batch (gs_chart), read by FamReparamT only.
+struct ChartArgs {
+    int32_t kind;      // 1 linear, 2 polar
+    int32_t pad_;
+    double B[16];      // linear: B (n × n, row-major)
+    double c[4];       // linear: c
+};
+
 struct FamArgs {
     const char* P;       // parameter records (device)
     int64_t p_stride;    // bytes between records; 0 = one shared record
     double shared[8];    // scalar parameters shared by every system
     DeflArgs defl;       // the deflation of gs_solve_batch_deflated_f64
     StatArgs stat;       // the index sets and datum of gs_solve_batch_stationarity_f64
+    ChartArgs chart;     // the chart of gs_solve_batch_reparametrized_f64
 };
 
 GS_DEV const char* record(const FamArgs& a, int64_t idx) { return a.P + (a.p_stride ? idx * a.p_stride : 0); }
@@ -508,6 +517,92 @@ template <class Act, int NE> struct FamStationarityT {
         }
     }
 };
+
+// The registered charts x = g(y) of Reparametrized, with their Jacobians G = ∂g/∂y.
+// Linear: x = B y + c (each row a left fold), G = B.
+template <int N> struct ChartLinearT {
+    double B[N][N], c[N];
+    GS_DEV void load(const FamArgs& a) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            c[i] = a.chart.c[i];
+#pragma unroll
+            for (int j = 0; j < N; ++j) B[i][j] = a.chart.B[i * N + j];
+        }
+    }
+    GS_DEV void g(const double (&y)[N], double (&x)[N]) const {
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            double acc = B[i][0] * y[0];
+#pragma unroll
+            for (int j = 1; j < N; ++j) acc = acc + B[i][j] * y[j];
+            x[i] = acc + c[i];
+        }
+    }
+    GS_DEV void G(const double (&)[N], double (&Gm)[N][N]) const {
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+#pragma unroll
+            for (int j = 0; j < N; ++j) Gm[i][j] = B[i][j];
+    }
+};
+
+// Polar (the chart of test/nonlinear/reparametrized.jl:31-38): x = (r cos θ, r sin θ),
+// G = [cos θ, −r sin θ; sin θ, r cos θ].
+struct ChartPolarT {
+    GS_DEV void load(const FamArgs&) {}
+    GS_DEV void g(const double (&y)[2], double (&x)[2]) const {
+        x[0] = y[0] * cos(y[1]);
+        x[1] = y[0] * sin(y[1]);
+    }
+    GS_DEV void G(const double (&y)[2], double (&Gm)[2][2]) const {
+        const double s = sin(y[1]), c = cos(y[1]);
+        Gm[0][0] = c;
+        Gm[0][1] = -y[0] * s;
+        Gm[1][0] = s;
+        Gm[1][1] = y[0] * c;
+    }
+};
+
+// Reparametrized(NonlinearProblem(F), g) (src/nonlinear/reparametrized.jl:1-60, covariant = false):
+// the solve iterates the chart coordinates y with F̃(y) = F(g(y)) and J̃(y) = J(g(y)) G(y) (each
+// entry a left fold over k), and the batch receives x* = g(y*) (:860-865). X holds y₀ on entry.
+template <class Inner, class Chart> struct FamReparamT {
+    using T = typename Inner::T;
+    static constexpr int N = Inner::N;
+    static constexpr bool kMapsOutput = true;
+    static_assert(sizeof(T) == 8, "the device charts are Float64");
+    Inner in;
+    Chart ch;
+    GS_DEV void load(const FamArgs& a, int64_t idx) {
+        in.load(a, idx);
+        ch.load(a);
+    }
+    GS_DEV void F(const T (&y)[N], T (&r)[N]) const {
+        T x[N];
+        ch.g(y, x);
+        in.F(x, r);
+    }
+    GS_DEV void J(const T (&y)[N], T (&Jm)[N][N]) const {
+        T x[N], Ji[N][N], Gm[N][N];
+        ch.g(y, x);
+        in.J(x, Ji);
+        ch.G(y, Gm);
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                T acc = Ji[i][0] * Gm[0][j];
+#pragma unroll
+                for (int k = 1; k < N; ++k) acc = acc + Ji[i][k] * Gm[k][j];
+                Jm[i][j] = acc;
+            }
+    }
+    GS_DEV void output(const T (&y)[N], T (&x)[N]) const { ch.g(y, x); }
+};
+
+template <class F, class = void> struct maps_output : std::false_type {};
+template <class F> struct maps_output<F, std::void_t<decltype(F::kMapsOutput)>> : std::true_type {};
 
 // Whether a family holds a full state beyond its unknowns (X rows of kStateLen entries).
 template <class F, class = void> struct holds_state : std::false_type {};

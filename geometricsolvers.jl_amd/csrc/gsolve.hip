@@ -386,6 +386,56 @@ int gs_solve_batch_stationarity_f64(const gs_method* method, const gs_family* ac
     return finish_launch(rc, st, timed);
 }
 
+int gs_solve_batch_reparametrized_f64(const gs_method* method, const gs_family* family, const gs_chart* chart,
+                                      int64_t nsys, double* X, const void* P, int64_t p_stride, gs_status_f64* S,
+                                      uint8_t* codes, const int64_t* order, const gs_options_f64* options,
+                                      int32_t workgroupsize, void* stream) {
+    g_last_error.clear();
+    const FamilySpec* fs = nullptr;
+    int rc = validate_batch<double>(method, family, nsys, X, P, p_stride, S, codes, options, workgroupsize, &fs);
+    if (rc != GS_OK) return rc;
+    if (!chart) return set_error(GS_EINVAL, "chart must not be NULL");
+    if (family->id == GS_FAM_FPU_GL3 || family->id == GS_ACT_HENON_HEILES)
+        return set_error(GS_EFAMILY, "a Reparametrized batch wraps one of the lane-per-system families (1-5)");
+    if (chart->kind == GS_CHART_POLAR) {
+        if (fs->n != 2) return set_error(GS_EFAMILY, "the polar chart maps 2 coordinates: families 1 and 2");
+    } else if (chart->kind == GS_CHART_LINEAR) {
+        for (int k = 0; k < fs->n * fs->n; ++k)
+            if (!std::isfinite(chart->B[k])) return set_error(GS_EINVAL, "the linear chart needs a finite B");
+        for (int k = 0; k < fs->n; ++k)
+            if (!std::isfinite(chart->c[k])) return set_error(GS_EINVAL, "the linear chart needs a finite c");
+    } else {
+        return set_error(GS_EINVAL, "chart kind must be GS_CHART_LINEAR or GS_CHART_POLAR");
+    }
+    if (method->globalization == GS_DOGLEG || method->reuse > 0)
+        return set_error(GS_EFAMILY,
+                         "a Reparametrized batch takes the Static or Backtracking globalization without reuse");
+    if (nsys == 0) return GS_OK;
+    KernelArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.fam.P = static_cast<const char*>(P);
+    a.fam.p_stride = p_stride;
+    for (int k = 0; k < 8; ++k) a.fam.shared[k] = family->shared[k];
+    a.fam.chart.kind = chart->kind;
+    for (int k = 0; k < 16; ++k) a.fam.chart.B[k] = chart->B[k];
+    for (int k = 0; k < 4; ++k) a.fam.chart.c[k] = chart->c[k];
+    a.nsys = nsys;
+    a.X = X;
+    a.S = S;
+    a.codes = codes;
+    a.order = order;
+    a.q = dev_params(*method, *options);
+    const bool diag = S != nullptr;
+    const int wg = workgroupsize > 0 ? workgroupsize : kSmallMaxWorkgroup;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const bool mixed = method->factor_precision == GS_FACTOR_F32;
+    const bool refine = method->refinement == GS_REFINEMENT;
+    bool timed = timing_begin(st);
+    const int glob = method->globalization == GS_BACKTRACKING ? kBacktracking : kStatic;
+    rc = launch_small_reparam(family->id, chart->kind, glob, a, mixed, refine, diag, wg, st);
+    return finish_launch(rc, st, timed);
+}
+
 int gs_solve_batch_f32(const gs_method* method, const gs_family* family, int64_t nsys, float* X, const void* P,
                        int64_t p_stride, gs_status_f32* S, uint8_t* codes, const int64_t* order,
                        const gs_options_f64* options, int32_t workgroupsize, void* stream) {

@@ -116,8 +116,16 @@ void solve_small_kernel(const KernelArgsT<typename Fam::T> a) {
     for (int k = 0; k < N; ++k) x[k] = xp[k];
     SmallSolve<Fam, GLOB, MIXED, REFINE, DIAG, REUSE> solver(fam, a.q);
     auto R = solver.solve(x);
+    if constexpr (maps_output<Fam>::value) {
+        // the batch receives x* = g(y*) of a Reparametrized problem's chart
+        T xo[N];
+        fam.output(x, xo);
 #pragma unroll
-    for (int k = 0; k < N; ++k) xp[k] = x[k];
+        for (int k = 0; k < N; ++k) xp[k] = xo[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < N; ++k) xp[k] = x[k];
+    }
     if (DIAG) {
         store_status(static_cast<Status*>(a.S) + idx, R.code, R.iterations, R.fnorm, R.stepnorm, R.failures, R.promoted,
                      R.gradratio, R.rate, R.contracted, R.reused);
@@ -182,6 +190,11 @@ int launch_small_deflated(int family, int glob, const KernelArgs& a, bool mixed,
 // registered action (FamStationarityT), |E| = 2 or 6.
 int launch_small_stationarity(int action, int ne, int glob, const KernelArgs& a, bool mixed, bool refine, bool diag,
                               int wg, hipStream_t st);
+
+// Defined in small_reparam.hip: Static and Backtracking on a Reparametrized batch (FamReparamT) with
+// a linear chart (families 1–5) or the polar chart (the n = 2 families).
+int launch_small_reparam(int family, int chart, int glob, const KernelArgs& a, bool mixed, bool refine, bool diag,
+                         int wg, hipStream_t st);
 
 // Defined in small_f32.hip: every form of a Float32 batch.
 int launch_small_f32(int family, int glob, bool reuse, const KernelArgsT<float>& a, bool refine, bool diag, int wg,

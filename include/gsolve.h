@@ -258,6 +258,30 @@ int gs_solve_batch_stationarity_f64(const gs_method* method, const gs_family* ac
                                     uint8_t* codes, const int64_t* order, const gs_options_f64* options,
                                     int32_t workgroupsize, void* stream);
 
+/* Reparametrized(NonlinearProblem(F), g) (src/nonlinear/reparametrized.jl:1-60, covariant = false):
+ * Newton in chart coordinates y with F̃(y) = F(g(y)) and J̃ = J(g(y)) ∂g(y); the registered charts. */
+enum gs_chart_kind {
+    GS_CHART_LINEAR = 1,        /* x = B y + c, B n × n row-major in B[0 .. n²-1], c in c[0 .. n-1]   */
+    GS_CHART_POLAR = 2          /* x = (y₁ cos y₂, y₁ sin y₂) (n = 2)                                  */
+};
+
+typedef struct gs_chart {
+    int32_t kind;               /* gs_chart_kind                                                      */
+    int32_t pad_;
+    double B[16];
+    double c[4];
+} gs_chart;
+
+/* solve_batch! on Reparametrized(NonlinearProblem(family), chart) (reparametrized.jl:981-1031, both
+ * forms): X holds the chart coordinates y₀ on entry and receives x* = g(y*) (:860-865); S or codes
+ * as for gs_solve_batch_f64. Families 1–5 with the linear chart, 1–2 with the polar chart; Static
+ * and Backtracking, plain, mixed and refined, Float64; DogLeg and reuse = k have no device form
+ * here (GS_EFAMILY). */
+int gs_solve_batch_reparametrized_f64(const gs_method* method, const gs_family* family, const gs_chart* chart,
+                                      int64_t nsys, double* X, const void* P, int64_t p_stride, gs_status_f64* S,
+                                      uint8_t* codes, const int64_t* order, const gs_options_f64* options,
+                                      int32_t workgroupsize, void* stream);
+
 /* The Float32 twin of gs_solve_batch_f64: X holds nsys × n floats (Vector{SVector{n,Float32}}), the
  * parameter records hold floats (Int64 codes for family 2), S receives SolverStatus{Float32}. The
  * options and the method's floats arrive as doubles and are used as Float32 (R(x), as Options{Float32}

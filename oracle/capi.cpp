@@ -136,12 +136,62 @@ static void run_batch_d(const gs_deflation* d, const Method<R>& m, const Options
     });
 }
 
+// A Reparametrized batch: the problems of `make` in the chart coordinates of `ch`; X holds y₀ on entry
+// and receives x* = g(y*) (reparametrized.jl:860-865).
+template <int N, class SS, class Chart, class MakeProb>
+static void run_batch_chart(const Chart& ch, const Method<double>& m, const Options<double>& o, int64_t nsys,
+                            double* X, SS* S, uint8_t* codes, const int64_t* order, int nthreads, MakeProb make) {
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+    for (int64_t i = 0; i < nsys; ++i) {
+        const int64_t idx = order ? order[i] - 1 : i;
+        ReparamProb<N, decltype(make(idx)), Chart> prob{make(idx), ch};
+        Solver<double, N, decltype(prob)> solver(prob, m, o);
+        Vec<double, N> y;
+        for (int k = 0; k < N; ++k) y[k] = X[idx * N + k];
+        Status<double> st = solver.solve(y);
+        const Vec<double, N> x = ch.g(y);
+        for (int k = 0; k < N; ++k) X[idx * N + k] = x[k];
+        if (S) write_status(S + idx, st);
+        if (codes) codes[idx] = st.code;
+    }
+}
+
+template <int N, class SS, class MakeProb>
+static void run_batch_d(const gs_chart* c, const Method<double>& m, const Options<double>& o, int64_t nsys,
+                        double* X, SS* S, uint8_t* codes, const int64_t* order, int nthreads, MakeProb make) {
+    if constexpr (N == 2) {
+        if (c->kind == GS_CHART_POLAR) {
+            run_batch_chart<N>(ChartPolar{}, m, o, nsys, X, S, codes, order, nthreads, make);
+            return;
+        }
+    }
+    ChartLinear<N> ch;
+    for (int i = 0; i < N; ++i) {
+        ch.c[i] = c->c[i];
+        for (int j = 0; j < N; ++j) ch.B[i][j] = c->B[i * N + j];
+    }
+    run_batch_chart<N>(ch, m, o, nsys, X, S, codes, order, nthreads, make);
+}
+
+template <int N, class R, class SS, class MakeProb>
+static void run_batch_d(std::pair<const gs_deflation*, const gs_chart*> w, const Method<R>& m, const Options<R>& o,
+                        int64_t nsys, R* X, SS* S, uint8_t* codes, const int64_t* order, int nthreads, MakeProb make) {
+    if constexpr (std::is_same_v<R, double>) {
+        if (w.second) {
+            run_batch_d<N>(w.second, m, o, nsys, X, S, codes, order, nthreads, make);
+            return;
+        }
+    }
+    run_batch_d<N>(w.first, m, o, nsys, X, S, codes, order, nthreads, make);
+}
+
 // The lane-per-system families (1-5) in the working precision R, records of R (Int64 codes for 2),
-// deflated where `defl` is given.
+// deflated where `defl` is given, reparametrized where `chart` is.
 template <class R, class SS>
 static int solve_small_batch(const Method<R>& m, const Options<R>& o, const gs_family* fam, int64_t nsys, R* X,
                              const void* P, int64_t p_stride, SS* S, uint8_t* codes, const int64_t* order,
-                             int nthreads, const gs_deflation* defl = nullptr) {
+                             int nthreads, const gs_deflation* defl_ = nullptr, const gs_chart* chart = nullptr) {
+    const auto defl = std::make_pair(defl_, chart);
     const char* Pb = static_cast<const char*>(P);
     auto rec = [&](int64_t idx) -> const char* { return Pb + (p_stride ? idx * p_stride : 0); };
     switch (fam->id) {
@@ -306,6 +356,19 @@ void orc_hh_action_eval(double h, double lam, const double* x, double* g, double
         g[i] = gv[i];
         for (int j = 0; j < 6; ++j) H[6 * i + j] = Hm[i][j];
     }
+}
+
+// The host restatement of gs_solve_batch_reparametrized_f64 (include/gsolve.h).
+int orc_solve_batch_reparametrized_f64(const gs_method* method, const gs_family* fam, const gs_chart* chart,
+                                       int64_t nsys, double* X, const void* P, int64_t p_stride, gs_status_f64* S,
+                                       uint8_t* codes, const int64_t* order, const gs_options_f64* options,
+                                       int nthreads) {
+    if (!method || !fam || !options || !chart || nsys < 0) return GS_EINVAL;
+    if (chart->kind == GS_CHART_POLAR && fam->n != 2) return GS_EFAMILY;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    Method<double> m = method_from<double>(*method);
+    Options<double> o = options_from<double>(*options);
+    return solve_small_batch<double>(m, o, fam, nsys, X, P, p_stride, S, codes, order, nthreads, nullptr, chart);
 }
 
 // Julia's Float64 x^n for an integer n (the deflation's power), for the KATs.

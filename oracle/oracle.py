@@ -40,6 +40,8 @@ def lib():
         L.orc_solve_batch_stationarity_f64.restype = ctypes.c_int
         L.orc_solve_batch_stationarity_f64.argtypes = [vp, vp, vp, ctypes.c_int64, vp, vp, ctypes.c_int64, vp, vp, vp,
                                                        vp, ctypes.c_int]
+        L.orc_solve_batch_reparametrized_f64.restype = ctypes.c_int
+        L.orc_solve_batch_reparametrized_f64.argtypes = L.orc_solve_batch_deflated_f64.argtypes
         L.orc_hh_action_eval.restype = None
         L.orc_hh_action_eval.argtypes = [ctypes.c_double, ctypes.c_double, vp, vp, vp]
         L.orc_pow_int.restype = None
@@ -108,14 +110,15 @@ def host_records(family, p, nsys: int, real=np.float64):
 
 
 def solve_batch(method, family, X: np.ndarray, p, options, order=None, nthreads: int = 0, deflated=None,
-                stationarity=None, b=None):
+                stationarity=None, b=None, reparametrized=None):
     """The oracle's batched solve on host arrays: returns (X, statuses) with X overwritten.
 
     `method`, `options`: objects with `to_c()` giving the ABI structs of include/gsolve.h;
     `family`: a registered family (id, n, p_len); `order`: 1-based permutation or None;
     `deflated`: a Deflated (its roots, power and shift; its problem is `family`'s) or None;
     `stationarity`: a StationarityProblem over the action `family` (X rows are full states), with
-    the datum `b` overriding its own, or None.
+    the datum `b` overriding its own, or None; `reparametrized`: a Reparametrized over `family`
+    (X holds the chart coordinates y₀ and receives x* = g(y*)), or None.
     """
     if stationarity is not None:
         return _solve_stationarity(method, family, X, p, options, order, nthreads, stationarity, b)
@@ -129,7 +132,12 @@ def solve_batch(method, family, X: np.ndarray, p, options, order=None, nthreads:
     fam = _family_struct(family, shared)
     S = np.zeros(nsys, dtype=_status_dtype_f32() if f32 else _status_dtype())
     o = None if order is None else np.ascontiguousarray(np.asarray(order, dtype=np.int64))
-    if deflated is not None:
+    if reparametrized is not None:
+        cc = reparametrized.to_c(family.n)
+        rc = lib().orc_solve_batch_reparametrized_f64(
+            ctypes.byref(cm), ctypes.byref(fam), ctypes.byref(cc), nsys, _ptr(X), _ptr(rec), stride, _ptr(S), None,
+            None if o is None else _ptr(o), ctypes.byref(co), nthreads)
+    elif deflated is not None:
         cd = deflated.to_c(family.n)
         rc = lib().orc_solve_batch_deflated_f64(
             ctypes.byref(cm), ctypes.byref(fam), ctypes.byref(cd), nsys, _ptr(X), _ptr(rec), stride, _ptr(S), None,
