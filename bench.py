@@ -95,6 +95,13 @@ def _event_ms(events):
     return [a.elapsed_time(b) for a, b in events]
 
 
+def _traffic(name):
+    """HBM bytes per launch of a kernel, from the calibrated FETCH_SIZE + WRITE_SIZE counters
+    committed under profiles/ (None where no such file exists)."""
+    tfile = ROOT / "profiles" / name
+    return json.loads(tfile.read_text()).get("bytes_per_launch") if tfile.exists() else None
+
+
 def bench_henon_heiles(args, gs, workloads, torch):
     dev = torch.device("cuda:0")
     lib = gs.load_library()
@@ -179,10 +186,7 @@ def bench_henon_heiles(args, gs, workloads, torch):
     value = conv_learned / learned_el
     plain_value = conv_plain / plain_el
     achieved = MLP_FLOPS_PER_SYSTEM * NSYS / (fused_kms * 1e-3) / 1e12
-    traffic = None
-    tfile = ROOT / "profiles" / "traffic_learned_solve_henon_heiles_2p20.json"
-    if tfile.exists():
-        traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
+    traffic = _traffic("traffic_learned_solve_henon_heiles_2p20.json")
     solve_kms = float(np.mean(solve_ms))
     return {
         "metric": "converged systems/s at residual ≤1e-12 (FP64 check), learned+mixed vs plain, 1 GPU",
@@ -328,7 +332,7 @@ def bench_fpu_gl3(args, gs, workloads, torch):
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": _traffic("traffic_fpu_gl3_65536.json"),
             "hbm_frac": FPU_BYTES_PER_SYSTEM * n / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         },
         "cpu_baseline": cpu,
