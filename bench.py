@@ -407,7 +407,7 @@ def bench_expsum(args, gs, workloads, torch):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": _traffic("traffic_lsq_expsum_262144.json"),
         },
         "cpu_baseline": cpu,
     }
