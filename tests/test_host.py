@@ -281,3 +281,20 @@ def test_float32_options_and_status_layout(gs):
                                                                            32, 36, 40, 41]
     assert gs.status_buffer(5, "cpu", "float32").shape == (5, 44)
 
+
+
+def test_wrapper_forms_have_no_host_path(gs):
+    """The Deflated, StationarityProblem and Reparametrized batches refuse host tensors as the plain
+    batch does: the library solves on the GPU and nothing falls back to the CPU."""
+    import torch
+
+    X2 = torch.zeros((4, 2), dtype=torch.float64)
+    p2 = gs.PerSystem(torch.zeros((4, 2), dtype=torch.float64))
+    prob = gs.NonlinearProblem(gs.Coupled2())
+    with pytest.raises(gs.ArgumentError, match="device array"):
+        gs.solve_batch(X2, gs.Deflated(prob, ((1.0, 2.0),)), gs.Newton(), p2)
+    with pytest.raises(gs.ArgumentError, match="device array"):
+        gs.solve_batch(X2, gs.Reparametrized(prob, gs.LinearChart(np.eye(2))), gs.Newton(), p2)
+    X6 = torch.zeros((4, 6), dtype=torch.float64)
+    with pytest.raises(gs.ArgumentError, match="device array"):
+        gs.solve_batch(X6, gs.StationarityProblem(gs.HenonHeilesAction(), E=(3, 4), U=(5, 6)), gs.Newton(), (0.1, 1.0))
