@@ -397,7 +397,7 @@ end
 
 # The roots as the flat record of gs_deflation, or `nothing` where the device form does not hold
 # them (more than 4 roots or 32 entries, a power that is not an integer in 1:64); such a Deflated,
-# and every method the device form does not take (DogLeg, reuse, best = true, a Float32 batch), runs
+# and every method the device form does not take (reuse, best = true, a Float32 batch), runs
 # the reference's generic batch, which also raises the reference's own errors.
 function device_deflation(def::Deflated, n::Int)
     nr = length(def.roots)
@@ -414,7 +414,7 @@ function device_deflation(def::Deflated, n::Int)
 end
 
 served_deflated(fam::HIPFamily, cm, ::Type{T}) where {T} = !(fam isa FpuGl3) && T === Float64 &&
-    cm !== nothing && cm.globalization in (0, 1) && cm.reuse == 0 && cm.best == 0
+    cm !== nothing && cm.globalization in (0, 1, 3) && cm.reuse == 0 && cm.best == 0
 
 function hip_launch_deflated!(X, S, codes, def::Deflated, cd::GsDeflation, cm::GsMethod, p, opt, order, ws)
     fam = def.prob.F
@@ -645,7 +645,7 @@ function hip_launch_stationarity!(X, S, codes, cs::GsStationarity, cm::GsMethod,
     return nothing
 end
 
-served_stationarity(cm) = cm !== nothing && cm.globalization in (0, 1) && cm.reuse == 0
+served_stationarity(cm) = cm !== nothing && cm.globalization in (0, 1, 3) && cm.reuse == 0
 
 # The status form: X holds each system's full state (SVector{6, Float64}); p = (h, λ) shared or
 # PerSystem of SVector{2, Float64}; the datum from the keyword `b` or the problem's own.

@@ -284,8 +284,7 @@ int gs_solve_batch_deflated_f64(const gs_method* method, const gs_family* family
                          "residual, not by the inner residual that status.fnorm reports");
     if (family->id == GS_FAM_FPU_GL3 || family->id == GS_ACT_HENON_HEILES)
         return set_error(GS_EFAMILY, "a Deflated batch wraps one of the lane-per-system families (1-5)");
-    if (method->globalization == GS_DOGLEG || method->reuse > 0)
-        return set_error(GS_EFAMILY, "a Deflated batch takes the Static or Backtracking globalization without reuse");
+    if (method->reuse > 0) return set_error(GS_EFAMILY, "a Deflated batch takes no reuse on the device");
     if (nsys == 0) return GS_OK;
     KernelArgs a;
     std::memset(&a, 0, sizeof(a));
@@ -308,7 +307,9 @@ int gs_solve_batch_deflated_f64(const gs_method* method, const gs_family* family
     const bool mixed = method->factor_precision == GS_FACTOR_F32;
     const bool refine = method->refinement == GS_REFINEMENT;
     bool timed = timing_begin(st);
-    const int glob = method->globalization == GS_BACKTRACKING ? kBacktracking : kStatic;
+    const int glob = method->globalization == GS_BACKTRACKING ? kBacktracking
+                     : method->globalization == GS_DOGLEG    ? kDogLeg
+                                                             : kStatic;
     rc = launch_small_deflated(family->id, glob, a, mixed, refine, diag, wg, st);
     return finish_launch(rc, st, timed);
 }
@@ -347,9 +348,7 @@ int gs_solve_batch_stationarity_f64(const gs_method* method, const gs_family* ac
     if (ne != 2 && ne != 6)
         return set_error(GS_EFAMILY, "the device StationarityProblem batch takes |E| = 2 or 6 equations, got " +
                                          std::to_string(ne));
-    if (method->globalization == GS_DOGLEG || method->reuse > 0)
-        return set_error(GS_EFAMILY,
-                         "a StationarityProblem batch takes the Static or Backtracking globalization without reuse");
+    if (method->reuse > 0) return set_error(GS_EFAMILY, "a StationarityProblem batch takes no reuse on the device");
     if (nsys == 0) return GS_OK;
     if (t.b_kind == GS_DATUM_PER_SYSTEM) {
         if (!t.b_data) return set_error(GS_EINVAL, "a per-system datum needs b_data");
@@ -381,7 +380,9 @@ int gs_solve_batch_stationarity_f64(const gs_method* method, const gs_family* ac
     const bool mixed = method->factor_precision == GS_FACTOR_F32;
     const bool refine = method->refinement == GS_REFINEMENT;
     bool timed = timing_begin(st);
-    const int glob = method->globalization == GS_BACKTRACKING ? kBacktracking : kStatic;
+    const int glob = method->globalization == GS_BACKTRACKING ? kBacktracking
+                     : method->globalization == GS_DOGLEG    ? kDogLeg
+                                                             : kStatic;
     rc = launch_small_stationarity(action->id, ne, glob, a, mixed, refine, diag, wg, st);
     return finish_launch(rc, st, timed);
 }

@@ -1,5 +1,5 @@
 // libgsolve_hip: the lane-per-system kernels of a Deflated batch (src/nonlinear/deflation.jl:714-760):
-// each family wrapped in FamDeflatedT, with the Static and Backtracking globalizations, in a
+// each family wrapped in FamDeflatedT, with the Static, Backtracking and DogLeg globalizations, in a
 // translation unit of its own.
 #include "small_launch.hpp"
 
@@ -30,6 +30,7 @@ int launch_deflated_glob(int family, const KernelArgs& a, bool mixed, bool refin
 int launch_small_deflated(int family, int glob, const KernelArgs& a, bool mixed, bool refine, bool diag, int wg,
                           hipStream_t st) {
     if (glob == kBacktracking) return launch_deflated_glob<kBacktracking>(family, a, mixed, refine, diag, wg, st);
+    if (glob == kDogLeg) return launch_deflated_glob<kDogLeg>(family, a, mixed, refine, diag, wg, st);
     return launch_deflated_glob<kStatic>(family, a, mixed, refine, diag, wg, st);
 }
 

@@ -1,6 +1,6 @@
 // libgsolve_hip: the lane-per-system kernels of a StationarityProblem batch
 // (src/nonlinear/stationarity.jl:642-698): a registered action wrapped in FamStationarityT, with |E|
-// = 2 or 6 unknowns, the Static and Backtracking globalizations, in a translation unit of its own.
+// = 2 or 6 unknowns, the Static, Backtracking and DogLeg globalizations, in a translation unit of its own.
 #include "small_launch.hpp"
 
 namespace gs {
@@ -31,6 +31,7 @@ int launch_stationarity_glob(int action, int ne, const KernelArgs& a, bool mixed
 int launch_small_stationarity(int action, int ne, int glob, const KernelArgs& a, bool mixed, bool refine, bool diag,
                               int wg, hipStream_t st) {
     if (glob == kBacktracking) return launch_stationarity_glob<kBacktracking>(action, ne, a, mixed, refine, diag, wg, st);
+    if (glob == kDogLeg) return launch_stationarity_glob<kDogLeg>(action, ne, a, mixed, refine, diag, wg, st);
     return launch_stationarity_glob<kStatic>(action, ne, a, mixed, refine, diag, wg, st);
 }
 

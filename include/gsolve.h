@@ -221,9 +221,9 @@ typedef struct gs_deflation {
 /* gs_solve_batch_f64 on Deflated(NonlinearProblem(family), roots; power, shift): the steps and the
  * line search take the deflated residual M(x) F(x) with the product-rule Jacobian M J + F ∇Mᵀ
  * (deflation.jl:236-399), and the status reports, and the residual test reads, the inner
- * ‖F‖ = ‖M F‖ / M (converged_status / test_status, deflation.jl:514-548). Families 1–5, Static and
- * Backtracking, plain, mixed and refined; a Newton with best = true is refused as the reference
- * refuses it (deflation.jl:594-604); DogLeg and reuse = k have no deflated device form (GS_EFAMILY). */
+ * ‖F‖ = ‖M F‖ / M (converged_status / test_status, deflation.jl:514-548). Families 1–5, Static,
+ * Backtracking and DogLeg, plain, mixed and refined; a Newton with best = true is refused as the
+ * reference refuses it (deflation.jl:594-604); reuse = k has no deflated device form (GS_EFAMILY). */
 int gs_solve_batch_deflated_f64(const gs_method* method, const gs_family* family, const gs_deflation* deflation,
                                 int64_t nsys, double* X, const void* P, int64_t p_stride, gs_status_f64* S,
                                 uint8_t* codes, const int64_t* order, const gs_options_f64* options,
@@ -252,7 +252,7 @@ typedef struct gs_stationarity {
  * full state (M doubles, M = 6 for GS_ACT_HENON_HEILES); the solve iterates the entries U and writes
  * them back, the others keep their bits; S or codes as for gs_solve_batch_f64. Served: the registered
  * actions, |E| = 2 or 6 (GS_EFAMILY otherwise), LUFactorization (Float64 or Float32 factor),
- * Static and Backtracking, refinement, regularization, best. */
+ * Static, Backtracking and DogLeg, refinement, regularization, best; no reuse = k. */
 int gs_solve_batch_stationarity_f64(const gs_method* method, const gs_family* action, const gs_stationarity* stat,
                                     int64_t nsys, double* X, const void* P, int64_t p_stride, gs_status_f64* S,
                                     uint8_t* codes, const int64_t* order, const gs_options_f64* options,
