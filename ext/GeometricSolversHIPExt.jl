@@ -645,7 +645,7 @@ function hip_launch_stationarity!(X, S, codes, cs::GsStationarity, cm::GsMethod,
     return nothing
 end
 
-served_stationarity(cm) = cm !== nothing && cm.globalization in (0, 1, 3) && cm.reuse == 0
+served_stationarity(cm) = cm !== nothing && cm.globalization in (0, 1, 3)
 
 # The status form: X holds each system's full state (SVector{6, Float64}); p = (h, λ) shared or
 # PerSystem of SVector{2, Float64}; the datum from the keyword `b` or the problem's own.

@@ -348,7 +348,6 @@ int gs_solve_batch_stationarity_f64(const gs_method* method, const gs_family* ac
     if (ne != 2 && ne != 6)
         return set_error(GS_EFAMILY, "the device StationarityProblem batch takes |E| = 2 or 6 equations, got " +
                                          std::to_string(ne));
-    if (method->reuse > 0) return set_error(GS_EFAMILY, "a StationarityProblem batch takes no reuse on the device");
     if (nsys == 0) return GS_OK;
     if (t.b_kind == GS_DATUM_PER_SYSTEM) {
         if (!t.b_data) return set_error(GS_EINVAL, "a per-system datum needs b_data");
@@ -383,7 +382,7 @@ int gs_solve_batch_stationarity_f64(const gs_method* method, const gs_family* ac
     const int glob = method->globalization == GS_BACKTRACKING ? kBacktracking
                      : method->globalization == GS_DOGLEG    ? kDogLeg
                                                              : kStatic;
-    rc = launch_small_stationarity(action->id, ne, glob, a, mixed, refine, diag, wg, st);
+    rc = launch_small_stationarity(action->id, ne, glob, method->reuse > 0, a, mixed, refine, diag, wg, st);
     return finish_launch(rc, st, timed);
 }
 

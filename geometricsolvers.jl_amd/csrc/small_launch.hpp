@@ -187,9 +187,9 @@ int launch_small_deflated(int family, int glob, const KernelArgs& a, bool mixed,
                           hipStream_t st);
 
 // Defined in small_stationarity.hip: Static and Backtracking on a StationarityProblem batch of a
-// registered action (FamStationarityT), |E| = 2 or 6.
-int launch_small_stationarity(int action, int ne, int glob, const KernelArgs& a, bool mixed, bool refine, bool diag,
-                              int wg, hipStream_t st);
+// registered action (FamStationarityT), |E| = 2 or 6, with or without reuse = k.
+int launch_small_stationarity(int action, int ne, int glob, bool reuse, const KernelArgs& a, bool mixed, bool refine,
+                              bool diag, int wg, hipStream_t st);
 
 // Defined in small_reparam.hip: Static and Backtracking on a Reparametrized batch (FamReparamT) with
 // a linear chart (families 1–5) or the polar chart (the n = 2 families).

@@ -252,7 +252,7 @@ typedef struct gs_stationarity {
  * full state (M doubles, M = 6 for GS_ACT_HENON_HEILES); the solve iterates the entries U and writes
  * them back, the others keep their bits; S or codes as for gs_solve_batch_f64. Served: the registered
  * actions, |E| = 2 or 6 (GS_EFAMILY otherwise), LUFactorization (Float64 or Float32 factor),
- * Static, Backtracking and DogLeg, refinement, regularization, best; no reuse = k. */
+ * Static, Backtracking and DogLeg, refinement, regularization, best, reuse = k (not with DogLeg). */
 int gs_solve_batch_stationarity_f64(const gs_method* method, const gs_family* action, const gs_stationarity* stat,
                                     int64_t nsys, double* X, const void* P, int64_t p_stride, gs_status_f64* S,
                                     uint8_t* codes, const int64_t* order, const gs_options_f64* options,

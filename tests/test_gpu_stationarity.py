@@ -36,6 +36,9 @@ METHODS = {
     "mixed-refine": lambda gs: gs.Newton(linsolve=gs.LUFactorization("float32"), refinement=gs.Refinement()),
     "best-backtracking": lambda gs: gs.Newton(globalization=gs.Backtracking(), best=True),
     "dogleg": lambda gs: gs.Newton(globalization=gs.DogLeg()),
+    "reuse3": lambda gs: gs.Newton(reuse=3),
+    "mixed-reuse2-backtracking": lambda gs: gs.Newton(linsolve=gs.LUFactorization("float32"), refinement=gs.Refinement(),
+                                                      globalization=gs.Backtracking(), reuse=2),
 }
 
 
@@ -91,8 +94,6 @@ def test_refusals(gs, device):
     X = torch.zeros((4, 6), dtype=torch.float64, device=device)
     with pytest.raises(gs.ArgumentError, match="\\|E\\| = 2 or 6"):
         gs.solve_batch(X, gs.StationarityProblem(act, E=(1, 2, 3), U=(4, 5, 6)), gs.Newton(), (H, 1.0))
-    with pytest.raises(gs.ArgumentError, match="no reuse"):
-        gs.solve_batch(X, gs.StationarityProblem(act), gs.Newton(reuse=2), (H, 1.0))
     with pytest.raises(gs.ArgumentError, match="StationarityProblem"):
         gs.solve_batch(X, gs.NonlinearProblem(act), gs.Newton(), (H, 1.0))
     with pytest.raises(gs.DimensionMismatch):
