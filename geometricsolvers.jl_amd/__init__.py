@@ -33,7 +33,7 @@ __all__ = [
     "SINGULAR", "NONFINITE", "LINESEARCH_FAILED", "STATUS_DTYPE", "STATUS_DTYPE_F32", "L2Norm", "MaxNorm", "Options",
     "LUFactorization", "Static", "Backtracking", "DogLeg", "LevenbergMarquardt", "NoRefinement", "Refinement", "Newton",
     "NonlinearProblem", "Deflated", "StationarityProblem", "HenonHeilesAction", "Reparametrized", "LinearChart",
-    "PolarChart", "Coupled2", "Hetero2", "ScalarSquare", "TanhLinear3",
+    "PolarChart", "Coupled2", "Hetero2", "ScalarSquare", "TanhLinear3", "QuadMap2",
     "HenonHeilesMidpoint", "FpuGl3", "PerSystem", "BatchParams", "parameters", "solve_batch",
     "LeastSquaresProblem", "ExpSumFits", "ExpSumData",
     "status_buffer", "statuses", "isconverged", "batch_summary", "BatchSummary", "load_library",
@@ -471,6 +471,18 @@ class TanhLinear3(_SimpleFamily):
     """A_p x + 0.1 tanh.(x) − b_p, n = 3; a record is vec(A_p) (column-major, 9) then b_p (3)."""
 
     id, n, p_len = _abi.GS_FAM_TANH_LINEAR3, 3, 12
+
+    @property
+    def p_dtype(self):
+        return _torch().float64
+
+
+class QuadMap2(_SimpleFamily):
+    """u = A x, F_i = (u_i + c_i u_i²) − b_i, n = 2: the systems f_mod_r1 and f_ill_r1 of the reference's
+    mixed-precision tests (test/nonlinear/refinement.jl:40-96), which take c = (0.1, −0.2); a record is
+    vec(A) (column-major, 4), then b (2), then c (2)."""
+
+    id, n, p_len = _abi.GS_FAM_QUAD_MAP2, 2, 8
 
     @property
     def p_dtype(self):

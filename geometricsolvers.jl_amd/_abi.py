@@ -33,6 +33,7 @@ GS_FAM_TANH_LINEAR3 = 4
 GS_FAM_HENON_HEILES_MIDPOINT = 5
 GS_FAM_FPU_GL3 = 6
 GS_FAM_EXPSUM_LSQ = 7
+GS_FAM_QUAD_MAP2 = 9
 GS_ACT_HENON_HEILES = 8
 
 GS_DATUM_NONE, GS_DATUM_SHARED, GS_DATUM_PER_SYSTEM = 0, 1, 2

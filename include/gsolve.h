@@ -171,9 +171,11 @@ enum gs_family_id {
     GS_FAM_HENON_HEILES_MIDPOINT = 5, /* implicit-midpoint step of Hénon–Heiles : n = 4, P = x_k (4), shared[0] = h */
     GS_FAM_FPU_GL3 = 6,          /* Gauss–Legendre s = 3 stage system of a 16-site FPU-β chain : n = 48           */
     GS_FAM_EXPSUM_LSQ = 7,       /* exponential-sum least-squares fit (LevenbergMarquardt)                       */
-    GS_ACT_HENON_HEILES = 8      /* an action S(x, p), not a residual: the two-step discrete action of the       *
+    GS_ACT_HENON_HEILES = 8,     /* an action S(x, p), not a residual: the two-step discrete action of the       *
                                   * variational midpoint rule for Hénon–Heiles, x = (q₋, q, q₊) (6 entries),     *
                                   * P = (h, λ); taken by gs_solve_batch_stationarity_f64 only                   */
+    GS_FAM_QUAD_MAP2 = 9         /* u = A x, F_i = (u_i + c_i u_i²) − b_i : n = 2, P = A (4, column-major), b (2),  *
+                                  * c (2) — f_mod_r1 / f_ill_r1 of test/nonlinear/refinement.jl:40-96           */
 };
 
 typedef struct gs_family {

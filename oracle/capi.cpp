@@ -222,6 +222,16 @@ static int solve_small_batch(const Method<R>& m, const Options<R>& o, const gs_f
                 return pr;
             });
             return 0;
+        case GS_FAM_QUAD_MAP2:
+            run_batch_d<2>(defl, m, o, nsys, X, S, codes, order, nthreads, [&](int64_t idx) {
+                const R* p = reinterpret_cast<const R*>(rec(idx));
+                QuadMap2<R> pr;
+                for (int k = 0; k < 4; ++k) pr.A[k] = p[k];
+                for (int k = 0; k < 2; ++k) pr.b[k] = p[4 + k];
+                for (int k = 0; k < 2; ++k) pr.c[k] = p[6 + k];
+                return pr;
+            });
+            return 0;
         case GS_FAM_HENON_HEILES_MIDPOINT: {
             const R h = R(fam->shared[0]);
             run_batch_d<4>(defl, m, o, nsys, X, S, codes, order, nthreads, [&](int64_t idx) {
@@ -504,32 +514,228 @@ double orc_backtrack_step(double phi0, double d0, double a, double pa, double ap
 
 int orc_max_threads(void) { return omp_get_max_threads(); }
 
-// One solve of a loop-test fixture (families.hpp, 101-103) with parameters p; x is overwritten.
+}  // extern "C"
+
+namespace orc {
+
+// The fixture `fixture` (families.hpp, 101-110) in the working precision R with parameters p, passed
+// to `run` as a problem; GS_EINVAL for an unknown fixture or one with no form in R.
+template <class R, class Run> static int with_fixture(int fixture, const double* p, Run&& run) {
+    switch (fixture) {
+        case 101: return run(DiagSquare2<R>{R(p[0]), R(p[1])});
+        case 102: return run(Affine2<R>{R(p[0]), R(p[1])});
+        case 103: return run(InfJacobian1<R>{});
+        case 104:
+            if constexpr (std::is_same<R, double>::value) return run(Floor2{});
+            return GS_EINVAL;
+        case 105: {
+            Cubic2<R> c;
+            for (int k = 0; k < 4; ++k) c.A[k] = R(p[k]);
+            for (int k = 0; k < 2; ++k) c.c[k] = R(p[4 + k]);
+            for (int k = 0; k < 2; ++k) c.b[k] = R(p[6 + k]);
+            return run(c);
+        }
+        case 106: return run(LinearConstJ1<R>{R(p[0]), R(p[1])});
+        case 107: return run(JumpJacobian1<R>{});
+        case 108: return run(FloorResidual1<R>{R(p[0])});
+        case 109: return run(CycleStep1<R>{p[0] != 0.0});
+        case 110: return run(JumpStep1<R>{});
+        case 9: {
+            QuadMap2<R> q;
+            for (int k = 0; k < 4; ++k) q.A[k] = R(p[k]);
+            for (int k = 0; k < 2; ++k) q.b[k] = R(p[4 + k]);
+            for (int k = 0; k < 2; ++k) q.c[k] = R(p[6 + k]);
+            return run(q);
+        }
+    }
+    return GS_EINVAL;
+}
+
+template <class R> static void write_status_any(gs_status_f64* out, const Status<R>& s) {
+    Status<double> d;
+    d.code = s.code;
+    d.f_calls = s.f_calls;
+    d.iterations = s.iterations;
+    d.fnorm = double(s.fnorm);
+    d.stepnorm = double(s.stepnorm);
+    d.step_failures = s.step_failures;
+    d.promoted = s.promoted;
+    d.stationary = s.stationary;
+    d.jvp_calls = s.jvp_calls;
+    d.rank = s.rank;
+    d.nullnorm = double(s.nullnorm);
+    d.gradratio = double(s.gradratio);
+    d.rate = double(s.rate);
+    d.contracted = s.contracted;
+    d.reused = s.reused;
+    write_status(out, d);
+}
+
+template <class R>
+static int solve_fixture_t(int fixture, const gs_method* method, const gs_options_f64* options, double* x,
+                           const double* p, gs_status_f64* S, long* counts) {
+    Method<R> m = method_from<R>(*method);
+    Options<R> o = options_from<R>(*options);
+    return with_fixture<R>(fixture, p, [&](auto prob) {
+        constexpr int N = decltype(prob)::N;
+        Solver<R, N, decltype(prob)> solver(prob, m, o);
+        Vec<R, N> v;
+        for (int k = 0; k < N; ++k) v[k] = R(x[k]);
+        Status<R> st = solver.solve(v);
+        for (int k = 0; k < N; ++k) x[k] = double(v[k]);
+        write_status_any<R>(S, st);
+        if (counts) {
+            counts[0] = solver.prob.nf;
+            counts[1] = solver.prob.nj;
+        }
+        return 0;
+    });
+}
+
+}  // namespace orc
+
+extern "C" {
+
+// One solve of a fixture (families.hpp, 101-110; 9 is the quadratic-map family) with parameters p; x
+// is overwritten. `f32` runs it in Float32 (x, p and the status fields pass through as doubles).
 // counts (may be NULL) receives the residual and the Jacobian calls of the solve.
+int orc_solve_fixture_t(int fixture, int f32, const gs_method* method, const gs_options_f64* options, double* x,
+                        const double* p, gs_status_f64* S, long* counts) {
+    return f32 ? solve_fixture_t<float>(fixture, method, options, x, p, S, counts)
+               : solve_fixture_t<double>(fixture, method, options, x, p, S, counts);
+}
+
 int orc_solve_fixture_counts(int fixture, const gs_method* method, const gs_options_f64* options, double* x,
                              const double* p, gs_status_f64* S, long* counts) {
+    return solve_fixture_t<double>(fixture, method, options, x, p, S, counts);
+}
+
+// F(x) and J(x) (column-major) of a fixture in Float64, or in Float32 (f32 = 1: x and p are rounded
+// to Float32, and F and J pass back as doubles).
+int orc_fixture_eval(int fixture, int f32, const double* p, const double* x, double* F, double* J) {
+    auto run = [&](auto zero) {
+        using R = decltype(zero);
+        return with_fixture<R>(fixture, p, [&](auto prob) {
+            constexpr int N = decltype(prob)::N;
+            Vec<R, N> v;
+            for (int k = 0; k < N; ++k) v[k] = R(x[k]);
+            const Vec<R, N> r = prob.F(v);
+            const Mat<R, N> Jm = prob.J(v);
+            for (int k = 0; k < N; ++k) F[k] = double(r[k]);
+            for (int k = 0; k < N * N; ++k) J[k] = double(Jm[k]);
+            return 0;
+        });
+    };
+    return f32 ? run(0.0f) : run(0.0);
+}
+
+// One step!! (newton.jl:1671-1675) of `method` from x on a fixture, after start!! (newton.jl:1135-1144),
+// as the reference's per-step tests take it (test/nonlinear/refinement.jl:406-416, 634-645): x is
+// overwritten with x₊; out = {code, failures, state promoted, info promoted, fnorm, stepnorm, radius,
+// residual calls (F(x) included)}.
+int orc_fixture_step(int fixture, const gs_method* method, double* x, const double* p, double* out) {
     Method<double> m = method_from<double>(*method);
-    Options<double> o = options_from<double>(*options);
-    auto run = [&](auto prob) {
+    Options<double> o;
+    return with_fixture<double>(fixture, p, [&](auto prob) {
         constexpr int N = decltype(prob)::N;
         Solver<double, N, decltype(prob)> solver(prob, m, o);
         Vec<double, N> v;
         for (int k = 0; k < N; ++k) v[k] = x[k];
-        Status<double> st = solver.solve(v);
+        Vec<double, N> Fx = solver.prob.F(v);
+        NState<double, N> st = solver.start(v, Fx, rnorm(Fx));
+        StepInfo<double> info = solver.step(v, Fx, st);
         for (int k = 0; k < N; ++k) x[k] = v[k];
-        write_status(S, st);
-        if (counts) {
-            counts[0] = prob.nf;
-            counts[1] = prob.nj;
-        }
+        out[0] = info.code;
+        out[1] = info.failures;
+        out[2] = st.promoted ? 1.0 : 0.0;
+        out[3] = info.promoted ? 1.0 : 0.0;
+        out[4] = info.fnorm;
+        out[5] = info.stepnorm;
+        out[6] = st.radius;
+        out[7] = double(solver.prob.nf);
         return 0;
+    });
+}
+
+// refine!! (refinement.jl:180-244) on a fixture at x, as the reference's unit checks call it
+// (test/nonlinear/refinement.jl:328-404): the direction d (overwritten with the result), the factor of
+// kind fac_kind (0: J(x) factorised in Float32; 1: J(x) in Float64; 2: the Float32 matrix −I,
+// factorised in its own precision), Refinement(eta, maxiter) or, with refine = 0, NoRefinement();
+// Fx = F(x) (zero_fx = 0) or the zero vector; fnorm given (has_fnorm = 1) or formed from Fx. The
+// product is J(x) v, the JVP of an analytic Jacobian. *failed receives the flag.
+int orc_refine_fixture(int fixture, const double* p, const double* x, double* d, int fac_kind, int refine, double eta,
+                       int maxiter, int zero_fx, int has_fnorm, double fnorm, int* failed) {
+    Method<double> m;
+    m.refinement = refine != 0;
+    m.ref_eta = eta;
+    m.ref_maxiter = maxiter;
+    m.regularization = 0.0;
+    Options<double> o;
+    return with_fixture<double>(fixture, p, [&](auto prob) {
+        constexpr int N = decltype(prob)::N;
+        Solver<double, N, decltype(prob)> solver(prob, m, o);
+        Vec<double, N> v, dv;
+        for (int k = 0; k < N; ++k) v[k] = x[k];
+        for (int k = 0; k < N; ++k) dv[k] = d[k];
+        Vec<double, N> Fx = solver.prob.F(v);
+        if (zero_fx)
+            for (int k = 0; k < N; ++k) Fx[k] = 0.0;
+        const Mat<double, N> J = solver.prob.J(v);
+        const double* fn = has_fnorm ? &fnorm : nullptr;
+        bool fl = false;
+        if (fac_kind == 0) {
+            dv = solver.template refine_with<float>(dv, lu_factorize<float, double, N>(J), nullptr, v, Fx, fl, fn);
+        } else if (fac_kind == 1) {
+            dv = solver.template refine_with<double>(dv, lu_factorize<double, double, N>(J), nullptr, v, Fx, fl, fn);
+        } else {
+            Mat<float, N> B{};
+            for (int k = 0; k < N; ++k) B[k + N * k] = -1.0f;
+            dv = solver.template refine_with<float>(dv, lu_factorize<float, float, N>(B), nullptr, v, Fx, fl, fn);
+        }
+        for (int k = 0; k < N; ++k) d[k] = dv[k];
+        *failed = fl ? 1 : 0;
+        return 0;
+    });
+}
+
+// read_mixed_stall (newton.jl:1437-1451) on a state (promoted, slow_steps) and a step
+// (fnorm, stepnorm, failures, promoted = false): out = {state promoted, slow_steps, info failures,
+// info promoted, info fnorm, info stepnorm}.
+void orc_read_mixed_stall(int promoted, int slow_steps, double fnorm, double stepnorm, int failures, double fnorm_old,
+                          double lin_rel, double* out) {
+    Method<double> m;
+    Options<double> o;
+    Affine2<double> prob{0.0, 0.0};
+    Solver<double, 2, Affine2<double>> solver(prob, m, o);
+    NState<double, 2> st;
+    st.promoted = promoted != 0;
+    st.slow_steps = slow_steps;
+    StepInfo<double> info = step_info<double>(SUCCESS, fnorm, stepnorm, failures, false, false);
+    solver.read_mixed_stall(st, info, fnorm_old, lin_rel);
+    out[0] = st.promoted ? 1.0 : 0.0;
+    out[1] = st.slow_steps;
+    out[2] = info.failures;
+    out[3] = info.promoted ? 1.0 : 0.0;
+    out[4] = info.fnorm;
+    out[5] = info.stepnorm;
+}
+
+// rmaxnorm (reductions.jl:67) of a length-L vector (L ≤ 4), in Float64 or Float32 (f32 = 1: the
+// entries are rounded to Float32 first).
+double orc_rmaxnorm(const double* a, int L, int f32) {
+    auto run = [&](auto zero) -> double {
+        using R = decltype(zero);
+        switch (L) {
+            case 0: return double(rmaxnorm(std::array<R, 0>{}));
+#define ORC_RM(LL) \
+    case LL: {     \
+        std::array<R, LL> v; for (int k = 0; k < LL; ++k) v[k] = R(a[k]); return double(rmaxnorm(v)); }
+            ORC_RM(1) ORC_RM(2) ORC_RM(3) ORC_RM(4)
+#undef ORC_RM
+        }
+        return nan_<double>();
     };
-    switch (fixture) {
-        case 101: return run(DiagSquare2<double>{p[0], p[1]});
-        case 102: return run(Affine2<double>{p[0], p[1]});
-        case 103: return run(InfJacobian1<double>{});
-    }
-    return GS_EINVAL;
+    return f32 ? run(0.0f) : run(0.0);
 }
 
 // One DogLeg step!! (newton.jl:1725-1737) from x0 on a DogLegPw residual (families.hpp), as the

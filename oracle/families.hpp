@@ -277,6 +277,162 @@ template <class T> struct DogLegPw {
     }
 };
 
+// Family 9, the quadratic map of the reference's mixed-precision tests (test/nonlinear/refinement.jl:
+// 40-96): u = A x, F_i = (u_i + c_i u_i²) − b_i, J = Diagonal(1 .+ 2c .* u) · A. f_mod_r1 / j_mod_r1
+// and f_ill_r1 / j_ill_r1 are A_MOD, B_MOD and A_ILL, B_ILL with c = (0.1, −0.2): u₂ − 0.2 u₂² and
+// u₂ + (−0.2) u₂² are the same IEEE operations, and so are T(0.2), T(0.4) and 2 T(0.1), 2 T(0.2).
+// Record: A (4, column-major), b (2), c (2).
+template <class T> struct QuadMap2 {
+    static constexpr int N = 2;
+    Mat<T, 2> A;
+    Vec<T, 2> b, c;
+    long nf = 0;
+    mutable long nj = 0;
+    Vec<T, 2> F(const Vec<T, 2>& x) {
+        ++nf;
+        const Vec<T, 2> u = matvec<T, 2>(A, x);
+        return {(u[0] + c[0] * (u[0] * u[0])) - b[0], (u[1] + c[1] * (u[1] * u[1])) - b[1]};
+    }
+    Mat<T, 2> J(const Vec<T, 2>& x) const {
+        ++nj;
+        const Vec<T, 2> u = matvec<T, 2>(A, x);
+        const T d0 = T(1) + (T(2) * c[0]) * u[0];
+        const T d1 = T(1) + (T(2) * c[1]) * u[1];
+        return {d0 * A[0], d1 * A[1], d0 * A[2], d1 * A[3]};
+    }
+};
+
+// Fixtures of the reference's mixed-precision and stopping tests, for the oracle's KATs:
+//   104: f_floor_r1 / j_floor_r1 (refinement.jl:131-139): F rounds x through Float32, so ‖F‖ floors
+//        near eps(Float32); J = Diagonal(2x) in Float64
+//   105: A x + c .* x.^3 − b, J = A + Diagonal(3c .* x.^2) (refinement.jl:690-720, Check 1); p = vec(A)
+//        column-major, c, b
+//   106: linear_problem(a) with linear_newton(b) (stopping.jl:28-29): F = a x, J = b; p = (a, b)
+//   107: F = x with J = −1 at x = 1 and 2 elsewhere (stopping.jl:705-712)
+//   108: FloorResidual (stopping.jl:661-666): F = (x − 1) + δ cos(x / (δ 10⁻⁶)), J = 1; p = (δ)
+//   109: CycleStep (stopping.jl:553-563) on F = x: the steps 1 → 1/2 → 1/4 → −1/4 → 1/4 → −1/4 …;
+//        p = (limited)
+//   110: JumpStep (stopping.jl:582-590) on F = x − 100: the steps 0 → 1 → 2 → 2 + 1/1024, then by 1
+// 109 and 110 are methods, not residuals: the solve loop takes their `custom_step` in place of the
+// Newton step (their J is never read).
+struct Floor2 {
+    static constexpr int N = 2;
+    long nf = 0;
+    mutable long nj = 0;
+    Vec<double, 2> F(const Vec<double, 2>& x) {
+        ++nf;
+        const float a = float(x[0]), b = float(x[1]);
+        return {double(a * a - 2.0f), double(b * b - 3.0f)};
+    }
+    Mat<double, 2> J(const Vec<double, 2>& x) const {
+        ++nj;
+        return {2.0 * x[0], 0.0, 0.0, 2.0 * x[1]};
+    }
+};
+template <class T> struct Cubic2 {
+    static constexpr int N = 2;
+    Mat<T, 2> A;
+    Vec<T, 2> c, b;
+    long nf = 0;
+    mutable long nj = 0;
+    Vec<T, 2> F(const Vec<T, 2>& x) {
+        ++nf;
+        const Vec<T, 2> Ax = matvec<T, 2>(A, x);
+        Vec<T, 2> r;
+        for (int i = 0; i < 2; ++i) r[i] = (Ax[i] + c[i] * ((x[i] * x[i]) * x[i])) - b[i];
+        return r;
+    }
+    Mat<T, 2> J(const Vec<T, 2>& x) const {
+        ++nj;
+        Mat<T, 2> J = A;
+        for (int i = 0; i < 2; ++i) J[i + 2 * i] = A[i + 2 * i] + (T(3) * c[i]) * (x[i] * x[i]);
+        return J;
+    }
+};
+template <class T> struct LinearConstJ1 {
+    static constexpr int N = 1;
+    T a, b;
+    long nf = 0;
+    mutable long nj = 0;
+    Vec<T, 1> F(const Vec<T, 1>& x) {
+        ++nf;
+        return {a * x[0]};
+    }
+    Mat<T, 1> J(const Vec<T, 1>&) const {
+        ++nj;
+        return {b};
+    }
+};
+template <class T> struct JumpJacobian1 {
+    static constexpr int N = 1;
+    long nf = 0;
+    mutable long nj = 0;
+    Vec<T, 1> F(const Vec<T, 1>& x) {
+        ++nf;
+        return {x[0]};
+    }
+    Mat<T, 1> J(const Vec<T, 1>& x) const {
+        ++nj;
+        return {x[0] == T(1) ? T(-1) : T(2)};
+    }
+};
+template <class T> struct FloorResidual1 {
+    static constexpr int N = 1;
+    T delta;
+    long nf = 0;
+    mutable long nj = 0;
+    Vec<T, 1> F(const Vec<T, 1>& x) {
+        ++nf;
+        return {(x[0] - T(1)) + delta * std::cos(x[0] / (delta * T(1e-6)))};
+    }
+    Mat<T, 1> J(const Vec<T, 1>&) const {
+        ++nj;
+        return {T(1)};
+    }
+};
+template <class T> struct CycleStep1 {
+    static constexpr int N = 1;
+    bool limited;
+    long nf = 0;
+    mutable long nj = 0;
+    Vec<T, 1> F(const Vec<T, 1>& x) {
+        ++nf;
+        return {x[0]};
+    }
+    Mat<T, 1> J(const Vec<T, 1>&) const { return {T(1)}; }
+    StepInfo<T> custom_step(Vec<T, 1>& x, Vec<T, 1>& Fx) {
+        const T v = x[0];
+        const T h = T(1) / T(2), q = T(1) / T(4);
+        const T vp = v == T(1) ? h : (v == h ? q : (v == q ? -q : q));
+        const Vec<T, 1> xp{vp};
+        const Vec<T, 1> Fp = F(xp);
+        StepInfo<T> info = step_info<T>(SUCCESS, std::fabs(Fp[0]), std::fabs(vp - v), 0, false, limited);
+        x = xp;
+        Fx = Fp;
+        return info;
+    }
+};
+template <class T> struct JumpStep1 {
+    static constexpr int N = 1;
+    long nf = 0;
+    mutable long nj = 0;
+    Vec<T, 1> F(const Vec<T, 1>& x) {
+        ++nf;
+        return {x[0] - T(100)};
+    }
+    Mat<T, 1> J(const Vec<T, 1>&) const { return {T(1)}; }
+    StepInfo<T> custom_step(Vec<T, 1>& x, Vec<T, 1>& Fx) {
+        const T v = x[0];
+        const T vp = v == T(0) ? T(1) : (v == T(1) ? T(2) : (v == T(2) ? T(2) + T(1) / T(1024) : v + T(1)));
+        const Vec<T, 1> xp{vp};
+        const Vec<T, 1> Fp = F(xp);
+        StepInfo<T> info = step_info<T>(SUCCESS, std::fabs(Fp[0]), std::fabs(vp - v), 0, false, false);
+        x = xp;
+        Fx = Fp;
+        return info;
+    }
+};
+
 // The two-step discrete action of the variational midpoint rule for Hénon–Heiles, a registered
 // action of StationarityProblem: x = (q₋, q, q₊) (6 entries), S(x) = L_d(q₋, q) + L_d(q, q₊),
 // L_d(a, b) = |b − a|²/(2h) − h V((a + b)/2), V(q) = (q₁² + q₂²)/2 + λ (q₁² q₂ − q₂³/3). ∇S and ∇²S

@@ -218,6 +218,7 @@ template <class R, size_t L> inline R rnorm(const std::array<R, L>& a) {
 
 // rmaxnorm (reductions.jl:67): max |aᵢ|, NaN-propagating.
 template <class R, size_t L> inline R rmaxnorm(const std::array<R, L>& a) {
+    if constexpr (L == 0) return R(0);
     R m = std::fabs(a[0]);
     for (size_t i = 1; i < L; ++i) m = jmax(m, std::fabs(a[i]));
     return m;
@@ -522,6 +523,12 @@ template <class R, int N, class P> struct DeflatedProb {
 template <class P, class = void> struct is_deflated_prob : std::false_type {};
 template <class P> struct is_deflated_prob<P, std::void_t<decltype(P::kDeflated)>> : std::true_type {};
 
+// A test fixture that is a method of its own (the reference's stopping tests define step!! for a
+// custom method, test/nonlinear/stopping.jl:553-590): its `custom_step` replaces the Newton step.
+template <class P, class = void> struct has_custom_step : std::false_type {};
+template <class P>
+struct has_custom_step<P, std::void_t<decltype(&P::custom_step)>> : std::true_type {};
+
 // grad_ratio_and_norm (src/nonlinear/newton.jl:1287-1298): ‖Jᴴ F‖/(‖J‖_F ‖F‖) and ‖J‖_F.
 template <class T, int N>
 inline void grad_ratio_and_norm(const Mat<T, N>& J, const Vec<T, N>& Fx, T fnorm, T& gr, T& jnorm) {
@@ -823,11 +830,13 @@ template <class R, int N, class P> struct Solver {
 
     // refine_passes!! with the product of `product(held, x, ·)`.
     template <class TF>
+    // `fnorm_in` is the caller's ‖F(x)‖ where it holds one (the `fnorm` argument of refine!!); a step
+    // passes none (newton_direction, newton.jl:1343-1358), and the norm is formed from Fx.
     Vec<R, N> refine_with(Vec<R, N> d, const StaticLU<TF, N>& fac, const Mat<R, N>* held, const Vec<R, N>& x,
-                          const Vec<R, N>& Fx, bool& failed) {
+                          const Vec<R, N>& Fx, bool& failed, const R* fnorm_in = nullptr) {
         failed = false;
         if (!m.refinement) return d;
-        R fnorm = rnorm(Fx);
+        R fnorm = fnorm_in ? *fnorm_in : rnorm(Fx);
         if (fnorm == R(0)) return d;
         if (!std::isfinite(fnorm)) {
             failed = true;
@@ -1354,6 +1363,7 @@ template <class R, int N, class P> struct Solver {
     }
 
     StepInfo<R> step(Vec<R, N>& x, Vec<R, N>& Fx, NState<R, N>& st) {
+        if constexpr (has_custom_step<P>::value) return prob.custom_step(x, Fx);
         StepInfo<R> info = m.glob == G_DOGLEG
                                ? (m.factor_f32 ? dogleg_mixed_step(x, Fx, st) : dogleg_plain_step(x, Fx, st))
                            : m.reuse > 0 ? (m.factor_f32 ? mixed_step_reuse(x, Fx, st) : plain_step_reuse(x, Fx, st))

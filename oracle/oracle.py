@@ -83,6 +83,20 @@ def lib():
         L.orc_dogleg_pw_step.argtypes = [ctypes.c_int, vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, vp]
         L.orc_solve_fixture_counts.restype = ctypes.c_int
         L.orc_solve_fixture_counts.argtypes = [ctypes.c_int, vp, vp, vp, vp, vp, vp]
+        L.orc_solve_fixture_t.restype = ctypes.c_int
+        L.orc_solve_fixture_t.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, vp, vp]
+        L.orc_fixture_eval.restype = ctypes.c_int
+        L.orc_fixture_eval.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp, vp, vp]
+        L.orc_fixture_step.restype = ctypes.c_int
+        L.orc_fixture_step.argtypes = [ctypes.c_int, vp, vp, vp, vp]
+        L.orc_refine_fixture.restype = ctypes.c_int
+        L.orc_refine_fixture.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, vp]
+        L.orc_read_mixed_stall.restype = None
+        L.orc_read_mixed_stall.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                           ctypes.c_double, ctypes.c_double, vp]
+        L.orc_rmaxnorm.restype = ctypes.c_double
+        L.orc_rmaxnorm.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         _LIB = L
     return _LIB
 
@@ -252,21 +266,78 @@ def rnorm(a) -> float:
     return lib().orc_rnorm(_ptr(a), len(a))
 
 
-def solve_fixture(fixture: int, method, options, x, p=(0.0, 0.0), counts: bool = False):
-    """One solve of a loop-test fixture (101 y.^2 − q, 102 2y − q, 103 x² − 4 with J = Inf); with
-    `counts`, also (residual calls, Jacobian calls)."""
+def solve_fixture(fixture: int, method, options, x, p=(0.0, 0.0), counts: bool = False, f32: bool = False):
+    """One solve of a fixture of oracle/families.hpp (101 y.^2 − q, 102 2y − q, 103 x² − 4 with J = Inf,
+    104 f_floor_r1, 105 the cubic of Check 1, 106 a x with J = b, 107 x with the jumping Jacobian,
+    108 FloorResidual, 109 CycleStep, 110 JumpStep; 9 the quadratic-map family); with `counts`, also
+    (residual calls, Jacobian calls). `f32` solves in Float32 (x, p and the status pass as doubles)."""
     x = np.ascontiguousarray(np.atleast_1d(np.asarray(x, dtype=np.float64))).copy()
-    pp = np.ascontiguousarray(np.asarray(p, dtype=np.float64))
+    pp = np.ascontiguousarray(np.atleast_1d(np.asarray(p, dtype=np.float64)))
     S = np.zeros(1, dtype=_status_dtype())
     c = np.zeros(2, dtype=np.int64)
     cm, co = method.to_c(), options.to_c()
-    rc = lib().orc_solve_fixture_counts(fixture, ctypes.byref(cm), ctypes.byref(co), _ptr(x), _ptr(pp), _ptr(S),
-                                        _ptr(c))
+    rc = lib().orc_solve_fixture_t(fixture, int(f32), ctypes.byref(cm), ctypes.byref(co), _ptr(x), _ptr(pp), _ptr(S),
+                                   _ptr(c))
     if rc != 0:
         raise RuntimeError(f"oracle fixture failed with {rc}")
     if counts:
         return x, S[0], (int(c[0]), int(c[1]))
     return x, S[0]
+
+
+def fixture_eval(fixture: int, x, p=(0.0, 0.0), f32: bool = False):
+    """(F(x), J(x)) of a fixture in Float64 (or Float32, returned as doubles)."""
+    x = np.ascontiguousarray(np.atleast_1d(np.asarray(x, dtype=np.float64)))
+    pp = np.ascontiguousarray(np.atleast_1d(np.asarray(p, dtype=np.float64)))
+    n = len(x)
+    F = np.zeros(n)
+    J = np.zeros(n * n)
+    assert lib().orc_fixture_eval(fixture, int(f32), _ptr(pp), _ptr(x), _ptr(F), _ptr(J)) == 0
+    return F, J.reshape(n, n, order="F")
+
+
+def fixture_step(fixture: int, method, x, p=(0.0, 0.0)) -> dict:
+    """One step!! of `method` from x on a fixture, after start!!: x₊ and the step's record."""
+    x = np.ascontiguousarray(np.atleast_1d(np.asarray(x, dtype=np.float64))).copy()
+    pp = np.ascontiguousarray(np.atleast_1d(np.asarray(p, dtype=np.float64)))
+    out = np.zeros(8)
+    cm = method.to_c()
+    assert lib().orc_fixture_step(fixture, ctypes.byref(cm), _ptr(x), _ptr(pp), _ptr(out)) == 0
+    return dict(x=x, code=int(out[0]), failures=int(out[1]), promoted=bool(out[2]), info_promoted=bool(out[3]),
+                fnorm=out[4], stepnorm=out[5], radius=out[6], f_calls=int(out[7]))
+
+
+def refine_fixture(fixture: int, p, x, d, fac: str = "f32", refinement=(1.4901161193847656e-08, 5), zero_fx=False,
+                   fnorm=None):
+    """refine!! on a fixture at x from the direction d: `fac` is "f32" or "f64" (J(x) factorised in
+    that precision) or "bad" (the Float32 matrix −I); `refinement` = (η, maxiter) or None for
+    NoRefinement(); `fnorm` the caller's ‖F(x)‖ or None. Returns (d, failed)."""
+    x = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+    pp = np.ascontiguousarray(np.atleast_1d(np.asarray(p, dtype=np.float64)))
+    dd = np.ascontiguousarray(np.asarray(d, dtype=np.float64)).copy()
+    kind = {"f32": 0, "f64": 1, "bad": 2}[fac]
+    eta, maxiter = refinement if refinement is not None else (0.0, 0)
+    failed = ctypes.c_int()
+    assert lib().orc_refine_fixture(fixture, _ptr(pp), _ptr(x), _ptr(dd), kind, int(refinement is not None),
+                                    float(eta), int(maxiter), int(zero_fx), int(fnorm is not None),
+                                    float(fnorm) if fnorm is not None else 0.0, ctypes.byref(failed)) == 0
+    return dd, bool(failed.value)
+
+
+def read_mixed_stall(promoted: bool, slow_steps: int, fnorm: float, stepnorm: float, fnorm_old: float,
+                     lin_rel: float, failures: int = 0) -> dict:
+    """read_mixed_stall on a state and a SUCCESS step: the state and step it returns."""
+    out = np.zeros(6)
+    lib().orc_read_mixed_stall(int(promoted), int(slow_steps), float(fnorm), float(stepnorm), int(failures),
+                               float(fnorm_old), float(lin_rel), _ptr(out))
+    return dict(promoted=bool(out[0]), slow_steps=int(out[1]), failures=int(out[2]), info_promoted=bool(out[3]),
+                fnorm=out[4], stepnorm=out[5])
+
+
+def rmaxnorm(a, f32: bool = False) -> float:
+    """rmaxnorm of a vector of length ≤ 4 in Float64 (or Float32)."""
+    a = np.ascontiguousarray(np.atleast_1d(np.asarray(a, dtype=np.float64)))
+    return lib().orc_rmaxnorm(_ptr(a), len(a), int(f32))
 
 
 def solve_expsum_batch(method, options, X8, K, m, y_offset, Y, nthreads: int = 0):
