@@ -17,6 +17,8 @@
 namespace gs {
 
 int set_error(int code, const std::string& msg);
+// One call at a time per device, held through the synchronisation (defined with gs_solve_batch_f64).
+std::unique_lock<std::recursive_mutex> device_call_lock();
 
 namespace {
 
@@ -141,6 +143,7 @@ int gs_tally_codes(const uint8_t* codes, int64_t nsys, int64_t counts[6], void* 
     if (nsys == 0) return GS_OK;
     if (!codes) return set_error(GS_EINVAL, "codes must not be NULL");
     hipStream_t st = static_cast<hipStream_t>(stream);
+    auto lock = device_call_lock();
     const int blocks = blocks_for(nsys);
     void* ws = nullptr;
     int rc = workspace(size_t(blocks) * 6 * sizeof(int64_t), &ws);
@@ -163,6 +166,7 @@ int gs_batch_summary_f64(const gs_status_f64* S, int64_t nsys, double* fnorm, in
     if (nsys == 0) return GS_OK;
     if (!S) return set_error(GS_EINVAL, "S must not be NULL");
     hipStream_t st = static_cast<hipStream_t>(stream);
+    auto lock = device_call_lock();
     const int blocks = blocks_for(nsys);
     void* ws = nullptr;
     int rc = workspace(size_t(blocks) * sizeof(SummaryPartial), &ws);

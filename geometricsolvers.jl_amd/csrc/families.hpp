@@ -160,6 +160,46 @@ template <class R> struct FamTanhLinear3T {
     }
 };
 
+// The quadratic map of the reference's mixed-precision tests (test/nonlinear/refinement.jl:40-96):
+// u = A x (StaticArrays' muladd fold), F_i = (u_i + c_i u_i²) − b_i, J = Diagonal(1 .+ 2c .* u) · A;
+// record = vec(A) (column-major, 4), b (2), c (2). f_mod_r1 and f_ill_r1 take c = (0.1, −0.2).
+template <class R> struct FamQuadMap2T {
+    using T = R;
+    static constexpr int N = 2;
+    T A[2][2], b[2], c[2];
+    GS_DEV void load(const FamArgs& a, int64_t idx) {
+        const T* p = reinterpret_cast<const T*>(record(a, idx));
+        A[0][0] = p[0];
+        A[1][0] = p[1];
+        A[0][1] = p[2];
+        A[1][1] = p[3];
+        b[0] = p[4];
+        b[1] = p[5];
+        c[0] = p[6];
+        c[1] = p[7];
+    }
+    GS_DEV void u_of(const T (&x)[2], T (&u)[2]) const {
+#pragma unroll
+        for (int i = 0; i < 2; ++i) u[i] = tfma(A[i][1], x[1], A[i][0] * x[0]);
+    }
+    GS_DEV void F(const T (&x)[2], T (&r)[2]) const {
+        T u[2];
+        u_of(x, u);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) r[i] = (u[i] + c[i] * (u[i] * u[i])) - b[i];
+    }
+    GS_DEV void J(const T (&x)[2], T (&Jm)[2][2]) const {
+        T u[2];
+        u_of(x, u);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const T d = T(1) + (T(2) * c[i]) * u[i];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) Jm[i][j] = d * A[i][j];
+        }
+    }
+};
+
 // Implicit midpoint of Hénon–Heiles, x = (q₁, q₂, p₁, p₂), record = x_k (4), shared[0] = h:
 // F = x − x_k − h f((x_k + x)/2), f = (p₁, p₂, −q₁ − 2q₁q₂, −q₂ − q₁² + q₂²), J = I − (h/2) Df.
 template <class R> struct FamHenonHeilesMidpointT {

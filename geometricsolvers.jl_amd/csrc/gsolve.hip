@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <cmath>
+#include <mutex>
 #include <string>
 
 #include "../../include/gsolve.h"
@@ -103,6 +104,7 @@ static const FamilySpec* family_spec(int id) {
         {GS_FAM_HETERO2, 2, 1, 8, 8, 8, 8},
         {GS_FAM_SCALAR_SQUARE, 1, 1, 8, 8, 4, 4},
         {GS_FAM_TANH_LINEAR3, 3, 12, 96, 8, 48, 4},
+        {GS_FAM_QUAD_MAP2, 2, 8, 64, 8, 32, 4},
         {GS_FAM_HENON_HEILES_MIDPOINT, 4, 4, 32, 16, 16, 16},
         {GS_FAM_FPU_GL3, 48, 32, 256, 8, 0, 0},
         {GS_ACT_HENON_HEILES, 6, 2, 16, 8, 0, 0},
@@ -162,12 +164,104 @@ static int validate_batch(const gs_method* method, const gs_family* family, int6
         return fail(GS_EINVAL, "P and p_stride must be " + std::to_string(align) + "-byte aligned for this family");
     if ((reinterpret_cast<uintptr_t>(X) % sizeof(T)) != 0)
         return fail(GS_EINVAL, "X must be " + std::to_string(sizeof(T)) + "-byte aligned");
-    if (workgroupsize > kSmallMaxWorkgroup)
-        return fail(GS_EINVAL, "workgroupsize " + std::to_string(workgroupsize) +
-                                   " exceeds the solve kernel's maximum of " + std::to_string(kSmallMaxWorkgroup) +
-                                   " work-items");
     return GS_OK;
 }
+
+// ---- One call at a time per device ----
+// The library keeps device buffers of its own per device (workspaces, hand-over flags, the order
+// check's bitmap). Every entry point that uses one holds
+// its device's lock from its first enqueue through its synchronisation, so two calls on one device
+// never share such a buffer at once, whatever streams they run on.
+static std::recursive_mutex g_device_lock[64];
+
+std::unique_lock<std::recursive_mutex> device_call_lock() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    return std::unique_lock<std::recursive_mutex>(g_device_lock[dev]);
+}
+
+// ---- isperm(order) on the device (batched.jl:133-135) ----
+// Word 0 of `w` is the error flag, words 1.. a bitmap of the values seen; a fill kernel clears both
+// (no memset node, so the sequence stays valid inside a captured graph).
+__global__ __launch_bounds__(256) void order_clear_kernel(uint32_t* w, int64_t nwords) {
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < nwords; i += int64_t(gridDim.x) * 256) w[i] = 0u;
+}
+
+__global__ __launch_bounds__(256) void order_mark_kernel(const int64_t* order, int64_t nsys, uint32_t* w) {
+    uint32_t* bits = w + 1;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < nsys; i += int64_t(gridDim.x) * 256) {
+        const int64_t v = order[i];
+        if (v < 1 || v > nsys) {
+            atomicOr(w, 1u);  // out of 1:nsys
+            continue;
+        }
+        const uint64_t k = uint64_t(v - 1);
+        const uint32_t bit = 1u << (k & 31u);
+        if (atomicOr(bits + (k >> 5), bit) & bit) atomicOr(w, 2u);  // a value repeats
+    }
+}
+
+// Words of the check for a batch of nsys systems: the flag and ceil(nsys / 32) bitmap words.
+static int64_t order_words(int64_t nsys) { return 1 + (nsys + 31) / 32; }
+
+// The cached device buffer of the check (the entry points that allocate), per device; grown under
+// the device's call lock, freed by gs_shutdown.
+struct OrderWorkspace {
+    uint32_t* ptr = nullptr;
+    int64_t words = 0;
+};
+static OrderWorkspace g_order_ws[64];
+
+static int order_workspace(int64_t words, uint32_t** out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return fail(GS_EHIP, "hipGetDevice failed");
+    OrderWorkspace& w = g_order_ws[dev];
+    if (w.words < words) {
+        if (w.ptr) (void)hipFree(w.ptr);
+        w = OrderWorkspace{};
+        if (hipMalloc(reinterpret_cast<void**>(&w.ptr), size_t(words) * 4) != hipSuccess)
+            return fail(GS_EHIP, "hipMalloc of the order check's bitmap failed");
+        w.words = words;
+    }
+    *out = w.ptr;
+    return GS_OK;
+}
+
+void release_order() {
+    for (int d = 0; d < 64; ++d) {
+        if (!g_order_ws[d].ptr) continue;
+        if (hipSetDevice(d) == hipSuccess) (void)hipFree(g_order_ws[d].ptr);
+        g_order_ws[d] = OrderWorkspace{};
+    }
+}
+
+// Refuse an `order` that is not a permutation of 1:nsys before any solve writes anything (the
+// reference refuses it on the host, batched.jl:134-135). `w` is a device buffer of at least
+// order_words(nsys) words, or null for the cached workspace. Synchronises `st`.
+static int check_order(const int64_t* order, int64_t nsys, hipStream_t st, uint32_t* w) {
+    if (!order || nsys == 0) return GS_OK;
+    const int64_t words = order_words(nsys);
+    if (!w) {
+        const int rc = order_workspace(words, &w);
+        if (rc != GS_OK) return rc;
+    }
+    const unsigned gc = unsigned(std::min<int64_t>((words + 255) / 256, 4096));
+    const unsigned gm = unsigned(std::min<int64_t>((nsys + 255) / 256, 8192));
+    hipLaunchKernelGGL(order_clear_kernel, dim3(gc), dim3(256), 0, st, w, words);
+    hipLaunchKernelGGL(order_mark_kernel, dim3(gm), dim3(256), 0, st, order, nsys, w);
+    uint32_t flag = 0;
+    if (hipGetLastError() != hipSuccess || hipMemcpyAsync(&flag, w, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return fail(GS_EHIP, "the order check failed to run");
+    if (flag != 0) return fail(GS_EINVAL, "order is not a permutation of 1:" + std::to_string(nsys));
+    return GS_OK;
+}
+
+// The workgroup of a lane-per-system launch: a requested size above the kernels' bound of
+// kSmallMaxWorkgroup work-items runs as workgroups of that bound (the systems are independent, so
+// how they group changes no result; batched.jl:200-210 leaves any size up to the device's maximum to
+// the launch).
+static int small_workgroup(int32_t w) { return (w <= 0 || w > kSmallMaxWorkgroup) ? kSmallMaxWorkgroup : w; }
 
 // The launch, its timing bracket, and the synchronisation of `_launch!` (batched.jl:171-195).
 static int finish_launch(int rc, hipStream_t st, bool timed) {
@@ -198,9 +292,14 @@ void gs_abi_sizes(int64_t out[4]) {
     out[3] = sizeof(gs_family);
 }
 
-int gs_solve_batch_f64(const gs_method* method, const gs_family* family, int64_t nsys, double* X, const void* P,
-                       int64_t p_stride, gs_status_f64* S, uint8_t* codes, const int64_t* order,
-                       const gs_options_f64* options, int32_t workgroupsize, void* stream) {
+}  // extern "C"
+
+namespace gs {
+
+// gs_solve_batch_f64.
+static int solve_batch_f64(const gs_method* method, const gs_family* family, int64_t nsys, double* X, const void* P,
+                           int64_t p_stride, gs_status_f64* S, uint8_t* codes, const int64_t* order,
+                           const gs_options_f64* options, int32_t workgroupsize, void* stream) {
     g_last_error.clear();
     const FamilySpec* fs = nullptr;
     int rc = validate_batch<double>(method, family, nsys, X, P, p_stride, S, codes, options, workgroupsize, &fs);
@@ -208,6 +307,9 @@ int gs_solve_batch_f64(const gs_method* method, const gs_family* family, int64_t
     if (family->id == GS_ACT_HENON_HEILES)
         return set_error(GS_EFAMILY, "an action is solved as a StationarityProblem (gs_solve_batch_stationarity_f64)");
     if (nsys == 0) return rc;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    auto lock = device_call_lock();
+    if ((rc = check_order(order, nsys, st, nullptr)) != GS_OK) return rc;
     KernelArgs a;
     std::memset(&a, 0, sizeof(a));
     a.fam.P = static_cast<const char*>(P);
@@ -220,8 +322,7 @@ int gs_solve_batch_f64(const gs_method* method, const gs_family* family, int64_t
     a.order = order;
     a.q = dev_params(*method, *options);
     const bool diag = S != nullptr;
-    const int wg = workgroupsize > 0 ? workgroupsize : kSmallMaxWorkgroup;
-    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int wg = small_workgroup(workgroupsize);
     const bool mixed = method->factor_precision == GS_FACTOR_F32;
     const bool refine = method->refinement == GS_REFINEMENT;
     const int glob = method->globalization;
@@ -252,6 +353,16 @@ int gs_solve_batch_f64(const gs_method* method, const gs_family* family, int64_t
     else
         rc = launch_small_family<kBacktracking, false>(family->id, a, mixed, refine, diag, wg, st);
     return finish_launch(rc, st, timed);
+}
+
+}  // namespace gs
+
+extern "C" {
+
+int gs_solve_batch_f64(const gs_method* method, const gs_family* family, int64_t nsys, double* X, const void* P,
+                       int64_t p_stride, gs_status_f64* S, uint8_t* codes, const int64_t* order,
+                       const gs_options_f64* options, int32_t workgroupsize, void* stream) {
+    return solve_batch_f64(method, family, nsys, X, P, p_stride, S, codes, order, options, workgroupsize, stream);
 }
 
 int gs_solve_batch_deflated_f64(const gs_method* method, const gs_family* family, const gs_deflation* deflation,
@@ -302,8 +413,10 @@ int gs_solve_batch_deflated_f64(const gs_method* method, const gs_family* family
     a.order = order;
     a.q = dev_params(*method, *options);
     const bool diag = S != nullptr;
-    const int wg = workgroupsize > 0 ? workgroupsize : kSmallMaxWorkgroup;
+    const int wg = small_workgroup(workgroupsize);
     hipStream_t st = static_cast<hipStream_t>(stream);
+    auto lock = device_call_lock();
+    if ((rc = check_order(order, nsys, st, nullptr)) != GS_OK) return rc;
     const bool mixed = method->factor_precision == GS_FACTOR_F32;
     const bool refine = method->refinement == GS_REFINEMENT;
     bool timed = timing_begin(st);
@@ -374,8 +487,10 @@ int gs_solve_batch_stationarity_f64(const gs_method* method, const gs_family* ac
     a.order = order;
     a.q = dev_params(*method, *options);
     const bool diag = S != nullptr;
-    const int wg = workgroupsize > 0 ? workgroupsize : kSmallMaxWorkgroup;
+    const int wg = small_workgroup(workgroupsize);
     hipStream_t st = static_cast<hipStream_t>(stream);
+    auto lock = device_call_lock();
+    if ((rc = check_order(order, nsys, st, nullptr)) != GS_OK) return rc;
     const bool mixed = method->factor_precision == GS_FACTOR_F32;
     const bool refine = method->refinement == GS_REFINEMENT;
     bool timed = timing_begin(st);
@@ -426,8 +541,10 @@ int gs_solve_batch_reparametrized_f64(const gs_method* method, const gs_family* 
     a.order = order;
     a.q = dev_params(*method, *options);
     const bool diag = S != nullptr;
-    const int wg = workgroupsize > 0 ? workgroupsize : kSmallMaxWorkgroup;
+    const int wg = small_workgroup(workgroupsize);
     hipStream_t st = static_cast<hipStream_t>(stream);
+    auto lock = device_call_lock();
+    if ((rc = check_order(order, nsys, st, nullptr)) != GS_OK) return rc;
     const bool mixed = method->factor_precision == GS_FACTOR_F32;
     const bool refine = method->refinement == GS_REFINEMENT;
     bool timed = timing_begin(st);
@@ -454,8 +571,10 @@ int gs_solve_batch_f32(const gs_method* method, const gs_family* family, int64_t
     a.codes = codes;
     a.order = order;
     a.q = dev_params_t<float>(*method, *options);
-    const int wg = workgroupsize > 0 ? workgroupsize : kSmallMaxWorkgroup;
+    const int wg = small_workgroup(workgroupsize);
     hipStream_t st = static_cast<hipStream_t>(stream);
+    auto lock = device_call_lock();
+    if ((rc = check_order(order, nsys, st, nullptr)) != GS_OK) return rc;
     bool timed = timing_begin(st);
     rc = launch_small_f32(family->id, method->globalization, method->reuse > 0, a,
                           method->refinement == GS_REFINEMENT, S != nullptr, wg, st);
@@ -471,6 +590,7 @@ int gs_shutdown(void) {
     release_mlp();
     release_reduce();
     release_mid();
+    release_order();
     if (g_ev[0]) (void)hipEventDestroy(g_ev[0]);
     if (g_ev[1]) (void)hipEventDestroy(g_ev[1]);
     g_ev[0] = g_ev[1] = nullptr;

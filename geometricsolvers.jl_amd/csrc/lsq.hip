@@ -28,6 +28,8 @@
 namespace gs {
 
 int set_error(int code, const std::string& msg);
+// One call at a time per device, held through the synchronisation (defined with gs_solve_batch_f64).
+std::unique_lock<std::recursive_mutex> device_call_lock();
 
 namespace {
 
@@ -990,6 +992,7 @@ int gs_solve_expsum_lsq_f64(const gs_method* method, int64_t nfit, double* X, co
     if ((reinterpret_cast<uintptr_t>(X) % 8) || (reinterpret_cast<uintptr_t>(S) % 8))
         return set_error(GS_EINVAL, "X and S must be 8-byte aligned");
     hipStream_t st = static_cast<hipStream_t>(stream);
+    auto lock = device_call_lock();
     // workspace: bins, cursors, 5 bucket starts, 4 queues, 1 error count, the ordered list
     const size_t nb = size_t(kBins);
     const int64_t words = int64_t(2 * nb + 5 + 4 + 1);

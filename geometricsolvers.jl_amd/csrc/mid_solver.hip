@@ -946,8 +946,10 @@ void release_mid() {
 int launch_fpu_gl3(const MidLaunch& L0, hipStream_t st) {
     if (L0.nsys > 0x7fffffffLL) return set_error(GS_EINVAL, "a wave-per-system batch holds at most 2^31-1 systems");
     MidLaunch L = L0;
-    const int rc = redo_flags(size_t(L.nsys), &L.redo);
-    if (rc != GS_OK) return rc;
+    if (!L.redo) {
+        const int rc = redo_flags(size_t(L.nsys), &L.redo);
+        if (rc != GS_OK) return rc;
+    }
     if (L.mixed) return L.refine ? launch_variant<true, true>(L, st) : launch_variant<true, false>(L, st);
     return L.refine ? launch_variant<false, true>(L, st) : launch_variant<false, false>(L, st);
 }

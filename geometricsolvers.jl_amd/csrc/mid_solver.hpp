@@ -28,7 +28,8 @@ struct MidLaunch {
     uint8_t* codes;
     const int64_t* order;
     bool mixed, refine;
-    uint8_t* redo;        // the hand-over flags between the two kernels (set by launch_fpu_gl3)
+    uint8_t* redo;        // the hand-over flags between the two kernels, nsys bytes; null: the
+                          // library's own (launch_fpu_gl3 sets them)
 };
 
 // The FPU-β Gauss–Legendre s = 3 stage system (GS_FAM_FPU_GL3, n = 48) with the Static

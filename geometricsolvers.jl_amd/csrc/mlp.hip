@@ -26,6 +26,8 @@
 namespace gs {
 
 int set_error(int code, const std::string& msg);
+// One call at a time per device, held through the synchronisation (defined with gs_solve_batch_f64).
+std::unique_lock<std::recursive_mutex> device_call_lock();
 
 namespace {
 
@@ -653,6 +655,7 @@ int gs_solve_learned_henon_heiles_f64(const gs_mlp* net, const gs_method* method
         (reinterpret_cast<uintptr_t>(S) % 8) || (reinterpret_cast<uintptr_t>(net->packed) % 16))
         return set_error(GS_EINVAL, "Xk, X and the packed weights must be 16-byte aligned, S 8-byte aligned");
     hipStream_t st = static_cast<hipStream_t>(stream);
+    auto lock = device_call_lock();
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return set_error(GS_EHIP, "hipGetDevice failed");
     const int v = method->factor_precision == GS_FACTOR_F32 ? 1 : 0;

@@ -169,6 +169,7 @@ int launch_small_family(int family, const KernelArgsT<T>& a, bool mixed, bool re
         case GS_FAM_HETERO2: return launch_glob<FamHetero2T<T>, GLOB, REUSE>(a, mixed, refine, diag, wg, st);
         case GS_FAM_SCALAR_SQUARE: return launch_glob<FamScalarSquareT<T>, GLOB, REUSE>(a, mixed, refine, diag, wg, st);
         case GS_FAM_TANH_LINEAR3: return launch_glob<FamTanhLinear3T<T>, GLOB, REUSE>(a, mixed, refine, diag, wg, st);
+        case GS_FAM_QUAD_MAP2: return launch_glob<FamQuadMap2T<T>, GLOB, REUSE>(a, mixed, refine, diag, wg, st);
         case GS_FAM_HENON_HEILES_MIDPOINT:
             return launch_glob<FamHenonHeilesMidpointT<T>, GLOB, REUSE>(a, mixed, refine, diag, wg, st);
         default: return set_error(GS_EFAMILY, "problem family is not registered");

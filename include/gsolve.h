@@ -16,6 +16,9 @@
  *  - The return value is 0 or a negative GS_E* code; gs_last_error() then holds the message.
  *    Numerical failures never surface here: they are ReturnCodes inside S / codes.
  *  - No C++ exception crosses this boundary.
+ *  - The library keeps device buffers of its own (workspaces, flags); calls that use them on one
+ *    device are serialised by a per-device lock held from the first enqueue to the synchronisation,
+ *    so two threads may call on the same device on different streams.
  */
 #ifndef GSOLVE_H
 #define GSOLVE_H
@@ -193,8 +196,11 @@ typedef struct gs_family {
  *   S        nsys statuses (may be NULL when codes != NULL).
  *   codes    nsys ReturnCode bytes (may be NULL when S != NULL) — the counts form's buffer.
  *   order    NULL, or a permutation of 1..nsys (1-based, as Julia): work-item i solves order[i].
- *            The caller validates it (isperm, batched.jl:134); the library reads it as given.
- *   workgroupsize 0 = library default, else the workgroup size of the launch (1..1024).
+ *            The library checks it on the device before any solve (isperm, batched.jl:134-135):
+ *            an entry outside 1..nsys or a repeated one returns GS_EINVAL with nothing written.
+ *   workgroupsize 0 = library default, else the workgroup size of the launch (1..1024); a size
+ *            above a kernel's own bound runs as workgroups of that bound (the systems are
+ *            independent, so the grouping changes no result).
  */
 int gs_solve_batch_f64(const gs_method* method, const gs_family* family, int64_t nsys,
                        double* X, const void* P, int64_t p_stride, gs_status_f64* S,

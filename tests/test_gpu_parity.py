@@ -334,14 +334,16 @@ def test_workgroup_sizes_agree(gs, workloads, device):
     xk = workloads.henon_heiles_states(n, seed=3)
     opts = gs.Options(f_abstol=1e-12, f_reltol=0.0)
     ref = run_gpu(gs, device, gs.Newton(), gs.HenonHeilesMidpoint(), xk.copy(), gs.BatchParams(0.1, xk), opts)
-    for ws in (1, 7, 64, 256):
+    # sizes above the kernel's bound of 256 run as workgroups of 256, as any size up to the device's
+    # maximum of 1024 reaches the reference's launch (batched.jl:200-210)
+    for ws in (1, 7, 64, 256, 257, 1000, 1024):
         got = run_gpu(gs, device, gs.Newton(), gs.HenonHeilesMidpoint(), xk.copy(), gs.BatchParams(0.1, xk), opts,
                       workgroupsize=ws)
         assert np.array_equal(got[0], ref[0]) and same_bytes(got[1], ref[1])
-    # a size above the kernel's maximum is reported before any launch, as an ArgumentError
-    with pytest.raises(gs.ArgumentError, match="exceeds the solve kernel's maximum of 256"):
+    # above the device's maximum the binding refuses it before any launch, as an ArgumentError
+    with pytest.raises(gs.ArgumentError, match="exceeds the device maximum of 1024"):
         run_gpu(gs, device, gs.Newton(), gs.HenonHeilesMidpoint(), xk.copy(), gs.BatchParams(0.1, xk), opts,
-                workgroupsize=1024)
+                workgroupsize=1025)
 
 
 def test_batch_summary(gs, workloads, device):
