@@ -205,6 +205,7 @@ class GsMlp(ctypes.Structure):
 # Every symbol that include/gsolve.h declares.
 EXPORTED_SYMBOLS = (
     "gs_solve_batch_f64",
+    "gs_solve_batch_f64_noalloc",
     "gs_tally_codes",
     "gs_batch_summary_f64",
     "gs_last_error",
@@ -253,6 +254,8 @@ def load_library(path: os.PathLike | None = None):
         ctypes.POINTER(GsMethod), ctypes.POINTER(GsFamily), ctypes.c_int64, vp, vp, ctypes.c_int64,
         vp, vp, vp, ctypes.POINTER(GsOptionsF64), ctypes.c_int32, vp,
     ]
+    lib.gs_solve_batch_f64_noalloc.restype = ctypes.c_int
+    lib.gs_solve_batch_f64_noalloc.argtypes = lib.gs_solve_batch_f64.argtypes
     lib.gs_tally_codes.restype = ctypes.c_int
     lib.gs_tally_codes.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64), vp]
     lib.gs_batch_summary_f64.restype = ctypes.c_int

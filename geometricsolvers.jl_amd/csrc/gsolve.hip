@@ -169,7 +169,7 @@ static int validate_batch(const gs_method* method, const gs_family* family, int6
 
 // ---- One call at a time per device ----
 // The library keeps device buffers of its own per device (workspaces, hand-over flags, the order
-// check's bitmap). Every entry point that uses one holds
+// check's bitmap, the no-allocation variant's `scratch` array). Every entry point that uses one holds
 // its device's lock from its first enqueue through its synchronisation, so two calls on one device
 // never share such a buffer at once, whatever streams they run on.
 static std::recursive_mutex g_device_lock[64];
@@ -257,6 +257,34 @@ static int check_order(const int64_t* order, int64_t nsys, hipStream_t st, uint3
     return GS_OK;
 }
 
+// ---- The no-allocation variant's scratch array ----
+// One array per GPU the library runs on, held until the process ends. Within a call it serves, in
+// turn, the order check (flag and bitmap: up to kNoAllocMaxOrdered systems) and then the n = 48
+// solve's hand-over flags (one byte per system: up to kNoAllocMaxMid systems); the order check clears
+// what it reads, and every n = 48 system writes its own flag before the second launch reads it.
+constexpr int64_t kNoAllocMaxOrdered = int64_t(1) << 24;
+constexpr int64_t kNoAllocMaxMid = int64_t(1) << 20;
+constexpr size_t kScratchBytes = size_t(1 + (kNoAllocMaxOrdered + 31) / 32) * 4;
+static_assert(kScratchBytes >= size_t(kNoAllocMaxMid), "the flags of the n = 48 solve fit the scratch array");
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+static uint8_t* g_scratch_ptr[64] = {};
+
+// The device address of this device's `scratch` (the array's name in host code is a stand-in, not a
+// device address), looked up at the first call on each device, under the device's call lock.
+static int scratch_address(uint8_t** out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return fail(GS_EHIP, "hipGetDevice failed");
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(GS_EHIP, std::string("hipGetSymbolAddress of the scratch array failed: ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<uint8_t*>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return GS_OK;
+}
+
 // The workgroup of a lane-per-system launch: a requested size above the kernels' bound of
 // kSmallMaxWorkgroup work-items runs as workgroups of that bound (the systems are independent, so
 // how they group changes no result; batched.jl:200-210 leaves any size up to the device's maximum to
@@ -296,10 +324,11 @@ void gs_abi_sizes(int64_t out[4]) {
 
 namespace gs {
 
-// gs_solve_batch_f64.
+// gs_solve_batch_f64, and with `noalloc` its variant that allocates nothing: the order check's
+// buffer and the n = 48 solve's flags are then this device's `scratch` array.
 static int solve_batch_f64(const gs_method* method, const gs_family* family, int64_t nsys, double* X, const void* P,
                            int64_t p_stride, gs_status_f64* S, uint8_t* codes, const int64_t* order,
-                           const gs_options_f64* options, int32_t workgroupsize, void* stream) {
+                           const gs_options_f64* options, int32_t workgroupsize, void* stream, bool noalloc) {
     g_last_error.clear();
     const FamilySpec* fs = nullptr;
     int rc = validate_batch<double>(method, family, nsys, X, P, p_stride, S, codes, options, workgroupsize, &fs);
@@ -307,9 +336,17 @@ static int solve_batch_f64(const gs_method* method, const gs_family* family, int
     if (family->id == GS_ACT_HENON_HEILES)
         return set_error(GS_EFAMILY, "an action is solved as a StationarityProblem (gs_solve_batch_stationarity_f64)");
     if (nsys == 0) return rc;
+    if (noalloc && order && nsys > kNoAllocMaxOrdered)
+        return fail(GS_EINVAL, "gs_solve_batch_f64_noalloc checks an order of at most " +
+                                   std::to_string(kNoAllocMaxOrdered) + " systems; got " + std::to_string(nsys));
+    if (noalloc && family->id == GS_FAM_FPU_GL3 && nsys > kNoAllocMaxMid)
+        return fail(GS_EINVAL, "gs_solve_batch_f64_noalloc solves at most " + std::to_string(kNoAllocMaxMid) +
+                                   " FpuGl3 systems; got " + std::to_string(nsys));
     hipStream_t st = static_cast<hipStream_t>(stream);
     auto lock = device_call_lock();
-    if ((rc = check_order(order, nsys, st, nullptr)) != GS_OK) return rc;
+    uint8_t* scr = nullptr;
+    if (noalloc && (rc = scratch_address(&scr)) != GS_OK) return rc;
+    if ((rc = check_order(order, nsys, st, reinterpret_cast<uint32_t*>(scr))) != GS_OK) return rc;
     KernelArgs a;
     std::memset(&a, 0, sizeof(a));
     a.fam.P = static_cast<const char*>(P);
@@ -342,6 +379,7 @@ static int solve_batch_f64(const gs_method* method, const gs_family* family, int
         L.order = order;
         L.mixed = mixed;
         L.refine = refine;
+        L.redo = scr;  // null: the library's own flags
         return finish_launch(launch_fpu_gl3(L, st), st, timed);
     }
     if (glob == GS_DOGLEG)
@@ -362,7 +400,16 @@ extern "C" {
 int gs_solve_batch_f64(const gs_method* method, const gs_family* family, int64_t nsys, double* X, const void* P,
                        int64_t p_stride, gs_status_f64* S, uint8_t* codes, const int64_t* order,
                        const gs_options_f64* options, int32_t workgroupsize, void* stream) {
-    return solve_batch_f64(method, family, nsys, X, P, p_stride, S, codes, order, options, workgroupsize, stream);
+    return solve_batch_f64(method, family, nsys, X, P, p_stride, S, codes, order, options, workgroupsize, stream,
+                           false);
+}
+
+int gs_solve_batch_f64_noalloc(const gs_method* method, const gs_family* family, int64_t nsys, double* X,
+                               const void* P, int64_t p_stride, gs_status_f64* S, uint8_t* codes,
+                               const int64_t* order, const gs_options_f64* options, int32_t workgroupsize,
+                               void* stream) {
+    return solve_batch_f64(method, family, nsys, X, P, p_stride, S, codes, order, options, workgroupsize, stream,
+                           true);
 }
 
 int gs_solve_batch_deflated_f64(const gs_method* method, const gs_family* family, const gs_deflation* deflation,

@@ -207,6 +207,19 @@ int gs_solve_batch_f64(const gs_method* method, const gs_family* family, int64_t
                        uint8_t* codes, const int64_t* order, const gs_options_f64* options,
                        int32_t workgroupsize, void* stream);
 
+/* gs_solve_batch_f64 with no allocation: no counterpart in the reference; for callers that cannot
+ * allocate device memory (code under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments, semantics and results as gs_solve_batch_f64. The device buffers the
+ * call needs (the order check's bitmap, the FpuGl3 solve's hand-over flags) are a fixed array of
+ * 2 MiB per GPU that the library holds until the process ends, so the call refuses (GS_EINVAL) an
+ * `order` over more than 2^24 systems and an FpuGl3 batch of more than 2^20 systems. Every call on a
+ * GPU shares that GPU's array: calls that overlapped on one GPU would race on it, and the library's
+ * per-device lock keeps the calls of one process from overlapping. */
+int gs_solve_batch_f64_noalloc(const gs_method* method, const gs_family* family, int64_t nsys,
+                               double* X, const void* P, int64_t p_stride, gs_status_f64* S,
+                               uint8_t* codes, const int64_t* order, const gs_options_f64* options,
+                               int32_t workgroupsize, void* stream);
+
 /* Free every device resource the library caches (the workspaces of the reductions, the
  * least-squares batch and the learned start, the least-squares graphs, streams and events, and the
  * calling thread's timing events) on every device, and restore the current device. No call may be
