@@ -626,8 +626,16 @@ function device_stationarity(prob::StationarityProblem, b_src, m::Int)
 end
 
 _action_records(p::Tuple{Real, Real}) = (ROCArray([Float64(p[1]), Float64(p[2])]), Int64(0))
-_action_records(p::SVector{2}) = _action_records((p[1], p[2]))
-_action_records(p::PerSystem) = (p.data, Int64(16))
+_action_records(p::SVector{2, <:Real}) = _action_records((p[1], p[2]))
+_action_records(p::PerSystem{<:ROCVector{SVector{2, Float64}}}) = (p.data, Int64(16))
+
+# The action's parameters the device reads: (h, λ) shared, or one SVector{2, Float64} per system in
+# device memory (16-byte records). Anything else (a host vector, another record type) is no device
+# launch: it goes to the generic method, which refuses or runs it as the reference does.
+served_action_params(p::Tuple{Real, Real}) = true
+served_action_params(p::SVector{2, <:Real}) = true
+served_action_params(p::PerSystem{<:ROCVector{SVector{2, Float64}}}) = true
+served_action_params(p) = false
 
 function hip_launch_stationarity!(X, S, codes, cs::GsStationarity, cm::GsMethod, p, opt, order, ws)
     P, stride = _action_records(p)
@@ -657,7 +665,7 @@ function solve_batch!(X::ROCVector{SVector{6, Float64}}, S::ROCVector{SolverStat
     b_src = b === nothing ? prob.b : b
     cm = device_method(Adapt_real(method))
     cs = device_stationarity(prob, b_src, prob.E isa Colon ? 6 : length(prob.E))
-    (cs === nothing || !served_stationarity(cm)) && return invoke(solve_batch!,
+    (cs === nothing || !served_stationarity(cm) || !served_action_params(p)) && return invoke(solve_batch!,
         Tuple{AbstractArray, AbstractArray{SolverStatus{Float64}}, StationarityProblem, Any, Any},
         X, S, prob, method, p; order, options, b, workgroupsize, kwargs...)
     _validate_batch(X, p, order)
@@ -680,7 +688,7 @@ function solve_batch!(X::ROCVector{SVector{6, Float64}}, prob::StationarityProbl
     b_src = b === nothing ? prob.b : b
     cm = device_method(Adapt_real(method))
     cs = device_stationarity(prob, b_src, prob.E isa Colon ? 6 : length(prob.E))
-    (cs === nothing || !served_stationarity(cm)) && return invoke(solve_batch!,
+    (cs === nothing || !served_stationarity(cm) || !served_action_params(p)) && return invoke(solve_batch!,
         Tuple{AbstractArray, StationarityProblem, Any, Any},
         X, prob, method, p; order, options, b, workgroupsize, kwargs...)
     _validate_batch(X, p, order)

@@ -949,7 +949,7 @@ def solve_batch(X, *args, order=None, options: Optional[Options] = None, workgro
         raise ArgumentError("method must be a Newton")
     if isinstance(method.globalization, LevenbergMarquardt):
         raise ArgumentError(
-            "the device solve of a NonlinearProblem takes Static() or Backtracking(); "
+            "the device solve of a NonlinearProblem takes Static(), Backtracking() or DogLeg(); "
             "LevenbergMarquardt() is the least-squares batch's (a LeastSquaresProblem)")
     fam = prob.F
     _check_x(X, fam.n, allow_f32=True)
@@ -1035,7 +1035,8 @@ def _solve_stationarity(X, S, prob, method, p, order, options, workgroupsize, kw
     if not isinstance(method, Newton):
         raise ArgumentError("method must be a Newton")
     if isinstance(method.globalization, LevenbergMarquardt):
-        raise ArgumentError("the device StationarityProblem batch takes Static() or Backtracking()")
+        raise ArgumentError("the device StationarityProblem batch takes Static(), Backtracking() or DogLeg(); "
+                            "LevenbergMarquardt has no stationarity form")
     act = prob.S
     _check_x(X, act.n)
     order_h = _validate_batch(X, p, order)

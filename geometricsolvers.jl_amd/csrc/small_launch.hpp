@@ -183,12 +183,12 @@ int launch_small_reuse(int family, int glob, const KernelArgs& a, bool mixed, bo
 // Defined in small_dogleg.hip: DogLeg.
 int launch_small_dogleg(int family, const KernelArgs& a, bool mixed, bool refine, bool diag, int wg, hipStream_t st);
 
-// Defined in small_deflated.hip: Static and Backtracking on a Deflated batch (FamDeflatedT).
+// Defined in small_deflated.hip: Static, Backtracking and DogLeg on a Deflated batch (FamDeflatedT).
 int launch_small_deflated(int family, int glob, const KernelArgs& a, bool mixed, bool refine, bool diag, int wg,
                           hipStream_t st);
 
-// Defined in small_stationarity.hip: Static and Backtracking on a StationarityProblem batch of a
-// registered action (FamStationarityT), |E| = 2 or 6, with or without reuse = k.
+// Defined in small_stationarity.hip: Static, Backtracking and DogLeg on a StationarityProblem batch
+// of a registered action (FamStationarityT), |E| = 2 or 6, with or without reuse = k (not with DogLeg).
 int launch_small_stationarity(int action, int ne, int glob, bool reuse, const KernelArgs& a, bool mixed, bool refine,
                               bool diag, int wg, hipStream_t st);
 

@@ -89,6 +89,23 @@ def test_full_state_and_datum_forms_bitwise(gs, orc, device):
         assert np.abs(grad_cs(x[i], H, 1.0)[2:4] + 0.01).max() <= 2e-13
 
 
+@pytest.mark.parametrize("name", list(METHODS))
+def test_full_state_every_method_bitwise(gs, orc, device, name):
+    """E = U = : (|E| = 6), the other compiled width, under every method form: the 6 × 6 LU, the
+    reused factor and the DogLeg model at that size."""
+    n = 2048
+    X0 = states(n, 31)
+    X0[::3, 4:6] += np.random.default_rng(32).uniform(-1, 1, size=(len(X0[::3]), 2))
+    act = gs.HenonHeilesAction()
+    p = (H, 1.0)
+    prob = gs.StationarityProblem(act, b=(0.01, -0.02, 0.03, 0.0, 0.02, -0.01))
+    method = METHODS[name](gs)
+    opts = gs.Options(max_iterations=30, **OPTS)
+    x, st = _gpu(gs, device, prob, X0, p, method, opts)
+    xo, so = orc.solve_batch(method, act, X0.copy(), p, opts, stationarity=prob)
+    assert_parity(st, x, so, xo, bitwise=True)
+
+
 def test_refusals(gs, device):
     act = gs.HenonHeilesAction()
     X = torch.zeros((4, 6), dtype=torch.float64, device=device)
