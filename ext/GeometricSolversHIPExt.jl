@@ -75,7 +75,9 @@ struct GsFamily
     m::Int32
     p_len::Int32
     shared::NTuple{8, Float64}
+    data::Ptr{Cvoid}   # the family's own device data (A of DenseSin), C_NULL for the others
 end
+GsFamily(id, n, m, p_len, shared) = GsFamily(id, n, m, p_len, shared, C_NULL)
 
 # SolverStatus{Float64} crosses the ABI as itself: its C layout is gs_status_f64.
 function __init__()
@@ -83,6 +85,7 @@ function __init__()
     fieldoffset(SolverStatus{Float64}, 4) == 8 || error("SolverStatus{Float64}.fnorm is not at 8")
     fieldoffset(SolverStatus{Float64}, 10) == 32 || error("SolverStatus{Float64}.rank is not at 32")
     fieldoffset(SolverStatus{Float64}, 14) == 64 || error("SolverStatus{Float64}.contracted is not at 64")
+    ccall((:gs_version, libgsolve), Int32, ()) == 2 || error("libgsolve_hip is not ABI version 2")
     sizes = zeros(Int64, 4)
     ccall((:gs_abi_sizes, libgsolve), Cvoid, (Ptr{Int64},), sizes)
     sizes == [72, sizeof(GsOptionsF64), sizeof(GsMethod), sizeof(GsFamily)] ||

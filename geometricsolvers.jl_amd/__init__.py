@@ -33,7 +33,7 @@ __all__ = [
     "SINGULAR", "NONFINITE", "LINESEARCH_FAILED", "STATUS_DTYPE", "STATUS_DTYPE_F32", "L2Norm", "MaxNorm", "Options",
     "LUFactorization", "Static", "Backtracking", "DogLeg", "LevenbergMarquardt", "NoRefinement", "Refinement", "Newton",
     "NonlinearProblem", "Deflated", "StationarityProblem", "HenonHeilesAction", "Reparametrized", "LinearChart",
-    "PolarChart", "Coupled2", "Hetero2", "ScalarSquare", "TanhLinear3", "QuadMap2",
+    "PolarChart", "Coupled2", "Hetero2", "ScalarSquare", "TanhLinear3", "QuadMap2", "DenseSin",
     "HenonHeilesMidpoint", "FpuGl3", "PerSystem", "BatchParams", "parameters", "solve_batch",
     "LeastSquaresProblem", "ExpSumFits", "ExpSumData",
     "status_buffer", "statuses", "isconverged", "batch_summary", "BatchSummary", "load_library",
@@ -487,6 +487,39 @@ class QuadMap2(_SimpleFamily):
     @property
     def p_dtype(self):
         return _torch().float64
+
+
+class DenseSin(_SimpleFamily):
+    """TenEResidual(A) of the reference's device benchmark (benchmark/device_problems.jl:133-175):
+    F(x, b) = A x + sin.(x) ./ 10 − b, n = 16, 24 or 32, with A (n × n) the family's own, shared by the
+    batch, and p = PerSystem(B), one b per system. The Jacobian is A + Diagonal(cos.(x) ./ 10). Its sine
+    and cosine are one fixed algorithm (Cody–Waite reduction, Taylor polynomials), the same on the
+    device and in the oracle."""
+
+    id = _abi.GS_FAM_DENSE_SIN
+
+    def __init__(self, A):
+        a = np.asarray(A.detach().cpu().numpy() if hasattr(A, "detach") else A, dtype=np.float64)
+        if a.ndim != 2 or a.shape[0] != a.shape[1] or a.shape[0] not in (16, 24, 32):
+            raise ArgumentError("DenseSin takes a square A of order 16, 24 or 32")
+        self.A = a
+        self.n = self.p_len = a.shape[0]
+        self._dev = {}
+
+    @property
+    def p_dtype(self):
+        return _torch().float64
+
+    def host_data(self) -> np.ndarray:
+        """A column-major, as the library reads it."""
+        return np.ascontiguousarray(self.A.T).reshape(-1)
+
+    def device_data(self, device):
+        t = self._dev.get(str(device))
+        if t is None:
+            t = _torch().as_tensor(self.host_data(), device=device)
+            self._dev[str(device)] = t
+        return t
 
 
 class HenonHeilesMidpoint(Family):
@@ -997,6 +1030,9 @@ def solve_batch(X, *args, order=None, options: Optional[Options] = None, workgro
     cf.id, cf.n, cf.m, cf.p_len = fam.id, fam.n, fam.n, fam.p_len
     for k, v in enumerate(shared):
         cf.shared[k] = v
+    fdata = fam.device_data(X.device) if hasattr(fam, "device_data") else None
+    if fdata is not None:
+        cf.data = fdata.data_ptr()
     order_dev = None
     if order_h is not None:
         order_dev = torch.as_tensor(order_h, dtype=torch.int64).to(X.device)

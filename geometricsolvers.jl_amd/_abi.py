@@ -12,7 +12,7 @@ from pathlib import Path
 
 import numpy as np
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 LIB_DIR = Path(__file__).resolve().parent / "lib"
 LIB_PATH = LIB_DIR / "libgsolve_hip.so"
 
@@ -34,6 +34,7 @@ GS_FAM_HENON_HEILES_MIDPOINT = 5
 GS_FAM_FPU_GL3 = 6
 GS_FAM_EXPSUM_LSQ = 7
 GS_FAM_QUAD_MAP2 = 9
+GS_FAM_DENSE_SIN = 10
 GS_ACT_HENON_HEILES = 8
 
 GS_DATUM_NONE, GS_DATUM_SHARED, GS_DATUM_PER_SYSTEM = 0, 1, 2
@@ -117,6 +118,7 @@ class GsFamily(ctypes.Structure):
         ("m", ctypes.c_int32),
         ("p_len", ctypes.c_int32),
         ("shared", ctypes.c_double * 8),
+        ("data", ctypes.c_void_p),
     ]
 
 

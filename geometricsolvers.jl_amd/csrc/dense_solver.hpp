@@ -1,0 +1,32 @@
+// The launch of the dense mid-size solve (a segment of a wave per system, n = 16, 24 or 32): the
+// arguments gs_solve_batch_f64 hands to the kernels of dense_solver.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/gsolve.h"
+#include "small_solver.hpp"
+
+namespace gs {
+
+struct DenseLaunch {
+    DevParams q;
+    const double* A;      // the family's matrix (n × n, column-major), device
+    const char* P;        // parameter records (b, n doubles each)
+    int64_t p_stride;     // bytes between records (0: one shared record)
+    int64_t nsys;
+    double* X;
+    gs_status_f64* S;
+    uint8_t* codes;
+    const int64_t* order;
+};
+
+// The systems of one launch per workgroup at size n (n = 16: four per wave; 24 and 32: two).
+int dense_systems_per_block(int n);
+
+// GS_FAM_DENSE_SIN under Newton with the Static or Backtracking globalization, LU in FP64 or FP32
+// (mixed, with or without Refinement); returns a gs_error code (the launch only; the caller
+// synchronises).
+int launch_dense(int n, int glob, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st);
+
+}  // namespace gs

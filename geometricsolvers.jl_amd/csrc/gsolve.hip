@@ -12,6 +12,7 @@
 #include <string>
 
 #include "../../include/gsolve.h"
+#include "dense_solver.hpp"
 #include "mid_solver.hpp"
 #include "small_launch.hpp"
 
@@ -108,6 +109,7 @@ static const FamilySpec* family_spec(int id) {
         {GS_FAM_HENON_HEILES_MIDPOINT, 4, 4, 32, 16, 16, 16},
         {GS_FAM_FPU_GL3, 48, 32, 256, 8, 0, 0},
         {GS_ACT_HENON_HEILES, 6, 2, 16, 8, 0, 0},
+        {GS_FAM_DENSE_SIN, 0, 0, 0, 8, 0, 0},  // n = 16, 24 or 32, records of n doubles
     };
     for (const auto& s : specs)
         if (s.id == id) return &s;
@@ -128,8 +130,14 @@ static int validate_batch(const gs_method* method, const gs_family* family, int6
     if (!S && !codes) return fail(GS_EINVAL, "one of S and codes must be given");
     const FamilySpec* fs = family_spec(family->id);
     if (!fs) return fail(GS_EFAMILY, "problem family " + std::to_string(family->id) + " is not registered");
-    if (family->n != fs->n || family->m != fs->n)
+    if (family->id == GS_FAM_DENSE_SIN) {
+        if (!(family->n == 16 || family->n == 24 || family->n == 32) || family->m != family->n)
+            return fail(GS_EINVAL, "the dense family takes n = m = 16, 24 or 32");
+        if (!family->data || (reinterpret_cast<uintptr_t>(family->data) % 8) != 0)
+            return fail(GS_EINVAL, "the dense family needs its matrix A (family data, 8-byte aligned)");
+    } else if (family->n != fs->n || family->m != fs->n) {
         return fail(GS_EINVAL, "family " + std::to_string(family->id) + " has n = m = " + std::to_string(fs->n));
+    }
     if (f32 && fs->p_bytes32 == 0)
         return fail(GS_EFAMILY, "family " + std::to_string(family->id) + " has no Float32 form");
     if (method->linsolve != GS_LU) return fail(GS_EFAMILY, "only LUFactorization has a device form");
@@ -149,13 +157,16 @@ static int validate_batch(const gs_method* method, const gs_family* family, int6
     if (method->reuse < 0) return fail(GS_EINVAL, "reuse must be ≥ 1, or 0 for nothing");
     if (family->id == GS_FAM_FPU_GL3 && method->reuse > 0)
         return fail(GS_EFAMILY, "the wave-per-system solve of FpuGl3 takes no reuse");
+    if (family->id == GS_FAM_DENSE_SIN && (method->globalization == GS_DOGLEG || method->reuse > 0))
+        return fail(GS_EFAMILY, "the dense solve takes Static or Backtracking and no reuse");
     if (method->factor_precision != GS_FACTOR_WORKING && method->factor_precision != GS_FACTOR_F32)
         return fail(GS_EINVAL, "factor_precision must be GS_FACTOR_WORKING or GS_FACTOR_F32");
     if (method->refinement != GS_NO_REFINEMENT && method->refinement != GS_REFINEMENT)
         return fail(GS_EINVAL, "refinement must be GS_NO_REFINEMENT or GS_REFINEMENT");
     *spec = fs;
     if (nsys == 0) return GS_OK;
-    const int p_bytes = f32 ? fs->p_bytes32 : fs->p_bytes, align = f32 ? fs->align32 : fs->align;
+    const int p_bytes = family->id == GS_FAM_DENSE_SIN ? 8 * family->n : (f32 ? fs->p_bytes32 : fs->p_bytes);
+    const int align = f32 ? fs->align32 : fs->align;
     if (!X) return fail(GS_EINVAL, "X must not be NULL for a non-empty batch");
     if (!P) return fail(GS_EINVAL, "this family needs its parameter records P");
     if (p_stride != 0 && p_stride < p_bytes)
@@ -381,6 +392,23 @@ static int solve_batch_f64(const gs_method* method, const gs_family* family, int
         L.refine = refine;
         L.redo = scr;  // null: the library's own flags
         return finish_launch(launch_fpu_gl3(L, st), st, timed);
+    }
+    if (family->id == GS_FAM_DENSE_SIN) {
+        // a segment of a wave per system (the workgroup is the kernel's own: 256 work-items)
+        DenseLaunch L;
+        std::memset(&L, 0, sizeof(L));
+        L.q = a.q;
+        L.A = static_cast<const double*>(family->data);
+        L.P = static_cast<const char*>(P);
+        L.p_stride = p_stride;
+        L.nsys = nsys;
+        L.X = X;
+        L.S = S;
+        L.codes = codes;
+        L.order = order;
+        return finish_launch(launch_dense(family->n, glob == GS_BACKTRACKING ? kBacktracking : kStatic, mixed, refine,
+                                          diag, L, st),
+                             st, timed);
     }
     if (glob == GS_DOGLEG)
         rc = launch_small_dogleg(family->id, a, mixed, refine, diag, wg, st);

@@ -153,3 +153,29 @@ def expsum_residual_norms(X8, K, m, off, Y):
         f = (X8[i, :k, None] * np.exp(-X8[i, k:2 * k, None] * t[None, :])).sum(axis=0) - Y[off[i]:off[i] + mm]
         out[i] = np.sqrt((f * f).sum())
     return out
+
+
+def tene_matrix(n: int, seed: int | None = None) -> np.ndarray:
+    """A = 2I + R of the §10.E problem (benchmark/device_problems.jl:133-145): R with a zero diagonal
+    and entries uniform on [−1, 1] scaled by 1/(2(n − 1)), so each off-diagonal row sum is at most 1/2.
+    One matrix per n (numpy PCG64 from the seed n: the reference's Xoshiro stream is Julia's)."""
+    rng = np.random.default_rng(n if seed is None else seed)
+    R = 2.0 * rng.random((n, n)) - 1.0
+    np.fill_diagonal(R, 0.0)
+    R *= 0.5 / (n - 1)
+    return 2.0 * np.eye(n) + R
+
+
+def tene_batch(n: int, nsys: int, seed: int = 1):
+    """(A, B, X*) of the §10.E batch (device_problems.jl:159-175): each system's own root x* uniform on
+    [−1, 1]ⁿ and b = A x* + sin.(x*) / 10; the solve starts from x₀ = 0."""
+    A = tene_matrix(n)
+    rng = np.random.default_rng(seed)
+    Xs = 2.0 * rng.random((nsys, n)) - 1.0
+    B = Xs @ A.T + np.sin(Xs) / 10.0
+    return A, B, Xs
+
+
+def tene_residual(x: np.ndarray, A: np.ndarray, B: np.ndarray) -> np.ndarray:
+    """F(x, b) = A x + sin.(x) / 10 − b, row-wise in numpy (the host FP64 recheck)."""
+    return x @ A.T + np.sin(x) / 10.0 - B

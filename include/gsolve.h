@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define GS_ABI_VERSION 1
+#define GS_ABI_VERSION 2
 
 /* ReturnCode, one byte (src/base/returncodes.jl:14). */
 enum gs_return_code {
@@ -177,8 +177,11 @@ enum gs_family_id {
     GS_ACT_HENON_HEILES = 8,     /* an action S(x, p), not a residual: the two-step discrete action of the       *
                                   * variational midpoint rule for Hénon–Heiles, x = (q₋, q, q₊) (6 entries),     *
                                   * P = (h, λ); taken by gs_solve_batch_stationarity_f64 only                   */
-    GS_FAM_QUAD_MAP2 = 9         /* u = A x, F_i = (u_i + c_i u_i²) − b_i : n = 2, P = A (4, column-major), b (2),  *
+    GS_FAM_QUAD_MAP2 = 9,        /* u = A x, F_i = (u_i + c_i u_i²) − b_i : n = 2, P = A (4, column-major), b (2),  *
                                   * c (2) — f_mod_r1 / f_ill_r1 of test/nonlinear/refinement.jl:40-96           */
+    GS_FAM_DENSE_SIN = 10        /* A x + sin.(x)/10 − b, the §10.E problem of benchmark/device_problems.jl:133-175 : *
+                                  * n = 16, 24 or 32; `data` = A (n × n, column-major, shared by the batch),      *
+                                  * P = b (n doubles per system); one wave segment per system                   */
 };
 
 typedef struct gs_family {
@@ -187,6 +190,8 @@ typedef struct gs_family {
     int32_t m;        /* residual entries per system (= n when square)  */
     int32_t p_len;    /* scalars per parameter record                   */
     double shared[8]; /* scalar parameters that every system shares     */
+    const void* data; /* the family's own device data, which the reference keeps inside the problem
+                       * (GS_FAM_DENSE_SIN: the matrix A of TenEResidual(A)); NULL for the others */
 } gs_family;
 
 /* Status form of solve_batch! (batched.jl:257-287, 311-321) on FP64 systems.

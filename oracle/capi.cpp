@@ -232,6 +232,28 @@ static int solve_small_batch(const Method<R>& m, const Options<R>& o, const gs_f
                 return pr;
             });
             return 0;
+        case GS_FAM_DENSE_SIN: {
+            const R* A = static_cast<const R*>(fam->data);
+            if (!A) return GS_EINVAL;
+            auto run_n = [&](auto nn) {
+                constexpr int NN = decltype(nn)::value;
+                run_batch<NN>(m, o, nsys, X, S, codes, order, nthreads, [&](int64_t idx) {
+                    const R* p = reinterpret_cast<const R*>(rec(idx));
+                    DenseSin<R, NN> pr;
+                    pr.A = A;
+                    for (int k = 0; k < NN; ++k) pr.b[k] = p[k];
+                    return pr;
+                });
+                return 0;
+            };
+            if (defl.first || defl.second) return GS_EFAMILY;
+            switch (fam->n) {
+                case 16: return run_n(std::integral_constant<int, 16>{});
+                case 24: return run_n(std::integral_constant<int, 24>{});
+                case 32: return run_n(std::integral_constant<int, 32>{});
+            }
+            return GS_EINVAL;
+        }
         case GS_FAM_HENON_HEILES_MIDPOINT: {
             const R h = R(fam->shared[0]);
             run_batch_d<4>(defl, m, o, nsys, X, S, codes, order, nthreads, [&](int64_t idx) {
@@ -718,6 +740,45 @@ void orc_read_mixed_stall(int promoted, int slow_steps, double fnorm, double ste
     out[3] = info.promoted ? 1.0 : 0.0;
     out[4] = info.fnorm;
     out[5] = info.stepnorm;
+}
+
+// F(x) and J(x) (column-major) of family 10 at n = 16, 24 or 32: A column-major, b of one system.
+int orc_dense_eval(int n, const double* A, const double* b, const double* x, double* F, double* J) {
+    auto run = [&](auto nn) {
+        constexpr int NN = decltype(nn)::value;
+        DenseSin<double, NN> p;
+        p.A = A;
+        Vec<double, NN> v;
+        for (int k = 0; k < NN; ++k) {
+            p.b[k] = b[k];
+            v[k] = x[k];
+        }
+        const Vec<double, NN> r = p.F(v);
+        const Mat<double, NN> Jm = p.J(v);
+        for (int k = 0; k < NN; ++k) F[k] = r[k];
+        for (int k = 0; k < NN * NN; ++k) J[k] = Jm[k];
+        return 0;
+    };
+    switch (n) {
+        case 16: return run(std::integral_constant<int, 16>{});
+        case 24: return run(std::integral_constant<int, 24>{});
+        case 32: return run(std::integral_constant<int, 32>{});
+    }
+    return GS_EINVAL;
+}
+
+// The sine and cosine of family 10 (families.hpp fsincos), elementwise, in Float64 or Float32.
+void orc_fsincos(const double* x, double* s, double* c, int64_t len, int f32) {
+    for (int64_t i = 0; i < len; ++i) {
+        if (f32) {
+            float a, b;
+            fsincos<float>(float(x[i]), a, b);
+            s[i] = a;
+            c[i] = b;
+        } else {
+            fsincos<double>(x[i], s[i], c[i]);
+        }
+    }
 }
 
 // rmaxnorm (reductions.jl:67) of a length-L vector (L ≤ 4), in Float64 or Float32 (f32 = 1: the

@@ -302,6 +302,102 @@ template <class T> struct QuadMap2 {
     }
 };
 
+// The sine and cosine of family 10, one fixed algorithm that the device family writes out the same
+// way, so that F and J have the same bits on both sides (as the exponential of family 7): k =
+// rint(x · 2/π); r = x − k·π/2 with π/2 split in three parts (Cody–Waite, each product folded by
+// FMA); the Taylor polynomials of sin r and cos r in r² on |r| ≤ π/4 by Horner with FMA (through r¹⁷
+// and r¹⁶ in Float64, r¹¹ and r¹⁰ in Float32); then the quadrant k mod 4. The Taylor coefficients
+// (−1)^j/(2j+1)! and (−1)^j/(2j)! are rounded once to T. Within 2 ulp of the libm values on the
+// tested range (tests/test_oracle_dense.py); NaN for a non-finite x and for |x| ≥ 2^50 (2^22 in
+// Float32), where k would not be an exact integer; sin(±0) = ±0.
+template <class T> struct SinCosConst;
+template <> struct SinCosConst<double> {
+    static constexpr int terms = 8;
+    static constexpr double two_over_pi = 0x1.45f306dc9c883p-1;
+    static constexpr double p1 = 0x1.921fb54442d18p+0, p2 = 0x1.1a62633145c07p-54, p3 = -0x1.f1976b7ed8fbcp-110;
+    static constexpr double big = 0x1p50;
+    static constexpr double s[8] = {-0x1.5555555555555p-3, 0x1.1111111111111p-7, -0x1.a01a01a01a01ap-13,
+                                    0x1.71de3a556c734p-19, -0x1.ae64567f544e4p-26, 0x1.6124613a86d09p-33,
+                                    -0x1.ae7f3e733b81fp-41, 0x1.952c77030ad4ap-49};
+    static constexpr double c[8] = {-0x1.0000000000000p-1, 0x1.5555555555555p-5, -0x1.6c16c16c16c17p-10,
+                                    0x1.a01a01a01a01ap-16, -0x1.27e4fb7789f5cp-22, 0x1.1eed8eff8d898p-29,
+                                    -0x1.93974a8c07c9dp-37, 0x1.ae7f3e733b81fp-45};
+};
+template <> struct SinCosConst<float> {
+    static constexpr int terms = 5;
+    static constexpr float two_over_pi = 0x1.45f306p-1f;
+    static constexpr float p1 = 0x1.921fb6p+0f, p2 = -0x1.777a5cp-25f, p3 = -0x1.ee59dap-50f;
+    static constexpr float big = 0x1p22f;
+    static constexpr float s[8] = {-0x1.555556p-3f, 0x1.111112p-7f, -0x1.a01a02p-13f, 0x1.71de3ap-19f,
+                                   -0x1.ae6456p-26f, 0.0f, 0.0f, 0.0f};
+    static constexpr float c[8] = {-0x1.000000p-1f, 0x1.555556p-5f, -0x1.6c16c2p-10f, 0x1.a01a02p-16f,
+                                   -0x1.27e4fcp-22f, 0.0f, 0.0f, 0.0f};
+};
+
+template <class T> inline void fsincos(T x, T& sn, T& cs) {
+    using K = SinCosConst<T>;
+    if (!(std::fabs(x) < K::big)) {
+        sn = cs = std::numeric_limits<T>::quiet_NaN();
+        return;
+    }
+    if (x == T(0)) {  // sin(±0) = ±0
+        sn = x;
+        cs = T(1);
+        return;
+    }
+    const T k = std::rint(x * K::two_over_pi);
+    T r = std::fma(-k, K::p1, x);
+    r = std::fma(-k, K::p2, r);
+    r = std::fma(-k, K::p3, r);
+    const T z = r * r;
+    T ps = K::s[K::terms - 1], pc = K::c[K::terms - 1];
+    for (int j = K::terms - 2; j >= 0; --j) {
+        ps = std::fma(ps, z, K::s[j]);
+        pc = std::fma(pc, z, K::c[j]);
+    }
+    const T s0 = std::fma(r * z, ps, r);
+    const T c0 = std::fma(z, pc, T(1));
+    const int q = int(int64_t(k) & 3);
+    sn = q == 0 ? s0 : (q == 1 ? c0 : (q == 2 ? -s0 : -c0));
+    cs = q == 0 ? c0 : (q == 1 ? -s0 : (q == 2 ? -c0 : s0));
+}
+
+// Family 10, the §10.E problem of the reference's device benchmark (benchmark/device_problems.jl:
+// 133-175, TenEResidual): F(x, b) = A x + sin.(x) ./ 10 − b, with A shared by the batch (N × N,
+// column-major; the reference keeps it inside the residual) and b per system. J = A +
+// Diagonal(cos.(x) ./ 10): ForwardDiff's partials of the same expression (the matrix-vector
+// product's partials are A's entries exactly; the sine's are cos(x_i)·1, divided by 10).
+template <class T, int NN> struct DenseSin {
+    static constexpr int N = NN;
+    const T* A;
+    Vec<T, NN> b;
+    long nf = 0;
+    mutable long nj = 0;
+    Vec<T, NN> F(const Vec<T, NN>& x) {
+        ++nf;
+        Vec<T, NN> r;
+        for (int i = 0; i < NN; ++i) {
+            T acc = A[i] * x[0];
+            for (int j = 1; j < NN; ++j) acc = std::fma(A[i + j * NN], x[j], acc);
+            T sn, cs;
+            fsincos<T>(x[i], sn, cs);
+            r[i] = (acc + sn / T(10)) - b[i];
+        }
+        return r;
+    }
+    Mat<T, NN> J(const Vec<T, NN>& x) const {
+        ++nj;
+        Mat<T, NN> J;
+        for (int k = 0; k < NN * NN; ++k) J[k] = A[k];
+        for (int i = 0; i < NN; ++i) {
+            T sn, cs;
+            fsincos<T>(x[i], sn, cs);
+            J[i + i * NN] = A[i + i * NN] + cs / T(10);
+        }
+        return J;
+    }
+};
+
 // Fixtures of the reference's mixed-precision and stopping tests, for the oracle's KATs:
 //   104: f_floor_r1 / j_floor_r1 (refinement.jl:131-139): F rounds x through Float32, so ‖F‖ floors
 //        near eps(Float32); J = Diagonal(2x) in Float64

@@ -95,6 +95,10 @@ def lib():
         L.orc_read_mixed_stall.restype = None
         L.orc_read_mixed_stall.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                            ctypes.c_double, ctypes.c_double, vp]
+        L.orc_dense_eval.restype = ctypes.c_int
+        L.orc_dense_eval.argtypes = [ctypes.c_int, vp, vp, vp, vp, vp]
+        L.orc_fsincos.restype = None
+        L.orc_fsincos.argtypes = [vp, vp, vp, ctypes.c_int64, ctypes.c_int]
         L.orc_rmaxnorm.restype = ctypes.c_double
         L.orc_rmaxnorm.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         _LIB = L
@@ -143,7 +147,7 @@ def solve_batch(method, family, X: np.ndarray, p, options, order=None, nthreads:
     rec, stride, shared = host_records(family, p, nsys, real)
     cm = method.to_c()
     co = options.to_c()
-    fam = _family_struct(family, shared)
+    fam = _family_struct(family, shared, real)
     S = np.zeros(nsys, dtype=_status_dtype_f32() if f32 else _status_dtype())
     o = None if order is None else np.ascontiguousarray(np.asarray(order, dtype=np.int64))
     if reparametrized is not None:
@@ -214,14 +218,18 @@ def _status_dtype_f32():
 
 class _Fam(ctypes.Structure):
     _fields_ = [("id", ctypes.c_int32), ("n", ctypes.c_int32), ("m", ctypes.c_int32),
-                ("p_len", ctypes.c_int32), ("shared", ctypes.c_double * 8)]
+                ("p_len", ctypes.c_int32), ("shared", ctypes.c_double * 8), ("data", ctypes.c_void_p)]
 
 
-def _family_struct(family, shared) -> _Fam:
+def _family_struct(family, shared, real=np.float64) -> _Fam:
     f = _Fam()
     f.id, f.n, f.m, f.p_len = family.id, family.n, family.n, family.p_len
     for k, v in enumerate(shared):
         f.shared[k] = v
+    if hasattr(family, "host_data"):
+        data = np.ascontiguousarray(family.host_data(), dtype=real)
+        f.data = data.ctypes.data
+        f._keep = data  # the struct points into it for the call
     return f
 
 
@@ -436,3 +444,25 @@ def pow_int(x, n: int):
     out = np.empty_like(x)
     lib().orc_pow_int(_ptr(x), int(n), _ptr(out), x.size)
     return out
+
+
+def fsincos(x, f32: bool = False):
+    """The sine and cosine of the dense family (families.hpp fsincos), elementwise."""
+    x = np.ascontiguousarray(np.atleast_1d(np.asarray(x, dtype=np.float64)))
+    sn = np.empty_like(x)
+    cs = np.empty_like(x)
+    lib().orc_fsincos(_ptr(x), _ptr(sn), _ptr(cs), x.size, int(f32))
+    return sn, cs
+
+
+def dense_eval(A, b, x):
+    """(F(x), J(x)) of the dense family at one system: A (n × n), b and x (n)."""
+    A = np.asarray(A, dtype=np.float64)
+    n = A.shape[0]
+    Ac = np.ascontiguousarray(A.T).reshape(-1)
+    bb = np.ascontiguousarray(b, dtype=np.float64)
+    xx = np.ascontiguousarray(x, dtype=np.float64)
+    F = np.zeros(n)
+    J = np.zeros(n * n)
+    assert lib().orc_dense_eval(n, _ptr(Ac), _ptr(bb), _ptr(xx), _ptr(F), _ptr(J)) == 0
+    return F, J.reshape(n, n, order="F")

@@ -1,0 +1,112 @@
+"""Parity of the dense mid-size solve (a wave segment per system, GS_FAM_DENSE_SIN: the §10.E problem
+A x + sin.(x)/10 − b of benchmark/device_problems.jl:133-175, n = 16, 24, 32) with the CPU oracle,
+through the C ABI. The device evaluates the family's fixed sine and cosine and folds every sum in the
+oracle's order, so results are compared bit for bit: iterates and every status byte.
+
+The batch mixes the benchmark's start x₀ = 0 with far starts (the line search and the iteration cap
+work), starts that make F overflow and right-hand sides with NaN or Inf (NONFINITE), so each exit of
+the loop runs on the device.
+"""
+import numpy as np
+import pytest
+import torch
+
+from test_gpu_parity import assert_parity
+
+pytestmark = pytest.mark.gpu
+
+METHODS = {
+    "static": lambda gs: gs.Newton(),
+    "backtracking": lambda gs: gs.Newton(globalization=gs.Backtracking()),
+    "mixed-refine": lambda gs: gs.Newton(linsolve=gs.LUFactorization("float32"), refinement=gs.Refinement()),
+    "mixed-norefine": lambda gs: gs.Newton(linsolve=gs.LUFactorization("float32")),
+    "mixed-backtracking-refine": lambda gs: gs.Newton(linsolve=gs.LUFactorization("float32"),
+                                                      globalization=gs.Backtracking(), refinement=gs.Refinement()),
+    "plain-refine-regularized": lambda gs: gs.Newton(refinement=gs.Refinement(), regularization=1e-3),
+    "best-backtracking": lambda gs: gs.Newton(globalization=gs.Backtracking(), best=True),
+}
+
+
+def _batch(workloads, n, nsys, seed):
+    A, B, _ = workloads.tene_batch(n, nsys, seed=seed)
+    rng = np.random.default_rng(seed + 100)
+    X0 = np.zeros((nsys, n))
+    far = rng.random(nsys) < 0.2
+    X0[far] = rng.uniform(-30.0, 30.0, size=(int(far.sum()), n))
+    huge = np.nonzero(rng.random(nsys) < 0.01)[0]
+    X0[huge, 0] = 1e300
+    bad = np.nonzero(rng.random(nsys) < 0.01)[0]
+    B[bad, bad % n] = np.where(bad % 2 == 0, np.nan, np.inf)
+    return A, B, X0
+
+
+def _gpu(gs, device, fam, method, X0, B, opts, counts=False, order=None):
+    X = torch.as_tensor(X0, device=device).clone()
+    p = gs.PerSystem(torch.as_tensor(B, device=device))
+    prob = gs.NonlinearProblem(fam)
+    if counts:
+        c = gs.solve_batch(X, prob, method, p, opts, order=order)
+        return X.cpu().numpy(), c
+    S = gs.status_buffer(X.shape[0], device)
+    gs.solve_batch(X, S, prob, method, p, opts, order=order)
+    return X.cpu().numpy(), gs.statuses(S)
+
+
+@pytest.mark.parametrize("n", [16, 24, 32])
+@pytest.mark.parametrize("name", list(METHODS))
+def test_dense_every_method_bitwise(gs, orc, workloads, device, n, name):
+    nsys = 2048 if n < 32 else 1024
+    A, B, X0 = _batch(workloads, n, nsys, seed=n)
+    fam = gs.DenseSin(A)
+    m = METHODS[name](gs)
+    for opts in (gs.Options(f_abstol=1e-12, f_reltol=0.0, max_iterations=40),
+                 gs.Options(norm=gs.MaxNorm(), max_iterations=25)):
+        print(f"dense n={n} {name} norm={type(opts.norm).__name__}", flush=True)
+        x, st = _gpu(gs, device, fam, m, X0, B, opts)
+        xo, so = orc.solve_batch(m, fam, X0.copy(), gs.PerSystem(B), opts)
+        assert_parity(st, x, so, xo, bitwise=True)
+    codes = np.bincount(st["code"], minlength=6)
+    assert codes[0] > 0.7 * nsys and codes[4] > 0  # mostly SUCCESS, and the NONFINITE exits ran
+
+
+@pytest.mark.parametrize("name", ["static", "backtracking", "mixed-refine", "mixed-backtracking-refine"])
+def test_dense_nonmonotone_bitwise(gs, orc, workloads, device, name):
+    """A weak diagonal (A = 0.02 I + R/20) leaves sin.(x)/10 in charge: many roots, far starts that
+    wander, line searches that shorten, fail or stall, the iteration cap."""
+    n, nsys = 16, 2048
+    rng = np.random.default_rng(11)
+    A = 0.02 * np.eye(n) + 0.05 * (workloads.tene_matrix(n, seed=12) - 2.0 * np.eye(n))
+    B = rng.uniform(-0.1, 0.1, size=(nsys, n))
+    X0 = rng.uniform(-20.0, 20.0, size=(nsys, n))
+    fam = gs.DenseSin(A)
+    m = METHODS[name](gs)
+    opts = gs.Options(f_abstol=1e-12, f_reltol=0.0, max_iterations=60)
+    x, st = _gpu(gs, device, fam, m, X0, B, opts)
+    xo, so = orc.solve_batch(m, fam, X0.copy(), gs.PerSystem(B), opts)
+    assert_parity(st, x, so, xo, bitwise=True)
+    print(name, np.bincount(st["code"], minlength=6).tolist())
+
+
+def test_dense_counts_form_and_order(gs, orc, workloads, device):
+    n, nsys = 24, 3000
+    A, B, X0 = _batch(workloads, n, nsys, seed=7)
+    fam = gs.DenseSin(A)
+    opts = gs.Options(f_abstol=1e-12, f_reltol=0.0)
+    order = np.random.default_rng(8).permutation(nsys) + 1
+    x, counts = _gpu(gs, device, fam, gs.Newton(), X0, B, opts, counts=True, order=order)
+    xo, so = orc.solve_batch(gs.Newton(), fam, X0.copy(), gs.PerSystem(B), opts)
+    assert np.array_equal(x.view(np.uint64), xo.view(np.uint64))
+    want = np.bincount(so["code"], minlength=6)
+    assert [counts[k] for k in ("SUCCESS", "STALLED", "MAXITERS", "SINGULAR", "NONFINITE", "LINESEARCH_FAILED")] \
+        == want.tolist()
+
+
+def test_dense_full_size_converges_to_the_built_roots(gs, workloads, device):
+    """The benchmark's own batch from x₀ = 0 at 2^16 systems: every system converges to its root."""
+    for n in (16, 32):
+        A, B, Xs = workloads.tene_batch(n, 1 << 16)
+        x, st = _gpu(gs, device, gs.DenseSin(A), gs.Newton(), np.zeros((1 << 16, n)), B,
+                     gs.Options(f_abstol=1e-12, f_reltol=0.0))
+        assert (st["code"] == 0).all()
+        assert np.abs(x - Xs).max() < 1e-12
+        assert np.abs(workloads.tene_residual(x, A, B)).max() < 1e-12

@@ -24,7 +24,7 @@ def test_library_exports_every_declared_symbol(gs):
         assert hasattr(lib, name), name
     assert declared == set(abi.EXPORTED_SYMBOLS)
     loaded = gs.load_library()
-    assert loaded.gs_version() == 1
+    assert loaded.gs_version() == 2
 
 
 def test_status_layout_matches_julia(gs):
