@@ -112,27 +112,25 @@ template <int NP> GS_DEV bool seg_any(bool b) {
     return ((m >> base) & seg) != 0;
 }
 
-// The pivot choice of a segment: the largest key, and among equal keys the smallest tag (the tag
-// orders by logical row first); every lane receives the winner's key and tag.
-template <int CTRL> GS_DEV void piv_fold(double& v, int& t) {
-    const double ov = dpp<CTRL>(v);
-    const int ot = dpp<CTRL>(t);
-    const bool take = ov > v || (ov == v && ot < t);
-    v = take ? ov : v;
-    t = take ? ot : t;
+// The smallest 32-bit integer over the segment, in every lane of it (the DPP move folds into the min).
+template <int CTRL> GS_DEV int dpp_min(int v) {
+    return __builtin_amdgcn_update_dpp(int(0x7fffffff), v, CTRL, 0xF, 0xF, false);
 }
-template <int NP> GS_DEV void seg_argmax(double& v, int& t) {
-    piv_fold<0xB1>(v, t);
-    piv_fold<0x4E>(v, t);
-    piv_fold<0x141>(v, t);
-    piv_fold<0x140>(v, t);
-    if constexpr (NP == 32) {
-        const double ov = swz16(v);
-        const int ot = swz16(t);
-        const bool take = ov > v || (ov == v && ot < t);
-        v = take ? ov : v;
-        t = take ? ot : t;
-    }
+template <int NP> GS_DEV int seg_min(int v) {
+    v = __builtin_elementwise_min(v, dpp_min<0xB1>(v));
+    v = __builtin_elementwise_min(v, dpp_min<0x4E>(v));
+    v = __builtin_elementwise_min(v, dpp_min<0x141>(v));
+    v = __builtin_elementwise_min(v, dpp_min<0x140>(v));
+    if constexpr (NP == 32) v = __builtin_elementwise_min(v, swz16(v));
+    return v;
+}
+
+// The pivot choice of a segment: the largest key, and among the lanes holding it the smallest tag
+// (the tag orders by logical row first); every lane receives the winner's tag. Two reductions: the
+// maximum of the keys, then the minimum of the tags of the lanes that hold it.
+template <int NP> GS_DEV int seg_argmax(double key, int tag) {
+    const double kmax = seg_max<NP>(key);
+    return seg_min<NP>(key == kmax ? tag : int(0x7fffffff));
 }
 
 // ---- folds over an LDS vector, evaluated by every lane of the segment -----------------------------
@@ -239,6 +237,16 @@ template <int NN> struct FamDenseSin {
         double sn, cs;
         fsincos(x[i], sn, cs);
         return (acc + sn / 10.0) - b;
+    }
+    // J_ij at x from the lane's own diagonal term dg = cos(x_i)/10 (Jdiag): column j in parallel.
+    GS_DEV double Jdiag(const double* x, int i) const {
+        double sn, cs;
+        fsincos(x[i], sn, cs);
+        return cs / 10.0;
+    }
+    GS_DEV double Jentry(int i, int j, double dg) const {
+        const double a = A[i + j * N];
+        return j == i ? a + dg : a;
     }
     GS_DEV void Jrow(const double* x, int i, double (&row)[N]) const {
         double sn, cs;
@@ -351,9 +359,8 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG> struct DenseS
         for (int k = 0; k < N; ++k) {
             const bool cand = live && pos >= k;
             const double mag = double(tabs(a[k]));
-            double key = !cand ? -1.0 : (isnan_(mag) ? (pos == k ? double(__builtin_inf()) : -0.5) : mag);
-            int tag = (pos << 8) | sl;
-            seg_argmax<NP>(key, tag);
+            const double key = !cand ? -1.0 : (isnan_(mag) ? (pos == k ? double(__builtin_inf()) : -0.5) : mag);
+            const int tag = seg_argmax<NP>(key, (pos << 8) | sl);
             const int pw = tag >> 8, lw = tag & 255;
             if (pos == k) pos = pw;
             else if (sl == lw) pos = k;
@@ -676,31 +683,28 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG> struct DenseS
 
     // ---- the diagnostics of the last factorisation (grad_ratio_and_norm, newton.jl:1287-1298) ----
 
-    // ‖J‖_F (the left fold over the entries in column-major order, rnorm) and g = Jᴴ (sc F) into v.r,
-    // from the rows jr of J at the saved point: column j passes through v.rowd.
-    GS_DEV void jt_pass(const double (&jr)[N], double sc, bool sums, double& sumsq, double& mx, int e) {
+    // One pass over the columns of J at the saved point, column j formed in parallel (lane i its
+    // entry J_ij) in v.rowd: the left fold of the squares in column-major order (rnorm's sum; with
+    // e ≠ 0 of (J_ij/2^e)²) and the NaN-propagating largest |J_ij|, and, with `prod`, g = Jᴴ (sc F) into
+    // v.r (entry j the muladd fold over column j).
+    GS_DEV void diag_pass(double dg, int e, bool prod, double sc, double& sumsq, double& mx) {
         double s = 0.0, m = 0.0;
 #pragma unroll 1
         for (int j = 0; j < N; ++j) {
-            double cj = 0.0;
-#pragma unroll
-            for (int c = 0; c < N; ++c) cj = c == j ? jr[c] : cj;
-            if (live) v.rowd[sl] = cj;
+            if (live) v.rowd[sl] = fam.Jentry(sl, j, dg);
             wsync();
             const double* col = v.rowd;
-            if (sums) {
+            double acc = 0.0;
 #pragma unroll 8
-                for (int i = 0; i < N; ++i) {
-                    const double w = e != 0 ? ldexp2(col[i], -e) : col[i];
-                    s = (j == 0 && i == 0) ? w * w : s + w * w;
-                    m = jmax(m, __builtin_fabs(col[i]));
-                }
-            } else {
-                double acc = col[0] * (v.jF[0] * sc);
-#pragma unroll 8
-                for (int i = 1; i < N; ++i) acc = __builtin_fma(col[i], v.jF[i] * sc, acc);
-                if (sl == j) v.r[j] = acc;
+            for (int i = 0; i < N; ++i) {
+                const double w = col[i];
+                const double ws = e != 0 ? ldexp2(w, -e) : w;
+                s = (j == 0 && i == 0) ? ws * ws : s + ws * ws;
+                m = jmax(m, __builtin_fabs(w));
+                const double fi = sc != 1.0 ? v.jF[i] * sc : v.jF[i];
+                acc = i == 0 ? w * fi : __builtin_fma(w, fi, acc);
             }
+            if (prod && sl == j) v.r[j] = acc;
             wsync();
         }
         sumsq = s;
@@ -709,29 +713,27 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG> struct DenseS
 
     GS_DEV double grad_ratio() {
         if (isnan_(diag_fnorm)) return dnan();
-        double jr[N];
-        fam.Jrow(v.jx, live ? sl : 0, jr);
+        const double dg = live ? fam.Jdiag(v.jx, sl) : 0.0;
         const double fnorm = diag_fnorm;
         double s, m;
-        jt_pass(jr, 1.0, true, s, m, 0);
+        diag_pass(dg, 0, true, 1.0, s, m);
         double jnorm;
-        if (isfin(s) && s >= double(N * N) * kDblMin) {
+        if (__builtin_expect(isfin(s) && s >= double(N * N) * kDblMin, 1)) {
             jnorm = __builtin_sqrt(s);
         } else if (m == 0.0 || !isfin(m)) {
             jnorm = m;
         } else {
             const int e = exponent_of(m);
             double s2, m2;
-            jt_pass(jr, 1.0, true, s2, m2, e);
+            diag_pass(dg, e, false, 1.0, s2, m2);
             jnorm = ldexp2(__builtin_sqrt(s2), e);
         }
         if (jnorm == 0.0 || fnorm == 0.0) return 0.0;
-        double d1, d2;
-        jt_pass(jr, 1.0, false, d1, d2, 0);
         const double gnorm = vrnorm<N>(v.r);
         if (isfin(gnorm) || !(isfin(jnorm) && isfin(fnorm))) return gnorm / jnorm / fnorm;
         const int e = exponent_of(fnorm);
-        jt_pass(jr, ldexp2(1.0, -e), false, d1, d2, 0);
+        double s3, m3;
+        diag_pass(dg, 0, true, ldexp2(1.0, -e), s3, m3);
         return vrnorm<N>(v.r) / jnorm / ldexp2(fnorm, -e);
     }
 
