@@ -17,14 +17,19 @@ learned+mixed form: systems with code SUCCESS whose FP64 residual, rechecked on 
 timed region, satisfies ‖F(x)‖₂ ≤ 1e-12, divided by the wall time of the K timed steps; the plain
 form's figure and the ratio sit beside it in `config`.
 
-`--workload fpu_gl3` and `--workload expsum_lsq` print the same kind of line for the 65536-system
-FPU-β Gauss–Legendre stage systems (n = 48) and the 262144 exponential-sum fits.
+`--workload fpu_gl3`, `--workload expsum_lsq` and `--workload dense_sin` print the same kind of line for
+the 65536-system FPU-β Gauss–Legendre stage systems (n = 48), the 262144 exponential-sum fits and the
+2^20 dense systems of the reference's §10.E grid problem at n = 16; with `--sub-results` the default line
+also carries a compact line of each under `config.sub_results`, each measured in a child process after
+the headline's timed region.
 """
 from __future__ import annotations
 
 import argparse
+import faulthandler
 import json
 import os
+import subprocess
 import sys
 import time
 from pathlib import Path
@@ -115,6 +120,7 @@ def bench_henon_heiles(args, gs, workloads, torch):
     mixed = gs.Newton(linsolve=gs.LUFactorization(np.float32), refinement=gs.Refinement())
 
     # the network, trained before any clock starts
+    print("bench: training the network", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     Xt = torch.as_tensor(workloads.henon_heiles_states(NTRAIN, seed=100), device=dev)
     X1, _ = gs.generate_solutions(gs, Xt, H, opts)
@@ -126,6 +132,7 @@ def bench_henon_heiles(args, gs, workloads, torch):
     del Xt, X1
 
     # plain: K steps on fresh copies made before the clock
+    print("bench: timed steps", file=sys.stderr, flush=True)
     starts = [Xk.clone() for _ in range(K + W)]
     Sp = [gs.status_buffer(NSYS, dev) for _ in range(K)]
     Sw = gs.status_buffer(NSYS, dev)
@@ -170,6 +177,7 @@ def bench_henon_heiles(args, gs, workloads, torch):
     start_ms = float(np.mean(_event_ms(ev)))
 
     # host FP64 recheck of every step's result, outside the timed regions
+    print("bench: host recheck and CPU baseline", file=sys.stderr, flush=True)
     def converged(Xs, Ss):
         n = 0
         for X, S in zip(Xs, Ss):
@@ -369,7 +377,14 @@ def bench_expsum(args, gs, workloads, torch):
     # algorithmic HBM bytes of one step: y (8m), K and m (4 + 4), the offset (8), X in and out
     # (8n each) and the status (72) of every fit
     nbytes = float((8 * m_.astype(np.int64) + 16 + 2 * 8 * 2 * K_.astype(np.int64) + 72).sum())
-    achieved = nbytes / (el / args.steps) / 1e9
+    hbm_achieved = nbytes / (el / args.steps) / 1e9
+    # algorithmic FP64 flops of the step: per LM iteration of a fit with K terms (n = 2K) and m points,
+    # m (40K + 2n + 3n²) — the residual and Jacobian rows with their exponentials (≈ 40 per term), the
+    # Givens reduction of [J | −F] (3n² per row) and the right-hand side (2n) — over each fit's own
+    # iteration count (the damping trials beyond the first are not counted: a lower bound)
+    Kf = K_.astype(np.float64)
+    flops = float((st["iterations"].astype(np.float64) * m_.astype(np.float64) * (40 * Kf + 4 * Kf + 12 * Kf * Kf)).sum())
+    achieved = flops / (el / args.steps) / 1e12
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import oracle as orc
@@ -400,17 +415,132 @@ def bench_expsum(args, gs, workloads, torch):
                    "launch": "device bucketing + four persistent bucket kernels, one captured hipGraph per call shape",
                    "tuned_shapes": {str(k): [v[0], v[1]] for k, v in tuned.items()}, "tuning_seconds": tune_s},
         "roofline": {
-            "bound": "hbm",
-            "kernel": "lsq_kernel<2,4,6,8> (concurrent buckets); achieved = algorithmic bytes per step / step time. "
-                      "The kernels are bound by FP64 issue latency of the per-lane Givens sweeps, far from HBM",
+            "bound": "fp64 valu",
+            "kernel": "lsq_kernel<2,4,6,8> (concurrent buckets); achieved = m (40K + 2n + 3n^2) flop per LM "
+                      "iteration of each fit x its iterations / step time (a lower bound: damping trials not "
+                      "counted); peak = the FP64 vector rate",
             "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS,
             "traffic": _traffic("traffic_lsq_expsum_262144.json"),
+            "hbm_achieved_gbs": hbm_achieved,
+            "hbm_frac": hbm_achieved / HBM_PEAK_GBS,
         },
         "cpu_baseline": cpu,
     }
+
+
+# algorithmic flops of one Newton iteration of the dense §10.E family at size n: the LU (2/3 n³), the
+# two triangular solves (2n²) and the residual's matrix-vector product (2n²)
+def dense_flops_per_iter(n: int) -> int:
+    return 2 * n ** 3 // 3 + 4 * n * n
+
+
+def bench_dense(args, gs, workloads, torch, n: int = 16, nsys: int = 1 << 20):
+    """The §10.E problem of the reference's device grid (A x + sin.(x)/10 − b, benchmark/device_problems.jl:
+    133-175) at n = 16 from x₀ = 0, plain FP64 Newton, status form: a wave segment per system."""
+    dev = torch.device("cuda:0")
+    lib = gs.load_library()
+    A, B, _ = workloads.tene_batch(n, nsys)
+    fam = gs.DenseSin(A)
+    prob = gs.NonlinearProblem(fam)
+    Bd = torch.as_tensor(B, device=dev)
+    p = gs.PerSystem(Bd)
+    opts = gs.Options(f_abstol=TOL, f_reltol=0.0)
+    m = gs.Newton()
+    Xs = [torch.zeros((nsys, n), dtype=torch.float64, device=dev) for _ in range(args.steps + args.warmup)]
+    S = [gs.status_buffer(nsys, dev) for _ in range(args.steps)]
+    Sw = gs.status_buffer(nsys, dev)
+    for i in range(args.warmup):
+        gs.solve_batch(Xs[args.steps + i], Sw, prob, m, p, opts)
+    torch.cuda.synchronize()
+    lib.gs_set_kernel_timing(1)
+    kms = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        gs.solve_batch(Xs[i], S[i], prob, m, p, opts)
+        kms.append(lib.gs_last_kernel_ms())
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    lib.gs_set_kernel_timing(0)
+    conv, iters = 0, 0
+    for i in range(args.steps):
+        st = gs.statuses(S[i])
+        r = np.sqrt((workloads.tene_residual(Xs[i].cpu().numpy(), A, B) ** 2).sum(axis=1))
+        conv += int(((st["code"] == 0) & (r <= TOL)).sum())
+        iters += int(st["iterations"].sum())
+    kmean = float(np.mean(kms))
+    achieved = dense_flops_per_iter(n) * iters / args.steps / (kmean * 1e-3) / 1e12
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as orc
+
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+        k = 1 << 15
+        reps, cel, (_, so) = _timed_oracle(
+            lambda: orc.solve_batch(m, fam, np.zeros((k, n)), gs.PerSystem(B[:k]), opts, nthreads=threads), 8.0)
+        cpu = {"value": int((so["code"] == 0).sum()) * reps / cel, "unit": "converged systems/s", "cores": threads,
+               "kind": "port", "sample": f"plain FP64 Newton on the first {k} systems of the same batch x {reps} "
+                                         f"repetitions, {cel:.1f} s wall, OpenMP schedule(static) over systems"}
+    return {
+        "metric": "converged systems/s at residual ≤1e-12 (FP64 check), 1 GPU",
+        "value": conv / el,
+        "unit": "converged systems/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (the §10.E batch: A = 2I + R from seed n, x* ~ U[-1,1]^n from seed 1, numpy PCG64)",
+        "config": {"workload": f"dense_sin (§10.E A x + sin(x)/10 - b) n={n}, batch {nsys}, x0 = 0, abstol 1e-12, "
+                               "reltol 0, status form",
+                   "method": "Newton() + Static + static LU in FP64; a 16- or 32-lane wave segment per system",
+                   "kernel_ms": kmean, "mean_iterations": iters / args.steps / nsys,
+                   "reference_published_n16": "2.11e7 systems/s (RTX 4090, Float32, gs-newton; benchmark/REPORT.md), "
+                                              "context only"},
+        "roofline": {
+            "bound": "fp64 valu",
+            "kernel": f"dense_kernel<{n}, Static, FP64, status>; achieved = (2/3 n^3 + 4 n^2) flop per Newton "
+                      "iteration x the batch's iterations / kernel time",
+            "achieved": achieved,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS,
+            "traffic": None,
+        },
+        "cpu_baseline": cpu,
+    }
+
+
+def sub_results():
+    """Compact lines of the other configurations (the n = 48 stage systems, the uneven least-squares
+    batch, the dense n = 16 family), each measured by its own `--workload` run in a child process (fewer
+    steps, no CPU baseline), one after the other; a child that fails ends the series, and its exit status
+    is recorded in place of its line."""
+    out = {}
+    for name, w in (("fpu_gl3", "fpu_gl3"), ("expsum_lsq", "expsum_lsq"), ("dense_sin_n16", "dense_sin")):
+        cmd = [sys.executable, str(ROOT / "bench.py"), "--workload", w, "--steps", "3", "--warmup", "1",
+               "--no-cpu-baseline"]
+        print(f"bench: sub-result {name}", file=sys.stderr, flush=True)
+        try:
+            r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+        except subprocess.TimeoutExpired:
+            out[name] = {"error": "timed out after 300 s"}
+            break
+        if r.returncode != 0:
+            out[name] = {"error": f"exit status {r.returncode}", "stderr_tail": r.stderr[-1500:]}
+            break
+        line = json.loads(r.stdout.strip().splitlines()[-1])
+        out[name] = {"value": line["value"], "unit": line["unit"], "ms_per_step": line["ms_per_step"],
+                     "roofline_frac": line["roofline"]["frac"], "roofline_bound": line["roofline"]["bound"],
+                     "workload": line["config"]["workload"]}
+        if "plain_value" in line["config"]:
+            out[name]["plain_value"] = line["config"]["plain_value"]
+    return out
 
 
 def main() -> int:
@@ -419,9 +549,13 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--epochs", type=int, default=8)
-    ap.add_argument("--workload", choices=("henon_heiles", "fpu_gl3", "expsum_lsq"), default="henon_heiles")
+    ap.add_argument("--workload", choices=("henon_heiles", "fpu_gl3", "expsum_lsq", "dense_sin"),
+                    default="henon_heiles")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--sub-results", action="store_true",
+                    help="add compact lines of the other configurations (each in a child process) to the default line")
     args = ap.parse_args()
+    faulthandler.enable()  # a fatal signal prints the Python stack of every thread to stderr
     if args.gpus != 1:
         print("bench.py measures one GPU: the systems are independent and this build runs on a single "
               "MI355X; --gpus above 1 is not supported", file=sys.stderr)
@@ -435,10 +569,14 @@ def main() -> int:
 
     gs = __graft_entry__.load_package()
     workloads = importlib.import_module(gs.__name__ + ".workloads")
-    fn = {"henon_heiles": bench_henon_heiles, "fpu_gl3": bench_fpu_gl3, "expsum_lsq": bench_expsum}[args.workload]
+    fn = {"henon_heiles": bench_henon_heiles, "fpu_gl3": bench_fpu_gl3, "expsum_lsq": bench_expsum,
+          "dense_sin": bench_dense}[args.workload]
     if args.workload != "henon_heiles" and "--steps" not in sys.argv:
         args.steps, args.warmup = 5, 1
-    print(json.dumps(fn(args, gs, workloads, torch)), flush=True)
+    line = fn(args, gs, workloads, torch)
+    if args.workload == "henon_heiles" and args.sub_results:
+        line["config"]["sub_results"] = sub_results()
+    print(json.dumps(line), flush=True)
     return 0
 
 
