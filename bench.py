@@ -19,9 +19,9 @@ form's figure and the ratio sit beside it in `config`.
 
 `--workload fpu_gl3`, `--workload expsum_lsq` and `--workload dense_sin` print the same kind of line for
 the 65536-system FPU-β Gauss–Legendre stage systems (n = 48), the 262144 exponential-sum fits and the
-2^20 dense systems of the reference's §10.E grid problem at n = 16; with `--sub-results` the default line
-also carries a compact line of each under `config.sub_results`, each measured in a child process after
-the headline's timed region.
+2^20 dense systems of the reference's §10.E grid problem at n = 16. The default line also carries a
+compact line of each under `config.sub_results`, each measured in a child process after the headline's
+timed region (`--no-sub-results` leaves them out).
 """
 from __future__ import annotations
 
@@ -330,7 +330,7 @@ def bench_fpu_gl3(args, gs, workloads, torch):
                    "mean_iterations": out["mixed"][2], "mixed_kernel_ms": out["mixed"][3],
                    "plain_kernel_ms": out["plain"][3]},
         "roofline": {
-            "bound": "mfma",
+            "bound": "fp64 valu",
             "kernel": "fpu_gl3_kernel<mixed, refine> (one wave per system, each lane's factor row in VGPRs; "
                       "FP32 LU, FP64 refinement; VALU-bound, no MFMA); achieved = "
                       f"{FPU_FLOPS_PER_ITER} flop per Newton iteration x the batch's iterations / kernel time; "
@@ -552,8 +552,8 @@ def main() -> int:
     ap.add_argument("--workload", choices=("henon_heiles", "fpu_gl3", "expsum_lsq", "dense_sin"),
                     default="henon_heiles")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--sub-results", action="store_true",
-                    help="add compact lines of the other configurations (each in a child process) to the default line")
+    ap.add_argument("--no-sub-results", action="store_true",
+                    help="leave out the compact lines of the other configurations (each measured in a child process)")
     args = ap.parse_args()
     faulthandler.enable()  # a fatal signal prints the Python stack of every thread to stderr
     if args.gpus != 1:
@@ -574,7 +574,7 @@ def main() -> int:
     if args.workload != "henon_heiles" and "--steps" not in sys.argv:
         args.steps, args.warmup = 5, 1
     line = fn(args, gs, workloads, torch)
-    if args.workload == "henon_heiles" and args.sub_results:
+    if args.workload == "henon_heiles" and not args.no_sub_results:
         line["config"]["sub_results"] = sub_results()
     print(json.dumps(line), flush=True)
     return 0
