@@ -50,6 +50,8 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 struct alignas(16) MidShared {
     double x[kN], Fx[kN], d[kN], r[kN], t1[kN], t2[kN], t3[kN], bx[kN];
+    double jx[kN];      // the point of the last factorisation's Jacobian
+    double jsq[kN * 9]; // the squares of ‖J‖_F's nonzero entries, column by column
     double qp[32];      // q, p of the system
     double Qv[kN];      // stage positions
     double g[kN];       // stage forces
@@ -57,6 +59,13 @@ struct alignas(16) MidShared {
 };
 
 GS_DEV void sync() { __syncthreads(); }
+
+// LDS written by some lanes of the wave, then read by others (the workgroup is the one wave).
+GS_DEV void wsync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 GS_DEV int lane_id() { return int(threadIdx.x); }
 
@@ -307,7 +316,7 @@ GS_DEV void jmatvec(MidShared& s, const Fpu& f, double mu, const double* v, doub
 // 41-57). Lane j forms the nine entries of column j at rows a·16 + k−1, k, k+1 (k = j mod 16; zero
 // where the chain ends, and a zero square adds nothing to the nonnegative sum), then the fold takes
 // their squares column by column through v_readlane.
-GS_DEV double jnorm_fold(const MidShared& s, const Fpu& f) {
+GS_DEV double jnorm_fold(MidShared& s, const Fpu& f) {
     const int l = lane_id();
     const int j = l < kN ? l : 0;
     const int k = j % kSites;
@@ -319,11 +328,18 @@ GS_DEV double jnorm_fold(const MidShared& s, const Fpu& f) {
         v[t] = (l < kN && m >= 0 && m < kSites) ? jentry(s, f, (t / 3) * kSites + m, j) : 0.0;
         sq[t] = v[t] * v[t];
         mx = jmax(mx, __builtin_fabs(v[t]));
+        if (l < kN) s.jsq[9 * l + t] = sq[t];
     }
+    wsync();
+    // the fold over columns 0..47, entries t = 0..8 of each: the LDS array in order, read by every
+    // lane as 16-byte broadcasts
     double acc = 0.0;
-    for (int c = 0; c < kN; ++c) {
-#pragma unroll
-        for (int t = 0; t < 9; ++t) acc = acc + readlane(sq[t], c);
+    const double2* q = reinterpret_cast<const double2*>(s.jsq);
+#pragma unroll 8
+    for (int c = 0; c < kN * 9 / 2; ++c) {
+        const double2 w = q[c];
+        acc = acc + w.x;
+        acc = acc + w.y;
     }
     if (isfin(acc) && acc >= double(kN * kN) * kDblMin) return __builtin_sqrt(acc);
     // the scaled path: the NaN-propagating maximum of |J_ij|, then the fold of (J_ij/2^e)²
@@ -367,21 +383,21 @@ GS_DEV void jtmatvec(MidShared& s, const Fpu& f, const double* v, double sc, dou
     sync();
 }
 
-// grad_ratio_and_norm (newton.jl:1287-1298) at the point of K, with F in s.Fx.
-GS_DEV void grad_ratio(MidShared& s, const Fpu& f, double fnorm, double& gr, double& jn) {
+// grad_ratio_and_norm (newton.jl:1287-1298) at the point of K, with F in Fv (an LDS vector ≠ s.t3).
+GS_DEV void grad_ratio(MidShared& s, const Fpu& f, const double* Fv, double fnorm, double& gr, double& jn) {
     jn = jnorm_fold(s, f);
     if (jn == 0.0 || fnorm == 0.0) {
         gr = 0.0;
         return;
     }
-    jtmatvec(s, f, s.Fx, 1.0, s.t3);
+    jtmatvec(s, f, Fv, 1.0, s.t3);
     const double gn = fold_rnorm(s.t3);
     if (isfin(gn) || !(isfin(jn) && isfin(fnorm))) {
         gr = gn / jn / fnorm;
         return;
     }
     const int e = exponent_of(fnorm);
-    jtmatvec(s, f, s.Fx, ldexp2(1.0, -e), s.t3);
+    jtmatvec(s, f, Fv, ldexp2(1.0, -e), s.t3);
     gr = fold_rnorm(s.t3) / jn / ldexp2(fnorm, -e);
 }
 
@@ -506,10 +522,11 @@ template <class T, bool TAME> GS_DEV int lu_rows(T (&a)[kN], int& pos) {
         keep(a[k]);
         if (TAME) {
             const T mk = below ? lik : T(0);
+            // the pivot row's entries reach the other lanes by v_readlane (into SGPRs); a sched
+            // barrier every eight columns keeps only a few broadcasts in flight
 #pragma unroll
             for (int j = k + 1; j < kN; ++j) {
                 a[j] = a[j] - mk * readlane(a[j], piv);
-                // keep the broadcasts of a few columns in flight, not the whole row's in SGPRs
                 if ((j - k) % 8 == 0) __builtin_amdgcn_sched_barrier(0);
             }
         } else {
@@ -539,7 +556,8 @@ template <class T> GS_DEV T solve_rows(const T (&a)[kN], int pos, T y) {
 #pragma unroll
     for (int k = kN - 1; k >= 0; --k) {
         const int o = lowest_lane(__ballot(live && pos == k));
-        const T xk = readlane(y, o) / readlane(a[k], o);
+        const T quo = y / a[k];  // the owner's quotient (the other lanes' are not used)
+        const T xk = readlane(quo, o);
         const T t = y - a[k] * xk;
         y = l == o ? xk : (live && pos < k ? t : y);
     }
@@ -587,11 +605,13 @@ struct Info {
 
 struct NewtonState {
     double fnorm;      // the ‖F‖ the state carries
-    double gradratio;  // diagnostics of the last factorisation
+    double gradratio;  // diagnostics of the last factorisation, formed at exit
     double jnorm;
     bool promoted;
     int32_t slow;
     double best_fnorm;
+    double jfnorm;     // the ‖F‖ the last factorisation's diagnostics read (its point is s.jx)
+    bool have_j;
 };
 
 // globalize!! Static (linefunction.jl:114-131): x, Fx updated on success.
@@ -679,11 +699,23 @@ GS_DEV int factor_and_direction(MidShared& s, const Fpu& f, const DevParams& q, 
     return info;
 }
 
-// newton_factor (newton.jl:1302-1311) up to the factorisation: fnorm, J at x, its diagnostics.
+// newton_factor (newton.jl:1302-1311) up to the factorisation: J at x. Only the last
+// factorisation's diagnostics reach the status, so the point and the ‖F‖ they read are kept and
+// exit_diag forms them once (F and J are functions of x alone: the values are the same).
 GS_DEV void factor_diag(MidShared& s, const Fpu& f, NewtonState& st) {
-    const double fnorm = isnan_(st.fnorm) ? fold_rnorm(s.Fx) : st.fnorm;
+    st.jfnorm = isnan_(st.fnorm) ? fold_rnorm(s.Fx) : st.fnorm;
+    st.have_j = true;
+    const int l = lane_id();
+    if (l < kN) s.jx[l] = s.x[l];
     eval_K(s, f, s.x);
-    grad_ratio(s, f, fnorm, st.gradratio, st.jnorm);
+}
+
+// grad_ratio_and_norm (newton.jl:1287-1298) of the last factorisation, at exit.
+GS_DEV void exit_diag(MidShared& s, const Fpu& f, NewtonState& st) {
+    if (!st.have_j) return;
+    eval_K(s, f, s.jx);
+    eval_F(s, f, s.jx, s.t1);
+    grad_ratio(s, f, s.t1, st.jfnorm, st.gradratio, st.jnorm);
 }
 
 // newton_step!! plain (newton.jl:1679-1688).
@@ -732,6 +764,9 @@ GS_DEV Info mixed_step(MidShared& s, const Fpu& f, const DevParams& q, NewtonSta
         }
     }
     if (st.promoted) {
+        // The FP64 factor of a promoted state is left to the EXACT kernel, which solves such a
+        // system again from its start: the fast kernel then holds only the FP32 factor's registers.
+        if (!EXACT) return Info{kRedo, 0.0, 0.0, 0, false};
         bool failed;
         const int info64 = factor_and_direction<double, REFINE, EXACT>(s, f, q, was_promoted, failed);
         if (info64 == kRedoInfo) return Info{kRedo, 0.0, 0.0, 0, false};
@@ -790,7 +825,8 @@ GS_DEV void store_status_mid(gs_status_f64* dst, uint8_t code, int32_t iteration
 }
 
 // One wave per system. The kernel with EXACT = false factors with the zero-multiplier LU and hands a
-// system whose rows are ever not tame to the EXACT kernel, launched after it over the same grid:
+// system whose rows are ever not tame, or (mixed) whose state is ever promoted to the FP64 factor, to
+// the EXACT kernel, launched after it over the same grid:
 // that system keeps its input untouched and redo[idx] = 1; every other system writes redo[idx] = 0,
 // and the EXACT kernel solves only the systems marked 1, from the start.
 template <bool MIXED, bool REFINE, bool EXACT> __global__ __launch_bounds__(64) void fpu_gl3_kernel(const MidLaunch L) {
@@ -810,7 +846,7 @@ template <bool MIXED, bool REFINE, bool EXACT> __global__ __launch_bounds__(64) 
     // solve!! (solver.jl:277-378)
     eval_F(s, f, s.x, s.Fx);
     const double fnorm0 = fold_rnorm(s.Fx);
-    NewtonState st{fnorm0, dnan(), 0.0, false, 0, fnorm0};
+    NewtonState st{fnorm0, dnan(), 0.0, false, 0, fnorm0, 0.0, false};
     if (q.best && l < kN) s.bx[l] = s.x[l];
     uint8_t code = kSuccess;
     int32_t it = 0, failures = 0;
@@ -891,6 +927,7 @@ template <bool MIXED, bool REFINE, bool EXACT> __global__ __launch_bounds__(64) 
         sync();
     }
     if (l < kN) L.X[idx * kN + l] = s.x[l];
+    if (L.S) exit_diag(s, f, st);
     if (l == 0) {
         if (!EXACT) L.redo[idx] = 0;
         if (L.S) store_status_mid(L.S + idx, code, it, fnorm, stepnorm, failures, promoted, st.gradratio, rate, contracted);

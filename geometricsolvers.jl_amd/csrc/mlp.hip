@@ -356,7 +356,7 @@ __global__ __launch_bounds__(64 * kStartWaves) void learned_solve_kernel(const L
             predicted_start(xk, a.st.h, L.corr + lane * 4, a.st.nm, x);
             FamHenonHeilesMidpoint fam;
             fam.load(fa, s);
-            SmallSolve<FamHenonHeilesMidpoint, kStatic, MIXED, REFINE, true> sol(fam, a.q);
+            SmallSolve<FamHenonHeilesMidpoint, kStatic, MIXED, REFINE, true, false, true> sol(fam, a.q);
             const auto R = sol.solve(x);
             double2* xp = reinterpret_cast<double2*>(a.st.X0 + 4 * s);
             if (R.code == kSuccess) {

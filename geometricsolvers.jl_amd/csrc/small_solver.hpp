@@ -260,7 +260,10 @@ template <int N, class T> GS_DEV T grad_ratio(const T (&J)[N][N], const T (&Fx)[
 // One system
 // ------------------------------------------------------------------------------------------------
 
-template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = false> struct SmallSolve {
+// EAGER: the status's gradient ratio is formed at every factorisation (cheaper where a solve usually
+// ends after its first step); otherwise once, at exit, at the last factorisation's point.
+template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = false, bool EAGER = false>
+struct SmallSolve {
     static constexpr int N = Fam::N;
     using T = typename Fam::T;  // the working precision
     using TF = typename std::conditional<MIXED, float, T>::type;
@@ -272,6 +275,12 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
     T merit;       // norm2(Fx) for Backtracking
     T sfnorm;      // ‖Fx‖ that the state carries
     T gradratio;   // diagnostics of the last factorisation
+    // The point of the last factorisation's Jacobian and the ‖F‖ its diagnostics read: only the last
+    // factorisation's gradient ratio reaches the status, so it is formed once, at exit, from F and J
+    // evaluated again at that point (F and J are functions of x alone, so the values are the same).
+    T jx[N];
+    T jfn;
+    bool have_j;
     bool promoted;      // mixed precision: factorising in FP64
     int slow_steps;
     // reuse = k: the factor the state holds, in the factor precision, and the rule state FactorAge
@@ -484,11 +493,17 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
     // --- steps ----------------------------------------------------------------------------------
 
     GS_DEV void jacobian(const T (&x)[N], const T (&Fx)[N], T (&J)[N][N]) {
-        if (DIAG) {
-            T fnorm = residual_norm(Fx);
+        if (DIAG && EAGER) {
+            const T fnorm = residual_norm(Fx);
             fam.J(x, J);
             gradratio = grad_ratio<N>(J, Fx, fnorm);
         } else {
+            if (DIAG) {
+                jfn = residual_norm(Fx);
+#pragma unroll
+                for (int i = 0; i < N; ++i) jx[i] = x[i];
+                have_j = true;
+            }
             fam.J(x, J);
         }
         if (q.mu != T(0.0)) {
@@ -1047,6 +1062,7 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
         merit = GLOB == kBacktracking ? norm2<N>(Fx) : tnan<T>();
         sfnorm = fnorm0;
         gradratio = tnan<T>();
+        have_j = false;
         promoted = false;
         slow_steps = 0;
         age = 0;
@@ -1131,6 +1147,13 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = 
                     }
                 }
             }
+        }
+        if (DIAG && !EAGER && have_j) {
+            // grad_ratio_and_norm (newton.jl:1287-1298) of the last factorisation
+            T Fj[N], Jj[N][N];
+            fam.F(jx, Fj);
+            fam.J(jx, Jj);
+            gradratio = grad_ratio<N>(Jj, Fj, jfn);
         }
         R.gradratio = gradratio;
         R.reused = REUSE && reused && R.iterations > 0;  // reused_factor (newton.jl:1188-1191)
