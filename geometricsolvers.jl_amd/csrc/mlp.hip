@@ -176,23 +176,24 @@ GS_DEV double lane_value(double v, int src) {
     return __builtin_bit_cast(double, (uint64_t(uint32_t(hi)) << 32) | uint32_t(lo));
 }
 
-// The network on the kChunk systems of one wave pass. Lane l passes the state x of system l of the
-// chunk (any finite value past the batch); the outputs o + b4 of system l land in L.corr[4 l + j].
-// Tile t holds systems 16 t … 16 t + 15 as the B columns; every weight fragment is read from LDS once
-// and multiplies all kTiles tiles.
-GS_DEV void mlp_chunk(const MlpLds& L, const MlpNorm& nm, double h, const double (&x)[4]) {
+// The network on TILES × 16 systems of a chunk, from system `base` on. Lane l passes the state x of
+// system l of the chunk (any finite value past the batch); the outputs o + b4 of system b land in
+// L.corr[4 b + j]. Tile t holds systems base + 16 t … base + 16 t + 15 as the B columns; every weight
+// fragment is read from LDS once and multiplies all TILES tiles (the arithmetic of a system does not
+// depend on TILES).
+template <int TILES> GS_DEV void mlp_tiles(const MlpLds& L, const MlpNorm& nm, double h, const double (&x)[4], int base) {
     const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
     constexpr int kKs = kHid / 32, kOb = kHid / 16;
     // activations as packed BF16 pairs: nxt[t][s] is the B operand of tile t in k-step s
-    u32x4 act[kTiles][kKs], nxt[kTiles][kKs];
+    u32x4 act[TILES][kKs], nxt[TILES][kKs];
     const float uh = (float(h) - nm.in_shift[4]) * nm.in_scale[4];
     // layer 1 inputs: slots 0..3 of lane group 0 are the features of x, slot 0 of group 1 is h's
-    bf16x8 in[kTiles];
+    bf16x8 in[TILES];
 #pragma unroll
-    for (int t = 0; t < kTiles; ++t) {
+    for (int t = 0; t < TILES; ++t) {
         double xt[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) xt[j] = lane_value(x[j], 16 * t + r);
+        for (int j = 0; j < 4; ++j) xt[j] = lane_value(x[j], base + 16 * t + r);
 #pragma unroll
         for (int j = 0; j < 8; ++j) in[t][j] = __bf16(0.0f);
         if (g == 0) {
@@ -205,9 +206,9 @@ GS_DEV void mlp_chunk(const MlpLds& L, const MlpNorm& nm, double h, const double
     // The bias of unit block ob, rows 4 g … 4 g + 3: the C operand of the block's first MFMA.
     auto bias_of = [&](const float* b, int ob) { return *reinterpret_cast<const f32x4*>(b + ob * 16 + 4 * g); };
     // ReLU of unit block ob into the B operands of the next layer (blocks 2 s, 2 s + 1 → k-step s)
-    auto activate = [&](const f32x4 (&acc)[kTiles], int ob) {
+    auto activate = [&](const f32x4 (&acc)[TILES], int ob) {
 #pragma unroll
-        for (int t = 0; t < kTiles; ++t) {
+        for (int t = 0; t < TILES; ++t) {
             nxt[t][ob >> 1][(ob & 1) * 2] = relu_bf16x2(acc[t][0], acc[t][1]);
             nxt[t][ob >> 1][(ob & 1) * 2 + 1] = relu_bf16x2(acc[t][2], acc[t][3]);
         }
@@ -216,27 +217,27 @@ GS_DEV void mlp_chunk(const MlpLds& L, const MlpNorm& nm, double h, const double
     for (int ob = 0; ob < kOb; ++ob) {
         const bf16x8 wv = *reinterpret_cast<const bf16x8*>(L.w + kP1 + (ob * 64 + lane) * 8);
         const f32x4 bv = bias_of(L.bias, ob);
-        f32x4 acc[kTiles];
+        f32x4 acc[TILES];
 #pragma unroll
-        for (int t = 0; t < kTiles; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wv, in[t], bv, 0, 0, 0);
+        for (int t = 0; t < TILES; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wv, in[t], bv, 0, 0, 0);
         activate(acc, ob);
     }
     // a hidden layer, one unit block at a time: only that block's accumulators are live
     auto hidden = [&](int poff, const float* b) {
 #pragma unroll
-        for (int t = 0; t < kTiles; ++t) {
+        for (int t = 0; t < TILES; ++t) {
 #pragma unroll
             for (int ks = 0; ks < kKs; ++ks) act[t][ks] = nxt[t][ks];
         }
 #pragma unroll
         for (int ob = 0; ob < kOb; ++ob) {
             const f32x4 bv = bias_of(b, ob);
-            f32x4 acc[kTiles];
+            f32x4 acc[TILES];
 #pragma unroll
             for (int ks = 0; ks < kKs; ++ks) {
                 const bf16x8 wv = *reinterpret_cast<const bf16x8*>(L.w + poff + ((ks * kOb + ob) * 64 + lane) * 8);
 #pragma unroll
-                for (int t = 0; t < kTiles; ++t)
+                for (int t = 0; t < TILES; ++t)
                     acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wv, __builtin_bit_cast(bf16x8, act[t][ks]),
                                                                       ks == 0 ? bv : acc[t], 0, 0, 0);
             }
@@ -245,22 +246,27 @@ GS_DEV void mlp_chunk(const MlpLds& L, const MlpNorm& nm, double h, const double
     };
     hidden(kP2, L.bias + kHid);
     hidden(kP3, L.bias + 2 * kHid);
-    f32x4 o[kTiles];
+    f32x4 o[TILES];
     const f32x4 b4 = bias_of(L.bias + 3 * kHid, 0);
 #pragma unroll
     for (int ks = 0; ks < kKs; ++ks) {
         const bf16x8 wv = *reinterpret_cast<const bf16x8*>(L.w + kP4 + (ks * 64 + lane) * 8);
 #pragma unroll
-        for (int t = 0; t < kTiles; ++t)
+        for (int t = 0; t < TILES; ++t)
             o[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wv, __builtin_bit_cast(bf16x8, nxt[t][ks]), ks == 0 ? b4 : o[t],
                                                            0, 0, 0);
     }
     // outputs 0..3 are rows 0..3 of D: lane group 0
     if (g == 0) {
 #pragma unroll
-        for (int t = 0; t < kTiles; ++t) *reinterpret_cast<f32x4*>(L.corr + (16 * t + r) * 4) = o[t];
+        for (int t = 0; t < TILES; ++t) *reinterpret_cast<f32x4*>(L.corr + (base + 16 * t + r) * 4) = o[t];
     }
     wave_lds_fence();
+}
+
+// The network on the kChunk systems of one wave pass (kTiles tiles).
+GS_DEV void mlp_chunk(const MlpLds& L, const MlpNorm& nm, double h, const double (&x)[4]) {
+    mlp_tiles<kTiles>(L, nm, h, x, 0);
 }
 
 GS_DEV void load_state(const double* X, int64_t s, double (&x)[4]) {
@@ -315,7 +321,7 @@ struct LearnedArgs {
     DevParams q;    // the method of the solve from the learned start
     DevParams qf;   // the fallback: plain Newton from x_k
     gs_status_f64* S;
-    uint32_t* fallbacks;
+    uint32_t* counters;  // [0] systems that fell back, [1] waves whose hand-over wait ran out
 };
 
 GS_DEV void store_status_learned(gs_status_f64* dst, uint8_t code, int32_t iterations, double fnorm, double stepnorm,
@@ -332,49 +338,144 @@ GS_DEV void store_status_learned(gs_status_f64* dst, uint8_t code, int32_t itera
     w[8] = uint64_t(contracted ? 1 : 0);
 }
 
-// Each wave takes 64 systems at a time: the network on all of them (corrections into LDS), then one
-// lane per system runs the method from its predicted start; a lane whose solve does not end in
+// The workgroup of the fused solve: four producer waves (one per SIMD) run the network on BF16 MFMA
+// and eight consumer waves (two per SIMD) run Newton in FP64 on the VALU, so that the two pipes work
+// at once. A workgroup takes groups of kWsCons chunks (64 systems each) in rounds; in round r,
+// producer p forms the predicted starts of chunks 2p and 2p + 1 of the round's group into LDS slots
+// (round parity × kWsCons + chunk), and consumer c solves chunk c from its slot. Slot flags: full =
+// the rounds produced into the slot, done = the rounds its consumer has read; a producer refills a
+// slot once the round two back has been read.
+constexpr int kWsProd = 4, kWsCons = 8, kWsWaves = kWsProd + kWsCons;
+constexpr int kWsSlots = 2 * kWsCons;
+constexpr size_t kWsSlotBytes = size_t(kChunk) * 4 * sizeof(double);
+constexpr size_t kWsCorr = kLdsWeights + kLdsBias;
+constexpr size_t kWsSlotOff = kWsCorr + size_t(kWsProd) * kLdsCorr;
+constexpr size_t kWsFlagOff = kWsSlotOff + size_t(kWsSlots) * kWsSlotBytes;
+constexpr size_t kWsLds = kWsFlagOff + size_t(2 * kWsSlots) * sizeof(int);
+static_assert(kWsLds <= 160 * 1024, "the fused solve's LDS");
+static_assert(kWsProd * 2 == kWsCons, "two chunks per producer and round");
+// A wait gives up after this many polls (each an LDS read and a short sleep: over a second), so that
+// a broken hand-over ends the kernel with an error instead of hanging it.
+constexpr int kWsSpinLimit = 1 << 24;
+
+// Wait until *flag ≥ v (acquire at workgroup scope); false when the wait runs out.
+GS_DEV bool wait_flag(const int* flag, int v) {
+    for (int n = 0; n < kWsSpinLimit; ++n) {
+        if (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) >= v) return true;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return false;
+}
+
+// Publish v in *flag after the wave's earlier LDS accesses (release at workgroup scope).
+GS_DEV void post_flag(int* flag, int v) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    if ((threadIdx.x & 63) == 0) __hip_atomic_store(flag, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Producer p: the network and the predicted starts of chunks 2p and 2p + 1 of every round.
+GS_DEV void learned_produce(const LearnedArgs& a, const MlpLds& L, int p, int64_t nround, int64_t nchunk, double* slots,
+                            const int* done, int* full) {
+    const int lane = threadIdx.x & 63;
+    for (int64_t r = 0; r < nround; ++r) {
+        const int64_t g = int64_t(blockIdx.x) + r * gridDim.x;
+#pragma unroll 1
+        for (int i = 0; i < 2; ++i) {
+            const int c = 2 * p + i;
+            const int64_t chunk = g * kWsCons + c;
+            const int64_t s = chunk * kChunk + lane;
+            double y[4] = {0.0, 0.0, 0.0, 0.0};
+            if (chunk < nchunk) {  // uniform over the wave: the MFMAs run with every lane
+                {
+                    double xk[4] = {0.0, 0.0, 0.0, 0.0};
+                    if (s < a.st.nsys) load_state(a.st.Xk, s, xk);
+                    // two tiles at a time: half the activations' registers of mlp_chunk
+                    mlp_tiles<2>(L, a.st.nm, a.st.h, xk, 0);
+                    mlp_tiles<2>(L, a.st.nm, a.st.h, xk, 32);
+                }
+                if (s < a.st.nsys) {
+                    double xk[4];  // read again (from L2) rather than held through the network
+                    load_state(a.st.Xk, s, xk);
+                    predicted_start(xk, a.st.h, L.corr + lane * 4, a.st.nm, y);
+                }
+                wave_lds_fence();  // the outputs are read before the next chunk's network overwrites them
+            }
+            const int slot = int(r & 1) * kWsCons + c;
+            if (!wait_flag(done + slot, int(r) - 1)) {
+                if (lane == 0) atomicAdd(a.counters + 1, 1u);
+                return;
+            }
+            double2* dst = reinterpret_cast<double2*>(slots + (size_t(slot) * kChunk + lane) * 4);
+            dst[0] = double2{y[0], y[1]};
+            dst[1] = double2{y[2], y[3]};
+            post_flag(full + slot, int(r) + 1);
+        }
+    }
+}
+
+// Consumer c: chunk c of every round, from the learned start; a system whose solve does not end in
 // SUCCESS solves again from x_k with plain Newton, and that status is the system's.
 template <bool MIXED, bool REFINE>
-__global__ __launch_bounds__(64 * kStartWaves) void learned_solve_kernel(const LearnedArgs a) {
-    extern __shared__ __align__(16) unsigned char lds[];
-    const MlpLds L = mlp_lds_setup(lds, a.st.packed, a.st.params);
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t nchunk = (a.st.nsys + kChunk - 1) / kChunk;
+GS_DEV void learned_consume(const LearnedArgs& a, int c, int64_t nround, const double* slots, const int* full,
+                            int* done) {
+    const int lane = threadIdx.x & 63;
     FamArgs fa;
     fa.P = reinterpret_cast<const char*>(a.st.Xk);
     fa.p_stride = 32;
     fa.shared[0] = a.st.h;
     for (int k = 1; k < 8; ++k) fa.shared[k] = 0.0;
-    for (int64_t c = int64_t(blockIdx.x) * kStartWaves + wave; c < nchunk; c += int64_t(gridDim.x) * kStartWaves) {
-        const int64_t s = c * kChunk + lane;
-        double xk[4] = {0.0, 0.0, 0.0, 0.0};
-        if (s < a.st.nsys) load_state(a.st.Xk, s, xk);
-        mlp_chunk(L, a.st.nm, a.st.h, xk);
+    for (int64_t r = 0; r < nround; ++r) {
+        const int64_t g = int64_t(blockIdx.x) + r * gridDim.x;
+        const int64_t s = (g * kWsCons + c) * kChunk + lane;
+        const int slot = int(r & 1) * kWsCons + c;
+        if (!wait_flag(full + slot, int(r) + 1)) {
+            if (lane == 0) atomicAdd(a.counters + 1, 1u);
+            return;
+        }
+        const double2* src = reinterpret_cast<const double2*>(slots + (size_t(slot) * kChunk + lane) * 4);
+        const double2 u = src[0], v = src[1];
+        post_flag(done + slot, int(r) + 1);
         if (s < a.st.nsys) {
-            double x[4];
-            predicted_start(xk, a.st.h, L.corr + lane * 4, a.st.nm, x);
+            double x[4] = {u.x, u.y, v.x, v.y};
             FamHenonHeilesMidpoint fam;
             fam.load(fa, s);
-            SmallSolve<FamHenonHeilesMidpoint, kStatic, MIXED, REFINE, true, false, true> sol(fam, a.q);
+            SmallSolve<FamHenonHeilesMidpoint, kStatic, MIXED, REFINE, true> sol(fam, a.q);
             const auto R = sol.solve(x);
             double2* xp = reinterpret_cast<double2*>(a.st.X0 + 4 * s);
             if (R.code == kSuccess) {
                 store_status_learned(a.S + s, R.code, R.iterations, R.fnorm, R.stepnorm, R.failures, R.promoted,
                                      R.gradratio, R.rate, R.contracted);
             } else {
-#pragma unroll
-                for (int j = 0; j < 4; ++j) x[j] = xk[j];
+                load_state(a.st.Xk, s, x);
                 SmallSolve<FamHenonHeilesMidpoint, kStatic, false, false, true> fb(fam, a.qf);
                 const auto Rf = fb.solve(x);
-                atomicAdd(a.fallbacks, 1u);
+                atomicAdd(a.counters, 1u);
                 store_status_learned(a.S + s, Rf.code, Rf.iterations, Rf.fnorm, Rf.stepnorm, Rf.failures, Rf.promoted,
                                      Rf.gradratio, Rf.rate, Rf.contracted);
             }
             xp[0] = double2{x[0], x[1]};
             xp[1] = double2{x[2], x[3]};
         }
-        wave_lds_fence();
+    }
+}
+
+template <bool MIXED, bool REFINE>
+__global__ __launch_bounds__(64 * kWsWaves) void learned_solve_kernel(const LearnedArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    int* full = reinterpret_cast<int*>(lds + kWsFlagOff);
+    int* done = full + kWsSlots;
+    if (threadIdx.x < 2 * kWsSlots) full[threadIdx.x] = 0;  // full and done (mlp_lds_setup synchronises)
+    MlpLds L = mlp_lds_setup(lds, a.st.packed, a.st.params);
+    const int wave = threadIdx.x >> 6;
+    double* slots = reinterpret_cast<double*>(lds + kWsSlotOff);
+    const int64_t nchunk = (a.st.nsys + kChunk - 1) / kChunk;
+    const int64_t ngroup = (nchunk + kWsCons - 1) / kWsCons;
+    const int64_t nround = int64_t(blockIdx.x) < ngroup ? (ngroup - 1 - blockIdx.x) / gridDim.x + 1 : 0;
+    if (wave < kWsProd) {
+        L.corr = reinterpret_cast<float*>(lds + kWsCorr + size_t(wave) * kLdsCorr);
+        learned_produce(a, L, wave, nround, nchunk, slots, done, full);
+    } else {
+        learned_consume<MIXED, REFINE>(a, wave - kWsProd, nround, slots, full, done);
     }
 }
 
@@ -667,7 +768,7 @@ int gs_solve_learned_henon_heiles_f64(const gs_mlp* net, const gs_method* method
     static bool attr_kern[64][4] = {};
     const int ki = v * 2 + (method->refinement == GS_REFINEMENT ? 1 : 0);
     if (!attr_kern[dev][ki]) {
-        int rc = check(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(kStartLds)),
+        int rc = check(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(kWsLds)),
                        "hipFuncSetAttribute");
         if (rc) return rc;
         attr_kern[dev][ki] = true;
@@ -677,7 +778,7 @@ int gs_solve_learned_henon_heiles_f64(const gs_mlp* net, const gs_method* method
     int rc = mlp_workspace(256, &ws, true);
     if (rc) return rc;
     uint32_t* counter = static_cast<uint32_t*>(ws);
-    hipLaunchKernelGGL(fill_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<float*>(counter), int64_t(1), 0.0f);
+    hipLaunchKernelGGL(fill_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<float*>(counter), int64_t(2), 0.0f);
     LearnedArgs a;
     std::memset(&a, 0, sizeof(a));
     a.st.Xk = Xk;
@@ -697,27 +798,28 @@ int gs_solve_learned_henon_heiles_f64(const gs_mlp* net, const gs_method* method
     plain.static_alpha = 1.0;
     a.qf = dev_params(plain, *options);
     a.S = S;
-    a.fallbacks = counter;
+    a.counters = counter;
     const int64_t nchunk = (nsys + kChunk - 1) / kChunk;
-    int64_t blocks = (nchunk + kStartWaves - 1) / kStartWaves;
+    int64_t blocks = (nchunk + kWsCons - 1) / kWsCons;
     if (blocks > cus) blocks = cus;
-    const dim3 grid{unsigned(blocks), 1, 1}, block{64 * kStartWaves, 1, 1};
+    const dim3 grid{unsigned(blocks), 1, 1}, block{64 * kWsWaves, 1, 1};
     const bool refine = method->refinement == GS_REFINEMENT;
     bool timed = timing_begin(st);
     if (v) {
-        if (refine) hipLaunchKernelGGL((learned_solve_kernel<true, true>), grid, block, kStartLds, st, a);
-        else hipLaunchKernelGGL((learned_solve_kernel<true, false>), grid, block, kStartLds, st, a);
+        if (refine) hipLaunchKernelGGL((learned_solve_kernel<true, true>), grid, block, kWsLds, st, a);
+        else hipLaunchKernelGGL((learned_solve_kernel<true, false>), grid, block, kWsLds, st, a);
     } else {
-        if (refine) hipLaunchKernelGGL((learned_solve_kernel<false, true>), grid, block, kStartLds, st, a);
-        else hipLaunchKernelGGL((learned_solve_kernel<false, false>), grid, block, kStartLds, st, a);
+        if (refine) hipLaunchKernelGGL((learned_solve_kernel<false, true>), grid, block, kWsLds, st, a);
+        else hipLaunchKernelGGL((learned_solve_kernel<false, false>), grid, block, kWsLds, st, a);
     }
     timed = timing_mark(st, timed);
     if ((rc = check(hipGetLastError(), "fused learned solve launch failed"))) return rc;
-    uint32_t host = 0;
-    if ((rc = check(hipMemcpyAsync(&host, counter, 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync"))) return rc;
+    uint32_t host[2] = {0, 0};
+    if ((rc = check(hipMemcpyAsync(host, counter, 8, hipMemcpyDeviceToHost, st), "hipMemcpyAsync"))) return rc;
     if ((rc = check(hipStreamSynchronize(st), "hipStreamSynchronize"))) return rc;
     timing_end(timed);
-    if (fallbacks) *fallbacks = int64_t(host);
+    if (host[1] != 0) return set_error(GS_EHIP, "the fused learned solve's hand-over between its waves timed out");
+    if (fallbacks) *fallbacks = int64_t(host[0]);
     return GS_OK;
 }
 

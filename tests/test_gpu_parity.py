@@ -64,8 +64,14 @@ def assert_parity(st, x, so, xo, *, bitwise=False, near_tol=None, tol=None):
     assert (rel[mask] <= 1e-10).all(), f"iterates differ by {rel[mask].max():.3e}"
     if bitwise:
         assert np.array_equal(x.view(np.uint64), xo.view(np.uint64)), "iterates differ in bits"
-        assert np.array_equal(st.view(np.uint8).reshape(len(st), 72)[:, :66],
-                              so.view(np.uint8).reshape(len(so), 72)[:, :66]), "statuses differ in bits"
+        a = st.view(np.uint8).reshape(len(st), 72)[:, :66]
+        b = so.view(np.uint8).reshape(len(so), 72)[:, :66]
+        if not np.array_equal(a, b):
+            cols = np.nonzero((a != b).any(axis=0))[0]
+            names = sorted({n for n, (dt, off) in st.dtype.fields.items() for c in cols if off <= c < off + dt.itemsize})
+            rows = np.nonzero((a != b).any(axis=1))[0]
+            raise AssertionError(f"statuses differ in bits: bytes {cols.tolist()} (fields {names}) at {rows[:10]} "
+                                 f"({rows.size} systems); device {a[rows[0]].tolist()} oracle {b[rows[0]].tolist()}")
 
 
 def same_bytes(a, b):
