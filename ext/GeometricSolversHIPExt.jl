@@ -27,6 +27,7 @@ using GeometricSolvers: Options, SolverStatus, ReturnCode, L2Norm, MaxNorm, Stat
                         _validate_batch_problem, _validate_workgroupsize, resolve_options,
                         _tally_return_codes, ToReal
 using StaticArrays: SVector, SMatrix, @SMatrix
+using LinearAlgebra: Diagonal
 import GeometricSolvers: solve_batch!
 
 const libgsolve = get(ENV, "GSOLVE_HIP_LIB", "libgsolve_hip")
@@ -204,23 +205,52 @@ function fpu_gl3_jacobian(f::FpuGl3, z::SVector{48, T}, p) where {T}
     end)
 end
 
+"""
+    DenseSin(A)
+
+The dense device problem of the reference's benchmarks (`TenEResidual`,
+`benchmark/device_problems.jl:147-158`): `F(x, b) = A x + sin.(x)/10 − b`, the system's own `b` as
+its parameter (a `PerSystem` of `SVector{n, Float64}`), `J = A + Diagonal(cos.(x)/10)`. `A` (n × n,
+served on the device for n = 16, 24, 32) is kept as an `SMatrix` for the generic path and as a
+column-major device matrix, which the library reads through `gs_family.data`.
+"""
+struct DenseSin{N, L} <: HIPFamily
+    A::SMatrix{N, N, Float64, L}
+    Ad::ROCArray{Float64, 2}
+end
+function DenseSin(A::AbstractMatrix)
+    n = size(A, 1)
+    size(A, 2) == n || throw(DimensionMismatch("A must be square"))
+    S = SMatrix{n, n, Float64}(A)
+    return DenseSin(S, ROCArray(Matrix(S)))
+end
+(f::DenseSin{N})(x::SVector{N, T}, b) where {N, T} = f.A * x .+ sin.(x) ./ 10 .- b
+dense_sin_jacobian(f::DenseSin{N}, x::SVector{N, T}, _) where {N, T} = f.A + Diagonal(cos.(x) ./ 10)
+
 """The `NonlinearProblem` of a registered family, with its analytic Jacobian."""
 family_problem(f::HenonHeilesMidpoint) = NonlinearProblem(f, hh_midpoint_jacobian)
 family_problem(f::TanhLinear3) = NonlinearProblem(f, tanh_linear3_jacobian)
 family_problem(f::Coupled2) = NonlinearProblem(f, coupled2_jacobian)
 family_problem(f::ScalarSquare) = NonlinearProblem(f, scalar_square_jacobian)
 family_problem(f::FpuGl3) = NonlinearProblem(f, (z, p) -> fpu_gl3_jacobian(f, z, p))
+family_problem(f::DenseSin) = NonlinearProblem(f, (x, b) -> dense_sin_jacobian(f, x, b))
 
 family_id(::HenonHeilesMidpoint) = Int32(5)
 family_id(::TanhLinear3) = Int32(4)
 family_id(::Coupled2) = Int32(1)
 family_id(::ScalarSquare) = Int32(3)
 family_id(::FpuGl3) = Int32(6)
+family_id(::DenseSin) = Int32(10)
 family_n(::HenonHeilesMidpoint) = 4
 family_n(::TanhLinear3) = 3
 family_n(::Coupled2) = 2
 family_n(::ScalarSquare) = 1
 family_n(::FpuGl3) = 48
+family_n(::DenseSin{N}) where {N} = N
+
+# The family's own device data (`gs_family.data`): the matrix of DenseSin, none for the others.
+family_data(::HIPFamily) = C_NULL
+family_data(f::DenseSin) = pointer(f.Ad)
 
 # The parameter records of a batch in the working precision R: (device pointer, byte stride, shared
 # scalars, roots to keep). A record is the isbits element of the device array (its size is the
@@ -246,6 +276,10 @@ function family_records(::ScalarSquare, p::PerSystem, ::Type{R}) where {R}
     d = p.data::ROCVector{R}
     return pointer(d), Int64(sizeof(R)), (), d
 end
+function family_records(::DenseSin{N}, p::PerSystem, ::Type{Float64}) where {N}
+    d = p.data::ROCVector{SVector{N, Float64}}
+    return pointer(d), Int64(N * sizeof(Float64)), (), d
+end
 function family_records(f::FpuGl3, p::BatchParams, ::Type{Float64})
     QP = p.per_system::ROCVector{SVector{32, Float64}}
     return pointer(QP), Int64(256), (Float64(p.shared), f.β), QP
@@ -255,6 +289,9 @@ end
 # globalization without `reuse`, in Float64 only; anything else falls through to the generic path.
 served(::HIPFamily, cm, ::Type{R}) where {R} = true
 served(::FpuGl3, cm, ::Type{R}) where {R} = R === Float64 && cm.globalization == 0 && cm.reuse == 0
+# the dense kernel: n = 16, 24, 32 in Float64, Static or Backtracking, no `reuse`
+served(::DenseSin{N}, cm, ::Type{R}) where {N, R} = R === Float64 && N in (16, 24, 32) &&
+    cm.globalization in (0, 1) && cm.reuse == 0
 
 # ---- the method and options as ABI structs ----------------------------------------------------
 
@@ -317,13 +354,13 @@ end
 function hip_launch!(X, S, codes, fam::HIPFamily, cm::GsMethod, p, opt::Options{R}, order, ws) where {R}
     P, stride, shared, root = family_records(fam, p, R)
     cf = GsFamily(family_id(fam), Int32(family_n(fam)), Int32(family_n(fam)), Int32(0),
-        ntuple(k -> k <= length(shared) ? shared[k] : 0.0, 8))
+        ntuple(k -> k <= length(shared) ? shared[k] : 0.0, 8), family_data(fam))
     co = device_options(opt)
     order_dev = order === nothing ? nothing : ROCArray(Int64.(collect(order)))
     Sp = S === nothing ? C_NULL : pointer(S)
     Cp = codes === nothing ? C_NULL : pointer(codes)
     Op = order_dev === nothing ? C_NULL : pointer(order_dev)
-    GC.@preserve root order_dev begin
+    GC.@preserve root order_dev fam begin
         rc = if R === Float32
             # the Float32 twin (SolverStatus{Float32}, Float32 records and iterates)
             ccall((:gs_solve_batch_f32, libgsolve), Cint,
@@ -416,7 +453,7 @@ function device_deflation(def::Deflated, n::Int)
         ntuple(k -> k <= length(flat) ? flat[k] : 0.0, 32))
 end
 
-served_deflated(fam::HIPFamily, cm, ::Type{T}) where {T} = !(fam isa FpuGl3) && T === Float64 &&
+served_deflated(fam::HIPFamily, cm, ::Type{T}) where {T} = !(fam isa Union{FpuGl3, DenseSin}) && T === Float64 &&
     cm !== nothing && cm.globalization in (0, 1, 3) && cm.reuse == 0 && cm.best == 0
 
 function hip_launch_deflated!(X, S, codes, def::Deflated, cd::GsDeflation, cm::GsMethod, p, opt, order, ws)
@@ -510,7 +547,7 @@ device_chart(::PolarChart) = GsChart(Int32(2), Int32(0), ntuple(_ -> 0.0, 16), n
 
 const HIPReparam = Reparametrized{<:NonlinearProblem{<:HIPFamily}, <:Union{LinearChart, PolarChart}}
 
-served_reparam(rep, cm) = !rep.covariant && !(rep.prob.F isa FpuGl3) && cm !== nothing &&
+served_reparam(rep, cm) = !rep.covariant && !(rep.prob.F isa Union{FpuGl3, DenseSin}) && cm !== nothing &&
     cm.globalization in (0, 1) && cm.reuse == 0 && (!(rep.g isa PolarChart) || family_n(rep.prob.F) == 2)
 
 function hip_launch_reparam!(Y, S, codes, rep, cm::GsMethod, p, opt, order, ws)
@@ -758,6 +795,206 @@ function solve_batch!(X::ROCVector{SVector{8, Float64}}, prob::LeastSquaresProbl
     S = ROCArray{SolverStatus{Float64}}(undef, length(X))
     solve_batch!(X, S, prob, method, data; kwargs...)
     return _tally_return_codes([s.code for s in Array(S)])
+end
+
+# ---- the learned start (configuration 5; no counterpart in the reference) ---------------------
+
+const MLP_IN, MLP_HID, MLP_OUT = 32, 128, 16
+const MLP_NPARAM = 39312
+const MLP_NPACKED = 38912
+
+# gs_mlp of include/gsolve.h
+struct GsMlp
+    params::Ptr{Float32}
+    packed::Ptr{UInt16}
+    in_shift::NTuple{8, Float32}
+    in_scale::NTuple{8, Float32}
+    out_shift::NTuple{4, Float32}
+    out_scale::NTuple{4, Float32}
+end
+
+"""
+    LearnedStart(; seed = 0)
+
+A multilayer perceptron (3 × 128 ReLU layers, BF16 operands on MFMA) that predicts the Newton start
+of the `HenonHeilesMidpoint` step as a correction to the explicit midpoint step, trained on the GPU
+by `train!` on solutions the method generates, and used by `solve_batch_learned!`. The parameters
+(FP32), their packed BF16 copy, the Adam moments and the normalisation live on the device.
+"""
+mutable struct LearnedStart
+    params::ROCVector{Float32}
+    packed::ROCVector{UInt16}
+    m::ROCVector{Float32}
+    v::ROCVector{Float32}
+    grad::ROCVector{Float32}
+    loss::ROCVector{Float32}
+    step::Int
+    in_shift::NTuple{8, Float32}
+    in_scale::NTuple{8, Float32}
+    out_shift::NTuple{4, Float32}
+    out_scale::NTuple{4, Float32}
+end
+
+# A small generator of its own (splitmix64; the package does not depend on Random): uniform draws
+# in [0, 1), normal draws by Box–Muller, and Fisher–Yates permutations.
+mutable struct SplitMix
+    s::UInt64
+end
+function next!(g::SplitMix)
+    g.s += 0x9e3779b97f4a7c15
+    z = g.s
+    z = (z ⊻ (z >> 30)) * 0xbf58476d1ce4e5b9
+    z = (z ⊻ (z >> 27)) * 0x94d049bb133111eb
+    return z ⊻ (z >> 31)
+end
+uniform!(g::SplitMix) = Float64(next!(g) >> 11) * 0x1.0p-53
+randn!(g::SplitMix) = sqrt(-2 * log(1 - uniform!(g))) * cos(2π * uniform!(g))
+function randperm!(g::SplitMix, n)
+    v = collect(1:n)
+    for i in n:-1:2
+        j = Int(next!(g) % UInt64(i)) + 1
+        v[i], v[j] = v[j], v[i]
+    end
+    return v
+end
+
+# He-normal hidden weights (fan-in 5 for the first layer, whose other inputs are padding), a small
+# output layer, zero biases; each W row-major, inputs × outputs, in the order W1 b1 W2 b2 W3 b3 W4 b4.
+function mlp_initial_parameters(seed)
+    rng = SplitMix(UInt64(seed))
+    P = zeros(Float32, MLP_NPARAM)
+    w2 = MLP_IN * MLP_HID + MLP_HID
+    w3 = w2 + MLP_HID * MLP_HID + MLP_HID
+    w4 = w3 + MLP_HID * MLP_HID + MLP_HID
+    for i in 0:4, j in 0:(MLP_HID - 1)
+        P[i * MLP_HID + j + 1] = Float32(randn!(rng) * sqrt(2 / 5))
+    end
+    for off in (w2, w3), i in 0:(MLP_HID - 1), j in 0:(MLP_HID - 1)
+        P[off + i * MLP_HID + j + 1] = Float32(randn!(rng) * sqrt(2 / MLP_HID))
+    end
+    for i in 0:(MLP_HID - 1), j in 0:3
+        P[w4 + i * MLP_OUT + j + 1] = Float32(randn!(rng) * sqrt(1 / MLP_HID) * 0.1)
+    end
+    return P
+end
+
+function mlp_check(rc)
+    rc == 0 || gs_error(rc)
+    return nothing
+end
+
+function LearnedStart(; seed::Integer = 0)
+    net = LearnedStart(ROCArray(mlp_initial_parameters(seed)), ROCArray(zeros(UInt16, MLP_NPACKED)),
+        ROCArray(zeros(Float32, MLP_NPARAM)), ROCArray(zeros(Float32, MLP_NPARAM)),
+        ROCArray(zeros(Float32, MLP_NPARAM)), ROCArray(zeros(Float32, 1)), 0,
+        ntuple(_ -> 0.0f0, 8), ntuple(_ -> 1.0f0, 8), ntuple(_ -> 0.0f0, 4), ntuple(_ -> 1.0f0, 4))
+    mlp_check(ccall((:gs_mlp_pack, libgsolve), Cint, (Ptr{Float32}, Ptr{UInt16}, Ptr{Cvoid}),
+        pointer(net.params), pointer(net.packed), stream().stream))
+    return net
+end
+
+gs_mlp(net::LearnedStart) = GsMlp(pointer(net.params), pointer(net.packed), net.in_shift, net.in_scale,
+    net.out_shift, net.out_scale)
+
+hh_field(x) = SVector(x[3], x[4], -x[1] - 2 * x[1] * x[2], (-x[2] - x[1] * x[1]) + x[2] * x[2])
+
+"""
+    fit_normalization!(net, Xk, X1, h)
+
+The features' and targets' shifts and scales (means and standard deviations) from training pairs:
+states `Xk` and the method's solutions `X1` of their midpoint steps (device vectors of
+`SVector{4, Float64}`).
+"""
+function fit_normalization!(net::LearnedStart, Xk::ROCVector{SVector{4, Float64}},
+        X1::ROCVector{SVector{4, Float64}}, h::Real)
+    xk, x1 = Array(Xk), Array(X1)
+    corr = map(xk, x1) do x, y
+        xt = x + h * hh_field(x + (h / 2) * hh_field(x))
+        (y - xt) / (h * h * h)
+    end
+    msd(v, j) = (μ = sum(e[j] for e in v) / length(v); σ = sqrt(sum((e[j] - μ)^2 for e in v) / length(v)); (μ, σ))
+    s_in = [msd(xk, j) for j in 1:4]
+    s_out = [msd(corr, j) for j in 1:4]
+    net.in_shift = ntuple(j -> j <= 4 ? Float32(s_in[j][1]) : (j == 5 ? Float32(h) : 0.0f0), 8)
+    net.in_scale = ntuple(j -> j <= 4 ? (s_in[j][2] > 0 ? Float32(1 / s_in[j][2]) : 1.0f0) : (j == 5 ? 1.0f0 : 0.0f0), 8)
+    net.out_shift = ntuple(j -> Float32(s_out[j][1]), 4)
+    net.out_scale = ntuple(j -> s_out[j][2] > 0 ? Float32(s_out[j][2]) : 1.0f0, 4)
+    return net
+end
+
+"""
+    train!(net, Xk, X1, h; epochs = 8, batch = 4096, lr = 3e-3, lr_final = 3e-5, seed = 0)
+
+Minibatch Adam with a cosine learning-rate schedule over shuffled epochs, on the device's forward and
+backward kernels (`gs_mlp_grad`, `gs_mlp_adam`); returns the mean loss of each epoch.
+"""
+function train!(net::LearnedStart, Xk::ROCVector{SVector{4, Float64}}, X1::ROCVector{SVector{4, Float64}},
+        h::Real; epochs::Integer = 8, batch::Integer = 4096, lr::Real = 3e-3, lr_final::Real = 3e-5,
+        seed::Integer = 0)
+    n = length(Xk)
+    batch % 32 == 0 || throw(ArgumentError("batch must be a multiple of 32"))
+    steps = n ÷ batch
+    steps ≥ 1 || throw(ArgumentError("fewer training pairs than one batch"))
+    rng = SplitMix(UInt64(seed) + 0x5eed)
+    total = epochs * steps
+    history = Float64[]
+    t = 0
+    for _ in 1:epochs
+        idx = ROCArray(Int64.(randperm!(rng, n) .- 1))
+        acc = 0.0
+        for k in 0:(steps - 1)
+            GC.@preserve idx begin
+                mlp_check(ccall((:gs_mlp_grad, libgsolve), Cint,
+                    (Ref{GsMlp}, Ptr{Cvoid}, Ptr{Cvoid}, Float64, Ptr{Int64}, Int32, Ptr{Float32}, Ptr{Float32}, Ptr{Cvoid}),
+                    gs_mlp(net), pointer(Xk), pointer(X1), Float64(h), pointer(idx) + 8 * k * batch, Int32(batch),
+                    pointer(net.grad), pointer(net.loss), stream().stream))
+            end
+            acc += Float64(Array(net.loss)[1])
+            rate = lr_final + (lr - lr_final) * (1 + cos(π * t / max(total - 1, 1))) / 2
+            net.step += 1
+            mlp_check(ccall((:gs_mlp_adam, libgsolve), Cint,
+                (Ptr{Float32}, Ptr{UInt16}, Ptr{Float32}, Ptr{Float32}, Ptr{Float32}, Int64, Float32, Float32,
+                    Float32, Float32, Ptr{Cvoid}),
+                pointer(net.params), pointer(net.packed), pointer(net.grad), pointer(net.m), pointer(net.v),
+                Int64(net.step), Float32(rate), 0.9f0, 0.999f0, 1.0f-8, stream().stream))
+            t += 1
+        end
+        push!(history, acc / steps)
+    end
+    return history
+end
+
+"""
+    solve_batch_learned!(X, S, prob::NonlinearProblem{HenonHeilesMidpoint}, method, p::BatchParams, net;
+                         options = nothing, kwargs...)
+
+`solve_batch!` from the learned start: for each system the start x̂₊ of the network, `method` from
+x̂₊, and plain Newton from x_k for every system whose solve from x̂₊ does not return SUCCESS (that
+solve's status is then the system's), in one fused kernel. `method` is Newton with
+`LUFactorization()` or `LUFactorization(Float32)`, `Static`, with or without `Refinement`. Returns the
+number of systems that fell back.
+"""
+function solve_batch_learned!(X::ROCVector{SVector{4, Float64}}, S::ROCVector{SolverStatus{Float64}},
+        prob::NonlinearProblem{HenonHeilesMidpoint}, method::Newton, p::BatchParams, net::LearnedStart;
+        options::Union{Nothing, Options} = nothing, kwargs...)
+    cm = device_method(method)
+    (cm === nothing || cm.globalization != 0 || cm.reuse != 0) && throw(ArgumentError(
+        "the learned start takes Newton with LUFactorization, the Static globalization and no reuse"))
+    Xk = p.per_system::ROCVector{SVector{4, Float64}}
+    length(Xk) == length(X) || throw(DimensionMismatch(
+        "BatchParams length $(length(Xk)) does not match batch length $(length(X))"))
+    length(S) == length(X) || throw(DimensionMismatch(
+        "status vector length $(length(S)) does not match batch length $(length(X))"))
+    opt = resolve_options(Float64, options, kwargs)
+    isempty(X) && return 0
+    fallbacks = Ref{Int64}(0)
+    rc = ccall((:gs_solve_learned_henon_heiles_f64, libgsolve), Cint,
+        (Ref{GsMlp}, Ref{GsMethod}, Int64, Ptr{Cvoid}, Float64, Ptr{Cvoid}, Ptr{Cvoid}, Ref{GsOptionsF64},
+            Ref{Int64}, Ptr{Cvoid}),
+        gs_mlp(net), cm, length(X), pointer(Xk), Float64(p.shared), pointer(X), pointer(S), device_options(opt),
+        fallbacks, stream().stream)
+    rc == 0 || gs_error(rc)
+    return Int(fallbacks[])
 end
 
 end # module
