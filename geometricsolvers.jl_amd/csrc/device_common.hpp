@@ -90,11 +90,42 @@ template <int L, class T> GS_DEV T rnorm_scaled(const T (&a)[L]) {
     return ldexp2(tsqrt(s), e);
 }
 
+// The correctly rounded square root of a finite s ≥ 2^-767: the operations LLVM's AMDGPU backend
+// emits for an FP64 sqrt (a reciprocal square root refined by two Goldschmidt steps and two
+// residual corrections, every step a fused multiply-add) without its range scaling (by 2^256 below
+// 2^-767) and its select of s itself for ±0 and +∞, which are identities in that range. Same
+// operands, same operations: the bits of __builtin_sqrt(s), in 10 instructions instead of 21.
+GS_DEV double sqrt_inrange(double s) {
+    double r = __builtin_amdgcn_rsq(s);
+    double g = s * r;
+    double h = r * 0.5;
+    const double e = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, e, g);
+    double d = __builtin_fma(-g, g, s);
+    h = __builtin_fma(h, e, h);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, s);
+    return __builtin_fma(d, h, g);
+}
+constexpr double kSqrtInrangeMin = 0x1p-767;
+
 // rnorm (reductions.jl:41-57); the sum is in the element type (Float32 or Float64).
 template <int L, class T> GS_DEV T rnorm(const T (&a)[L]) {
     T s = norm2<L>(a);
+    if constexpr (sizeof(T) == 8) {
+        if (__builtin_expect(isfin(s) && s >= kSqrtInrangeMin, 1)) return sqrt_inrange(s);
+    }
     if (__builtin_expect(isfin(s) && s >= T(L) * Lim<T>::min, 1)) return tsqrt(s);
     return rnorm_scaled<L>(a);
+}
+
+// The largest |entry| of an array as a fold of IEEE maxNum, which passes over NaN entries (NaN only
+// when every entry is NaN).
+template <int L, class T> GS_DEV T max_abs(const T (&a)[L]) {
+    T m = tabs(a[0]);
+#pragma unroll
+    for (int i = 1; i < L; ++i) m = tfmax(m, tabs(a[i]));
+    return m;
 }
 
 // rmaxnorm (reductions.jl:67).
@@ -117,13 +148,10 @@ template <int L, class T> GS_DEV T rdot(const T (&a)[L], const T (&b)[L]) {
 // holds a non-finite entry (matrix_exponent, factors.jl:833-843, whose largest-abs-bits maximum makes
 // a NaN the largest entry). A float max over |a| orders the finite entries as their bits do.
 template <int L, class T> GS_DEV int max_exponent(const T (&a)[L], int emax) {
-    T m = tabs(a[0]);
-    bool nan_seen = isnan_(a[0]);
+    const T m = max_abs<L>(a);
+    bool nan_seen = false;
 #pragma unroll
-    for (int i = 1; i < L; ++i) {
-        m = tfmax(m, tabs(a[i]));
-        nan_seen |= isnan_(a[i]);
-    }
+    for (int i = 0; i < L; ++i) nan_seen |= isnan_(a[i]);
     if (nan_seen || !isfin(m) || m == T(0)) return 0;
     int e = exponent_of(m);
     return e < -emax ? -emax : (e > emax ? emax : e);
@@ -138,6 +166,14 @@ constexpr int headroom(int n) {
         v >>= 1;
     }
     return 2 + b;
+}
+
+// 2^e as a compile-time constant of type T.
+template <class T> constexpr T exp2i(int e) {
+    T v = T(1);
+    for (int i = 0; i < e; ++i) v *= T(2);
+    for (int i = 0; i > e; --i) v /= T(2);
+    return v;
 }
 
 // rhs_exponent (factors.jl:880-885) for a right-hand side of length L and factor exponent e.

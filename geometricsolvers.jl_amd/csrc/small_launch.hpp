@@ -25,22 +25,30 @@ template <class T> struct KernelArgsT {
     void* S;  // gs_status_f64 or gs_status_f32
     uint8_t* codes;
     const int64_t* order;
+    int32_t stage = 0;  // 1: statuses of whole waves go through LDS (set by launch_diag)
 };
 using KernelArgs = KernelArgsT<double>;
 
 // The status of one system in the byte layout of SolverStatus{Float64}, as nine 8-byte words.
-GS_DEV void store_status(gs_status_f64* dst, uint8_t code, int32_t iterations, double fnorm, double stepnorm,
-                         int32_t failures, bool promoted, double gradratio, double rate, bool contracted,
-                         bool reused);
+GS_DEV void pack_status(uint64_t (&w)[9], uint8_t code, int32_t iterations, double fnorm, double stepnorm,
+                        int32_t failures, bool promoted, double gradratio, double rate, bool contracted, bool reused) {
+    w[0] = uint64_t(code) | (uint64_t(uint32_t(iterations)) << 32);
+    w[1] = __builtin_bit_cast(uint64_t, fnorm);
+    w[2] = __builtin_bit_cast(uint64_t, stepnorm);
+    w[3] = uint64_t(uint32_t(failures)) | (uint64_t(promoted ? 1 : 0) << 32);
+    w[4] = uint64_t(uint32_t(-1));  // rank = -1: an LU factor reports no rank
+    w[5] = __builtin_bit_cast(uint64_t, dnan());  // nullnorm
+    w[6] = __builtin_bit_cast(uint64_t, gradratio);
+    w[7] = __builtin_bit_cast(uint64_t, rate);
+    w[8] = uint64_t(contracted ? 1 : 0) | (uint64_t(reused ? 1 : 0) << 8);
+}
 
 // The status of one system in the byte layout of SolverStatus{Float32} (44 bytes), as eleven 4-byte
 // words: code @0, f_calls @2, iterations @4, fnorm @8, stepnorm @12, step_failures @16, promoted @20,
 // stationary @21, jvp_calls @22, rank @24, nullnorm @28, gradratio @32, rate @36, contracted @40,
 // reused @41.
-GS_DEV void store_status(gs_status_f32* dst, uint8_t code, int32_t iterations, float fnorm, float stepnorm,
-                         int32_t failures, bool promoted, float gradratio, float rate, bool contracted,
-                         bool reused) {
-    uint32_t* w = reinterpret_cast<uint32_t*>(dst);
+GS_DEV void pack_status(uint32_t (&w)[11], uint8_t code, int32_t iterations, float fnorm, float stepnorm,
+                        int32_t failures, bool promoted, float gradratio, float rate, bool contracted, bool reused) {
     w[0] = uint32_t(code);
     w[1] = uint32_t(iterations);
     w[2] = __builtin_bit_cast(uint32_t, fnorm);
@@ -54,22 +62,57 @@ GS_DEV void store_status(gs_status_f32* dst, uint8_t code, int32_t iterations, f
     w[10] = uint32_t(contracted ? 1 : 0) | (uint32_t(reused ? 1 : 0) << 8);
 }
 
+// The status words of one working precision: 9 × 8 bytes (Float64) or 11 × 4 bytes (Float32).
+template <class T> struct StatusWords {
+    using W = uint64_t;
+    using Status = gs_status_f64;
+    static constexpr int kCount = 9;
+};
+template <> struct StatusWords<float> {
+    using W = uint32_t;
+    using Status = gs_status_f32;
+    static constexpr int kCount = 11;
+};
+
+template <class W, int NW> GS_DEV void store_status_lane(void* dst, const W (&w)[NW]) {
+    W* d = static_cast<W*>(dst);
+#pragma unroll
+    for (int k = 0; k < NW; ++k) d[k] = w[k];
+}
+
 GS_DEV void store_status(gs_status_f64* dst, uint8_t code, int32_t iterations, double fnorm, double stepnorm,
                          int32_t failures, bool promoted, double gradratio, double rate, bool contracted,
                          bool reused) {
-    uint64_t* w = reinterpret_cast<uint64_t*>(dst);
-    uint64_t w0 = uint64_t(code) | (uint64_t(uint32_t(iterations)) << 32);
-    uint64_t w3 = uint64_t(uint32_t(failures)) | (uint64_t(promoted ? 1 : 0) << 32);
-    uint64_t w4 = uint64_t(uint32_t(-1));  // rank = -1: an LU factor reports no rank
-    w[0] = w0;
-    w[1] = __builtin_bit_cast(uint64_t, fnorm);
-    w[2] = __builtin_bit_cast(uint64_t, stepnorm);
-    w[3] = w3;
-    w[4] = w4;
-    w[5] = __builtin_bit_cast(uint64_t, dnan());  // nullnorm
-    w[6] = __builtin_bit_cast(uint64_t, gradratio);
-    w[7] = __builtin_bit_cast(uint64_t, rate);
-    w[8] = uint64_t(contracted ? 1 : 0) | (uint64_t(reused ? 1 : 0) << 8);
+    uint64_t w[9];
+    pack_status(w, code, iterations, fnorm, stepnorm, failures, promoted, gradratio, rate, contracted, reused);
+    store_status_lane(dst, w);
+}
+
+// LDS bytes per wave of the staged status store (64 statuses).
+template <class T> constexpr int status_stage_bytes() {
+    return 64 * StatusWords<T>::kCount * int(sizeof(typename StatusWords<T>::W));
+}
+
+// The statuses of a wave's 64 consecutive systems, written as one contiguous span of 64 × 72 (or 44)
+// bytes: each lane puts its words in the wave's LDS block, and the wave then stores the block in
+// 16-byte pieces of consecutive lanes, so that every store instruction covers whole cache lines
+// (a lane's own 72-byte record, stored word by word, makes each of the nine store instructions touch
+// all 36 lines of the span). `lds` and `dst` are 16-byte aligned.
+template <class W, int NW> GS_DEV void store_status_wave(W* lds, void* dst, const W (&w)[NW]) {
+    const int lane = int(threadIdx.x & 63u);
+#pragma unroll
+    for (int k = 0; k < NW; ++k) lds[lane * NW + k] = w[k];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    constexpr int kPieces = 64 * NW * int(sizeof(W)) / 16;
+    const uint4* src = reinterpret_cast<const uint4*>(lds);
+    uint4* out = static_cast<uint4*>(dst);
+#pragma unroll
+    for (int r = 0; r < (kPieces + 63) / 64; ++r) {
+        const int c = lane + 64 * r;
+        if (kPieces % 64 == 0 || c < kPieces) out[c] = src[c];
+    }
 }
 
 // The largest workgroup of the lane-per-system kernels (their launch bound) and their default.
@@ -91,6 +134,10 @@ void solve_small_kernel(const KernelArgsT<typename Fam::T> a) {
     using Status = typename std::conditional<sizeof(T) == 8, gs_status_f64, gs_status_f32>::type;
     constexpr int N = Fam::N;
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    // A wave of 64 consecutive systems in natural order writes its statuses through LDS
+    // (store_status_wave); the launch sizes the LDS for it exactly when the status buffer is 16-byte
+    // aligned, the batch has no `order` and the workgroup is whole waves (a.stage).
+    const bool staged = DIAG && a.stage && int64_t(blockIdx.x) * blockDim.x + (threadIdx.x & ~63u) + 64 <= a.nsys;
     if (i >= a.nsys) return;
     const int64_t idx = a.order ? a.order[i] - 1 : i;
     Fam fam;
@@ -104,8 +151,10 @@ void solve_small_kernel(const KernelArgsT<typename Fam::T> a) {
         auto R = solver.solve(x);
         fam.scatter(xp, x);
         if (DIAG) {
-            store_status(static_cast<Status*>(a.S) + idx, R.code, R.iterations, R.fnorm, R.stepnorm, R.failures,
-                         R.promoted, R.gradratio, R.rate, R.contracted, R.reused);
+            typename StatusWords<T>::W w[StatusWords<T>::kCount];
+            pack_status(w, R.code, R.iterations, R.fnorm, R.stepnorm, R.failures, R.promoted, R.gradratio, R.rate,
+                        R.contracted, R.reused);
+            store_status_lane(static_cast<Status*>(a.S) + idx, w);
         } else {
             a.codes[idx] = R.code;
         }
@@ -127,8 +176,17 @@ void solve_small_kernel(const KernelArgsT<typename Fam::T> a) {
         for (int k = 0; k < N; ++k) xp[k] = x[k];
     }
     if (DIAG) {
-        store_status(static_cast<Status*>(a.S) + idx, R.code, R.iterations, R.fnorm, R.stepnorm, R.failures, R.promoted,
-                     R.gradratio, R.rate, R.contracted, R.reused);
+        typename StatusWords<T>::W w[StatusWords<T>::kCount];
+        pack_status(w, R.code, R.iterations, R.fnorm, R.stepnorm, R.failures, R.promoted, R.gradratio, R.rate,
+                    R.contracted, R.reused);
+        if (staged) {
+            extern __shared__ __align__(16) unsigned char gs_status_stage[];
+            auto* lds = reinterpret_cast<typename StatusWords<T>::W*>(gs_status_stage +
+                                                                     (threadIdx.x >> 6) * status_stage_bytes<T>());
+            store_status_wave(lds, static_cast<Status*>(a.S) + (i - int64_t(threadIdx.x & 63u)), w);
+        } else {
+            store_status_lane(static_cast<Status*>(a.S) + idx, w);
+        }
     } else {
         a.codes[idx] = R.code;
     }
@@ -138,9 +196,14 @@ template <class Fam, int GLOB, bool MIXED, bool REFINE, bool REUSE>
 int launch_diag(const KernelArgsT<typename Fam::T>& a, bool diag, int wg, hipStream_t st) {
     const int64_t blocks = (a.nsys + wg - 1) / wg;
     if (blocks > 0x7fffffffLL) return set_error(GS_EINVAL, "batch too large for one launch");
-    if (diag)
+    if (diag) {
+        KernelArgsT<typename Fam::T> b = a;
+        b.stage = !holds_state<Fam>::value && a.order == nullptr && wg % 64 == 0 &&
+                  reinterpret_cast<uintptr_t>(a.S) % 16 == 0;
+        const size_t lds = b.stage ? size_t(wg / 64) * status_stage_bytes<typename Fam::T>() : 0;
         hipLaunchKernelGGL((solve_small_kernel<Fam, GLOB, MIXED, REFINE, true, REUSE>), dim3(unsigned(blocks)), dim3(wg),
-                           0, st, a);
+                           lds, st, b);
+    }
     else
         hipLaunchKernelGGL((solve_small_kernel<Fam, GLOB, MIXED, REFINE, false, REUSE>), dim3(unsigned(blocks)),
                            dim3(wg), 0, st, a);

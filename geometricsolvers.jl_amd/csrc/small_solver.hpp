@@ -148,8 +148,12 @@ template <class TF, int N, class T> GS_DEV void factorize(const T (&J)[N][N], Fa
     for (int i = 0; i < N; ++i)
 #pragma unroll
         for (int j = 0; j < N; ++j) flat[i * N + j] = J[i][j];
-    int e = max_exponent<N * N>(flat, Lim<T>::emax);
     constexpr bool same = sizeof(TF) == sizeof(T);
+    // A largest finite |entry| of at least 1 (below 2 for a narrower factor) gives e ≥ 0 (e = 0), and
+    // any NaN e = 0 as well, so no scaling: a wave whose lanes all have one skips the exponent scan.
+    const T m = max_abs<N * N>(flat);
+    const bool e0 = m >= T(1) && (same || m < T(2));
+    const int e = __all(e0) ? 0 : max_exponent<N * N>(flat, Lim<T>::emax);
     bool scale = e != 0 && (!same || e < 0);
     f.e = scale ? e : 0;
     // the prescale is exact, and a wave in which no lane scales skips it
@@ -170,6 +174,16 @@ template <class TF, int N, class T> GS_DEV void factorize(const T (&J)[N][N], Fa
 
 // ldiv!! in the working precision (factors.jl:931-936).
 template <int N, class T> GS_DEV void ldiv_same(const Factor<T, N>& f, const T (&r)[N], T (&d)[N]) {
+    // With f.e = 0 the right-hand side's exponent k is 0 unless its largest finite |entry| reaches
+    // 2^(emax − headroom + 1) (a NaN or ∞ entry gives 0 too): a wave whose lanes all have f.e = 0
+    // and smaller entries solves unscaled without the exponent scan.
+    constexpr T kbig = exp2i<T>(Lim<T>::emax - headroom(N) + 1);
+    if (__all(f.e == 0 && max_abs<N>(r) < kbig)) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) d[i] = r[i];
+        lu_backsolve<T, N>(f.A, f.piv, d);
+        return;
+    }
     const int k = rhs_exponent<N>(r, f.e);
     if (__any(k != 0)) {
 #pragma unroll

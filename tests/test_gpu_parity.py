@@ -372,3 +372,25 @@ def test_batch_summary(gs, workloads, device):
         assert bs.fnorm == pytest.approx(want, rel=1e-12)
     empty = gs.status_buffer(0, device)
     assert gs.batch_summary(empty) == gs.BatchSummary(0.0, True, 0, 0, 0)
+
+
+def test_status_store_paths_agree(gs, workloads, device):
+    """The statuses of whole waves in natural order are stored through LDS, those of a partial wave,
+    of a batch with `order` or of a status buffer that is not 16-byte aligned lane by lane: every
+    path writes the same bytes (the batch of 5000 ends in a partial wave)."""
+    n = 5000
+    xk = workloads.henon_heiles_states(n, seed=5)
+    opts = gs.Options(f_abstol=1e-12, f_reltol=0.0)
+    ref = run_gpu(gs, device, gs.Newton(), gs.HenonHeilesMidpoint(), xk.copy(), gs.BatchParams(0.1, xk), opts)
+    ident = list(range(1, n + 1))
+    got = run_gpu(gs, device, gs.Newton(), gs.HenonHeilesMidpoint(), xk.copy(), gs.BatchParams(0.1, xk), opts,
+                  order=ident)
+    assert np.array_equal(got[0], ref[0]) and same_bytes(got[1], ref[1])
+    # a status buffer one record (72 bytes) into an allocation: 8-byte but not 16-byte aligned
+    big = gs.status_buffer(n + 1, device)
+    S = big[1:]
+    assert S.data_ptr() % 16 == 8
+    X = torch.as_tensor(xk.copy(), device=device)
+    gs.solve_batch(X, S, gs.NonlinearProblem(gs.HenonHeilesMidpoint()), gs.Newton(),
+                   gs.BatchParams(0.1, torch.as_tensor(xk, device=device)), opts)
+    assert np.array_equal(X.cpu().numpy(), ref[0]) and same_bytes(gs.statuses(S), ref[1])
