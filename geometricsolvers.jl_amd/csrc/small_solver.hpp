@@ -464,10 +464,20 @@ struct SmallSolve {
     }
 
     // refine_passes!! (refinement.jl:211-244) with the product of `product<JVP>(J, ·)`.
+    // `rlast` receives ‖r‖ = ‖−F − J d‖ of the returned d when the refinement succeeds.
     template <class TFR, bool JVP = false>
     GS_DEV void refine(T (&d)[N], const Factor<TFR, N>& f, const T (&J)[N][N], const T (&Fx)[N], bool& failed) const {
+        T rlast;
+        refine<TFR, JVP>(d, f, J, Fx, failed, rlast);
+    }
+    template <class TFR, bool JVP = false>
+    GS_DEV void refine(T (&d)[N], const Factor<TFR, N>& f, const T (&J)[N][N], const T (&Fx)[N], bool& failed,
+                       T& rlast) const {
+        rlast = tnan<T>();
         failed = false;
-        T fnorm = rnorm<N>(Fx);
+        // Under Static without reuse the state's ‖F‖ is rnorm of this very residual (set from it at the
+        // start and by every step), so it is taken instead of folding F again.
+        T fnorm = (GLOB == kStatic && !REUSE) ? residual_norm(Fx) : rnorm<N>(Fx);
         if (fnorm == T(0.0)) return;
         if (!isfin(fnorm)) {
             failed = true;
@@ -482,6 +492,7 @@ struct SmallSolve {
             failed = true;
             return;
         }
+        rlast = rprev;
         if (rprev <= q.ref_eta_eff * fnorm) return;
         for (int it = 1; it <= q.ref_maxiter; ++it) {
             T dd[N], dc[N];
@@ -499,6 +510,7 @@ struct SmallSolve {
 #pragma unroll
             for (int i = 0; i < N; ++i) d[i] = dc[i];
             rprev = rcur;
+            rlast = rcur;
             if (rcur <= q.ref_eta_eff * fnorm) return;
         }
         failed = true;
@@ -579,13 +591,14 @@ struct SmallSolve {
             Factor<float, N> f;
             factorize<float, N>(J, f);
             bool ref_failed = false;
+            T rlast = tnan<T>();
             if (f.info > 0) {
                 ref_failed = true;
             } else {
                 ldiv<N>(f, Fx, d);
 #pragma unroll
                 for (int i = 0; i < N; ++i) d[i] = -d[i];
-                if (REFINE) refine<float>(d, f, J, Fx, ref_failed);
+                if (REFINE) refine<float>(d, f, J, Fx, ref_failed, rlast);
             }
             if (ref_failed) {
                 // promote_factor!! (newton.jl:1332-1339): refactorise the held Jacobian in FP64
@@ -601,12 +614,20 @@ struct SmallSolve {
                     for (int i = 0; i < N; ++i) d[i] = -d[i];
                 }
             } else if (fnorm_old != T(0.0) && isfin(fnorm_old)) {
-                // linear_residual_ratio (newton.jl:1422-1430)
-                T Jd[N], lin[N];
-                matvec<N>(J, d, Jd);
+                // linear_residual_ratio (newton.jl:1422-1430): ‖F + J d‖ / ‖F‖. A successful refinement
+                // has just formed r = −F − J d of this d and its norm, which has the same bits (−a + −b
+                // = −(a + b) exactly, and the norm of −v is that of v).
+                T lin_norm;
+                if (REFINE) {
+                    lin_norm = rlast;
+                } else {
+                    T Jd[N], lin[N];
+                    matvec<N>(J, d, Jd);
 #pragma unroll
-                for (int i = 0; i < N; ++i) lin[i] = T(1.0) * Fx[i] + T(1.0) * Jd[i];
-                lin_rel = rnorm<N>(lin) / fnorm_old;
+                    for (int i = 0; i < N; ++i) lin[i] = T(1.0) * Fx[i] + T(1.0) * Jd[i];
+                    lin_norm = rnorm<N>(lin);
+                }
+                lin_rel = lin_norm / fnorm_old;
             }
         }
         return ok;
