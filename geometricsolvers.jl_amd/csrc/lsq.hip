@@ -77,8 +77,9 @@ template <int N> struct Fit {
 // scale of the largest entry where the sum overflows or underflows.
 template <int N> GS_DEV double residual_norm(const ExpSumRow<N>& row, const double (&x)[N], double* maxabs) {
     double s = 0.0, mx = 0.0;
+    typename ExpSumRow<N>::Ahead ya(row, 0);
     for (int i = 0; i < row.m; ++i) {
-        double f = row.F(x, i);
+        double f = row.F(x, i, ya.take(i));
         s = i == 0 ? f * f : s + f * f;
         mx = i == 0 ? __builtin_fabs(f) : jmax(mx, __builtin_fabs(f));
     }
@@ -159,9 +160,10 @@ GS_DEV double grad_ratio_from(const ExpSumRow<N>& row, const double (&x)[N], con
 
 // Jᴴ F and the column sums of J∘J at x, folded over the rows.
 template <int N> GS_DEV void diag_rows(const ExpSumRow<N>& row, const double (&x)[N], double (&g)[N], double (&col)[N]) {
+    typename ExpSumRow<N>::Ahead ya(row, 0);
     for (int i = 0; i < row.m; ++i) {
         double Ji[N];
-        const double Fi = row.FJ(x, i, Ji);
+        const double Fi = row.FJ(x, i, ya.take(i), Ji);
         static_for<0, N>([&](auto kk) {
             constexpr int k = kk;
             g[k] = i == 0 ? Ji[k] * Fi : __builtin_fma(Ji[k], Fi, g[k]);
@@ -208,12 +210,14 @@ template <bool DIAG, int N>
 GS_DEV PassB pass_b(const ExpSumRow<N>& row, const double (&x)[N], const double (&d)[N], const double (&xp)[N],
                     double (&g)[N], double (&col)[N]) {
     PassB o{0.0, 0.0, 0.0, 0.0};
+    typename ExpSumRow<N>::Ahead ya(row, 0);
     for (int i = 0; i < row.m; ++i) {
         double Ji[N];
-        const double Fi = row.FJ(x, i, Ji);
+        const double yi = ya.take(i);
+        const double Fi = row.FJ(x, i, yi, Ji);
         double jd = Ji[0] * d[0];
         static_for<1, N>([&](auto kk) { jd = __builtin_fma(Ji[int(kk)], d[int(kk)], jd); });
-        const double fp = row.F(xp, i);
+        const double fp = row.F(xp, i, yi);
         const bool first = i == 0;
         o.rd = first ? Fi * jd : o.rd + Fi * jd;
         o.n2 = first ? jd * jd : o.n2 + jd * jd;
@@ -233,9 +237,10 @@ GS_DEV PassB pass_b(const ExpSumRow<N>& row, const double (&x)[N], const double 
 // ‖F + J δ‖ (rnorm with its scaled path), for a predicted reduction that rounds to ≤ 0.
 template <int N> GS_DEV double linear_norm(const ExpSumRow<N>& row, const double (&x)[N], const double (&d)[N]) {
     double s = 0.0, mx = 0.0;
+    typename ExpSumRow<N>::Ahead ya(row, 0);
     for (int i = 0; i < row.m; ++i) {
         double Ji[N];
-        const double Fi = row.FJ(x, i, Ji);
+        const double Fi = row.FJ(x, i, ya.take(i), Ji);
         double jd = Ji[0] * d[0];
         static_for<1, N>([&](auto kk) { jd = __builtin_fma(Ji[int(kk)], d[int(kk)], jd); });
         const double l = 1.0 * Fi + 1.0 * jd;
@@ -324,6 +329,22 @@ template <int N, bool SCALED> GS_DEV StepOut lm_step(Fit<N>& f, const LsqParams&
             static_for<j + 1, N>([&](auto ii) {
                 constexpr int i = ii;
                 const double b = R[i][j];
+                if (__all(givens_plain_ok(R[j][j], b))) {
+                    double cs, sn, r;
+                    givens_plain(R[j][j], b, cs, sn, r);
+                    R[j][j] = r;
+                    R[i][j] = 0.0;
+                    static_for<j + 1, N>([&](auto kk) {
+                        constexpr int k = kk;
+                        const double u = R[j][k], v = R[i][k];
+                        R[j][k] = cs * u + sn * v;
+                        R[i][k] = -sn * u + cs * v;
+                    });
+                    const double ur = c[j], vr = c[i];
+                    c[j] = cs * ur + sn * vr;
+                    c[i] = -sn * ur + cs * vr;
+                    return;
+                }
                 const bool skip = b == 0.0;
                 double cs, sn, r;
                 givens(R[j][j], b, cs, sn, r);
@@ -341,9 +362,11 @@ template <int N, bool SCALED> GS_DEV StepOut lm_step(Fit<N>& f, const LsqParams&
             });
         });
         // the remaining rows, streamed in order
+        typename ExpSumRow<N>::Ahead ya(f.row, N);
+#pragma unroll 1
         for (int i = N; i < f.row.m; ++i) {
             double w[N];
-            const double Fi = f.row.FJ(f.x, i, w);
+            const double Fi = f.row.FJ(f.x, i, ya.take(i), w);
             static_for<0, N>([&](auto kk) {
                 constexpr int k = kk;
                 jfinite &= isfin(w[k]);

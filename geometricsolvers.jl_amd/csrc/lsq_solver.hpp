@@ -62,6 +62,23 @@ GS_DEV void givens(double f, double g, double& c, double& s, double& r) {
     r = g == 0.0 ? f : (f == 0.0 ? g : rr);
 }
 
+// givens() where it needs none of its special handling: f ≠ 0, g ≠ 0 and max(|f|, |g|) in
+// [2^-383, 2^511]. There the rescaling factors are 1, neither zero case applies and f² + g² lies in
+// [2^-766, 2^1023], where sqrt_inrange has the bits of the square root, so the results are givens()'s
+// bit for bit, without its selects.
+GS_DEV bool givens_plain_ok(double f, double g) {
+    const double scale = jmax(__builtin_fabs(f), __builtin_fabs(g));  // NaN fails the range test
+    return f != 0.0 && g != 0.0 && scale >= 0x1p-383 && scale <= 0x1p511;
+}
+GS_DEV void givens_plain(double f, double g, double& c, double& s, double& r) {
+    const double rr = sqrt_inrange(f * f + g * g);
+    const double cc = f / rr, ss = g / rr;
+    const bool flip = __builtin_fabs(f) > __builtin_fabs(g) && cc < 0.0;
+    c = flip ? -cc : cc;
+    s = flip ? -ss : ss;
+    r = flip ? -rr : rr;
+}
+
 // The Taylor coefficients 1/j!, j = 13 … 2, rounded to double. Read from constant memory inside the
 // row loops, they stay in scalar registers (the FMA takes one scalar operand) instead of occupying
 // VGPRs for the life of the kernel.
@@ -100,14 +117,17 @@ template <int N> struct ExpSumRow {
     const double* y;
     int m;
     GS_DEV double t(int i) const { return (5.0 * double(i)) / double(m - 1); }
-    GS_DEV double F(const double (&x)[N], int i) const {
+    GS_DEV double F(const double (&x)[N], int i) const { return F(x, i, y[i]); }
+    GS_DEV double FJ(const double (&x)[N], int i, double (&Ji)[N]) const { return FJ(x, i, y[i], Ji); }
+    // the same with y_i given (read ahead by the caller's row loop)
+    GS_DEV double F(const double (&x)[N], int i, double yi) const {
         const double ti = t(i);
         double s = x[0] * expsum_exp(-x[K] * ti);
 #pragma unroll
         for (int k = 1; k < K; ++k) s = s + x[k] * expsum_exp(-x[K + k] * ti);
-        return s - y[i];
+        return s - yi;
     }
-    GS_DEV double FJ(const double (&x)[N], int i, double (&Ji)[N]) const {
+    GS_DEV double FJ(const double (&x)[N], int i, double yi, double (&Ji)[N]) const {
         const double ti = t(i);
         double s = 0.0;
 #pragma unroll
@@ -117,8 +137,21 @@ template <int N> struct ExpSumRow {
             Ji[k] = e;
             Ji[K + k] = ((-x[k]) * ti) * e;
         }
-        return s - y[i];
+        return s - yi;
     }
+    // y_i of a row loop from row i on: the load of the next row's value is issued one row ahead, so
+    // that its latency hides under the row's arithmetic (the index stays in 0 … m − 1)
+    struct Ahead {
+        const double* y;
+        int last;
+        double next;
+        GS_DEV Ahead(const ExpSumRow& r, int i0) : y(r.y), last(r.m - 1), next(r.y[i0 < r.m - 1 ? i0 : r.m - 1]) {}
+        GS_DEV double take(int i) {
+            const double v = next;
+            next = y[i + 1 < last ? i + 1 : last];
+            return v;
+        }
+    };
     // J_ik alone (column k of row i)
     GS_DEV double J(const double (&x)[N], int i, int k) const {
         const double ti = t(i);
@@ -145,8 +178,25 @@ template <int B, int E, class Fn> GS_DEV void static_for(Fn&& fn) {
 template <int K0, int N> GS_DEV void rotate_in(double (&R)[N][N], double (&c)[N], double (&w)[N], double& wr) {
     static_for<K0, N>([&](auto kk) {
         constexpr int k = kk;
-        // a zero entry is skipped (the rotation is computed and not applied), without a branch
         const double b = w[k];
+        if (__all(givens_plain_ok(R[k][k], b))) {
+            // the common case of every lane of the wave: no zero entry, no rescaling
+            double cs, sn, r;
+            givens_plain(R[k][k], b, cs, sn, r);
+            R[k][k] = r;
+            w[k] = 0.0;
+            static_for<k + 1, N>([&](auto ll) {
+                constexpr int l = ll;
+                const double u = R[k][l], v = w[l];
+                R[k][l] = cs * u + sn * v;
+                w[l] = -sn * u + cs * v;
+            });
+            const double ur = c[k];
+            c[k] = cs * ur + sn * wr;
+            wr = -sn * ur + cs * wr;
+            return;
+        }
+        // a zero entry is skipped (the rotation is computed and not applied), without a branch
         const bool skip = b == 0.0;
         double cs, sn, r;
         givens(R[k][k], b, cs, sn, r);
