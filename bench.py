@@ -457,9 +457,13 @@ def dense_flops_per_iter(n: int) -> int:
     return 2 * n ** 3 // 3 + 4 * n * n
 
 
-def bench_dense(args, gs, workloads, torch, n: int = 16, nsys: int = 1 << 20):
+def bench_dense(args, gs, workloads, torch):
     """The §10.E problem of the reference's device grid (A x + sin.(x)/10 − b, benchmark/device_problems.jl:
-    133-175) at n = 16 from x₀ = 0, plain FP64 Newton, status form: a wave segment per system."""
+    133-175) at n = args.dense_n (16 … 64) from x₀ = 0, plain FP64 Newton, status form: a wave segment
+    per system, its width (lanes per system) from the package's autotuner on this batch before the
+    clock. 2^20 systems up to n = 32, 2^18 above."""
+    n = args.dense_n
+    nsys = 1 << 20 if n <= 32 else 1 << 18
     dev = torch.device("cuda:0")
     lib = gs.load_library()
     A, B, _ = workloads.tene_batch(n, nsys)
@@ -469,6 +473,7 @@ def bench_dense(args, gs, workloads, torch, n: int = 16, nsys: int = 1 << 20):
     p = gs.PerSystem(Bd)
     opts = gs.Options(f_abstol=TOL, f_reltol=0.0)
     m = gs.Newton()
+    tuned = gs.tune_dense(torch.zeros((nsys, n), dtype=torch.float64, device=dev), prob, m, p, opts, repeats=1)
     Xs = [torch.zeros((nsys, n), dtype=torch.float64, device=dev) for _ in range(args.steps + args.warmup)]
     S = [gs.status_buffer(nsys, dev) for _ in range(args.steps)]
     Sw = gs.status_buffer(nsys, dev)
@@ -497,7 +502,7 @@ def bench_dense(args, gs, workloads, torch, n: int = 16, nsys: int = 1 << 20):
         from oracle import oracle as orc
 
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
-        k = 1 << 15
+        k = 1 << 15 if n <= 32 else 1 << 12
         reps, cel, (_, so) = _timed_oracle(
             lambda: orc.solve_batch(m, fam, np.zeros((k, n)), gs.PerSystem(B[:k]), opts, nthreads=threads), 8.0)
         cpu = {"value": int((so["code"] == 0).sum()) * reps / cel, "unit": "converged systems/s", "cores": threads,
@@ -518,13 +523,15 @@ def bench_dense(args, gs, workloads, torch, n: int = 16, nsys: int = 1 << 20):
         "data": "synthetic (the §10.E batch: A = 2I + R from seed n, x* ~ U[-1,1]^n from seed 1, numpy PCG64)",
         "config": {"workload": f"dense_sin (§10.E A x + sin(x)/10 - b) n={n}, batch {nsys}, x0 = 0, abstol 1e-12, "
                                "reltol 0, status form",
-                   "method": "Newton() + Static + static LU in FP64; a 16- or 32-lane wave segment per system",
+                   "method": "Newton() + Static + static LU in FP64; a wave segment of 16, 32 or 64 lanes per system",
+                   "lanes_per_system": tuned["lanes"],
+                   "tuner_seconds": {str(k): v for k, v in tuned["seconds"].items()},
                    "kernel_ms": kmean, "mean_iterations": iters / args.steps / nsys,
                    "reference_published_n16": "2.11e7 systems/s (RTX 4090, Float32, gs-newton; benchmark/REPORT.md), "
                                               "context only"},
         "roofline": {
             "bound": "fp64 valu",
-            "kernel": f"dense_kernel<{n}, Static, FP64, status>; achieved = (2/3 n^3 + 4 n^2) flop per Newton "
+            "kernel": f"dense_kernel<{n}, {tuned['lanes']} lanes, Static, FP64, status>; achieved = (2/3 n^3 + 4 n^2) flop per Newton "
                       "iteration x the batch's iterations / kernel time",
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS,
@@ -538,12 +545,13 @@ def bench_dense(args, gs, workloads, torch, n: int = 16, nsys: int = 1 << 20):
 
 def sub_results():
     """Compact lines of the other configurations (the n = 48 stage systems, the uneven least-squares
-    batch, the dense n = 16 family), each measured by its own `--workload` run in a child process (fewer
+    batch, the dense family at n = 16 and n = 64), each measured by its own `--workload` run in a child process (fewer
     steps, no CPU baseline), one after the other; a child that fails ends the series, and its exit status
     is recorded in place of its line."""
     out = {}
-    for name, w in (("fpu_gl3", "fpu_gl3"), ("expsum_lsq", "expsum_lsq"), ("dense_sin_n16", "dense_sin")):
-        cmd = [sys.executable, str(ROOT / "bench.py"), "--workload", w, "--steps", "3", "--warmup", "1",
+    for name, w in (("fpu_gl3", ["fpu_gl3"]), ("expsum_lsq", ["expsum_lsq"]),
+                    ("dense_sin_n16", ["dense_sin", "--dense-n", "16"]), ("dense_sin_n64", ["dense_sin", "--dense-n", "64"])):
+        cmd = [sys.executable, str(ROOT / "bench.py"), "--workload", *w, "--steps", "3", "--warmup", "1",
                "--no-cpu-baseline"]
         print(f"bench: sub-result {name}", file=sys.stderr, flush=True)
         try:
@@ -571,6 +579,8 @@ def main() -> int:
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--workload", choices=("henon_heiles", "fpu_gl3", "expsum_lsq", "dense_sin"),
                     default="henon_heiles")
+    ap.add_argument("--dense-n", type=int, choices=(16, 24, 32, 48, 64), default=16,
+                    help="the size of the dense_sin workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--save-net", default=None, help="save the trained network (.npz) of the default workload")
     ap.add_argument("--load-net", default=None, help="load a network saved by --save-net instead of training")

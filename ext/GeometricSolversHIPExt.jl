@@ -211,7 +211,7 @@ end
 The dense device problem of the reference's benchmarks (`TenEResidual`,
 `benchmark/device_problems.jl:147-158`): `F(x, b) = A x + sin.(x)/10 − b`, the system's own `b` as
 its parameter (a `PerSystem` of `SVector{n, Float64}`), `J = A + Diagonal(cos.(x)/10)`. `A` (n × n,
-served on the device for n = 16, 24, 32) is kept as an `SMatrix` for the generic path and as a
+served on the device for n = 16, 24, 32, 48, 64) is kept as an `SMatrix` for the generic path and as a
 column-major device matrix, which the library reads through `gs_family.data`.
 """
 struct DenseSin{N, L} <: HIPFamily
@@ -289,8 +289,8 @@ end
 # globalization without `reuse`, in Float64 only; anything else falls through to the generic path.
 served(::HIPFamily, cm, ::Type{R}) where {R} = true
 served(::FpuGl3, cm, ::Type{R}) where {R} = R === Float64 && cm.globalization == 0 && cm.reuse == 0
-# the dense kernel: n = 16, 24, 32 in Float64, Static or Backtracking, no `reuse`
-served(::DenseSin{N}, cm, ::Type{R}) where {N, R} = R === Float64 && N in (16, 24, 32) &&
+# the dense kernel: n = 16, 24, 32, 48, 64 in Float64, Static or Backtracking, no `reuse`
+served(::DenseSin{N}, cm, ::Type{R}) where {N, R} = R === Float64 && N in (16, 24, 32, 48, 64) &&
     cm.globalization in (0, 1) && cm.reuse == 0
 
 # ---- the method and options as ABI structs ----------------------------------------------------

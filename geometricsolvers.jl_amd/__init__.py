@@ -491,7 +491,7 @@ class QuadMap2(_SimpleFamily):
 
 class DenseSin(_SimpleFamily):
     """TenEResidual(A) of the reference's device benchmark (benchmark/device_problems.jl:133-175):
-    F(x, b) = A x + sin.(x) ./ 10 − b, n = 16, 24 or 32, with A (n × n) the family's own, shared by the
+    F(x, b) = A x + sin.(x) ./ 10 − b, n = 16, 24, 32, 48 or 64, with A (n × n) the family's own, shared by the
     batch, and p = PerSystem(B), one b per system. The Jacobian is A + Diagonal(cos.(x) ./ 10). Its sine
     and cosine are one fixed algorithm (Cody–Waite reduction, Taylor polynomials), the same on the
     device and in the oracle."""
@@ -500,8 +500,8 @@ class DenseSin(_SimpleFamily):
 
     def __init__(self, A):
         a = np.asarray(A.detach().cpu().numpy() if hasattr(A, "detach") else A, dtype=np.float64)
-        if a.ndim != 2 or a.shape[0] != a.shape[1] or a.shape[0] not in (16, 24, 32):
-            raise ArgumentError("DenseSin takes a square A of order 16, 24 or 32")
+        if a.ndim != 2 or a.shape[0] != a.shape[1] or a.shape[0] not in (16, 24, 32, 48, 64):
+            raise ArgumentError("DenseSin takes a square A of order 16, 24, 32, 48 or 64")
         self.A = a
         self.n = self.p_len = a.shape[0]
         self._dev = {}
@@ -1212,6 +1212,6 @@ def batch_summary(S) -> BatchSummary:
 
 
 from . import autotune  # noqa: E402  (the least-squares launch-shape autotuner)
-from .autotune import tune_lsq  # noqa: E402
+from .autotune import tune_lsq, tune_dense, tune_workgroup, dense_set_shape, dense_get_shape  # noqa: E402
 from . import learned  # noqa: E402  (the learned start; imports the names above lazily)
 from .learned import LearnedStart, generate_solutions, solve_batch_learned  # noqa: E402

@@ -224,6 +224,8 @@ EXPORTED_SYMBOLS = (
     "gs_lsq_set_shape",
     "gs_lsq_get_shape",
     "gs_lsq_set_graph",
+    "gs_dense_set_shape",
+    "gs_dense_get_shape",
     "gs_mlp_start_henon_heiles",
     "gs_solve_learned_henon_heiles_f64",
     "gs_mlp_grad",
@@ -297,6 +299,10 @@ def load_library(path: os.PathLike | None = None):
     lib.gs_lsq_set_shape.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
     lib.gs_lsq_get_shape.restype = ctypes.c_int
     lib.gs_lsq_get_shape.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_void_p]
+    lib.gs_dense_set_shape.restype = ctypes.c_int
+    lib.gs_dense_set_shape.argtypes = [ctypes.c_int, ctypes.c_int]
+    lib.gs_dense_get_shape.restype = ctypes.c_int
+    lib.gs_dense_get_shape.argtypes = [ctypes.c_int]
     lib.gs_lsq_set_graph.restype = ctypes.c_int
     lib.gs_lsq_set_graph.argtypes = [ctypes.c_int]
     lib.gs_shutdown.restype = ctypes.c_int

@@ -1,910 +1,83 @@
-// libgsolve_hip, dense mid-size systems (n = 16, 24, 32): a segment of a wave per system.
-//
-// The §10.E problem of the reference's device benchmark (benchmark/device_problems.jl:133-175),
-// F(x, b) = A x + sin.(x)/10 − b with A shared by the batch and b per system (GS_FAM_DENSE_SIN), under
-// Newton with the Static or Backtracking globalization, the static LU in FP64 or in FP32 with FP64
-// refinement and promotion: the reference's R1 loop solve!! (src/nonlinear/solver.jl:277-378) with
-// newton_step!! (src/nonlinear/newton.jl:1302-1485, 1604-1720) per system, as the lane-per-system
-// kernels run it for n ≤ 8.
-//
-// Layout. A system owns a segment of NP = 16 lanes (n = 16: four systems per wave) or NP = 32 lanes
-// (n = 24, 32: two per wave); lane i < n of the segment owns row i of the Jacobian and of its factor
-// (registers: n doubles, or n floats for the FP32 factor) and entry i of every vector (LDS, one
-// region per system). The scalar state of a solve is computed by every lane of its segment alike, so
-// control flow is uniform within a segment; segments of one wave diverge under EXEC like lanes.
-// Nothing crosses a segment: the pivot search is a segment reduction (DPP within the 16-lane rows and
-// a swizzle across the two rows of a 32-lane segment), the pivot row and the triangular solves'
-// finished entries pass through the system's LDS region, and the only workgroup barrier is the one
-// after staging A in LDS. LDS traffic within a segment needs no barrier (a wave's LDS operations
-// complete in order); a wave-scope fence keeps the compiler from moving a read across another
-// lane's write.
-//
-// The factorisation is the reference's unblocked LU in the factor's precision (lu.jl:24-64): the
-// first-largest pivot over the logical rows ≥ k (a NaN |a_kk| keeps row k), rows that never move
-// (each lane carries the logical index `pos` of its row; a swap exchanges two indices), exact a/p
-// multipliers and the rank-1 update without FMA, applied only to the rows below the pivot — so both
-// factor precisions match the oracle bit for bit. Sums the reference folds left to right (norms,
-// products, ‖J‖_F) are folded in that order by every lane over the LDS vector (broadcast reads).
-//
-// Diagnostics. The status's gradient ratio ‖Jᴴ F‖/(‖J‖_F ‖F‖) is that of the last factorisation
-// (newton.jl:1287-1311, 1165-1231). It changes nothing else in a solve of a NonlinearProblem, so the
-// kernel keeps the point of each factorisation (x, F and the ‖F‖ it read) and evaluates the ratio
-// once, after the loop, from the last one.
+// libgsolve_hip, dense mid-size systems (n = 16, 24, 32, 48, 64): the launch dispatch over the
+// sizes and their lanes per system (the kernels are in dense_kernel.hpp, instantiated one size per
+// translation unit in dense_n16.hip … dense_n64.hip), and the shape setting of the autotuner.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <string>
-#include <type_traits>
 
 #include "dense_solver.hpp"
-#include "small_launch.hpp"
 
 namespace gs {
 
 int set_error(int code, const std::string& msg);
 
+// The launches of one size, by lanes per system (dense_n16.hip … dense_n64.hip).
+int launch_dense_n16(int np, int glob, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st);
+int launch_dense_n24(int np, int glob, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st);
+int launch_dense_n32(int np, int glob, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st);
+int launch_dense_n48(int np, int glob, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st);
+int launch_dense_n64(int np, int glob, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st);
+
 namespace {
 
-// ---- the shape of a launch ------------------------------------------------------------------------
+// The default lanes per system at size n: the smallest segment that holds the rows.
+constexpr int default_np(int n) { return n <= 16 ? 16 : (n <= 32 ? 32 : 64); }
 
-template <int N> struct Shape {
-    static_assert(N == 16 || N == 24 || N == 32, "the dense solve takes n = 16, 24 or 32");
-    static constexpr int NP = N <= 16 ? 16 : 32;  // lanes per system
-    static constexpr int G = 64 / NP;             // systems per wave
-    static constexpr int kWaves = 4;
-    static constexpr int kThreads = 64 * kWaves;
-    static constexpr int SPB = G * kWaves;        // systems per workgroup
-};
+// The lanes per system of each size (0: the default), set by gs_dense_set_shape; index by size
+// 16, 24, 32, 48, 64.
+std::atomic<int> g_dense_np[5] = {0, 0, 0, 0, 0};
+int size_slot(int n) { return n == 16 ? 0 : n == 24 ? 1 : n == 32 ? 2 : n == 48 ? 3 : n == 64 ? 4 : -1; }
 
-// One system's LDS region: the vectors of the solve, the pivot-row and solve broadcast buffers.
-template <int N> struct alignas(16) SysVecs {
-    double x[N], Fx[N], d[N], xt[N], Ft[N], r[N], t[N], u[N];
-    double jx[N], jF[N];  // the point of the last factorisation and its residual (diagnostics)
-    double bx[N];         // the best iterate (best = true)
-    double rowd[N];       // the pivot row (FP64 factor), a finished solve entry, a column of J
-    float rowf[N];        // the same for the FP32 factor
-    double yd[N];
-    float yf[N];
-};
-
-// ---- segment primitives ---------------------------------------------------------------------------
-
-// The compiler's wave-scope fence: no LDS access moves across it (the hardware keeps a wave's LDS
-// accesses in order, so no instruction is needed).
-GS_DEV void wsync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// A value brought by DPP from another lane of the same row of 16.
-template <int CTRL> GS_DEV int dpp(int v) { return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xF, 0xF, false); }
-template <int CTRL> GS_DEV double dpp(double v) {
-    const uint64_t b = __builtin_bit_cast(uint64_t, v);
-    const uint32_t lo = uint32_t(dpp<CTRL>(int(uint32_t(b))));
-    const uint32_t hi = uint32_t(dpp<CTRL>(int(uint32_t(b >> 32))));
-    return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
-}
-// The value of lane l ^ 16 within a group of 32 (ds_swizzle, bit mode: and 0x1F, xor 0x10).
-GS_DEV int swz16(int v) { return __builtin_amdgcn_ds_swizzle(v, 0x401F); }
-GS_DEV double swz16(double v) {
-    const uint64_t b = __builtin_bit_cast(uint64_t, v);
-    const uint32_t lo = uint32_t(swz16(int(uint32_t(b))));
-    const uint32_t hi = uint32_t(swz16(int(uint32_t(b >> 32))));
-    return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
-}
-
-// The maximum of a NaN-free value over the segment, in every lane of it: quad swaps, the half-row
-// and row mirrors give each row of 16 its maximum; a 32-lane segment then takes its other row's.
-template <int NP> GS_DEV double seg_max(double v) {
-    v = __builtin_fmax(v, dpp<0xB1>(v));   // quad_perm [1,0,3,2]
-    v = __builtin_fmax(v, dpp<0x4E>(v));   // quad_perm [2,3,0,1]
-    v = __builtin_fmax(v, dpp<0x141>(v));  // row_half_mirror
-    v = __builtin_fmax(v, dpp<0x140>(v));  // row_mirror
-    if constexpr (NP == 32) v = __builtin_fmax(v, swz16(v));
-    return v;
-}
-
-// Whether any lane of the segment holds b.
-template <int NP> GS_DEV bool seg_any(bool b) {
-    const uint64_t m = __ballot(b);
-    const int base = int(threadIdx.x & 63) & ~(NP - 1);
-    const uint64_t seg = NP == 64 ? ~0ull : ((1ull << NP) - 1);
-    return ((m >> base) & seg) != 0;
-}
-
-// The smallest 32-bit integer over the segment, in every lane of it (the DPP move folds into the min).
-template <int CTRL> GS_DEV int dpp_min(int v) {
-    return __builtin_amdgcn_update_dpp(int(0x7fffffff), v, CTRL, 0xF, 0xF, false);
-}
-template <int NP> GS_DEV int seg_min(int v) {
-    v = __builtin_elementwise_min(v, dpp_min<0xB1>(v));
-    v = __builtin_elementwise_min(v, dpp_min<0x4E>(v));
-    v = __builtin_elementwise_min(v, dpp_min<0x141>(v));
-    v = __builtin_elementwise_min(v, dpp_min<0x140>(v));
-    if constexpr (NP == 32) v = __builtin_elementwise_min(v, swz16(v));
-    return v;
-}
-
-// The pivot choice of a segment: the largest key, and among the lanes holding it the smallest tag
-// (the tag orders by logical row first); every lane receives the winner's tag. Two reductions: the
-// maximum of the keys, then the minimum of the tags of the lanes that hold it.
-template <int NP> GS_DEV int seg_argmax(double key, int tag) {
-    const double kmax = seg_max<NP>(key);
-    return seg_min<NP>(key == kmax ? tag : int(0x7fffffff));
-}
-
-// ---- folds over an LDS vector, evaluated by every lane of the segment -----------------------------
-
-template <int N> GS_DEV double vnorm2(const double* v) {
-    double s = v[0] * v[0];
-#pragma unroll 8
-    for (int i = 1; i < N; ++i) s = s + v[i] * v[i];
-    return s;
-}
-template <int N> GS_DEV double vrmax(const double* v) {
-    double m = __builtin_fabs(v[0]);
-#pragma unroll 8
-    for (int i = 1; i < N; ++i) m = jmax(m, __builtin_fabs(v[i]));
-    return m;
-}
-// rnorm (reductions.jl:41-57)
-template <int N> GS_DEV double vrnorm(const double* v) {
-    const double s = vnorm2<N>(v);
-    if (__builtin_expect(isfin(s) && s >= double(N) * kDblMin, 1)) return __builtin_sqrt(s);
-    const double m = vrmax<N>(v);
-    if (m == 0.0 || !isfin(m)) return m;
-    const int e = exponent_of(m);
-    double t = ldexp2(v[0], -e);
-    double acc = t * t;
-#pragma unroll 8
-    for (int i = 1; i < N; ++i) {
-        t = ldexp2(v[i], -e);
-        acc = acc + t * t;
-    }
-    return ldexp2(__builtin_sqrt(acc), e);
-}
-// rdot (reductions.jl:77-85)
-template <int N> GS_DEV double vdot(const double* a, const double* b) {
-    double s = a[0] * b[0];
-#pragma unroll 8
-    for (int i = 1; i < N; ++i) s = s + a[i] * b[i];
-    return s;
-}
-
-// The exponent of the largest |entry| over the segment's entries `a` (lanes with `live`), clamped to
-// ±emax; 0 for a zero vector or one with a non-finite entry (max_exponent, factors.jl:833-843).
-template <int NP> GS_DEV int seg_exponent(double a, bool live) {
-    const double m0 = live ? __builtin_fabs(a) : 0.0;
-    const bool nan_any = seg_any<NP>(live && isnan_(m0));
-    const double m = seg_max<NP>(isnan_(m0) ? 0.0 : m0);
-    if (nan_any || !isfin(m) || m == 0.0) return 0;
-    const int e = exponent_of(m);
-    return e < -kEmax64 ? -kEmax64 : (e > kEmax64 ? kEmax64 : e);
-}
-
-// ---- the family: A x + sin.(x)/10 − b ---------------------------------------------------------------
-
-// sin and cos of the family: k = rint(x · 2/π); r = x − k·π/2 with π/2 in three parts (Cody–Waite,
-// each product folded by FMA); Taylor polynomials of sin r and cos r in r² through r¹⁷ and r¹⁶ by
-// Horner with FMA; the quadrant k mod 4. The coefficients (−1)^j/(2j+1)! and (−1)^j/(2j)! are
-// rounded once. NaN for a non-finite x or |x| ≥ 2^50; sin(±0) = ±0. The oracle evaluates the same steps, so the
-// residual and the Jacobian have the same bits on both sides.
-__constant__ double kSinC[8] = {-0x1.5555555555555p-3, 0x1.1111111111111p-7, -0x1.a01a01a01a01ap-13,
-                                0x1.71de3a556c734p-19, -0x1.ae64567f544e4p-26, 0x1.6124613a86d09p-33,
-                                -0x1.ae7f3e733b81fp-41, 0x1.952c77030ad4ap-49};
-__constant__ double kCosC[8] = {-0x1.0000000000000p-1, 0x1.5555555555555p-5, -0x1.6c16c16c16c17p-10,
-                                0x1.a01a01a01a01ap-16, -0x1.27e4fb7789f5cp-22, 0x1.1eed8eff8d898p-29,
-                                -0x1.93974a8c07c9dp-37, 0x1.ae7f3e733b81fp-45};
-
-GS_DEV void fsincos(double x, double& sn, double& cs) {
-    if (!(__builtin_fabs(x) < 0x1p50)) {
-        sn = cs = dnan();
-        return;
-    }
-    if (x == 0.0) {  // sin(±0) = ±0
-        sn = x;
-        cs = 1.0;
-        return;
-    }
-    const double k = __builtin_rint(x * 0x1.45f306dc9c883p-1);
-    double r = __builtin_fma(-k, 0x1.921fb54442d18p+0, x);
-    r = __builtin_fma(-k, 0x1.1a62633145c07p-54, r);
-    r = __builtin_fma(-k, -0x1.f1976b7ed8fbcp-110, r);
-    const double z = r * r;
-    double ps = kSinC[7], pc = kCosC[7];
-#pragma unroll
-    for (int j = 6; j >= 0; --j) {
-        ps = __builtin_fma(ps, z, kSinC[j]);
-        pc = __builtin_fma(pc, z, kCosC[j]);
-    }
-    const double s0 = __builtin_fma(r * z, ps, r);
-    const double c0 = __builtin_fma(z, pc, 1.0);
-    const int q = int(int64_t(k) & 3);
-    sn = q == 0 ? s0 : (q == 1 ? c0 : (q == 2 ? -s0 : -c0));
-    cs = q == 0 ? c0 : (q == 1 ? -s0 : (q == 2 ? -c0 : s0));
-}
-
-// TenEResidual (device_problems.jl:147-157): row i of F and of J = A + Diagonal(cos.(x)/10), the
-// matrix-vector product as StaticArrays' muladd fold over the row.
-template <int NN> struct FamDenseSin {
-    static constexpr int N = NN;
-    const double* A;  // LDS, column-major
-    double b;         // this lane's entry of the system's b
-    GS_DEV double F(const double* x, int i) const {
-        double acc = A[i] * x[0];
-#pragma unroll 8
-        for (int j = 1; j < N; ++j) acc = __builtin_fma(A[i + j * N], x[j], acc);
-        double sn, cs;
-        fsincos(x[i], sn, cs);
-        return (acc + sn / 10.0) - b;
-    }
-    // J_ij at x from the lane's own diagonal term dg = cos(x_i)/10 (Jdiag): column j in parallel.
-    GS_DEV double Jdiag(const double* x, int i) const {
-        double sn, cs;
-        fsincos(x[i], sn, cs);
-        return cs / 10.0;
-    }
-    GS_DEV double Jentry(int i, int j, double dg) const {
-        const double a = A[i + j * N];
-        return j == i ? a + dg : a;
-    }
-    GS_DEV void Jrow(const double* x, int i, double (&row)[N]) const {
-        double sn, cs;
-        fsincos(x[i], sn, cs);
-        const double c10 = cs / 10.0;
-#pragma unroll
-        for (int j = 0; j < N; ++j) {
-            const double a = A[i + j * N];
-            row[j] = j == i ? a + c10 : a;
-        }
-    }
-};
-
-// ---- the solve of one system --------------------------------------------------------------------
-
-template <class Fam, int GLOB, bool MIXED, bool REFINE, bool DIAG> struct DenseSolve {
-    static constexpr int N = Fam::N;
-    static constexpr int NP = Shape<N>::NP;
-    using Step = StepT<double>;
-
-    const Fam& fam;
-    const DevParams& q;
-    SysVecs<N>& v;
-    const int sl;      // the lane's row within the segment
-    const bool live;   // sl < N
-
-    // NewtonState (newton.jl:715-744): uniform within the segment
-    double merit, sfnorm;
-    bool promoted;
-    int slow;
-    double diag_fnorm;  // the ‖F‖ the last factorisation read; NaN before the first
-
-    GS_DEV DenseSolve(const Fam& f, const DevParams& p, SysVecs<N>& vv, int lane)
-        : fam(f), q(p), v(vv), sl(lane), live(lane < N) {}
-
-    // F at the LDS vector xin into out (every lane of the segment, then the fence).
-    GS_DEV void eval_F(const double* xin, double* out) {
-        if (live) out[sl] = fam.F(xin, sl);
-        wsync();
-    }
-
-    GS_DEV void copy(const double* from, double* to) {
-        if (live) to[sl] = from[sl];
-        wsync();
-    }
-
-    // residual_norm (newton.jl:1274-1278)
-    GS_DEV double residual_norm() const {
-        if (GLOB == kBacktracking) {
-            if (isfin(merit) && merit >= double(N) * kDblMin) return __builtin_sqrt(merit);
-            return vrnorm<N>(v.Fx);
-        }
-        return isnan_(sfnorm) ? vrnorm<N>(v.Fx) : sfnorm;
-    }
-
-    // ---- the Jacobian, its factor and the solves ----
-
-    // newton_factor (newton.jl:1302-1311) up to the factorisation: Jreg's row at x, the point kept
-    // for the diagnostics.
-    GS_DEV void jacobian(double (&jr)[N]) {
-        if (DIAG) {
-            diag_fnorm = residual_norm();
-            if (live) {
-                v.jx[sl] = v.x[sl];
-                v.jF[sl] = v.Fx[sl];
-            }
-        }
-        fam.Jrow(v.x, live ? sl : 0, jr);
-        if (q.mu != 0.0) {
-#pragma unroll
-            for (int j = 0; j < N; ++j) jr[j] = j == sl ? jr[j] + q.mu : jr[j];
-        }
-    }
-
-    template <class TF> GS_DEV TF* rowbuf() {
-        if constexpr (sizeof(TF) == 8) return v.rowd;
-        else return v.rowf;
-    }
-    template <class TF> GS_DEV TF* ybuf() {
-        if constexpr (sizeof(TF) == 8) return v.yd;
-        else return v.yf;
-    }
-
-    // factor_matrix (factors.jl:802-811) + factorize!! (lu.jl:24-64) on the segment's rows: a
-    // receives this lane's row of Jreg/2^e in TF, LU-factored; pos its logical row; returns info (the
-    // first zero pivot, 1-based, or 0); fe the prescale exponent (applied when ≠ 0 in FP32, only when
-    // < 0 in FP64).
-    template <class TF> GS_DEV int factor(const double (&jr)[N], TF (&a)[N], int& pos, int& fe) {
-        double mx = 0.0;
-        bool nan_seen = false;
-#pragma unroll
-        for (int j = 0; j < N; ++j) {
-            mx = __builtin_fmax(mx, __builtin_fabs(jr[j]));
-            nan_seen |= isnan_(jr[j]);
-        }
-        const bool nan_any = seg_any<NP>(live && nan_seen);
-        const double wmx = seg_max<NP>(live ? mx : 0.0);
-        int e0 = 0;
-        if (!nan_any && isfin(wmx) && wmx != 0.0) {
-            const int ex = exponent_of(wmx);
-            e0 = ex < -kEmax64 ? -kEmax64 : (ex > kEmax64 ? kEmax64 : ex);
-        }
-        fe = sizeof(TF) == 4 ? e0 : (e0 < 0 ? e0 : 0);
-#pragma unroll
-        for (int j = 0; j < N; ++j) a[j] = TF(fe != 0 ? ldexp2(jr[j], -fe) : jr[j]);
-        TF* rb = rowbuf<TF>();
-        pos = live ? sl : 255;
-        int info = 0;
-#pragma unroll
-        for (int k = 0; k < N; ++k) {
-            const bool cand = live && pos >= k;
-            const double mag = double(tabs(a[k]));
-            const double key = !cand ? -1.0 : (isnan_(mag) ? (pos == k ? double(__builtin_inf()) : -0.5) : mag);
-            const int tag = seg_argmax<NP>(key, (pos << 8) | sl);
-            const int pw = tag >> 8, lw = tag & 255;
-            if (pos == k) pos = pw;
-            else if (sl == lw) pos = k;
-            if (sl == lw) {
-#pragma unroll
-                for (int j = k; j < N; ++j) rb[j] = a[j];
-            }
-            wsync();
-            const TF ukk = rb[k];
-            const bool zero = ukk == TF(0);
-            if (zero && info == 0) info = k + 1;
-            if (live && pos > k && !zero) {
-                const TF lik = a[k] / ukk;
-                a[k] = lik;
-#pragma unroll
-                for (int j = k + 1; j < N; ++j) a[j] = a[j] - lik * rb[j];
-            }
-            wsync();
-        }
-        return info;
-    }
-
-    // backsolve (lu.jl:67-92) on the segment's rows: y is this lane's entry of the right-hand side
-    // (rows never moved); returns, in the lane of logical row p, entry p of the solution.
-    template <class TF> GS_DEV TF backsolve(const TF (&a)[N], int pos, TF y) {
-        TF* yb = ybuf<TF>();
-#pragma unroll
-        for (int k = 0; k < N - 1; ++k) {
-            if (pos == k) yb[k] = y;
-            wsync();
-            const TF yk = yb[k];
-            if (live && pos > k) y = y - a[k] * yk;
-        }
-#pragma unroll
-        for (int k = N - 1; k >= 0; --k) {
-            if (pos == k) {
-                y = y / a[k];
-                yb[k] = y;
-            }
-            wsync();
-            const TF xk = yb[k];
-            if (live && pos < k) y = y - a[k] * xk;
-        }
-        return y;
-    }
-
-    // ldiv!! with FP64 factors (factors.jl:931-936): out = J \ r.
-    GS_DEV void ldiv(const double (&a)[N], int pos, int fe, const double* r, double* out) {
-        const double ri = live ? r[sl] : 0.0;
-        const int m = seg_exponent<NP>(ri, live);
-        const int lo = fe < 0 ? fe : 0;
-        const int c = m < lo ? lo : (m > 0 ? 0 : m);
-        const int alt = m - (kEmax64 - headroom(N));
-        const int k = c > alt ? c : alt;
-        double y = k != 0 ? ldexp2(ri, -k) : ri;
-        y = backsolve<double>(a, pos, y);
-        const int e = k - fe;
-        if (live) out[pos] = e != 0 ? ldexp2(y, e) : y;
-        wsync();
-    }
-
-    // ldiv!! with FP32 factors (factors.jl:943-949): r/s in FP32 with s = max|r|, then
-    // a·frac·2^(ex−e) in FP64.
-    GS_DEV void ldiv(const float (&a)[N], int pos, int fe, const double* r, double* out) {
-        const double ri = live ? r[sl] : 0.0;
-        const double ar = __builtin_fabs(ri);
-        const bool nan_any = seg_any<NP>(live && isnan_(ar));
-        const double sm = nan_any ? dnan() : seg_max<NP>(ar);
-        if (sm == 0.0) {
-            if (live) out[sl] = 0.0;
-            wsync();
-            return;
-        }
-        float y = float(ri / sm);
-        y = backsolve<float>(a, pos, y);
-        const int ex = isfin(sm) ? __builtin_amdgcn_frexp_exp(sm) : 0;
-        const double frac = isfin(sm) ? __builtin_amdgcn_frexp_mant(sm) : sm;
-        if (live) out[pos] = ldexp2(double(y) * frac, ex - fe);
-        wsync();
-    }
-
-    // out = Jreg v (StaticArrays' muladd fold over the row).
-    GS_DEV void matvec(const double (&jr)[N], const double* vin, double* out) {
-        double acc = jr[0] * vin[0];
-#pragma unroll
-        for (int j = 1; j < N; ++j) acc = __builtin_fma(jr[j], vin[j], acc);
-        if (live) out[sl] = acc;
-        wsync();
-    }
-
-    // The direction d = −(J \ F) from a factor in TF, then refine_passes!! (refinement.jl:211-244) with
-    // the held Jreg; returns refine_passes!!' `failed` (false without refinement).
-    template <class TF>
-    GS_DEV bool direction(const double (&jr)[N], const TF (&a)[N], int pos, int fe, bool refine) {
-        ldiv(a, pos, fe, v.Fx, v.d);
-        if (live) v.d[sl] = -v.d[sl];
-        wsync();
-        if (!REFINE || !refine) return false;
-        const double fnorm = vrnorm<N>(v.Fx);
-        if (fnorm == 0.0) return false;
-        if (!isfin(fnorm)) return true;
-        matvec(jr, v.d, v.t);
-        if (live) v.r[sl] = -v.Fx[sl] + -v.t[sl];
-        wsync();
-        double rprev = vrnorm<N>(v.r);
-        if (!isfin(rprev)) return true;
-        if (rprev <= q.ref_eta_eff * fnorm) return false;
-        for (int it = 1; it <= q.ref_maxiter; ++it) {
-            ldiv(a, pos, fe, v.r, v.t);
-            if (live) v.u[sl] = v.d[sl] + v.t[sl];
-            wsync();
-            matvec(jr, v.u, v.t);
-            if (live) v.r[sl] = -v.Fx[sl] + -v.t[sl];
-            wsync();
-            const double rcur = vrnorm<N>(v.r);
-            if (!(rcur < rprev)) return true;
-            copy(v.u, v.d);
-            rprev = rcur;
-            if (rcur <= q.ref_eta_eff * fnorm) return false;
-        }
-        return true;
-    }
-
-    // ---- globalization ----
-
-    // globalize!! Static (linefunction.jl:114-131)
-    GS_DEV Step step_static() {
-        const double dn = vrnorm<N>(v.d);
-        const double stepnorm = __builtin_fabs(q.alpha) * dn;
-        if (!isfin(stepnorm)) return Step{kNonfinite, false, false, 0, vrnorm<N>(v.Fx), stepnorm, false};
-        if (live) v.xt[sl] = v.x[sl] + q.alpha * v.d[sl];
-        wsync();
-        eval_F(v.xt, v.Ft);
-        const double fn = vrnorm<N>(v.Ft);
-        if (!isfin(fn)) return Step{kNonfinite, false, false, 0, vrnorm<N>(v.Fx), stepnorm, false};
-        if (live) {
-            v.x[sl] = v.xt[sl];
-            v.Fx[sl] = v.Ft[sl];
-        }
-        wsync();
-        return Step{kSuccess, false, false, 0, fn, stepnorm, false};
-    }
-
-    // backtrack_step (backtracking.jl:320-340)
-    GS_DEV static double backtrack_step(double phi0, double d0, double a, double pa, double ap, double pp, double p) {
-        if (!isfin(pa)) return 0.1 * a;
-        double an = dnan();
-        const double s = __builtin_fabs(d0);
-        const double r1 = (pa - phi0) / s + a;
-        if (!(ap > 0.0)) {
-            if (r1 > 0.0) an = (a * a) / (2.0 * r1);
-        } else if (isfin(pp) && a != ap) {
-            const double q1 = r1 / (a * a);
-            const double q2 = ((pp - phi0) / s + ap) / (ap * ap);
-            const double A = (q1 - q2) / (a - ap);
-            const double B = (a * q2 - ap * q1) / (a - ap);
-            const double disc = B * B + 3.0 * A;
-            if (disc >= 0.0) an = B > 0.0 ? 1.0 / (B + __builtin_sqrt(disc)) : (-B + __builtin_sqrt(disc)) / (3.0 * A);
-        }
-        if (!(isfin(an) && an > 0.0)) an = p * a;
-        return jmin(jmax(an, 0.1 * a), p * a);
-    }
-
-    // globalize_merit!! for Backtracking (linefunction.jl:159-199 → linesearch.jl:154-171 →
-    // backtracking.jl:354-390); `exact`: ExactStep (slope −2φ₀) or MeasuredSlope (φ′(0) = 2⟨F, J d⟩).
-    GS_DEV Step step_backtracking(bool exact) {
-        const double phi0 = merit;
-        const double dn = vrnorm<N>(v.d);
-        if (!isfin(dn)) return Step{kNonfinite, false, false, 0, vrnorm<N>(v.Fx), dn, false};
-        double d0;
-        if (exact) {
-            d0 = -2.0 * phi0;
-        } else {
-            if (live) v.xt[sl] = v.x[sl] + 0.0 * v.d[sl];
-            wsync();
-            eval_F(v.xt, v.Ft);
-            double ja[N];
-            fam.Jrow(v.xt, live ? sl : 0, ja);
-            matvec(ja, v.d, v.t);
-            d0 = 2.0 * vdot<N>(v.Ft, v.t);
-        }
-        if (!(isfin(phi0) && isfin(d0) && d0 < 0.0)) {
-            const uint8_t c = !(isfin(phi0) && isfin(d0)) ? kNonfinite : (d0 > 0.0 ? kLsFailed : kStalled);
-            if (c == kLsFailed) return Step{kLsFailed, false, false, 1, vrnorm<N>(v.Fx), 0.0, false};
-            return Step{c, false, false, 0, vrnorm<N>(v.Fx), 0.0, false};
-        }
-        const double tau = __builtin_fmax(4.0 * kDblEps * __builtin_fabs(phi0), 4.0 * kDblTiny);
-        const double amin = jmax(tau / __builtin_fabs(d0), kDblMin);
-        double alpha = __builtin_fmin(__builtin_fmax(1.0, amin), double(__builtin_inf()));
-        double aa = alpha, pa = dnan();
-        double ap = dnan(), pp = dnan();
-        int frozen = 0;
-        uint8_t code = kLsFailed;
-        for (int trips = 0; trips < q.bt_maxiter; ++trips) {
-            aa = alpha;
-            if (live) v.xt[sl] = v.x[sl] + alpha * v.d[sl];
-            wsync();
-            eval_F(v.xt, v.Ft);
-            pa = vnorm2<N>(v.Ft);
-            frozen = (pa == phi0 && alpha <= Lim<double>::sqrteps) ? frozen + 1 : 0;
-            if (pa - phi0 <= jmin(0.0, q.c1 * alpha * d0 + tau)) {
-                code = (isfin(pa) && pa - phi0 < -tau) ? kSuccess : (__builtin_fabs(pa - phi0) <= tau ? kStalled : kLsFailed);
-                break;
-            } else if (frozen >= 2) {
-                code = kStalled;
-                break;
-            } else if (alpha <= amin) {
-                code = __builtin_fabs(pa - phi0) <= tau ? kStalled : kLsFailed;
-                break;
-            }
-            const double an = isfin(pa) ? backtrack_step(phi0, d0, alpha, pa, ap, pp, q.p)
-                                        : jmin(0.1 * alpha, __builtin_sqrt(alpha) * __builtin_sqrt(amin));
-            ap = alpha;
-            pp = pa;
-            alpha = jmax(an, amin);
-        }
-        if (code == kSuccess || (code == kLsFailed && isfin(pa) && pa < phi0)) {
-            if (live) {
-                v.x[sl] = v.x[sl] + aa * v.d[sl];
-                v.Fx[sl] = v.Ft[sl];
-            }
-            wsync();
-            merit = pa;
-            return Step{code, false, aa < 1.0, code == kLsFailed ? 1 : 0, __builtin_sqrt(pa), aa * dn, false};
-        }
-        if (code == kLsFailed) return Step{kLsFailed, false, false, 1, vrnorm<N>(v.Fx), 0.0, false};
-        return Step{code == kStalled ? kStalled : kNonfinite, false, false, 0, vrnorm<N>(v.Fx), 0.0, false};
-    }
-
-    GS_DEV Step globalize(bool exact) {
-        if (GLOB == kStatic) return step_static();
-        return step_backtracking(exact);
-    }
-
-    // ---- steps ----
-
-    // newton_step!! in the working precision (newton.jl:1679-1688).
-    GS_DEV Step plain_step() {
-        double jr[N];
-        jacobian(jr);
-        double a[N];
-        int pos, fe;
-        const int info = factor<double>(jr, a, pos, fe);
-        Step s;
-        if (info > 0) {
-            s = Step{kSingular, false, false, 0, vrnorm<N>(v.Fx), 0.0, false};
-        } else {
-            direction<double>(jr, a, pos, fe, true);
-            s = globalize(q.mu == 0.0 && !REFINE);
-        }
-        sfnorm = s.fnorm;
-        return s;
-    }
-
-    // mixed_step!! / mixed_newton!! with fresh factors (newton.jl:1604-1666, 1692-1720): the FP32
-    // factor and its refinement, or the FP64 one of a promoted state; a failed FP32 factor or
-    // refinement promotes and refactorises the held Jacobian in FP64 (promote_factor!!, newton.jl:
-    // 1332-1339).
-    GS_DEV Step mixed_step() {
-        const double fnorm_old = residual_norm();
-        const bool was_promoted = promoted;
-        double jr[N];
-        jacobian(jr);
-        int failures = 0;
-        double lin_rel = 0.0;
-        bool ok = true;
-        if (!promoted) {
-            float a[N];
-            int pos, fe;
-            const int info = factor<float>(jr, a, pos, fe);
-            const bool ref_failed = info > 0 || direction<float>(jr, a, pos, fe, true);
-            if (ref_failed) {
-                failures += 1;
-                promoted = true;
-            } else if (fnorm_old != 0.0 && isfin(fnorm_old)) {
-                // linear_residual_ratio (newton.jl:1422-1430)
-                matvec(jr, v.d, v.t);
-                if (live) v.u[sl] = 1.0 * v.Fx[sl] + 1.0 * v.t[sl];
-                wsync();
-                lin_rel = vrnorm<N>(v.u) / fnorm_old;
-            }
-        }
-        if (promoted) {
-            double a[N];
-            int pos, fe;
-            const int info = factor<double>(jr, a, pos, fe);
-            if (info > 0) ok = false;
-            else direction<double>(jr, a, pos, fe, was_promoted);
-        }
-        if (!ok) {
-            Step s{kSingular, promoted, false, failures, vrnorm<N>(v.Fx), 0.0, false};
-            sfnorm = s.fnorm;
-            return s;
-        }
-        const bool promoted_step = promoted && !was_promoted;
-        Step s = globalize(q.mu == 0.0 && promoted);
-        s.failures += failures;
-        if (promoted_step) {
-            s.promoted = true;
-            slow = 0;
-        } else if (!was_promoted) {
-            // read_mixed_stall (newton.jl:1437-1451), η_lin = max(√eps, 1e-4)
-            if (s.fnorm > 0.5 * fnorm_old && lin_rel > 1e-4) {
-                const int sw = slow + 1;
-                if (sw >= 2) {
-                    promoted = true;
-                    slow = 0;
-                    s.failures += 1;
-                    s.promoted = true;
-                } else {
-                    slow = sw;
-                }
-            } else {
-                slow = 0;
-            }
-        }
-        sfnorm = s.fnorm;
-        return s;
-    }
-
-    // ---- the diagnostics of the last factorisation (grad_ratio_and_norm, newton.jl:1287-1298) ----
-
-    // One pass over the columns of J at the saved point, column j formed in parallel (lane i its
-    // entry J_ij) in v.rowd: the left fold of the squares in column-major order (rnorm's sum; with
-    // e ≠ 0 of (J_ij/2^e)²) and the NaN-propagating largest |J_ij|, and, with `prod`, g = Jᴴ (sc F) into
-    // v.r (entry j the muladd fold over column j).
-    GS_DEV void diag_pass(double dg, int e, bool prod, double sc, double& sumsq, double& mx) {
-        double s = 0.0, m = 0.0;
-#pragma unroll 1
-        for (int j = 0; j < N; ++j) {
-            if (live) v.rowd[sl] = fam.Jentry(sl, j, dg);
-            wsync();
-            const double* col = v.rowd;
-            double acc = 0.0;
-#pragma unroll 8
-            for (int i = 0; i < N; ++i) {
-                const double w = col[i];
-                const double ws = e != 0 ? ldexp2(w, -e) : w;
-                s = (j == 0 && i == 0) ? ws * ws : s + ws * ws;
-                m = jmax(m, __builtin_fabs(w));
-                const double fi = sc != 1.0 ? v.jF[i] * sc : v.jF[i];
-                acc = i == 0 ? w * fi : __builtin_fma(w, fi, acc);
-            }
-            if (prod && sl == j) v.r[j] = acc;
-            wsync();
-        }
-        sumsq = s;
-        mx = m;
-    }
-
-    GS_DEV double grad_ratio() {
-        if (isnan_(diag_fnorm)) return dnan();
-        const double dg = live ? fam.Jdiag(v.jx, sl) : 0.0;
-        const double fnorm = diag_fnorm;
-        double s, m;
-        diag_pass(dg, 0, true, 1.0, s, m);
-        double jnorm;
-        if (__builtin_expect(isfin(s) && s >= double(N * N) * kDblMin, 1)) {
-            jnorm = __builtin_sqrt(s);
-        } else if (m == 0.0 || !isfin(m)) {
-            jnorm = m;
-        } else {
-            const int e = exponent_of(m);
-            double s2, m2;
-            diag_pass(dg, e, false, 1.0, s2, m2);
-            jnorm = ldexp2(__builtin_sqrt(s2), e);
-        }
-        if (jnorm == 0.0 || fnorm == 0.0) return 0.0;
-        const double gnorm = vrnorm<N>(v.r);
-        if (isfin(gnorm) || !(isfin(jnorm) && isfin(fnorm))) return gnorm / jnorm / fnorm;
-        const int e = exponent_of(fnorm);
-        double s3, m3;
-        diag_pass(dg, 0, true, ldexp2(1.0, -e), s3, m3);
-        return vrnorm<N>(v.r) / jnorm / ldexp2(fnorm, -e);
-    }
-
-    // ---- the loop (solver.jl:277-378), as SmallSolve::solve ----
-
-    struct Result {
-        uint8_t code;
-        bool contracted, promoted;
-        int32_t iterations, failures;
-        double fnorm, stepnorm, rate, gradratio;
-    };
-
-    GS_DEV Result solve() {
-        eval_F(v.x, v.Fx);
-        const double fnorm0 = vrnorm<N>(v.Fx);
-        merit = GLOB == kBacktracking ? vnorm2<N>(v.Fx) : dnan();
-        sfnorm = fnorm0;
-        promoted = false;
-        slow = 0;
-        diag_fnorm = dnan();
-        double best_fn = fnorm0;
-        if (q.best) copy(v.x, v.bx);
-        Result R{kSuccess, false, false, 0, 0, fnorm0, 0.0, dnan(), dnan()};
-        if (!isfin(fnorm0)) {
-            R.code = kNonfinite;
-        } else {
-            const double ftest0 = q.norm_max ? vrmax<N>(v.Fx) : fnorm0;
-            const double ftol = jmax(q.f_abstol, q.f_reltol * ftest0);
-            if (0 >= q.min_iterations && ftest0 <= ftol) {
-                R.code = kSuccess;
-            } else if (q.max_iterations == 0) {
-                R.code = kMaxIters;
-            } else {
-                int stalls = 0;
-                while (true) {
-                    const double rate_k = R.rate;
-                    Step s = MIXED ? mixed_step() : plain_step();
-                    if (q.best && s.fnorm < best_fn) {
-                        best_fn = s.fnorm;
-                        copy(v.x, v.bx);
-                    }
-                    R.rate = R.stepnorm > 0.0 ? s.stepnorm / R.stepnorm : dnan();
-                    R.code = s.code;
-                    R.iterations += 1;
-                    R.fnorm = s.fnorm;
-                    R.stepnorm = s.stepnorm;
-                    R.failures += s.failures;
-                    R.promoted |= s.promoted;
-                    if (s.code == kSingular || s.code == kNonfinite || s.code == kMaxIters) break;
-                    if (s.code == kLsFailed && s.stepnorm == 0.0) break;
-                    const double ftest = q.norm_max ? vrmax<N>(v.Fx) : R.fnorm;
-                    if (R.iterations >= q.min_iterations && ftest <= ftol) {
-                        R.code = kSuccess;
-                        break;
-                    }
-                    const bool steptested = s.code != kStalled;
-                    const double xnorm = steptested ? vrnorm<N>(v.x) : dnan();
-                    const double xtol = jmax(q.x_abstol, q.x_reltol * xnorm);
-                    const bool past_min = R.iterations >= q.min_iterations;
-                    bool contracted = false;
-                    if (!s.shortened && q.rate_test && past_min) {
-                        const double th = jmax(R.rate, rate_k);
-                        contracted = R.stepnorm > 0.0 && th < 1.0 && th / (1.0 - th) * R.stepnorm <= q.kappa * xtol;
-                    }
-                    const bool step_conv = past_min && R.stepnorm > 0.0 && R.stepnorm <= xtol;
-                    const bool zero_step = s.code == kSuccess && s.stepnorm == 0.0 && past_min;
-                    const bool roundoff = q.roundoff && past_min && R.rate >= 1.0 && rate_k < 1.0;
-                    if (contracted || (steptested && (step_conv || zero_step || roundoff))) {
-                        R.code = contracted ? kSuccess : kStalled;
-                        R.contracted = contracted;
-                        break;
-                    }
-                    if (s.code == kStalled) {
-                        stalls += 1;
-                        if (stalls >= q.max_stalls) {
-                            R.code = kStalled;
-                            break;
-                        }
-                    } else {
-                        stalls = 0;
-                    }
-                    if (R.iterations >= q.max_iterations) {
-                        R.code = kMaxIters;
-                        break;
-                    }
-                }
-            }
-        }
-        // restore_best (newton.jl:792-799)
-        if (q.best && R.code != kSuccess && isfin(best_fn) && best_fn < R.fnorm) {
-            copy(v.bx, v.x);
-            R.fnorm = best_fn;
-        }
-        if (DIAG) R.gradratio = grad_ratio();
-        return R;
-    }
-};
-
-template <int N, int GLOB, bool MIXED, bool REFINE, bool DIAG>
-__global__ __launch_bounds__(Shape<N>::kThreads) void dense_kernel(const DenseLaunch L) {
-    using Sh = Shape<N>;
-    __shared__ double sA[N * N];
-    __shared__ SysVecs<N> sv[Sh::SPB];
-    for (int t = int(threadIdx.x); t < N * N; t += Sh::kThreads) sA[t] = L.A[t];
-    __syncthreads();
-    const int lane = int(threadIdx.x) & 63, wave = int(threadIdx.x) >> 6;
-    const int g = lane / Sh::NP, sl = lane & (Sh::NP - 1);
-    const int sw = wave * Sh::G + g;
-    const int64_t i = int64_t(blockIdx.x) * Sh::SPB + sw;
-    if (i >= L.nsys) return;  // the whole segment leaves together
-    const int64_t idx = L.order ? L.order[i] - 1 : i;
-    SysVecs<N>& v = sv[sw];
-    FamDenseSin<N> fam;
-    fam.A = sA;
-    const bool live = sl < N;
-    fam.b = live ? reinterpret_cast<const double*>(L.P + (L.p_stride ? idx * L.p_stride : 0))[sl] : 0.0;
-    double* xp = L.X + idx * N;
-    if (live) v.x[sl] = xp[sl];
-    wsync();
-    DenseSolve<FamDenseSin<N>, GLOB, MIXED, REFINE, DIAG> solver(fam, L.q, v, sl);
-    const auto R = solver.solve();
-    if (live) xp[sl] = v.x[sl];
-    if (sl == 0) {
-        if (DIAG) {
-            store_status(L.S + idx, R.code, R.iterations, R.fnorm, R.stepnorm, R.failures, R.promoted, R.gradratio,
-                         R.rate, R.contracted, false);
-        } else {
-            L.codes[idx] = R.code;
-        }
-    }
-}
-
-template <int N, int GLOB, bool MIXED, bool REFINE>
-int launch_diag(bool diag, const DenseLaunch& L, hipStream_t st) {
-    using Sh = Shape<N>;
-    const int64_t blocks = (L.nsys + Sh::SPB - 1) / Sh::SPB;
-    if (blocks > 0x7fffffffLL) return set_error(GS_EINVAL, "batch too large for one launch");
-    if (diag)
-        hipLaunchKernelGGL((dense_kernel<N, GLOB, MIXED, REFINE, true>), dim3(unsigned(blocks)), dim3(Sh::kThreads), 0,
-                           st, L);
-    else
-        hipLaunchKernelGGL((dense_kernel<N, GLOB, MIXED, REFINE, false>), dim3(unsigned(blocks)), dim3(Sh::kThreads), 0,
-                           st, L);
-    return GS_OK;
-}
-
-template <int N> int launch_n(int glob, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st) {
-    if (glob == kBacktracking) {
-        if (mixed) return refine ? launch_diag<N, kBacktracking, true, true>(diag, L, st)
-                                 : launch_diag<N, kBacktracking, true, false>(diag, L, st);
-        return refine ? launch_diag<N, kBacktracking, false, true>(diag, L, st)
-                      : launch_diag<N, kBacktracking, false, false>(diag, L, st);
-    }
-    if (mixed) return refine ? launch_diag<N, kStatic, true, true>(diag, L, st)
-                             : launch_diag<N, kStatic, true, false>(diag, L, st);
-    return refine ? launch_diag<N, kStatic, false, true>(diag, L, st) : launch_diag<N, kStatic, false, false>(diag, L, st);
+// The lanes per system a size takes: 16 or 32 at n = 16, 32 or 64 at n = 24 and 32, 64 above.
+bool lanes_allowed(int n, int lanes) {
+    if (n == 16) return lanes == 16 || lanes == 32;
+    if (n == 24 || n == 32) return lanes == 32 || lanes == 64;
+    return lanes == 64;
 }
 
 }  // namespace
 
-int dense_systems_per_block(int n) {
-    return n <= 16 ? Shape<16>::SPB : Shape<32>::SPB;
+int lanes_of(int n) {
+    const int k = size_slot(n);
+    const int np = k < 0 ? 0 : g_dense_np[k].load(std::memory_order_relaxed);
+    return np != 0 ? np : default_np(n);
+}
+
+int dense_systems_per_block(int n) { return 64 / lanes_of(n) * 4; }
+
+int dense_set_shape(int n, int lanes) {
+    const int k = size_slot(n);
+    if (k < 0) return set_error(GS_EINVAL, "the dense family takes n = 16, 24, 32, 48 or 64");
+    if (lanes != 0 && !lanes_allowed(n, lanes))
+        return set_error(GS_EINVAL, "lanes per system at n = " + std::to_string(n) + " must be 0 (default) or " +
+                                        (n == 16 ? "16 or 32" : (n <= 32 ? "32 or 64" : "64")));
+    g_dense_np[k].store(lanes, std::memory_order_relaxed);
+    return GS_OK;
 }
 
 int launch_dense(int n, int glob, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st) {
+    const int np = lanes_of(n);
     switch (n) {
-        case 16: return launch_n<16>(glob, mixed, refine, diag, L, st);
-        case 24: return launch_n<24>(glob, mixed, refine, diag, L, st);
-        case 32: return launch_n<32>(glob, mixed, refine, diag, L, st);
+        case 16: return launch_dense_n16(np, glob, mixed, refine, diag, L, st);
+        case 24: return launch_dense_n24(np, glob, mixed, refine, diag, L, st);
+        case 32: return launch_dense_n32(np, glob, mixed, refine, diag, L, st);
+        case 48: return launch_dense_n48(np, glob, mixed, refine, diag, L, st);
+        case 64: return launch_dense_n64(np, glob, mixed, refine, diag, L, st);
     }
-    return set_error(GS_EINVAL, "the dense family takes n = 16, 24 or 32");
+    return set_error(GS_EINVAL, "the dense family takes n = 16, 24, 32, 48 or 64");
 }
 
 }  // namespace gs
+
+extern "C" {
+
+int gs_dense_set_shape(int n, int lanes) { return gs::dense_set_shape(n, lanes); }
+
+int gs_dense_get_shape(int n) {
+    if (!(n == 16 || n == 24 || n == 32 || n == 48 || n == 64))
+        return gs::set_error(GS_EINVAL, "the dense family takes n = 16, 24, 32, 48 or 64");
+    return gs::lanes_of(n);
+}
+
+}  // extern "C"

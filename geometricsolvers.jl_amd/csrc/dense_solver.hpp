@@ -1,5 +1,5 @@
-// The launch of the dense mid-size solve (a segment of a wave per system, n = 16, 24 or 32): the
-// arguments gs_solve_batch_f64 hands to the kernels of dense_solver.hip.
+// The launch of the dense mid-size solve (a segment of 16, 32 or 64 lanes per system, n = 16, 24, 32,
+// 48 or 64): the arguments gs_solve_batch_f64 hands to the kernels of dense_solver.hip.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -21,10 +21,17 @@ struct DenseLaunch {
     const int64_t* order;
 };
 
-// The systems of one launch per workgroup at size n (n = 16: four per wave; 24 and 32: two).
+// The systems of one launch per workgroup at size n with its current lanes per system.
 int dense_systems_per_block(int n);
 
-// GS_FAM_DENSE_SIN under Newton with the Static or Backtracking globalization, LU in FP64 or FP32
+// The lanes per system (segment width: 16, 32 or 64, at least n; 0 = the default, the smallest that
+// holds n rows) of the launches at size n from now on; GS_EINVAL for another size or width.
+int dense_set_shape(int n, int lanes);
+
+// The lanes per system the next launch at size n uses.
+int lanes_of(int n);
+
+// GS_FAM_DENSE_SIN (n = 16, 24, 32, 48 or 64) under Newton with the Static or Backtracking globalization, LU in FP64 or FP32
 // (mixed, with or without Refinement); returns a gs_error code (the launch only; the caller
 // synchronises).
 int launch_dense(int n, int glob, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st);

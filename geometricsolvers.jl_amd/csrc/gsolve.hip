@@ -109,7 +109,7 @@ static const FamilySpec* family_spec(int id) {
         {GS_FAM_HENON_HEILES_MIDPOINT, 4, 4, 32, 16, 16, 16},
         {GS_FAM_FPU_GL3, 48, 32, 256, 8, 0, 0},
         {GS_ACT_HENON_HEILES, 6, 2, 16, 8, 0, 0},
-        {GS_FAM_DENSE_SIN, 0, 0, 0, 8, 0, 0},  // n = 16, 24 or 32, records of n doubles
+        {GS_FAM_DENSE_SIN, 0, 0, 0, 8, 0, 0},  // n = 16, 24, 32, 48 or 64, records of n doubles
     };
     for (const auto& s : specs)
         if (s.id == id) return &s;
@@ -131,8 +131,9 @@ static int validate_batch(const gs_method* method, const gs_family* family, int6
     const FamilySpec* fs = family_spec(family->id);
     if (!fs) return fail(GS_EFAMILY, "problem family " + std::to_string(family->id) + " is not registered");
     if (family->id == GS_FAM_DENSE_SIN) {
-        if (!(family->n == 16 || family->n == 24 || family->n == 32) || family->m != family->n)
-            return fail(GS_EINVAL, "the dense family takes n = m = 16, 24 or 32");
+        if (!(family->n == 16 || family->n == 24 || family->n == 32 || family->n == 48 || family->n == 64) ||
+            family->m != family->n)
+            return fail(GS_EINVAL, "the dense family takes n = m = 16, 24, 32, 48 or 64");
         if (!family->data || (reinterpret_cast<uintptr_t>(family->data) % 8) != 0)
             return fail(GS_EINVAL, "the dense family needs its matrix A (family data, 8-byte aligned)");
     } else if (family->n != fs->n || family->m != fs->n) {

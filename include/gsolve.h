@@ -180,8 +180,9 @@ enum gs_family_id {
     GS_FAM_QUAD_MAP2 = 9,        /* u = A x, F_i = (u_i + c_i u_i²) − b_i : n = 2, P = A (4, column-major), b (2),  *
                                   * c (2) — f_mod_r1 / f_ill_r1 of test/nonlinear/refinement.jl:40-96           */
     GS_FAM_DENSE_SIN = 10        /* A x + sin.(x)/10 − b, the §10.E problem of benchmark/device_problems.jl:133-175 : *
-                                  * n = 16, 24 or 32; `data` = A (n × n, column-major, shared by the batch),      *
-                                  * P = b (n doubles per system); one wave segment per system                   */
+                                  * n = 16, 24, 32, 48 or 64; `data` = A (n × n, column-major, shared by the      *
+                                  * batch), P = b (n doubles per system); a segment of 16, 32 or 64 lanes of a  *
+                                  * wave per system (gs_dense_set_shape)                                        */
 };
 
 typedef struct gs_family {
@@ -353,6 +354,15 @@ int gs_solve_expsum_lsq_f64(const gs_method* method, int64_t nfit, double* X, co
  * workgroups for a batch of nfit fits). */
 int gs_lsq_set_shape(int k, int threads, int blocks_per_cu);
 int gs_lsq_get_shape(int k, int64_t nfit, int64_t out[3]);
+
+/* The launch shape of the dense family (GS_FAM_DENSE_SIN) at size n (16, 24, 32, 48 or 64): `lanes`
+ * per system, the wave segment that holds one system's rows (16, 32 or 64 and at least n; 16 lanes
+ * put four systems in a wave, 64 one), or 0 for the default (the smallest that holds n). Results do
+ * not depend on it. Set by the package's autotuner; no counterpart in the reference.
+ * gs_dense_get_shape reports the lanes per system the next launch at size n uses (or a negative
+ * error code). */
+int gs_dense_set_shape(int n, int lanes);
+int gs_dense_get_shape(int n);
 
 /* The least-squares launch sequence (the device sort and the four bucket kernels) is captured into
  * a hipGraph and replayed while a call's pointers, sizes, method, options and shapes are those of the

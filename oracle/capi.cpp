@@ -251,6 +251,8 @@ static int solve_small_batch(const Method<R>& m, const Options<R>& o, const gs_f
                 case 16: return run_n(std::integral_constant<int, 16>{});
                 case 24: return run_n(std::integral_constant<int, 24>{});
                 case 32: return run_n(std::integral_constant<int, 32>{});
+                case 48: return run_n(std::integral_constant<int, 48>{});
+                case 64: return run_n(std::integral_constant<int, 64>{});
             }
             return GS_EINVAL;
         }
@@ -742,7 +744,7 @@ void orc_read_mixed_stall(int promoted, int slow_steps, double fnorm, double ste
     out[5] = info.stepnorm;
 }
 
-// F(x) and J(x) (column-major) of family 10 at n = 16, 24 or 32: A column-major, b of one system.
+// F(x) and J(x) (column-major) of family 10 at n = 16, 24, 32, 48 or 64: A column-major, b of one system.
 int orc_dense_eval(int n, const double* A, const double* b, const double* x, double* F, double* J) {
     auto run = [&](auto nn) {
         constexpr int NN = decltype(nn)::value;
@@ -763,6 +765,8 @@ int orc_dense_eval(int n, const double* A, const double* b, const double* x, dou
         case 16: return run(std::integral_constant<int, 16>{});
         case 24: return run(std::integral_constant<int, 24>{});
         case 32: return run(std::integral_constant<int, 32>{});
+        case 48: return run(std::integral_constant<int, 48>{});
+        case 64: return run(std::integral_constant<int, 64>{});
     }
     return GS_EINVAL;
 }

@@ -1,5 +1,5 @@
 """Parity of the dense mid-size solve (a wave segment per system, GS_FAM_DENSE_SIN: the §10.E problem
-A x + sin.(x)/10 − b of benchmark/device_problems.jl:133-175, n = 16, 24, 32) with the CPU oracle,
+A x + sin.(x)/10 − b of benchmark/device_problems.jl:133-175, n = 16, 24, 32, 48, 64) with the CPU oracle,
 through the C ABI. The device evaluates the family's fixed sine and cosine and folds every sum in the
 oracle's order, so results are compared bit for bit: iterates and every status byte.
 
@@ -52,10 +52,10 @@ def _gpu(gs, device, fam, method, X0, B, opts, counts=False, order=None):
     return X.cpu().numpy(), gs.statuses(S)
 
 
-@pytest.mark.parametrize("n", [16, 24, 32])
+@pytest.mark.parametrize("n", [16, 24, 32, 48, 64])
 @pytest.mark.parametrize("name", list(METHODS))
 def test_dense_every_method_bitwise(gs, orc, workloads, device, n, name):
-    nsys = 2048 if n < 32 else 1024
+    nsys = 2048 if n < 32 else (1024 if n == 32 else 512)
     A, B, X0 = _batch(workloads, n, nsys, seed=n)
     fam = gs.DenseSin(A)
     m = METHODS[name](gs)
@@ -103,10 +103,67 @@ def test_dense_counts_form_and_order(gs, orc, workloads, device):
 
 def test_dense_full_size_converges_to_the_built_roots(gs, workloads, device):
     """The benchmark's own batch from x₀ = 0 at 2^16 systems: every system converges to its root."""
-    for n in (16, 32):
+    for n in (16, 32, 64):
         A, B, Xs = workloads.tene_batch(n, 1 << 16)
         x, st = _gpu(gs, device, gs.DenseSin(A), gs.Newton(), np.zeros((1 << 16, n)), B,
                      gs.Options(f_abstol=1e-12, f_reltol=0.0))
         assert (st["code"] == 0).all()
         assert np.abs(x - Xs).max() < 1e-12
         assert np.abs(workloads.tene_residual(x, A, B)).max() < 1e-12
+
+
+@pytest.mark.parametrize("n", [16, 24, 32])
+def test_dense_lanes_per_system_agree_bitwise(gs, orc, workloads, device, n):
+    """Every segment width a size takes (four, two or one systems per wave) writes the same bytes,
+    those of the oracle; the width is the launch's choice alone."""
+    A, B, X0 = _batch(workloads, n, 1000, seed=n + 1)
+    fam = gs.DenseSin(A)
+    m = METHODS["mixed-backtracking-refine"](gs)
+    opts = gs.Options(f_abstol=1e-12, f_reltol=0.0, max_iterations=40)
+    xo, so = orc.solve_batch(m, fam, X0.copy(), gs.PerSystem(B), opts)
+    try:
+        for lanes in gs.autotune.DENSE_LANES[n]:
+            gs.dense_set_shape(n, lanes)
+            assert gs.dense_get_shape(n) == lanes
+            print(f"dense n={n} lanes={lanes}", flush=True)
+            x, st = _gpu(gs, device, fam, m, X0, B, opts)
+            assert_parity(st, x, so, xo, bitwise=True)
+    finally:
+        gs.dense_set_shape(n, 0)
+    assert gs.dense_get_shape(n) == gs.autotune.DENSE_LANES[n][0]
+
+
+def test_dense_tuner_sets_a_shape_that_changes_no_result(gs, workloads, device):
+    n, nsys = 16, 1 << 14
+    A, B, _ = workloads.tene_batch(n, nsys)
+    fam = gs.DenseSin(A)
+    prob = gs.NonlinearProblem(fam)
+    p = gs.PerSystem(torch.as_tensor(B, device=device))
+    opts = gs.Options(f_abstol=1e-12, f_reltol=0.0)
+    X0 = torch.zeros((nsys, n), dtype=torch.float64, device=device)
+    ref = _gpu(gs, device, fam, gs.Newton(), X0.cpu().numpy(), B, opts)
+    try:
+        res = gs.tune_dense(X0, prob, gs.Newton(), p, opts, repeats=1)
+        assert res["lanes"] in gs.autotune.DENSE_LANES[n] and gs.dense_get_shape(n) == res["lanes"]
+        assert set(res["seconds"]) == set(gs.autotune.DENSE_LANES[n])
+        got = _gpu(gs, device, fam, gs.Newton(), X0.cpu().numpy(), B, opts)
+        assert np.array_equal(got[0].view(np.uint64), ref[0].view(np.uint64))
+        assert got[1].tobytes() == ref[1].tobytes()
+    finally:
+        gs.dense_set_shape(n, 0)
+    assert (X0 == 0).all()  # the tuner solved copies
+
+
+def test_workgroup_tuner_changes_no_result(gs, workloads, device):
+    xk = workloads.henon_heiles_states(1 << 14, seed=9)
+    Xk = torch.as_tensor(xk, device=device)
+    prob = gs.NonlinearProblem(gs.HenonHeilesMidpoint())
+    p = gs.BatchParams(0.1, Xk)
+    opts = gs.Options(f_abstol=1e-12, f_reltol=0.0)
+    res = gs.tune_workgroup(Xk, prob, gs.Newton(), p, opts, repeats=1)
+    assert res["workgroupsize"] in (64, 128, 256) and set(res["seconds"]) == {64, 128, 256}
+    X1, X2 = Xk.clone(), Xk.clone()
+    S1, S2 = gs.status_buffer(Xk.shape[0], device), gs.status_buffer(Xk.shape[0], device)
+    gs.solve_batch(X1, S1, prob, gs.Newton(), p, opts)
+    gs.solve_batch(X2, S2, prob, gs.Newton(), p, opts, workgroupsize=res["workgroupsize"])
+    assert torch.equal(X1, X2) and torch.equal(S1, S2)

@@ -207,6 +207,34 @@ def test_least_squares_entry_point_checks_method_without_gpu(gs):
     assert lib.gs_lsq_set_shape(1, 64, 65) == abi.GS_EINVAL and lib.gs_lsq_set_shape(1, 96, 0) == abi.GS_EINVAL
 
 
+def test_dense_shape_setting_without_gpu(gs):
+    """gs_dense_set_shape takes, per size, the segment widths that hold its rows (0 restores the
+    default) and refuses any other width or size; it touches no device."""
+    lib = gs.load_library()
+    from importlib import import_module
+
+    abi = import_module(gs.__name__ + "._abi")
+    allowed = {16: (16, 32), 24: (32, 64), 32: (32, 64), 48: (64,), 64: (64,)}
+    assert allowed == gs.autotune.DENSE_LANES
+    try:
+        for n, widths in allowed.items():
+            assert lib.gs_dense_get_shape(n) == widths[0]
+            for lanes in (8, 16, 32, 64, 128):
+                rc = lib.gs_dense_set_shape(n, lanes)
+                if lanes in widths:
+                    assert rc == 0 and lib.gs_dense_get_shape(n) == lanes
+                else:
+                    assert rc == abi.GS_EINVAL and f"n = {n}" in abi.last_error(lib)
+            assert lib.gs_dense_set_shape(n, 0) == 0 and lib.gs_dense_get_shape(n) == widths[0]
+        assert lib.gs_dense_set_shape(40, 64) == abi.GS_EINVAL and lib.gs_dense_get_shape(40) == abi.GS_EINVAL
+    finally:
+        for n in allowed:
+            lib.gs_dense_set_shape(n, 0)
+    with pytest.raises(gs.ArgumentError, match="order 16, 24, 32, 48 or 64"):
+        gs.DenseSin(np.eye(40))
+    assert gs.DenseSin(np.eye(64) * 2.0).n == 64
+
+
 def test_reuse_method_and_entry_checks(gs):
     """Newton(; reuse = k) crosses the ABI as gs_method.reuse; the constructor's refusals
     (newton.jl:60-90) and the entry points' refusal of reuse where no held factor exists (the
