@@ -437,8 +437,8 @@ end
 
 # The roots as the flat record of gs_deflation, or `nothing` where the device form does not hold
 # them (more than 4 roots or 32 entries, a power that is not an integer in 1:64); such a Deflated,
-# and every method the device form does not take (reuse, best = true, a Float32 batch), runs
-# the reference's generic batch, which also raises the reference's own errors.
+# and every method the device form does not take (best = true, a Float32 batch), runs the
+# reference's generic batch, which also raises the reference's own errors.
 function device_deflation(def::Deflated, n::Int)
     nr = length(def.roots)
     (1 <= nr <= 4 && nr * n <= 32) || return nothing
@@ -454,7 +454,7 @@ function device_deflation(def::Deflated, n::Int)
 end
 
 served_deflated(fam::HIPFamily, cm, ::Type{T}) where {T} = !(fam isa Union{FpuGl3, DenseSin}) && T === Float64 &&
-    cm !== nothing && cm.globalization in (0, 1, 3) && cm.reuse == 0 && cm.best == 0
+    cm !== nothing && cm.globalization in (0, 1, 3) && (cm.reuse == 0 || cm.globalization != 3) && cm.best == 0
 
 function hip_launch_deflated!(X, S, codes, def::Deflated, cd::GsDeflation, cm::GsMethod, p, opt, order, ws)
     fam = def.prob.F
@@ -548,7 +548,8 @@ device_chart(::PolarChart) = GsChart(Int32(2), Int32(0), ntuple(_ -> 0.0, 16), n
 const HIPReparam = Reparametrized{<:NonlinearProblem{<:HIPFamily}, <:Union{LinearChart, PolarChart}}
 
 served_reparam(rep, cm) = !rep.covariant && !(rep.prob.F isa Union{FpuGl3, DenseSin}) && cm !== nothing &&
-    cm.globalization in (0, 1) && cm.reuse == 0 && (!(rep.g isa PolarChart) || family_n(rep.prob.F) == 2)
+    cm.globalization in (0, 1, 3) && (cm.reuse == 0 || cm.globalization != 3) &&
+    (!(rep.g isa PolarChart) || family_n(rep.prob.F) == 2)
 
 function hip_launch_reparam!(Y, S, codes, rep, cm::GsMethod, p, opt, order, ws)
     fam = rep.prob.F

@@ -471,7 +471,6 @@ int gs_solve_batch_deflated_f64(const gs_method* method, const gs_family* family
                          "residual, not by the inner residual that status.fnorm reports");
     if (family->id == GS_FAM_FPU_GL3 || family->id == GS_ACT_HENON_HEILES)
         return set_error(GS_EFAMILY, "a Deflated batch wraps one of the lane-per-system families (1-5)");
-    if (method->reuse > 0) return set_error(GS_EFAMILY, "a Deflated batch takes no reuse on the device");
     if (nsys == 0) return GS_OK;
     KernelArgs a;
     std::memset(&a, 0, sizeof(a));
@@ -499,7 +498,7 @@ int gs_solve_batch_deflated_f64(const gs_method* method, const gs_family* family
     const int glob = method->globalization == GS_BACKTRACKING ? kBacktracking
                      : method->globalization == GS_DOGLEG    ? kDogLeg
                                                              : kStatic;
-    rc = launch_small_deflated(family->id, glob, a, mixed, refine, diag, wg, st);
+    rc = launch_small_deflated(family->id, glob, method->reuse > 0, a, mixed, refine, diag, wg, st);
     return finish_launch(rc, st, timed);
 }
 
@@ -598,9 +597,6 @@ int gs_solve_batch_reparametrized_f64(const gs_method* method, const gs_family* 
     } else {
         return set_error(GS_EINVAL, "chart kind must be GS_CHART_LINEAR or GS_CHART_POLAR");
     }
-    if (method->globalization == GS_DOGLEG || method->reuse > 0)
-        return set_error(GS_EFAMILY,
-                         "a Reparametrized batch takes the Static or Backtracking globalization without reuse");
     if (nsys == 0) return GS_OK;
     KernelArgs a;
     std::memset(&a, 0, sizeof(a));
@@ -624,8 +620,10 @@ int gs_solve_batch_reparametrized_f64(const gs_method* method, const gs_family* 
     const bool mixed = method->factor_precision == GS_FACTOR_F32;
     const bool refine = method->refinement == GS_REFINEMENT;
     bool timed = timing_begin(st);
-    const int glob = method->globalization == GS_BACKTRACKING ? kBacktracking : kStatic;
-    rc = launch_small_reparam(family->id, chart->kind, glob, a, mixed, refine, diag, wg, st);
+    const int glob = method->globalization == GS_BACKTRACKING ? kBacktracking
+                     : method->globalization == GS_DOGLEG    ? kDogLeg
+                                                             : kStatic;
+    rc = launch_small_reparam(family->id, chart->kind, glob, method->reuse > 0, a, mixed, refine, diag, wg, st);
     return finish_launch(rc, st, timed);
 }
 

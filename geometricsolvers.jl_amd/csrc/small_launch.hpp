@@ -246,19 +246,21 @@ int launch_small_reuse(int family, int glob, const KernelArgs& a, bool mixed, bo
 // Defined in small_dogleg.hip: DogLeg.
 int launch_small_dogleg(int family, const KernelArgs& a, bool mixed, bool refine, bool diag, int wg, hipStream_t st);
 
-// Defined in small_deflated.hip: Static, Backtracking and DogLeg on a Deflated batch (FamDeflatedT).
-int launch_small_deflated(int family, int glob, const KernelArgs& a, bool mixed, bool refine, bool diag, int wg,
-                          hipStream_t st);
+// Defined in small_deflated.hip: Static, Backtracking and DogLeg on a Deflated batch (FamDeflatedT),
+// with or without reuse = k (not with DogLeg).
+int launch_small_deflated(int family, int glob, bool reuse, const KernelArgs& a, bool mixed, bool refine, bool diag,
+                          int wg, hipStream_t st);
 
 // Defined in small_stationarity.hip: Static, Backtracking and DogLeg on a StationarityProblem batch
 // of a registered action (FamStationarityT), |E| = 2 or 6, with or without reuse = k (not with DogLeg).
 int launch_small_stationarity(int action, int ne, int glob, bool reuse, const KernelArgs& a, bool mixed, bool refine,
                               bool diag, int wg, hipStream_t st);
 
-// Defined in small_reparam.hip: Static and Backtracking on a Reparametrized batch (FamReparamT) with
-// a linear chart (families 1–5) or the polar chart (the n = 2 families).
-int launch_small_reparam(int family, int chart, int glob, const KernelArgs& a, bool mixed, bool refine, bool diag,
-                         int wg, hipStream_t st);
+// Defined in small_reparam.hip: Static, Backtracking and DogLeg on a Reparametrized batch
+// (FamReparamT) with a linear chart (families 1–5) or the polar chart (the n = 2 families), with or
+// without reuse = k (not with DogLeg).
+int launch_small_reparam(int family, int chart, int glob, bool reuse, const KernelArgs& a, bool mixed, bool refine,
+                         bool diag, int wg, hipStream_t st);
 
 // Defined in small_f32.hip: every form of a Float32 batch.
 int launch_small_f32(int family, int glob, bool reuse, const KernelArgsT<float>& a, bool refine, bool diag, int wg,

@@ -454,9 +454,14 @@ struct SmallSolve {
 
     // The product of a refinement (newton.jl:1344-1384): the held regularised Jacobian J of a fresh
     // factor, or, along a reused factor (JVP), the JVP at x with J = J(x): 1·J(x) v + μ v where μ > 0
-    // (RegularizedJVP).
-    template <bool JVP> GS_DEV void product(const T (&J)[N][N], const T (&v)[N], T (&Jv)[N]) const {
-        matvec<N>(J, v, Jv);
+    // (RegularizedJVP). A Deflated family's JVP is its own product M (J v) + ⟨∇M, v⟩ F at x
+    // (deflation.jl:352-357), not the deflated Jacobian times v.
+    template <bool JVP> GS_DEV void product(const T (&J)[N][N], const T (&x)[N], const T (&v)[N], T (&Jv)[N]) const {
+        if constexpr (JVP && is_deflated<Fam>::value) {
+            fam.Jv(x, v, Jv);
+        } else {
+            matvec<N>(J, v, Jv);
+        }
         if (JVP && q.mu != T(0.0)) {
 #pragma unroll
             for (int i = 0; i < N; ++i) Jv[i] = T(1.0) * Jv[i] + q.mu * v[i];
@@ -464,15 +469,17 @@ struct SmallSolve {
     }
 
     // refine_passes!! (refinement.jl:211-244) with the product of `product<JVP>(J, ·)`.
-    // `rlast` receives ‖r‖ = ‖−F − J d‖ of the returned d when the refinement succeeds.
+    // `rlast` receives ‖r‖ = ‖−F − J d‖ of the returned d when the refinement succeeds; x is the
+    // point of the JVP (along a reused factor).
     template <class TFR, bool JVP = false>
-    GS_DEV void refine(T (&d)[N], const Factor<TFR, N>& f, const T (&J)[N][N], const T (&Fx)[N], bool& failed) const {
+    GS_DEV void refine(T (&d)[N], const Factor<TFR, N>& f, const T (&J)[N][N], const T (&Fx)[N], const T (&x)[N],
+                       bool& failed) const {
         T rlast;
-        refine<TFR, JVP>(d, f, J, Fx, failed, rlast);
+        refine<TFR, JVP>(d, f, J, Fx, x, failed, rlast);
     }
     template <class TFR, bool JVP = false>
-    GS_DEV void refine(T (&d)[N], const Factor<TFR, N>& f, const T (&J)[N][N], const T (&Fx)[N], bool& failed,
-                       T& rlast) const {
+    GS_DEV void refine(T (&d)[N], const Factor<TFR, N>& f, const T (&J)[N][N], const T (&Fx)[N], const T (&x)[N],
+                       bool& failed, T& rlast) const {
         rlast = tnan<T>();
         failed = false;
         // Under Static without reuse the state's ‖F‖ is rnorm of this very residual (set from it at the
@@ -484,7 +491,7 @@ struct SmallSolve {
             return;
         }
         T Jv[N], r[N];
-        product<JVP>(J, d, Jv);
+        product<JVP>(J, x, d, Jv);
 #pragma unroll
         for (int i = 0; i < N; ++i) r[i] = -Fx[i] + -Jv[i];
         T rprev = rnorm<N>(r);
@@ -499,7 +506,7 @@ struct SmallSolve {
             ldiv<N>(f, r, dd);
 #pragma unroll
             for (int i = 0; i < N; ++i) dc[i] = d[i] + dd[i];
-            product<JVP>(J, dc, Jv);
+            product<JVP>(J, x, dc, Jv);
 #pragma unroll
             for (int i = 0; i < N; ++i) r[i] = -Fx[i] + -Jv[i];
             T rcur = rnorm<N>(r);
@@ -554,7 +561,7 @@ struct SmallSolve {
             for (int i = 0; i < N; ++i) d[i] = -d[i];
             if (REFINE) {
                 bool failed;
-                refine<T>(d, f, J, Fx, failed);
+                refine<T>(d, f, J, Fx, x, failed);
             }
             bool exact = q.mu == T(0.0) && !REFINE;
             s = globalize(x, Fx, d, exact);
@@ -584,7 +591,7 @@ struct SmallSolve {
                 for (int i = 0; i < N; ++i) d[i] = -d[i];
                 if (REFINE) {
                     bool failed;
-                    refine<T>(d, f, J, Fx, failed);
+                    refine<T>(d, f, J, Fx, x, failed);
                 }
             }
         } else {
@@ -598,7 +605,7 @@ struct SmallSolve {
                 ldiv<N>(f, Fx, d);
 #pragma unroll
                 for (int i = 0; i < N; ++i) d[i] = -d[i];
-                if (REFINE) refine<float>(d, f, J, Fx, ref_failed, rlast);
+                if (REFINE) refine<float>(d, f, J, Fx, x, ref_failed, rlast);
             }
             if (ref_failed) {
                 // promote_factor!! (newton.jl:1332-1339): refactorise the held Jacobian in FP64
@@ -805,7 +812,7 @@ struct SmallSolve {
             for (int i = 0; i < N; ++i) d[i] = -d[i];
             if (REFINE) {
                 bool failed;
-                refine<T>(d, f, J, Fx, failed);
+                refine<T>(d, f, J, Fx, x, failed);
             }
             const T nd = rnorm<N>(d);
             newton = isfin(nd) && nd != T(0.0);
@@ -872,7 +879,7 @@ struct SmallSolve {
             if (REFINE) {
                 T Jx[N][N];
                 fam.J(x, Jx);
-                refine<TF, true>(d, kept, Jx, Fx, ref_failed);
+                refine<TF, true>(d, kept, Jx, Fx, x, ref_failed);
             }
             if (ref_failed) {
                 failures = 1;
@@ -899,7 +906,7 @@ struct SmallSolve {
             for (int i = 0; i < N; ++i) d[i] = -d[i];
             if (REFINE) {
                 bool failed;
-                refine<TF>(d, kept, J, Fx, failed);
+                refine<TF>(d, kept, J, Fx, x, failed);
             }
             s = globalize(x, Fx, d, q.mu == T(0.0) && !REFINE);
         }
@@ -939,7 +946,7 @@ struct SmallSolve {
                     for (int i = 0; i < N; ++i) d[i] = -d[i];
                     if (REFINE) {
                         bool failed;
-                        refine<T>(d, f, J, Fx, failed);
+                        refine<T>(d, f, J, Fx, x, failed);
                     }
                 }
                 have = true;
@@ -951,7 +958,7 @@ struct SmallSolve {
                 if (REFINE) {
                     T Jx[N][N];
                     fam.J(x, Jx);
-                    refine<TF, true>(d, kept, Jx, Fx, ref_failed);
+                    refine<TF, true>(d, kept, Jx, Fx, x, ref_failed);
                 }
                 if (!ref_failed) {
                     md_fresh = false;
@@ -971,7 +978,7 @@ struct SmallSolve {
                     ldiv<N>(f, Fx, d);
 #pragma unroll
                     for (int i = 0; i < N; ++i) d[i] = -d[i];
-                    if (REFINE) refine<float>(d, f, J, Fx, ref_failed);
+                    if (REFINE) refine<float>(d, f, J, Fx, x, ref_failed);
                 }
                 if (ref_failed) {
                     // promote_factor!! (newton.jl:1332-1339): refactorise the held Jacobian in FP64

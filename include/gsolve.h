@@ -249,8 +249,9 @@ typedef struct gs_deflation {
  * line search take the deflated residual M(x) F(x) with the product-rule Jacobian M J + F ∇Mᵀ
  * (deflation.jl:236-399), and the status reports, and the residual test reads, the inner
  * ‖F‖ = ‖M F‖ / M (converged_status / test_status, deflation.jl:514-548). Families 1–5, Static,
- * Backtracking and DogLeg, plain, mixed and refined; a Newton with best = true is refused as the
- * reference refuses it (deflation.jl:594-604); reuse = k has no deflated device form (GS_EFAMILY). */
+ * Backtracking and DogLeg, plain, mixed and refined, reuse = k under Static and Backtracking (the
+ * refinement along a held factor takes the deflated JVP M (J v) + ⟨∇M, v⟩ F, deflation.jl:352-357);
+ * a Newton with best = true is refused as the reference refuses it (deflation.jl:594-604). */
 int gs_solve_batch_deflated_f64(const gs_method* method, const gs_family* family, const gs_deflation* deflation,
                                 int64_t nsys, double* X, const void* P, int64_t p_stride, gs_status_f64* S,
                                 uint8_t* codes, const int64_t* order, const gs_options_f64* options,
@@ -301,9 +302,9 @@ typedef struct gs_chart {
 
 /* solve_batch! on Reparametrized(NonlinearProblem(family), chart) (reparametrized.jl:981-1031, both
  * forms): X holds the chart coordinates y₀ on entry and receives x* = g(y*) (:860-865); S or codes
- * as for gs_solve_batch_f64. Families 1–5 with the linear chart, 1–2 with the polar chart; Static
- * and Backtracking, plain, mixed and refined, Float64; DogLeg and reuse = k have no device form
- * here (GS_EFAMILY). */
+ * as for gs_solve_batch_f64. Families 1–5 with the linear chart, 1–2 with the polar chart; Static,
+ * Backtracking and DogLeg, plain, mixed and refined, reuse = k (not with DogLeg; the JVP along a
+ * held factor is J̃(y) v, reparametrized.jl:373-395), Float64. */
 int gs_solve_batch_reparametrized_f64(const gs_method* method, const gs_family* family, const gs_chart* chart,
                                       int64_t nsys, double* X, const void* P, int64_t p_stride, gs_status_f64* S,
                                       uint8_t* codes, const int64_t* order, const gs_options_f64* options,

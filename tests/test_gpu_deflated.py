@@ -50,6 +50,12 @@ METHODS = {
     "dogleg": lambda gs: gs.Newton(globalization=gs.DogLeg()),
     "mixed-dogleg": lambda gs: gs.Newton(linsolve=gs.LUFactorization("float32"), globalization=gs.DogLeg(radius=2.0),
                                          refinement=gs.Refinement()),
+    # reuse = k: the refinement along a held factor takes the deflated JVP M (J v) + ⟨∇M, v⟩ F
+    "reuse3": lambda gs: gs.Newton(reuse=3),
+    "reuse2-backtracking-refine": lambda gs: gs.Newton(reuse=2, globalization=gs.Backtracking(),
+                                                       refinement=gs.Refinement()),
+    "mixed-reuse3-refine": lambda gs: gs.Newton(linsolve=gs.LUFactorization("float32"), reuse=3,
+                                                refinement=gs.Refinement()),
 }
 
 
@@ -133,15 +139,13 @@ def test_one_deflation_for_a_batch(gs, orc, device):
 
 
 def test_refusals(gs, device):
-    """best = true is refused as the reference refuses it; reuse and a Float32 batch have no deflated
-    device form."""
+    """best = true is refused as the reference refuses it; a Float32 batch has no deflated device
+    form."""
     fam = gs.Coupled2()
     defl = gs.Deflated(gs.NonlinearProblem(fam), ((1.0, 2.0),))
     X = torch.zeros((4, 2), dtype=torch.float64, device=device)
     p = gs.PerSystem(torch.zeros((4, 2), dtype=torch.float64, device=device))
     with pytest.raises(gs.ArgumentError, match="best = true"):
         gs.solve_batch(X, defl, gs.Newton(best=True), p)
-    with pytest.raises(gs.ArgumentError, match="no reuse"):
-        gs.solve_batch(X, defl, gs.Newton(reuse=2), p)
     with pytest.raises(gs.ArgumentError, match="element type of its roots"):
         gs.solve_batch(X.float(), defl, gs.Newton(), gs.PerSystem(torch.zeros((4, 2), device=device)))

@@ -41,6 +41,14 @@ METHODS = {
     "refine-regularized": lambda gs: gs.Newton(refinement=gs.Refinement(), regularization=1e-6),
     "mixed-refine": lambda gs: gs.Newton(linsolve=gs.LUFactorization("float32"), refinement=gs.Refinement()),
     "best": lambda gs: gs.Newton(best=True),
+    "dogleg": lambda gs: gs.Newton(globalization=gs.DogLeg()),
+    "mixed-dogleg-refine": lambda gs: gs.Newton(linsolve=gs.LUFactorization("float32"),
+                                                globalization=gs.DogLeg(radius=2.0), refinement=gs.Refinement()),
+    "reuse3": lambda gs: gs.Newton(reuse=3),
+    "reuse2-backtracking-refine": lambda gs: gs.Newton(reuse=2, globalization=gs.Backtracking(),
+                                                       refinement=gs.Refinement()),
+    "mixed-reuse3-refine": lambda gs: gs.Newton(linsolve=gs.LUFactorization("float32"), reuse=3,
+                                                refinement=gs.Refinement()),
 }
 
 
@@ -93,7 +101,8 @@ def test_polar_chart_and_tanh_to_tolerance(gs, orc, workloads, device):
     Y0 = np.column_stack([r0 + rng.uniform(-1.5, 1.5, n), math.pi / 4 + rng.uniform(-0.3, 0.3, n)])
     p = gs.PerSystem(np.tile([10.0, 10.0], (n, 1)))
     rep = gs.Reparametrized(gs.NonlinearProblem(gs.Coupled2()), gs.PolarChart())
-    for m in (gs.Newton(), gs.Newton(globalization=gs.Backtracking())):
+    for m in (gs.Newton(), gs.Newton(globalization=gs.Backtracking()), gs.Newton(globalization=gs.DogLeg()),
+              gs.Newton(reuse=3, refinement=gs.Refinement())):
         opts = gs.Options(max_iterations=40, **TOL)
         x, st = _gpu(gs, device, rep, Y0, p, m, opts)
         xo, so = orc.solve_batch(m, gs.Coupled2(), Y0.copy(), p, opts, reparametrized=rep)
@@ -129,6 +138,6 @@ def test_refusals(gs, device):
     rep = gs.Reparametrized(gs.NonlinearProblem(gs.TanhLinear3()), gs.PolarChart())
     with pytest.raises(gs.ArgumentError, match="polar chart maps 2"):
         gs.solve_batch(X, rep, gs.Newton(), p)
-    rep = gs.Reparametrized(gs.NonlinearProblem(gs.TanhLinear3()), gs.LinearChart(np.eye(3)))
-    with pytest.raises(gs.ArgumentError, match="Static or Backtracking"):
-        gs.solve_batch(X, rep, gs.Newton(globalization=gs.DogLeg()), p)
+    with pytest.raises(gs.ArgumentError, match="takes no reuse"):
+        gs.solve_batch(X, gs.Reparametrized(gs.NonlinearProblem(gs.TanhLinear3()), gs.LinearChart(np.eye(3))),
+                       gs.Newton(globalization=gs.DogLeg(), reuse=2), p)
