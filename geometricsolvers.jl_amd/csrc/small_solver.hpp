@@ -91,8 +91,12 @@ template <class TF, int N> GS_DEV int lu_inplace(TF (&A)[N][N], int (&piv)[N]) {
         if (akk == TF(0)) {
             if (info == 0) info = k + 1;
         } else {
+            // a / 1 is a itself (IEEE division by one is exact): a wave whose pivots in this column
+            // are all exactly 1 keeps the column as its multipliers
+            if (!__all(akk == TF(1))) {
 #pragma unroll
-            for (int i = k + 1; i < N; ++i) A[i][k] = A[i][k] / akk;
+                for (int i = k + 1; i < N; ++i) A[i][k] = A[i][k] / akk;
+            }
 #pragma unroll
             for (int j = k + 1; j < N; ++j)
 #pragma unroll
@@ -125,7 +129,8 @@ template <class TF, int N> GS_DEV void lu_backsolve(const TF (&A)[N][N], const i
     }
 #pragma unroll
     for (int k = N - 1; k >= 0; --k) {
-        TF xk = x[k] / A[k][k];
+        TF xk = x[k];
+        if (!__all(A[k][k] == TF(1))) xk = xk / A[k][k];  // x / 1 is x itself
         x[k] = xk;
 #pragma unroll
         for (int i = 0; i < k; ++i) x[i] = x[i] - A[i][k] * xk;
