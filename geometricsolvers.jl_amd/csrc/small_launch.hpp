@@ -25,7 +25,7 @@ template <class T> struct KernelArgsT {
     void* S;  // gs_status_f64 or gs_status_f32
     uint8_t* codes;
     const int64_t* order;
-    int32_t stage = 0;  // 1: statuses of whole waves go through LDS (set by launch_diag)
+    int32_t stage = 0;  // 1: rows and statuses of whole waves go through LDS (set by launch_diag)
 };
 using KernelArgs = KernelArgsT<double>;
 
@@ -93,11 +93,40 @@ template <class T> constexpr int status_stage_bytes() {
     return 64 * StatusWords<T>::kCount * int(sizeof(typename StatusWords<T>::W));
 }
 
+// LDS bytes per wave of the staged stores of a lane-per-system kernel: the statuses, then the rows
+// of X, one after the other in the same block.
+template <class T, int N> constexpr int small_stage_bytes() {
+    return status_stage_bytes<T>() > 64 * N * int(sizeof(T)) ? status_stage_bytes<T>() : 64 * N * int(sizeof(T));
+}
+
 // The statuses of a wave's 64 consecutive systems, written as one contiguous span of 64 × 72 (or 44)
 // bytes: each lane puts its words in the wave's LDS block, and the wave then stores the block in
 // 16-byte pieces of consecutive lanes, so that every store instruction covers whole cache lines
 // (a lane's own 72-byte record, stored word by word, makes each of the nine store instructions touch
 // all 36 lines of the span). `lds` and `dst` are 16-byte aligned.
+// The rows of X of a wave's 64 consecutive systems (N entries each) as one contiguous span, through
+// LDS like store_status_wave; `lds` and `dst` are 16-byte aligned.
+template <class T, int N> GS_DEV void store_rows_wave(T* lds, T* dst, const T (&v)[N]) {
+    const int lane = int(threadIdx.x & 63u);
+#pragma unroll
+    for (int k = 0; k < N; ++k) lds[lane * N + k] = v[k];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    constexpr int kPieces = 64 * N * int(sizeof(T)) / 16;
+    const uint4* src = reinterpret_cast<const uint4*>(lds);
+    uint4* out = reinterpret_cast<uint4*>(dst);
+#pragma unroll
+    for (int r = 0; r < (kPieces + 63) / 64; ++r) {
+        const int c = lane + 64 * r;
+        if (kPieces % 64 == 0 || c < kPieces) out[c] = src[c];
+    }
+    // the block's reads are done before the wave writes it again
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <class W, int NW> GS_DEV void store_status_wave(W* lds, void* dst, const W (&w)[NW]) {
     const int lane = int(threadIdx.x & 63u);
 #pragma unroll
@@ -134,9 +163,10 @@ void solve_small_kernel(const KernelArgsT<typename Fam::T> a) {
     using Status = typename std::conditional<sizeof(T) == 8, gs_status_f64, gs_status_f32>::type;
     constexpr int N = Fam::N;
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    // A wave of 64 consecutive systems in natural order writes its statuses through LDS
-    // (store_status_wave); the launch sizes the LDS for it exactly when the status buffer is 16-byte
-    // aligned, the batch has no `order` and the workgroup is whole waves (a.stage).
+    // A wave of 64 consecutive systems in natural order writes its rows of X and its statuses through
+    // LDS (store_rows_wave, store_status_wave); the launch sizes the LDS for it exactly when X and the
+    // status buffer are 16-byte aligned, the batch has no `order` and the workgroup is whole waves
+    // (a.stage).
     const bool staged = DIAG && a.stage && int64_t(blockIdx.x) * blockDim.x + (threadIdx.x & ~63u) + 64 <= a.nsys;
     if (i >= a.nsys) return;
     const int64_t idx = a.order ? a.order[i] - 1 : i;
@@ -165,15 +195,13 @@ void solve_small_kernel(const KernelArgsT<typename Fam::T> a) {
     for (int k = 0; k < N; ++k) x[k] = xp[k];
     SmallSolve<Fam, GLOB, MIXED, REFINE, DIAG, REUSE> solver(fam, a.q);
     auto R = solver.solve(x);
+    T xo[N];
     if constexpr (maps_output<Fam>::value) {
         // the batch receives x* = g(y*) of a Reparametrized problem's chart
-        T xo[N];
         fam.output(x, xo);
-#pragma unroll
-        for (int k = 0; k < N; ++k) xp[k] = xo[k];
     } else {
 #pragma unroll
-        for (int k = 0; k < N; ++k) xp[k] = x[k];
+        for (int k = 0; k < N; ++k) xo[k] = x[k];
     }
     if (DIAG) {
         typename StatusWords<T>::W w[StatusWords<T>::kCount];
@@ -181,13 +209,18 @@ void solve_small_kernel(const KernelArgsT<typename Fam::T> a) {
                     R.contracted, R.reused);
         if (staged) {
             extern __shared__ __align__(16) unsigned char gs_status_stage[];
-            auto* lds = reinterpret_cast<typename StatusWords<T>::W*>(gs_status_stage +
-                                                                     (threadIdx.x >> 6) * status_stage_bytes<T>());
-            store_status_wave(lds, static_cast<Status*>(a.S) + (i - int64_t(threadIdx.x & 63u)), w);
+            unsigned char* blk = gs_status_stage + (threadIdx.x >> 6) * small_stage_bytes<T, N>();
+            const int64_t i0 = i - int64_t(threadIdx.x & 63u);
+            store_rows_wave(reinterpret_cast<T*>(blk), a.X + i0 * N, xo);
+            store_status_wave(reinterpret_cast<typename StatusWords<T>::W*>(blk), static_cast<Status*>(a.S) + i0, w);
         } else {
+#pragma unroll
+            for (int k = 0; k < N; ++k) xp[k] = xo[k];
             store_status_lane(static_cast<Status*>(a.S) + idx, w);
         }
     } else {
+#pragma unroll
+        for (int k = 0; k < N; ++k) xp[k] = xo[k];
         a.codes[idx] = R.code;
     }
 }
@@ -199,8 +232,8 @@ int launch_diag(const KernelArgsT<typename Fam::T>& a, bool diag, int wg, hipStr
     if (diag) {
         KernelArgsT<typename Fam::T> b = a;
         b.stage = !holds_state<Fam>::value && a.order == nullptr && wg % 64 == 0 &&
-                  reinterpret_cast<uintptr_t>(a.S) % 16 == 0;
-        const size_t lds = b.stage ? size_t(wg / 64) * status_stage_bytes<typename Fam::T>() : 0;
+                  reinterpret_cast<uintptr_t>(a.S) % 16 == 0 && reinterpret_cast<uintptr_t>(a.X) % 16 == 0;
+        const size_t lds = b.stage ? size_t(wg / 64) * small_stage_bytes<typename Fam::T, Fam::N>() : 0;
         hipLaunchKernelGGL((solve_small_kernel<Fam, GLOB, MIXED, REFINE, true, REUSE>), dim3(unsigned(blocks)), dim3(wg),
                            lds, st, b);
     }

@@ -294,10 +294,11 @@ struct SmallSolve {
     T merit;       // norm2(Fx) for Backtracking
     T sfnorm;      // ‖Fx‖ that the state carries
     T gradratio;   // diagnostics of the last factorisation
-    // The point of the last factorisation's Jacobian and the ‖F‖ its diagnostics read: only the last
-    // factorisation's gradient ratio reaches the status, so it is formed once, at exit, from F and J
-    // evaluated again at that point (F and J are functions of x alone, so the values are the same).
-    T jx[N];
+    // The point of the last factorisation's Jacobian, the residual there and the ‖F‖ its diagnostics
+    // read: only the last factorisation's gradient ratio reaches the status, so it is formed once, at
+    // exit, from that residual and J evaluated again at that point (J is a function of x alone, so
+    // the values are the same).
+    T jx[N], jF[N];  // jF: F at jx, the residual the factorisation read
     T jfn;
     bool have_j;
     bool promoted;      // mixed precision: factorising in FP64
@@ -539,7 +540,10 @@ struct SmallSolve {
             if (DIAG) {
                 jfn = residual_norm(Fx);
 #pragma unroll
-                for (int i = 0; i < N; ++i) jx[i] = x[i];
+                for (int i = 0; i < N; ++i) {
+                    jx[i] = x[i];
+                    jF[i] = Fx[i];
+                }
                 have_j = true;
             }
             fam.J(x, J);
@@ -1196,11 +1200,11 @@ struct SmallSolve {
             }
         }
         if (DIAG && !EAGER && have_j) {
-            // grad_ratio_and_norm (newton.jl:1287-1298) of the last factorisation
-            T Fj[N], Jj[N][N];
-            fam.F(jx, Fj);
+            // grad_ratio_and_norm (newton.jl:1287-1298) of the last factorisation: the residual it
+            // read is kept (jF), the Jacobian evaluated again at its point
+            T Jj[N][N];
             fam.J(jx, Jj);
-            gradratio = grad_ratio<N>(Jj, Fj, jfn);
+            gradratio = grad_ratio<N>(Jj, jF, jfn);
         }
         R.gradratio = gradratio;
         R.reused = REUSE && reused && R.iterations > 0;  // reused_factor (newton.jl:1188-1191)

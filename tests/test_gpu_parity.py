@@ -394,3 +394,22 @@ def test_status_store_paths_agree(gs, workloads, device):
     gs.solve_batch(X, S, gs.NonlinearProblem(gs.HenonHeilesMidpoint()), gs.Newton(),
                    gs.BatchParams(0.1, torch.as_tensor(xk, device=device)), opts)
     assert np.array_equal(X.cpu().numpy(), ref[0]) and same_bytes(gs.statuses(S), ref[1])
+
+
+def test_row_store_paths_agree(gs, workloads, device):
+    """Rows of X of whole waves go through LDS only where X is 16-byte aligned: a batch of 3-entry
+    rows (24 bytes) one row into an allocation takes the lane-by-lane store, with the same bytes."""
+    rec, X0 = workloads.tanh_linear3_batch(3000, 41)
+    opts = gs.Options(f_abstol=1e-12, f_reltol=0.0)
+    prob = gs.NonlinearProblem(gs.TanhLinear3())
+    p = gs.PerSystem(torch.as_tensor(rec, device=device))
+    Xa = torch.as_tensor(X0, device=device).clone()
+    Sa = gs.status_buffer(3000, device)
+    gs.solve_batch(Xa, Sa, prob, gs.Newton(), p, opts)
+    big = torch.empty((3001, 3), dtype=torch.float64, device=device)
+    Xu = big[1:]
+    assert Xu.data_ptr() % 16 == 8
+    Xu.copy_(torch.as_tensor(X0, device=device))
+    Su = gs.status_buffer(3000, device)
+    gs.solve_batch(Xu, Su, prob, gs.Newton(), p, opts)
+    assert torch.equal(Xa, Xu) and same_bytes(gs.statuses(Sa), gs.statuses(Su))
