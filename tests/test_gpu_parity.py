@@ -413,3 +413,23 @@ def test_row_store_paths_agree(gs, workloads, device):
     Su = gs.status_buffer(3000, device)
     gs.solve_batch(Xu, Su, prob, gs.Newton(), p, opts)
     assert torch.equal(Xa, Xu) and same_bytes(gs.statuses(Sa), gs.statuses(Su))
+
+
+def test_record_load_paths_agree(gs, workloads, device):
+    """The parameter records of whole waves arrive through LDS only where they are 16-byte aligned
+    with a 16-byte multiple stride: records one double into an allocation take the lane-by-lane
+    loads, with the same bytes out."""
+    rec, X0 = workloads.tanh_linear3_batch(3000, 43)
+    opts = gs.Options(f_abstol=1e-12, f_reltol=0.0)
+    prob = gs.NonlinearProblem(gs.TanhLinear3())
+    out = []
+    for shift in (0, 1):
+        flat = torch.empty(3000 * 12 + shift, dtype=torch.float64, device=device)
+        R = flat[shift:].view(3000, 12)
+        R.copy_(torch.as_tensor(rec, device=device))
+        assert (R.data_ptr() % 16 == 0) == (shift == 0)
+        X = torch.as_tensor(X0, device=device).clone()
+        S = gs.status_buffer(3000, device)
+        gs.solve_batch(X, S, prob, gs.Newton(), gs.PerSystem(R), opts)
+        out.append((X, gs.statuses(S)))
+    assert torch.equal(out[0][0], out[1][0]) and same_bytes(out[0][1], out[1][1])
