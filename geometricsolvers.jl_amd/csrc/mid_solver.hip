@@ -601,6 +601,7 @@ struct Info {
     double fnorm, stepnorm;
     int32_t failures;
     bool promoted;
+    bool shortened = false;  // a line search took less than the full step
 };
 
 struct NewtonState {
@@ -612,7 +613,26 @@ struct NewtonState {
     double best_fnorm;
     double jfnorm;     // the ‖F‖ the last factorisation's diagnostics read (its point is s.jx)
     bool have_j;
+    double merit;      // norm2(Fx) under Backtracking (the line search's φ), NaN under Static
 };
+
+// residual_norm (newton.jl:1274-1278): the state's ‖F‖ under Static; under Backtracking √merit
+// where the merit is finite and at least n·floatmin, else rnorm(F).
+template <int GLOB> GS_DEV double state_fnorm(MidShared& s, const NewtonState& st) {
+    if (GLOB == kBacktracking) {
+        if (isfin(st.merit) && st.merit >= double(kN) * kDblMin) return __builtin_sqrt(st.merit);
+        return fold_rnorm(s.Fx);
+    }
+    return isnan_(st.fnorm) ? fold_rnorm(s.Fx) : st.fnorm;
+}
+
+// rdot (reductions.jl:77-85) of two LDS vectors: a left fold of products, no FMA.
+GS_DEV double fold_dot(const double* a, const double* b) {
+    double t = a[0] * b[0];
+#pragma unroll 8
+    for (int i = 1; i < kN; ++i) t = t + a[i] * b[i];
+    return t;
+}
 
 // globalize!! Static (linefunction.jl:114-131): x, Fx updated on success.
 GS_DEV Info globalize_static(MidShared& s, const Fpu& f, const DevParams& q) {
@@ -631,6 +651,85 @@ GS_DEV Info globalize_static(MidShared& s, const Fpu& f, const DevParams& q) {
     }
     sync();
     return Info{kSuccess, fnp, stepnorm, 0, false};
+}
+
+// globalize_merit!! for Backtracking (linefunction.jl:159-199 → linesearch.jl:154-171 →
+// backtracking.jl:354-390), as the lane-per-system step_backtracking: `exact` takes ExactStep's slope
+// −2φ₀, otherwise MeasuredSlope's φ′(0) = 2⟨F, J d⟩ at x + 0·d. Trials in s.t1 (point) and s.t2
+// (residual); x, Fx and the merit updated where a step is taken.
+GS_DEV Info globalize_backtracking(MidShared& s, const Fpu& f, const DevParams& q, NewtonState& st, bool exact) {
+    const int l = lane_id();
+    const double phi0 = st.merit;
+    const double dn = fold_rnorm(s.d);
+    if (!isfin(dn)) return Info{kNonfinite, fold_rnorm(s.Fx), dn, 0, false};
+    double d0;
+    if (exact) {
+        d0 = -2.0 * phi0;
+    } else {
+        if (l < kN) s.t1[l] = s.x[l] + 0.0 * s.d[l];
+        sync();
+        eval_F(s, f, s.t1, s.t2);
+        eval_K(s, f, s.t1);
+        jmatvec(s, f, 0.0, s.d, s.t3);
+        d0 = 2.0 * fold_dot(s.t2, s.t3);
+    }
+    if (!(isfin(phi0) && isfin(d0) && d0 < 0.0)) {
+        const uint8_t c = !(isfin(phi0) && isfin(d0)) ? kNonfinite : (d0 > 0.0 ? kLsFailed : kStalled);
+        // a LINESEARCH_FAILED anchor keeps x with φ = NaN: the failure takes no step
+        if (c == kLsFailed) return Info{kLsFailed, fold_rnorm(s.Fx), 0.0, 1, false};
+        return Info{c, fold_rnorm(s.Fx), 0.0, 0, false};
+    }
+    const double tau = __builtin_fmax(4.0 * kDblEps * __builtin_fabs(phi0), 4.0 * kDblTiny);
+    const double amin = jmax(tau / __builtin_fabs(d0), kDblMin);
+    double alpha = __builtin_fmin(__builtin_fmax(1.0, amin), __builtin_inf());
+    double aa = alpha, pa = dnan(), ap = dnan(), pp = dnan();
+    int frozen = 0;
+    uint8_t code = kLsFailed;
+    const double sqrteps = Lim<double>::sqrteps;
+    for (int trips = 0; trips < q.bt_maxiter; ++trips) {
+        aa = alpha;
+        if (l < kN) s.t1[l] = s.x[l] + alpha * s.d[l];
+        sync();
+        eval_F(s, f, s.t1, s.t2);
+        pa = fold_norm2(s.t2);
+        frozen = (pa == phi0 && alpha <= sqrteps) ? frozen + 1 : 0;
+        if (pa - phi0 <= jmin(0.0, q.c1 * alpha * d0 + tau)) {
+            code = (isfin(pa) && pa - phi0 < -tau) ? kSuccess
+                                                   : (__builtin_fabs(pa - phi0) <= tau ? kStalled : kLsFailed);
+            break;
+        } else if (frozen >= 2) {
+            code = kStalled;
+            break;
+        } else if (alpha <= amin) {
+            code = __builtin_fabs(pa - phi0) <= tau ? kStalled : kLsFailed;
+            break;
+        }
+        const double an = isfin(pa) ? backtrack_next<double>(phi0, d0, alpha, pa, ap, pp, q.p)
+                                    : jmin(0.1 * alpha, __builtin_sqrt(alpha) * __builtin_sqrt(amin));
+        ap = alpha;
+        pp = pa;
+        alpha = jmax(an, amin);
+    }
+    const bool take = code == kSuccess || (code == kLsFailed && isfin(pa) && pa < phi0);
+    if (take) {
+        // the last trial is x + aa d and its residual
+        if (l < kN) {
+            s.x[l] = s.t1[l];
+            s.Fx[l] = s.t2[l];
+        }
+        sync();
+        st.merit = pa;
+        Info r{code, __builtin_sqrt(pa), aa * dn, code == kLsFailed ? 1 : 0, false};
+        r.shortened = aa < 1.0;
+        return r;
+    }
+    if (code == kLsFailed) return Info{kLsFailed, fold_rnorm(s.Fx), 0.0, 1, false};
+    return Info{code == kStalled ? kStalled : kNonfinite, fold_rnorm(s.Fx), 0.0, 0, false};
+}
+
+template <int GLOB> GS_DEV Info globalize(MidShared& s, const Fpu& f, const DevParams& q, NewtonState& st, bool exact) {
+    if (GLOB == kBacktracking) return globalize_backtracking(s, f, q, st, exact);
+    return globalize_static(s, f, q);
 }
 
 // The step's direction d = −(J \ F) into s.d from a register factor, then — when `refine` holds —
@@ -702,8 +801,8 @@ GS_DEV int factor_and_direction(MidShared& s, const Fpu& f, const DevParams& q, 
 // newton_factor (newton.jl:1302-1311) up to the factorisation: J at x. Only the last
 // factorisation's diagnostics reach the status, so the point and the ‖F‖ they read are kept and
 // exit_diag forms them once (F and J are functions of x alone: the values are the same).
-GS_DEV void factor_diag(MidShared& s, const Fpu& f, NewtonState& st) {
-    st.jfnorm = isnan_(st.fnorm) ? fold_rnorm(s.Fx) : st.fnorm;
+template <int GLOB> GS_DEV void factor_diag(MidShared& s, const Fpu& f, NewtonState& st) {
+    st.jfnorm = state_fnorm<GLOB>(s, st);
     st.have_j = true;
     const int l = lane_id();
     if (l < kN) s.jx[l] = s.x[l];
@@ -719,13 +818,14 @@ GS_DEV void exit_diag(MidShared& s, const Fpu& f, NewtonState& st) {
 }
 
 // newton_step!! plain (newton.jl:1679-1688).
-template <bool REFINE, bool EXACT>
+template <bool REFINE, bool EXACT, int GLOB>
 GS_DEV Info plain_step(MidShared& s, const Fpu& f, const DevParams& q, NewtonState& st) {
-    factor_diag(s, f, st);
+    factor_diag<GLOB>(s, f, st);
     bool failed;
     const int info = factor_and_direction<double, REFINE, EXACT>(s, f, q, true, failed);
     if (info == kRedoInfo) return Info{kRedo, 0.0, 0.0, 0, false};
-    const Info r = info > 0 ? Info{kSingular, fold_rnorm(s.Fx), 0.0, 0, false} : globalize_static(s, f, q);
+    const Info r = info > 0 ? Info{kSingular, fold_rnorm(s.Fx), 0.0, 0, false}
+                            : globalize<GLOB>(s, f, q, st, q.mu == 0.0 && !REFINE);
     st.fnorm = r.fnorm;
     return r;
 }
@@ -743,14 +843,14 @@ GS_DEV double linear_residual_ratio(MidShared& s, const Fpu& f, const DevParams&
 // mixed_step!! + mixed_newton!! with fresh factors (newton.jl:1604-1666, 1692-1720). The FP64
 // factor serves both a state promoted before the step (with refinement) and the promotion after a
 // failed FP32 step (promote_factor!!, newton.jl:1332-1339: no refinement), from one site.
-template <bool REFINE, bool EXACT>
+template <bool REFINE, bool EXACT, int GLOB>
 GS_DEV Info mixed_step(MidShared& s, const Fpu& f, const DevParams& q, NewtonState& st) {
-    const double fnorm_old = isnan_(st.fnorm) ? fold_rnorm(s.Fx) : st.fnorm;
+    const double fnorm_old = state_fnorm<GLOB>(s, st);
     const bool was_promoted = st.promoted;
     bool ok = true;
     int32_t failures = 0;
     double lin_rel = 0.0;
-    factor_diag(s, f, st);
+    factor_diag<GLOB>(s, f, st);
     if (!was_promoted) {
         bool dir_failed;
         const int info32 = factor_and_direction<float, REFINE, EXACT>(s, f, q, true, dir_failed);
@@ -779,7 +879,7 @@ GS_DEV Info mixed_step(MidShared& s, const Fpu& f, const DevParams& q, NewtonSta
         return r;
     }
     const bool promoted_step = md_promoted && !was_promoted;
-    Info r = globalize_static(s, f, q);
+    Info r = globalize<GLOB>(s, f, q, st, q.mu == 0.0 && md_promoted);
     r.failures += failures;
     if (promoted_step) {
         r.promoted = true;
@@ -829,7 +929,8 @@ GS_DEV void store_status_mid(gs_status_f64* dst, uint8_t code, int32_t iteration
 // the EXACT kernel, launched after it over the same grid:
 // that system keeps its input untouched and redo[idx] = 1; every other system writes redo[idx] = 0,
 // and the EXACT kernel solves only the systems marked 1, from the start.
-template <bool MIXED, bool REFINE, bool EXACT> __global__ __launch_bounds__(64) void fpu_gl3_kernel(const MidLaunch L) {
+template <bool MIXED, bool REFINE, bool EXACT, int GLOB>
+__global__ __launch_bounds__(64) void fpu_gl3_kernel(const MidLaunch L) {
     __shared__ MidShared s;
     const int l = lane_id();
     const int64_t i = blockIdx.x;
@@ -846,7 +947,8 @@ template <bool MIXED, bool REFINE, bool EXACT> __global__ __launch_bounds__(64) 
     // solve!! (solver.jl:277-378)
     eval_F(s, f, s.x, s.Fx);
     const double fnorm0 = fold_rnorm(s.Fx);
-    NewtonState st{fnorm0, dnan(), 0.0, false, 0, fnorm0, 0.0, false};
+    NewtonState st{fnorm0, dnan(), 0.0, false, 0, fnorm0, 0.0, false,
+                   GLOB == kBacktracking ? fold_norm2(s.Fx) : dnan()};
     if (q.best && l < kN) s.bx[l] = s.x[l];
     uint8_t code = kSuccess;
     int32_t it = 0, failures = 0;
@@ -869,7 +971,7 @@ template <bool MIXED, bool REFINE, bool EXACT> __global__ __launch_bounds__(64) 
     int32_t stalls = 0;
     while (!done) {
         const double rate_k = rate;
-        const Info info = MIXED ? mixed_step<REFINE, EXACT>(s, f, q, st) : plain_step<REFINE, EXACT>(s, f, q, st);
+        const Info info = MIXED ? mixed_step<REFINE, EXACT, GLOB>(s, f, q, st) : plain_step<REFINE, EXACT, GLOB>(s, f, q, st);
         if (!EXACT && info.code == kRedo) {
             if (l == 0) L.redo[idx] = 1;
             return;
@@ -888,23 +990,26 @@ template <bool MIXED, bool REFINE, bool EXACT> __global__ __launch_bounds__(64) 
         failures += info.failures;
         promoted = promoted || info.promoted;
         if (code == kSingular || code == kNonfinite || code == kMaxIters) break;
+        if (code == kLsFailed && info.stepnorm == 0.0) break;
         const double ftest = q.norm_max ? fold_rmax(s.Fx) : fnorm;
         if (it >= q.min_iterations && ftest <= jmax(q.f_abstol, q.f_reltol * ftest0)) {
             code = kSuccess;
             break;
         }
-        const double xnorm = fold_rnorm(s.x);
+        // a stalled line search passes only the residual test
+        const bool steptested = code != kStalled;
+        const double xnorm = steptested ? fold_rnorm(s.x) : dnan();
         const double xtol = jmax(q.x_abstol, q.x_reltol * xnorm);
         const bool past_min = it >= q.min_iterations;
         bool ctr = false;
-        if (q.rate_test && past_min) {
+        if (!info.shortened && q.rate_test && past_min) {
             const double th = jmax(rate, rate_k);
             ctr = stepnorm > 0.0 && th < 1.0 && th / (1.0 - th) * stepnorm <= q.kappa * xtol;
         }
         const bool step_conv = past_min && it > 0 && stepnorm > 0.0 && stepnorm <= xtol;
         const bool zero_step = code == kSuccess && info.stepnorm == 0.0 && past_min;
         const bool rstop = q.roundoff && past_min && rate >= 1.0 && rate_k < 1.0;
-        if (ctr || step_conv || zero_step || rstop) {
+        if (ctr || (steptested && (step_conv || zero_step || rstop))) {
             code = ctr ? kSuccess : kStalled;
             contracted = ctr;
             break;
@@ -962,10 +1067,15 @@ int redo_flags(size_t bytes, uint8_t** out) {
     return GS_OK;
 }
 
-template <bool MIXED, bool REFINE> int launch_variant(const MidLaunch& L, hipStream_t st) {
-    hipLaunchKernelGGL((fpu_gl3_kernel<MIXED, REFINE, false>), dim3(unsigned(L.nsys)), dim3(64), 0, st, L);
-    hipLaunchKernelGGL((fpu_gl3_kernel<MIXED, REFINE, true>), dim3(unsigned(L.nsys)), dim3(64), 0, st, L);
+template <bool MIXED, bool REFINE, int GLOB> int launch_glob_variant(const MidLaunch& L, hipStream_t st) {
+    hipLaunchKernelGGL((fpu_gl3_kernel<MIXED, REFINE, false, GLOB>), dim3(unsigned(L.nsys)), dim3(64), 0, st, L);
+    hipLaunchKernelGGL((fpu_gl3_kernel<MIXED, REFINE, true, GLOB>), dim3(unsigned(L.nsys)), dim3(64), 0, st, L);
     return GS_OK;
+}
+
+template <bool MIXED, bool REFINE> int launch_variant(const MidLaunch& L, hipStream_t st) {
+    if (L.glob == kBacktracking) return launch_glob_variant<MIXED, REFINE, kBacktracking>(L, st);
+    return launch_glob_variant<MIXED, REFINE, kStatic>(L, st);
 }
 
 }  // namespace

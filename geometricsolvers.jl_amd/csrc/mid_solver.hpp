@@ -28,12 +28,13 @@ struct MidLaunch {
     uint8_t* codes;
     const int64_t* order;
     bool mixed, refine;
+    int32_t glob;         // kStatic or kBacktracking
     uint8_t* redo;        // the hand-over flags between the two kernels, nsys bytes; null: the
                           // library's own (launch_fpu_gl3 sets them)
 };
 
-// The FPU-β Gauss–Legendre s = 3 stage system (GS_FAM_FPU_GL3, n = 48) with the Static
-// globalization; returns a gs_error code (the launch only; the caller synchronises).
+// The FPU-β Gauss–Legendre s = 3 stage system (GS_FAM_FPU_GL3, n = 48) with the Static or the
+// Backtracking globalization; returns a gs_error code (the launch only; the caller synchronises).
 int launch_fpu_gl3(const MidLaunch& L, hipStream_t st);
 
 }  // namespace gs

@@ -275,6 +275,27 @@ template <int N, class T> GS_DEV T grad_ratio(const T (&J)[N][N], const T (&Fx)[
     return grad_ratio_scaled<N>(J, Fx, jnorm, fnorm);
 }
 
+// backtrack_step (backtracking.jl:320-340): the next trial step from the last two, by quadratic then
+// cubic interpolation, kept in [0.1 a, p a].
+template <class T> GS_DEV T backtrack_next(T phi0, T d0, T a, T pa, T ap, T pp, T p) {
+    if (!isfin(pa)) return T(0.1) * a;
+    T an = tnan<T>();
+    T s = tabs(d0);
+    T r1 = (pa - phi0) / s + a;
+    if (!(ap > T(0.0))) {
+        if (r1 > T(0.0)) an = (a * a) / (T(2.0) * r1);
+    } else if (isfin(pp) && a != ap) {
+        T q1 = r1 / (a * a);
+        T q2 = ((pp - phi0) / s + ap) / (ap * ap);
+        T A = (q1 - q2) / (a - ap);
+        T B = (a * q2 - ap * q1) / (a - ap);
+        T disc = B * B + T(3.0) * A;
+        if (disc >= T(0.0)) an = B > T(0.0) ? T(1.0) / (B + tsqrt(disc)) : (-B + tsqrt(disc)) / (T(3.0) * A);
+    }
+    if (!(isfin(an) && an > T(0.0))) an = p * a;
+    return jmin(jmax(an, T(0.1) * a), p * a);
+}
+
 // ------------------------------------------------------------------------------------------------
 // One system
 // ------------------------------------------------------------------------------------------------
@@ -345,24 +366,8 @@ struct SmallSolve {
         return Step{kSuccess, false, false, 0, fn, stepnorm};
     }
 
-    // backtrack_step (backtracking.jl:320-340)
     GS_DEV static T backtrack_step(T phi0, T d0, T a, T pa, T ap, T pp, T p) {
-        if (!isfin(pa)) return T(0.1) * a;
-        T an = tnan<T>();
-        T s = tabs(d0);
-        T r1 = (pa - phi0) / s + a;
-        if (!(ap > T(0.0))) {
-            if (r1 > T(0.0)) an = (a * a) / (T(2.0) * r1);
-        } else if (isfin(pp) && a != ap) {
-            T q1 = r1 / (a * a);
-            T q2 = ((pp - phi0) / s + ap) / (ap * ap);
-            T A = (q1 - q2) / (a - ap);
-            T B = (a * q2 - ap * q1) / (a - ap);
-            T disc = B * B + T(3.0) * A;
-            if (disc >= T(0.0)) an = B > T(0.0) ? T(1.0) / (B + tsqrt(disc)) : (-B + tsqrt(disc)) / (T(3.0) * A);
-        }
-        if (!(isfin(an) && an > T(0.0))) an = p * a;
-        return jmin(jmax(an, T(0.1) * a), p * a);
+        return backtrack_next<T>(phi0, d0, a, pa, ap, pp, p);
     }
 
     // globalize_merit!! for Backtracking (linefunction.jl:159-199 → linesearch.jl:154-171 →

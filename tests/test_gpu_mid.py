@@ -156,3 +156,44 @@ def test_exact_kernel_bitwise(gs, orc, workloads, device, case):
     _assert_bitwise(z, st, zo, so)
     assert (st["iterations"] > 0).all()
     print(case, np.bincount(st["code"], minlength=6).tolist(), np.bincount(st["iterations"]).tolist())
+
+
+@pytest.mark.parametrize("case", ["exact_slope", "measured_slope", "refined", "mixed", "hostile", "short_search"])
+def test_backtracking_bitwise(gs, orc, workloads, device, case):
+    """Backtracking (backtracking.jl:354-390) on the wave-per-system solve: the plain method takes
+    ExactStep's slope −2φ₀, a regularised or refined one MeasuredSlope's 2⟨F, J d⟩, the mixed one
+    switches at its promotion (newton.jl:1244-1254). Half of the records have 40 times the amplitude,
+    where the search shortens some steps (their iterates and iteration counts then differ from
+    Static's); `hostile` adds non-finite records and nonzero starts, `short_search` caps the search at
+    2 trips with a steep sufficient-decrease constant, so searches end at LINESEARCH_FAILED. Every
+    case is compared with the oracle bit for bit."""
+    rng = np.random.default_rng(7)
+    QP = workloads.fpu_states(256, seed=8)
+    QP[0:128] *= 40.0
+    Z0 = np.zeros((256, 48))
+    bt = gs.Backtracking()
+    opts = gs.Options(max_iterations=40, **TOL)
+    if case == "exact_slope":
+        method = gs.Newton(globalization=bt)
+    elif case == "measured_slope":
+        method = gs.Newton(globalization=bt, regularization=1e-3)
+    elif case == "refined":
+        method = gs.Newton(globalization=bt, refinement=gs.Refinement())
+    elif case == "mixed":
+        method = gs.Newton(globalization=bt, linsolve=gs.LUFactorization(np.float32), refinement=gs.Refinement())
+    elif case == "hostile":
+        method = gs.Newton(globalization=bt)
+        QP[200, 3] = np.nan
+        QP[201, 20] = np.inf
+        Z0[64:192] = rng.standard_normal((128, 48)) * 2.0
+    else:
+        method = gs.Newton(globalization=gs.Backtracking(c1=0.9, maxiter=2))
+        Z0[128:] = rng.standard_normal((128, 48)) * 0.8
+    z, st = _run(gs, device, method, QP, Z0, opts)
+    zo, so = _oracle(gs, orc, method, QP, Z0, opts)
+    _assert_bitwise(z, st, zo, so)
+    if case == "exact_slope":
+        zs, ss = _oracle(gs, orc, gs.Newton(), QP, Z0, opts)
+        assert (ss["iterations"] != so["iterations"]).any() and (zs != zo).any(axis=1).any()
+    print(case, np.bincount(st["code"], minlength=8).tolist(), np.bincount(st["iterations"]).tolist(),
+          "failures", int(st["step_failures"].sum()), "promoted", int(st["promoted"].sum()))

@@ -267,7 +267,7 @@ def test_reuse_method_and_entry_checks(gs):
 
 def test_dogleg_entry_checks(gs):
     """gs_solve_batch_f64 refuses a DogLeg radius outside the constructor's clamp and DogLeg with
-    reuse (dogleg.jl:44-60, newton.jl:66-70) before any device call; FpuGl3 takes Static only."""
+    reuse (dogleg.jl:44-60, newton.jl:66-70) before any device call; FpuGl3 takes no DogLeg."""
     lib = gs.load_library()
     from importlib import import_module
 
