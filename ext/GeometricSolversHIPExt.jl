@@ -286,9 +286,9 @@ function family_records(f::FpuGl3, p::BatchParams, ::Type{Float64})
 end
 
 # What the library serves of a family beyond the method checks: the n = 48 kernel takes Static or
-# Backtracking without `reuse`, in Float64 only; anything else falls through to the generic path.
+# Backtracking, with or without `reuse`, in Float64 only; anything else falls through to the generic path.
 served(::HIPFamily, cm, ::Type{R}) where {R} = true
-served(::FpuGl3, cm, ::Type{R}) where {R} = R === Float64 && cm.globalization in (0, 1) && cm.reuse == 0
+served(::FpuGl3, cm, ::Type{R}) where {R} = R === Float64 && cm.globalization in (0, 1)
 # the dense kernel: n = 16, 24, 32, 48, 64 in Float64, Static or Backtracking, no `reuse`
 served(::DenseSin{N}, cm, ::Type{R}) where {N, R} = R === Float64 && N in (16, 24, 32, 48, 64) &&
     cm.globalization in (0, 1) && cm.reuse == 0

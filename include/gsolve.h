@@ -125,8 +125,9 @@ typedef struct gs_options_f64 {
  * LevenbergMarquardt (src/globalization/levenbergmarquardt.jl:48-98) and Refinement
  * (src/nonlinear/refinement.jl:27-49), and DogLeg (src/globalization/dogleg.jl:1-80, the trust-region
  * step of newton.jl:1725-1780). `reuse = k` (newton.jl:1781-1874) is the period of the held
- * factor of the lane-per-system solve with Static or Backtracking. `keep_factor` keeps a factor
- * across the solves of one state; a batch solves each system once, so it has no field here. */
+ * factor of the lane-per-system and the n = 48 solves with Static or Backtracking. `keep_factor`
+ * keeps a factor across the solves of one state; a batch solves each system once, so it has no
+ * field here. */
 enum gs_globalization { GS_STATIC = 0, GS_BACKTRACKING = 1, GS_LEVENBERG_MARQUARDT = 2, GS_DOGLEG = 3 };
 enum gs_factor_precision { GS_FACTOR_WORKING = 0 /* LUFactorization() */, GS_FACTOR_F32 = 1 };
 enum gs_refinement { GS_NO_REFINEMENT = 0, GS_REFINEMENT = 1 };
@@ -207,7 +208,7 @@ typedef struct gs_family {
  *   workgroupsize 0 = library default, else the workgroup size of the launch (1..1024); a size
  *            above a kernel's own bound runs as workgroups of that bound (the systems are
  *            independent, so the grouping changes no result).
- * GS_FAM_FPU_GL3 takes Static or Backtracking without reuse; GS_FAM_DENSE_SIN Static or
+ * GS_FAM_FPU_GL3 takes Static or Backtracking, with or without reuse; GS_FAM_DENSE_SIN Static or
  * Backtracking without reuse (GS_EFAMILY otherwise).
  */
 int gs_solve_batch_f64(const gs_method* method, const gs_family* family, int64_t nsys,

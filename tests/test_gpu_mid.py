@@ -197,3 +197,52 @@ def test_backtracking_bitwise(gs, orc, workloads, device, case):
         assert (ss["iterations"] != so["iterations"]).any() and (zs != zo).any(axis=1).any()
     print(case, np.bincount(st["code"], minlength=8).tolist(), np.bincount(st["iterations"]).tolist(),
           "failures", int(st["step_failures"].sum()), "promoted", int(st["promoted"].sum()))
+
+
+@pytest.mark.parametrize("glob", ["static", "backtracking"])
+@pytest.mark.parametrize("k", [2, 3])
+@pytest.mark.parametrize("form", ["plain", "refined", "mixed", "regularized"])
+def test_reuse_bitwise(gs, orc, workloads, device, glob, k, form):
+    """reuse = k (newton.jl:1781-1874) on the wave-per-system solve: the factor's rows stay in
+    registers across steps; a step along them refines with the JVP at x, a failed refinement or a
+    search that fails without a step refactorises, a step of rate ≥ 1 refreshes the factor at the
+    next step. Half of the records have 40 times the amplitude and half of the starts are nonzero,
+    so steps along the held factor fail, stall and refresh. Compared with the oracle bit for bit,
+    the `reused` flag included."""
+    QP = workloads.fpu_states(256, seed=9)
+    QP[0:128] *= 40.0
+    Z0 = np.zeros((256, 48))
+    Z0[64:192] = np.random.default_rng(2).standard_normal((128, 48)) * 0.5
+    kw = dict(globalization=gs.Backtracking() if glob == "backtracking" else gs.Static(), reuse=k)
+    if form == "refined":
+        kw["refinement"] = gs.Refinement()
+    elif form == "mixed":
+        kw.update(linsolve=gs.LUFactorization(np.float32), refinement=gs.Refinement())
+    elif form == "regularized":
+        kw["regularization"] = 1e-3
+    method = gs.Newton(**kw)
+    opts = gs.Options(max_iterations=60, **TOL)
+    z, st = _run(gs, device, method, QP, Z0, opts)
+    zo, so = _oracle(gs, orc, method, QP, Z0, opts)
+    _assert_bitwise(z, st, zo, so)
+    assert st["reused"].any()
+    print(glob, k, form, np.bincount(st["code"], minlength=8).tolist(), "failures", int(st["step_failures"].sum()),
+          "reused", int(st["reused"].sum()))
+
+
+def test_reuse_exact_kernel_bitwise(gs, orc, workloads, device):
+    """reuse = 2 on systems the second kernel solves (rows beyond 2^975 from a regularisation of
+    1e300; FP32 rows with −0 entries at h = 1e-25), against the oracle bit for bit."""
+    QP = workloads.fpu_states(128, seed=6)
+    Z0 = np.random.default_rng(1).standard_normal((128, 48)) * 0.1
+    opts = gs.Options(max_iterations=5, **TOL)
+    for h, method in ((H, gs.Newton(regularization=1e300, reuse=2)),
+                      (1e-25, gs.Newton(linsolve=gs.LUFactorization(np.float32), refinement=gs.Refinement(),
+                                        reuse=2))):
+        Z = torch.as_tensor(Z0, device=device).clone()
+        S = gs.status_buffer(len(Z0), device)
+        gs.solve_batch(Z, S, gs.NonlinearProblem(gs.FpuGl3()), method,
+                       gs.BatchParams(h, torch.as_tensor(QP, device=device)), opts)
+        torch.cuda.synchronize()
+        zo, so = orc.solve_batch(method, gs.FpuGl3(), np.array(Z0), gs.BatchParams(h, QP), opts)
+        _assert_bitwise(Z.cpu().numpy(), gs.statuses(S), zo, so)
