@@ -285,10 +285,11 @@ function family_records(f::FpuGl3, p::BatchParams, ::Type{Float64})
     return pointer(QP), Int64(256), (Float64(p.shared), f.β), QP
 end
 
-# What the library serves of a family beyond the method checks: the n = 48 kernel takes Static or
-# Backtracking, with or without `reuse`, in Float64 only; anything else falls through to the generic path.
+# What the library serves of a family beyond the method checks: the n = 48 kernel takes Static,
+# Backtracking (with or without `reuse`) or DogLeg, in Float64 only; anything else falls through
+# to the generic path.
 served(::HIPFamily, cm, ::Type{R}) where {R} = true
-served(::FpuGl3, cm, ::Type{R}) where {R} = R === Float64 && cm.globalization in (0, 1)
+served(::FpuGl3, cm, ::Type{R}) where {R} = R === Float64 && cm.globalization in (0, 1, 3)
 # the dense kernel: n = 16, 24, 32, 48, 64 in Float64, Static or Backtracking, no `reuse`
 served(::DenseSin{N}, cm, ::Type{R}) where {N, R} = R === Float64 && N in (16, 24, 32, 48, 64) &&
     cm.globalization in (0, 1) && cm.reuse == 0

@@ -153,9 +153,6 @@ static int validate_batch(const gs_method* method, const gs_family* family, int6
             return fail(GS_EINVAL, "the DogLeg radius must be 0 (nothing) or in [1e-6, 1e10]");
         if (method->reuse != 0) return fail(GS_EINVAL, "Newton with a DogLeg globalization takes no reuse");
     }
-    if (family->id == GS_FAM_FPU_GL3 && method->globalization != GS_STATIC &&
-        method->globalization != GS_BACKTRACKING)
-        return fail(GS_EFAMILY, "the wave-per-system solve of FpuGl3 takes Static or Backtracking");
     if (method->reuse < 0) return fail(GS_EINVAL, "reuse must be ≥ 1, or 0 for nothing");
     if (family->id == GS_FAM_DENSE_SIN && (method->globalization == GS_DOGLEG || method->reuse > 0))
         return fail(GS_EFAMILY, "the dense solve takes Static or Backtracking and no reuse");
@@ -390,7 +387,7 @@ static int solve_batch_f64(const gs_method* method, const gs_family* family, int
         L.order = order;
         L.mixed = mixed;
         L.refine = refine;
-        L.glob = glob == GS_BACKTRACKING ? kBacktracking : kStatic;
+        L.glob = glob == GS_BACKTRACKING ? kBacktracking : (glob == GS_DOGLEG ? kDogLeg : kStatic);
         L.reuse = method->reuse > 0;
         L.redo = scr;  // null: the library's own flags
         return finish_launch(launch_fpu_gl3(L, st), st, timed);

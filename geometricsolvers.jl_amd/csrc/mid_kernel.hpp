@@ -51,7 +51,7 @@ __constant__ double kGlA2[9] = {0x1.6c16c16c16c17p-7, -0x1.0f9922995a9d1p-7, 0x1
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 struct alignas(16) MidShared {
-    double x[kN], Fx[kN], d[kN], r[kN], t1[kN], t2[kN], t3[kN], bx[kN];
+    double x[kN], Fx[kN], d[kN], r[kN], t1[kN], t2[kN], t3[kN], t4[kN], bx[kN];
     double jx[kN];      // the point of the last factorisation's Jacobian
     double jsq[kN * 9]; // the squares of ‖J‖_F's nonzero entries, column by column
     double qp[32];      // q, p of the system
@@ -363,7 +363,7 @@ GS_DEV double jnorm_fold(MidShared& s, const Fpu& f) {
 }
 
 // Jᴴ (sc F) into out (column folds over the nonzero rows).
-GS_DEV void jtmatvec(MidShared& s, const Fpu& f, const double* v, double sc, double* out) {
+GS_DEV void jtmatvec(MidShared& s, const Fpu& f, double mu, const double* v, double sc, double* out) {
     const int l = lane_id();
     if (l < kN) {
         const int k = l % kSites;
@@ -374,7 +374,7 @@ GS_DEV void jtmatvec(MidShared& s, const Fpu& f, const double* v, double sc, dou
                 const int m = k + dm;
                 if (m < 0 || m >= kSites) continue;
                 const int i = a * kSites + m;
-                const double Jik = jentry(s, f, i, l);
+                const double Jik = jreg(s, f, mu, i, l);
                 const double vi = v[i] * sc;
                 acc = first ? Jik * vi : __builtin_fma(Jik, vi, acc);
                 first = false;
@@ -392,14 +392,14 @@ GS_DEV void grad_ratio(MidShared& s, const Fpu& f, const double* Fv, double fnor
         gr = 0.0;
         return;
     }
-    jtmatvec(s, f, Fv, 1.0, s.t3);
+    jtmatvec(s, f, 0.0, Fv, 1.0, s.t3);
     const double gn = fold_rnorm(s.t3);
     if (isfin(gn) || !(isfin(jn) && isfin(fnorm))) {
         gr = gn / jn / fnorm;
         return;
     }
     const int e = exponent_of(fnorm);
-    jtmatvec(s, f, Fv, ldexp2(1.0, -e), s.t3);
+    jtmatvec(s, f, 0.0, Fv, ldexp2(1.0, -e), s.t3);
     gr = fold_rnorm(s.t3) / jn / ldexp2(fnorm, -e);
 }
 
@@ -604,6 +604,7 @@ struct Info {
     int32_t failures;
     bool promoted;
     bool shortened = false;  // a line search took less than the full step
+    bool limited = false;    // a trust radius, not the Newton step, set the step's norm
 };
 
 struct NewtonState {
@@ -616,6 +617,7 @@ struct NewtonState {
     double jfnorm;     // the ‖F‖ the last factorisation's diagnostics read (its point is s.jx)
     bool have_j;
     double merit;      // norm2(Fx) under Backtracking (the line search's φ), NaN under Static
+    double radius;     // DogLeg: the trust radius (TrustRadius, newton.jl:637-640), 0 until set
 };
 
 // residual_norm (newton.jl:1274-1278): the state's ‖F‖ under Static; under Backtracking √merit
@@ -729,6 +731,107 @@ GS_DEV Info globalize_backtracking(MidShared& s, const Fpu& f, const DevParams& 
     return Info{code == kStalled ? kStalled : kNonfinite, fold_rnorm(s.Fx), 0.0, 0, false};
 }
 
+// DogLeg (dogleg.jl:116-206, newton.jl:1725-1780): the radius clamps of the reference.
+GS_DEV double dl_clamp(double v) { return v > 1e10 ? 1e10 : (v < 1e-6 ? 1e-6 : v); }
+GS_DEV double dl_shrink(double v) { return jmax(0.25 * v, 1e-6); }
+GS_DEV double dl_grow(double v) { return jmin(4.0 * v, 1e10); }
+
+// The trial loop of dogleg_step, as the lane-per-system dogleg_trial, on the model matrix Jreg at the
+// factorisation's point (the stiffnesses in s.K) with the Newton step s.d (read where `newton`): the
+// Newton step inside the radius, else the Cauchy point cut to the radius, else the point of norm δ
+// on the segment between them; a trial is accepted when pred > 0 and ared ≥ 1e-4 pred (unsquared
+// norms), a rejection shrinks δ to max(δ/4, 1e-6), and a rejection at 1e-6 fails. ĝ in s.r, the
+// trial step in s.t3, its point in s.t2 and residual in s.t4; x, Fx and the radius updated.
+GS_DEV Info dogleg_trial(MidShared& s, const Fpu& f, const DevParams& q, NewtonState& st, bool newton) {
+    const int l = lane_id();
+    const double lo = 1e-6;
+    const double fnorm = fold_rnorm(s.Fx);
+    if (fnorm == 0.0) return Info{kSuccess, fnorm, 0.0, 0, false};
+    const double nN = newton ? fold_rnorm(s.d) : __builtin_inf();
+    // the Cauchy direction from the unit residual: g̃ = Jregᵀ (F/‖F‖), ĝ = g̃/‖g̃‖
+    const double finv = 1.0 / fnorm;
+    jtmatvec(s, f, q.mu, s.Fx, finv, s.r);
+    const double gnorm = fold_rnorm(s.r);
+    const double ginv = 1.0 / gnorm;
+    if (l < kN) s.r[l] = ginv * s.r[l];
+    sync();
+    jmatvec(s, f, q.mu, s.r, s.t1);
+    const double jnorm = fold_rnorm(s.t1);
+    const bool cauchy = gnorm != 0.0 && isfin(gnorm) && jnorm != 0.0 && isfin(jnorm);
+    const double nC = fnorm * ((gnorm / jnorm) / jnorm);
+    double delta = st.radius;
+    if (delta == 0.0 && (newton || cauchy)) {
+        const double n0 = newton ? nN : nC;
+        delta = n0 < lo ? 2.0 * lo : dl_clamp(n0);
+    }
+    while (true) {
+        bool boundary;
+        const bool inside = newton && nN <= delta;
+        if (inside) {
+            if (l < kN) s.t3[l] = s.d[l];
+            boundary = nN >= delta;
+        } else if (!cauchy) {
+            st.radius = delta;
+            return Info{kStalled, fnorm, 0.0, 0, false};
+        } else if (!newton || nC >= delta) {
+            const double a = -jmin(nC, delta);
+            if (l < kN) s.t3[l] = a * s.r[l];
+            boundary = nC >= delta;
+        } else {
+            // the root t ≥ 0 of ‖a + t b̂‖ = 1 in units of δ, a = −(nC/δ) ĝ (s.t2), b = sN + nC ĝ (s.t1)
+            const double ac = -(nC / delta);
+            if (l < kN) {
+                s.t2[l] = ac * s.r[l];
+                s.t1[l] = 1.0 * s.d[l] + nC * s.r[l];
+            }
+            sync();
+            const double binv = 1.0 / fold_rnorm(s.t1);
+            if (l < kN) s.t1[l] = binv * s.t1[l];
+            sync();
+            const double ab = fold_dot(s.t2, s.t1);
+            const double c = 1.0 - fold_norm2(s.t2);
+            const double ab2 = ab * ab;
+            const double t = c / (ab + __builtin_sqrt(ab2 + c));
+            const double dt = delta * t;
+            if (l < kN) s.t3[l] = delta * s.t2[l] + dt * s.t1[l];
+            boundary = true;
+        }
+        sync();
+        jmatvec(s, f, q.mu, s.t3, s.t1);
+        if (l < kN) s.t1[l] = 1.0 * s.Fx[l] + 1.0 * s.t1[l];
+        sync();
+        const double pred = fnorm - fold_rnorm(s.t1);
+        if (l < kN) s.t2[l] = s.x[l] + 1.0 * s.t3[l];
+        sync();
+        eval_F(s, f, s.t2, s.t4);
+        const double fnp = fold_rnorm(s.t4);
+        const double ared = fnorm - fnp;
+        if (pred > 0.0 && ared >= 1e-4 * pred) {
+            const double rho = ared / pred;
+            if (rho < 0.1) {
+                delta = nN < delta ? jmax(nN, lo) : dl_shrink(delta);
+            } else if (rho > 0.75 && boundary) {
+                delta = dl_grow(delta);
+            }
+            const double stepnorm = fold_rnorm(s.t3);
+            if (l < kN) {
+                s.x[l] = s.t2[l];
+                s.Fx[l] = s.t4[l];
+            }
+            sync();
+            st.radius = delta;
+            Info r{kSuccess, fnp, stepnorm, 0, false};
+            r.limited = !inside;
+            return r;
+        }
+        if (delta == lo) {
+            st.radius = delta;
+            return Info{kLsFailed, fnorm, 0.0, 1, false};
+        }
+        delta = dl_shrink(delta);
+    }
+}
+
 template <int GLOB> GS_DEV Info globalize(MidShared& s, const Fpu& f, const DevParams& q, NewtonState& st, bool exact) {
     if (GLOB == kBacktracking) return globalize_backtracking(s, f, q, st, exact);
     return globalize_static(s, f, q);
@@ -829,14 +932,26 @@ GS_DEV void exit_diag(MidShared& s, const Fpu& f, NewtonState& st) {
 }
 
 // newton_step!! plain (newton.jl:1679-1688).
+// Under DogLeg (newton.jl:1725-1746, dogleg_direction): a failed factor, or a Newton step of zero or
+// non-finite norm, leaves the Cauchy step.
 template <bool REFINE, bool EXACT, int GLOB>
 GS_DEV Info plain_step(MidShared& s, const Fpu& f, const DevParams& q, NewtonState& st) {
     factor_diag<GLOB>(s, f, st);
     bool failed;
     const int info = factor_and_direction<double, REFINE, EXACT>(s, f, q, true, failed);
     if (info == kRedoInfo) return Info{kRedo, 0.0, 0.0, 0, false};
-    const Info r = info > 0 ? Info{kSingular, fold_rnorm(s.Fx), 0.0, 0, false}
-                            : globalize<GLOB>(s, f, q, st, q.mu == 0.0 && !REFINE);
+    Info r;
+    if constexpr (GLOB == kDogLeg) {
+        bool newton = false;
+        if (info == 0) {
+            const double nd = fold_rnorm(s.d);
+            newton = isfin(nd) && nd != 0.0;
+        }
+        r = dogleg_trial(s, f, q, st, newton);
+    } else {
+        r = info > 0 ? Info{kSingular, fold_rnorm(s.Fx), 0.0, 0, false}
+                     : globalize<GLOB>(s, f, q, st, q.mu == 0.0 && !REFINE);
+    }
     st.fnorm = r.fnorm;
     return r;
 }
@@ -884,13 +999,24 @@ GS_DEV Info mixed_step(MidShared& s, const Fpu& f, const DevParams& q, NewtonSta
         ok = info64 == 0;
     }
     const bool md_promoted = st.promoted;
-    if (!ok) {
-        Info r{kSingular, fold_rnorm(s.Fx), 0.0, failures, st.promoted};
-        st.fnorm = r.fnorm;
-        return r;
-    }
+    Info r;
     const bool promoted_step = md_promoted && !was_promoted;
-    Info r = globalize<GLOB>(s, f, q, st, q.mu == 0.0 && md_promoted);
+    if constexpr (GLOB == kDogLeg) {
+        // mixed_step!! with DogLeg (newton.jl:1757-1780): no factor leaves the Cauchy step
+        bool newton = false;
+        if (ok) {
+            const double nd = fold_rnorm(s.d);
+            newton = isfin(nd) && nd != 0.0;
+        }
+        r = dogleg_trial(s, f, q, st, newton);
+    } else {
+        if (!ok) {
+            r = Info{kSingular, fold_rnorm(s.Fx), 0.0, failures, st.promoted};
+            st.fnorm = r.fnorm;
+            return r;
+        }
+        r = globalize<GLOB>(s, f, q, st, q.mu == 0.0 && md_promoted);
+    }
     r.failures += failures;
     if (promoted_step) {
         r.promoted = true;
@@ -1110,7 +1236,7 @@ __global__ __launch_bounds__(64) void fpu_gl3_kernel(const MidLaunch L) {
     eval_F(s, f, s.x, s.Fx);
     const double fnorm0 = fold_rnorm(s.Fx);
     NewtonState st{fnorm0, dnan(), 0.0, false, 0, fnorm0, 0.0, false,
-                   GLOB == kBacktracking ? fold_norm2(s.Fx) : dnan()};
+                   GLOB == kBacktracking ? fold_norm2(s.Fx) : dnan(), q.dl_radius};
     if (q.best && l < kN) s.bx[l] = s.x[l];
     uint8_t code = kSuccess;
     int32_t it = 0, failures = 0;
@@ -1173,8 +1299,8 @@ __global__ __launch_bounds__(64) void fpu_gl3_kernel(const MidLaunch L) {
             code = kSuccess;
             break;
         }
-        // a stalled line search passes only the residual test
-        const bool steptested = code != kStalled;
+        // a stalled line search, or a step that a trust radius limited, passes only the residual test
+        const bool steptested = code != kStalled && !info.limited;
         const double xnorm = steptested ? fold_rnorm(s.x) : dnan();
         const double xtol = jmax(q.x_abstol, q.x_reltol * xnorm);
         const bool past_min = it >= q.min_iterations;
@@ -1226,6 +1352,16 @@ template <bool MIXED, bool REFINE, int GLOB, bool REUSE> int launch_glob_variant
 }
 
 template <bool REUSE> int launch_forms(const MidLaunch& L, hipStream_t st) {
+    if constexpr (!REUSE) {
+        // DogLeg takes no reuse (newton.jl:66-70)
+        if (L.glob == kDogLeg) {
+            if (L.mixed)
+                return L.refine ? launch_glob_variant<true, true, kDogLeg, false>(L, st)
+                                : launch_glob_variant<true, false, kDogLeg, false>(L, st);
+            return L.refine ? launch_glob_variant<false, true, kDogLeg, false>(L, st)
+                            : launch_glob_variant<false, false, kDogLeg, false>(L, st);
+        }
+    }
     const bool bt = L.glob == kBacktracking;
     if (L.mixed) {
         if (L.refine)

@@ -28,14 +28,14 @@ struct MidLaunch {
     uint8_t* codes;
     const int64_t* order;
     bool mixed, refine;
-    int32_t glob;         // kStatic or kBacktracking
+    int32_t glob;         // kStatic, kBacktracking or kDogLeg
     bool reuse;           // reuse = k (the period in q.reuse)
     uint8_t* redo;        // the hand-over flags between the two kernels, nsys bytes; null: the
                           // library's own (launch_fpu_gl3 sets them)
 };
 
-// The FPU-β Gauss–Legendre s = 3 stage system (GS_FAM_FPU_GL3, n = 48) with the Static or the
-// Backtracking globalization, with or without reuse = k; returns a gs_error code (the launch only;
+// The FPU-β Gauss–Legendre s = 3 stage system (GS_FAM_FPU_GL3, n = 48) with the Static, the
+// Backtracking (with or without reuse = k) or the DogLeg globalization; returns a gs_error code (the launch only;
 // the caller synchronises).
 int launch_fpu_gl3(const MidLaunch& L, hipStream_t st);
 

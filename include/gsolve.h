@@ -208,8 +208,8 @@ typedef struct gs_family {
  *   workgroupsize 0 = library default, else the workgroup size of the launch (1..1024); a size
  *            above a kernel's own bound runs as workgroups of that bound (the systems are
  *            independent, so the grouping changes no result).
- * GS_FAM_FPU_GL3 takes Static or Backtracking, with or without reuse; GS_FAM_DENSE_SIN Static or
- * Backtracking without reuse (GS_EFAMILY otherwise).
+ * GS_FAM_FPU_GL3 takes Static, Backtracking (with or without reuse) and DogLeg; GS_FAM_DENSE_SIN
+ * Static or Backtracking without reuse (GS_EFAMILY otherwise).
  */
 int gs_solve_batch_f64(const gs_method* method, const gs_family* family, int64_t nsys,
                        double* X, const void* P, int64_t p_stride, gs_status_f64* S,

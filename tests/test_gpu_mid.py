@@ -246,3 +246,38 @@ def test_reuse_exact_kernel_bitwise(gs, orc, workloads, device):
         torch.cuda.synchronize()
         zo, so = orc.solve_batch(method, gs.FpuGl3(), np.array(Z0), gs.BatchParams(h, QP), opts)
         _assert_bitwise(Z.cpu().numpy(), gs.statuses(S), zo, so)
+
+
+@pytest.mark.parametrize("case", ["default", "radius", "wide_radius", "refined", "mixed", "regularized", "hostile"])
+def test_dogleg_bitwise(gs, orc, workloads, device, case):
+    """DogLeg (dogleg.jl:116-206, newton.jl:1725-1780) on the wave-per-system solve: the Cauchy
+    direction from Jregᵀ F, the Newton step inside the radius, the Cauchy point or the dogleg segment
+    point, the radius updates, and the limited step that passes only the residual test. A small
+    initial radius (1e-3) forces Cauchy and segment steps at every start; a wide one (1e10) forces
+    rejected trials first. Compared with the oracle bit for bit."""
+    QP = workloads.fpu_states(256, seed=10)
+    QP[0:128] *= 40.0
+    Z0 = np.zeros((256, 48))
+    Z0[64:192] = np.random.default_rng(3).standard_normal((128, 48)) * 0.5
+    kw = dict(globalization=gs.DogLeg())
+    if case == "radius":
+        kw["globalization"] = gs.DogLeg(radius=1e-3)
+    elif case == "wide_radius":
+        kw["globalization"] = gs.DogLeg(radius=1e10)
+        Z0[0:64] = np.random.default_rng(4).standard_normal((64, 48)) * 3.0
+    elif case == "refined":
+        kw["refinement"] = gs.Refinement()
+    elif case == "mixed":
+        kw.update(linsolve=gs.LUFactorization(np.float32), refinement=gs.Refinement())
+    elif case == "regularized":
+        kw["regularization"] = 1e-3
+    elif case == "hostile":
+        QP[200, 3] = np.nan
+        Z0[0:64] = np.random.default_rng(4).standard_normal((64, 48)) * 3.0
+    method = gs.Newton(**kw)
+    opts = gs.Options(max_iterations=60, **TOL)
+    z, st = _run(gs, device, method, QP, Z0, opts)
+    zo, so = _oracle(gs, orc, method, QP, Z0, opts)
+    _assert_bitwise(z, st, zo, so)
+    print(case, np.bincount(st["code"], minlength=8).tolist(), "iterations", float(st["iterations"].mean()),
+          "failures", int(st["step_failures"].sum()))

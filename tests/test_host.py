@@ -269,7 +269,7 @@ def test_reuse_method_and_entry_checks(gs):
 
 def test_dogleg_entry_checks(gs):
     """gs_solve_batch_f64 refuses a DogLeg radius outside the constructor's clamp and DogLeg with
-    reuse (dogleg.jl:44-60, newton.jl:66-70) before any device call; FpuGl3 takes no DogLeg."""
+    reuse (dogleg.jl:44-60, newton.jl:66-70) before any device call; the dense solve takes no DogLeg."""
     lib = gs.load_library()
     from importlib import import_module
 
@@ -288,7 +288,8 @@ def test_dogleg_entry_checks(gs):
     rc = lib.gs_solve_batch_f64(ctypes.byref(m), ctypes.byref(fam), 1, None, None, 0, None, codes, None,
                                 ctypes.byref(o), 0, None)
     assert rc == abi.GS_EINVAL and "no reuse" in abi.last_error(lib)
-    fam.id, fam.n, fam.m, fam.p_len = abi.GS_FAM_FPU_GL3, 48, 48, 32
+    A = (ctypes.c_double * 256)()
+    fam.id, fam.n, fam.m, fam.p_len, fam.data = abi.GS_FAM_DENSE_SIN, 16, 16, 16, ctypes.addressof(A)
     m.reuse = 0
     rc = lib.gs_solve_batch_f64(ctypes.byref(m), ctypes.byref(fam), 1, None, None, 0, None, codes, None,
                                 ctypes.byref(o), 0, None)
