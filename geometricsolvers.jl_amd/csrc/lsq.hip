@@ -699,7 +699,12 @@ template <int N> GS_DEV void fit_store(Fit<N>& f, const LsqArgs& a) {
 // The persistent kernel of one bucket (fits with n = N). Every lane loops over LM outer steps; a lane
 // without a fit takes the next one from the bucket's queue (one atomic per wave per refill), and the
 // wave leaves the loop once the queue is empty and every lane is idle.
-template <int N, bool SCALED> __global__ __launch_bounds__(256) void lsq_kernel(const LsqArgs a) {
+// The n = 6 kernel is held to 256 registers (2 waves per SIMD, about 60 words of its state spilled
+// to scratch): measured 130 against 135-138 ms per step of the 262144-fit batch at 1 wave per SIMD
+// with 314 registers. The n = 8 kernel stays at 1 wave per SIMD: held to 256 registers it spills
+// about 230 words and runs no faster.
+template <int N, bool SCALED>
+__global__ __launch_bounds__(256, N == 6 ? 2 : 1) void lsq_kernel(const LsqArgs a) {
     // the bucket's range comes from the device-side sort, so the launch does not depend on the data;
     // a batch with an invalid fit solves nothing (the host reports the error after the launch)
     if (*a.bad != 0) return;
