@@ -290,9 +290,9 @@ end
 # to the generic path.
 served(::HIPFamily, cm, ::Type{R}) where {R} = true
 served(::FpuGl3, cm, ::Type{R}) where {R} = R === Float64 && cm.globalization in (0, 1, 3)
-# the dense kernel: n = 16, 24, 32, 48, 64 in Float64, Static or Backtracking, no `reuse`
+# the dense kernel: n = 16, 24, 32, 48, 64 in Float64, Static or Backtracking, with or without `reuse`
 served(::DenseSin{N}, cm, ::Type{R}) where {N, R} = R === Float64 && N in (16, 24, 32, 48, 64) &&
-    cm.globalization in (0, 1) && cm.reuse == 0
+    cm.globalization in (0, 1)
 
 # ---- the method and options as ABI structs ----------------------------------------------------
 

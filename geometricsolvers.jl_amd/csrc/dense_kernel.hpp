@@ -4,8 +4,8 @@
 //
 // The §10.E problem of the reference's device benchmark (benchmark/device_problems.jl:133-175),
 // F(x, b) = A x + sin.(x)/10 − b with A shared by the batch and b per system (GS_FAM_DENSE_SIN), under
-// Newton with the Static or Backtracking globalization, the static LU in FP64 or in FP32 with FP64
-// refinement and promotion: the reference's R1 loop solve!! (src/nonlinear/solver.jl:277-378) with
+// Newton with the Static or Backtracking globalization, with or without reuse = k, the static LU in
+// FP64 or in FP32 with FP64 refinement and promotion: the reference's R1 loop solve!! (src/nonlinear/solver.jl:277-378) with
 // newton_step!! (src/nonlinear/newton.jl:1302-1485, 1604-1720) per system, as the lane-per-system
 // kernels run it for n ≤ 8.
 //
@@ -305,11 +305,12 @@ template <int N> struct JacRow<N, false> {
 
 // ---- the solve of one system --------------------------------------------------------------------
 
-template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG> struct DenseSolve {
+template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = false> struct DenseSolve {
     static constexpr int N = Fam::N;
     static constexpr int NP = NP_;
     using JR = JacRow<N>;
     using Step = StepT<double>;
+    using TF = typename std::conditional<MIXED, float, double>::type;  // the held factor's type
 
     const Fam& fam;
     const DevParams& q;
@@ -322,6 +323,13 @@ template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG> stru
     bool promoted;
     int slow;
     double diag_fnorm;  // the ‖F‖ the last factorisation read; NaN before the first
+    // reuse = k: the factor the state holds (this lane's row, its logical row and the prescale) and
+    // the rule state FactorAge (newton.jl:621-633): its age, whether the last step stalled, whether
+    // the last step went along the held factor
+    TF ka[N];
+    int kpos, kfe;
+    int age;
+    bool stalled, reused;
 
     GS_DEV DenseSolve(const Fam& f, const DevParams& p, SysVecs<N>& vv, int lane)
         : fam(f), q(p), v(vv), sl(lane), live(lane < N) {}
@@ -369,6 +377,18 @@ template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG> stru
             jr.i = live ? sl : 0;
             jr.dg = fam.Jdiag(v.x, jr.i);
             jr.mu = q.mu;
+        }
+    }
+
+    // This lane's row of J at the LDS point xin, without the regularization (the JVP of F there).
+    GS_DEV void jrow_at(const double* xin, JR& ja) {
+        if constexpr (holds_jrow<N>()) {
+            fam.Jrow(xin, live ? sl : 0, ja.r);
+        } else {
+            ja.A = fam.A;
+            ja.i = live ? sl : 0;
+            ja.dg = fam.Jdiag(xin, ja.i);
+            ja.mu = 0.0;
         }
     }
 
@@ -589,14 +609,7 @@ template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG> stru
             wsync();
             eval_F(v.xt, v.Ft);
             JR ja;  // J at x + 0·d, without the regularization
-            if constexpr (holds_jrow<N>()) {
-                fam.Jrow(v.xt, live ? sl : 0, ja.r);
-            } else {
-                ja.A = fam.A;
-                ja.i = live ? sl : 0;
-                ja.dg = fam.Jdiag(v.xt, ja.i);
-                ja.mu = 0.0;
-            }
+            jrow_at(v.xt, ja);
             matvec(ja, v.d, v.t);
             d0 = 2.0 * vdot<N>(v.Ft, v.t);
         }
@@ -739,6 +752,162 @@ template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG> stru
         return s;
     }
 
+    // ---- reuse = k (newton.jl:1781-1874), as the lane-per-system reuse steps ----
+
+    GS_DEV int fresh_age() const { return q.reuse == 1 ? 0 : 1; }                // newton.jl:629
+    GS_DEV int next_age() const { return age + 1 >= q.reuse ? 0 : age + 1; }      // newton.jl:632
+
+    // reuse_result (newton.jl:1814-1820): a SINGULAR step resets the age, a STALLED one refreshes
+    // the factor at the next step.
+    GS_DEV void set_rule(uint8_t code, int age_next, bool along_held) {
+        age = code == kSingular ? 0 : age_next;
+        stalled = code == kStalled;
+        reused = along_held;
+    }
+
+    // newton_step!! with reuse in the working precision (newton.jl:1781-1810): a step along the held
+    // factor refines with the JVP at x and takes the measured slope; a failed refinement (one
+    // step-rule failure) or a line search that fails without a step refactorises at x in the step.
+    GS_DEV Step plain_step_reuse() {
+        int failures = 0;
+        if (!(age == 0 || stalled)) {
+            const int age_next = next_age();
+            JR ja;
+            jrow_at(v.x, ja);
+            if (direction<TF>(ja, ka, kpos, kfe, true)) {
+                failures = 1;
+            } else {
+                // a failed search without a step leaves x, Fx and the merit as they were
+                Step s = globalize(false);
+                if (!(s.code == kLsFailed && s.stepnorm == 0.0)) {
+                    sfnorm = s.fnorm;
+                    set_rule(s.code, age_next, true);
+                    return s;
+                }
+            }
+        }
+        JR jr;
+        jacobian(jr);
+        const int info = factor<TF>(jr, ka, kpos, kfe);
+        Step s;
+        if (info > 0) {
+            s = Step{kSingular, false, false, 0, vrnorm<N>(v.Fx), 0.0, false};
+        } else {
+            direction<TF>(jr, ka, kpos, kfe, true);
+            s = globalize(q.mu == 0.0 && !REFINE);
+        }
+        s.failures += failures;
+        sfnorm = s.fnorm;
+        set_rule(s.code, fresh_age(), false);
+        return s;
+    }
+
+    // mixed_step!! with reuse (newton.jl:1823-1874) over mixed_newton!! with store = true
+    // (newton.jl:1604-1666): a promoted state factorises in FP64 at every step; otherwise the step
+    // runs along the held FP32 factor until its age or a stall refreshes it, and a held direction
+    // whose refinement fails is refactorised in FP32 at x (one failure) before any promotion. A line
+    // search along the held factor that fails without a step is retried once from a fresh factor.
+    GS_DEV Step mixed_step_reuse() {
+        const double fnorm_old = residual_norm();
+        const bool state_promoted = promoted;
+        bool fresh = promoted || age == 0 || stalled;
+        const int age_fresh = fresh_age();
+        int age_next = fresh ? age_fresh : next_age();
+        while (true) {
+            const bool was_fresh = fresh;
+            bool ok = true, md_fresh = true, have = false;
+            int failures = 0;
+            double lin_rel = 0.0;
+            JR jr;
+            if (promoted) {
+                jacobian(jr);
+                double a[N];
+                int pos, fe;
+                if (factor<double>(jr, a, pos, fe) > 0) ok = false;
+                else direction<double>(jr, a, pos, fe, true);
+                have = true;
+            } else if (!fresh) {
+                JR ja;
+                jrow_at(v.x, ja);
+                if (!direction<TF>(ja, ka, kpos, kfe, true)) {
+                    md_fresh = false;
+                    have = true;
+                }
+            }
+            if (!have) {
+                jacobian(jr);
+                if (!fresh) failures += 1;
+                const int info = factor<TF>(jr, ka, kpos, kfe);
+                if (info > 0 || direction<TF>(jr, ka, kpos, kfe, true)) {
+                    // promote_factor!! (newton.jl:1332-1339): the held Jacobian in FP64, no refinement
+                    failures += 1;
+                    promoted = true;
+                    double a[N];
+                    int pos, fe;
+                    if (factor<double>(jr, a, pos, fe) > 0) ok = false;
+                    else direction<double>(jr, a, pos, fe, false);
+                } else if (fnorm_old != 0.0 && isfin(fnorm_old)) {
+                    // linear_residual_ratio (newton.jl:1422-1430)
+                    matvec(jr, v.d, v.t);
+                    if (live) v.u[sl] = 1.0 * v.Fx[sl] + 1.0 * v.t[sl];
+                    wsync();
+                    lin_rel = vrnorm<N>(v.u) / fnorm_old;
+                }
+            }
+            fresh = md_fresh;
+            if (!ok) {
+                set_rule(kSingular, 0, !md_fresh);
+                return Step{kSingular, promoted, false, failures, vrnorm<N>(v.Fx), 0.0, false};
+            }
+            const bool promoted_step = promoted && !state_promoted;
+            if (promoted_step) {
+                age_next = 0;
+            } else if (md_fresh && !was_fresh) {
+                age_next = age_fresh;
+            }
+            // newton_stepkind (newton.jl:1244-1254)
+            Step s = globalize(md_fresh && q.mu == 0.0 && promoted);
+            if (!md_fresh && s.code == kLsFailed && s.stepnorm == 0.0) {
+                fresh = true;
+                age_next = age_fresh;
+                continue;
+            }
+            s.failures += failures;
+            if (promoted_step) {
+                s.promoted = true;
+                slow = 0;
+            } else if (md_fresh && !promoted) {
+                // read_mixed_stall (newton.jl:1437-1451), η_lin = max(√eps, 1e-4)
+                if (s.fnorm > 0.5 * fnorm_old && lin_rel > 1e-4) {
+                    const int sw = slow + 1;
+                    if (sw >= 2) {
+                        promoted = true;
+                        slow = 0;
+                        s.failures += 1;
+                        s.promoted = true;
+                    } else {
+                        slow = sw;
+                    }
+                } else {
+                    slow = 0;
+                }
+            }
+            sfnorm = s.fnorm;
+            set_rule(s.code, age_next, !md_fresh);
+            return s;
+        }
+    }
+
+    GS_DEV Step step() {
+        if constexpr (REUSE) {
+            if constexpr (MIXED) return mixed_step_reuse();
+            else return plain_step_reuse();
+        } else {
+            if constexpr (MIXED) return mixed_step();
+            else return plain_step();
+        }
+    }
+
     // ---- the diagnostics of the last factorisation (grad_ratio_and_norm, newton.jl:1287-1298) ----
 
     // One pass over the columns of J at the saved point, column j formed in parallel (lane i its
@@ -799,7 +968,7 @@ template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG> stru
 
     struct Result {
         uint8_t code;
-        bool contracted, promoted;
+        bool contracted, promoted, reused;
         int32_t iterations, failures;
         double fnorm, stepnorm, rate, gradratio;
     };
@@ -812,9 +981,14 @@ template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG> stru
         promoted = false;
         slow = 0;
         diag_fnorm = dnan();
+        kpos = 0;
+        kfe = 0;
+        age = 0;
+        stalled = false;
+        reused = false;
         double best_fn = fnorm0;
         if (q.best) copy(v.x, v.bx);
-        Result R{kSuccess, false, false, 0, 0, fnorm0, 0.0, dnan(), dnan()};
+        Result R{kSuccess, false, false, false, 0, 0, fnorm0, 0.0, dnan(), dnan()};
         if (!isfin(fnorm0)) {
             R.code = kNonfinite;
         } else {
@@ -828,7 +1002,7 @@ template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG> stru
                 int stalls = 0;
                 while (true) {
                     const double rate_k = R.rate;
-                    Step s = MIXED ? mixed_step() : plain_step();
+                    Step s = step();
                     if (q.best && s.fnorm < best_fn) {
                         best_fn = s.fnorm;
                         copy(v.x, v.bx);
@@ -840,6 +1014,9 @@ template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG> stru
                     R.stepnorm = s.stepnorm;
                     R.failures += s.failures;
                     R.promoted |= s.promoted;
+                    // observe_rate!! (newton.jl:1205-1210): a step of rate θ ≥ 1 along a held factor
+                    // refreshes the factor at the next step
+                    if (REUSE && reused && R.rate >= 1.0) stalled = true;
                     if (s.code == kSingular || s.code == kNonfinite || s.code == kMaxIters) break;
                     if (s.code == kLsFailed && s.stepnorm == 0.0) break;
                     const double ftest = q.norm_max ? vrmax<N>(v.Fx) : R.fnorm;
@@ -886,11 +1063,12 @@ template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG> stru
             R.fnorm = best_fn;
         }
         if (DIAG) R.gradratio = grad_ratio();
+        R.reused = REUSE && reused && R.iterations > 0;  // reused_factor (newton.jl:1188-1191)
         return R;
     }
 };
 
-template <int N, int NP, int GLOB, bool MIXED, bool REFINE, bool DIAG>
+template <int N, int NP, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE>
 __global__ __launch_bounds__(kDenseThreads) void dense_kernel(const DenseLaunch L) {
     using Sh = Shape<N, NP>;
     __shared__ double sA[N * N];
@@ -911,45 +1089,47 @@ __global__ __launch_bounds__(kDenseThreads) void dense_kernel(const DenseLaunch 
     double* xp = L.X + idx * N;
     if (live) v.x[sl] = xp[sl];
     wsync();
-    DenseSolve<FamDenseSin<N>, NP, GLOB, MIXED, REFINE, DIAG> solver(fam, L.q, v, sl);
+    DenseSolve<FamDenseSin<N>, NP, GLOB, MIXED, REFINE, DIAG, REUSE> solver(fam, L.q, v, sl);
     const auto R = solver.solve();
     if (live) xp[sl] = v.x[sl];
     if (sl == 0) {
         if (DIAG) {
             store_status(L.S + idx, R.code, R.iterations, R.fnorm, R.stepnorm, R.failures, R.promoted, R.gradratio,
-                         R.rate, R.contracted, false);
+                         R.rate, R.contracted, R.reused);
         } else {
             L.codes[idx] = R.code;
         }
     }
 }
 
-template <int N, int NP, int GLOB, bool MIXED, bool REFINE>
+template <int N, int NP, int GLOB, bool MIXED, bool REFINE, bool REUSE>
 int launch_diag(bool diag, const DenseLaunch& L, hipStream_t st) {
     using Sh = Shape<N, NP>;
     const int64_t blocks = (L.nsys + Sh::SPB - 1) / Sh::SPB;
     if (blocks > 0x7fffffffLL) return set_error(GS_EINVAL, "batch too large for one launch");
     if (diag)
-        hipLaunchKernelGGL((dense_kernel<N, NP, GLOB, MIXED, REFINE, true>), dim3(unsigned(blocks)), dim3(Sh::kThreads),
-                           0, st, L);
+        hipLaunchKernelGGL((dense_kernel<N, NP, GLOB, MIXED, REFINE, true, REUSE>), dim3(unsigned(blocks)),
+                           dim3(Sh::kThreads), 0, st, L);
     else
-        hipLaunchKernelGGL((dense_kernel<N, NP, GLOB, MIXED, REFINE, false>), dim3(unsigned(blocks)),
+        hipLaunchKernelGGL((dense_kernel<N, NP, GLOB, MIXED, REFINE, false, REUSE>), dim3(unsigned(blocks)),
                            dim3(Sh::kThreads), 0, st, L);
     return GS_OK;
 }
 
-template <int N, int NP>
+// The launches of one size and segment width; REUSE: the reuse = k forms (their own translation
+// units, dense_r16.hip … dense_r64.hip).
+template <int N, int NP, bool REUSE = false>
 int launch_np(int glob, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st) {
     if (glob == kBacktracking) {
-        if (mixed) return refine ? launch_diag<N, NP, kBacktracking, true, true>(diag, L, st)
-                                 : launch_diag<N, NP, kBacktracking, true, false>(diag, L, st);
-        return refine ? launch_diag<N, NP, kBacktracking, false, true>(diag, L, st)
-                      : launch_diag<N, NP, kBacktracking, false, false>(diag, L, st);
+        if (mixed) return refine ? launch_diag<N, NP, kBacktracking, true, true, REUSE>(diag, L, st)
+                                 : launch_diag<N, NP, kBacktracking, true, false, REUSE>(diag, L, st);
+        return refine ? launch_diag<N, NP, kBacktracking, false, true, REUSE>(diag, L, st)
+                      : launch_diag<N, NP, kBacktracking, false, false, REUSE>(diag, L, st);
     }
-    if (mixed) return refine ? launch_diag<N, NP, kStatic, true, true>(diag, L, st)
-                             : launch_diag<N, NP, kStatic, true, false>(diag, L, st);
-    return refine ? launch_diag<N, NP, kStatic, false, true>(diag, L, st)
-                  : launch_diag<N, NP, kStatic, false, false>(diag, L, st);
+    if (mixed) return refine ? launch_diag<N, NP, kStatic, true, true, REUSE>(diag, L, st)
+                             : launch_diag<N, NP, kStatic, true, false, REUSE>(diag, L, st);
+    return refine ? launch_diag<N, NP, kStatic, false, true, REUSE>(diag, L, st)
+                  : launch_diag<N, NP, kStatic, false, false, REUSE>(diag, L, st);
 }
 
 }  // namespace

@@ -1,6 +1,7 @@
 // libgsolve_hip, dense mid-size systems (n = 16, 24, 32, 48, 64): the launch dispatch over the
 // sizes and their lanes per system (the kernels are in dense_kernel.hpp, instantiated one size per
-// translation unit in dense_n16.hip … dense_n64.hip), and the shape setting of the autotuner.
+// translation unit in dense_n16.hip … dense_n64.hip, and with reuse = k in dense_r16.hip …
+// dense_r64.hip), and the shape setting of the autotuner.
 #include <hip/hip_runtime.h>
 
 #include <atomic>
@@ -18,6 +19,12 @@ int launch_dense_n24(int np, int glob, bool mixed, bool refine, bool diag, const
 int launch_dense_n32(int np, int glob, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st);
 int launch_dense_n48(int np, int glob, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st);
 int launch_dense_n64(int np, int glob, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st);
+// The reuse = k launches (dense_r16.hip … dense_r64.hip).
+int launch_dense_reuse_n16(int np, int glob, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st);
+int launch_dense_reuse_n24(int np, int glob, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st);
+int launch_dense_reuse_n32(int np, int glob, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st);
+int launch_dense_reuse_n48(int np, int glob, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st);
+int launch_dense_reuse_n64(int np, int glob, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st);
 
 namespace {
 
@@ -56,8 +63,19 @@ int dense_set_shape(int n, int lanes) {
     return GS_OK;
 }
 
-int launch_dense(int n, int glob, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st) {
+int launch_dense(int n, int glob, bool mixed, bool refine, bool diag, bool reuse, const DenseLaunch& L,
+                 hipStream_t st) {
     const int np = lanes_of(n);
+    if (reuse) {
+        switch (n) {
+            case 16: return launch_dense_reuse_n16(np, glob, mixed, refine, diag, L, st);
+            case 24: return launch_dense_reuse_n24(np, glob, mixed, refine, diag, L, st);
+            case 32: return launch_dense_reuse_n32(np, glob, mixed, refine, diag, L, st);
+            case 48: return launch_dense_reuse_n48(np, glob, mixed, refine, diag, L, st);
+            case 64: return launch_dense_reuse_n64(np, glob, mixed, refine, diag, L, st);
+        }
+        return set_error(GS_EINVAL, "the dense family takes n = 16, 24, 32, 48 or 64");
+    }
     switch (n) {
         case 16: return launch_dense_n16(np, glob, mixed, refine, diag, L, st);
         case 24: return launch_dense_n24(np, glob, mixed, refine, diag, L, st);

@@ -31,9 +31,10 @@ int dense_set_shape(int n, int lanes);
 // The lanes per system the next launch at size n uses.
 int lanes_of(int n);
 
-// GS_FAM_DENSE_SIN (n = 16, 24, 32, 48 or 64) under Newton with the Static or Backtracking globalization, LU in FP64 or FP32
-// (mixed, with or without Refinement); returns a gs_error code (the launch only; the caller
-// synchronises).
-int launch_dense(int n, int glob, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st);
+// GS_FAM_DENSE_SIN (n = 16, 24, 32, 48 or 64) under Newton with the Static or Backtracking
+// globalization, with or without reuse = k, LU in FP64 or FP32 (mixed, with or without Refinement);
+// returns a gs_error code (the launch only; the caller synchronises).
+int launch_dense(int n, int glob, bool mixed, bool refine, bool diag, bool reuse, const DenseLaunch& L,
+                 hipStream_t st);
 
 }  // namespace gs

@@ -154,8 +154,8 @@ static int validate_batch(const gs_method* method, const gs_family* family, int6
         if (method->reuse != 0) return fail(GS_EINVAL, "Newton with a DogLeg globalization takes no reuse");
     }
     if (method->reuse < 0) return fail(GS_EINVAL, "reuse must be ≥ 1, or 0 for nothing");
-    if (family->id == GS_FAM_DENSE_SIN && (method->globalization == GS_DOGLEG || method->reuse > 0))
-        return fail(GS_EFAMILY, "the dense solve takes Static or Backtracking and no reuse");
+    if (family->id == GS_FAM_DENSE_SIN && method->globalization == GS_DOGLEG)
+        return fail(GS_EFAMILY, "the dense solve takes Static or Backtracking");
     if (method->factor_precision != GS_FACTOR_WORKING && method->factor_precision != GS_FACTOR_F32)
         return fail(GS_EINVAL, "factor_precision must be GS_FACTOR_WORKING or GS_FACTOR_F32");
     if (method->refinement != GS_NO_REFINEMENT && method->refinement != GS_REFINEMENT)
@@ -406,7 +406,7 @@ static int solve_batch_f64(const gs_method* method, const gs_family* family, int
         L.codes = codes;
         L.order = order;
         return finish_launch(launch_dense(family->n, glob == GS_BACKTRACKING ? kBacktracking : kStatic, mixed, refine,
-                                          diag, L, st),
+                                          diag, method->reuse > 0, L, st),
                              st, timed);
     }
     if (glob == GS_DOGLEG)

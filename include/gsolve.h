@@ -125,7 +125,7 @@ typedef struct gs_options_f64 {
  * LevenbergMarquardt (src/globalization/levenbergmarquardt.jl:48-98) and Refinement
  * (src/nonlinear/refinement.jl:27-49), and DogLeg (src/globalization/dogleg.jl:1-80, the trust-region
  * step of newton.jl:1725-1780). `reuse = k` (newton.jl:1781-1874) is the period of the held
- * factor of the lane-per-system and the n = 48 solves with Static or Backtracking. `keep_factor`
+ * factor of the lane-per-system, n = 48 and dense solves with Static or Backtracking. `keep_factor`
  * keeps a factor across the solves of one state; a batch solves each system once, so it has no
  * field here. */
 enum gs_globalization { GS_STATIC = 0, GS_BACKTRACKING = 1, GS_LEVENBERG_MARQUARDT = 2, GS_DOGLEG = 3 };
@@ -209,7 +209,7 @@ typedef struct gs_family {
  *            above a kernel's own bound runs as workgroups of that bound (the systems are
  *            independent, so the grouping changes no result).
  * GS_FAM_FPU_GL3 takes Static, Backtracking (with or without reuse) and DogLeg; GS_FAM_DENSE_SIN
- * Static or Backtracking without reuse (GS_EFAMILY otherwise).
+ * Static or Backtracking, with or without reuse (GS_EFAMILY otherwise).
  */
 int gs_solve_batch_f64(const gs_method* method, const gs_family* family, int64_t nsys,
                        double* X, const void* P, int64_t p_stride, gs_status_f64* S,

@@ -24,6 +24,12 @@ METHODS = {
                                                       globalization=gs.Backtracking(), refinement=gs.Refinement()),
     "plain-refine-regularized": lambda gs: gs.Newton(refinement=gs.Refinement(), regularization=1e-3),
     "best-backtracking": lambda gs: gs.Newton(globalization=gs.Backtracking(), best=True),
+    # reuse = k (newton.jl:1781-1874): the device grid's gs-reuse4 column, and held factors under
+    # Backtracking and in the mixed form
+    "reuse4": lambda gs: gs.Newton(reuse=4),
+    "backtracking-reuse3": lambda gs: gs.Newton(globalization=gs.Backtracking(), reuse=3),
+    "mixed-refine-reuse2": lambda gs: gs.Newton(linsolve=gs.LUFactorization("float32"), refinement=gs.Refinement(),
+                                                reuse=2),
 }
 
 

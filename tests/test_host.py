@@ -237,8 +237,8 @@ def test_dense_shape_setting_without_gpu(gs):
 
 def test_reuse_method_and_entry_checks(gs):
     """Newton(; reuse = k) crosses the ABI as gs_method.reuse; the constructor's refusals
-    (newton.jl:60-90) and the entry points' refusal of reuse where no held factor exists (the
-    least-squares batch, the dense mid-size solve) happen before any device call."""
+    (newton.jl:60-90) and the entry point's refusal of reuse where no held factor exists (the
+    least-squares batch) happen before any device call."""
     assert gs.Newton(reuse=3).to_c().reuse == 3 and gs.Newton().to_c().reuse == 0
     assert gs.Newton(reuse=2, keep_factor=True).to_c().reuse == 2
     with pytest.raises(gs.ArgumentError, match="reuse must be ≥ 1"):
@@ -259,12 +259,7 @@ def test_reuse_method_and_entry_checks(gs):
     rc = lib.gs_solve_batch_f64(ctypes.byref(gs.Newton(reuse=2).to_c()), ctypes.byref(fam), 1, None, None, 0, None,
                                 None, None, ctypes.byref(o), 0, None)
     assert rc == abi.GS_EINVAL  # neither S nor codes: refused first, before the family checks
-    codes = (ctypes.c_uint8 * 1)()
-    A = (ctypes.c_double * 256)()
-    fam.id, fam.n, fam.m, fam.p_len, fam.data = abi.GS_FAM_DENSE_SIN, 16, 16, 16, ctypes.addressof(A)
-    rc = lib.gs_solve_batch_f64(ctypes.byref(gs.Newton(reuse=2).to_c()), ctypes.byref(fam), 1, None, None, 0, None,
-                                codes, None, ctypes.byref(o), 0, None)
-    assert rc == abi.GS_EFAMILY and "no reuse" in abi.last_error(lib)
+
 
 
 def test_dogleg_entry_checks(gs):
