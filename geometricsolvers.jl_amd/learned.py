@@ -34,16 +34,24 @@ def _lib():
     return _abi.load_library()
 
 
-def _stream(t):
-    from . import _stream_ptr
+# The package's stream and error helpers, bound at first use (the package imports this module).
+_HELPERS = []
 
-    return _stream_ptr(t)
+
+def _helpers():
+    if not _HELPERS:
+        from . import _raise_on as r, _stream_ptr as sp
+
+        _HELPERS.extend((sp, r))
+    return _HELPERS
+
+
+def _stream(t):
+    return _helpers()[0](t)
 
 
 def _raise_on(rc, lib):
-    from . import _raise_on as r
-
-    r(rc, lib)
+    _helpers()[1](rc, lib)
 
 
 IN, HID, OUT = _abi.GS_MLP_IN, _abi.GS_MLP_HIDDEN, _abi.GS_MLP_OUT
@@ -86,6 +94,10 @@ class LearnedStart:
         self.loss_buf = torch.zeros(1, dtype=torch.float32, device=device)
         self.step = 0
         self.norm = {"in_shift": [0.0] * 8, "in_scale": [1.0] * 8, "out_shift": [0.0] * 4, "out_scale": [1.0] * 4}
+        # the C records of the last call: the network's (keyed by its pointers and normalisation) and
+        # the method's and options' (keyed by the identity of those frozen values)
+        self._c_key, self._c = None, None
+        self._records = (None, None, None, None)
         self.pack()
 
     def pack(self):
@@ -99,16 +111,32 @@ class LearnedStart:
         self.pack()
 
     def to_c(self) -> _abi.GsMlp:
-        c = _abi.GsMlp()
-        c.params = self.params.data_ptr()
-        c.packed = self.packed.data_ptr()
-        for j in range(8):
-            c.in_shift[j] = self.norm["in_shift"][j]
-            c.in_scale[j] = self.norm["in_scale"][j]
-        for j in range(4):
-            c.out_shift[j] = self.norm["out_shift"][j]
-            c.out_scale[j] = self.norm["out_scale"][j]
-        return c
+        """The gs_mlp record of the network: built again only when its tensors or its normalisation
+        changed since the last call (the library reads it and keeps no reference)."""
+        nm = self.norm
+        key = (self.params.data_ptr(), self.packed.data_ptr(), tuple(nm["in_shift"]), tuple(nm["in_scale"]),
+               tuple(nm["out_shift"]), tuple(nm["out_scale"]))
+        if key != self._c_key:
+            c = _abi.GsMlp()
+            c.params = self.params.data_ptr()
+            c.packed = self.packed.data_ptr()
+            for j in range(8):
+                c.in_shift[j] = nm["in_shift"][j]
+                c.in_scale[j] = nm["in_scale"][j]
+            for j in range(4):
+                c.out_shift[j] = nm["out_shift"][j]
+                c.out_scale[j] = nm["out_scale"][j]
+            self._c_key, self._c = key, c
+        return self._c
+
+    def method_records(self, method, opts):
+        """The gs_method and gs_options_f64 records of a fused call, reused while the same (frozen)
+        method and options are passed."""
+        m0, o0, cm, co = self._records
+        if method is not m0 or opts is not o0:
+            cm, co = method.to_c(), opts.to_c()
+            self._records = (method, opts, cm, co)
+        return cm, co
 
     def fit_normalization(self, Xk, X1, h: float):
         """Feature and target shifts and scales from training pairs (means and standard deviations)."""
@@ -221,8 +249,7 @@ def solve_batch_learned(gs, X, S, prob, method, p, net: LearnedStart, options=No
     opts = gs.resolve_options(options, {})
     if fused and fallback is None and isinstance(method.globalization, gs.Static):
         lib = _lib()
-        cm = method.to_c()
-        co = opts.to_c()
+        cm, co = net.method_records(method, opts)
         net_c = net.to_c()
         nb = ctypes.c_int64(0)
         rc = lib.gs_solve_learned_henon_heiles_f64(
