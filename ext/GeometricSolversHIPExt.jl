@@ -210,22 +210,25 @@ end
 
 The dense device problem of the reference's benchmarks (`TenEResidual`,
 `benchmark/device_problems.jl:147-158`): `F(x, b) = A x + sin.(x)/10 − b`, the system's own `b` as
-its parameter (a `PerSystem` of `SVector{n, Float64}`), `J = A + Diagonal(cos.(x)/10)`. `A` (n × n,
-served on the device for n = 16, 24, 32, 48, 64) is kept as an `SMatrix` for the generic path and as a
-column-major device matrix, which the library reads through `gs_family.data`.
+its parameter (a `PerSystem` of `SVector{n, T}`), `J = A + Diagonal(cos.(x)/10)`. `A` (n × n,
+served on the device for n = 16, 24, 32, 48, 64) is kept as an `SMatrix` for the generic path and as
+column-major device matrices in Float64 and rounded to Float32 (the `tenE_matrix(n, Float32)` of a
+Float32 batch), which the library reads through `gs_family.data`.
 """
 struct DenseSin{N, L} <: HIPFamily
     A::SMatrix{N, N, Float64, L}
     Ad::ROCArray{Float64, 2}
+    Ad32::ROCArray{Float32, 2}
 end
 function DenseSin(A::AbstractMatrix)
     n = size(A, 1)
     size(A, 2) == n || throw(DimensionMismatch("A must be square"))
     S = SMatrix{n, n, Float64}(A)
-    return DenseSin(S, ROCArray(Matrix(S)))
+    return DenseSin(S, ROCArray(Matrix(S)), ROCArray(Float32.(Matrix(S))))
 end
-(f::DenseSin{N})(x::SVector{N, T}, b) where {N, T} = f.A * x .+ sin.(x) ./ 10 .- b
-dense_sin_jacobian(f::DenseSin{N}, x::SVector{N, T}, _) where {N, T} = f.A + Diagonal(cos.(x) ./ 10)
+(f::DenseSin{N})(x::SVector{N, T}, b) where {N, T} = SMatrix{N, N, T}(f.A) * x .+ sin.(x) ./ 10 .- b
+dense_sin_jacobian(f::DenseSin{N}, x::SVector{N, T}, _) where {N, T} =
+    SMatrix{N, N, T}(f.A) + Diagonal(cos.(x) ./ 10)
 
 """The `NonlinearProblem` of a registered family, with its analytic Jacobian."""
 family_problem(f::HenonHeilesMidpoint) = NonlinearProblem(f, hh_midpoint_jacobian)
@@ -248,9 +251,10 @@ family_n(::ScalarSquare) = 1
 family_n(::FpuGl3) = 48
 family_n(::DenseSin{N}) where {N} = N
 
-# The family's own device data (`gs_family.data`): the matrix of DenseSin, none for the others.
-family_data(::HIPFamily) = C_NULL
-family_data(f::DenseSin) = pointer(f.Ad)
+# The family's own device data (`gs_family.data`) for a batch in R: the matrix of DenseSin, none for
+# the others.
+family_data(::HIPFamily, ::Type) = C_NULL
+family_data(f::DenseSin, ::Type{R}) where {R} = R === Float32 ? pointer(f.Ad32) : pointer(f.Ad)
 
 # The parameter records of a batch in the working precision R: (device pointer, byte stride, shared
 # scalars, roots to keep). A record is the isbits element of the device array (its size is the
@@ -276,9 +280,9 @@ function family_records(::ScalarSquare, p::PerSystem, ::Type{R}) where {R}
     d = p.data::ROCVector{R}
     return pointer(d), Int64(sizeof(R)), (), d
 end
-function family_records(::DenseSin{N}, p::PerSystem, ::Type{Float64}) where {N}
-    d = p.data::ROCVector{SVector{N, Float64}}
-    return pointer(d), Int64(N * sizeof(Float64)), (), d
+function family_records(::DenseSin{N}, p::PerSystem, ::Type{R}) where {N, R}
+    d = p.data::ROCVector{SVector{N, R}}
+    return pointer(d), Int64(N * sizeof(R)), (), d
 end
 function family_records(f::FpuGl3, p::BatchParams, ::Type{Float64})
     QP = p.per_system::ROCVector{SVector{32, Float64}}
@@ -290,9 +294,10 @@ end
 # to the generic path.
 served(::HIPFamily, cm, ::Type{R}) where {R} = true
 served(::FpuGl3, cm, ::Type{R}) where {R} = R === Float64 && cm.globalization in (0, 1, 3)
-# the dense kernel: n = 16, 24, 32, 48, 64 in Float64, Static or Backtracking, with or without `reuse`
-served(::DenseSin{N}, cm, ::Type{R}) where {N, R} = R === Float64 && N in (16, 24, 32, 48, 64) &&
-    cm.globalization in (0, 1)
+# the dense kernel: n = 16, 24, 32, 48, 64 in Float64 or Float32, Static or Backtracking, with or
+# without `reuse`
+served(::DenseSin{N}, cm, ::Type{R}) where {N, R} = (R === Float64 || R === Float32) &&
+    N in (16, 24, 32, 48, 64) && cm.globalization in (0, 1)
 
 # ---- the method and options as ABI structs ----------------------------------------------------
 
@@ -355,7 +360,7 @@ end
 function hip_launch!(X, S, codes, fam::HIPFamily, cm::GsMethod, p, opt::Options{R}, order, ws) where {R}
     P, stride, shared, root = family_records(fam, p, R)
     cf = GsFamily(family_id(fam), Int32(family_n(fam)), Int32(family_n(fam)), Int32(0),
-        ntuple(k -> k <= length(shared) ? shared[k] : 0.0, 8), family_data(fam))
+        ntuple(k -> k <= length(shared) ? shared[k] : 0.0, 8), family_data(fam, R))
     co = device_options(opt)
     order_dev = order === nothing ? nothing : ROCArray(Int64.(collect(order)))
     Sp = S === nothing ? C_NULL : pointer(S)

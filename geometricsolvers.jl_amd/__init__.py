@@ -494,7 +494,8 @@ class DenseSin(_SimpleFamily):
     F(x, b) = A x + sin.(x) ./ 10 − b, n = 16, 24, 32, 48 or 64, with A (n × n) the family's own, shared by the
     batch, and p = PerSystem(B), one b per system. The Jacobian is A + Diagonal(cos.(x) ./ 10). Its sine
     and cosine are one fixed algorithm (Cody–Waite reduction, Taylor polynomials), the same on the
-    device and in the oracle."""
+    device and in the oracle. A Float32 batch (the device grid's element type, devices.jl:47) reads A
+    rounded to Float32, as tenE_matrix(n, Float32) holds it."""
 
     id = _abi.GS_FAM_DENSE_SIN
 
@@ -514,11 +515,15 @@ class DenseSin(_SimpleFamily):
         """A column-major, as the library reads it."""
         return np.ascontiguousarray(self.A.T).reshape(-1)
 
-    def device_data(self, device):
-        t = self._dev.get(str(device))
+    def device_data(self, device, dtype=None):
+        """A on `device` (column-major) in `dtype` (torch.float64 unless given), kept per device and type."""
+        torch = _torch()
+        dtype = torch.float64 if dtype is None else dtype
+        key = (str(device), str(dtype))
+        t = self._dev.get(key)
         if t is None:
-            t = _torch().as_tensor(self.host_data(), device=device)
-            self._dev[str(device)] = t
+            t = torch.as_tensor(self.host_data(), device=device).to(dtype).contiguous()
+            self._dev[key] = t
         return t
 
 
@@ -1030,7 +1035,7 @@ def solve_batch(X, *args, order=None, options: Optional[Options] = None, workgro
     cf.id, cf.n, cf.m, cf.p_len = fam.id, fam.n, fam.n, fam.p_len
     for k, v in enumerate(shared):
         cf.shared[k] = v
-    fdata = fam.device_data(X.device) if hasattr(fam, "device_data") else None
+    fdata = fam.device_data(X.device, X.dtype) if hasattr(fam, "device_data") else None
     if fdata is not None:
         cf.data = fdata.data_ptr()
     order_dev = None

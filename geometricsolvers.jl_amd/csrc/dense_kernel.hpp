@@ -65,13 +65,14 @@ template <int N, int NP_> struct Shape {
 constexpr int kDenseThreads = 256;  // every shape's workgroup
 
 
-// One system's LDS region: the vectors of the solve, the pivot-row and solve broadcast buffers.
-template <int N> struct alignas(16) SysVecs {
-    double x[N], Fx[N], d[N], xt[N], Ft[N], r[N], t[N], u[N];
-    double jx[N], jF[N];  // the point of the last factorisation and its residual (diagnostics)
-    double bx[N];         // the best iterate (best = true)
-    double rowd[N];       // the pivot row (FP64 factor), a finished solve entry, a column of J
-    float rowf[N];        // the same for the FP32 factor
+// One system's LDS region in the working precision T: the vectors of the solve, the pivot-row and
+// solve broadcast buffers.
+template <class T, int N> struct alignas(16) SysVecs {
+    T x[N], Fx[N], d[N], xt[N], Ft[N], r[N], t[N], u[N];
+    T jx[N], jF[N];  // the point of the last factorisation and its residual (diagnostics)
+    T bx[N];         // the best iterate (best = true)
+    double rowd[N];  // the pivot row (FP64 factor), a finished solve entry, a column of J in FP64
+    float rowf[N];   // the same for the FP32 factor and J in FP32
     double yd[N];
     float yf[N];
 };
@@ -157,51 +158,53 @@ template <int NP> GS_DEV int seg_argmax(double key, int tag) {
 
 // ---- folds over an LDS vector, evaluated by every lane of the segment -----------------------------
 
-template <int N> GS_DEV double vnorm2(const double* v) {
-    double s = v[0] * v[0];
+template <int N, class T> GS_DEV T vnorm2(const T* v) {
+    T s = v[0] * v[0];
 #pragma unroll 8
     for (int i = 1; i < N; ++i) s = s + v[i] * v[i];
     return s;
 }
-template <int N> GS_DEV double vrmax(const double* v) {
-    double m = __builtin_fabs(v[0]);
+template <int N, class T> GS_DEV T vrmax(const T* v) {
+    T m = tabs(v[0]);
 #pragma unroll 8
-    for (int i = 1; i < N; ++i) m = jmax(m, __builtin_fabs(v[i]));
+    for (int i = 1; i < N; ++i) m = jmax(m, tabs(v[i]));
     return m;
 }
 // rnorm (reductions.jl:41-57)
-template <int N> GS_DEV double vrnorm(const double* v) {
-    const double s = vnorm2<N>(v);
-    if (__builtin_expect(isfin(s) && s >= double(N) * kDblMin, 1)) return __builtin_sqrt(s);
-    const double m = vrmax<N>(v);
-    if (m == 0.0 || !isfin(m)) return m;
+template <int N, class T> GS_DEV T vrnorm(const T* v) {
+    const T s = vnorm2<N>(v);
+    if (__builtin_expect(isfin(s) && s >= T(N) * Lim<T>::min, 1)) return tsqrt(s);
+    const T m = vrmax<N>(v);
+    if (m == T(0) || !isfin(m)) return m;
     const int e = exponent_of(m);
-    double t = ldexp2(v[0], -e);
-    double acc = t * t;
+    T t = ldexp2(v[0], -e);
+    T acc = t * t;
 #pragma unroll 8
     for (int i = 1; i < N; ++i) {
         t = ldexp2(v[i], -e);
         acc = acc + t * t;
     }
-    return ldexp2(__builtin_sqrt(acc), e);
+    return ldexp2(tsqrt(acc), e);
 }
 // rdot (reductions.jl:77-85)
-template <int N> GS_DEV double vdot(const double* a, const double* b) {
-    double s = a[0] * b[0];
+template <int N, class T> GS_DEV T vdot(const T* a, const T* b) {
+    T s = a[0] * b[0];
 #pragma unroll 8
     for (int i = 1; i < N; ++i) s = s + a[i] * b[i];
     return s;
 }
 
 // The exponent of the largest |entry| over the segment's entries `a` (lanes with `live`), clamped to
-// ±emax; 0 for a zero vector or one with a non-finite entry (max_exponent, factors.jl:833-843).
-template <int NP> GS_DEV int seg_exponent(double a, bool live) {
-    const double m0 = live ? __builtin_fabs(a) : 0.0;
+// ±emax of T; 0 for a zero vector or one with a non-finite entry (max_exponent, factors.jl:833-843).
+// The maximum runs in FP64 (exact for an FP32 entry).
+template <int NP, class T> GS_DEV int seg_exponent(T a, bool live) {
+    const double m0 = live ? double(tabs(a)) : 0.0;
     const bool nan_any = seg_any<NP>(live && isnan_(m0));
     const double m = seg_max<NP>(isnan_(m0) ? 0.0 : m0);
     if (nan_any || !isfin(m) || m == 0.0) return 0;
     const int e = exponent_of(m);
-    return e < -kEmax64 ? -kEmax64 : (e > kEmax64 ? kEmax64 : e);
+    constexpr int emax = Lim<T>::emax;
+    return e < -emax ? -emax : (e > emax ? emax : e);
 }
 
 // ---- the family: A x + sin.(x)/10 − b ---------------------------------------------------------------
@@ -246,37 +249,71 @@ GS_DEV void fsincos(double x, double& sn, double& cs) {
     cs = q == 0 ? c0 : (q == 1 ? -s0 : (q == 2 ? -c0 : s0));
 }
 
+// The same steps in FP32: π/2 in three FP32 parts, the polynomials through r¹¹ and r¹⁰ (coefficients
+// rounded once to FP32); NaN for a non-finite x or |x| ≥ 2^22.
+__constant__ float kSinCf[5] = {-0x1.555556p-3f, 0x1.111112p-7f, -0x1.a01a02p-13f, 0x1.71de3ap-19f, -0x1.ae6456p-26f};
+__constant__ float kCosCf[5] = {-0x1.000000p-1f, 0x1.555556p-5f, -0x1.6c16c2p-10f, 0x1.a01a02p-16f, -0x1.27e4fcp-22f};
+
+GS_DEV void fsincos(float x, float& sn, float& cs) {
+    if (!(__builtin_fabsf(x) < 0x1p22f)) {
+        sn = cs = tnan<float>();
+        return;
+    }
+    if (x == 0.0f) {  // sin(±0) = ±0
+        sn = x;
+        cs = 1.0f;
+        return;
+    }
+    const float k = __builtin_rintf(x * 0x1.45f306p-1f);
+    float r = __builtin_fmaf(-k, 0x1.921fb6p+0f, x);
+    r = __builtin_fmaf(-k, -0x1.777a5cp-25f, r);
+    r = __builtin_fmaf(-k, -0x1.ee59dap-50f, r);
+    const float z = r * r;
+    float ps = kSinCf[4], pc = kCosCf[4];
+#pragma unroll
+    for (int j = 3; j >= 0; --j) {
+        ps = __builtin_fmaf(ps, z, kSinCf[j]);
+        pc = __builtin_fmaf(pc, z, kCosCf[j]);
+    }
+    const float s0 = __builtin_fmaf(r * z, ps, r);
+    const float c0 = __builtin_fmaf(z, pc, 1.0f);
+    const int q = int(int64_t(k) & 3);
+    sn = q == 0 ? s0 : (q == 1 ? c0 : (q == 2 ? -s0 : -c0));
+    cs = q == 0 ? c0 : (q == 1 ? -s0 : (q == 2 ? -c0 : s0));
+}
+
 // TenEResidual (device_problems.jl:147-157): row i of F and of J = A + Diagonal(cos.(x)/10), the
 // matrix-vector product as StaticArrays' muladd fold over the row.
-template <int NN> struct FamDenseSin {
+template <class R, int NN> struct FamDenseSin {
+    using T = R;
     static constexpr int N = NN;
-    const double* A;  // LDS, column-major
-    double b;         // this lane's entry of the system's b
-    GS_DEV double F(const double* x, int i) const {
-        double acc = A[i] * x[0];
+    const T* A;  // LDS, column-major
+    T b;         // this lane's entry of the system's b
+    GS_DEV T F(const T* x, int i) const {
+        T acc = A[i] * x[0];
 #pragma unroll 8
-        for (int j = 1; j < N; ++j) acc = __builtin_fma(A[i + j * N], x[j], acc);
-        double sn, cs;
+        for (int j = 1; j < N; ++j) acc = tfma(A[i + j * N], x[j], acc);
+        T sn, cs;
         fsincos(x[i], sn, cs);
-        return (acc + sn / 10.0) - b;
+        return (acc + sn / T(10)) - b;
     }
     // J_ij at x from the lane's own diagonal term dg = cos(x_i)/10 (Jdiag): column j in parallel.
-    GS_DEV double Jdiag(const double* x, int i) const {
-        double sn, cs;
+    GS_DEV T Jdiag(const T* x, int i) const {
+        T sn, cs;
         fsincos(x[i], sn, cs);
-        return cs / 10.0;
+        return cs / T(10);
     }
-    GS_DEV double Jentry(int i, int j, double dg) const {
-        const double a = A[i + j * N];
+    GS_DEV T Jentry(int i, int j, T dg) const {
+        const T a = A[i + j * N];
         return j == i ? a + dg : a;
     }
-    GS_DEV void Jrow(const double* x, int i, double (&row)[N]) const {
-        double sn, cs;
+    GS_DEV void Jrow(const T* x, int i, T (&row)[N]) const {
+        T sn, cs;
         fsincos(x[i], sn, cs);
-        const double c10 = cs / 10.0;
+        const T c10 = cs / T(10);
 #pragma unroll
         for (int j = 0; j < N; ++j) {
-            const double a = A[i + j * N];
+            const T a = A[i + j * N];
             row[j] = j == i ? a + c10 : a;
         }
     }
@@ -287,19 +324,19 @@ template <int NN> struct FamDenseSin {
 // again from A in LDS and the lane's diagonal term wherever it is read, with the same operations
 // (a + cos(x_i)/10, then + μ where μ ≠ 0), so the same bits.
 template <int N> constexpr bool holds_jrow() { return N <= 32; }
-template <int N, bool HOLD = holds_jrow<N>()> struct JacRow {
-    double r[N];
-    GS_DEV double operator[](int j) const { return r[j]; }
+template <class T, int N, bool HOLD = holds_jrow<N>()> struct JacRow {
+    T r[N];
+    GS_DEV T operator[](int j) const { return r[j]; }
 };
-template <int N> struct JacRow<N, false> {
-    const double* A;  // LDS, column-major
-    int i;            // the row
-    double dg, mu;    // cos(x_i)/10 and the regularization
-    GS_DEV double operator[](int j) const {
-        const double a = A[i + j * N];
+template <class T, int N> struct JacRow<T, N, false> {
+    const T* A;  // LDS, column-major
+    int i;       // the row
+    T dg, mu;    // cos(x_i)/10 and the regularization
+    GS_DEV T operator[](int j) const {
+        const T a = A[i + j * N];
         if (j != i) return a;
-        const double d = a + dg;
-        return mu != 0.0 ? d + mu : d;
+        const T d = a + dg;
+        return mu != T(0) ? d + mu : d;
     }
 };
 
@@ -308,21 +345,23 @@ template <int N> struct JacRow<N, false> {
 template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE = false> struct DenseSolve {
     static constexpr int N = Fam::N;
     static constexpr int NP = NP_;
-    using JR = JacRow<N>;
-    using Step = StepT<double>;
-    using TF = typename std::conditional<MIXED, float, double>::type;  // the held factor's type
+    using T = typename Fam::T;  // the working precision
+    static_assert(sizeof(T) == 8 || !MIXED, "a Float32 batch factorises in its working precision");
+    using JR = JacRow<T, N>;
+    using Step = StepT<T>;
+    using TF = typename std::conditional<MIXED, float, T>::type;  // the factor's (and held factor's) type
 
     const Fam& fam;
-    const DevParams& q;
-    SysVecs<N>& v;
+    const DevParamsT<T>& q;
+    SysVecs<T, N>& v;
     const int sl;      // the lane's row within the segment
     const bool live;   // sl < N
 
     // NewtonState (newton.jl:715-744): uniform within the segment
-    double merit, sfnorm;
+    T merit, sfnorm;
     bool promoted;
     int slow;
-    double diag_fnorm;  // the ‖F‖ the last factorisation read; NaN before the first
+    T diag_fnorm;  // the ‖F‖ the last factorisation read; NaN before the first
     // reuse = k: the factor the state holds (this lane's row, its logical row and the prescale) and
     // the rule state FactorAge (newton.jl:621-633): its age, whether the last step stalled, whether
     // the last step went along the held factor
@@ -331,24 +370,24 @@ template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool
     int age;
     bool stalled, reused;
 
-    GS_DEV DenseSolve(const Fam& f, const DevParams& p, SysVecs<N>& vv, int lane)
+    GS_DEV DenseSolve(const Fam& f, const DevParamsT<T>& p, SysVecs<T, N>& vv, int lane)
         : fam(f), q(p), v(vv), sl(lane), live(lane < N) {}
 
     // F at the LDS vector xin into out (every lane of the segment, then the fence).
-    GS_DEV void eval_F(const double* xin, double* out) {
+    GS_DEV void eval_F(const T* xin, T* out) {
         if (live) out[sl] = fam.F(xin, sl);
         wsync();
     }
 
-    GS_DEV void copy(const double* from, double* to) {
+    GS_DEV void copy(const T* from, T* to) {
         if (live) to[sl] = from[sl];
         wsync();
     }
 
     // residual_norm (newton.jl:1274-1278)
-    GS_DEV double residual_norm() const {
+    GS_DEV T residual_norm() const {
         if (GLOB == kBacktracking) {
-            if (isfin(merit) && merit >= double(N) * kDblMin) return __builtin_sqrt(merit);
+            if (isfin(merit) && merit >= T(N) * Lim<T>::min) return tsqrt(merit);
             return vrnorm<N>(v.Fx);
         }
         return isnan_(sfnorm) ? vrnorm<N>(v.Fx) : sfnorm;
@@ -368,7 +407,7 @@ template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool
         }
         if constexpr (holds_jrow<N>()) {
             fam.Jrow(v.x, live ? sl : 0, jr.r);
-            if (q.mu != 0.0) {
+            if (q.mu != T(0)) {
 #pragma unroll
                 for (int j = 0; j < N; ++j) jr.r[j] = j == sl ? jr.r[j] + q.mu : jr.r[j];
             }
@@ -381,49 +420,51 @@ template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool
     }
 
     // This lane's row of J at the LDS point xin, without the regularization (the JVP of F there).
-    GS_DEV void jrow_at(const double* xin, JR& ja) {
+    GS_DEV void jrow_at(const T* xin, JR& ja) {
         if constexpr (holds_jrow<N>()) {
             fam.Jrow(xin, live ? sl : 0, ja.r);
         } else {
             ja.A = fam.A;
             ja.i = live ? sl : 0;
             ja.dg = fam.Jdiag(xin, ja.i);
-            ja.mu = 0.0;
+            ja.mu = T(0);
         }
     }
 
-    template <class TF> GS_DEV TF* rowbuf() {
-        if constexpr (sizeof(TF) == 8) return v.rowd;
+    template <class TB> GS_DEV TB* rowbuf() {
+        if constexpr (sizeof(TB) == 8) return v.rowd;
         else return v.rowf;
     }
-    template <class TF> GS_DEV TF* ybuf() {
-        if constexpr (sizeof(TF) == 8) return v.yd;
+    template <class TB> GS_DEV TB* ybuf() {
+        if constexpr (sizeof(TB) == 8) return v.yd;
         else return v.yf;
     }
 
     // factor_matrix (factors.jl:802-811) + factorize!! (lu.jl:24-64) on the segment's rows: a
-    // receives this lane's row of Jreg/2^e in TF, LU-factored; pos its logical row; returns info (the
-    // first zero pivot, 1-based, or 0); fe the prescale exponent (applied when ≠ 0 in FP32, only when
-    // < 0 in FP64).
-    template <class TF> GS_DEV int factor(const JR& jr, TF (&a)[N], int& pos, int& fe) {
+    // receives this lane's row of Jreg/2^e in TB, LU-factored; pos its logical row; returns info (the
+    // first zero pivot, 1-based, or 0); fe the prescale exponent (applied when ≠ 0 for a factor
+    // narrower than T, only when < 0 in the working precision). The row maximum runs in FP64 (exact).
+    template <class TB> GS_DEV int factor(const JR& jr, TB (&a)[N], int& pos, int& fe) {
         double mx = 0.0;
         bool nan_seen = false;
 #pragma unroll
         for (int j = 0; j < N; ++j) {
-            mx = __builtin_fmax(mx, __builtin_fabs(jr[j]));
+            mx = __builtin_fmax(mx, double(tabs(jr[j])));
             nan_seen |= isnan_(jr[j]);
         }
         const bool nan_any = seg_any<NP>(live && nan_seen);
         const double wmx = seg_max<NP>(live ? mx : 0.0);
         int e0 = 0;
+        constexpr int emax = Lim<T>::emax;
         if (!nan_any && isfin(wmx) && wmx != 0.0) {
             const int ex = exponent_of(wmx);
-            e0 = ex < -kEmax64 ? -kEmax64 : (ex > kEmax64 ? kEmax64 : ex);
+            e0 = ex < -emax ? -emax : (ex > emax ? emax : ex);
         }
-        fe = sizeof(TF) == 4 ? e0 : (e0 < 0 ? e0 : 0);
+        constexpr bool same = sizeof(TB) == sizeof(T);
+        fe = same ? (e0 < 0 ? e0 : 0) : e0;
 #pragma unroll
-        for (int j = 0; j < N; ++j) a[j] = TF(fe != 0 ? ldexp2(jr[j], -fe) : jr[j]);
-        TF* rb = rowbuf<TF>();
+        for (int j = 0; j < N; ++j) a[j] = TB(fe != 0 ? ldexp2(jr[j], -fe) : jr[j]);
+        TB* rb = rowbuf<TB>();
         pos = live ? sl : 255;
         int info = 0;
 #pragma unroll
@@ -440,11 +481,11 @@ template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool
                 for (int j = k; j < N; ++j) rb[j] = a[j];
             }
             wsync();
-            const TF ukk = rb[k];
-            const bool zero = ukk == TF(0);
+            const TB ukk = rb[k];
+            const bool zero = ukk == TB(0);
             if (zero && info == 0) info = k + 1;
             if (live && pos > k && !zero) {
-                const TF lik = a[k] / ukk;
+                const TB lik = a[k] / ukk;
                 a[k] = lik;
 #pragma unroll
                 for (int j = k + 1; j < N; ++j) a[j] = a[j] - lik * rb[j];
@@ -456,13 +497,13 @@ template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool
 
     // backsolve (lu.jl:67-92) on the segment's rows: y is this lane's entry of the right-hand side
     // (rows never moved); returns, in the lane of logical row p, entry p of the solution.
-    template <class TF> GS_DEV TF backsolve(const TF (&a)[N], int pos, TF y) {
-        TF* yb = ybuf<TF>();
+    template <class TB> GS_DEV TB backsolve(const TB (&a)[N], int pos, TB y) {
+        TB* yb = ybuf<TB>();
 #pragma unroll
         for (int k = 0; k < N - 1; ++k) {
             if (pos == k) yb[k] = y;
             wsync();
-            const TF yk = yb[k];
+            const TB yk = yb[k];
             if (live && pos > k) y = y - a[k] * yk;
         }
 #pragma unroll
@@ -472,30 +513,35 @@ template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool
                 yb[k] = y;
             }
             wsync();
-            const TF xk = yb[k];
+            const TB xk = yb[k];
             if (live && pos < k) y = y - a[k] * xk;
         }
         return y;
     }
 
-    // ldiv!! with FP64 factors (factors.jl:931-936): out = J \ r.
-    GS_DEV void ldiv(const double (&a)[N], int pos, int fe, const double* r, double* out) {
-        const double ri = live ? r[sl] : 0.0;
+    // ldiv!! with factors in the working precision (factors.jl:931-936): out = J \ r.
+    GS_DEV void ldiv_same(const T (&a)[N], int pos, int fe, const T* r, T* out) {
+        const T ri = live ? r[sl] : T(0);
         const int m = seg_exponent<NP>(ri, live);
         const int lo = fe < 0 ? fe : 0;
         const int c = m < lo ? lo : (m > 0 ? 0 : m);
-        const int alt = m - (kEmax64 - headroom(N));
+        const int alt = m - (Lim<T>::emax - headroom(N));
         const int k = c > alt ? c : alt;
-        double y = k != 0 ? ldexp2(ri, -k) : ri;
-        y = backsolve<double>(a, pos, y);
+        T y = k != 0 ? ldexp2(ri, -k) : ri;
+        y = backsolve<T>(a, pos, y);
         const int e = k - fe;
         if (live) out[pos] = e != 0 ? ldexp2(y, e) : y;
         wsync();
     }
 
-    // ldiv!! with FP32 factors (factors.jl:943-949): r/s in FP32 with s = max|r|, then
-    // a·frac·2^(ex−e) in FP64.
-    GS_DEV void ldiv(const float (&a)[N], int pos, int fe, const double* r, double* out) {
+    template <class TB> GS_DEV void ldiv(const TB (&a)[N], int pos, int fe, const T* r, T* out) {
+        if constexpr (sizeof(TB) == sizeof(T)) ldiv_same(a, pos, fe, r, out);
+        else ldiv_mixed(a, pos, fe, r, out);
+    }
+
+    // ldiv!! with FP32 factors in an FP64 solve (factors.jl:943-949): r/s in FP32 with s = max|r|,
+    // then a·frac·2^(ex−e) in FP64.
+    GS_DEV void ldiv_mixed(const float (&a)[N], int pos, int fe, const double* r, double* out) {
         const double ri = live ? r[sl] : 0.0;
         const double ar = __builtin_fabs(ri);
         const bool nan_any = seg_any<NP>(live && isnan_(ar));
@@ -514,29 +560,29 @@ template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool
     }
 
     // out = Jreg v (StaticArrays' muladd fold over the row).
-    GS_DEV void matvec(const JR& jr, const double* vin, double* out) {
-        double acc = jr[0] * vin[0];
+    GS_DEV void matvec(const JR& jr, const T* vin, T* out) {
+        T acc = jr[0] * vin[0];
 #pragma unroll
-        for (int j = 1; j < N; ++j) acc = __builtin_fma(jr[j], vin[j], acc);
+        for (int j = 1; j < N; ++j) acc = tfma(jr[j], vin[j], acc);
         if (live) out[sl] = acc;
         wsync();
     }
 
     // The direction d = −(J \ F) from a factor in TF, then refine_passes!! (refinement.jl:211-244) with
     // the held Jreg; returns refine_passes!!' `failed` (false without refinement).
-    template <class TF>
-    GS_DEV bool direction(const JR& jr, const TF (&a)[N], int pos, int fe, bool refine) {
+    template <class TB>
+    GS_DEV bool direction(const JR& jr, const TB (&a)[N], int pos, int fe, bool refine) {
         ldiv(a, pos, fe, v.Fx, v.d);
         if (live) v.d[sl] = -v.d[sl];
         wsync();
         if (!REFINE || !refine) return false;
-        const double fnorm = vrnorm<N>(v.Fx);
-        if (fnorm == 0.0) return false;
+        const T fnorm = vrnorm<N>(v.Fx);
+        if (fnorm == T(0)) return false;
         if (!isfin(fnorm)) return true;
         matvec(jr, v.d, v.t);
         if (live) v.r[sl] = -v.Fx[sl] + -v.t[sl];
         wsync();
-        double rprev = vrnorm<N>(v.r);
+        T rprev = vrnorm<N>(v.r);
         if (!isfin(rprev)) return true;
         if (rprev <= q.ref_eta_eff * fnorm) return false;
         for (int it = 1; it <= q.ref_maxiter; ++it) {
@@ -546,7 +592,7 @@ template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool
             matvec(jr, v.u, v.t);
             if (live) v.r[sl] = -v.Fx[sl] + -v.t[sl];
             wsync();
-            const double rcur = vrnorm<N>(v.r);
+            const T rcur = vrnorm<N>(v.r);
             if (!(rcur < rprev)) return true;
             copy(v.u, v.d);
             rprev = rcur;
@@ -559,13 +605,13 @@ template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool
 
     // globalize!! Static (linefunction.jl:114-131)
     GS_DEV Step step_static() {
-        const double dn = vrnorm<N>(v.d);
-        const double stepnorm = __builtin_fabs(q.alpha) * dn;
+        const T dn = vrnorm<N>(v.d);
+        const T stepnorm = tabs(q.alpha) * dn;
         if (!isfin(stepnorm)) return Step{kNonfinite, false, false, 0, vrnorm<N>(v.Fx), stepnorm, false};
         if (live) v.xt[sl] = v.x[sl] + q.alpha * v.d[sl];
         wsync();
         eval_F(v.xt, v.Ft);
-        const double fn = vrnorm<N>(v.Ft);
+        const T fn = vrnorm<N>(v.Ft);
         if (!isfin(fn)) return Step{kNonfinite, false, false, 0, vrnorm<N>(v.Fx), stepnorm, false};
         if (live) {
             v.x[sl] = v.xt[sl];
@@ -575,54 +621,34 @@ template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool
         return Step{kSuccess, false, false, 0, fn, stepnorm, false};
     }
 
-    // backtrack_step (backtracking.jl:320-340)
-    GS_DEV static double backtrack_step(double phi0, double d0, double a, double pa, double ap, double pp, double p) {
-        if (!isfin(pa)) return 0.1 * a;
-        double an = dnan();
-        const double s = __builtin_fabs(d0);
-        const double r1 = (pa - phi0) / s + a;
-        if (!(ap > 0.0)) {
-            if (r1 > 0.0) an = (a * a) / (2.0 * r1);
-        } else if (isfin(pp) && a != ap) {
-            const double q1 = r1 / (a * a);
-            const double q2 = ((pp - phi0) / s + ap) / (ap * ap);
-            const double A = (q1 - q2) / (a - ap);
-            const double B = (a * q2 - ap * q1) / (a - ap);
-            const double disc = B * B + 3.0 * A;
-            if (disc >= 0.0) an = B > 0.0 ? 1.0 / (B + __builtin_sqrt(disc)) : (-B + __builtin_sqrt(disc)) / (3.0 * A);
-        }
-        if (!(isfin(an) && an > 0.0)) an = p * a;
-        return jmin(jmax(an, 0.1 * a), p * a);
-    }
-
     // globalize_merit!! for Backtracking (linefunction.jl:159-199 → linesearch.jl:154-171 →
     // backtracking.jl:354-390); `exact`: ExactStep (slope −2φ₀) or MeasuredSlope (φ′(0) = 2⟨F, J d⟩).
     GS_DEV Step step_backtracking(bool exact) {
-        const double phi0 = merit;
-        const double dn = vrnorm<N>(v.d);
+        const T phi0 = merit;
+        const T dn = vrnorm<N>(v.d);
         if (!isfin(dn)) return Step{kNonfinite, false, false, 0, vrnorm<N>(v.Fx), dn, false};
-        double d0;
+        T d0;
         if (exact) {
-            d0 = -2.0 * phi0;
+            d0 = -T(2) * phi0;
         } else {
-            if (live) v.xt[sl] = v.x[sl] + 0.0 * v.d[sl];
+            if (live) v.xt[sl] = v.x[sl] + T(0) * v.d[sl];
             wsync();
             eval_F(v.xt, v.Ft);
             JR ja;  // J at x + 0·d, without the regularization
             jrow_at(v.xt, ja);
             matvec(ja, v.d, v.t);
-            d0 = 2.0 * vdot<N>(v.Ft, v.t);
+            d0 = T(2) * vdot<N>(v.Ft, v.t);
         }
-        if (!(isfin(phi0) && isfin(d0) && d0 < 0.0)) {
-            const uint8_t c = !(isfin(phi0) && isfin(d0)) ? kNonfinite : (d0 > 0.0 ? kLsFailed : kStalled);
-            if (c == kLsFailed) return Step{kLsFailed, false, false, 1, vrnorm<N>(v.Fx), 0.0, false};
-            return Step{c, false, false, 0, vrnorm<N>(v.Fx), 0.0, false};
+        if (!(isfin(phi0) && isfin(d0) && d0 < T(0))) {
+            const uint8_t c = !(isfin(phi0) && isfin(d0)) ? kNonfinite : (d0 > T(0) ? kLsFailed : kStalled);
+            if (c == kLsFailed) return Step{kLsFailed, false, false, 1, vrnorm<N>(v.Fx), T(0), false};
+            return Step{c, false, false, 0, vrnorm<N>(v.Fx), T(0), false};
         }
-        const double tau = __builtin_fmax(4.0 * kDblEps * __builtin_fabs(phi0), 4.0 * kDblTiny);
-        const double amin = jmax(tau / __builtin_fabs(d0), kDblMin);
-        double alpha = __builtin_fmin(__builtin_fmax(1.0, amin), double(__builtin_inf()));
-        double aa = alpha, pa = dnan();
-        double ap = dnan(), pp = dnan();
+        const T tau = tfmax(T(4) * Lim<T>::eps * tabs(phi0), T(4) * Lim<T>::tiny);
+        const T amin = jmax(tau / tabs(d0), Lim<T>::min);
+        T alpha = tfmin(tfmax(T(1), amin), T(__builtin_inf()));
+        T aa = alpha, pa = tnan<T>();
+        T ap = tnan<T>(), pp = tnan<T>();
         int frozen = 0;
         uint8_t code = kLsFailed;
         for (int trips = 0; trips < q.bt_maxiter; ++trips) {
@@ -631,19 +657,19 @@ template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool
             wsync();
             eval_F(v.xt, v.Ft);
             pa = vnorm2<N>(v.Ft);
-            frozen = (pa == phi0 && alpha <= Lim<double>::sqrteps) ? frozen + 1 : 0;
-            if (pa - phi0 <= jmin(0.0, q.c1 * alpha * d0 + tau)) {
-                code = (isfin(pa) && pa - phi0 < -tau) ? kSuccess : (__builtin_fabs(pa - phi0) <= tau ? kStalled : kLsFailed);
+            frozen = (pa == phi0 && alpha <= Lim<T>::sqrteps) ? frozen + 1 : 0;
+            if (pa - phi0 <= jmin(T(0), q.c1 * alpha * d0 + tau)) {
+                code = (isfin(pa) && pa - phi0 < -tau) ? kSuccess : (tabs(pa - phi0) <= tau ? kStalled : kLsFailed);
                 break;
             } else if (frozen >= 2) {
                 code = kStalled;
                 break;
             } else if (alpha <= amin) {
-                code = __builtin_fabs(pa - phi0) <= tau ? kStalled : kLsFailed;
+                code = tabs(pa - phi0) <= tau ? kStalled : kLsFailed;
                 break;
             }
-            const double an = isfin(pa) ? backtrack_step(phi0, d0, alpha, pa, ap, pp, q.p)
-                                        : jmin(0.1 * alpha, __builtin_sqrt(alpha) * __builtin_sqrt(amin));
+            const T an = isfin(pa) ? backtrack_next<T>(phi0, d0, alpha, pa, ap, pp, q.p)
+                                   : jmin(T(0.1) * alpha, tsqrt(alpha) * tsqrt(amin));
             ap = alpha;
             pp = pa;
             alpha = jmax(an, amin);
@@ -655,10 +681,10 @@ template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool
             }
             wsync();
             merit = pa;
-            return Step{code, false, aa < 1.0, code == kLsFailed ? 1 : 0, __builtin_sqrt(pa), aa * dn, false};
+            return Step{code, false, aa < T(1), code == kLsFailed ? 1 : 0, tsqrt(pa), aa * dn, false};
         }
-        if (code == kLsFailed) return Step{kLsFailed, false, false, 1, vrnorm<N>(v.Fx), 0.0, false};
-        return Step{code == kStalled ? kStalled : kNonfinite, false, false, 0, vrnorm<N>(v.Fx), 0.0, false};
+        if (code == kLsFailed) return Step{kLsFailed, false, false, 1, vrnorm<N>(v.Fx), T(0), false};
+        return Step{code == kStalled ? kStalled : kNonfinite, false, false, 0, vrnorm<N>(v.Fx), T(0), false};
     }
 
     GS_DEV Step globalize(bool exact) {
@@ -672,15 +698,15 @@ template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool
     GS_DEV Step plain_step() {
         JR jr;
         jacobian(jr);
-        double a[N];
+        T a[N];
         int pos, fe;
-        const int info = factor<double>(jr, a, pos, fe);
+        const int info = factor<T>(jr, a, pos, fe);
         Step s;
         if (info > 0) {
-            s = Step{kSingular, false, false, 0, vrnorm<N>(v.Fx), 0.0, false};
+            s = Step{kSingular, false, false, 0, vrnorm<N>(v.Fx), T(0), false};
         } else {
-            direction<double>(jr, a, pos, fe, true);
-            s = globalize(q.mu == 0.0 && !REFINE);
+            direction<T>(jr, a, pos, fe, true);
+            s = globalize(q.mu == T(0) && !REFINE);
         }
         sfnorm = s.fnorm;
         return s;
@@ -779,7 +805,7 @@ template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool
             } else {
                 // a failed search without a step leaves x, Fx and the merit as they were
                 Step s = globalize(false);
-                if (!(s.code == kLsFailed && s.stepnorm == 0.0)) {
+                if (!(s.code == kLsFailed && s.stepnorm == T(0))) {
                     sfnorm = s.fnorm;
                     set_rule(s.code, age_next, true);
                     return s;
@@ -791,10 +817,10 @@ template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool
         const int info = factor<TF>(jr, ka, kpos, kfe);
         Step s;
         if (info > 0) {
-            s = Step{kSingular, false, false, 0, vrnorm<N>(v.Fx), 0.0, false};
+            s = Step{kSingular, false, false, 0, vrnorm<N>(v.Fx), T(0), false};
         } else {
             direction<TF>(jr, ka, kpos, kfe, true);
-            s = globalize(q.mu == 0.0 && !REFINE);
+            s = globalize(q.mu == T(0) && !REFINE);
         }
         s.failures += failures;
         sfnorm = s.fnorm;
@@ -914,22 +940,22 @@ template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool
     // entry J_ij) in v.rowd: the left fold of the squares in column-major order (rnorm's sum; with
     // e ≠ 0 of (J_ij/2^e)²) and the NaN-propagating largest |J_ij|, and, with `prod`, g = Jᴴ (sc F) into
     // v.r (entry j the muladd fold over column j).
-    GS_DEV void diag_pass(double dg, int e, bool prod, double sc, double& sumsq, double& mx) {
-        double s = 0.0, m = 0.0;
+    GS_DEV void diag_pass(T dg, int e, bool prod, T sc, T& sumsq, T& mx) {
+        T s = T(0), m = T(0);
+        T* col = rowbuf<T>();
 #pragma unroll 1
         for (int j = 0; j < N; ++j) {
-            if (live) v.rowd[sl] = fam.Jentry(sl, j, dg);
+            if (live) col[sl] = fam.Jentry(sl, j, dg);
             wsync();
-            const double* col = v.rowd;
-            double acc = 0.0;
+            T acc = T(0);
 #pragma unroll 8
             for (int i = 0; i < N; ++i) {
-                const double w = col[i];
-                const double ws = e != 0 ? ldexp2(w, -e) : w;
+                const T w = col[i];
+                const T ws = e != 0 ? ldexp2(w, -e) : w;
                 s = (j == 0 && i == 0) ? ws * ws : s + ws * ws;
-                m = jmax(m, __builtin_fabs(w));
-                const double fi = sc != 1.0 ? v.jF[i] * sc : v.jF[i];
-                acc = i == 0 ? w * fi : __builtin_fma(w, fi, acc);
+                m = jmax(m, tabs(w));
+                const T fi = sc != T(1) ? v.jF[i] * sc : v.jF[i];
+                acc = i == 0 ? w * fi : tfma(w, fi, acc);
             }
             if (prod && sl == j) v.r[j] = acc;
             wsync();
@@ -938,29 +964,29 @@ template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool
         mx = m;
     }
 
-    GS_DEV double grad_ratio() {
-        if (isnan_(diag_fnorm)) return dnan();
-        const double dg = live ? fam.Jdiag(v.jx, sl) : 0.0;
-        const double fnorm = diag_fnorm;
-        double s, m;
-        diag_pass(dg, 0, true, 1.0, s, m);
-        double jnorm;
-        if (__builtin_expect(isfin(s) && s >= double(N * N) * kDblMin, 1)) {
-            jnorm = __builtin_sqrt(s);
-        } else if (m == 0.0 || !isfin(m)) {
+    GS_DEV T grad_ratio() {
+        if (isnan_(diag_fnorm)) return tnan<T>();
+        const T dg = live ? fam.Jdiag(v.jx, sl) : T(0);
+        const T fnorm = diag_fnorm;
+        T s, m;
+        diag_pass(dg, 0, true, T(1), s, m);
+        T jnorm;
+        if (__builtin_expect(isfin(s) && s >= T(N * N) * Lim<T>::min, 1)) {
+            jnorm = tsqrt(s);
+        } else if (m == T(0) || !isfin(m)) {
             jnorm = m;
         } else {
             const int e = exponent_of(m);
-            double s2, m2;
-            diag_pass(dg, e, false, 1.0, s2, m2);
-            jnorm = ldexp2(__builtin_sqrt(s2), e);
+            T s2, m2;
+            diag_pass(dg, e, false, T(1), s2, m2);
+            jnorm = ldexp2(tsqrt(s2), e);
         }
-        if (jnorm == 0.0 || fnorm == 0.0) return 0.0;
-        const double gnorm = vrnorm<N>(v.r);
+        if (jnorm == T(0) || fnorm == T(0)) return T(0);
+        const T gnorm = vrnorm<N>(v.r);
         if (isfin(gnorm) || !(isfin(jnorm) && isfin(fnorm))) return gnorm / jnorm / fnorm;
         const int e = exponent_of(fnorm);
-        double s3, m3;
-        diag_pass(dg, 0, true, ldexp2(1.0, -e), s3, m3);
+        T s3, m3;
+        diag_pass(dg, 0, true, ldexp2(T(1), -e), s3, m3);
         return vrnorm<N>(v.r) / jnorm / ldexp2(fnorm, -e);
     }
 
@@ -970,30 +996,30 @@ template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool
         uint8_t code;
         bool contracted, promoted, reused;
         int32_t iterations, failures;
-        double fnorm, stepnorm, rate, gradratio;
+        T fnorm, stepnorm, rate, gradratio;
     };
 
     GS_DEV Result solve() {
         eval_F(v.x, v.Fx);
-        const double fnorm0 = vrnorm<N>(v.Fx);
-        merit = GLOB == kBacktracking ? vnorm2<N>(v.Fx) : dnan();
+        const T fnorm0 = vrnorm<N>(v.Fx);
+        merit = GLOB == kBacktracking ? vnorm2<N>(v.Fx) : tnan<T>();
         sfnorm = fnorm0;
         promoted = false;
         slow = 0;
-        diag_fnorm = dnan();
+        diag_fnorm = tnan<T>();
         kpos = 0;
         kfe = 0;
         age = 0;
         stalled = false;
         reused = false;
-        double best_fn = fnorm0;
+        T best_fn = fnorm0;
         if (q.best) copy(v.x, v.bx);
-        Result R{kSuccess, false, false, false, 0, 0, fnorm0, 0.0, dnan(), dnan()};
+        Result R{kSuccess, false, false, false, 0, 0, fnorm0, T(0), tnan<T>(), tnan<T>()};
         if (!isfin(fnorm0)) {
             R.code = kNonfinite;
         } else {
-            const double ftest0 = q.norm_max ? vrmax<N>(v.Fx) : fnorm0;
-            const double ftol = jmax(q.f_abstol, q.f_reltol * ftest0);
+            const T ftest0 = q.norm_max ? vrmax<N>(v.Fx) : fnorm0;
+            const T ftol = jmax(q.f_abstol, q.f_reltol * ftest0);
             if (0 >= q.min_iterations && ftest0 <= ftol) {
                 R.code = kSuccess;
             } else if (q.max_iterations == 0) {
@@ -1001,13 +1027,13 @@ template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool
             } else {
                 int stalls = 0;
                 while (true) {
-                    const double rate_k = R.rate;
+                    const T rate_k = R.rate;
                     Step s = step();
                     if (q.best && s.fnorm < best_fn) {
                         best_fn = s.fnorm;
                         copy(v.x, v.bx);
                     }
-                    R.rate = R.stepnorm > 0.0 ? s.stepnorm / R.stepnorm : dnan();
+                    R.rate = R.stepnorm > T(0) ? s.stepnorm / R.stepnorm : tnan<T>();
                     R.code = s.code;
                     R.iterations += 1;
                     R.fnorm = s.fnorm;
@@ -1016,26 +1042,26 @@ template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool
                     R.promoted |= s.promoted;
                     // observe_rate!! (newton.jl:1205-1210): a step of rate θ ≥ 1 along a held factor
                     // refreshes the factor at the next step
-                    if (REUSE && reused && R.rate >= 1.0) stalled = true;
+                    if (REUSE && reused && R.rate >= T(1)) stalled = true;
                     if (s.code == kSingular || s.code == kNonfinite || s.code == kMaxIters) break;
-                    if (s.code == kLsFailed && s.stepnorm == 0.0) break;
-                    const double ftest = q.norm_max ? vrmax<N>(v.Fx) : R.fnorm;
+                    if (s.code == kLsFailed && s.stepnorm == T(0)) break;
+                    const T ftest = q.norm_max ? vrmax<N>(v.Fx) : R.fnorm;
                     if (R.iterations >= q.min_iterations && ftest <= ftol) {
                         R.code = kSuccess;
                         break;
                     }
                     const bool steptested = s.code != kStalled;
-                    const double xnorm = steptested ? vrnorm<N>(v.x) : dnan();
-                    const double xtol = jmax(q.x_abstol, q.x_reltol * xnorm);
+                    const T xnorm = steptested ? vrnorm<N>(v.x) : tnan<T>();
+                    const T xtol = jmax(q.x_abstol, q.x_reltol * xnorm);
                     const bool past_min = R.iterations >= q.min_iterations;
                     bool contracted = false;
                     if (!s.shortened && q.rate_test && past_min) {
-                        const double th = jmax(R.rate, rate_k);
-                        contracted = R.stepnorm > 0.0 && th < 1.0 && th / (1.0 - th) * R.stepnorm <= q.kappa * xtol;
+                        const T th = jmax(R.rate, rate_k);
+                        contracted = R.stepnorm > T(0) && th < T(1) && th / (T(1) - th) * R.stepnorm <= q.kappa * xtol;
                     }
-                    const bool step_conv = past_min && R.stepnorm > 0.0 && R.stepnorm <= xtol;
-                    const bool zero_step = s.code == kSuccess && s.stepnorm == 0.0 && past_min;
-                    const bool roundoff = q.roundoff && past_min && R.rate >= 1.0 && rate_k < 1.0;
+                    const bool step_conv = past_min && R.stepnorm > T(0) && R.stepnorm <= xtol;
+                    const bool zero_step = s.code == kSuccess && s.stepnorm == T(0) && past_min;
+                    const bool roundoff = q.roundoff && past_min && R.rate >= T(1) && rate_k < T(1);
                     if (contracted || (steptested && (step_conv || zero_step || roundoff))) {
                         R.code = contracted ? kSuccess : kStalled;
                         R.contracted = contracted;
@@ -1068,11 +1094,20 @@ template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool
     }
 };
 
-template <int N, int NP, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE>
-__global__ __launch_bounds__(kDenseThreads) void dense_kernel(const DenseLaunch L) {
+// The status record of one system in the layout of SolverStatus{T}.
+template <class T>
+GS_DEV void store_status_t(typename StatusWords<T>::Status* dst, uint8_t code, int32_t iterations, T fnorm, T stepnorm,
+                           int32_t failures, bool promoted, T gradratio, T rate, bool contracted, bool reused) {
+    typename StatusWords<T>::W w[StatusWords<T>::kCount];
+    pack_status(w, code, iterations, fnorm, stepnorm, failures, promoted, gradratio, rate, contracted, reused);
+    store_status_lane(dst, w);
+}
+
+template <class T, int N, int NP, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE>
+__global__ __launch_bounds__(kDenseThreads) void dense_kernel(const DenseLaunchT<T> L) {
     using Sh = Shape<N, NP>;
-    __shared__ double sA[N * N];
-    __shared__ SysVecs<N> sv[Sh::SPB];
+    __shared__ T sA[N * N];
+    __shared__ SysVecs<T, N> sv[Sh::SPB];
     for (int t = int(threadIdx.x); t < N * N; t += Sh::kThreads) sA[t] = L.A[t];
     __syncthreads();
     const int lane = int(threadIdx.x) & 63, wave = int(threadIdx.x) >> 6;
@@ -1081,39 +1116,57 @@ __global__ __launch_bounds__(kDenseThreads) void dense_kernel(const DenseLaunch 
     const int64_t i = int64_t(blockIdx.x) * Sh::SPB + sw;
     if (i >= L.nsys) return;  // the whole segment leaves together
     const int64_t idx = L.order ? L.order[i] - 1 : i;
-    SysVecs<N>& v = sv[sw];
-    FamDenseSin<N> fam;
+    SysVecs<T, N>& v = sv[sw];
+    FamDenseSin<T, N> fam;
     fam.A = sA;
     const bool live = sl < N;
-    fam.b = live ? reinterpret_cast<const double*>(L.P + (L.p_stride ? idx * L.p_stride : 0))[sl] : 0.0;
-    double* xp = L.X + idx * N;
+    fam.b = live ? reinterpret_cast<const T*>(L.P + (L.p_stride ? idx * L.p_stride : 0))[sl] : T(0);
+    T* xp = L.X + idx * N;
     if (live) v.x[sl] = xp[sl];
     wsync();
-    DenseSolve<FamDenseSin<N>, NP, GLOB, MIXED, REFINE, DIAG, REUSE> solver(fam, L.q, v, sl);
+    DenseSolve<FamDenseSin<T, N>, NP, GLOB, MIXED, REFINE, DIAG, REUSE> solver(fam, L.q, v, sl);
     const auto R = solver.solve();
     if (live) xp[sl] = v.x[sl];
     if (sl == 0) {
         if (DIAG) {
-            store_status(L.S + idx, R.code, R.iterations, R.fnorm, R.stepnorm, R.failures, R.promoted, R.gradratio,
-                         R.rate, R.contracted, R.reused);
+            store_status_t<T>(L.S + idx, R.code, R.iterations, R.fnorm, R.stepnorm, R.failures, R.promoted,
+                              R.gradratio, R.rate, R.contracted, R.reused);
         } else {
             L.codes[idx] = R.code;
         }
     }
 }
 
-template <int N, int NP, int GLOB, bool MIXED, bool REFINE, bool REUSE>
-int launch_diag(bool diag, const DenseLaunch& L, hipStream_t st) {
+template <class T, int N, int NP, int GLOB, bool MIXED, bool REFINE, bool REUSE>
+int launch_diag(bool diag, const DenseLaunchT<T>& L, hipStream_t st) {
     using Sh = Shape<N, NP>;
     const int64_t blocks = (L.nsys + Sh::SPB - 1) / Sh::SPB;
     if (blocks > 0x7fffffffLL) return set_error(GS_EINVAL, "batch too large for one launch");
     if (diag)
-        hipLaunchKernelGGL((dense_kernel<N, NP, GLOB, MIXED, REFINE, true, REUSE>), dim3(unsigned(blocks)),
+        hipLaunchKernelGGL((dense_kernel<T, N, NP, GLOB, MIXED, REFINE, true, REUSE>), dim3(unsigned(blocks)),
                            dim3(Sh::kThreads), 0, st, L);
     else
-        hipLaunchKernelGGL((dense_kernel<N, NP, GLOB, MIXED, REFINE, false, REUSE>), dim3(unsigned(blocks)),
+        hipLaunchKernelGGL((dense_kernel<T, N, NP, GLOB, MIXED, REFINE, false, REUSE>), dim3(unsigned(blocks)),
                            dim3(Sh::kThreads), 0, st, L);
     return GS_OK;
+}
+
+// The Float32 launches of one size and segment width (no mixed form), with and without reuse = k
+// (dense_f16.hip … dense_f64.hip).
+template <int N, int NP>
+int launch_np_f32(int glob, bool refine, bool diag, bool reuse, const DenseLaunchT<float>& L, hipStream_t st) {
+    if (reuse) {
+        if (glob == kBacktracking)
+            return refine ? launch_diag<float, N, NP, kBacktracking, false, true, true>(diag, L, st)
+                          : launch_diag<float, N, NP, kBacktracking, false, false, true>(diag, L, st);
+        return refine ? launch_diag<float, N, NP, kStatic, false, true, true>(diag, L, st)
+                      : launch_diag<float, N, NP, kStatic, false, false, true>(diag, L, st);
+    }
+    if (glob == kBacktracking)
+        return refine ? launch_diag<float, N, NP, kBacktracking, false, true, false>(diag, L, st)
+                      : launch_diag<float, N, NP, kBacktracking, false, false, false>(diag, L, st);
+    return refine ? launch_diag<float, N, NP, kStatic, false, true, false>(diag, L, st)
+                  : launch_diag<float, N, NP, kStatic, false, false, false>(diag, L, st);
 }
 
 // The launches of one size and segment width; REUSE: the reuse = k forms (their own translation
@@ -1121,15 +1174,15 @@ int launch_diag(bool diag, const DenseLaunch& L, hipStream_t st) {
 template <int N, int NP, bool REUSE = false>
 int launch_np(int glob, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st) {
     if (glob == kBacktracking) {
-        if (mixed) return refine ? launch_diag<N, NP, kBacktracking, true, true, REUSE>(diag, L, st)
-                                 : launch_diag<N, NP, kBacktracking, true, false, REUSE>(diag, L, st);
-        return refine ? launch_diag<N, NP, kBacktracking, false, true, REUSE>(diag, L, st)
-                      : launch_diag<N, NP, kBacktracking, false, false, REUSE>(diag, L, st);
+        if (mixed) return refine ? launch_diag<double, N, NP, kBacktracking, true, true, REUSE>(diag, L, st)
+                                 : launch_diag<double, N, NP, kBacktracking, true, false, REUSE>(diag, L, st);
+        return refine ? launch_diag<double, N, NP, kBacktracking, false, true, REUSE>(diag, L, st)
+                      : launch_diag<double, N, NP, kBacktracking, false, false, REUSE>(diag, L, st);
     }
-    if (mixed) return refine ? launch_diag<N, NP, kStatic, true, true, REUSE>(diag, L, st)
-                             : launch_diag<N, NP, kStatic, true, false, REUSE>(diag, L, st);
-    return refine ? launch_diag<N, NP, kStatic, false, true, REUSE>(diag, L, st)
-                  : launch_diag<N, NP, kStatic, false, false, REUSE>(diag, L, st);
+    if (mixed) return refine ? launch_diag<double, N, NP, kStatic, true, true, REUSE>(diag, L, st)
+                             : launch_diag<double, N, NP, kStatic, true, false, REUSE>(diag, L, st);
+    return refine ? launch_diag<double, N, NP, kStatic, false, true, REUSE>(diag, L, st)
+                  : launch_diag<double, N, NP, kStatic, false, false, REUSE>(diag, L, st);
 }
 
 }  // namespace

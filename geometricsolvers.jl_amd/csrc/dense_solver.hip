@@ -1,7 +1,8 @@
 // libgsolve_hip, dense mid-size systems (n = 16, 24, 32, 48, 64): the launch dispatch over the
 // sizes and their lanes per system (the kernels are in dense_kernel.hpp, instantiated one size per
-// translation unit in dense_n16.hip … dense_n64.hip, and with reuse = k in dense_r16.hip …
-// dense_r64.hip), and the shape setting of the autotuner.
+// translation unit in dense_n16.hip … dense_n64.hip, with reuse = k in dense_r16.hip …
+// dense_r64.hip, and in Float32 in dense_f16.hip … dense_f64.hip), and the shape setting of the
+// autotuner.
 #include <hip/hip_runtime.h>
 
 #include <atomic>
@@ -19,6 +20,17 @@ int launch_dense_n24(int np, int glob, bool mixed, bool refine, bool diag, const
 int launch_dense_n32(int np, int glob, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st);
 int launch_dense_n48(int np, int glob, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st);
 int launch_dense_n64(int np, int glob, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st);
+// The Float32 launches (dense_f16.hip … dense_f64.hip).
+int launch_dense_f32_n16(int np, int glob, bool refine, bool diag, bool reuse, const DenseLaunchT<float>& L,
+                         hipStream_t st);
+int launch_dense_f32_n24(int np, int glob, bool refine, bool diag, bool reuse, const DenseLaunchT<float>& L,
+                         hipStream_t st);
+int launch_dense_f32_n32(int np, int glob, bool refine, bool diag, bool reuse, const DenseLaunchT<float>& L,
+                         hipStream_t st);
+int launch_dense_f32_n48(int np, int glob, bool refine, bool diag, bool reuse, const DenseLaunchT<float>& L,
+                         hipStream_t st);
+int launch_dense_f32_n64(int np, int glob, bool refine, bool diag, bool reuse, const DenseLaunchT<float>& L,
+                         hipStream_t st);
 // The reuse = k launches (dense_r16.hip … dense_r64.hip).
 int launch_dense_reuse_n16(int np, int glob, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st);
 int launch_dense_reuse_n24(int np, int glob, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st);
@@ -82,6 +94,19 @@ int launch_dense(int n, int glob, bool mixed, bool refine, bool diag, bool reuse
         case 32: return launch_dense_n32(np, glob, mixed, refine, diag, L, st);
         case 48: return launch_dense_n48(np, glob, mixed, refine, diag, L, st);
         case 64: return launch_dense_n64(np, glob, mixed, refine, diag, L, st);
+    }
+    return set_error(GS_EINVAL, "the dense family takes n = 16, 24, 32, 48 or 64");
+}
+
+int launch_dense_f32(int n, int glob, bool refine, bool diag, bool reuse, const DenseLaunchT<float>& L,
+                     hipStream_t st) {
+    const int np = lanes_of(n);
+    switch (n) {
+        case 16: return launch_dense_f32_n16(np, glob, refine, diag, reuse, L, st);
+        case 24: return launch_dense_f32_n24(np, glob, refine, diag, reuse, L, st);
+        case 32: return launch_dense_f32_n32(np, glob, refine, diag, reuse, L, st);
+        case 48: return launch_dense_f32_n48(np, glob, refine, diag, reuse, L, st);
+        case 64: return launch_dense_f32_n64(np, glob, refine, diag, reuse, L, st);
     }
     return set_error(GS_EINVAL, "the dense family takes n = 16, 24, 32, 48 or 64");
 }

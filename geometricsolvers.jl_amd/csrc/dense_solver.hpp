@@ -4,22 +4,26 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "../../include/gsolve.h"
 #include "small_solver.hpp"
 
 namespace gs {
 
-struct DenseLaunch {
-    DevParams q;
-    const double* A;      // the family's matrix (n × n, column-major), device
-    const char* P;        // parameter records (b, n doubles each)
+// In the working precision T (double, or float for a Float32 batch).
+template <class T> struct DenseLaunchT {
+    DevParamsT<T> q;
+    const T* A;           // the family's matrix (n × n, column-major), device
+    const char* P;        // parameter records (b, n entries of T each)
     int64_t p_stride;     // bytes between records (0: one shared record)
     int64_t nsys;
-    double* X;
-    gs_status_f64* S;
+    T* X;
+    typename std::conditional<sizeof(T) == 8, gs_status_f64, gs_status_f32>::type* S;
     uint8_t* codes;
     const int64_t* order;
 };
+using DenseLaunch = DenseLaunchT<double>;
 
 // The systems of one launch per workgroup at size n with its current lanes per system.
 int dense_systems_per_block(int n);
@@ -36,5 +40,9 @@ int lanes_of(int n);
 // returns a gs_error code (the launch only; the caller synchronises).
 int launch_dense(int n, int glob, bool mixed, bool refine, bool diag, bool reuse, const DenseLaunch& L,
                  hipStream_t st);
+
+// The same in Float32 (a Float32 batch: the factor is in the working precision, so no mixed form).
+int launch_dense_f32(int n, int glob, bool refine, bool diag, bool reuse, const DenseLaunchT<float>& L,
+                     hipStream_t st);
 
 }  // namespace gs

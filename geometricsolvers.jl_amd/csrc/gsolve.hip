@@ -109,7 +109,7 @@ static const FamilySpec* family_spec(int id) {
         {GS_FAM_HENON_HEILES_MIDPOINT, 4, 4, 32, 16, 16, 16},
         {GS_FAM_FPU_GL3, 48, 32, 256, 8, 0, 0},
         {GS_ACT_HENON_HEILES, 6, 2, 16, 8, 0, 0},
-        {GS_FAM_DENSE_SIN, 0, 0, 0, 8, 0, 0},  // n = 16, 24, 32, 48 or 64, records of n doubles
+        {GS_FAM_DENSE_SIN, 0, 0, 0, 8, 0, 4},  // n = 16, 24, 32, 48 or 64, records of n entries
     };
     for (const auto& s : specs)
         if (s.id == id) return &s;
@@ -139,7 +139,7 @@ static int validate_batch(const gs_method* method, const gs_family* family, int6
     } else if (family->n != fs->n || family->m != fs->n) {
         return fail(GS_EINVAL, "family " + std::to_string(family->id) + " has n = m = " + std::to_string(fs->n));
     }
-    if (f32 && fs->p_bytes32 == 0)
+    if (f32 && fs->p_bytes32 == 0 && family->id != GS_FAM_DENSE_SIN)
         return fail(GS_EFAMILY, "family " + std::to_string(family->id) + " has no Float32 form");
     if (method->linsolve != GS_LU) return fail(GS_EFAMILY, "only LUFactorization has a device form");
     if (method->globalization != GS_STATIC && method->globalization != GS_BACKTRACKING &&
@@ -162,7 +162,7 @@ static int validate_batch(const gs_method* method, const gs_family* family, int6
         return fail(GS_EINVAL, "refinement must be GS_NO_REFINEMENT or GS_REFINEMENT");
     *spec = fs;
     if (nsys == 0) return GS_OK;
-    const int p_bytes = family->id == GS_FAM_DENSE_SIN ? 8 * family->n : (f32 ? fs->p_bytes32 : fs->p_bytes);
+    const int p_bytes = family->id == GS_FAM_DENSE_SIN ? int(sizeof(T)) * family->n : (f32 ? fs->p_bytes32 : fs->p_bytes);
     const int align = f32 ? fs->align32 : fs->align;
     if (!X) return fail(GS_EINVAL, "X must not be NULL for a non-empty batch");
     if (!P) return fail(GS_EINVAL, "this family needs its parameter records P");
@@ -648,6 +648,23 @@ int gs_solve_batch_f32(const gs_method* method, const gs_family* family, int64_t
     auto lock = device_call_lock();
     if ((rc = check_order(order, nsys, st, nullptr)) != GS_OK) return rc;
     bool timed = timing_begin(st);
+    if (family->id == GS_FAM_DENSE_SIN) {
+        // a segment of a wave per system; the factor is in the working precision (Float32)
+        DenseLaunchT<float> L;
+        std::memset(&L, 0, sizeof(L));
+        L.q = a.q;
+        L.A = static_cast<const float*>(family->data);
+        L.P = static_cast<const char*>(P);
+        L.p_stride = p_stride;
+        L.nsys = nsys;
+        L.X = X;
+        L.S = S;
+        L.codes = codes;
+        L.order = order;
+        rc = launch_dense_f32(family->n, method->globalization == GS_BACKTRACKING ? kBacktracking : kStatic,
+                              method->refinement == GS_REFINEMENT, S != nullptr, method->reuse > 0, L, st);
+        return finish_launch(rc, st, timed);
+    }
     rc = launch_small_f32(family->id, method->globalization, method->reuse > 0, a,
                           method->refinement == GS_REFINEMENT, S != nullptr, wg, st);
     return finish_launch(rc, st, timed);

@@ -318,7 +318,9 @@ int gs_solve_batch_reparametrized_f64(const gs_method* method, const gs_family* 
  * options and the method's floats arrive as doubles and are used as Float32 (R(x), as Options{Float32}
  * and ToReal{Float32} hold them; an Options{Float32} passes through exactly). A Float32 batch has no
  * mixed-precision form (its factor is in its working precision already: factor_precision is ignored).
- * The families of the lane-per-system kernels (1–5) are served; family 6 is refused (GS_EFAMILY). */
+ * The families of the lane-per-system kernels (1–5) and the dense family 10 (its matrix A in
+ * gs_family.data as floats; Static or Backtracking, with or without reuse) are served; family 6 is
+ * refused (GS_EFAMILY). */
 int gs_solve_batch_f32(const gs_method* method, const gs_family* family, int64_t nsys,
                        float* X, const void* P, int64_t p_stride, gs_status_f32* S,
                        uint8_t* codes, const int64_t* order, const gs_options_f64* options,

@@ -173,3 +173,53 @@ def test_workgroup_tuner_changes_no_result(gs, workloads, device):
     gs.solve_batch(X1, S1, prob, gs.Newton(), p, opts)
     gs.solve_batch(X2, S2, prob, gs.Newton(), p, opts, workgroupsize=res["workgroupsize"])
     assert torch.equal(X1, X2) and torch.equal(S1, S2)
+
+
+F32_METHODS = {
+    "static": lambda gs: gs.Newton(),
+    "backtracking": lambda gs: gs.Newton(globalization=gs.Backtracking()),
+    "reuse4": lambda gs: gs.Newton(reuse=4),
+    "backtracking-reuse3": lambda gs: gs.Newton(globalization=gs.Backtracking(), reuse=3),
+    "refine-regularized": lambda gs: gs.Newton(refinement=gs.Refinement(), regularization=1e-3),
+    "best-backtracking": lambda gs: gs.Newton(globalization=gs.Backtracking(), best=True),
+    # LUFactorization(Float32) on a Float32 batch is the working precision: the same solve
+    "f32-factor": lambda gs: gs.Newton(linsolve=gs.LUFactorization("float32")),
+}
+
+
+@pytest.mark.parametrize("n", [16, 24, 32, 48, 64])
+@pytest.mark.parametrize("name", list(F32_METHODS))
+def test_dense_float32_bitwise(gs, orc, workloads, device, n, name):
+    """A Float32 batch of the dense family (the element type of the reference's device grid,
+    benchmark/devices.jl:47): A, b and the start in Float32, the family's FP32 sine and cosine, the
+    LU and every fold in Float32, SolverStatus{Float32}. Compared with the oracle's Float32 solve byte
+    for byte (iterates and the 42 meaningful status bytes), under the grid's tolerance √eps(Float32)
+    and under MaxNorm."""
+    nsys = 2048 if n < 32 else (1024 if n == 32 else 512)
+    A, B, X0 = _batch(workloads, n, nsys, seed=n + 1)
+    # a start of 1e300 becomes 1e30: far beyond the others and finite in Float32
+    A32, B32 = A.astype(np.float32), B.astype(np.float32)
+    X32 = np.where(np.abs(X0) > 1e30, 1e30, X0).astype(np.float32)
+    fam = gs.DenseSin(A32.astype(np.float64))
+    m = F32_METHODS[name](gs)
+    tol = float(np.sqrt(np.finfo(np.float32).eps))
+    for opts in (gs.Options(eltype="float32", f_abstol=tol, f_reltol=0.0, max_iterations=40),
+                 gs.Options(eltype="float32", norm=gs.MaxNorm(), max_iterations=25)):
+        print(f"dense f32 n={n} {name} norm={type(opts.norm).__name__}", flush=True)
+        X = torch.as_tensor(X32, device=device).clone()
+        S = gs.status_buffer(nsys, device, "float32")
+        gs.solve_batch(X, S, gs.NonlinearProblem(fam), m, gs.PerSystem(torch.as_tensor(B32, device=device)), opts)
+        x, st = X.cpu().numpy(), gs.statuses(S)
+        xo, so = orc.solve_batch(m, fam, X32.copy(), gs.PerSystem(B32), opts)
+        assert x.dtype == np.float32 and st.dtype.itemsize == 44
+        bad_x = np.nonzero((x.view(np.uint32) != xo.view(np.uint32)).any(axis=1))[0]
+        sb = st.view(np.uint8).reshape(nsys, 44)[:, :42]
+        ob = so.view(np.uint8).reshape(nsys, 44)[:, :42]
+        bad_s = np.nonzero((sb != ob).any(axis=1))[0]
+        msg = f"status {bad_s[0]}: {st[bad_s[0]]} vs {so[bad_s[0]]}" if bad_s.size else (
+            f"x {bad_x[0]}" if bad_x.size else "")
+        assert bad_s.size == 0 and bad_x.size == 0, f"{bad_s.size} statuses, {bad_x.size} iterates differ; {msg}"
+    codes = np.bincount(st["code"], minlength=6)
+    assert codes[0] > 0.6 * nsys and codes[4] > 0
+    if "reuse" in name:
+        assert st["reused"].any()
