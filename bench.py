@@ -543,14 +543,122 @@ def bench_dense(args, gs, workloads, torch):
     }
 
 
+FP32_PEAK_TFLOPS = 157.3
+
+# The reference's published R2 grid cells for the §10.E problem at n = 16 in Float32 (RTX 4090;
+# benchmark/REPORT.md, the grid table): systems/s at 10^6 and 10^7 systems.
+REFERENCE_GRID_N16 = {"gs-newton": {"1e6": 2.06e7, "1e7": 2.11e7},
+                      "gs-reuse4": {"1e6": 9.95e6, "1e7": 1.01e7},
+                      "gs-backtracking": {"1e6": 1.54e7, "1e7": 1.56e7}}
+
+
+def bench_dense_grid(args, gs, workloads, torch):
+    """The reference's device throughput grid (benchmark/devices.jl:446-500) at n = args.dense_n in its
+    element type, Float32: the §10.E batch from x₀ = 0 with f_abstol = √eps(Float32), f_reltol = 0,
+    max_iterations = 200, for the variants gs-newton (Newton()), gs-reuse4 (Newton(reuse = 4)) and
+    gs-backtracking (Newton(Backtracking())), at 10^6 and 10^7 systems. As the reference's cell, a
+    launch is timed to its synchronisation on a fresh start made before the clock (one warm-up, then
+    the median of `steps` launches), and the cell is n_sys / that time. `value` is gs-newton at 10^7."""
+    n = args.dense_n
+    dev = torch.device("cuda:0")
+    tol = float(np.sqrt(np.finfo(np.float32).eps))
+    opts = gs.Options(eltype="float32", f_abstol=tol, f_reltol=0.0, max_iterations=200)
+    A, _, _ = workloads.tene_batch(n, 1)
+    A32 = A.astype(np.float32)
+    fam = gs.DenseSin(A32.astype(np.float64))
+    prob = gs.NonlinearProblem(fam)
+    variants = {"gs-newton": gs.Newton(), "gs-reuse4": gs.Newton(reuse=4),
+                "gs-backtracking": gs.Newton(globalization=gs.Backtracking())}
+    K = max(args.steps, 3)
+    cells, kernel_ms, codes_out, iters_out = {}, {}, {}, {}
+    flops = 0.0
+    for nsys, label in ((10**6, "1e6"), (10**7, "1e7")):
+        rng = np.random.default_rng(1)
+        xs = (2.0 * rng.random((nsys, n), dtype=np.float64) - 1.0).astype(np.float32)
+        B = (xs @ A32.T + np.sin(xs) / np.float32(10)).astype(np.float32)  # b = A x* + sin.(x*)/10 in Float32
+        p = gs.PerSystem(torch.as_tensor(B, device=dev))
+        del xs
+        S = gs.status_buffer(nsys, dev, "float32")
+        for name, m in variants.items():
+            gs.solve_batch(torch.zeros((nsys, n), dtype=torch.float32, device=dev), S, prob, m, p, opts)
+            torch.cuda.synchronize()
+            times = []
+            for _ in range(K):
+                X = torch.zeros((nsys, n), dtype=torch.float32, device=dev)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                gs.solve_batch(X, S, prob, m, p, opts)
+                times.append(time.perf_counter() - t0)
+            med = float(np.median(times))
+            st = gs.statuses(S)
+            cells.setdefault(name, {})[label] = nsys / med
+            kernel_ms.setdefault(name, {})[label] = med * 1e3
+            codes_out.setdefault(name, {})[label] = np.bincount(st["code"], minlength=6).tolist()
+            iters_out.setdefault(name, {})[label] = float(st["iterations"].mean())
+            if name == "gs-newton" and label == "1e7":
+                flops = dense_flops_per_iter(n) * float(st["iterations"].sum()) / med / 1e12
+            del X
+        del S, p, B
+        torch.cuda.empty_cache()
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as orc
+
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+        k = 1 << 15
+        rng = np.random.default_rng(1)
+        xs = (2.0 * rng.random((k, n)) - 1.0).astype(np.float32)
+        Bk = (xs @ A32.T + np.sin(xs) / np.float32(10)).astype(np.float32)
+        reps, cel, (_, so) = _timed_oracle(
+            lambda: orc.solve_batch(gs.Newton(), fam, np.zeros((k, n), dtype=np.float32), gs.PerSystem(Bk), opts,
+                                    nthreads=threads), 8.0)
+        cpu = {"value": k * reps / cel, "unit": "systems/s", "cores": threads, "kind": "port",
+               "sample": f"gs-newton in Float32 on {k} systems of the same kind x {reps} repetitions, {cel:.1f} s "
+                         "wall, OpenMP schedule(static) over systems"}
+    ref = REFERENCE_GRID_N16 if n == 16 else None
+    return {
+        "metric": "systems/s, the §10.E R2 device grid in Float32 (benchmark/devices.jl), 1 GPU",
+        "value": cells["gs-newton"]["1e7"],
+        "unit": "systems/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": 1,
+        "ms_per_step": kernel_ms["gs-newton"]["1e7"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (A = 2I + R from seed n, x* ~ U[-1,1]^n, numpy PCG64; rounded to Float32)",
+        "config": {"workload": f"dense_grid (§10.E A x + sin(x)/10 - b) n={n}, Float32, x0 = 0, abstol sqrt(eps(Float32)), "
+                               "reltol 0, max_iterations 200, status form",
+                   "cells_systems_per_s": cells, "launch_ms": kernel_ms, "codes": codes_out,
+                   "mean_iterations": iters_out,
+                   "reference_published_rtx4090": ref,
+                   "vs_reference_rtx4090": ({v: {c: cells[v][c] / ref[v][c] for c in ref[v]} for v in ref}
+                                            if ref else None)},
+        "roofline": {
+            "bound": "fp32 valu",
+            "kernel": f"dense_kernel<float, {n}, Static, status>, gs-newton at 10^7 systems; achieved = (2/3 n^3 + 4 n^2) "
+                      "flop per Newton iteration x the batch's iterations / launch time",
+            "achieved": flops,
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": flops / FP32_PEAK_TFLOPS,
+            "traffic": None,
+        },
+        "cpu_baseline": cpu,
+    }
+
+
 def sub_results():
     """Compact lines of the other configurations (the n = 48 stage systems, the uneven least-squares
-    batch, the dense family at n = 16 and n = 64), each measured by its own `--workload` run in a child process (fewer
+    batch, the dense family at n = 16 and n = 64, the Float32 device grid at n = 16), each measured by its own `--workload` run in a child process (fewer
     steps, no CPU baseline), one after the other; a child that fails ends the series, and its exit status
     is recorded in place of its line."""
     out = {}
     for name, w in (("fpu_gl3", ["fpu_gl3"]), ("expsum_lsq", ["expsum_lsq"]),
-                    ("dense_sin_n16", ["dense_sin", "--dense-n", "16"]), ("dense_sin_n64", ["dense_sin", "--dense-n", "64"])):
+                    ("dense_sin_n16", ["dense_sin", "--dense-n", "16"]), ("dense_sin_n64", ["dense_sin", "--dense-n", "64"]),
+                    ("dense_grid_n16", ["dense_grid", "--dense-n", "16"])):
         cmd = [sys.executable, str(ROOT / "bench.py"), "--workload", *w, "--steps", "3", "--warmup", "1",
                "--no-cpu-baseline"]
         print(f"bench: sub-result {name}", file=sys.stderr, flush=True)
@@ -568,6 +676,9 @@ def sub_results():
                      "workload": line["config"]["workload"]}
         if "plain_value" in line["config"]:
             out[name]["plain_value"] = line["config"]["plain_value"]
+        if "cells_systems_per_s" in line["config"]:
+            out[name]["cells_systems_per_s"] = line["config"]["cells_systems_per_s"]
+            out[name]["vs_reference_rtx4090"] = line["config"]["vs_reference_rtx4090"]
     return out
 
 
@@ -577,10 +688,10 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--epochs", type=int, default=8)
-    ap.add_argument("--workload", choices=("henon_heiles", "fpu_gl3", "expsum_lsq", "dense_sin"),
+    ap.add_argument("--workload", choices=("henon_heiles", "fpu_gl3", "expsum_lsq", "dense_sin", "dense_grid"),
                     default="henon_heiles")
     ap.add_argument("--dense-n", type=int, choices=(16, 24, 32, 48, 64), default=16,
-                    help="the size of the dense_sin workload")
+                    help="the size of the dense_sin and dense_grid workloads")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--save-net", default=None, help="save the trained network (.npz) of the default workload")
     ap.add_argument("--load-net", default=None, help="load a network saved by --save-net instead of training")
@@ -602,7 +713,7 @@ def main() -> int:
     gs = __graft_entry__.load_package()
     workloads = importlib.import_module(gs.__name__ + ".workloads")
     fn = {"henon_heiles": bench_henon_heiles, "fpu_gl3": bench_fpu_gl3, "expsum_lsq": bench_expsum,
-          "dense_sin": bench_dense}[args.workload]
+          "dense_sin": bench_dense, "dense_grid": bench_dense_grid}[args.workload]
     if args.workload != "henon_heiles" and "--steps" not in sys.argv:
         args.steps, args.warmup = 5, 1
     line = fn(args, gs, workloads, torch)
