@@ -545,11 +545,29 @@ def bench_dense(args, gs, workloads, torch):
 
 FP32_PEAK_TFLOPS = 157.3
 
-# The reference's published R2 grid cells for the §10.E problem at n = 16 in Float32 (RTX 4090;
-# benchmark/REPORT.md, the grid table): systems/s at 10^6 and 10^7 systems.
-REFERENCE_GRID_N16 = {"gs-newton": {"1e6": 2.06e7, "1e7": 2.11e7},
-                      "gs-reuse4": {"1e6": 9.95e6, "1e7": 1.01e7},
-                      "gs-backtracking": {"1e6": 1.54e7, "1e7": 1.56e7}}
+# The reference's published R2 grid cells for the §10.E problem in Float32 (benchmark/REPORT.md, the
+# grid table): systems/s at 10^6 and 10^7 systems, per machine, n and variant. The ROCm card's band
+# stops at n = 8 (benchmark/devices.jl BANDS).
+def _cells(rows):
+    return {v: {"1e6": a, "1e7": b} for v, a, b in rows}
+
+
+REFERENCE_GRID = {
+    "rtx4090": {
+        2: _cells([("gs-newton", 2.10e9, 2.24e9), ("gs-reuse4", 6.61e8, 7.09e8), ("gs-backtracking", 1.55e9, 1.80e9)]),
+        4: _cells([("gs-newton", 9.06e8, 1.01e9), ("gs-reuse4", 2.19e8, 2.29e8), ("gs-backtracking", 6.71e8, 7.24e8)]),
+        6: _cells([("gs-newton", 2.93e8, 3.05e8), ("gs-reuse4", 9.69e7, 1.01e8), ("gs-backtracking", 2.31e8, 2.39e8)]),
+        8: _cells([("gs-newton", 1.43e8, 1.47e8), ("gs-reuse4", 5.07e7, 5.19e7), ("gs-backtracking", 1.16e8, 1.20e8)]),
+        12: _cells([("gs-newton", 4.49e7, 4.55e7), ("gs-reuse4", 2.01e7, 2.03e7), ("gs-backtracking", 3.72e7, 3.76e7)]),
+        16: _cells([("gs-newton", 2.06e7, 2.11e7), ("gs-reuse4", 9.95e6, 1.01e7), ("gs-backtracking", 1.54e7, 1.56e7)]),
+    },
+    "rx7900xtx": {
+        2: _cells([("gs-newton", 2.30e9, 2.05e9), ("gs-reuse4", 4.47e8, 7.11e8), ("gs-backtracking", 3.90e9, 2.32e9)]),
+        4: _cells([("gs-newton", 2.49e8, 4.58e8), ("gs-reuse4", 1.56e8, 1.68e8), ("gs-backtracking", 2.07e8, 3.74e8)]),
+        6: _cells([("gs-newton", 1.45e8, 2.72e8), ("gs-reuse4", 8.62e7, 9.59e7), ("gs-backtracking", 1.52e8, 1.98e8)]),
+        8: _cells([("gs-newton", 6.05e7, 6.23e7), ("gs-reuse4", 1.98e7, 2.08e7), ("gs-backtracking", 4.45e7, 4.61e7)]),
+    },
+}
 
 
 def bench_dense_grid(args, gs, workloads, torch):
@@ -615,7 +633,7 @@ def bench_dense_grid(args, gs, workloads, torch):
         cpu = {"value": k * reps / cel, "unit": "systems/s", "cores": threads, "kind": "port",
                "sample": f"gs-newton in Float32 on {k} systems of the same kind x {reps} repetitions, {cel:.1f} s "
                          "wall, OpenMP schedule(static) over systems"}
-    ref = REFERENCE_GRID_N16 if n == 16 else None
+    ref = {m: REFERENCE_GRID[m][n] for m in REFERENCE_GRID if n in REFERENCE_GRID[m]}
     return {
         "metric": "systems/s, the §10.E R2 device grid in Float32 (benchmark/devices.jl), 1 GPU",
         "value": cells["gs-newton"]["1e7"],
@@ -633,13 +651,15 @@ def bench_dense_grid(args, gs, workloads, torch):
                                "reltol 0, max_iterations 200, status form",
                    "cells_systems_per_s": cells, "launch_ms": kernel_ms, "codes": codes_out,
                    "mean_iterations": iters_out,
-                   "reference_published_rtx4090": ref,
-                   "vs_reference_rtx4090": ({v: {c: cells[v][c] / ref[v][c] for c in ref[v]} for v in ref}
-                                            if ref else None)},
+                   "reference_published": ref,
+                   "vs_reference": {m: {v: {c: cells[v][c] / r[v][c] for c in r[v]} for v in r}
+                                    for m, r in ref.items()}},
         "roofline": {
             "bound": "fp32 valu",
-            "kernel": f"dense_kernel<float, {n}, Static, status>, gs-newton at 10^7 systems; achieved = (2/3 n^3 + 4 n^2) "
-                      "flop per Newton iteration x the batch's iterations / launch time",
+            "kernel": (f"dense_kernel<float, {n}, Static, status>" if n > 8 else
+                       f"solve_small_kernel<DenseSin {n}, Static, Float32, status>") +
+                      ", gs-newton at 10^7 systems; achieved = (2/3 n^3 + 4 n^2) flop per Newton iteration x the "
+                      "batch's iterations / launch time",
             "achieved": flops,
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
@@ -678,7 +698,7 @@ def sub_results():
             out[name]["plain_value"] = line["config"]["plain_value"]
         if "cells_systems_per_s" in line["config"]:
             out[name]["cells_systems_per_s"] = line["config"]["cells_systems_per_s"]
-            out[name]["vs_reference_rtx4090"] = line["config"]["vs_reference_rtx4090"]
+            out[name]["vs_reference"] = line["config"]["vs_reference"]
     return out
 
 
@@ -690,7 +710,7 @@ def main() -> int:
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--workload", choices=("henon_heiles", "fpu_gl3", "expsum_lsq", "dense_sin", "dense_grid"),
                     default="henon_heiles")
-    ap.add_argument("--dense-n", type=int, choices=(16, 24, 32, 48, 64), default=16,
+    ap.add_argument("--dense-n", type=int, choices=(2, 4, 6, 8, 12, 16, 24, 32, 48, 64), default=16,
                     help="the size of the dense_sin and dense_grid workloads")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--save-net", default=None, help="save the trained network (.npz) of the default workload")

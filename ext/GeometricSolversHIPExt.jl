@@ -211,9 +211,9 @@ end
 The dense device problem of the reference's benchmarks (`TenEResidual`,
 `benchmark/device_problems.jl:147-158`): `F(x, b) = A x + sin.(x)/10 − b`, the system's own `b` as
 its parameter (a `PerSystem` of `SVector{n, T}`), `J = A + Diagonal(cos.(x)/10)`. `A` (n × n,
-served on the device for n = 16, 24, 32, 48, 64) is kept as an `SMatrix` for the generic path and as
-column-major device matrices in Float64 and rounded to Float32 (the `tenE_matrix(n, Float32)` of a
-Float32 batch), which the library reads through `gs_family.data`.
+served on the device for n = 2, 4, 6, 8, 12, 16, 24, 32, 48, 64) is kept as an `SMatrix` for the
+generic path and as column-major device matrices in Float64 and rounded to Float32 (the
+`tenE_matrix(n, Float32)` of a Float32 batch), which the library reads through `gs_family.data`.
 """
 struct DenseSin{N, L} <: HIPFamily
     A::SMatrix{N, N, Float64, L}
@@ -294,10 +294,12 @@ end
 # to the generic path.
 served(::HIPFamily, cm, ::Type{R}) where {R} = true
 served(::FpuGl3, cm, ::Type{R}) where {R} = R === Float64 && cm.globalization in (0, 1, 3)
-# the dense kernel: n = 16, 24, 32, 48, 64 in Float64 or Float32, Static or Backtracking, with or
+# the dense family in Float64 or Float32: n = 2, 4, 6, 8 on the lane-per-system kernels (every
+# globalization), n = 12, 16, 24, 32, 48, 64 on the dense kernel (Static or Backtracking); with or
 # without `reuse`
 served(::DenseSin{N}, cm, ::Type{R}) where {N, R} = (R === Float64 || R === Float32) &&
-    N in (16, 24, 32, 48, 64) && cm.globalization in (0, 1)
+    ((N in (2, 4, 6, 8) && cm.globalization in (0, 1, 3)) ||
+     (N in (12, 16, 24, 32, 48, 64) && cm.globalization in (0, 1)))
 
 # ---- the method and options as ABI structs ----------------------------------------------------
 

@@ -489,9 +489,13 @@ class QuadMap2(_SimpleFamily):
         return _torch().float64
 
 
+# The orders of the dense family: up to 8 one lane per system, from 12 a wave segment per system.
+DENSE_ORDERS = (2, 4, 6, 8, 12, 16, 24, 32, 48, 64)
+
+
 class DenseSin(_SimpleFamily):
     """TenEResidual(A) of the reference's device benchmark (benchmark/device_problems.jl:133-175):
-    F(x, b) = A x + sin.(x) ./ 10 − b, n = 16, 24, 32, 48 or 64, with A (n × n) the family's own, shared by the
+    F(x, b) = A x + sin.(x) ./ 10 − b, n in DENSE_ORDERS, with A (n × n) the family's own, shared by the
     batch, and p = PerSystem(B), one b per system. The Jacobian is A + Diagonal(cos.(x) ./ 10). Its sine
     and cosine are one fixed algorithm (Cody–Waite reduction, Taylor polynomials), the same on the
     device and in the oracle. A Float32 batch (the device grid's element type, devices.jl:47) reads A
@@ -501,8 +505,8 @@ class DenseSin(_SimpleFamily):
 
     def __init__(self, A):
         a = np.asarray(A.detach().cpu().numpy() if hasattr(A, "detach") else A, dtype=np.float64)
-        if a.ndim != 2 or a.shape[0] != a.shape[1] or a.shape[0] not in (16, 24, 32, 48, 64):
-            raise ArgumentError("DenseSin takes a square A of order 16, 24, 32, 48 or 64")
+        if a.ndim != 2 or a.shape[0] != a.shape[1] or a.shape[0] not in DENSE_ORDERS:
+            raise ArgumentError("DenseSin takes a square A of order 2, 4, 6, 8, 12, 16, 24, 32, 48 or 64")
         self.A = a
         self.n = self.p_len = a.shape[0]
         self._dev = {}

@@ -79,7 +79,7 @@ def tune_lsq(X0, data, method=None, options=None, candidates=DEFAULT_CANDIDATES,
 
 # ---- the dense mid-size family: lanes per system ----------------------------------------------------
 
-DENSE_LANES = {16: (16, 32), 24: (32, 64), 32: (32, 64), 48: (64,), 64: (64,)}
+DENSE_LANES = {12: (16, 32), 16: (16, 32), 24: (32, 64), 32: (32, 64), 48: (64,), 64: (64,)}
 
 
 def dense_get_shape(n: int) -> int:
@@ -134,8 +134,11 @@ def tune_dense(X0, prob, method, p, options=None, repeats: int = 2) -> dict:
     if not isinstance(fam, DenseSin):
         raise TypeError("tune_dense takes a NonlinearProblem over DenseSin")
     n = fam.n
+    if n not in DENSE_LANES:
+        # n ≤ 8 runs one lane per system: no segment width to choose
+        return {"n": n, "lanes": None, "seconds": {}}
     X = torch.empty_like(X0)
-    S = status_buffer(X0.shape[0], X0.device)
+    S = status_buffer(X0.shape[0], X0.device, "float32" if X0.dtype == torch.float32 else "float64")
     run = _timer(lambda: solve_batch(X, S, prob, method, p, options), X, X0, repeats)
     timings = {}
     for lanes in DENSE_LANES[n]:

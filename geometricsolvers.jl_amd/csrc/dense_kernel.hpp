@@ -54,7 +54,8 @@ namespace {
 // NP lanes per system (16, 32 or 64, at least n), G = 64 / NP systems per wave, four waves per
 // workgroup. The lanes per system are the launch's tunable choice (gs_dense_set_shape).
 template <int N, int NP_> struct Shape {
-    static_assert(N == 16 || N == 24 || N == 32 || N == 48 || N == 64, "the dense solve takes n = 16, 24, 32, 48 or 64");
+    static_assert(N == 12 || N == 16 || N == 24 || N == 32 || N == 48 || N == 64,
+                  "the dense solve takes n = 12, 16, 24, 32, 48 or 64");
     static_assert((NP_ == 16 || NP_ == 32 || NP_ == 64) && NP_ >= N, "a segment of 16, 32 or 64 lanes, one per row");
     static constexpr int NP = NP_;                // lanes per system
     static constexpr int G = 64 / NP;             // systems per wave
@@ -208,79 +209,6 @@ template <int NP, class T> GS_DEV int seg_exponent(T a, bool live) {
 }
 
 // ---- the family: A x + sin.(x)/10 − b ---------------------------------------------------------------
-
-// sin and cos of the family: k = rint(x · 2/π); r = x − k·π/2 with π/2 in three parts (Cody–Waite,
-// each product folded by FMA); Taylor polynomials of sin r and cos r in r² through r¹⁷ and r¹⁶ by
-// Horner with FMA; the quadrant k mod 4. The coefficients (−1)^j/(2j+1)! and (−1)^j/(2j)! are
-// rounded once. NaN for a non-finite x or |x| ≥ 2^50; sin(±0) = ±0. The oracle evaluates the same steps, so the
-// residual and the Jacobian have the same bits on both sides.
-__constant__ double kSinC[8] = {-0x1.5555555555555p-3, 0x1.1111111111111p-7, -0x1.a01a01a01a01ap-13,
-                                0x1.71de3a556c734p-19, -0x1.ae64567f544e4p-26, 0x1.6124613a86d09p-33,
-                                -0x1.ae7f3e733b81fp-41, 0x1.952c77030ad4ap-49};
-__constant__ double kCosC[8] = {-0x1.0000000000000p-1, 0x1.5555555555555p-5, -0x1.6c16c16c16c17p-10,
-                                0x1.a01a01a01a01ap-16, -0x1.27e4fb7789f5cp-22, 0x1.1eed8eff8d898p-29,
-                                -0x1.93974a8c07c9dp-37, 0x1.ae7f3e733b81fp-45};
-
-GS_DEV void fsincos(double x, double& sn, double& cs) {
-    if (!(__builtin_fabs(x) < 0x1p50)) {
-        sn = cs = dnan();
-        return;
-    }
-    if (x == 0.0) {  // sin(±0) = ±0
-        sn = x;
-        cs = 1.0;
-        return;
-    }
-    const double k = __builtin_rint(x * 0x1.45f306dc9c883p-1);
-    double r = __builtin_fma(-k, 0x1.921fb54442d18p+0, x);
-    r = __builtin_fma(-k, 0x1.1a62633145c07p-54, r);
-    r = __builtin_fma(-k, -0x1.f1976b7ed8fbcp-110, r);
-    const double z = r * r;
-    double ps = kSinC[7], pc = kCosC[7];
-#pragma unroll
-    for (int j = 6; j >= 0; --j) {
-        ps = __builtin_fma(ps, z, kSinC[j]);
-        pc = __builtin_fma(pc, z, kCosC[j]);
-    }
-    const double s0 = __builtin_fma(r * z, ps, r);
-    const double c0 = __builtin_fma(z, pc, 1.0);
-    const int q = int(int64_t(k) & 3);
-    sn = q == 0 ? s0 : (q == 1 ? c0 : (q == 2 ? -s0 : -c0));
-    cs = q == 0 ? c0 : (q == 1 ? -s0 : (q == 2 ? -c0 : s0));
-}
-
-// The same steps in FP32: π/2 in three FP32 parts, the polynomials through r¹¹ and r¹⁰ (coefficients
-// rounded once to FP32); NaN for a non-finite x or |x| ≥ 2^22.
-__constant__ float kSinCf[5] = {-0x1.555556p-3f, 0x1.111112p-7f, -0x1.a01a02p-13f, 0x1.71de3ap-19f, -0x1.ae6456p-26f};
-__constant__ float kCosCf[5] = {-0x1.000000p-1f, 0x1.555556p-5f, -0x1.6c16c2p-10f, 0x1.a01a02p-16f, -0x1.27e4fcp-22f};
-
-GS_DEV void fsincos(float x, float& sn, float& cs) {
-    if (!(__builtin_fabsf(x) < 0x1p22f)) {
-        sn = cs = tnan<float>();
-        return;
-    }
-    if (x == 0.0f) {  // sin(±0) = ±0
-        sn = x;
-        cs = 1.0f;
-        return;
-    }
-    const float k = __builtin_rintf(x * 0x1.45f306p-1f);
-    float r = __builtin_fmaf(-k, 0x1.921fb6p+0f, x);
-    r = __builtin_fmaf(-k, -0x1.777a5cp-25f, r);
-    r = __builtin_fmaf(-k, -0x1.ee59dap-50f, r);
-    const float z = r * r;
-    float ps = kSinCf[4], pc = kCosCf[4];
-#pragma unroll
-    for (int j = 3; j >= 0; --j) {
-        ps = __builtin_fmaf(ps, z, kSinCf[j]);
-        pc = __builtin_fmaf(pc, z, kCosCf[j]);
-    }
-    const float s0 = __builtin_fmaf(r * z, ps, r);
-    const float c0 = __builtin_fmaf(z, pc, 1.0f);
-    const int q = int(int64_t(k) & 3);
-    sn = q == 0 ? s0 : (q == 1 ? c0 : (q == 2 ? -s0 : -c0));
-    cs = q == 0 ? c0 : (q == 1 ? -s0 : (q == 2 ? -c0 : s0));
-}
 
 // TenEResidual (device_problems.jl:147-157): row i of F and of J = A + Diagonal(cos.(x)/10), the
 // matrix-vector product as StaticArrays' muladd fold over the row.

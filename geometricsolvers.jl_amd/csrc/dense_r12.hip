@@ -1,0 +1,13 @@
+// The dense mid-size solve at n = 12 with reuse = k (dense_kernel.hpp), lanes per system 16 or 32.
+#include "dense_kernel.hpp"
+
+namespace gs {
+
+int launch_dense_reuse_n12(int np, int glob, bool mixed, bool refine, bool diag, const DenseLaunch& L,
+                           hipStream_t st) {
+    if (np == 16) return launch_np<12, 16, true>(glob, mixed, refine, diag, L, st);
+    if (np == 32) return launch_np<12, 32, true>(glob, mixed, refine, diag, L, st);
+    return set_error(GS_EINVAL, "no dense launch with these lanes per system at n = 12");
+}
+
+}  // namespace gs

@@ -47,6 +47,7 @@ struct FamArgs {
     const char* P;       // parameter records (device)
     int64_t p_stride;    // bytes between records; 0 = one shared record
     double shared[8];    // scalar parameters shared by every system
+    const void* data;    // the family's own device data (gs_family.data: A of the dense family)
     DeflArgs defl;       // the deflation of gs_solve_batch_deflated_f64
     StatArgs stat;       // the index sets and datum of gs_solve_batch_stationarity_f64
     ChartArgs chart;     // the chart of gs_solve_batch_reparametrized_f64
@@ -286,6 +287,114 @@ GS_DEV double pow_int(double x, int n) {
     const double err = __builtin_fma(y, xnlo, x * ynlo);
     return (isfin(x) && isfin(err)) ? __builtin_fma(x, y, err) : x * y;
 }
+
+// The sine and cosine of the dense family (family 10), one fixed algorithm that the oracle evaluates
+// the same way, so that its F and J have the same bits on both sides: k = rint(x · 2/π); r = x − k·π/2
+// with π/2 in three parts (Cody–Waite, each product folded by FMA); the Taylor polynomials of sin r
+// and cos r in r² by Horner with FMA (through r¹⁷ and r¹⁶ in FP64, r¹¹ and r¹⁰ in FP32; the
+// coefficients (−1)^j/(2j+1)! and (−1)^j/(2j)! rounded once); the quadrant k mod 4. NaN for a
+// non-finite x or |x| ≥ 2^50 (2^22 in FP32); sin(±0) = ±0.
+GS_DEV void fsincos(double x, double& sn, double& cs) {
+    constexpr double kS[8] = {-0x1.5555555555555p-3, 0x1.1111111111111p-7, -0x1.a01a01a01a01ap-13,
+                              0x1.71de3a556c734p-19, -0x1.ae64567f544e4p-26, 0x1.6124613a86d09p-33,
+                              -0x1.ae7f3e733b81fp-41, 0x1.952c77030ad4ap-49};
+    constexpr double kC[8] = {-0x1.0000000000000p-1, 0x1.5555555555555p-5, -0x1.6c16c16c16c17p-10,
+                              0x1.a01a01a01a01ap-16, -0x1.27e4fb7789f5cp-22, 0x1.1eed8eff8d898p-29,
+                              -0x1.93974a8c07c9dp-37, 0x1.ae7f3e733b81fp-45};
+    if (!(__builtin_fabs(x) < 0x1p50)) {
+        sn = cs = dnan();
+        return;
+    }
+    if (x == 0.0) {  // sin(±0) = ±0
+        sn = x;
+        cs = 1.0;
+        return;
+    }
+    const double k = __builtin_rint(x * 0x1.45f306dc9c883p-1);
+    double r = __builtin_fma(-k, 0x1.921fb54442d18p+0, x);
+    r = __builtin_fma(-k, 0x1.1a62633145c07p-54, r);
+    r = __builtin_fma(-k, -0x1.f1976b7ed8fbcp-110, r);
+    const double z = r * r;
+    double ps = kS[7], pc = kC[7];
+#pragma unroll
+    for (int j = 6; j >= 0; --j) {
+        ps = __builtin_fma(ps, z, kS[j]);
+        pc = __builtin_fma(pc, z, kC[j]);
+    }
+    const double s0 = __builtin_fma(r * z, ps, r);
+    const double c0 = __builtin_fma(z, pc, 1.0);
+    const int q = int(int64_t(k) & 3);
+    sn = q == 0 ? s0 : (q == 1 ? c0 : (q == 2 ? -s0 : -c0));
+    cs = q == 0 ? c0 : (q == 1 ? -s0 : (q == 2 ? -c0 : s0));
+}
+
+GS_DEV void fsincos(float x, float& sn, float& cs) {
+    constexpr float kS[5] = {-0x1.555556p-3f, 0x1.111112p-7f, -0x1.a01a02p-13f, 0x1.71de3ap-19f, -0x1.ae6456p-26f};
+    constexpr float kC[5] = {-0x1.000000p-1f, 0x1.555556p-5f, -0x1.6c16c2p-10f, 0x1.a01a02p-16f, -0x1.27e4fcp-22f};
+    if (!(__builtin_fabsf(x) < 0x1p22f)) {
+        sn = cs = tnan<float>();
+        return;
+    }
+    if (x == 0.0f) {  // sin(±0) = ±0
+        sn = x;
+        cs = 1.0f;
+        return;
+    }
+    const float k = __builtin_rintf(x * 0x1.45f306p-1f);
+    float r = __builtin_fmaf(-k, 0x1.921fb6p+0f, x);
+    r = __builtin_fmaf(-k, -0x1.777a5cp-25f, r);
+    r = __builtin_fmaf(-k, -0x1.ee59dap-50f, r);
+    const float z = r * r;
+    float ps = kS[4], pc = kC[4];
+#pragma unroll
+    for (int j = 3; j >= 0; --j) {
+        ps = __builtin_fmaf(ps, z, kS[j]);
+        pc = __builtin_fmaf(pc, z, kC[j]);
+    }
+    const float s0 = __builtin_fmaf(r * z, ps, r);
+    const float c0 = __builtin_fmaf(z, pc, 1.0f);
+    const int q = int(int64_t(k) & 3);
+    sn = q == 0 ? s0 : (q == 1 ? c0 : (q == 2 ? -s0 : -c0));
+    cs = q == 0 ? c0 : (q == 1 ? -s0 : (q == 2 ? -c0 : s0));
+}
+
+// The dense family (TenEResidual, device_problems.jl:147-157) at n ≤ 8 for the lane-per-system
+// kernels: F = A x + sin.(x)/10 − b and J = A + Diagonal(cos.(x)/10), the product as StaticArrays'
+// muladd fold over each row. A (column-major, shared by the batch) stays in device memory: every
+// lane reads the same entries, so they come through the scalar cache; b is the system's record.
+template <class R, int NN> struct FamDenseSinSmallT {
+    using T = R;
+    static constexpr int N = NN;
+    const T* A;
+    T b[N];
+    GS_DEV void load(const FamArgs& a, int64_t idx) {
+        A = static_cast<const T*>(a.data);
+        const T* p = reinterpret_cast<const T*>(record(a, idx));
+#pragma unroll
+        for (int i = 0; i < N; ++i) b[i] = p[i];
+    }
+    GS_DEV void F(const T (&x)[N], T (&r)[N]) const {
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            T acc = A[i] * x[0];
+#pragma unroll
+            for (int j = 1; j < N; ++j) acc = tfma(A[i + j * N], x[j], acc);
+            T sn, cs;
+            fsincos(x[i], sn, cs);
+            r[i] = (acc + sn / T(10)) - b[i];
+        }
+    }
+    GS_DEV void J(const T (&x)[N], T (&Jm)[N][N]) const {
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+#pragma unroll
+            for (int j = 0; j < N; ++j) Jm[i][j] = A[i + j * N];
+            T sn, cs;
+            fsincos(x[i], sn, cs);
+            Jm[i][i] = A[i + i * N] + cs / T(10);
+        }
+    }
+};
 
 // Deflated(prob, roots; power = p, shift = σ) around family Inner: the residual M(x) F(x) with
 // M = ∏ᵢ (‖x − rᵢ‖^(−p) + σ) (_deflate_scale, deflation.jl:251-261), the Jacobian M J + F ∇Mᵀ

@@ -99,6 +99,11 @@ struct FamilySpec {
     int p_bytes32, align32;      // the same for a Float32 batch (0: no Float32 form)
 };
 
+// The sizes of the dense family: n ≤ 8 on the lane-per-system kernels, above on the dense kernel.
+static bool dense_size(int n) {
+    return n == 2 || n == 4 || n == 6 || n == 8 || n == 12 || n == 16 || n == 24 || n == 32 || n == 48 || n == 64;
+}
+
 static const FamilySpec* family_spec(int id) {
     static const FamilySpec specs[] = {
         {GS_FAM_COUPLED2, 2, 2, 16, 8, 8, 4},
@@ -131,9 +136,8 @@ static int validate_batch(const gs_method* method, const gs_family* family, int6
     const FamilySpec* fs = family_spec(family->id);
     if (!fs) return fail(GS_EFAMILY, "problem family " + std::to_string(family->id) + " is not registered");
     if (family->id == GS_FAM_DENSE_SIN) {
-        if (!(family->n == 16 || family->n == 24 || family->n == 32 || family->n == 48 || family->n == 64) ||
-            family->m != family->n)
-            return fail(GS_EINVAL, "the dense family takes n = m = 16, 24, 32, 48 or 64");
+        if (!dense_size(family->n) || family->m != family->n)
+            return fail(GS_EINVAL, "the dense family takes n = m = 2, 4, 6, 8, 12, 16, 24, 32, 48 or 64");
         if (!family->data || (reinterpret_cast<uintptr_t>(family->data) % 8) != 0)
             return fail(GS_EINVAL, "the dense family needs its matrix A (family data, 8-byte aligned)");
     } else if (family->n != fs->n || family->m != fs->n) {
@@ -154,8 +158,8 @@ static int validate_batch(const gs_method* method, const gs_family* family, int6
         if (method->reuse != 0) return fail(GS_EINVAL, "Newton with a DogLeg globalization takes no reuse");
     }
     if (method->reuse < 0) return fail(GS_EINVAL, "reuse must be ≥ 1, or 0 for nothing");
-    if (family->id == GS_FAM_DENSE_SIN && method->globalization == GS_DOGLEG)
-        return fail(GS_EFAMILY, "the dense solve takes Static or Backtracking");
+    if (family->id == GS_FAM_DENSE_SIN && family->n > 8 && method->globalization == GS_DOGLEG)
+        return fail(GS_EFAMILY, "the dense solve at n > 8 takes Static or Backtracking");
     if (method->factor_precision != GS_FACTOR_WORKING && method->factor_precision != GS_FACTOR_F32)
         return fail(GS_EINVAL, "factor_precision must be GS_FACTOR_WORKING or GS_FACTOR_F32");
     if (method->refinement != GS_NO_REFINEMENT && method->refinement != GS_REFINEMENT)
@@ -391,6 +395,13 @@ static int solve_batch_f64(const gs_method* method, const gs_family* family, int
         L.reuse = method->reuse > 0;
         L.redo = scr;  // null: the library's own flags
         return finish_launch(launch_fpu_gl3(L, st), st, timed);
+    }
+    if (family->id == GS_FAM_DENSE_SIN && family->n <= 8) {
+        // one lane per system, A read from device memory
+        a.fam.data = family->data;
+        const int g = glob == GS_BACKTRACKING ? kBacktracking : (glob == GS_DOGLEG ? kDogLeg : kStatic);
+        return finish_launch(launch_small_dense(family->n, g, method->reuse > 0, a, mixed, refine, diag, wg, st), st,
+                             timed);
     }
     if (family->id == GS_FAM_DENSE_SIN) {
         // a segment of a wave per system (the workgroup is the kernel's own: 256 work-items)
@@ -648,6 +659,15 @@ int gs_solve_batch_f32(const gs_method* method, const gs_family* family, int64_t
     auto lock = device_call_lock();
     if ((rc = check_order(order, nsys, st, nullptr)) != GS_OK) return rc;
     bool timed = timing_begin(st);
+    if (family->id == GS_FAM_DENSE_SIN && family->n <= 8) {
+        a.fam.data = family->data;
+        const int g = method->globalization == GS_BACKTRACKING ? kBacktracking
+                      : method->globalization == GS_DOGLEG    ? kDogLeg
+                                                              : kStatic;
+        rc = launch_small_dense_f32(family->n, g, method->reuse > 0, a, method->refinement == GS_REFINEMENT,
+                                    S != nullptr, wg, st);
+        return finish_launch(rc, st, timed);
+    }
     if (family->id == GS_FAM_DENSE_SIN) {
         // a segment of a wave per system; the factor is in the working precision (Float32)
         DenseLaunchT<float> L;

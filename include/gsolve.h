@@ -181,9 +181,10 @@ enum gs_family_id {
     GS_FAM_QUAD_MAP2 = 9,        /* u = A x, F_i = (u_i + c_i u_i²) − b_i : n = 2, P = A (4, column-major), b (2),  *
                                   * c (2) — f_mod_r1 / f_ill_r1 of test/nonlinear/refinement.jl:40-96           */
     GS_FAM_DENSE_SIN = 10        /* A x + sin.(x)/10 − b, the §10.E problem of benchmark/device_problems.jl:133-175 : *
-                                  * n = 16, 24, 32, 48 or 64; `data` = A (n × n, column-major, shared by the      *
-                                  * batch), P = b (n doubles per system); a segment of 16, 32 or 64 lanes of a  *
-                                  * wave per system (gs_dense_set_shape)                                        */
+                                  * n = 2, 4, 6, 8 (one lane per system) or 12, 16, 24, 32, 48, 64 (a segment    *
+                                  * of 16, 32 or 64 lanes of a wave per system, gs_dense_set_shape); `data` = A  *
+                                  * (n × n, column-major, shared by the batch, in the batch's precision), P = b  *
+                                  * (n entries per system)                                                      */
 };
 
 typedef struct gs_family {
@@ -209,7 +210,8 @@ typedef struct gs_family {
  *            above a kernel's own bound runs as workgroups of that bound (the systems are
  *            independent, so the grouping changes no result).
  * GS_FAM_FPU_GL3 takes Static, Backtracking (with or without reuse) and DogLeg; GS_FAM_DENSE_SIN
- * Static or Backtracking, with or without reuse (GS_EFAMILY otherwise).
+ * every globalization at n ≤ 8 and Static or Backtracking above, with or without reuse (GS_EFAMILY
+ * otherwise).
  */
 int gs_solve_batch_f64(const gs_method* method, const gs_family* family, int64_t nsys,
                        double* X, const void* P, int64_t p_stride, gs_status_f64* S,
@@ -361,7 +363,7 @@ int gs_solve_expsum_lsq_f64(const gs_method* method, int64_t nfit, double* X, co
 int gs_lsq_set_shape(int k, int threads, int blocks_per_cu);
 int gs_lsq_get_shape(int k, int64_t nfit, int64_t out[3]);
 
-/* The launch shape of the dense family (GS_FAM_DENSE_SIN) at size n (16, 24, 32, 48 or 64): `lanes`
+/* The launch shape of the dense family (GS_FAM_DENSE_SIN) at size n (12, 16, 24, 32, 48 or 64): `lanes`
  * per system, the wave segment that holds one system's rows (16, 32 or 64 and at least n; 16 lanes
  * put four systems in a wave, 64 one), or 0 for the default (the smallest that holds n). Results do
  * not depend on it. Set by the package's autotuner; no counterpart in the reference.

@@ -248,6 +248,11 @@ static int solve_small_batch(const Method<R>& m, const Options<R>& o, const gs_f
             };
             if (defl.first || defl.second) return GS_EFAMILY;
             switch (fam->n) {
+                case 2: return run_n(std::integral_constant<int, 2>{});
+                case 4: return run_n(std::integral_constant<int, 4>{});
+                case 6: return run_n(std::integral_constant<int, 6>{});
+                case 8: return run_n(std::integral_constant<int, 8>{});
+                case 12: return run_n(std::integral_constant<int, 12>{});
                 case 16: return run_n(std::integral_constant<int, 16>{});
                 case 24: return run_n(std::integral_constant<int, 24>{});
                 case 32: return run_n(std::integral_constant<int, 32>{});
@@ -762,6 +767,11 @@ int orc_dense_eval(int n, const double* A, const double* b, const double* x, dou
         return 0;
     };
     switch (n) {
+        case 2: return run(std::integral_constant<int, 2>{});
+        case 4: return run(std::integral_constant<int, 4>{});
+        case 6: return run(std::integral_constant<int, 6>{});
+        case 8: return run(std::integral_constant<int, 8>{});
+        case 12: return run(std::integral_constant<int, 12>{});
         case 16: return run(std::integral_constant<int, 16>{});
         case 24: return run(std::integral_constant<int, 24>{});
         case 32: return run(std::integral_constant<int, 32>{});

@@ -1,5 +1,6 @@
-"""Parity of the dense mid-size solve (a wave segment per system, GS_FAM_DENSE_SIN: the §10.E problem
-A x + sin.(x)/10 − b of benchmark/device_problems.jl:133-175, n = 16, 24, 32, 48, 64) with the CPU oracle,
+"""Parity of the dense family (GS_FAM_DENSE_SIN: the §10.E problem A x + sin.(x)/10 − b of
+benchmark/device_problems.jl:133-175; one lane per system at n = 2, 4, 6, 8, a wave segment per system
+at n = 12, 16, 24, 32, 48, 64) with the CPU oracle,
 through the C ABI. The device evaluates the family's fixed sine and cosine and folds every sum in the
 oracle's order, so results are compared bit for bit: iterates and every status byte.
 
@@ -58,7 +59,7 @@ def _gpu(gs, device, fam, method, X0, B, opts, counts=False, order=None):
     return X.cpu().numpy(), gs.statuses(S)
 
 
-@pytest.mark.parametrize("n", [16, 24, 32, 48, 64])
+@pytest.mark.parametrize("n", [2, 4, 6, 8, 12, 16, 24, 32, 48, 64])
 @pytest.mark.parametrize("name", list(METHODS))
 def test_dense_every_method_bitwise(gs, orc, workloads, device, n, name):
     nsys = 2048 if n < 32 else (1024 if n == 32 else 512)
@@ -118,7 +119,7 @@ def test_dense_full_size_converges_to_the_built_roots(gs, workloads, device):
         assert np.abs(workloads.tene_residual(x, A, B)).max() < 1e-12
 
 
-@pytest.mark.parametrize("n", [16, 24, 32])
+@pytest.mark.parametrize("n", [12, 16, 24, 32])
 def test_dense_lanes_per_system_agree_bitwise(gs, orc, workloads, device, n):
     """Every segment width a size takes (four, two or one systems per wave) writes the same bytes,
     those of the oracle; the width is the launch's choice alone."""
@@ -187,7 +188,7 @@ F32_METHODS = {
 }
 
 
-@pytest.mark.parametrize("n", [16, 24, 32, 48, 64])
+@pytest.mark.parametrize("n", [2, 4, 6, 8, 12, 16, 24, 32, 48, 64])
 @pytest.mark.parametrize("name", list(F32_METHODS))
 def test_dense_float32_bitwise(gs, orc, workloads, device, n, name):
     """A Float32 batch of the dense family (the element type of the reference's device grid,
@@ -223,3 +224,33 @@ def test_dense_float32_bitwise(gs, orc, workloads, device, n, name):
     assert codes[0] > 0.6 * nsys and codes[4] > 0
     if "reuse" in name:
         assert st["reused"].any()
+
+
+@pytest.mark.parametrize("n", [2, 4, 6, 8])
+@pytest.mark.parametrize("eltype", ["float64", "float32"])
+def test_dense_small_dogleg_bitwise(gs, orc, workloads, device, n, eltype):
+    """DogLeg on the lane-per-system form of the dense family (n ≤ 8), in both precisions, against the
+    oracle bit for bit."""
+    A, B, X0 = _batch(workloads, n, 2048, seed=n + 7)
+    X0 = np.where(np.abs(X0) > 1e30, 1e30, X0)
+    dt = np.float32 if eltype == "float32" else np.float64
+    fam = gs.DenseSin(A.astype(dt).astype(np.float64))
+    Bt, Xt = B.astype(dt), X0.astype(dt)
+    tol = float(np.sqrt(np.finfo(np.float32).eps)) if eltype == "float32" else 1e-12
+    for m in (gs.Newton(globalization=gs.DogLeg()), gs.Newton(globalization=gs.DogLeg(radius=1e-2))):
+        opts = gs.Options(eltype=eltype, f_abstol=tol, f_reltol=0.0, max_iterations=60)
+        X = torch.as_tensor(Xt, device=device).clone()
+        S = gs.status_buffer(len(Xt), device, eltype)
+        gs.solve_batch(X, S, gs.NonlinearProblem(fam), m, gs.PerSystem(torch.as_tensor(Bt, device=device)), opts)
+        x, st = X.cpu().numpy(), gs.statuses(S)
+        xo, so = orc.solve_batch(m, fam, Xt.copy(), gs.PerSystem(Bt), opts)
+        w = 4 if eltype == "float32" else 8
+        u = np.uint32 if eltype == "float32" else np.uint64
+        size = st.dtype.itemsize
+        meaningful = 42 if eltype == "float32" else 66
+        bad_x = np.nonzero((x.view(u) != xo.view(u)).any(axis=1))[0]
+        sb = st.view(np.uint8).reshape(len(st), size)[:, :meaningful]
+        ob = so.view(np.uint8).reshape(len(so), size)[:, :meaningful]
+        bad_s = np.nonzero((sb != ob).any(axis=1))[0]
+        assert bad_s.size == 0 and bad_x.size == 0, f"{bad_s.size} statuses, {bad_x.size} iterates differ (w={w})"
+        print(f"dense n={n} {eltype} dogleg", np.bincount(st["code"], minlength=6).tolist(), flush=True)

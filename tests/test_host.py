@@ -214,7 +214,7 @@ def test_dense_shape_setting_without_gpu(gs):
     from importlib import import_module
 
     abi = import_module(gs.__name__ + "._abi")
-    allowed = {16: (16, 32), 24: (32, 64), 32: (32, 64), 48: (64,), 64: (64,)}
+    allowed = {12: (16, 32), 16: (16, 32), 24: (32, 64), 32: (32, 64), 48: (64,), 64: (64,)}
     assert allowed == gs.autotune.DENSE_LANES
     try:
         for n, widths in allowed.items():
@@ -227,11 +227,14 @@ def test_dense_shape_setting_without_gpu(gs):
                     assert rc == abi.GS_EINVAL and f"n = {n}" in abi.last_error(lib)
             assert lib.gs_dense_set_shape(n, 0) == 0 and lib.gs_dense_get_shape(n) == widths[0]
         assert lib.gs_dense_set_shape(40, 64) == abi.GS_EINVAL and lib.gs_dense_get_shape(40) == abi.GS_EINVAL
+        # n ≤ 8 runs one lane per system: no segment width
+        assert lib.gs_dense_set_shape(8, 16) == abi.GS_EINVAL and lib.gs_dense_get_shape(8) == abi.GS_EINVAL
     finally:
         for n in allowed:
             lib.gs_dense_set_shape(n, 0)
-    with pytest.raises(gs.ArgumentError, match="order 16, 24, 32, 48 or 64"):
+    with pytest.raises(gs.ArgumentError, match="order 2, 4, 6, 8, 12, 16, 24, 32, 48 or 64"):
         gs.DenseSin(np.eye(40))
+    assert gs.DenseSin(np.eye(6)).n == 6
     assert gs.DenseSin(np.eye(64) * 2.0).n == 64
 
 

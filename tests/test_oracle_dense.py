@@ -30,7 +30,7 @@ def test_fsincos_within_one_ulp(orc):
         assert (np.abs(got.astype(np.float32) - w32) <= np.spacing(np.abs(w32))).all()
 
 
-@pytest.mark.parametrize("n", [16, 24, 32, 48, 64])
+@pytest.mark.parametrize("n", [2, 4, 6, 8, 12, 16, 24, 32, 48, 64])
 def test_dense_jacobian_is_the_derivative(orc, workloads, n):
     A, B, _ = workloads.tene_batch(n, 4, seed=3)
     rng = np.random.default_rng(n)
