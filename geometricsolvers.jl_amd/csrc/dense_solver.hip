@@ -22,8 +22,6 @@ int launch_dense_n32(int np, int glob, bool mixed, bool refine, bool diag, const
 int launch_dense_n48(int np, int glob, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st);
 int launch_dense_n64(int np, int glob, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st);
 // The Float32 launches (dense_f16.hip … dense_f64.hip).
-int launch_dense_f32_n12(int np, int glob, bool refine, bool diag, bool reuse, const DenseLaunchT<float>& L,
-                         hipStream_t st);
 int launch_dense_f32_n16(int np, int glob, bool refine, bool diag, bool reuse, const DenseLaunchT<float>& L,
                          hipStream_t st);
 int launch_dense_f32_n24(int np, int glob, bool refine, bool diag, bool reuse, const DenseLaunchT<float>& L,
@@ -110,7 +108,6 @@ int launch_dense_f32(int n, int glob, bool refine, bool diag, bool reuse, const 
                      hipStream_t st) {
     const int np = lanes_of(n);
     switch (n) {
-        case 12: return launch_dense_f32_n12(np, glob, refine, diag, reuse, L, st);
         case 16: return launch_dense_f32_n16(np, glob, refine, diag, reuse, L, st);
         case 24: return launch_dense_f32_n24(np, glob, refine, diag, reuse, L, st);
         case 32: return launch_dense_f32_n32(np, glob, refine, diag, reuse, L, st);

@@ -1,6 +1,6 @@
-// libgsolve_hip: the dense family (GS_FAM_DENSE_SIN) at n = 2, 4, 6 and 8 on the lane-per-system
-// kernels (FamDenseSinSmallT), in FP64 and in Float32, every globalization, with or without
-// reuse = k (not with DogLeg), in a translation unit of its own.
+// libgsolve_hip: the dense family (GS_FAM_DENSE_SIN) at n = 2, 4, 6 and 8 (and 12 in Float32) on the
+// lane-per-system kernels (FamDenseSinSmallT), in FP64 and in Float32, every globalization, with or
+// without reuse = k (not with DogLeg), in a translation unit of its own.
 #include "small_launch.hpp"
 
 namespace gs {
@@ -27,8 +27,13 @@ int launch_dense_small_t(int n, int glob, bool reuse, const KernelArgsT<T>& a, b
         case 4: return launch_dense_small_n<T, 4>(glob, reuse, a, mixed, refine, diag, wg, st);
         case 6: return launch_dense_small_n<T, 6>(glob, reuse, a, mixed, refine, diag, wg, st);
         case 8: return launch_dense_small_n<T, 8>(glob, reuse, a, mixed, refine, diag, wg, st);
+        case 12:
+            // Float32 only: there the rows fit a lane and run faster than on the segment kernel; an
+            // FP64 batch at n = 12 takes the segment kernel
+            if constexpr (sizeof(T) == 4) return launch_dense_small_n<T, 12>(glob, reuse, a, mixed, refine, diag, wg, st);
+            break;
     }
-    return set_error(GS_EINVAL, "the lane-per-system dense family takes n = 2, 4, 6 or 8");
+    return set_error(GS_EINVAL, "the lane-per-system dense family takes n = 2, 4, 6, 8 (and 12 in Float32)");
 }
 
 }  // namespace
