@@ -617,6 +617,17 @@ __global__ void fill_kernel(float* p, int64_t n, float v) {
     if (i < n) p[i] = v;
 }
 
+// The fused solve's two counters (fallbacks, hand-over time-outs) copied into pinned host memory by
+// the kernel queued right behind the solve, so the host reads them after the one synchronisation
+// without a copy of its own, and cleared for the next solve.
+__global__ void publish_counters_kernel(uint32_t* counters, uint32_t* host) {
+    if (threadIdx.x < 2) {
+        host[threadIdx.x] = counters[threadIdx.x];
+        counters[threadIdx.x] = 0;
+    }
+    __threadfence_system();
+}
+
 // Adam (Kingma & Ba) on the FP32 parameters, then the packed BF16 copy of the weights.
 __global__ void adam_kernel(float* p, const float* grad, float* m, float* v, int64_t t, float lr, float b1, float b2,
                             float eps, __bf16* packed) {
@@ -656,11 +667,40 @@ __global__ void pack_kernel(const float* p, __bf16* packed) {
 struct MlpWorkspace {
     void* ptr = nullptr;
     size_t bytes = 0;
+    bool clear = false;  // the fused solve's counters hold zeros (publish_counters_kernel cleared them)
 };
 std::mutex g_mlp_mutex;
 MlpWorkspace g_mlp_ws[64];
 
 MlpWorkspace g_mlp_counter[64];  // the fused solve's fallback counter (its own, so training can overlap)
+
+// The pinned, device-mapped host words publish_counters_kernel writes (per device).
+struct MlpPublish {
+    uint32_t* host = nullptr;
+    uint32_t* dev = nullptr;
+};
+MlpPublish g_mlp_pub[64];
+
+int mlp_publish(MlpPublish* out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return set_error(GS_EHIP, "hipGetDevice failed");
+    std::lock_guard<std::mutex> lock(g_mlp_mutex);
+    MlpPublish& w = g_mlp_pub[dev];
+    if (!w.host) {
+        void* h = nullptr;
+        if (hipHostMalloc(&h, 64, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
+            return set_error(GS_EHIP, "hipHostMalloc of the counter read-back failed");
+        void* d = nullptr;
+        if (hipHostGetDevicePointer(&d, h, 0) != hipSuccess) {
+            (void)hipHostFree(h);
+            return set_error(GS_EHIP, "hipHostGetDevicePointer of the counter read-back failed");
+        }
+        w.host = static_cast<uint32_t*>(h);
+        w.dev = static_cast<uint32_t*>(d);
+    }
+    *out = w;
+    return GS_OK;
+}
 
 int mlp_workspace(size_t bytes, void** out, bool counter = false) {
     int dev = 0;
@@ -673,6 +713,7 @@ int mlp_workspace(size_t bytes, void** out, bool counter = false) {
         w.bytes = 0;
         if (hipMalloc(&w.ptr, bytes) != hipSuccess) return set_error(GS_EHIP, "hipMalloc of the training workspace failed");
         w.bytes = bytes;
+        w.clear = false;
     }
     *out = w.ptr;
     return GS_OK;
@@ -685,7 +726,7 @@ int check(hipError_t e, const char* what) {
 
 }  // namespace
 
-// Free the training workspace and the fallback counter of every device (gs_shutdown; no call may be
+// Free the training workspace, the fallback counter and its pinned read-back of every device (gs_shutdown; no call may be
 // in flight).
 void release_mlp() {
     std::lock_guard<std::mutex> lock(g_mlp_mutex);
@@ -694,6 +735,10 @@ void release_mlp() {
             if (!w->ptr) continue;
             if (hipSetDevice(d) == hipSuccess) (void)hipFree(w->ptr);
             *w = MlpWorkspace{};
+        }
+        if (g_mlp_pub[d].host) {
+            if (hipSetDevice(d) == hipSuccess) (void)hipHostFree(g_mlp_pub[d].host);
+            g_mlp_pub[d] = MlpPublish{};
         }
     }
 }
@@ -778,7 +823,14 @@ int gs_solve_learned_henon_heiles_f64(const gs_mlp* net, const gs_method* method
     int rc = mlp_workspace(256, &ws, true);
     if (rc) return rc;
     uint32_t* counter = static_cast<uint32_t*>(ws);
-    hipLaunchKernelGGL(fill_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<float*>(counter), int64_t(2), 0.0f);
+    MlpPublish pub;
+    if ((rc = mlp_publish(&pub))) return rc;
+    // The counters start at zero: cleared by the last solve's read-back, or here after a new
+    // allocation or a call that ended before its read-back. The calls of one device do not overlap
+    // (device_call_lock), so the flag needs no lock of its own.
+    bool& clear = g_mlp_counter[dev].clear;
+    if (!clear) hipLaunchKernelGGL(fill_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<float*>(counter), int64_t(2), 0.0f);
+    clear = false;
     LearnedArgs a;
     std::memset(&a, 0, sizeof(a));
     a.st.Xk = Xk;
@@ -814,10 +866,13 @@ int gs_solve_learned_henon_heiles_f64(const gs_mlp* net, const gs_method* method
     }
     timed = timing_mark(st, timed);
     if ((rc = check(hipGetLastError(), "fused learned solve launch failed"))) return rc;
-    uint32_t host[2] = {0, 0};
-    if ((rc = check(hipMemcpyAsync(host, counter, 8, hipMemcpyDeviceToHost, st), "hipMemcpyAsync"))) return rc;
+    hipLaunchKernelGGL(publish_counters_kernel, dim3(1), dim3(64), 0, st, counter, pub.dev);
+    if ((rc = check(hipGetLastError(), "counter read-back launch failed"))) return rc;
     if ((rc = check(hipStreamSynchronize(st), "hipStreamSynchronize"))) return rc;
+    clear = true;
     timing_end(timed);
+    const uint32_t host[2] = {reinterpret_cast<volatile uint32_t*>(pub.host)[0],
+                              reinterpret_cast<volatile uint32_t*>(pub.host)[1]};
     if (host[1] != 0) return set_error(GS_EHIP, "the fused learned solve's hand-over between its waves timed out");
     if (fallbacks) *fallbacks = int64_t(host[0]);
     return GS_OK;
