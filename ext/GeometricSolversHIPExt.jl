@@ -555,62 +555,75 @@ device_chart(::PolarChart) = GsChart(Int32(2), Int32(0), ntuple(_ -> 0.0, 16), n
 
 const HIPReparam = Reparametrized{<:NonlinearProblem{<:HIPFamily}, <:Union{LinearChart, PolarChart}}
 
-served_reparam(rep, cm) = !rep.covariant && !(rep.prob.F isa Union{FpuGl3, DenseSin}) && cm !== nothing &&
+# A Float32 batch takes the linear chart only (its B and c are used as Float32, as ToReal{Float32}
+# adapts them); the polar chart in Float32 runs the reference's generic path.
+served_reparam(rep, cm, ::Type{T} = Float64) where {T} = !rep.covariant &&
+    !(rep.prob.F isa Union{FpuGl3, DenseSin}) && cm !== nothing &&
     cm.globalization in (0, 1, 3) && (cm.reuse == 0 || cm.globalization != 3) &&
-    (!(rep.g isa PolarChart) || family_n(rep.prob.F) == 2)
+    (!(rep.g isa PolarChart) || (T === Float64 && family_n(rep.prob.F) == 2))
 
-function hip_launch_reparam!(Y, S, codes, rep, cm::GsMethod, p, opt, order, ws)
+function hip_launch_reparam!(Y, S, codes, rep, cm::GsMethod, p, opt::Options{R}, order, ws) where {R}
     fam = rep.prob.F
-    P, stride, shared, root = family_records(fam, p, Float64)
+    P, stride, shared, root = family_records(fam, p, R)
     cf = GsFamily(family_id(fam), Int32(family_n(fam)), Int32(family_n(fam)), Int32(0),
         ntuple(k -> k <= length(shared) ? shared[k] : 0.0, 8))
     order_dev = order === nothing ? nothing : ROCArray(Int64.(collect(order)))
+    Sp = S === nothing ? C_NULL : pointer(S)
+    Cp = codes === nothing ? C_NULL : pointer(codes)
+    Op = order_dev === nothing ? C_NULL : pointer(order_dev)
     GC.@preserve root order_dev begin
-        rc = ccall((:gs_solve_batch_reparametrized_f64, libgsolve), Cint,
-            (Ref{GsMethod}, Ref{GsFamily}, Ref{GsChart}, Int64, Ptr{Cvoid}, Ptr{Cvoid}, Int64,
-                Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ref{GsOptionsF64}, Int32, Ptr{Cvoid}),
-            cm, cf, device_chart(rep.g), length(Y), pointer(Y), P, stride, S === nothing ? C_NULL : pointer(S),
-            codes === nothing ? C_NULL : pointer(codes), order_dev === nothing ? C_NULL : pointer(order_dev),
-            device_options(opt), Int32(something(ws, 0)), stream().stream)
+        rc = if R === Float32
+            ccall((:gs_solve_batch_reparametrized_f32, libgsolve), Cint,
+                (Ref{GsMethod}, Ref{GsFamily}, Ref{GsChart}, Int64, Ptr{Cvoid}, Ptr{Cvoid}, Int64,
+                    Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ref{GsOptionsF64}, Int32, Ptr{Cvoid}),
+                cm, cf, device_chart(rep.g), length(Y), pointer(Y), P, stride, Sp, Cp, Op,
+                device_options(opt), Int32(something(ws, 0)), stream().stream)
+        else
+            ccall((:gs_solve_batch_reparametrized_f64, libgsolve), Cint,
+                (Ref{GsMethod}, Ref{GsFamily}, Ref{GsChart}, Int64, Ptr{Cvoid}, Ptr{Cvoid}, Int64,
+                    Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ref{GsOptionsF64}, Int32, Ptr{Cvoid}),
+                cm, cf, device_chart(rep.g), length(Y), pointer(Y), P, stride, Sp, Cp, Op,
+                device_options(opt), Int32(something(ws, 0)), stream().stream)
+        end
     end
     rc == 0 || gs_error(rc)
     return nothing
 end
 
-# The status form: Y holds y₀ and receives x* = g(y*).
-function solve_batch!(Y::ROCVector{SVector{N, Float64}}, S::ROCVector{SolverStatus{Float64}},
+# The status form: Y holds y₀ and receives x* = g(y*); Float64 or Float32.
+function solve_batch!(Y::ROCVector{SVector{N, T}}, S::ROCVector{SolverStatus{T}},
         rep::HIPReparam, method::Newton = Newton(), p = nothing;
         order::Union{Nothing, AbstractVector{<:Integer}} = nothing,
         options::Union{Nothing, Options} = nothing,
-        workgroupsize::Union{Nothing, Integer} = nothing, kwargs...) where {N}
-    cm = device_method(Adapt_real(method))
-    served_reparam(rep, cm) || return invoke(solve_batch!,
-        Tuple{AbstractVector{<:Union{SVector, Number}}, AbstractVector{SolverStatus{Float64}}, Reparametrized, Any, Any},
+        workgroupsize::Union{Nothing, Integer} = nothing, kwargs...) where {N, T <: Union{Float32, Float64}}
+    cm = device_method(Adapt_real(method, T))
+    served_reparam(rep, cm, T) || return invoke(solve_batch!,
+        Tuple{AbstractVector{<:Union{SVector, Number}}, AbstractVector{SolverStatus{T}}, Reparametrized, Any, Any},
         Y, S, rep, method, p; order, options, workgroupsize, kwargs...)
     length(S) == length(Y) || throw(DimensionMismatch(
         "status vector length $(length(S)) does not match batch length $(length(Y))"))
     _validate_batch(Y, p, order)
     _validate_batch_problem(rep)
     ws = _validate_workgroupsize(workgroupsize)
-    opt = resolve_options(Float64, options, kwargs)
+    opt = resolve_options(T, options, kwargs)
     isempty(Y) && return S
     hip_launch_reparam!(Y, S, nothing, rep, cm, p, opt, order, ws)
     return S
 end
 
 # The counts form.
-function solve_batch!(Y::ROCVector{SVector{N, Float64}}, rep::HIPReparam, method::Newton = Newton(), p = nothing;
+function solve_batch!(Y::ROCVector{SVector{N, T}}, rep::HIPReparam, method::Newton = Newton(), p = nothing;
         order::Union{Nothing, AbstractVector{<:Integer}} = nothing,
         options::Union{Nothing, Options} = nothing,
-        workgroupsize::Union{Nothing, Integer} = nothing, kwargs...) where {N}
-    cm = device_method(Adapt_real(method))
-    served_reparam(rep, cm) || return invoke(solve_batch!,
+        workgroupsize::Union{Nothing, Integer} = nothing, kwargs...) where {N, T <: Union{Float32, Float64}}
+    cm = device_method(Adapt_real(method, T))
+    served_reparam(rep, cm, T) || return invoke(solve_batch!,
         Tuple{AbstractVector{<:Union{SVector, Number}}, Reparametrized, Any, Any},
         Y, rep, method, p; order, options, workgroupsize, kwargs...)
     _validate_batch(Y, p, order)
     _validate_batch_problem(rep)
     ws = _validate_workgroupsize(workgroupsize)
-    opt = resolve_options(Float64, options, kwargs)
+    opt = resolve_options(T, options, kwargs)
     isempty(Y) && return _tally_return_codes(ReturnCode[])
     codes = ROCArray{ReturnCode}(undef, length(Y))
     hip_launch_reparam!(Y, nothing, codes, rep, cm, p, opt, order, ws)
@@ -651,8 +664,9 @@ _stat_indices(::Colon) = ()
 _stat_indices(idx) = Tuple(Int32.(collect(idx)))
 
 # The index sets and the datum as gs_stationarity, or `nothing` where the device form does not hold
-# them (|E| other than 2 or 6, a callable datum): those run the reference's generic batch.
-function device_stationarity(prob::StationarityProblem, b_src, m::Int)
+# them (|E| other than 2 or 6, a callable datum): those run the reference's generic batch. A shared
+# datum travels as doubles (the library uses it as T); a PerSystem datum holds SVector{m, T}.
+function device_stationarity(prob::StationarityProblem, b_src, m::Int, ::Type{T} = Float64) where {T}
     m in (2, 6) || return nothing
     e, u = _stat_indices(prob.E), _stat_indices(prob.U)
     pad(t) = ntuple(k -> k <= length(t) ? t[k] : Int32(0), 8)
@@ -664,39 +678,50 @@ function device_stationarity(prob::StationarityProblem, b_src, m::Int)
         length(b_src) == m || throw(ArgumentError("datum b must have length $m, got $(length(b_src))"))
         v = Float64.(collect(b_src))
         kind, shared = Int32(1), ntuple(k -> k <= m ? v[k] : 0.0, 8)
-    elseif b_src isa PerSystem && b_src.data isa ROCVector{<:SVector{M, Float64} where {M}}
+    elseif b_src isa PerSystem && b_src.data isa ROCVector{<:SVector{M, T} where {M}}
         length(eltype(b_src.data)) == m || throw(ArgumentError(
             "datum b must have length $m, got $(length(eltype(b_src.data)))"))
-        kind, data, stride = Int32(2), Ptr{Cvoid}(pointer(b_src.data)), Int64(8m)
+        kind, data, stride = Int32(2), Ptr{Cvoid}(pointer(b_src.data)), Int64(sizeof(T) * m)
     else
         return nothing
     end
     return GsStationarity(Int32(length(e)), kind, pad(e), pad(u), shared, data, stride)
 end
 
-_action_records(p::Tuple{Real, Real}) = (ROCArray([Float64(p[1]), Float64(p[2])]), Int64(0))
-_action_records(p::SVector{2, <:Real}) = _action_records((p[1], p[2]))
-_action_records(p::PerSystem{<:ROCVector{SVector{2, Float64}}}) = (p.data, Int64(16))
+_action_records(p::Tuple{Real, Real}, ::Type{T}) where {T} = (ROCArray([T(p[1]), T(p[2])]), Int64(0))
+_action_records(p::SVector{2, <:Real}, ::Type{T}) where {T} = _action_records((p[1], p[2]), T)
+_action_records(p::PerSystem{<:ROCVector{SVector{2, T}}}, ::Type{T}) where {T} = (p.data, Int64(2 * sizeof(T)))
 
-# The action's parameters the device reads: (h, λ) shared, or one SVector{2, Float64} per system in
-# device memory (16-byte records). Anything else (a host vector, another record type) is no device
+# The action's parameters the device reads: (h, λ) shared, or one SVector{2, T} per system in device
+# memory, T the batch's precision. Anything else (a host vector, another record type) is no device
 # launch: it goes to the generic method, which refuses or runs it as the reference does.
-served_action_params(p::Tuple{Real, Real}) = true
-served_action_params(p::SVector{2, <:Real}) = true
-served_action_params(p::PerSystem{<:ROCVector{SVector{2, Float64}}}) = true
-served_action_params(p) = false
+served_action_params(p::Tuple{Real, Real}, ::Type) = true
+served_action_params(p::SVector{2, <:Real}, ::Type) = true
+served_action_params(p::PerSystem{<:ROCVector{SVector{2, T}}}, ::Type{T}) where {T} = true
+served_action_params(p, ::Type) = false
 
-function hip_launch_stationarity!(X, S, codes, cs::GsStationarity, cm::GsMethod, p, opt, order, ws)
-    P, stride = _action_records(p)
+function hip_launch_stationarity!(X, S, codes, cs::GsStationarity, cm::GsMethod, p, opt::Options{R}, order,
+        ws) where {R}
+    P, stride = _action_records(p, R)
     cf = GsFamily(Int32(8), Int32(6), Int32(6), Int32(2), ntuple(_ -> 0.0, 8))
     order_dev = order === nothing ? nothing : ROCArray(Int64.(collect(order)))
+    Sp = S === nothing ? C_NULL : pointer(S)
+    Cp = codes === nothing ? C_NULL : pointer(codes)
+    Op = order_dev === nothing ? C_NULL : pointer(order_dev)
     GC.@preserve P order_dev begin
-        rc = ccall((:gs_solve_batch_stationarity_f64, libgsolve), Cint,
-            (Ref{GsMethod}, Ref{GsFamily}, Ref{GsStationarity}, Int64, Ptr{Cvoid}, Ptr{Cvoid}, Int64,
-                Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ref{GsOptionsF64}, Int32, Ptr{Cvoid}),
-            cm, cf, cs, length(X), pointer(X), pointer(P), stride, S === nothing ? C_NULL : pointer(S),
-            codes === nothing ? C_NULL : pointer(codes), order_dev === nothing ? C_NULL : pointer(order_dev),
-            device_options(opt), Int32(something(ws, 0)), stream().stream)
+        rc = if R === Float32
+            ccall((:gs_solve_batch_stationarity_f32, libgsolve), Cint,
+                (Ref{GsMethod}, Ref{GsFamily}, Ref{GsStationarity}, Int64, Ptr{Cvoid}, Ptr{Cvoid}, Int64,
+                    Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ref{GsOptionsF64}, Int32, Ptr{Cvoid}),
+                cm, cf, cs, length(X), pointer(X), pointer(P), stride, Sp, Cp, Op,
+                device_options(opt), Int32(something(ws, 0)), stream().stream)
+        else
+            ccall((:gs_solve_batch_stationarity_f64, libgsolve), Cint,
+                (Ref{GsMethod}, Ref{GsFamily}, Ref{GsStationarity}, Int64, Ptr{Cvoid}, Ptr{Cvoid}, Int64,
+                    Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ref{GsOptionsF64}, Int32, Ptr{Cvoid}),
+                cm, cf, cs, length(X), pointer(X), pointer(P), stride, Sp, Cp, Op,
+                device_options(opt), Int32(something(ws, 0)), stream().stream)
+        end
     end
     rc == 0 || gs_error(rc)
     return nothing
@@ -704,46 +729,46 @@ end
 
 served_stationarity(cm) = cm !== nothing && cm.globalization in (0, 1, 3)
 
-# The status form: X holds each system's full state (SVector{6, Float64}); p = (h, λ) shared or
-# PerSystem of SVector{2, Float64}; the datum from the keyword `b` or the problem's own.
-function solve_batch!(X::ROCVector{SVector{6, Float64}}, S::ROCVector{SolverStatus{Float64}},
+# The status form: X holds each system's full state (SVector{6, T}, T Float64 or Float32); p = (h, λ)
+# shared or PerSystem of SVector{2, T}; the datum from the keyword `b` or the problem's own.
+function solve_batch!(X::ROCVector{SVector{6, T}}, S::ROCVector{SolverStatus{T}},
         prob::StationarityProblem{HenonHeilesAction}, method::Newton = Newton(), p = nothing;
         order::Union{Nothing, AbstractVector{<:Integer}} = nothing,
         options::Union{Nothing, Options} = nothing, b = nothing,
-        workgroupsize::Union{Nothing, Integer} = nothing, kwargs...)
+        workgroupsize::Union{Nothing, Integer} = nothing, kwargs...) where {T <: Union{Float32, Float64}}
     b_src = b === nothing ? prob.b : b
-    cm = device_method(Adapt_real(method))
-    cs = device_stationarity(prob, b_src, prob.E isa Colon ? 6 : length(prob.E))
-    (cs === nothing || !served_stationarity(cm) || !served_action_params(p)) && return invoke(solve_batch!,
-        Tuple{AbstractArray, AbstractArray{SolverStatus{Float64}}, StationarityProblem, Any, Any},
+    cm = device_method(Adapt_real(method, T))
+    cs = device_stationarity(prob, b_src, prob.E isa Colon ? 6 : length(prob.E), T)
+    (cs === nothing || !served_stationarity(cm) || !served_action_params(p, T)) && return invoke(solve_batch!,
+        Tuple{AbstractArray, AbstractArray{SolverStatus{T}}, StationarityProblem, Any, Any},
         X, S, prob, method, p; order, options, b, workgroupsize, kwargs...)
     _validate_batch(X, p, order)
     b_src isa PerSystem && _validate_batch(X, b_src, nothing)
     length(S) == length(X) || throw(DimensionMismatch(
         "status vector S has length $(length(S)), expected $(length(X)) to match the batch X"))
     ws = _validate_workgroupsize(workgroupsize)
-    opt = resolve_options(Float64, options, kwargs)
+    opt = resolve_options(T, options, kwargs)
     isempty(X) && return S
     GC.@preserve b_src hip_launch_stationarity!(X, S, nothing, cs, cm, p, opt, order, ws)
     return S
 end
 
 # The counts form.
-function solve_batch!(X::ROCVector{SVector{6, Float64}}, prob::StationarityProblem{HenonHeilesAction},
+function solve_batch!(X::ROCVector{SVector{6, T}}, prob::StationarityProblem{HenonHeilesAction},
         method::Newton = Newton(), p = nothing;
         order::Union{Nothing, AbstractVector{<:Integer}} = nothing,
         options::Union{Nothing, Options} = nothing, b = nothing,
-        workgroupsize::Union{Nothing, Integer} = nothing, kwargs...)
+        workgroupsize::Union{Nothing, Integer} = nothing, kwargs...) where {T <: Union{Float32, Float64}}
     b_src = b === nothing ? prob.b : b
-    cm = device_method(Adapt_real(method))
-    cs = device_stationarity(prob, b_src, prob.E isa Colon ? 6 : length(prob.E))
-    (cs === nothing || !served_stationarity(cm) || !served_action_params(p)) && return invoke(solve_batch!,
+    cm = device_method(Adapt_real(method, T))
+    cs = device_stationarity(prob, b_src, prob.E isa Colon ? 6 : length(prob.E), T)
+    (cs === nothing || !served_stationarity(cm) || !served_action_params(p, T)) && return invoke(solve_batch!,
         Tuple{AbstractArray, StationarityProblem, Any, Any},
         X, prob, method, p; order, options, b, workgroupsize, kwargs...)
     _validate_batch(X, p, order)
     b_src isa PerSystem && _validate_batch(X, b_src, nothing)
     ws = _validate_workgroupsize(workgroupsize)
-    opt = resolve_options(Float64, options, kwargs)
+    opt = resolve_options(T, options, kwargs)
     isempty(X) && return _tally_return_codes(ReturnCode[])
     codes = ROCArray{ReturnCode}(undef, length(X))
     GC.@preserve b_src hip_launch_stationarity!(X, nothing, codes, cs, cm, p, opt, order, ws)

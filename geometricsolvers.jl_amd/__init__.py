@@ -631,7 +631,7 @@ class StationarityProblem:
     def n_equations(self, M: int) -> int:
         return M if self.E is None else len(self.E)
 
-    def to_c(self, M: int, b_kw, nsys: int, device=None):
+    def to_c(self, M: int, b_kw, nsys: int, device=None, dtype=np.float64):
         """gs_stationarity for full states of M entries and the datum `b_kw` (or the problem's own):
         the checks of _validate_stationarity_solve / _validate_stationarity_batch
         (stationarity.jl:133-157, 600-618). Returns (struct, the datum array to keep alive)."""
@@ -654,14 +654,15 @@ class StationarityProblem:
             torch = _torch()
             if device is None:
                 arr = np.ascontiguousarray(np.asarray(data.cpu() if isinstance(data, torch.Tensor) else data,
-                                                      dtype=np.float64).reshape(len(data), -1))
+                                                      dtype=dtype).reshape(len(data), -1))
                 ptr, stride, rows, width = arr.ctypes.data, arr.strides[0], arr.shape[0], arr.shape[1]
                 keep = arr
             else:
-                t = data if isinstance(data, torch.Tensor) else torch.as_tensor(np.asarray(data, dtype=np.float64))
-                t = t.to(device=device, dtype=torch.float64)
+                tdt = torch.float32 if np.dtype(dtype) == np.float32 else torch.float64
+                t = data if isinstance(data, torch.Tensor) else torch.as_tensor(np.asarray(data, dtype=dtype))
+                t = t.to(device=device, dtype=tdt)
                 t = (t.reshape(len(t), -1) if t.dim() > 0 else t).contiguous()
-                ptr, stride, rows, width = t.data_ptr(), t.stride(0) * 8, t.shape[0], t.shape[1]
+                ptr, stride, rows, width = t.data_ptr(), t.stride(0) * t.element_size(), t.shape[0], t.shape[1]
                 keep = t
             if rows != nsys:
                 raise DimensionMismatch(f"PerSystem datum has {rows} entries, expected {nsys} to match the batch")
@@ -998,9 +999,9 @@ def solve_batch(X, *args, order=None, options: Optional[Options] = None, workgro
     elt = _eltype(X)
     cdefl = cchart = None
     if reparam is not None:
-        if elt != "float64":
-            raise ArgumentError("the device Reparametrized batch is Float64")
         cchart = reparam.to_c(fam.n)
+        if elt == "float32" and cchart.kind != _abi.GS_CHART_LINEAR:
+            raise ArgumentError("the Float32 Reparametrized batch takes the linear chart")
     if deflation is not None:
         if elt != "float64":
             raise ArgumentError(
@@ -1050,8 +1051,9 @@ def solve_batch(X, *args, order=None, options: Optional[Options] = None, workgro
         codes = torch.empty(nsys, dtype=torch.uint8, device=X.device)
     copt = opt.to_c()
     if cchart is not None:
-        entry = functools.partial(lib.gs_solve_batch_reparametrized_f64, ctypes.byref(cm), ctypes.byref(cf),
-                                  ctypes.byref(cchart))
+        entry = functools.partial(
+            lib.gs_solve_batch_reparametrized_f32 if elt == "float32" else lib.gs_solve_batch_reparametrized_f64,
+            ctypes.byref(cm), ctypes.byref(cf), ctypes.byref(cchart))
     elif cdefl is not None:
         entry = functools.partial(lib.gs_solve_batch_deflated_f64, ctypes.byref(cm), ctypes.byref(cf),
                                   ctypes.byref(cdefl))
@@ -1083,20 +1085,21 @@ def _solve_stationarity(X, S, prob, method, p, order, options, workgroupsize, kw
         raise ArgumentError("the device StationarityProblem batch takes Static(), Backtracking() or DogLeg(); "
                             "LevenbergMarquardt has no stationarity form")
     act = prob.S
-    _check_x(X, act.n)
+    _check_x(X, act.n, allow_f32=True)
+    elt = _eltype(X)
     order_h = _validate_batch(X, p, order)
     if isinstance(b_kw if b_kw is not None else prob.b, PerSystem):
         _validate_batch(X, b_kw if b_kw is not None else prob.b, None)
     nsys = X.shape[0]
     if S is not None:
-        _check_status_buffer(S, nsys, X)
+        _check_status_buffer(S, nsys, X, STATUS_DTYPE_F32 if elt == "float32" else STATUS_DTYPE)
     ws = _validate_workgroupsize(workgroupsize)
-    opt = resolve_options(options, kwargs, "float64")
+    opt = resolve_options(options, kwargs, elt)
     cm = method.to_c()
     if X.device.type != "cuda":
         raise ArgumentError(
             "X must be a device array (a HIP tensor): libgsolve_hip solves on the GPU and has no host path")
-    cs, keep = prob.to_c(act.n, b_kw, nsys, X.device)
+    cs, keep = prob.to_c(act.n, b_kw, nsys, X.device, np.float32 if elt == "float32" else np.float64)
     if nsys == 0:
         return S if S is not None else {k: 0 for k in _CODE_NAMES}
     lib = load_library()
@@ -1107,7 +1110,8 @@ def _solve_stationarity(X, S, prob, method, p, order, options, workgroupsize, kw
     order_dev = None if order_h is None else torch.as_tensor(order_h, dtype=torch.int64).to(X.device)
     codes = torch.empty(nsys, dtype=torch.uint8, device=X.device) if S is None else None
     copt = opt.to_c()
-    rc = lib.gs_solve_batch_stationarity_f64(
+    entry = lib.gs_solve_batch_stationarity_f32 if elt == "float32" else lib.gs_solve_batch_stationarity_f64
+    rc = entry(
         ctypes.byref(cm), ctypes.byref(cf), ctypes.byref(cs), nsys, ctypes.c_void_p(X.data_ptr()),
         ctypes.c_void_p(P.data_ptr()), p_stride,
         ctypes.c_void_p(S.data_ptr()) if S is not None else None,
@@ -1123,9 +1127,10 @@ def _solve_stationarity(X, S, prob, method, p, order, options, workgroupsize, kw
     return {k: int(counts[i]) for i, k in enumerate(_CODE_NAMES)}
 
 
-def _check_status_buffer(S, nsys: int, X) -> None:
+def _check_status_buffer(S, nsys: int, X, dtype=None) -> None:
     torch = _torch()
-    if S.dtype != torch.uint8 or S.dim() != 2 or S.shape[1] != STATUS_DTYPE.itemsize:
+    size = (STATUS_DTYPE if dtype is None else dtype).itemsize
+    if S.dtype != torch.uint8 or S.dim() != 2 or S.shape[1] != size:
         raise ArgumentError("S must be a status buffer (status_buffer(nsys, device))")
     if S.shape[0] != nsys:
         raise DimensionMismatch(f"status vector length {S.shape[0]} does not match batch length {nsys}")

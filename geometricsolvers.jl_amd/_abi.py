@@ -221,6 +221,8 @@ EXPORTED_SYMBOLS = (
     "gs_solve_batch_deflated_f64",
     "gs_solve_batch_stationarity_f64",
     "gs_solve_batch_reparametrized_f64",
+    "gs_solve_batch_stationarity_f32",
+    "gs_solve_batch_reparametrized_f32",
     "gs_lsq_set_shape",
     "gs_lsq_get_shape",
     "gs_lsq_set_graph",
@@ -324,6 +326,10 @@ def load_library(path: os.PathLike | None = None):
         ctypes.POINTER(GsMethod), ctypes.POINTER(GsFamily), ctypes.POINTER(GsChart), ctypes.c_int64, vp, vp,
         ctypes.c_int64, vp, vp, vp, ctypes.POINTER(GsOptionsF64), ctypes.c_int32, vp,
     ]
+    lib.gs_solve_batch_stationarity_f32.restype = ctypes.c_int
+    lib.gs_solve_batch_stationarity_f32.argtypes = lib.gs_solve_batch_stationarity_f64.argtypes
+    lib.gs_solve_batch_reparametrized_f32.restype = ctypes.c_int
+    lib.gs_solve_batch_reparametrized_f32.argtypes = lib.gs_solve_batch_reparametrized_f64.argtypes
     if lib.gs_version() != ABI_VERSION:
         raise RuntimeError(f"libgsolve_hip ABI version {lib.gs_version()} != {ABI_VERSION}")
     sizes = (ctypes.c_int64 * 4)()

@@ -31,7 +31,7 @@ struct StatArgs {
     int32_t b_kind;        // 0 none, 1 shared, 2 per system
     int32_t pad_;
     double b_shared[8];
-    const double* b_data;  // per-system datum records (device)
+    const void* b_data;    // per-system datum records (device), |E| entries of the working precision
     int64_t b_stride;      // bytes between datum records
 };
 
@@ -600,10 +600,9 @@ template <class Act, int NE> struct FamStationarityT {
             }
         has_b = a.stat.b_kind != 0;
         if (a.stat.b_kind == 2) {
-            const double* bp =
-                reinterpret_cast<const double*>(reinterpret_cast<const char*>(a.stat.b_data) + idx * a.stat.b_stride);
+            const T* bp = reinterpret_cast<const T*>(reinterpret_cast<const char*>(a.stat.b_data) + idx * a.stat.b_stride);
 #pragma unroll
-            for (int k = 0; k < NE; ++k) b[k] = T(bp[k]);
+            for (int k = 0; k < NE; ++k) b[k] = bp[k];
         } else {
             // the kernel argument read at constant offsets (a pointer into it would copy it to scratch)
 #pragma unroll
@@ -668,27 +667,29 @@ template <class Act, int NE> struct FamStationarityT {
 };
 
 // The registered charts x = g(y) of Reparametrized, with their Jacobians G = ∂g/∂y.
-// Linear: x = B y + c (each row a left fold), G = B.
-template <int N> struct ChartLinearT {
-    double B[N][N], c[N];
+// Linear: x = B y + c (each row a left fold), G = B; in the working precision R (a Float32 batch
+// takes B and c rounded to Float32, as ToReal{Float32} adapts a chart's captured constants).
+template <int N, class R = double> struct ChartLinearT {
+    using T = R;
+    T B[N][N], c[N];
     GS_DEV void load(const FamArgs& a) {
 #pragma unroll
         for (int i = 0; i < N; ++i) {
-            c[i] = a.chart.c[i];
+            c[i] = T(a.chart.c[i]);
 #pragma unroll
-            for (int j = 0; j < N; ++j) B[i][j] = a.chart.B[i * N + j];
+            for (int j = 0; j < N; ++j) B[i][j] = T(a.chart.B[i * N + j]);
         }
     }
-    GS_DEV void g(const double (&y)[N], double (&x)[N]) const {
+    GS_DEV void g(const T (&y)[N], T (&x)[N]) const {
 #pragma unroll
         for (int i = 0; i < N; ++i) {
-            double acc = B[i][0] * y[0];
+            T acc = B[i][0] * y[0];
 #pragma unroll
             for (int j = 1; j < N; ++j) acc = acc + B[i][j] * y[j];
             x[i] = acc + c[i];
         }
     }
-    GS_DEV void G(const double (&)[N], double (&Gm)[N][N]) const {
+    GS_DEV void G(const T (&)[N], T (&Gm)[N][N]) const {
 #pragma unroll
         for (int i = 0; i < N; ++i)
 #pragma unroll
@@ -699,6 +700,7 @@ template <int N> struct ChartLinearT {
 // Polar (the chart of test/nonlinear/reparametrized.jl:31-38): x = (r cos θ, r sin θ),
 // G = [cos θ, −r sin θ; sin θ, r cos θ].
 struct ChartPolarT {
+    using T = double;
     GS_DEV void load(const FamArgs&) {}
     GS_DEV void g(const double (&y)[2], double (&x)[2]) const {
         x[0] = y[0] * cos(y[1]);
@@ -720,7 +722,7 @@ template <class Inner, class Chart> struct FamReparamT {
     using T = typename Inner::T;
     static constexpr int N = Inner::N;
     static constexpr bool kMapsOutput = true;
-    static_assert(sizeof(T) == 8, "the device charts are Float64");
+    static_assert(std::is_same<typename Chart::T, T>::value, "the chart computes in the family's precision");
     Inner in;
     Chart ch;
     GS_DEV void load(const FamArgs& a, int64_t idx) {

@@ -113,7 +113,7 @@ static const FamilySpec* family_spec(int id) {
         {GS_FAM_QUAD_MAP2, 2, 8, 64, 8, 32, 4},
         {GS_FAM_HENON_HEILES_MIDPOINT, 4, 4, 32, 16, 16, 16},
         {GS_FAM_FPU_GL3, 48, 32, 256, 8, 0, 0},
-        {GS_ACT_HENON_HEILES, 6, 2, 16, 8, 0, 0},
+        {GS_ACT_HENON_HEILES, 6, 2, 16, 8, 8, 4},
         {GS_FAM_DENSE_SIN, 0, 0, 0, 8, 0, 4},  // n = 16, 24, 32, 48 or 64, records of n entries
     };
     for (const auto& s : specs)
@@ -433,6 +433,143 @@ static int solve_batch_f64(const gs_method* method, const gs_family* family, int
 
 }  // namespace gs
 
+// gs_solve_batch_stationarity_f64 and _f32: T is the working precision, Status its status record.
+template <class T, class Status>
+static int solve_stationarity(const gs_method* method, const gs_family* action, const gs_stationarity* stat,
+                              int64_t nsys, T* X, const void* P, int64_t p_stride, Status* S, uint8_t* codes,
+                              const int64_t* order, const gs_options_f64* options, int32_t workgroupsize,
+                              void* stream) {
+    g_last_error.clear();
+    if (!stat) return set_error(GS_EINVAL, "stat must not be NULL");
+    if (action && action->id != GS_ACT_HENON_HEILES)
+        return set_error(GS_EFAMILY, "a StationarityProblem batch takes a registered action (GS_ACT_HENON_HEILES)");
+    const FamilySpec* fs = nullptr;
+    int rc = validate_batch<T>(method, action, nsys, X, P, p_stride, S, codes, options, workgroupsize, &fs);
+    if (rc != GS_OK) return rc;
+    // the constructor's index-set checks (stationarity.jl:87-103) and the solve's
+    // (_validate_stationarity_solve, :133-157), then the device's own limits
+    const gs_stationarity& t = *stat;
+    const int M = fs->n;
+    if (t.n_eq < 0 || t.n_eq > 8) return set_error(GS_EINVAL, "n_eq must be 0 (E = U = :) or in 1:8");
+    const int ne = t.n_eq == 0 ? M : t.n_eq;
+    for (int k = 0; k < t.n_eq; ++k) {
+        if (t.e[k] < 1) return set_error(GS_EINVAL, "indices in E must be >= 1");
+        if (t.u[k] < 1) return set_error(GS_EINVAL, "indices in U must be >= 1");
+    }
+    for (int k = 0; k < t.n_eq; ++k)
+        for (int j = k + 1; j < t.n_eq; ++j) {
+            if (t.e[k] == t.e[j]) return set_error(GS_EINVAL, "index set E has duplicate indices");
+            if (t.u[k] == t.u[j]) return set_error(GS_EINVAL, "index set U has duplicate indices");
+        }
+    for (int k = 0; k < t.n_eq; ++k)
+        if (t.e[k] > M) return set_error(GS_EINVAL, "index in E exceeds " + std::to_string(M));
+    for (int k = 0; k < t.n_eq; ++k)
+        if (t.u[k] > M) return set_error(GS_EINVAL, "index in U exceeds " + std::to_string(M));
+    if (t.b_kind < GS_DATUM_NONE || t.b_kind > GS_DATUM_PER_SYSTEM) return set_error(GS_EINVAL, "b_kind must be a gs_datum_kind");
+    if (ne != 2 && ne != 6)
+        return set_error(GS_EFAMILY, "the device StationarityProblem batch takes |E| = 2 or 6 equations, got " +
+                                         std::to_string(ne));
+    if (nsys == 0) return GS_OK;
+    if (t.b_kind == GS_DATUM_PER_SYSTEM) {
+        if (!t.b_data) return set_error(GS_EINVAL, "a per-system datum needs b_data");
+        constexpr int w = int(sizeof(T));
+        if ((reinterpret_cast<uintptr_t>(t.b_data) % w) != 0 || (t.b_stride % w) != 0 || t.b_stride < w * ne)
+            return set_error(GS_EINVAL, "b_data and b_stride must be " + std::to_string(w) +
+                                            "-byte aligned, b_stride at least " + std::to_string(w) + "·|E| bytes");
+    }
+    KernelArgsT<T> a;
+    std::memset(&a, 0, sizeof(a));
+    a.fam.P = static_cast<const char*>(P);
+    a.fam.p_stride = p_stride;
+    for (int k = 0; k < 8; ++k) a.fam.shared[k] = action->shared[k];
+    for (int k = 0; k < ne; ++k) {
+        a.fam.stat.e[k] = t.n_eq == 0 ? k : t.e[k] - 1;
+        a.fam.stat.u[k] = t.n_eq == 0 ? k : t.u[k] - 1;
+        a.fam.stat.b_shared[k] = t.b_shared[k];
+    }
+    a.fam.stat.b_kind = t.b_kind;
+    a.fam.stat.b_data = t.b_data;
+    a.fam.stat.b_stride = t.b_stride;
+    a.nsys = nsys;
+    a.X = X;
+    a.S = S;
+    a.codes = codes;
+    a.order = order;
+    a.q = dev_params_t<T>(*method, *options);
+    const bool diag = S != nullptr;
+    const int wg = small_workgroup(workgroupsize);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    auto lock = device_call_lock();
+    if ((rc = check_order(order, nsys, st, nullptr)) != GS_OK) return rc;
+    const bool mixed = method->factor_precision == GS_FACTOR_F32;
+    const bool refine = method->refinement == GS_REFINEMENT;
+    bool timed = timing_begin(st);
+    const int glob = method->globalization == GS_BACKTRACKING ? kBacktracking
+                     : method->globalization == GS_DOGLEG    ? kDogLeg
+                                                             : kStatic;
+    if constexpr (sizeof(T) == 8)
+        rc = launch_small_stationarity(action->id, ne, glob, method->reuse > 0, a, mixed, refine, diag, wg, st);
+    else
+        rc = launch_small_stationarity_f32(action->id, ne, glob, method->reuse > 0, a, refine, diag, wg, st);
+    return finish_launch(rc, st, timed);
+}
+
+// gs_solve_batch_reparametrized_f64 and _f32: T is the working precision, Status its status record.
+template <class T, class Status>
+static int solve_reparametrized(const gs_method* method, const gs_family* family, const gs_chart* chart, int64_t nsys,
+                                T* X, const void* P, int64_t p_stride, Status* S, uint8_t* codes, const int64_t* order,
+                                const gs_options_f64* options, int32_t workgroupsize, void* stream) {
+    g_last_error.clear();
+    const FamilySpec* fs = nullptr;
+    int rc = validate_batch<T>(method, family, nsys, X, P, p_stride, S, codes, options, workgroupsize, &fs);
+    if (rc != GS_OK) return rc;
+    if (!chart) return set_error(GS_EINVAL, "chart must not be NULL");
+    if (family->id == GS_FAM_FPU_GL3 || family->id == GS_ACT_HENON_HEILES)
+        return set_error(GS_EFAMILY, "a Reparametrized batch wraps one of the lane-per-system families (1-5)");
+    if (chart->kind == GS_CHART_POLAR) {
+        if (sizeof(T) == 4) return set_error(GS_EFAMILY, "the Float32 Reparametrized batch takes the linear chart");
+        if (fs->n != 2) return set_error(GS_EFAMILY, "the polar chart maps 2 coordinates: families 1 and 2");
+    } else if (chart->kind == GS_CHART_LINEAR) {
+        for (int k = 0; k < fs->n * fs->n; ++k)
+            if (!std::isfinite(chart->B[k])) return set_error(GS_EINVAL, "the linear chart needs a finite B");
+        for (int k = 0; k < fs->n; ++k)
+            if (!std::isfinite(chart->c[k])) return set_error(GS_EINVAL, "the linear chart needs a finite c");
+    } else {
+        return set_error(GS_EINVAL, "chart kind must be GS_CHART_LINEAR or GS_CHART_POLAR");
+    }
+    if (nsys == 0) return GS_OK;
+    KernelArgsT<T> a;
+    std::memset(&a, 0, sizeof(a));
+    a.fam.P = static_cast<const char*>(P);
+    a.fam.p_stride = p_stride;
+    for (int k = 0; k < 8; ++k) a.fam.shared[k] = family->shared[k];
+    a.fam.chart.kind = chart->kind;
+    for (int k = 0; k < 16; ++k) a.fam.chart.B[k] = chart->B[k];
+    for (int k = 0; k < 4; ++k) a.fam.chart.c[k] = chart->c[k];
+    a.nsys = nsys;
+    a.X = X;
+    a.S = S;
+    a.codes = codes;
+    a.order = order;
+    a.q = dev_params_t<T>(*method, *options);
+    const bool diag = S != nullptr;
+    const int wg = small_workgroup(workgroupsize);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    auto lock = device_call_lock();
+    if ((rc = check_order(order, nsys, st, nullptr)) != GS_OK) return rc;
+    const bool mixed = method->factor_precision == GS_FACTOR_F32;
+    const bool refine = method->refinement == GS_REFINEMENT;
+    bool timed = timing_begin(st);
+    const int glob = method->globalization == GS_BACKTRACKING ? kBacktracking
+                     : method->globalization == GS_DOGLEG    ? kDogLeg
+                                                             : kStatic;
+    if constexpr (sizeof(T) == 8)
+        rc = launch_small_reparam(family->id, chart->kind, glob, method->reuse > 0, a, mixed, refine, diag, wg, st);
+    else
+        rc = launch_small_reparam_f32(family->id, chart->kind, glob, method->reuse > 0, a, refine, diag, wg, st);
+    return finish_launch(rc, st, timed);
+}
+
 extern "C" {
 
 int gs_solve_batch_f64(const gs_method* method, const gs_family* family, int64_t nsys, double* X, const void* P,
@@ -515,125 +652,32 @@ int gs_solve_batch_stationarity_f64(const gs_method* method, const gs_family* ac
                                     int64_t nsys, double* X, const void* P, int64_t p_stride, gs_status_f64* S,
                                     uint8_t* codes, const int64_t* order, const gs_options_f64* options,
                                     int32_t workgroupsize, void* stream) {
-    g_last_error.clear();
-    if (!stat) return set_error(GS_EINVAL, "stat must not be NULL");
-    if (action && action->id != GS_ACT_HENON_HEILES)
-        return set_error(GS_EFAMILY, "a StationarityProblem batch takes a registered action (GS_ACT_HENON_HEILES)");
-    const FamilySpec* fs = nullptr;
-    int rc = validate_batch<double>(method, action, nsys, X, P, p_stride, S, codes, options, workgroupsize, &fs);
-    if (rc != GS_OK) return rc;
-    // the constructor's index-set checks (stationarity.jl:87-103) and the solve's
-    // (_validate_stationarity_solve, :133-157), then the device's own limits
-    const gs_stationarity& t = *stat;
-    const int M = fs->n;
-    if (t.n_eq < 0 || t.n_eq > 8) return set_error(GS_EINVAL, "n_eq must be 0 (E = U = :) or in 1:8");
-    const int ne = t.n_eq == 0 ? M : t.n_eq;
-    for (int k = 0; k < t.n_eq; ++k) {
-        if (t.e[k] < 1) return set_error(GS_EINVAL, "indices in E must be >= 1");
-        if (t.u[k] < 1) return set_error(GS_EINVAL, "indices in U must be >= 1");
-    }
-    for (int k = 0; k < t.n_eq; ++k)
-        for (int j = k + 1; j < t.n_eq; ++j) {
-            if (t.e[k] == t.e[j]) return set_error(GS_EINVAL, "index set E has duplicate indices");
-            if (t.u[k] == t.u[j]) return set_error(GS_EINVAL, "index set U has duplicate indices");
-        }
-    for (int k = 0; k < t.n_eq; ++k)
-        if (t.e[k] > M) return set_error(GS_EINVAL, "index in E exceeds " + std::to_string(M));
-    for (int k = 0; k < t.n_eq; ++k)
-        if (t.u[k] > M) return set_error(GS_EINVAL, "index in U exceeds " + std::to_string(M));
-    if (t.b_kind < GS_DATUM_NONE || t.b_kind > GS_DATUM_PER_SYSTEM) return set_error(GS_EINVAL, "b_kind must be a gs_datum_kind");
-    if (ne != 2 && ne != 6)
-        return set_error(GS_EFAMILY, "the device StationarityProblem batch takes |E| = 2 or 6 equations, got " +
-                                         std::to_string(ne));
-    if (nsys == 0) return GS_OK;
-    if (t.b_kind == GS_DATUM_PER_SYSTEM) {
-        if (!t.b_data) return set_error(GS_EINVAL, "a per-system datum needs b_data");
-        if ((reinterpret_cast<uintptr_t>(t.b_data) % 8) != 0 || (t.b_stride % 8) != 0 || t.b_stride < 8 * ne)
-            return set_error(GS_EINVAL, "b_data and b_stride must be 8-byte aligned, b_stride at least 8·|E| bytes");
-    }
-    KernelArgs a;
-    std::memset(&a, 0, sizeof(a));
-    a.fam.P = static_cast<const char*>(P);
-    a.fam.p_stride = p_stride;
-    for (int k = 0; k < 8; ++k) a.fam.shared[k] = action->shared[k];
-    for (int k = 0; k < ne; ++k) {
-        a.fam.stat.e[k] = t.n_eq == 0 ? k : t.e[k] - 1;
-        a.fam.stat.u[k] = t.n_eq == 0 ? k : t.u[k] - 1;
-        a.fam.stat.b_shared[k] = t.b_shared[k];
-    }
-    a.fam.stat.b_kind = t.b_kind;
-    a.fam.stat.b_data = t.b_data;
-    a.fam.stat.b_stride = t.b_stride;
-    a.nsys = nsys;
-    a.X = X;
-    a.S = S;
-    a.codes = codes;
-    a.order = order;
-    a.q = dev_params(*method, *options);
-    const bool diag = S != nullptr;
-    const int wg = small_workgroup(workgroupsize);
-    hipStream_t st = static_cast<hipStream_t>(stream);
-    auto lock = device_call_lock();
-    if ((rc = check_order(order, nsys, st, nullptr)) != GS_OK) return rc;
-    const bool mixed = method->factor_precision == GS_FACTOR_F32;
-    const bool refine = method->refinement == GS_REFINEMENT;
-    bool timed = timing_begin(st);
-    const int glob = method->globalization == GS_BACKTRACKING ? kBacktracking
-                     : method->globalization == GS_DOGLEG    ? kDogLeg
-                                                             : kStatic;
-    rc = launch_small_stationarity(action->id, ne, glob, method->reuse > 0, a, mixed, refine, diag, wg, st);
-    return finish_launch(rc, st, timed);
+    return solve_stationarity(method, action, stat, nsys, X, P, p_stride, S, codes, order, options, workgroupsize,
+                              stream);
+}
+
+int gs_solve_batch_stationarity_f32(const gs_method* method, const gs_family* action, const gs_stationarity* stat,
+                                    int64_t nsys, float* X, const void* P, int64_t p_stride, gs_status_f32* S,
+                                    uint8_t* codes, const int64_t* order, const gs_options_f64* options,
+                                    int32_t workgroupsize, void* stream) {
+    return solve_stationarity(method, action, stat, nsys, X, P, p_stride, S, codes, order, options, workgroupsize,
+                              stream);
 }
 
 int gs_solve_batch_reparametrized_f64(const gs_method* method, const gs_family* family, const gs_chart* chart,
                                       int64_t nsys, double* X, const void* P, int64_t p_stride, gs_status_f64* S,
                                       uint8_t* codes, const int64_t* order, const gs_options_f64* options,
                                       int32_t workgroupsize, void* stream) {
-    g_last_error.clear();
-    const FamilySpec* fs = nullptr;
-    int rc = validate_batch<double>(method, family, nsys, X, P, p_stride, S, codes, options, workgroupsize, &fs);
-    if (rc != GS_OK) return rc;
-    if (!chart) return set_error(GS_EINVAL, "chart must not be NULL");
-    if (family->id == GS_FAM_FPU_GL3 || family->id == GS_ACT_HENON_HEILES)
-        return set_error(GS_EFAMILY, "a Reparametrized batch wraps one of the lane-per-system families (1-5)");
-    if (chart->kind == GS_CHART_POLAR) {
-        if (fs->n != 2) return set_error(GS_EFAMILY, "the polar chart maps 2 coordinates: families 1 and 2");
-    } else if (chart->kind == GS_CHART_LINEAR) {
-        for (int k = 0; k < fs->n * fs->n; ++k)
-            if (!std::isfinite(chart->B[k])) return set_error(GS_EINVAL, "the linear chart needs a finite B");
-        for (int k = 0; k < fs->n; ++k)
-            if (!std::isfinite(chart->c[k])) return set_error(GS_EINVAL, "the linear chart needs a finite c");
-    } else {
-        return set_error(GS_EINVAL, "chart kind must be GS_CHART_LINEAR or GS_CHART_POLAR");
-    }
-    if (nsys == 0) return GS_OK;
-    KernelArgs a;
-    std::memset(&a, 0, sizeof(a));
-    a.fam.P = static_cast<const char*>(P);
-    a.fam.p_stride = p_stride;
-    for (int k = 0; k < 8; ++k) a.fam.shared[k] = family->shared[k];
-    a.fam.chart.kind = chart->kind;
-    for (int k = 0; k < 16; ++k) a.fam.chart.B[k] = chart->B[k];
-    for (int k = 0; k < 4; ++k) a.fam.chart.c[k] = chart->c[k];
-    a.nsys = nsys;
-    a.X = X;
-    a.S = S;
-    a.codes = codes;
-    a.order = order;
-    a.q = dev_params(*method, *options);
-    const bool diag = S != nullptr;
-    const int wg = small_workgroup(workgroupsize);
-    hipStream_t st = static_cast<hipStream_t>(stream);
-    auto lock = device_call_lock();
-    if ((rc = check_order(order, nsys, st, nullptr)) != GS_OK) return rc;
-    const bool mixed = method->factor_precision == GS_FACTOR_F32;
-    const bool refine = method->refinement == GS_REFINEMENT;
-    bool timed = timing_begin(st);
-    const int glob = method->globalization == GS_BACKTRACKING ? kBacktracking
-                     : method->globalization == GS_DOGLEG    ? kDogLeg
-                                                             : kStatic;
-    rc = launch_small_reparam(family->id, chart->kind, glob, method->reuse > 0, a, mixed, refine, diag, wg, st);
-    return finish_launch(rc, st, timed);
+    return solve_reparametrized(method, family, chart, nsys, X, P, p_stride, S, codes, order, options, workgroupsize,
+                                stream);
+}
+
+int gs_solve_batch_reparametrized_f32(const gs_method* method, const gs_family* family, const gs_chart* chart,
+                                      int64_t nsys, float* X, const void* P, int64_t p_stride, gs_status_f32* S,
+                                      uint8_t* codes, const int64_t* order, const gs_options_f64* options,
+                                      int32_t workgroupsize, void* stream) {
+    return solve_reparametrized(method, family, chart, nsys, X, P, p_stride, S, codes, order, options, workgroupsize,
+                                stream);
 }
 
 int gs_solve_batch_f32(const gs_method* method, const gs_family* family, int64_t nsys, float* X, const void* P,
@@ -642,7 +686,10 @@ int gs_solve_batch_f32(const gs_method* method, const gs_family* family, int64_t
     g_last_error.clear();
     const FamilySpec* fs = nullptr;
     int rc = validate_batch<float>(method, family, nsys, X, P, p_stride, S, codes, options, workgroupsize, &fs);
-    if (rc != GS_OK || nsys == 0) return rc;
+    if (rc != GS_OK) return rc;
+    if (family->id == GS_ACT_HENON_HEILES)
+        return set_error(GS_EFAMILY, "an action is solved as a StationarityProblem (gs_solve_batch_stationarity_f32)");
+    if (nsys == 0) return rc;
     KernelArgsT<float> a;
     std::memset(&a, 0, sizeof(a));
     a.fam.P = static_cast<const char*>(P);

@@ -295,6 +295,13 @@ int launch_small_stationarity(int action, int ne, int glob, bool reuse, const Ke
 int launch_small_reparam(int family, int chart, int glob, bool reuse, const KernelArgs& a, bool mixed, bool refine,
                          bool diag, int wg, hipStream_t st);
 
+// Defined in small_stationarity_f32.hip and small_reparam_f32.hip: the Float32 forms of the two
+// batches above (no mixed form; the Reparametrized batch with the linear chart only).
+int launch_small_stationarity_f32(int action, int ne, int glob, bool reuse, const KernelArgsT<float>& a, bool refine,
+                                  bool diag, int wg, hipStream_t st);
+int launch_small_reparam_f32(int family, int chart, int glob, bool reuse, const KernelArgsT<float>& a, bool refine,
+                             bool diag, int wg, hipStream_t st);
+
 // Defined in small_dense.hip: the dense family at n = 2, 4, 6, 8 (FamDenseSinSmallT, A in
 // a.fam.data), in FP64 and Float32; glob is kStatic, kBacktracking or kDogLeg.
 int launch_small_dense(int n, int glob, bool reuse, const KernelArgs& a, bool mixed, bool refine, bool diag, int wg,

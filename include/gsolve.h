@@ -268,7 +268,8 @@ int gs_solve_batch_deflated_f64(const gs_method* method, const gs_family* family
 enum gs_datum_kind {
     GS_DATUM_NONE = 0,          /* b = nothing                                                       */
     GS_DATUM_SHARED = 1,        /* one datum for every system: b_shared[0 .. |E|-1] (a Number repeated) */
-    GS_DATUM_PER_SYSTEM = 2     /* PerSystem(b): system i reads |E| doubles at b_data + i·b_stride bytes */
+    GS_DATUM_PER_SYSTEM = 2     /* PerSystem(b): system i reads |E| entries of the batch's precision at
+                                   b_data + i·b_stride bytes (doubles; floats in the Float32 batch)   */
 };
 
 typedef struct gs_stationarity {
@@ -277,7 +278,7 @@ typedef struct gs_stationarity {
     int32_t e[8];               /* E (1-based), n_eq entries                                          */
     int32_t u[8];               /* U (1-based), n_eq entries                                          */
     double b_shared[8];
-    const double* b_data;       /* device                                                             */
+    const void* b_data;         /* device                                                             */
     int64_t b_stride;           /* bytes between the datum records of two systems                     */
 } gs_stationarity;
 
@@ -309,7 +310,7 @@ typedef struct gs_chart {
  * forms): X holds the chart coordinates y₀ on entry and receives x* = g(y*) (:860-865); S or codes
  * as for gs_solve_batch_f64. Families 1–5 with the linear chart, 1–2 with the polar chart; Static,
  * Backtracking and DogLeg, plain, mixed and refined, reuse = k (not with DogLeg; the JVP along a
- * held factor is J̃(y) v, reparametrized.jl:373-395), Float64. */
+ * held factor is J̃(y) v, reparametrized.jl:373-395). */
 int gs_solve_batch_reparametrized_f64(const gs_method* method, const gs_family* family, const gs_chart* chart,
                                       int64_t nsys, double* X, const void* P, int64_t p_stride, gs_status_f64* S,
                                       uint8_t* codes, const int64_t* order, const gs_options_f64* options,
@@ -327,6 +328,26 @@ int gs_solve_batch_f32(const gs_method* method, const gs_family* family, int64_t
                        float* X, const void* P, int64_t p_stride, gs_status_f32* S,
                        uint8_t* codes, const int64_t* order, const gs_options_f64* options,
                        int32_t workgroupsize, void* stream);
+
+/* The Float32 twin of gs_solve_batch_stationarity_f64 (stationarity.jl:642-698 on a batch of
+ * SVector{6,Float32} states): X holds nsys × M floats, the action's records (h, λ) are floats, a
+ * shared datum is used as Float32 and a per-system datum holds floats (b_data, b_stride 4-byte
+ * aligned); S receives SolverStatus{Float32}. Every form of the Float64 batch but the mixed one
+ * (factor_precision is ignored, as in gs_solve_batch_f32). */
+int gs_solve_batch_stationarity_f32(const gs_method* method, const gs_family* action, const gs_stationarity* stat,
+                                    int64_t nsys, float* X, const void* P, int64_t p_stride, gs_status_f32* S,
+                                    uint8_t* codes, const int64_t* order, const gs_options_f64* options,
+                                    int32_t workgroupsize, void* stream);
+
+/* The Float32 twin of gs_solve_batch_reparametrized_f64 (reparametrized.jl:981-1031 on a Float32
+ * batch, the chart adapted by ToReal{Float32}): the linear chart's B and c are used as Float32, the
+ * family's records hold floats as for gs_solve_batch_f32; S receives SolverStatus{Float32}. Families
+ * 1–5 with the linear chart (the polar chart is refused with GS_EFAMILY), every form of the Float64
+ * batch but the mixed one. */
+int gs_solve_batch_reparametrized_f32(const gs_method* method, const gs_family* family, const gs_chart* chart,
+                                      int64_t nsys, float* X, const void* P, int64_t p_stride, gs_status_f32* S,
+                                      uint8_t* codes, const int64_t* order, const gs_options_f64* options,
+                                      int32_t workgroupsize, void* stream);
 
 /* The six-count tally of a codes buffer (batched.jl:197-199), on the device; counts[6] is host memory. */
 int gs_tally_codes(const uint8_t* codes, int64_t nsys, int64_t counts[6], void* stream);

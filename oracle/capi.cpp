@@ -138,37 +138,37 @@ static void run_batch_d(const gs_deflation* d, const Method<R>& m, const Options
 
 // A Reparametrized batch: the problems of `make` in the chart coordinates of `ch`; X holds y₀ on entry
 // and receives x* = g(y*) (reparametrized.jl:860-865).
-template <int N, class SS, class Chart, class MakeProb>
-static void run_batch_chart(const Chart& ch, const Method<double>& m, const Options<double>& o, int64_t nsys,
-                            double* X, SS* S, uint8_t* codes, const int64_t* order, int nthreads, MakeProb make) {
+template <int N, class R, class SS, class Chart, class MakeProb>
+static void run_batch_chart(const Chart& ch, const Method<R>& m, const Options<R>& o, int64_t nsys, R* X, SS* S,
+                            uint8_t* codes, const int64_t* order, int nthreads, MakeProb make) {
 #pragma omp parallel for schedule(static) num_threads(nthreads)
     for (int64_t i = 0; i < nsys; ++i) {
         const int64_t idx = order ? order[i] - 1 : i;
         ReparamProb<N, decltype(make(idx)), Chart> prob{make(idx), ch};
-        Solver<double, N, decltype(prob)> solver(prob, m, o);
-        Vec<double, N> y;
+        Solver<R, N, decltype(prob)> solver(prob, m, o);
+        Vec<R, N> y;
         for (int k = 0; k < N; ++k) y[k] = X[idx * N + k];
-        Status<double> st = solver.solve(y);
-        const Vec<double, N> x = ch.g(y);
+        Status<R> st = solver.solve(y);
+        const Vec<R, N> x = ch.g(y);
         for (int k = 0; k < N; ++k) X[idx * N + k] = x[k];
         if (S) write_status(S + idx, st);
         if (codes) codes[idx] = st.code;
     }
 }
 
-template <int N, class SS, class MakeProb>
-static void run_batch_d(const gs_chart* c, const Method<double>& m, const Options<double>& o, int64_t nsys,
-                        double* X, SS* S, uint8_t* codes, const int64_t* order, int nthreads, MakeProb make) {
-    if constexpr (N == 2) {
+template <int N, class R, class SS, class MakeProb>
+static void run_batch_d(const gs_chart* c, const Method<R>& m, const Options<R>& o, int64_t nsys, R* X, SS* S,
+                        uint8_t* codes, const int64_t* order, int nthreads, MakeProb make) {
+    if constexpr (N == 2 && std::is_same_v<R, double>) {
         if (c->kind == GS_CHART_POLAR) {
             run_batch_chart<N>(ChartPolar{}, m, o, nsys, X, S, codes, order, nthreads, make);
             return;
         }
     }
-    ChartLinear<N> ch;
+    ChartLinear<N, R> ch;
     for (int i = 0; i < N; ++i) {
-        ch.c[i] = c->c[i];
-        for (int j = 0; j < N; ++j) ch.B[i][j] = c->B[i * N + j];
+        ch.c[i] = R(c->c[i]);
+        for (int j = 0; j < N; ++j) ch.B[i][j] = R(c->B[i * N + j]);
     }
     run_batch_chart<N>(ch, m, o, nsys, X, S, codes, order, nthreads, make);
 }
@@ -176,11 +176,9 @@ static void run_batch_d(const gs_chart* c, const Method<double>& m, const Option
 template <int N, class R, class SS, class MakeProb>
 static void run_batch_d(std::pair<const gs_deflation*, const gs_chart*> w, const Method<R>& m, const Options<R>& o,
                         int64_t nsys, R* X, SS* S, uint8_t* codes, const int64_t* order, int nthreads, MakeProb make) {
-    if constexpr (std::is_same_v<R, double>) {
-        if (w.second) {
-            run_batch_d<N>(w.second, m, o, nsys, X, S, codes, order, nthreads, make);
-            return;
-        }
+    if (w.second) {
+        run_batch_d<N>(w.second, m, o, nsys, X, S, codes, order, nthreads, make);
+        return;
     }
     run_batch_d<N>(w.first, m, o, nsys, X, S, codes, order, nthreads, make);
 }
@@ -279,16 +277,15 @@ static int solve_small_batch(const Method<R>& m, const Options<R>& o, const gs_f
 // A StationarityProblem batch over the action of `make`: each system's row of X is its full state
 // (A::M entries); the solve runs on the unknowns x[U] and writes back the full state x(u*)
 // (stationarity.jl:530-540).
-template <int N, class SS, class MakeAct>
-static void run_batch_stationarity(const Method<double>& m, const Options<double>& o, const gs_stationarity& t,
-                                   int64_t nsys, double* X, SS* S, uint8_t* codes, const int64_t* order, int nthreads,
-                                   MakeAct make) {
+template <int N, class R, class SS, class MakeAct>
+static void run_batch_stationarity(const Method<R>& m, const Options<R>& o, const gs_stationarity& t, int64_t nsys,
+                                   R* X, SS* S, uint8_t* codes, const int64_t* order, int nthreads, MakeAct make) {
 #pragma omp parallel for schedule(static) num_threads(nthreads)
     for (int64_t i = 0; i < nsys; ++i) {
         const int64_t idx = order ? order[i] - 1 : i;
         using Act = decltype(make(idx));
         constexpr int M = Act::M;
-        StationarityProb<double, N, Act> prob{make(idx)};
+        StationarityProb<R, N, Act> prob{make(idx)};
         for (int k = 0; k < M; ++k) prob.xref[k] = X[idx * M + k];
         for (int k = 0; k < N; ++k) {
             prob.e[k] = t.n_eq == 0 ? k : t.e[k] - 1;
@@ -296,15 +293,17 @@ static void run_batch_stationarity(const Method<double>& m, const Options<double
         }
         prob.has_b = t.b_kind != GS_DATUM_NONE;
         if (prob.has_b) {
-            const double* bp = t.b_kind == GS_DATUM_PER_SYSTEM
-                                   ? reinterpret_cast<const double*>(reinterpret_cast<const char*>(t.b_data) + idx * t.b_stride)
-                                   : t.b_shared;
-            for (int k = 0; k < N; ++k) prob.b[k] = bp[k];
+            if (t.b_kind == GS_DATUM_PER_SYSTEM) {
+                const R* bp = reinterpret_cast<const R*>(reinterpret_cast<const char*>(t.b_data) + idx * t.b_stride);
+                for (int k = 0; k < N; ++k) prob.b[k] = bp[k];
+            } else {
+                for (int k = 0; k < N; ++k) prob.b[k] = R(t.b_shared[k]);
+            }
         }
-        Vec<double, N> u = prob.extract(prob.xref);
-        Solver<double, N, decltype(prob)> solver(prob, m, o);
-        Status<double> st = solver.solve(u);
-        const Vec<double, M> x = solver.prob.embed(u);
+        Vec<R, N> u = prob.extract(prob.xref);
+        Solver<R, N, decltype(prob)> solver(prob, m, o);
+        Status<R> st = solver.solve(u);
+        const Vec<R, M> x = solver.prob.embed(u);
         for (int k = 0; k < M; ++k) X[idx * M + k] = x[k];
         if (S) write_status(S + idx, st);
         if (codes) codes[idx] = st.code;
@@ -427,6 +426,49 @@ int orc_solve_batch_f32(const gs_method* method, const gs_family* fam, int64_t n
     if (m.dl_radius != 0.0f) m.dl_radius = m.dl_radius > 1e10f ? 1e10f : (m.dl_radius < 1e-6f ? 1e-6f : m.dl_radius);
     Options<float> o = options_from<float>(*options);
     return solve_small_batch<float>(m, o, fam, nsys, X, P, p_stride, S, codes, order, nthreads);
+}
+
+// The host restatement of gs_solve_batch_stationarity_f32 (include/gsolve.h): the Float64 batch's
+// solve in Float32, the action's records and a per-system datum holding floats.
+int orc_solve_batch_stationarity_f32(const gs_method* method, const gs_family* act, const gs_stationarity* stat,
+                                     int64_t nsys, float* X, const void* P, int64_t p_stride, gs_status_f32* S,
+                                     uint8_t* codes, const int64_t* order, const gs_options_f64* options,
+                                     int nthreads) {
+    if (!method || !act || !options || !stat || nsys < 0 || act->id != GS_ACT_HENON_HEILES) return GS_EINVAL;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    Method<float> m = method_from<float>(*method);
+    m.factor_f32 = false;
+    if (m.dl_radius != 0.0f) m.dl_radius = m.dl_radius > 1e10f ? 1e10f : (m.dl_radius < 1e-6f ? 1e-6f : m.dl_radius);
+    Options<float> o = options_from<float>(*options);
+    const char* Pb = static_cast<const char*>(P);
+    auto make = [&](int64_t idx) {
+        const float* p = reinterpret_cast<const float*>(Pb + (p_stride ? idx * p_stride : 0));
+        return HenonHeilesAction<float>{p[0], p[1]};
+    };
+    const int ne = stat->n_eq == 0 ? 6 : stat->n_eq;
+    if (ne == 2)
+        run_batch_stationarity<2>(m, o, *stat, nsys, X, S, codes, order, nthreads, make);
+    else if (ne == 6)
+        run_batch_stationarity<6>(m, o, *stat, nsys, X, S, codes, order, nthreads, make);
+    else
+        return GS_EFAMILY;
+    return 0;
+}
+
+// The host restatement of gs_solve_batch_reparametrized_f32 (include/gsolve.h): the linear chart in
+// Float32.
+int orc_solve_batch_reparametrized_f32(const gs_method* method, const gs_family* fam, const gs_chart* chart,
+                                       int64_t nsys, float* X, const void* P, int64_t p_stride, gs_status_f32* S,
+                                       uint8_t* codes, const int64_t* order, const gs_options_f64* options,
+                                       int nthreads) {
+    if (!method || !fam || !options || !chart || nsys < 0) return GS_EINVAL;
+    if (chart->kind != GS_CHART_LINEAR) return GS_EFAMILY;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    Method<float> m = method_from<float>(*method);
+    m.factor_f32 = false;
+    if (m.dl_radius != 0.0f) m.dl_radius = m.dl_radius > 1e10f ? 1e10f : (m.dl_radius < 1e-6f ? 1e-6f : m.dl_radius);
+    Options<float> o = options_from<float>(*options);
+    return solve_small_batch<float>(m, o, fam, nsys, X, P, p_stride, S, codes, order, nthreads, nullptr, chart);
 }
 
 // F and J of the FPU-β Gauss–Legendre stage system at z (48), record qp = (q, p) (32): r (48) and

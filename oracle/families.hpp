@@ -638,18 +638,21 @@ template <class R, int N, class A> struct StationarityProb {
 // The registered charts of Reparametrized (x = g(y), G = ∂g/∂y) and the reparametrized problem
 // F̃(y) = F(g(y)), J̃ = J(g(y)) G(y) (_InnerReparametrized, reparametrized.jl:1-60 with
 // covariant = false; each entry of J G a left fold over k).
-template <int N> struct ChartLinear {
-    double B[N][N], c[N];
-    Vec<double, N> g(const Vec<double, N>& y) const {
-        Vec<double, N> x;
+// The linear chart in the working precision R_ (a Float32 batch holds B and c rounded to Float32, as
+// ToReal{Float32} adapts a chart's captured constants).
+template <int N, class R_ = double> struct ChartLinear {
+    using R = R_;
+    R B[N][N], c[N];
+    Vec<R, N> g(const Vec<R, N>& y) const {
+        Vec<R, N> x;
         for (int i = 0; i < N; ++i) {
-            double acc = B[i][0] * y[0];
+            R acc = B[i][0] * y[0];
             for (int j = 1; j < N; ++j) acc = acc + B[i][j] * y[j];
             x[i] = acc + c[i];
         }
         return x;
     }
-    void G(const Vec<double, N>&, double (&Gm)[N][N]) const {
+    void G(const Vec<R, N>&, R (&Gm)[N][N]) const {
         for (int i = 0; i < N; ++i)
             for (int j = 0; j < N; ++j) Gm[i][j] = B[i][j];
     }
@@ -657,6 +660,7 @@ template <int N> struct ChartLinear {
 
 // g_polar / dg_polar of test/nonlinear/reparametrized.jl:31-38.
 struct ChartPolar {
+    using R = double;
     Vec<double, 2> g(const Vec<double, 2>& y) const { return {y[0] * std::cos(y[1]), y[0] * std::sin(y[1])}; }
     void G(const Vec<double, 2>& y, double (&Gm)[2][2]) const {
         const double s = std::sin(y[1]), c = std::cos(y[1]);
@@ -668,21 +672,22 @@ struct ChartPolar {
 };
 
 template <int N, class P, class C> struct ReparamProb {
+    using R = typename C::R;
     P in;
     C ch;
     long nf = 0;
-    Vec<double, N> F(const Vec<double, N>& y) {
+    Vec<R, N> F(const Vec<R, N>& y) {
         ++nf;
         return in.F(ch.g(y));
     }
-    Mat<double, N> J(const Vec<double, N>& y) const {
-        const Mat<double, N> Ji = in.J(ch.g(y));
-        double Gm[N][N];
+    Mat<R, N> J(const Vec<R, N>& y) const {
+        const Mat<R, N> Ji = in.J(ch.g(y));
+        R Gm[N][N];
         ch.G(y, Gm);
-        Mat<double, N> Jm;
+        Mat<R, N> Jm;
         for (int i = 0; i < N; ++i)
             for (int j = 0; j < N; ++j) {
-                double acc = Ji[i] * Gm[0][j];
+                R acc = Ji[i] * Gm[0][j];
                 for (int k = 1; k < N; ++k) acc = acc + Ji[i + k * N] * Gm[k][j];
                 Jm[i + j * N] = acc;
             }

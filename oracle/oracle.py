@@ -42,6 +42,10 @@ def lib():
                                                        vp, ctypes.c_int]
         L.orc_solve_batch_reparametrized_f64.restype = ctypes.c_int
         L.orc_solve_batch_reparametrized_f64.argtypes = L.orc_solve_batch_deflated_f64.argtypes
+        L.orc_solve_batch_stationarity_f32.restype = ctypes.c_int
+        L.orc_solve_batch_stationarity_f32.argtypes = L.orc_solve_batch_stationarity_f64.argtypes
+        L.orc_solve_batch_reparametrized_f32.restype = ctypes.c_int
+        L.orc_solve_batch_reparametrized_f32.argtypes = L.orc_solve_batch_deflated_f64.argtypes
         L.orc_hh_action_eval.restype = None
         L.orc_hh_action_eval.argtypes = [ctypes.c_double, ctypes.c_double, vp, vp, vp]
         L.orc_pow_int.restype = None
@@ -152,7 +156,8 @@ def solve_batch(method, family, X: np.ndarray, p, options, order=None, nthreads:
     o = None if order is None else np.ascontiguousarray(np.asarray(order, dtype=np.int64))
     if reparametrized is not None:
         cc = reparametrized.to_c(family.n)
-        rc = lib().orc_solve_batch_reparametrized_f64(
+        entry = lib().orc_solve_batch_reparametrized_f32 if f32 else lib().orc_solve_batch_reparametrized_f64
+        rc = entry(
             ctypes.byref(cm), ctypes.byref(fam), ctypes.byref(cc), nsys, _ptr(X), _ptr(rec), stride, _ptr(S), None,
             None if o is None else _ptr(o), ctypes.byref(co), nthreads)
     elif deflated is not None:
@@ -171,15 +176,18 @@ def solve_batch(method, family, X: np.ndarray, p, options, order=None, nthreads:
 
 
 def _solve_stationarity(method, family, X, p, options, order, nthreads, prob, b):
-    X = np.ascontiguousarray(X, dtype=np.float64)
+    f32 = np.asarray(X).dtype == np.float32
+    real = np.float32 if f32 else np.float64
+    X = np.ascontiguousarray(X, dtype=real)
     nsys = X.shape[0]
-    rec, stride, shared = host_records(family, p, nsys, np.float64)
+    rec, stride, shared = host_records(family, p, nsys, real)
     cm, co = method.to_c(), options.to_c()
-    fam = _family_struct(family, shared)
-    cs, keep = prob.to_c(family.n, b, nsys)
-    S = np.zeros(nsys, dtype=_status_dtype())
+    fam = _family_struct(family, shared, real)
+    cs, keep = prob.to_c(family.n, b, nsys, dtype=real)
+    S = np.zeros(nsys, dtype=_status_dtype_f32() if f32 else _status_dtype())
     o = None if order is None else np.ascontiguousarray(np.asarray(order, dtype=np.int64))
-    rc = lib().orc_solve_batch_stationarity_f64(
+    entry = lib().orc_solve_batch_stationarity_f32 if f32 else lib().orc_solve_batch_stationarity_f64
+    rc = entry(
         ctypes.byref(cm), ctypes.byref(fam), ctypes.byref(cs), nsys, _ptr(X), _ptr(rec), stride, _ptr(S), None,
         None if o is None else _ptr(o), ctypes.byref(co), nthreads)
     del keep

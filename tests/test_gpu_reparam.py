@@ -141,3 +141,50 @@ def test_refusals(gs, device):
     with pytest.raises(gs.ArgumentError, match="takes no reuse"):
         gs.solve_batch(X, gs.Reparametrized(gs.NonlinearProblem(gs.TanhLinear3()), gs.LinearChart(np.eye(3))),
                        gs.Newton(globalization=gs.DogLeg(), reuse=2), p)
+
+
+F32_METHODS = {
+    "static": lambda gs: gs.Newton(),
+    "backtracking": lambda gs: gs.Newton(globalization=gs.Backtracking()),
+    "dogleg": lambda gs: gs.Newton(globalization=gs.DogLeg()),
+    "refine": lambda gs: gs.Newton(refinement=gs.Refinement()),
+    "reuse3": lambda gs: gs.Newton(reuse=3),
+    "best": lambda gs: gs.Newton(best=True),
+}
+
+
+@pytest.mark.parametrize("name", list(F32_METHODS))
+def test_float32_linear_chart_bitwise(gs, orc, workloads, device, name):
+    """gs_solve_batch_reparametrized_f32: the linear-chart cases of the Float64 test with Float32
+    starts and records (the chart's B and c rounded to Float32 on both sides), bit for bit against the
+    oracle's Float32 solve; the polar chart is refused in Float32."""
+    method = F32_METHODS[name](gs)
+    for fam, Y0, p, B, c in _linear_cases(gs, workloads, 4096):
+        Y0 = Y0.astype(np.float32)
+        if isinstance(p, gs.PerSystem):
+            data = np.asarray(p.data)
+            p = gs.PerSystem(data if data.dtype.kind == "i" else data.astype(np.float32))
+        else:
+            p = gs.BatchParams(p.shared, np.asarray(p.per_system).astype(np.float32))
+        rep = gs.Reparametrized(gs.NonlinearProblem(fam), gs.LinearChart(B, c))
+        for opts in (gs.Options(eltype="float32", f_abstol=1e-5, max_iterations=40),
+                     gs.Options(eltype="float32", norm=gs.MaxNorm(), f_abstol=1e-5, max_iterations=40)):
+            X = torch.as_tensor(Y0, device=device).clone()
+            if isinstance(p, gs.PerSystem):
+                pd = gs.PerSystem(torch.as_tensor(np.asarray(p.data), device=device))
+            else:
+                pd = gs.BatchParams(p.shared, torch.as_tensor(np.asarray(p.per_system), device=device))
+            S = gs.status_buffer(X.shape[0], device, "float32")
+            gs.solve_batch(X, S, rep, method, pd, opts)
+            x, st = X.cpu().numpy(), gs.statuses(S)
+            xo, so = orc.solve_batch(method, fam, Y0.copy(), p, opts, reparametrized=rep)
+            assert np.array_equal(x.view(np.uint32), xo.view(np.uint32)), type(fam).__name__
+            sb = st.view(np.uint8).reshape(len(st), 44)[:, :42]
+            ob = so.view(np.uint8).reshape(len(so), 44)[:, :42]
+            assert (sb == ob).all(), type(fam).__name__
+        print(name, type(fam).__name__, np.bincount(st["code"], minlength=6).tolist())
+    X = torch.zeros((4, 2), dtype=torch.float32, device=device)
+    polar = gs.Reparametrized(gs.NonlinearProblem(gs.Coupled2()), gs.PolarChart())
+    with pytest.raises(gs.ArgumentError, match="linear chart"):
+        gs.solve_batch(X, polar, method, gs.PerSystem(torch.ones((4, 2), dtype=torch.float32, device=device)),
+                       gs.Options(eltype="float32"))

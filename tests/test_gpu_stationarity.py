@@ -116,3 +116,55 @@ def test_refusals(gs, device):
     with pytest.raises(gs.DimensionMismatch):
         gs.solve_batch(X, gs.StationarityProblem(act, E=(3, 4), U=(5, 6)), gs.Newton(), (H, 1.0),
                        b=gs.PerSystem(np.zeros((3, 2))))
+
+
+F32_METHODS = {
+    "static": lambda gs: gs.Newton(),
+    "backtracking": lambda gs: gs.Newton(globalization=gs.Backtracking()),
+    "dogleg": lambda gs: gs.Newton(globalization=gs.DogLeg()),
+    "refine-regularized": lambda gs: gs.Newton(refinement=gs.Refinement(), regularization=1e-3),
+    "reuse3": lambda gs: gs.Newton(reuse=3),
+    "best-backtracking": lambda gs: gs.Newton(globalization=gs.Backtracking(), best=True),
+}
+
+
+def _assert_f32_bitwise(x, st, xo, so):
+    """Float32 iterates bit for bit and the 42 defined bytes of SolverStatus{Float32}."""
+    assert x.dtype == np.float32 and st.dtype.itemsize == 44
+    assert np.array_equal(x.view(np.uint32), xo.view(np.uint32))
+    sb = st.view(np.uint8).reshape(len(st), 44)[:, :42]
+    ob = so.view(np.uint8).reshape(len(so), 44)[:, :42]
+    bad = np.nonzero((sb != ob).any(axis=1))[0]
+    assert bad.size == 0, f"{bad.size} statuses differ; first {st[bad[0]]} vs {so[bad[0]]}"
+
+
+@pytest.mark.parametrize("name", list(F32_METHODS))
+def test_float32_batch_bitwise(gs, orc, device, name):
+    """gs_solve_batch_stationarity_f32: |E| = 2 with λ per system (Float32 records) and far-off starts,
+    |E| = 6 with a PerSystem Float32 datum; L2 and MaxNorm; bit for bit against the oracle's Float32 solve."""
+    n = 8192
+    X0 = states(n, 51)
+    X0[::4, 4:6] += np.random.default_rng(52).uniform(-3, 3, size=(n // 4, 2))
+    X0 = X0.astype(np.float32)
+    lam = np.random.default_rng(53).uniform(0.5, 1.5, size=n)
+    p = gs.PerSystem(np.stack([np.full(n, H), lam], axis=1).astype(np.float32))
+    act = gs.HenonHeilesAction()
+    method = F32_METHODS[name](gs)
+    prob = gs.StationarityProblem(act, E=(3, 4), U=(5, 6))
+    for opts in (gs.Options(eltype="float32", f_abstol=1e-5, max_iterations=40),
+                 gs.Options(eltype="float32", norm=gs.MaxNorm(), f_abstol=1e-5, max_iterations=40)):
+        X = torch.as_tensor(X0, device=device).clone()
+        S = gs.status_buffer(n, device, "float32")
+        gs.solve_batch(X, S, prob, method, gs.PerSystem(torch.as_tensor(p.data, device=device)), opts)
+        xo, so = orc.solve_batch(method, act, X0.copy(), p, opts, stationarity=prob)
+        _assert_f32_bitwise(X.cpu().numpy(), gs.statuses(S), xo, so)
+    bs = np.random.default_rng(54).uniform(-0.05, 0.05, size=(2048, 6)).astype(np.float32)
+    full = gs.StationarityProblem(act)
+    opts = gs.Options(eltype="float32", f_abstol=1e-5, max_iterations=30)
+    X = torch.as_tensor(X0[:2048], device=device).clone()
+    S = gs.status_buffer(2048, device, "float32")
+    gs.solve_batch(X, S, full, method, (H, 1.0), opts, b=gs.PerSystem(torch.as_tensor(bs, device=device)))
+    xo, so = orc.solve_batch(method, act, X0[:2048].copy(), (H, 1.0), opts, stationarity=full, b=gs.PerSystem(bs))
+    st = gs.statuses(S)
+    _assert_f32_bitwise(X.cpu().numpy(), st, xo, so)
+    print(name, np.bincount(st["code"], minlength=6).tolist())

@@ -78,3 +78,35 @@ def test_reparametrized_checks(gs):
         gs.Reparametrized(gs.NonlinearProblem(gs.TanhLinear3()), gs.PolarChart()).to_c(3)
     with pytest.raises(gs.DimensionMismatch):
         gs.Reparametrized(prob, gs.LinearChart(np.eye(3))).to_c(2)
+
+
+def test_float32_linear_chart(gs, orc):
+    """The Float32 batch with the linear chart: the reference's batch KAT in Float32 (x − c under
+    x = 2y reaches c to 8 eps(Float32)), and affine invariance against the plain Float32 solve (the
+    same iteration counts, the same solution to Float32 rounding); the polar chart has no Float32 form."""
+    fam = gs.Hetero2()
+    rep = gs.Reparametrized(gs.NonlinearProblem(fam), gs.LinearChart(2 * np.eye(2)))
+    o32 = gs.Options(eltype="float32", f_abstol=1e-6)
+    for m in (gs.Newton(), gs.Newton(globalization=gs.Backtracking()), gs.Newton(refinement=gs.Refinement())):
+        x, s = orc.solve_batch(m, fam, np.array([[1.0, 1.0], [0.0, 0.0]], dtype=np.float32),
+                               gs.PerSystem(np.array([[1], [1]])), o32, reparametrized=rep)
+        assert x.dtype == np.float32 and (s["code"] == gs.SUCCESS).all()
+        assert np.allclose(x, [[2.0, 3.0], [2.0, 3.0]], rtol=8 * np.finfo(np.float32).eps, atol=0)
+    rng = np.random.default_rng(6)
+    n = 512
+    fam = gs.Coupled2()
+    p = gs.PerSystem(np.tile([10.0, 10.0], (n, 1)).astype(np.float32))
+    X0 = (np.array(R1) + rng.uniform(-1.5, 1.5, size=(n, 2))).astype(np.float32)
+    B = np.array([[1.0, 0.5], [0.0, 2.0]])  # exact in Float32, so the chart is the same map
+    c = np.array([0.25, -1.0])
+    Y0 = np.linalg.solve(B, (X0.astype(np.float64) - c).T).T.astype(np.float32)
+    for m in (gs.Newton(), gs.Newton(globalization=gs.Backtracking())):
+        xp, sp = orc.solve_batch(m, fam, X0.copy(), p, o32)
+        rep = gs.Reparametrized(gs.NonlinearProblem(fam), gs.LinearChart(B, c))
+        x, s = orc.solve_batch(m, fam, Y0.copy(), p, o32, reparametrized=rep)
+        assert (s["code"] == gs.SUCCESS).all() and (sp["code"] == gs.SUCCESS).all()
+        assert np.abs(s["iterations"].astype(int) - sp["iterations"].astype(int)).max() <= 1
+        assert np.abs(x - xp).max() <= 1e-5 * np.abs(xp).max()
+    polar = gs.Reparametrized(gs.NonlinearProblem(fam), gs.PolarChart())
+    with pytest.raises(RuntimeError):
+        orc.solve_batch(gs.Newton(), fam, X0[:4].copy(), gs.PerSystem(p.data[:4]), o32, reparametrized=polar)

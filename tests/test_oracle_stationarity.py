@@ -172,3 +172,32 @@ def test_stationarity_checks(gs):
         S(act).to_c(6, gs.PerSystem(np.zeros((4, 2))), 4)
     with pytest.raises(gs.ArgumentError, match="registered action"):
         S(gs.Coupled2())
+
+
+def test_float32_batch_against_float64(gs, orc):
+    """The Float32 batch (SolverStatus{Float32}, records and datum in Float32): λ = 0 against the
+    closed-form solution of the linear equations, and λ = 1 against the Float64 solve, to Float32
+    rounding; the entries outside U keep their bits, every system reaches SUCCESS."""
+    n = 2048
+    X0 = states(n, 41)
+    act = gs.HenonHeilesAction()
+    opts32 = gs.Options(eltype="float32", f_abstol=1e-5, max_iterations=30)
+    prob = gs.StationarityProblem(act, E=(3, 4), U=(5, 6))
+    for m in (gs.Newton(), gs.Newton(globalization=gs.Backtracking()), gs.Newton(globalization=gs.DogLeg()),
+              gs.Newton(reuse=2)):
+        x64, s64 = orc.solve_batch(m, act, X0.copy(), (H, 1.0), gs.Options(f_abstol=1e-13, max_iterations=30),
+                                   stationarity=prob)
+        X32 = X0.astype(np.float32)
+        x32, s32 = orc.solve_batch(m, act, X32.copy(), (H, 1.0), opts32, stationarity=prob)
+        assert x32.dtype == np.float32 and s32.dtype.itemsize == 44
+        assert (s32["code"] == gs.SUCCESS).all() and (s64["code"] == gs.SUCCESS).all()
+        assert (x32[:, 0:4].view(np.uint32) == X32[:, 0:4].view(np.uint32)).all()
+        assert np.abs(x32 - x64).max() <= 1e-5
+    # λ = 0: q₊ solves a linear system; its closed form from the Float64 linear solve
+    x32, s32 = orc.solve_batch(gs.Newton(), act, X0.astype(np.float32), (H, 0.0), opts32, stationarity=prob)
+    Hm = hess_cd(np.zeros(6), H, 0.0)
+    for i in range(0, n, 97):
+        x = X0[i].copy()
+        g0 = grad_cs(np.concatenate([x[0:4], [0.0, 0.0]]), H, 0.0)
+        sol = np.linalg.solve(Hm[np.ix_([2, 3], [4, 5])], -g0[[2, 3]])
+        assert np.abs(x32[i, 4:6] - sol).max() <= 1e-5 * max(1.0, np.abs(sol).max())
