@@ -294,12 +294,11 @@ end
 # to the generic path.
 served(::HIPFamily, cm, ::Type{R}) where {R} = true
 served(::FpuGl3, cm, ::Type{R}) where {R} = R === Float64 && cm.globalization in (0, 1, 3)
-# the dense family in Float64 or Float32: n = 2, 4, 6, 8 on the lane-per-system kernels (every
-# globalization), n = 12, 16, 24, 32, 48, 64 on the dense kernel (Static or Backtracking); with or
-# without `reuse`
+# the dense family in Float64 or Float32: n = 2, 4, 6, 8 (and 12 in Float32) on the lane-per-system
+# kernels, n = 12, 16, 24, 32, 48, 64 on the dense kernel; Static, Backtracking (with or without
+# `reuse`) and DogLeg
 served(::DenseSin{N}, cm, ::Type{R}) where {N, R} = (R === Float64 || R === Float32) &&
-    ((N in (2, 4, 6, 8) && cm.globalization in (0, 1, 3)) ||
-     (N in (12, 16, 24, 32, 48, 64) && cm.globalization in (0, 1)))
+    N in (2, 4, 6, 8, 12, 16, 24, 32, 48, 64) && cm.globalization in (0, 1, 3)
 
 # ---- the method and options as ABI structs ----------------------------------------------------
 

@@ -16,7 +16,7 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIB = HERE / "lib" / "libgsolve_hip.so"
-SOURCES = ["gsolve.hip", "small_reuse.hip", "small_dogleg.hip", "small_f32.hip", "small_deflated.hip", "small_stationarity.hip", "small_reparam.hip", "small_stationarity_f32.hip", "small_reparam_f32.hip", "batch_reduce.hip", "lsq.hip", "mlp.hip", "mid_solver.hip", "mid_reuse.hip", "dense_solver.hip", "small_dense.hip", "dense_n12.hip", "dense_r12.hip", "dense_n16.hip", "dense_n24.hip", "dense_n32.hip", "dense_n48.hip", "dense_n64.hip", "dense_r16.hip", "dense_r24.hip", "dense_r32.hip", "dense_r48.hip", "dense_r64.hip", "dense_f16.hip", "dense_f24.hip", "dense_f32.hip", "dense_f48.hip", "dense_f64.hip"]
+SOURCES = ["gsolve.hip", "small_reuse.hip", "small_dogleg.hip", "small_f32.hip", "small_deflated.hip", "small_stationarity.hip", "small_reparam.hip", "small_stationarity_f32.hip", "small_reparam_f32.hip", "batch_reduce.hip", "lsq.hip", "mlp.hip", "mid_solver.hip", "mid_reuse.hip", "dense_solver.hip", "small_dense.hip", "dense_n12.hip", "dense_r12.hip", "dense_n16.hip", "dense_n24.hip", "dense_n32.hip", "dense_n48.hip", "dense_n64.hip", "dense_r16.hip", "dense_r24.hip", "dense_r32.hip", "dense_r48.hip", "dense_r64.hip", "dense_d12.hip", "dense_d16.hip", "dense_d24.hip", "dense_d32.hip", "dense_d48.hip", "dense_d64.hip", "dense_f16.hip", "dense_f24.hip", "dense_f32.hip", "dense_f48.hip", "dense_f64.hip"]
 ARCH = os.environ.get("GSOLVE_OFFLOAD_ARCH", "gfx950")
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",

@@ -297,6 +297,9 @@ template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool
     int kpos, kfe;
     int age;
     bool stalled, reused;
+    // DogLeg: the trust radius (TrustRadius, newton.jl:637-640), 0 until the first step sets it
+    T radius;
+    static_assert(!(REUSE && GLOB == kDogLeg), "Newton with a DogLeg globalization takes no reuse");
 
     GS_DEV DenseSolve(const Fam& f, const DevParamsT<T>& p, SysVecs<T, N>& vv, int lane)
         : fam(f), q(p), v(vv), sl(lane), live(lane < N) {}
@@ -620,6 +623,132 @@ template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool
         return step_backtracking(exact);
     }
 
+    // DogLeg (dogleg.jl:116-206, newton.jl:1725-1780): the radius clamps of the reference.
+    GS_DEV static T dl_clamp(T a) { return a > T(1e10) ? T(1e10) : (a < T(1e-6) ? T(1e-6) : a); }
+    GS_DEV static T dl_shrink(T a) { return jmax(T(0.25) * a, T(1e-6)); }
+    GS_DEV static T dl_grow(T a) { return jmin(T(4) * a, T(1e10)); }
+
+    // Jreg's diagonal entry of this lane's row at x (a_kk + cos(x_k)/10, then + μ where μ ≠ 0): the
+    // same operations as the row jacobian() forms, so the same bits.
+    GS_DEV T jreg_diag() const {
+        const int k = live ? sl : 0;
+        const T dd = fam.A[k + k * N] + fam.Jdiag(v.x, k);
+        return q.mu != T(0) ? dd + q.mu : dd;
+    }
+
+    // out = Jregᵀ (c v), this lane's entry k = sl: column k of Jreg is column k of A (contiguous in
+    // LDS) with the lane's own diagonal entry dkk; a left fold of fused multiply-adds over j, each
+    // term's c·v_j formed first (the lane-per-system dogleg_trial and the oracle's matvec_t).
+    GS_DEV void jtmatvec(T dkk, T c, const T* vin, T* out) {
+        const int k = live ? sl : 0;
+        const T* col = fam.A + k * N;
+        T acc = (k == 0 ? dkk : col[0]) * (c * vin[0]);
+#pragma unroll 8
+        for (int j = 1; j < N; ++j) acc = tfma(j == k ? dkk : col[j], c * vin[j], acc);
+        if (live) out[sl] = acc;
+        wsync();
+    }
+
+    // The trial loop of dogleg_step on the model matrix Jreg (the row jr, the lane's diagonal dkk)
+    // with the Newton step v.d (read where `newton`): the Newton step inside the radius, else the
+    // Cauchy point cut to the radius, else the point of norm δ on the segment between them; a trial
+    // is accepted when pred > 0 and ared ≥ 1e-4 pred (unsquared norms), a rejection shrinks δ to
+    // max(δ/4, 1e-6), and a rejection at 1e-6 fails. ĝ in v.r, the trial step in v.u, its point in
+    // v.xt and residual in v.Ft, the products in v.t; x, Fx and the radius updated.
+    GS_DEV Step dogleg_trial(const JR& jr, T dkk, bool newton) {
+        const T lo = T(1e-6);
+        const T fnorm = vrnorm<N>(v.Fx);
+        if (fnorm == T(0)) return Step{kSuccess, false, false, 0, fnorm, T(0), false};
+        const T nN = newton ? vrnorm<N>(v.d) : T(__builtin_inf());
+        // the Cauchy direction from the unit residual: g̃ = Jregᵀ (F/‖F‖), ĝ = g̃/‖g̃‖
+        const T finv = T(1) / fnorm;
+        jtmatvec(dkk, finv, v.Fx, v.r);
+        const T gnorm = vrnorm<N>(v.r);
+        const T ginv = T(1) / gnorm;
+        if (live) v.r[sl] = ginv * v.r[sl];
+        wsync();
+        matvec(jr, v.r, v.t);
+        const T jnorm = vrnorm<N>(v.t);
+        const bool cauchy = gnorm != T(0) && isfin(gnorm) && jnorm != T(0) && isfin(jnorm);
+        const T nC = fnorm * ((gnorm / jnorm) / jnorm);
+        T delta = radius;
+        if (delta == T(0) && (newton || cauchy)) {
+            const T n0 = newton ? nN : nC;
+            delta = n0 < lo ? T(2) * lo : dl_clamp(n0);
+        }
+        while (true) {
+            bool boundary;
+            const bool inside = newton && nN <= delta;
+            if (inside) {
+                if (live) v.u[sl] = v.d[sl];
+                boundary = nN >= delta;
+            } else if (!cauchy) {
+                radius = delta;
+                return Step{kStalled, false, false, 0, fnorm, T(0), false};
+            } else if (!newton || nC >= delta) {
+                const T a = -jmin(nC, delta);
+                if (live) v.u[sl] = a * v.r[sl];
+                boundary = nC >= delta;
+            } else {
+                // the root t ≥ 0 of ‖a + t b̂‖ = 1 in units of δ, a = −(nC/δ) ĝ (v.u), b = sN + nC ĝ (v.t)
+                const T ac = -(nC / delta);
+                if (live) {
+                    v.u[sl] = ac * v.r[sl];
+                    v.t[sl] = T(1) * v.d[sl] + nC * v.r[sl];
+                }
+                wsync();
+                const T binv = T(1) / vrnorm<N>(v.t);
+                if (live) v.t[sl] = binv * v.t[sl];
+                wsync();
+                const T ab = vdot<N>(v.u, v.t);
+                const T c = T(1) - vnorm2<N>(v.u);
+                const T ab2 = ab * ab;
+                const T t = c / (ab + tsqrt(ab2 + c));
+                const T dt = delta * t;
+                if (live) v.u[sl] = delta * v.u[sl] + dt * v.t[sl];
+                boundary = true;
+            }
+            wsync();
+            matvec(jr, v.u, v.t);
+            if (live) {
+                v.t[sl] = T(1) * v.Fx[sl] + T(1) * v.t[sl];
+                v.xt[sl] = v.x[sl] + T(1) * v.u[sl];
+            }
+            wsync();
+            const T pred = fnorm - vrnorm<N>(v.t);
+            eval_F(v.xt, v.Ft);
+            const T fnp = vrnorm<N>(v.Ft);
+            const T ared = fnorm - fnp;
+            if (pred > T(0) && ared >= T(1e-4) * pred) {
+                const T rho = ared / pred;
+                if (rho < T(0.1)) {
+                    delta = nN < delta ? jmax(nN, lo) : dl_shrink(delta);
+                } else if (rho > T(0.75) && boundary) {
+                    delta = dl_grow(delta);
+                }
+                const T stepnorm = vrnorm<N>(v.u);
+                if (live) {
+                    v.x[sl] = v.xt[sl];
+                    v.Fx[sl] = v.Ft[sl];
+                }
+                wsync();
+                radius = delta;
+                return Step{kSuccess, false, false, 0, fnp, stepnorm, !inside};
+            }
+            if (delta == lo) {
+                radius = delta;
+                return Step{kLsFailed, false, false, 1, fnorm, T(0), false};
+            }
+            delta = dl_shrink(delta);
+        }
+    }
+
+    // A Newton step of finite, nonzero norm in v.d (dogleg_direction, newton.jl:1741-1746).
+    GS_DEV bool newton_usable() const {
+        const T nd = vrnorm<N>(v.d);
+        return isfin(nd) && nd != T(0);
+    }
+
     // ---- steps ----
 
     // newton_step!! in the working precision (newton.jl:1679-1688).
@@ -630,7 +759,15 @@ template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool
         int pos, fe;
         const int info = factor<T>(jr, a, pos, fe);
         Step s;
-        if (info > 0) {
+        if constexpr (GLOB == kDogLeg) {
+            // newton_step!! with DogLeg (newton.jl:1725-1737): a failed factor leaves the Cauchy step
+            bool newton = false;
+            if (info <= 0) {
+                direction<T>(jr, a, pos, fe, true);
+                newton = newton_usable();
+            }
+            s = dogleg_trial(jr, jreg_diag(), newton);
+        } else if (info > 0) {
             s = Step{kSingular, false, false, 0, vrnorm<N>(v.Fx), T(0), false};
         } else {
             direction<T>(jr, a, pos, fe, true);
@@ -675,13 +812,19 @@ template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool
             if (info > 0) ok = false;
             else direction<double>(jr, a, pos, fe, was_promoted);
         }
-        if (!ok) {
-            Step s{kSingular, promoted, false, failures, vrnorm<N>(v.Fx), 0.0, false};
-            sfnorm = s.fnorm;
-            return s;
-        }
         const bool promoted_step = promoted && !was_promoted;
-        Step s = globalize(q.mu == 0.0 && promoted);
+        Step s;
+        if constexpr (GLOB == kDogLeg) {
+            // mixed_step!! with DogLeg (newton.jl:1757-1780): no factor leaves the Cauchy step
+            s = dogleg_trial(jr, jreg_diag(), ok && newton_usable());
+        } else {
+            if (!ok) {
+                s = Step{kSingular, promoted, false, failures, vrnorm<N>(v.Fx), 0.0, false};
+                sfnorm = s.fnorm;
+                return s;
+            }
+            s = globalize(q.mu == 0.0 && promoted);
+        }
         s.failures += failures;
         if (promoted_step) {
             s.promoted = true;
@@ -940,6 +1083,7 @@ template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool
         age = 0;
         stalled = false;
         reused = false;
+        radius = q.dl_radius;  // reset_rule (newton.jl:677)
         T best_fn = fnorm0;
         if (q.best) copy(v.x, v.bx);
         Result R{kSuccess, false, false, false, 0, 0, fnorm0, T(0), tnan<T>(), tnan<T>()};
@@ -978,7 +1122,9 @@ template <class Fam, int NP_, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool
                         R.code = kSuccess;
                         break;
                     }
-                    const bool steptested = s.code != kStalled;
+                    // a stalled line search, or a step that a trust radius limited, passes only the
+                    // residual test
+                    const bool steptested = s.code != kStalled && !s.limited;
                     const T xnorm = steptested ? vrnorm<N>(v.x) : tnan<T>();
                     const T xtol = jmax(q.x_abstol, q.x_reltol * xnorm);
                     const bool past_min = R.iterations >= q.min_iterations;
@@ -1083,6 +1229,11 @@ int launch_diag(bool diag, const DenseLaunchT<T>& L, hipStream_t st) {
 // (dense_f16.hip … dense_f64.hip).
 template <int N, int NP>
 int launch_np_f32(int glob, bool refine, bool diag, bool reuse, const DenseLaunchT<float>& L, hipStream_t st) {
+    if (glob == kDogLeg) {
+        if (reuse) return set_error(GS_EINVAL, "Newton with a DogLeg globalization takes no reuse");
+        return refine ? launch_diag<float, N, NP, kDogLeg, false, true, false>(diag, L, st)
+                      : launch_diag<float, N, NP, kDogLeg, false, false, false>(diag, L, st);
+    }
     if (reuse) {
         if (glob == kBacktracking)
             return refine ? launch_diag<float, N, NP, kBacktracking, false, true, true>(diag, L, st)
@@ -1101,6 +1252,8 @@ int launch_np_f32(int glob, bool refine, bool diag, bool reuse, const DenseLaunc
 // units, dense_r16.hip … dense_r64.hip).
 template <int N, int NP, bool REUSE = false>
 int launch_np(int glob, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st) {
+    if (glob != kStatic && glob != kBacktracking)
+        return set_error(GS_EINVAL, "this dense launch takes the Static or Backtracking globalization");
     if (glob == kBacktracking) {
         if (mixed) return refine ? launch_diag<double, N, NP, kBacktracking, true, true, REUSE>(diag, L, st)
                                  : launch_diag<double, N, NP, kBacktracking, true, false, REUSE>(diag, L, st);
@@ -1111,6 +1264,15 @@ int launch_np(int glob, bool mixed, bool refine, bool diag, const DenseLaunch& L
                              : launch_diag<double, N, NP, kStatic, true, false, REUSE>(diag, L, st);
     return refine ? launch_diag<double, N, NP, kStatic, false, true, REUSE>(diag, L, st)
                   : launch_diag<double, N, NP, kStatic, false, false, REUSE>(diag, L, st);
+}
+
+// The DogLeg launches of one size and segment width (dense_d12.hip … dense_d64.hip).
+template <int N, int NP>
+int launch_np_dogleg(bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st) {
+    if (mixed) return refine ? launch_diag<double, N, NP, kDogLeg, true, true, false>(diag, L, st)
+                             : launch_diag<double, N, NP, kDogLeg, true, false, false>(diag, L, st);
+    return refine ? launch_diag<double, N, NP, kDogLeg, false, true, false>(diag, L, st)
+                  : launch_diag<double, N, NP, kDogLeg, false, false, false>(diag, L, st);
 }
 
 }  // namespace

@@ -1,8 +1,8 @@
 // libgsolve_hip, dense mid-size systems (n = 12, 16, 24, 32, 48, 64): the launch dispatch over the
 // sizes and their lanes per system (the kernels are in dense_kernel.hpp, instantiated one size per
-// translation unit in dense_n16.hip … dense_n64.hip, with reuse = k in dense_r16.hip …
-// dense_r64.hip, and in Float32 in dense_f16.hip … dense_f64.hip), and the shape setting of the
-// autotuner.
+// translation unit in dense_n12.hip … dense_n64.hip, with reuse = k in dense_r12.hip …
+// dense_r64.hip, under DogLeg in dense_d12.hip … dense_d64.hip, and in Float32 in dense_f16.hip …
+// dense_f64.hip), and the shape setting of the autotuner.
 #include <hip/hip_runtime.h>
 
 #include <atomic>
@@ -32,6 +32,13 @@ int launch_dense_f32_n48(int np, int glob, bool refine, bool diag, bool reuse, c
                          hipStream_t st);
 int launch_dense_f32_n64(int np, int glob, bool refine, bool diag, bool reuse, const DenseLaunchT<float>& L,
                          hipStream_t st);
+// The DogLeg launches (dense_d12.hip … dense_d64.hip).
+int launch_dense_dogleg_n12(int np, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st);
+int launch_dense_dogleg_n16(int np, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st);
+int launch_dense_dogleg_n24(int np, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st);
+int launch_dense_dogleg_n32(int np, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st);
+int launch_dense_dogleg_n48(int np, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st);
+int launch_dense_dogleg_n64(int np, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st);
 // The reuse = k launches (dense_r16.hip … dense_r64.hip).
 int launch_dense_reuse_n12(int np, int glob, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st);
 int launch_dense_reuse_n16(int np, int glob, bool mixed, bool refine, bool diag, const DenseLaunch& L, hipStream_t st);
@@ -82,6 +89,18 @@ int dense_set_shape(int n, int lanes) {
 int launch_dense(int n, int glob, bool mixed, bool refine, bool diag, bool reuse, const DenseLaunch& L,
                  hipStream_t st) {
     const int np = lanes_of(n);
+    if (glob == kDogLeg) {
+        if (reuse) return set_error(GS_EINVAL, "Newton with a DogLeg globalization takes no reuse");
+        switch (n) {
+            case 12: return launch_dense_dogleg_n12(np, mixed, refine, diag, L, st);
+            case 16: return launch_dense_dogleg_n16(np, mixed, refine, diag, L, st);
+            case 24: return launch_dense_dogleg_n24(np, mixed, refine, diag, L, st);
+            case 32: return launch_dense_dogleg_n32(np, mixed, refine, diag, L, st);
+            case 48: return launch_dense_dogleg_n48(np, mixed, refine, diag, L, st);
+            case 64: return launch_dense_dogleg_n64(np, mixed, refine, diag, L, st);
+        }
+        return set_error(GS_EINVAL, "the dense kernel takes n = 12, 16, 24, 32, 48 or 64");
+    }
     if (reuse) {
         switch (n) {
             case 12: return launch_dense_reuse_n12(np, glob, mixed, refine, diag, L, st);

@@ -158,8 +158,6 @@ static int validate_batch(const gs_method* method, const gs_family* family, int6
         if (method->reuse != 0) return fail(GS_EINVAL, "Newton with a DogLeg globalization takes no reuse");
     }
     if (method->reuse < 0) return fail(GS_EINVAL, "reuse must be ≥ 1, or 0 for nothing");
-    if (family->id == GS_FAM_DENSE_SIN && family->n > 8 && method->globalization == GS_DOGLEG)
-        return fail(GS_EFAMILY, "the dense solve at n > 8 takes Static or Backtracking");
     if (method->factor_precision != GS_FACTOR_WORKING && method->factor_precision != GS_FACTOR_F32)
         return fail(GS_EINVAL, "factor_precision must be GS_FACTOR_WORKING or GS_FACTOR_F32");
     if (method->refinement != GS_NO_REFINEMENT && method->refinement != GS_REFINEMENT)
@@ -416,9 +414,8 @@ static int solve_batch_f64(const gs_method* method, const gs_family* family, int
         L.S = S;
         L.codes = codes;
         L.order = order;
-        return finish_launch(launch_dense(family->n, glob == GS_BACKTRACKING ? kBacktracking : kStatic, mixed, refine,
-                                          diag, method->reuse > 0, L, st),
-                             st, timed);
+        const int g = glob == GS_BACKTRACKING ? kBacktracking : glob == GS_DOGLEG ? kDogLeg : kStatic;
+        return finish_launch(launch_dense(family->n, g, mixed, refine, diag, method->reuse > 0, L, st), st, timed);
     }
     if (glob == GS_DOGLEG)
         rc = launch_small_dogleg(family->id, a, mixed, refine, diag, wg, st);
@@ -730,8 +727,10 @@ int gs_solve_batch_f32(const gs_method* method, const gs_family* family, int64_t
         L.S = S;
         L.codes = codes;
         L.order = order;
-        rc = launch_dense_f32(family->n, method->globalization == GS_BACKTRACKING ? kBacktracking : kStatic,
-                              method->refinement == GS_REFINEMENT, S != nullptr, method->reuse > 0, L, st);
+        const int g = method->globalization == GS_BACKTRACKING ? kBacktracking
+                      : method->globalization == GS_DOGLEG    ? kDogLeg
+                                                              : kStatic;
+        rc = launch_dense_f32(family->n, g, method->refinement == GS_REFINEMENT, S != nullptr, method->reuse > 0, L, st);
         return finish_launch(rc, st, timed);
     }
     rc = launch_small_f32(family->id, method->globalization, method->reuse > 0, a,

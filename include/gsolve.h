@@ -209,9 +209,8 @@ typedef struct gs_family {
  *   workgroupsize 0 = library default, else the workgroup size of the launch (1..1024); a size
  *            above a kernel's own bound runs as workgroups of that bound (the systems are
  *            independent, so the grouping changes no result).
- * GS_FAM_FPU_GL3 takes Static, Backtracking (with or without reuse) and DogLeg; GS_FAM_DENSE_SIN
- * every globalization at n ≤ 8 and Static or Backtracking above, with or without reuse (GS_EFAMILY
- * otherwise).
+ * GS_FAM_FPU_GL3 and GS_FAM_DENSE_SIN take Static, Backtracking (with or without reuse) and DogLeg
+ * (GS_EFAMILY otherwise).
  */
 int gs_solve_batch_f64(const gs_method* method, const gs_family* family, int64_t nsys,
                        double* X, const void* P, int64_t p_stride, gs_status_f64* S,
@@ -322,8 +321,8 @@ int gs_solve_batch_reparametrized_f64(const gs_method* method, const gs_family* 
  * and ToReal{Float32} hold them; an Options{Float32} passes through exactly). A Float32 batch has no
  * mixed-precision form (its factor is in its working precision already: factor_precision is ignored).
  * The families of the lane-per-system kernels (1–5) and the dense family 10 (its matrix A in
- * gs_family.data as floats; Static or Backtracking, with or without reuse) are served; family 6 is
- * refused (GS_EFAMILY). */
+ * gs_family.data as floats; Static, Backtracking with or without reuse, DogLeg) are served; family 6
+ * is refused (GS_EFAMILY). */
 int gs_solve_batch_f32(const gs_method* method, const gs_family* family, int64_t nsys,
                        float* X, const void* P, int64_t p_stride, gs_status_f32* S,
                        uint8_t* codes, const int64_t* order, const gs_options_f64* options,

@@ -226,18 +226,26 @@ def test_dense_float32_bitwise(gs, orc, workloads, device, n, name):
         assert st["reused"].any()
 
 
-@pytest.mark.parametrize("n", [2, 4, 6, 8])
+@pytest.mark.parametrize("n", [2, 4, 6, 8, 12, 16, 24, 32, 48, 64])
 @pytest.mark.parametrize("eltype", ["float64", "float32"])
-def test_dense_small_dogleg_bitwise(gs, orc, workloads, device, n, eltype):
-    """DogLeg on the lane-per-system form of the dense family (n ≤ 8), in both precisions, against the
-    oracle bit for bit."""
-    A, B, X0 = _batch(workloads, n, 2048, seed=n + 7)
+def test_dense_dogleg_bitwise(gs, orc, workloads, device, n, eltype):
+    """DogLeg on the dense family at every size (the lane-per-system form at n ≤ 8, and 12 in Float32;
+    the segment kernel above, whose Cauchy direction takes Jregᵀ F from the columns of A), in both
+    precisions, with the radius unset and set small, and in FP64 the mixed form with refinement and a
+    regularised form; against the oracle bit for bit."""
+    nsys = 2048 if n < 32 else (1024 if n == 32 else 512)
+    A, B, X0 = _batch(workloads, n, nsys, seed=n + 7)
     X0 = np.where(np.abs(X0) > 1e30, 1e30, X0)
     dt = np.float32 if eltype == "float32" else np.float64
     fam = gs.DenseSin(A.astype(dt).astype(np.float64))
     Bt, Xt = B.astype(dt), X0.astype(dt)
     tol = float(np.sqrt(np.finfo(np.float32).eps)) if eltype == "float32" else 1e-12
-    for m in (gs.Newton(globalization=gs.DogLeg()), gs.Newton(globalization=gs.DogLeg(radius=1e-2))):
+    methods = [gs.Newton(globalization=gs.DogLeg()), gs.Newton(globalization=gs.DogLeg(radius=1e-2))]
+    if eltype == "float64":
+        methods += [gs.Newton(globalization=gs.DogLeg(radius=2.0), linsolve=gs.LUFactorization("float32"),
+                              refinement=gs.Refinement()),
+                    gs.Newton(globalization=gs.DogLeg(), refinement=gs.Refinement(), regularization=1e-3)]
+    for m in methods:
         opts = gs.Options(eltype=eltype, f_abstol=tol, f_reltol=0.0, max_iterations=60)
         X = torch.as_tensor(Xt, device=device).clone()
         S = gs.status_buffer(len(Xt), device, eltype)
