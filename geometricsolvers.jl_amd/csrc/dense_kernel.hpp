@@ -1177,8 +1177,20 @@ GS_DEV void store_status_t(typename StatusWords<T>::Status* dst, uint8_t code, i
     store_status_lane(dst, w);
 }
 
+// Waves per SIMD the register allocator targets: 2 for the FP64 kernels at n = 64 beyond the plain
+// Static form (Backtracking, DogLeg, the mixed and refined forms), which need a few registers more
+// than 256 and would otherwise run at one wave per SIMD with the excess in AGPRs. Held to two waves
+// they spill 200-850 bytes per lane to scratch and ran faster for 2^18 systems: Backtracking 98.3 ->
+// 64.8 ms, DogLeg 112.4 -> 77.5 ms, the mixed form with refinement 103.0 -> 87.8 ms. The plain
+// Static forms (with or without reuse) fit in 248 registers and take no target.
+template <class T, int N, int GLOB, bool MIXED, bool REFINE> constexpr int dense_waves_per_eu() {
+    return (sizeof(T) == 8 && N == 64 && (GLOB != kStatic || MIXED || REFINE)) ? 2 : 1;
+}
+
 template <class T, int N, int NP, int GLOB, bool MIXED, bool REFINE, bool DIAG, bool REUSE>
-__global__ __launch_bounds__(kDenseThreads) void dense_kernel(const DenseLaunchT<T> L) {
+__global__ __launch_bounds__(kDenseThreads)
+__attribute__((amdgpu_waves_per_eu(dense_waves_per_eu<T, N, GLOB, MIXED, REFINE>())))
+void dense_kernel(const DenseLaunchT<T> L) {
     using Sh = Shape<N, NP>;
     __shared__ T sA[N * N];
     __shared__ SysVecs<T, N> sv[Sh::SPB];
